@@ -1,0 +1,215 @@
+/*
+ * ba_c_api.h -- C ABI of the MI355X bundle-adjustment engine (libslamplay_ba.so).
+ *
+ * Drop-in boundary for slamplay's BA path (backend/slam_ba_simple_g2o_ceres):
+ *   - ba_problem_view maps a slamplay::BALProblem with zero copies (BALProblem.h:59-98 accessors:
+ *     num_cameras/num_points/num_observations, camera_index, point_index, observations,
+ *     mutable_cameras, mutable_points);
+ *   - ba_solve replaces `void SolveBA(BALProblem&)` (test_bundle_adjustment_ceres.cpp:35,58-101):
+ *     one Snavely residual per observation with HuberLoss(1.0) (:75-86), trust-region
+ *     Levenberg-Marquardt with the Schur complement (:96-100), parameters updated in place (:83-84);
+ *   - ba_eval_residual_jacobian replaces the per-observation
+ *     AutoDiffCostFunction<SnavelyReprojectionError,2,9,3> evaluation
+ *     (utils/camera/SnavelyReprojectionError.h:42-92).
+ *
+ * All entry points are synchronous, return a ba_status (0 = OK) and never abort; the message of
+ * the last failure on the calling thread is returned by ba_last_error_string().  Host arrays are
+ * owned by the caller; device buffers are owned by the library and freed by the *_destroy calls.
+ * No allocation crosses the ABI.
+ */
+#ifndef MBA_BA_C_API_H
+#define MBA_BA_C_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BA_ABI_VERSION 1
+
+typedef enum ba_status {
+    BA_OK = 0,
+    BA_ERR_INVALID_ARGUMENT = 1,
+    BA_ERR_OUT_OF_MEMORY = 2,
+    BA_ERR_NON_FINITE = 3,
+    BA_ERR_HIP = 4,
+    BA_ERR_NO_DEVICE = 5,
+    BA_ERR_CAPACITY = 6,
+    BA_ERR_IO = 7,
+    BA_ERR_NOT_FOUND = 8
+} ba_status;
+
+typedef enum ba_precision {
+    BA_PRECISION_FP64 = 0,  /* every stage in FP64: the plain drop-in */
+    BA_PRECISION_MIXED = 1  /* FP32 residual/Jacobian/Hessian blocks and PCG matvecs; FP64 block
+                               inverses, dense-step refinement and the LM accept-test cost */
+} ba_precision;
+
+typedef enum ba_linear_solver {
+    BA_SOLVER_AUTO = 0,            /* dense Schur up to dense_schur_max_cameras, else iterative */
+    BA_SOLVER_DENSE_SCHUR = 1,     /* explicit reduced camera system + dense Cholesky */
+    BA_SOLVER_ITERATIVE_SCHUR = 2  /* implicit reduced camera system + block-Jacobi PCG */
+} ba_linear_solver;
+
+typedef enum ba_loss { BA_LOSS_NONE = 0, BA_LOSS_HUBER = 1 } ba_loss;
+
+typedef enum ba_termination { BA_CONVERGENCE = 0, BA_NO_CONVERGENCE = 1, BA_FAILURE = 2 } ba_termination;
+
+/* Zero-copy view of a BAL problem.  cameras: 9 doubles per camera (angle-axis[3], t[3], f, k1, k2),
+ * points: 3 doubles per point; both updated in place by ba_solve.  observations: x,y interleaved. */
+typedef struct ba_problem_view {
+    int32_t num_cameras;
+    int32_t num_points;
+    int32_t num_observations;
+    int32_t camera_block_size; /* must be 9 (BALProblem::camera_block_size without quaternions) */
+    const int32_t* camera_index;
+    const int32_t* point_index;
+    const double* observations;
+    double* cameras;
+    double* points;
+} ba_problem_view;
+
+/* Solver options.  Defaults (ba_default_options) are Ceres Solver::Options defaults, which is what
+ * the reference's SolveBA uses (it only sets SPARSE_SCHUR and progress output). */
+typedef struct ba_options {
+    int32_t max_num_iterations;                /* 50 */
+    int32_t max_num_consecutive_invalid_steps; /* 5 */
+    int32_t jacobi_scaling;                    /* 1 */
+    int32_t loss;                              /* BA_LOSS_HUBER */
+    int32_t linear_solver;                     /* BA_SOLVER_AUTO */
+    int32_t pcg_max_iterations;                /* 50 */
+    int32_t precision;                         /* BA_PRECISION_FP64 */
+    int32_t dense_schur_max_cameras;           /* 300 */
+    int32_t device;                            /* HIP device ordinal, 0 */
+    int32_t verbose;                           /* print a Ceres-style iteration table to stdout */
+    int32_t refine_steps;                      /* FP64 refinement steps of the dense mixed solve, 2 */
+    int32_t reserved0;
+    double loss_scale;                  /* 1.0 (HuberLoss(1.0)) */
+    double function_tolerance;          /* 1e-6 */
+    double gradient_tolerance;          /* 1e-10 */
+    double parameter_tolerance;         /* 1e-8 */
+    double initial_trust_region_radius; /* 1e4 */
+    double max_trust_region_radius;     /* 1e16 */
+    double min_trust_region_radius;     /* 1e-32 */
+    double min_relative_decrease;       /* 1e-3 */
+    double min_lm_diagonal;             /* 1e-6 */
+    double max_lm_diagonal;             /* 1e32 */
+    double pcg_tolerance;               /* 1e-6: stop when ||r|| <= tol * ||b|| */
+} ba_options;
+
+/* One row of the per-iteration trace (the Ceres progress table's columns). */
+typedef struct ba_iteration {
+    int32_t iteration;
+    int32_t step_is_successful;
+    int32_t step_is_valid;
+    int32_t linear_solver_iterations;
+    double cost;
+    double cost_change;
+    double gradient_max_norm;
+    double step_norm;
+    double relative_decrease;
+    double trust_region_radius;
+    double model_cost_change;
+} ba_iteration;
+
+typedef struct ba_summary {
+    double initial_cost;
+    double final_cost;
+    int32_t num_iterations; /* trace rows written, iteration 0 included */
+    int32_t num_successful_steps;
+    int32_t num_unsuccessful_steps;
+    int32_t termination_type; /* ba_termination */
+    int32_t linear_solver_used; /* BA_SOLVER_DENSE_SCHUR or BA_SOLVER_ITERATIVE_SCHUR */
+    int32_t total_linear_solver_iterations;
+    double total_time_ms;      /* wall time of the solve, device work included */
+    ba_iteration* trace;       /* optional caller-owned array of trace_capacity rows */
+    int32_t trace_capacity;
+    int32_t reserved0;
+} ba_summary;
+
+int32_t ba_abi_version(void);
+const char* ba_last_error_string(void);
+void ba_default_options(ba_options* options);
+int ba_device_count(int32_t* count);
+
+/* Raw (uncorrected) residuals r (2 per observation), Jacobians jac_camera (2x9 row-major per
+ * observation) and jac_point (2x3 row-major) in observation order, evaluated on the GPU in the
+ * requested precision (FP32 results are widened to double).  Replaces the AutoDiffCostFunction
+ * evaluation of SnavelyReprojectionError.h:42-92.  jac_* may be NULL. */
+int ba_eval_residual_jacobian(const ba_problem_view* problem, int32_t precision, double* residuals,
+                              double* jac_camera, double* jac_point);
+
+/* 1/2 sum_i rho(|r_i|^2) on the GPU in FP64 (loss = ba_loss, scale = Huber a). */
+int ba_eval_cost(const ba_problem_view* problem, int32_t loss, double loss_scale, double* cost);
+
+/* Full solve: upload, device-resident LM loop, write parameters back in place.
+ * Replaces SolveBA(BALProblem&) of test_bundle_adjustment_ceres.cpp:58-101. summary may be NULL. */
+int ba_solve(const ba_problem_view* problem, const ba_options* options, ba_summary* summary);
+
+/* Device-resident solver: create uploads the problem and builds the device layout once; run solves
+ * from the parameters currently held on the device; reset restores the parameters given at create;
+ * download copies the device parameters to host AoS arrays. */
+typedef struct ba_solver ba_solver;
+int ba_solver_create(const ba_problem_view* problem, const ba_options* options, ba_solver** out);
+int ba_solver_run(ba_solver* solver, ba_summary* summary);
+int ba_solver_reset(ba_solver* solver);
+int ba_solver_download(ba_solver* solver, double* cameras, double* points);
+int ba_solver_destroy(ba_solver* solver);
+
+/* Per-kernel timing collected with HIP events on the solver's stream while enabled. */
+typedef struct ba_kernel_time {
+    char name[48];
+    int64_t launches;
+    double total_ms;
+    double bytes_per_launch; /* algorithmic bytes of one launch (stated per kernel) */
+    double flops_per_launch;
+} ba_kernel_time;
+int ba_solver_set_profiling(ba_solver* solver, int32_t enabled);
+int ba_solver_kernel_times(ba_solver* solver, ba_kernel_time* out, int32_t capacity, int32_t* count);
+/* Observations in the solver's device layout (after dropping nothing) and device bytes held. */
+int ba_solver_info(ba_solver* solver, int64_t* device_bytes, int32_t* linear_solver);
+
+/* ---- Host BALProblem (utils/datasets/BALProblem.{h,cpp}) through opaque handles ---- */
+typedef struct ba_bal ba_bal;
+/* BALProblem::BALProblem(filename, use_quaternions) (BALProblem.cpp:78-139); use_quaternions must
+ * be 0 for solving (the Snavely functor is 9-parameter). */
+int ba_bal_load(const char* path, int32_t use_quaternions, ba_bal** out);
+/* A problem built from caller arrays (copied). */
+int ba_bal_from_arrays(int32_t num_cameras, int32_t num_points, int32_t num_observations,
+                       const int32_t* camera_index, const int32_t* point_index, const double* observations,
+                       const double* cameras, const double* points, ba_bal** out);
+int ba_bal_view(ba_bal* bal, ba_problem_view* view);
+int ba_bal_normalize(ba_bal* bal);                                                      /* :263-303 */
+int ba_bal_perturb(ba_bal* bal, double rotation_sigma, double translation_sigma, double point_sigma); /* :305-336 */
+/* BAL writer. legacy_header != 0 reproduces the reference's 4-integer header "Nc Nc Np No"
+ * (BALProblem.cpp:149), which its own reader cannot parse back; 0 writes "Nc Np No". */
+int ba_bal_write(ba_bal* bal, const char* path, int32_t legacy_header);
+int ba_bal_write_ply(ba_bal* bal, const char* path); /* :185-228 */
+int ba_bal_destroy(ba_bal* bal);
+
+/* ---- Seeded synthetic problems (the benchmark configurations) ---- */
+typedef struct ba_synth_params {
+    int32_t kind;        /* 1 circle (config 1), 2 random walk (configs 2-5) */
+    int32_t num_cameras;
+    int32_t num_points;  /* kind 2: 0 = draw tracks until target_observations is reached */
+    int32_t min_track;
+    int32_t max_track;
+    int32_t target_observations;
+    double zipf_s;       /* kind 2 track-length exponent; 0 = uniform on [min_track, max_track] */
+    double pixel_noise;  /* observation noise sigma (px) */
+    double rotation_sigma, translation_sigma, point_sigma; /* initial perturbation */
+    double focal_mean, focal_sigma, distortion_sigma;
+    uint64_t seed;
+} ba_synth_params;
+/* Fill defaults for a benchmark configuration (1..5; 4/5 describe one local-BA window). */
+int ba_synth_default(int32_t config, ba_synth_params* params);
+/* Generate a perturbed problem; if ground_truth is non-NULL it receives the exact parameters
+ * (9*Nc + 3*Np doubles, cameras then points) the observations were projected from. */
+int ba_synth_generate(const ba_synth_params* params, ba_bal** out, double* ground_truth, int64_t gt_capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MBA_BA_C_API_H */

@@ -1,0 +1,206 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/ba_oracle.cpp).
+
+Test infrastructure only: imported by tests/, bench.py's cpu_baseline leg and
+__graft_entry__.smoke() as the checker.  The product package never imports this module.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle_ba.so")
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class Options(ctypes.Structure):
+    """Mirror of oracle::Options (Ceres Solver::Options defaults unless stated)."""
+
+    _fields_ = [
+        ("max_num_iterations", ctypes.c_int32),
+        ("max_num_consecutive_invalid_steps", ctypes.c_int32),
+        ("jacobi_scaling", ctypes.c_int32),
+        ("loss", ctypes.c_int32),
+        ("linear_solver", ctypes.c_int32),
+        ("pcg_max_iterations", ctypes.c_int32),
+        ("loss_scale", ctypes.c_double),
+        ("function_tolerance", ctypes.c_double),
+        ("gradient_tolerance", ctypes.c_double),
+        ("parameter_tolerance", ctypes.c_double),
+        ("initial_trust_region_radius", ctypes.c_double),
+        ("max_trust_region_radius", ctypes.c_double),
+        ("min_trust_region_radius", ctypes.c_double),
+        ("min_relative_decrease", ctypes.c_double),
+        ("min_lm_diagonal", ctypes.c_double),
+        ("max_lm_diagonal", ctypes.c_double),
+        ("pcg_tolerance", ctypes.c_double),
+    ]
+
+
+class IterRecord(ctypes.Structure):
+    _fields_ = [
+        ("iteration", ctypes.c_int32),
+        ("step_is_successful", ctypes.c_int32),
+        ("step_is_valid", ctypes.c_int32),
+        ("linear_solver_iterations", ctypes.c_int32),
+        ("cost", ctypes.c_double),
+        ("cost_change", ctypes.c_double),
+        ("gradient_max_norm", ctypes.c_double),
+        ("step_norm", ctypes.c_double),
+        ("relative_decrease", ctypes.c_double),
+        ("trust_region_radius", ctypes.c_double),
+        ("model_cost_change", ctypes.c_double),
+    ]
+
+
+def default_options(**kw):
+    o = Options(
+        max_num_iterations=50,
+        max_num_consecutive_invalid_steps=5,
+        jacobi_scaling=1,
+        loss=1,
+        linear_solver=1,
+        pcg_max_iterations=50,
+        loss_scale=1.0,
+        function_tolerance=1e-6,
+        gradient_tolerance=1e-10,
+        parameter_tolerance=1e-8,
+        initial_trust_region_radius=1e4,
+        max_trust_region_radius=1e16,
+        min_trust_region_radius=1e-32,
+        min_relative_decrease=1e-3,
+        min_lm_diagonal=1e-6,
+        max_lm_diagonal=1e32,
+        pcg_tolerance=1e-6,
+    )
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.oracle_eval.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
+        L.oracle_residual.argtypes = [_f64p, _f64p, ctypes.c_double, ctypes.c_double, _f64p]
+        L.oracle_rotate_point.argtypes = [_f64p, _f64p, _f64p]
+        L.oracle_huber.argtypes = [ctypes.c_double, ctypes.c_double, _f64p]
+        L.oracle_cost.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, ctypes.c_int, ctypes.c_double]
+        L.oracle_cost.restype = ctypes.c_double
+        L.oracle_lm_step.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p,
+                                     ctypes.POINTER(Options), ctypes.c_double, _f64p, _f64p, _f64p]
+        L.oracle_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p,
+                                   ctypes.POINTER(Options), ctypes.POINTER(IterRecord), ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_int), _f64p, _f64p, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_normalize.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, _f64p]
+        L.oracle_perturb.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, _f64p, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double]
+        L.oracle_srand.argtypes = [ctypes.c_uint]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def eval_residual_jacobian(cam_idx, pt_idx, obs, cams, pts):
+    """Raw residuals (No,2), Jc (No,2,9), Jp (No,2,3) in observation order."""
+    ci, pi = _c(cam_idx, np.int32), _c(pt_idx, np.int32)
+    obs, cams, pts = _c(obs, np.float64), _c(cams, np.float64), _c(pts, np.float64)
+    no = ci.size
+    r = np.zeros((no, 2))
+    jc = np.zeros((no, 2, 9))
+    jp = np.zeros((no, 2, 3))
+    lib().oracle_eval(no, _p(ci, _i32p), _p(pi, _i32p), _p(obs, _f64p), _p(cams, _f64p), _p(pts, _f64p),
+                      _p(r, _f64p), _p(jc, _f64p), _p(jp, _f64p))
+    return r, jc, jp
+
+
+def residual(cam, pt, ox, oy):
+    cam, pt = _c(cam, np.float64), _c(pt, np.float64)
+    r = np.zeros(2)
+    lib().oracle_residual(_p(cam, _f64p), _p(pt, _f64p), ox, oy, _p(r, _f64p))
+    return r
+
+
+def rotate_point(aa, pt):
+    aa, pt = _c(aa, np.float64), _c(pt, np.float64)
+    out = np.zeros(3)
+    lib().oracle_rotate_point(_p(aa, _f64p), _p(pt, _f64p), _p(out, _f64p))
+    return out
+
+
+def huber(s, a=1.0):
+    rho = np.zeros(3)
+    lib().oracle_huber(float(s), float(a), _p(rho, _f64p))
+    return rho
+
+
+def cost(cam_idx, pt_idx, obs, cams, pts, loss=1, loss_scale=1.0):
+    ci, pi = _c(cam_idx, np.int32), _c(pt_idx, np.int32)
+    obs, cams, pts = _c(obs, np.float64), _c(cams, np.float64), _c(pts, np.float64)
+    return lib().oracle_cost(ci.size, _p(ci, _i32p), _p(pi, _i32p), _p(obs, _f64p), _p(cams, _f64p),
+                             _p(pts, _f64p), loss, loss_scale)
+
+
+def lm_step(nc, np_, cam_idx, pt_idx, obs, cams, pts, radius, options=None):
+    """One damped LM step at x (scaling from x). Returns (delta, model_cost_change, scale, iters)."""
+    o = options or default_options()
+    ci, pi = _c(cam_idx, np.int32), _c(pt_idx, np.int32)
+    obs, cams, pts = _c(obs, np.float64), _c(cams, np.float64), _c(pts, np.float64)
+    n = 9 * nc + 3 * np_
+    delta = np.zeros(n)
+    scale = np.zeros(n)
+    mcc = ctypes.c_double(0.0)
+    it = lib().oracle_lm_step(nc, np_, ci.size, _p(ci, _i32p), _p(pi, _i32p), _p(obs, _f64p), _p(cams, _f64p),
+                              _p(pts, _f64p), ctypes.byref(o), radius, _p(delta, _f64p), ctypes.byref(mcc),
+                              _p(scale, _f64p))
+    return delta, mcc.value, scale, it
+
+
+def solve(nc, np_, cam_idx, pt_idx, obs, cams, pts, options=None, trace_cap=512):
+    """Ceres-style LM. Returns dict(cams, pts, initial_cost, final_cost, termination, trace)."""
+    o = options or default_options()
+    ci, pi = _c(cam_idx, np.int32), _c(pt_idx, np.int32)
+    obs = _c(obs, np.float64)
+    cams = np.array(cams, dtype=np.float64, copy=True).reshape(-1)
+    pts = np.array(pts, dtype=np.float64, copy=True).reshape(-1)
+    trace = (IterRecord * trace_cap)()
+    nt = ctypes.c_int(0)
+    ic, fc = ctypes.c_double(0), ctypes.c_double(0)
+    term = ctypes.c_int(0)
+    lib().oracle_solve(nc, np_, ci.size, _p(ci, _i32p), _p(pi, _i32p), _p(obs, _f64p), _p(cams, _f64p),
+                       _p(pts, _f64p), ctypes.byref(o), trace, trace_cap, ctypes.byref(nt), ctypes.byref(ic),
+                       ctypes.byref(fc), ctypes.byref(term))
+    recs = [trace[i] for i in range(min(nt.value, trace_cap))]
+    tr = [{f: getattr(r, f) for f, _ in IterRecord._fields_} for r in recs]
+    return dict(cams=cams, pts=pts, initial_cost=ic.value, final_cost=fc.value, termination=term.value, trace=tr)
+
+
+def normalize(nc, np_, cams, pts):
+    cams = np.array(cams, dtype=np.float64, copy=True).reshape(-1)
+    pts = np.array(pts, dtype=np.float64, copy=True).reshape(-1)
+    lib().oracle_normalize(nc, np_, _p(cams, _f64p), _p(pts, _f64p))
+    return cams, pts
+
+
+def perturb(nc, np_, cams, pts, rotation_sigma, translation_sigma, point_sigma, seed=1):
+    """Reference Perturb on glibc rand(), seeded with srand(seed) first (glibc's default is 1)."""
+    cams = np.array(cams, dtype=np.float64, copy=True).reshape(-1)
+    pts = np.array(pts, dtype=np.float64, copy=True).reshape(-1)
+    lib().oracle_srand(seed)
+    lib().oracle_perturb(nc, np_, _p(cams, _f64p), _p(pts, _f64p), rotation_sigma, translation_sigma, point_sigma)
+    return cams, pts

@@ -1,0 +1,258 @@
+"""slamplay_amd -- MI355X-native bundle adjustment for slamplay's BA path.
+
+Python mirror of the host interface (include/mba/bal_problem.h) over the C ABI of
+libslamplay_ba.so:
+
+  * BALProblem    -- utils/datasets/BALProblem.{h,cpp}: load a BAL file, Normalize, Perturb,
+                     WriteToFile, WriteToPLYFile, and numpy views of the AoS arrays
+                     (cameras: 9 doubles each, points: 3, observations: x,y).
+  * SolveBA       -- drop-in for SolveBA(BALProblem&) of
+                     backend/slam_ba_simple_g2o_ceres/test_bundle_adjustment_ceres.cpp:58-101;
+                     solves on the GPU and updates the problem in place.
+  * Solver        -- device-resident solver handle (upload once, run many times).
+  * eval_residual_jacobian / eval_cost -- per-observation evaluation on the GPU.
+  * synthetic     -- seeded synthetic problems of the benchmark configurations.
+
+Every entry point runs the HIP kernels of libslamplay_ba.so; there is no CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi as C
+from ._capi import BAError
+
+__all__ = ["BALProblem", "SolveBA", "Solver", "default_options", "eval_residual_jacobian", "eval_cost",
+           "synthetic", "synth_params", "BAError", "device_count"]
+
+
+def default_options(**kw):
+    o = C.Options()
+    C.lib().ba_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(f"unknown option {k}")
+        setattr(o, k, v)
+    return o
+
+
+def device_count():
+    n = C.i32(0)
+    C.check(C.lib().ba_device_count(ctypes.byref(n)))
+    return n.value
+
+
+class BALProblem:
+    """slamplay BALProblem (utils/datasets/BALProblem.h:31-119) backed by a C++ ba_bal handle."""
+
+    def __init__(self, filename=None, use_quaternions=False, _handle=None):
+        if _handle is None:
+            if filename is None:
+                raise ValueError("filename required")
+            h = C.vp()
+            C.check(C.lib().ba_bal_load(str(filename).encode(), 1 if use_quaternions else 0, ctypes.byref(h)))
+            _handle = h
+        self._h = _handle
+        self._refresh()
+
+    @classmethod
+    def from_arrays(cls, camera_index, point_index, observations, cameras, points):
+        ci = np.ascontiguousarray(camera_index, dtype=np.int32)
+        pi = np.ascontiguousarray(point_index, dtype=np.int32)
+        obs = np.ascontiguousarray(observations, dtype=np.float64).reshape(-1)
+        cams = np.ascontiguousarray(cameras, dtype=np.float64).reshape(-1)
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1)
+        h = C.vp()
+        C.check(C.lib().ba_bal_from_arrays(cams.size // 9, pts.size // 3, ci.size, ci.ctypes.data_as(C.i32p),
+                                           pi.ctypes.data_as(C.i32p), obs.ctypes.data_as(C.f64p),
+                                           cams.ctypes.data_as(C.f64p), pts.ctypes.data_as(C.f64p), ctypes.byref(h)))
+        return cls(_handle=h)
+
+    def _refresh(self):
+        v = C.ProblemView()
+        C.check(C.lib().ba_bal_view(self._h, ctypes.byref(v)))
+        self._view = v
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and C._lib is not None:
+            C.lib().ba_bal_destroy(h)
+            self._h = None
+
+    # --- accessors (BALProblem.h:55-98) ---
+    def camera_block_size(self):
+        return self._view.camera_block_size
+
+    def point_block_size(self):
+        return 3
+
+    def num_cameras(self):
+        return self._view.num_cameras
+
+    def num_points(self):
+        return self._view.num_points
+
+    def num_observations(self):
+        return self._view.num_observations
+
+    def num_parameters(self):
+        return self.camera_block_size() * self.num_cameras() + 3 * self.num_points()
+
+    def _arr(self, ptr, n, dtype):
+        if n == 0:
+            return np.zeros(0, dtype=dtype)
+        return np.ctypeslib.as_array(ptr, shape=(n,)).view(dtype)
+
+    def camera_index(self):
+        return self._arr(self._view.camera_index, self.num_observations(), np.int32)
+
+    def point_index(self):
+        return self._arr(self._view.point_index, self.num_observations(), np.int32)
+
+    def observations(self):
+        return self._arr(self._view.observations, 2 * self.num_observations(), np.float64)
+
+    def mutable_cameras(self):
+        return self._arr(self._view.cameras, self.camera_block_size() * self.num_cameras(), np.float64)
+
+    def mutable_points(self):
+        return self._arr(self._view.points, 3 * self.num_points(), np.float64)
+
+    cameras = mutable_cameras
+    points = mutable_points
+
+    def view(self):
+        return self._view
+
+    # --- operations ---
+    def Normalize(self):
+        C.check(C.lib().ba_bal_normalize(self._h))
+
+    def Perturb(self, rotation_sigma, translation_sigma, point_sigma):
+        C.check(C.lib().ba_bal_perturb(self._h, rotation_sigma, translation_sigma, point_sigma))
+
+    def WriteToFile(self, filename, legacy_header=False):
+        C.check(C.lib().ba_bal_write(self._h, str(filename).encode(), 1 if legacy_header else 0))
+
+    def WriteToPLYFile(self, filename):
+        C.check(C.lib().ba_bal_write_ply(self._h, str(filename).encode()))
+
+
+def _summary(trace_cap):
+    s = C.Summary()
+    rows = (C.Iteration * trace_cap)()
+    s.trace = ctypes.cast(rows, ctypes.POINTER(C.Iteration))
+    s.trace_capacity = trace_cap
+    return s, rows
+
+
+def _summary_dict(s, rows):
+    n = min(s.num_iterations, s.trace_capacity)
+    trace = [{f: getattr(rows[i], f) for f, _ in C.Iteration._fields_} for i in range(n)]
+    return dict(initial_cost=s.initial_cost, final_cost=s.final_cost, num_iterations=s.num_iterations,
+                num_successful_steps=s.num_successful_steps, num_unsuccessful_steps=s.num_unsuccessful_steps,
+                termination=s.termination_type, linear_solver=s.linear_solver_used,
+                linear_solver_iterations=s.total_linear_solver_iterations, total_time_ms=s.total_time_ms,
+                trace=trace)
+
+
+def SolveBA(problem, options=None, **kw):
+    """Solve `problem` (a BALProblem) on the GPU in place; returns the summary as a dict."""
+    o = options if options is not None else default_options(**kw)
+    s, rows = _summary(max(8, o.max_num_iterations + 2))
+    C.check(C.lib().ba_solve(ctypes.byref(problem.view()), ctypes.byref(o), ctypes.byref(s)))
+    return _summary_dict(s, rows)
+
+
+class Solver:
+    """Device-resident solver: the problem is uploaded once; run() solves from the parameters held
+    on the device, reset() restores the initial ones, download() returns (cameras, points)."""
+
+    def __init__(self, problem, options=None, **kw):
+        self.options = options if options is not None else default_options(**kw)
+        self._nc, self._np = problem.num_cameras(), problem.num_points()
+        h = C.vp()
+        C.check(C.lib().ba_solver_create(ctypes.byref(problem.view()), ctypes.byref(self.options), ctypes.byref(h)))
+        self._h = h
+
+    def run(self):
+        s, rows = _summary(max(8, self.options.max_num_iterations + 2))
+        C.check(C.lib().ba_solver_run(self._h, ctypes.byref(s)))
+        return _summary_dict(s, rows)
+
+    def reset(self):
+        C.check(C.lib().ba_solver_reset(self._h))
+
+    def download(self):
+        cams = np.zeros(9 * self._nc)
+        pts = np.zeros(3 * self._np)
+        C.check(C.lib().ba_solver_download(self._h, cams.ctypes.data_as(C.f64p), pts.ctypes.data_as(C.f64p)))
+        return cams, pts
+
+    def set_profiling(self, on=True):
+        C.check(C.lib().ba_solver_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        buf = (C.KernelTime * 128)()
+        n = C.i32(0)
+        C.check(C.lib().ba_solver_kernel_times(self._h, buf, 128, ctypes.byref(n)))
+        return {buf[i].name.decode(): dict(launches=buf[i].launches, total_ms=buf[i].total_ms,
+                                           bytes_per_launch=buf[i].bytes_per_launch) for i in range(min(n.value, 128))}
+
+    def info(self):
+        b = C.i64(0)
+        ls = C.i32(0)
+        C.check(C.lib().ba_solver_info(self._h, ctypes.byref(b), ctypes.byref(ls)))
+        return dict(device_bytes=b.value, linear_solver=ls.value)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            C.lib().ba_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if C._lib is not None:
+            self.close()
+
+
+def eval_residual_jacobian(problem, precision=C.PRECISION_FP64, jacobians=True):
+    """GPU residuals (No,2) and, optionally, Jc (No,2,9) and Jp (No,2,3) in observation order."""
+    no = problem.num_observations()
+    r = np.zeros((no, 2))
+    jc = np.zeros((no, 2, 9)) if jacobians else None
+    jp = np.zeros((no, 2, 3)) if jacobians else None
+    C.check(C.lib().ba_eval_residual_jacobian(
+        ctypes.byref(problem.view()), precision, r.ctypes.data_as(C.f64p),
+        jc.ctypes.data_as(C.f64p) if jacobians else None, jp.ctypes.data_as(C.f64p) if jacobians else None))
+    return (r, jc, jp) if jacobians else r
+
+
+def eval_cost(problem, loss=C.LOSS_HUBER, loss_scale=1.0):
+    c = C.f64(0)
+    C.check(C.lib().ba_eval_cost(ctypes.byref(problem.view()), loss, loss_scale, ctypes.byref(c)))
+    return c.value
+
+
+def synth_params(config, **kw):
+    p = C.SynthParams()
+    C.check(C.lib().ba_synth_default(config, ctypes.byref(p)))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(f"unknown synthetic parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+def synthetic(config=1, seed=1, ground_truth=False, **kw):
+    """Seeded synthetic problem of benchmark configuration `config` (1..5)."""
+    p = synth_params(config, seed=seed, **kw)
+    h = C.vp()
+    if ground_truth:
+        # generous capacity: cameras then points
+        cap = 9 * p.num_cameras + 3 * max(p.num_points, p.target_observations)
+        gt = np.zeros(cap)
+        C.check(C.lib().ba_synth_generate(ctypes.byref(p), ctypes.byref(h), gt.ctypes.data_as(C.f64p), cap))
+        prob = BALProblem(_handle=h)
+        return prob, gt[: prob.num_parameters()]
+    C.check(C.lib().ba_synth_generate(ctypes.byref(p), ctypes.byref(h), None, 0))
+    return BALProblem(_handle=h)

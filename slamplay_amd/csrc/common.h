@@ -1,0 +1,76 @@
+// common.h -- error reporting, device buffers and launch helpers shared by the engine's sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mba/ba_c_api.h"
+
+namespace mba {
+
+// Records the message returned by ba_last_error_string() on this thread and returns code.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+#define BA_HIP(expr)                                                                                        \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess)                                                                               \
+            return ::mba::fail(e_ == hipErrorOutOfMemory ? BA_ERR_OUT_OF_MEMORY : BA_ERR_HIP,               \
+                               "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__);        \
+    } while (0)
+
+#define BA_TRY(expr)                   \
+    do {                               \
+        int s_ = (expr);               \
+        if (s_ != BA_OK) return s_;    \
+    } while (0)
+
+// Owning device allocation (hipMalloc / hipFree).
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(e == hipErrorOutOfMemory ? BA_ERR_OUT_OF_MEMORY : BA_ERR_HIP, "hipMalloc(%zu bytes): %s",
+                        count * sizeof(T), hipGetErrorString(e));
+        }
+        n = count;
+        return BA_OK;
+    }
+    int upload(const T* host, size_t count, hipStream_t s) {
+        BA_TRY(alloc(count));
+        if (count) BA_HIP(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+        return BA_OK;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Validates a problem view: sizes, camera_block_size 9, index ranges and finite inputs.
+int validate_view(const ba_problem_view* v, bool need_params);
+
+inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// Sets the calling thread's device; returns BA_ERR_NO_DEVICE if it does not exist.
+int use_device(int device);
+
+}  // namespace mba

@@ -1,0 +1,171 @@
+// eval.hip -- per-observation Snavely residual + Jacobian on the GPU (the parity entry point that
+// replaces one AutoDiffCostFunction<SnavelyReprojectionError,2,9,3> evaluation per residual block,
+// utils/camera/SnavelyReprojectionError.h:42-92), and the FP64 cost 1/2 sum rho(|r|^2).
+//
+// Layout: observations in caller order; cameras AoS (9 doubles) as in BALProblem.  One thread per
+// observation, camera prep computed once per camera by k_camera_prep_aos.
+#include "common.h"
+#include "snavely.h"
+
+namespace mba {
+
+template <typename Real>
+__global__ void k_camera_prep_aos(const double* __restrict__ cams, int nc, CameraPrep<Real>* __restrict__ prep) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double cam[9];
+    for (int i = 0; i < 9; ++i) cam[i] = cams[9 * (size_t)c + i];
+    CameraPrep<Real> P;
+    camera_prep<Real>(cam, P);
+    prep[c] = P;
+}
+
+template <typename Real>
+__global__ void k_eval_obs(int no, const int* __restrict__ ci, const int* __restrict__ pi,
+                           const double* __restrict__ obs, const double* __restrict__ pts,
+                           const CameraPrep<Real>* __restrict__ prep, double* __restrict__ r_out,
+                           double* __restrict__ jc_out, double* __restrict__ jp_out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= no) return;
+    const CameraPrep<Real> P = prep[ci[k]];
+    const int p = pi[k];
+    const double X[3] = {pts[3 * (size_t)p], pts[3 * (size_t)p + 1], pts[3 * (size_t)p + 2]};
+    Real r[2], Jc[18], Jp[6];
+    snavely_residual_jacobian<Real>(P, X, (Real)obs[2 * (size_t)k], (Real)obs[2 * (size_t)k + 1], r, Jc, Jp);
+    r_out[2 * (size_t)k] = (double)r[0];
+    r_out[2 * (size_t)k + 1] = (double)r[1];
+    if (jc_out)
+        for (int i = 0; i < 18; ++i) jc_out[18 * (size_t)k + i] = (double)Jc[i];
+    if (jp_out)
+        for (int i = 0; i < 6; ++i) jp_out[6 * (size_t)k + i] = (double)Jp[i];
+}
+
+// Per-block partial sums of 1/2 rho(|r|^2) in FP64, reduced in a fixed order by k_sum_partials.
+__global__ void k_cost_obs(int no, const int* __restrict__ ci, const int* __restrict__ pi,
+                           const double* __restrict__ obs, const double* __restrict__ pts,
+                           const CameraPrep<double>* __restrict__ prep, int loss, double a,
+                           double* __restrict__ partial) {
+    __shared__ double red[256];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double c = 0.0;
+    if (k < no) {
+        const CameraPrep<double> P = prep[ci[k]];
+        const int p = pi[k];
+        const double X[3] = {pts[3 * (size_t)p], pts[3 * (size_t)p + 1], pts[3 * (size_t)p + 2]};
+        double r[2];
+        snavely_residual<double>(P, X, obs[2 * (size_t)k], obs[2 * (size_t)k + 1], r);
+        c = half_rho(r[0] * r[0] + r[1] * r[1], loss, a);
+    }
+    red[threadIdx.x] = c;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_sum_partials(const double* __restrict__ partial, int n, double* __restrict__ out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += partial[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+namespace {
+
+struct EvalUpload {
+    DevBuf<int> ci, pi;
+    DevBuf<double> obs, cams, pts;
+    int upload(const ba_problem_view* v, hipStream_t s) {
+        BA_TRY(ci.upload(v->camera_index, (size_t)v->num_observations, s));
+        BA_TRY(pi.upload(v->point_index, (size_t)v->num_observations, s));
+        BA_TRY(obs.upload(v->observations, 2 * (size_t)v->num_observations, s));
+        BA_TRY(cams.upload(v->cameras, 9 * (size_t)v->num_cameras, s));
+        BA_TRY(pts.upload(v->points, 3 * (size_t)v->num_points, s));
+        return BA_OK;
+    }
+};
+
+struct StreamGuard {
+    hipStream_t s = nullptr;
+    ~StreamGuard() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+template <typename Real>
+int eval_impl(const ba_problem_view* v, double* residuals, double* jc, double* jp, hipStream_t s) {
+    EvalUpload u;
+    BA_TRY(u.upload(v, s));
+    const int no = v->num_observations, nc = v->num_cameras;
+    DevBuf<CameraPrep<Real>> prep;
+    DevBuf<double> r, Jc, Jp;
+    BA_TRY(prep.alloc(nc));
+    BA_TRY(r.alloc(2 * (size_t)no));
+    if (jc) BA_TRY(Jc.alloc(18 * (size_t)no));
+    if (jp) BA_TRY(Jp.alloc(6 * (size_t)no));
+    hipLaunchKernelGGL(k_camera_prep_aos<Real>, dim3(div_up(nc, 64)), dim3(64), 0, s, u.cams.p, nc, prep.p);
+    BA_HIP(hipGetLastError());
+    if (no > 0) {
+        hipLaunchKernelGGL(k_eval_obs<Real>, dim3(div_up(no, 256)), dim3(256), 0, s, no, u.ci.p, u.pi.p, u.obs.p,
+                           u.pts.p, prep.p, r.p, jc ? Jc.p : nullptr, jp ? Jp.p : nullptr);
+        BA_HIP(hipGetLastError());
+    }
+    BA_HIP(hipMemcpyAsync(residuals, r.p, 2 * (size_t)no * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (jc) BA_HIP(hipMemcpyAsync(jc, Jc.p, 18 * (size_t)no * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (jp) BA_HIP(hipMemcpyAsync(jp, Jp.p, 6 * (size_t)no * sizeof(double), hipMemcpyDeviceToHost, s));
+    BA_HIP(hipStreamSynchronize(s));
+    return BA_OK;
+}
+
+}  // namespace
+}  // namespace mba
+
+using namespace mba;
+
+extern "C" int ba_eval_residual_jacobian(const ba_problem_view* v, int32_t precision, double* residuals,
+                                         double* jac_camera, double* jac_point) {
+    clear_error();
+    BA_TRY(validate_view(v, true));
+    if (!residuals) return fail(BA_ERR_INVALID_ARGUMENT, "residuals output is NULL");
+    if (precision != BA_PRECISION_FP64 && precision != BA_PRECISION_MIXED)
+        return fail(BA_ERR_INVALID_ARGUMENT, "unknown precision %d", precision);
+    BA_TRY(use_device(0));
+    StreamGuard g;
+    BA_HIP(hipStreamCreateWithFlags(&g.s, hipStreamNonBlocking));
+    if (precision == BA_PRECISION_FP64) return eval_impl<double>(v, residuals, jac_camera, jac_point, g.s);
+    return eval_impl<float>(v, residuals, jac_camera, jac_point, g.s);
+}
+
+extern "C" int ba_eval_cost(const ba_problem_view* v, int32_t loss, double loss_scale, double* cost) {
+    clear_error();
+    BA_TRY(validate_view(v, true));
+    if (!cost) return fail(BA_ERR_INVALID_ARGUMENT, "cost output is NULL");
+    BA_TRY(use_device(0));
+    StreamGuard g;
+    BA_HIP(hipStreamCreateWithFlags(&g.s, hipStreamNonBlocking));
+    EvalUpload u;
+    BA_TRY(u.upload(v, g.s));
+    const int no = v->num_observations, nc = v->num_cameras;
+    DevBuf<CameraPrep<double>> prep;
+    BA_TRY(prep.alloc(nc));
+    const int nb = no > 0 ? div_up(no, 256) : 1;
+    DevBuf<double> partial, out;
+    BA_TRY(partial.alloc(nb));
+    BA_TRY(out.alloc(1));
+    hipLaunchKernelGGL(k_camera_prep_aos<double>, dim3(div_up(nc, 64)), dim3(64), 0, g.s, u.cams.p, nc, prep.p);
+    hipLaunchKernelGGL(k_cost_obs, dim3(nb), dim3(256), 0, g.s, no, u.ci.p, u.pi.p, u.obs.p, u.pts.p, prep.p, loss,
+                       loss_scale, partial.p);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, g.s, partial.p, nb, out.p);
+    BA_HIP(hipGetLastError());
+    BA_HIP(hipMemcpyAsync(cost, out.p, sizeof(double), hipMemcpyDeviceToHost, g.s));
+    BA_HIP(hipStreamSynchronize(g.s));
+    return BA_OK;
+}

@@ -1,0 +1,37 @@
+// solver.h -- host-side declarations shared by the single-problem solver sources.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "common.h"
+
+namespace mba {
+
+// Device layout built on the host once per problem (see lm_kernels.h for its meaning).
+struct Layout {
+    int nc = 0, np = 0, no = 0, n_chunks = 0;
+    std::vector<int> pt_perm;    // internal point -> caller point
+    std::vector<int> co_orig;    // camera-order observation -> caller observation
+    std::vector<int> cam_off;    // nc + 1
+    std::vector<int> co_pt;      // internal point of each camera-order observation
+    std::vector<double> co_uv;   // x, y per camera-order observation
+    std::vector<int> chunk_off, chunk_len, slot_j;
+};
+
+int build_layout(const ba_problem_view* v, Layout* L);
+
+struct SolverBase {
+    virtual ~SolverBase() = default;
+    virtual int run(ba_summary* summary) = 0;
+    virtual int reset() = 0;
+    virtual int download(double* cameras, double* points) = 0;
+    virtual int set_profiling(bool on) = 0;
+    virtual int kernel_times(ba_kernel_time* out, int cap, int* count) = 0;
+    virtual int64_t device_bytes() const = 0;
+    virtual int solver_kind() const = 0;
+};
+
+int make_solver(const ba_problem_view* v, const ba_options& o, std::unique_ptr<SolverBase>* out);
+
+}  // namespace mba

@@ -1,0 +1,602 @@
+// solver.hip -- host side of the device-resident LM bundle-adjustment solve (ba_solve and the
+// ba_solver_* handle API).  Replaces SolveBA(BALProblem&) of
+// backend/slam_ba_simple_g2o_ceres/test_bundle_adjustment_ceres.cpp:58-101: one Snavely residual
+// per observation (:70-87) with HuberLoss(1.0) (:78), parameters updated in place (:83-84),
+// Levenberg-Marquardt with points eliminated by the Schur complement (:96-100).
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "dense.h"
+#include "lm_kernels.h"
+#include "solver.h"
+
+namespace mba {
+
+namespace {
+
+Opts make_opts(const ba_options& o) {
+    Opts r{};
+    r.loss = o.loss;
+    r.max_iterations = o.max_num_iterations;
+    r.max_invalid = o.max_num_consecutive_invalid_steps;
+    r.jacobi_scaling = o.jacobi_scaling;
+    r.pcg_max_iterations = o.pcg_max_iterations;
+    r.loss_a = o.loss_scale;
+    r.ftol = o.function_tolerance;
+    r.gtol = o.gradient_tolerance;
+    r.ptol = o.parameter_tolerance;
+    r.max_radius = o.max_trust_region_radius;
+    r.min_radius = o.min_trust_region_radius;
+    r.min_rel_decrease = o.min_relative_decrease;
+    r.min_diag = o.min_lm_diagonal;
+    r.max_diag = o.max_lm_diagonal;
+    r.pcg_tol = o.pcg_tolerance;
+    return r;
+}
+
+int check_options(const ba_options* o) {
+    if (!o) return fail(BA_ERR_INVALID_ARGUMENT, "options is NULL");
+    if (o->max_num_iterations < 0) return fail(BA_ERR_INVALID_ARGUMENT, "max_num_iterations < 0");
+    if (o->max_num_consecutive_invalid_steps < 1) return fail(BA_ERR_INVALID_ARGUMENT, "max_num_consecutive_invalid_steps < 1");
+    if (o->loss != BA_LOSS_NONE && o->loss != BA_LOSS_HUBER) return fail(BA_ERR_INVALID_ARGUMENT, "unknown loss %d", o->loss);
+    if (o->loss == BA_LOSS_HUBER && !(o->loss_scale > 0.0)) return fail(BA_ERR_INVALID_ARGUMENT, "loss_scale must be > 0");
+    if (o->precision != BA_PRECISION_FP64 && o->precision != BA_PRECISION_MIXED)
+        return fail(BA_ERR_INVALID_ARGUMENT, "unknown precision %d", o->precision);
+    if (o->linear_solver < BA_SOLVER_AUTO || o->linear_solver > BA_SOLVER_ITERATIVE_SCHUR)
+        return fail(BA_ERR_INVALID_ARGUMENT, "unknown linear solver %d", o->linear_solver);
+    if (!(o->initial_trust_region_radius > 0.0)) return fail(BA_ERR_INVALID_ARGUMENT, "initial_trust_region_radius must be > 0");
+    if (!(o->min_lm_diagonal > 0.0) || !(o->max_lm_diagonal >= o->min_lm_diagonal))
+        return fail(BA_ERR_INVALID_ARGUMENT, "invalid LM diagonal bounds");
+    if (o->pcg_max_iterations < 0) return fail(BA_ERR_INVALID_ARGUMENT, "pcg_max_iterations < 0");
+    return BA_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// Host-built device layout shared by the FP64 and mixed solvers.
+int build_layout(const ba_problem_view* v, Layout* L) {
+    const int nc = v->num_cameras, np = v->num_points, no = v->num_observations;
+    L->nc = nc;
+    L->np = np;
+    L->no = no;
+    std::vector<int> track(np, 0);
+    for (int k = 0; k < no; ++k) track[v->point_index[k]]++;
+    // points by track length, longest first (ties by index): one wave = 64 similar tracks
+    L->pt_perm.resize(np);
+    std::iota(L->pt_perm.begin(), L->pt_perm.end(), 0);
+    std::stable_sort(L->pt_perm.begin(), L->pt_perm.end(), [&](int a, int b) { return track[a] > track[b]; });
+    std::vector<int> inv(np);
+    for (int i = 0; i < np; ++i) inv[L->pt_perm[i]] = i;
+    // observations by (camera, internal point)
+    std::vector<int64_t> key(no);
+    for (int k = 0; k < no; ++k) key[k] = ((int64_t)v->camera_index[k] << 32) | (int64_t)inv[v->point_index[k]];
+    L->co_orig.resize(no);
+    std::iota(L->co_orig.begin(), L->co_orig.end(), 0);
+    std::sort(L->co_orig.begin(), L->co_orig.end(), [&](int a, int b) { return key[a] < key[b] || (key[a] == key[b] && a < b); });
+    L->cam_off.assign(nc + 1, 0);
+    L->co_pt.resize(no);
+    L->co_uv.resize(2 * (size_t)no);
+    for (int j = 0; j < no; ++j) {
+        const int k = L->co_orig[j];
+        L->cam_off[v->camera_index[k] + 1]++;
+        L->co_pt[j] = inv[v->point_index[k]];
+        L->co_uv[2 * (size_t)j] = v->observations[2 * (size_t)k];
+        L->co_uv[2 * (size_t)j + 1] = v->observations[2 * (size_t)k + 1];
+    }
+    for (int c = 0; c < nc; ++c) L->cam_off[c + 1] += L->cam_off[c];
+    // SELL-64 slots of camera-order indices per internal point
+    L->n_chunks = (np + 63) / 64;
+    L->chunk_len.assign(L->n_chunks, 0);
+    L->chunk_off.assign(L->n_chunks + 1, 0);
+    for (int ch = 0; ch < L->n_chunks; ++ch) {
+        L->chunk_len[ch] = track[L->pt_perm[ch * 64]];
+        L->chunk_off[ch + 1] = L->chunk_off[ch] + 64 * L->chunk_len[ch];
+    }
+    L->slot_j.assign((size_t)L->chunk_off[L->n_chunks], -1);
+    std::vector<int> fill(np, 0);
+    for (int j = 0; j < no; ++j) {
+        const int ip = L->co_pt[j];
+        const int ch = ip >> 6, lane = ip & 63;
+        L->slot_j[(size_t)L->chunk_off[ch] + 64 * (size_t)fill[ip] + lane] = j;
+        fill[ip]++;
+    }
+    return BA_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+template <typename Real>
+struct Solver final : SolverBase {
+    ba_options opt{};
+    hipStream_t s = nullptr;
+    int device = 0;
+    int linear_solver = BA_SOLVER_ITERATIVE_SCHUR;
+    Layout L;
+    Dev<Real> d{};
+    DevBuf<int> cam_off, co_pt, chunk_off, chunk_len, slot_j;
+    DevBuf<double> co_uv64;
+    DevBuf<Real> co_uvr;
+    DevBuf<double> cam_x, pt_x, cam_x0, pt_x0, cam_c, pt_c, dcam, dpt;
+    DevBuf<CameraPrep<Real>> prep;
+    DevBuf<CameraPrep<double>> prep_c;
+    DevBuf<double> U, gc, sc, Ustar, M, Minv, bc, px, pr, pz, pp;
+    DevBuf<double> V, gp, sp, Vinv64;
+    DevBuf<Real> VinvR, qp, zs, pv, u;
+    DevBuf<double> cam_part, pt_part;
+    DevBuf<LMState> st;
+    DevBuf<ba_iteration> trace;
+    LMState* st_host = nullptr;  // pinned
+    DenseSchur<Real> dense;
+    // profiling
+    bool profiling = false;
+    struct Rec {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> event_pool;
+    size_t event_next = 0;
+    std::vector<ba_kernel_time> ktimes;
+
+    ~Solver() override {
+        if (st_host) (void)hipHostFree(st_host);
+        for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+        if (s) (void)hipStreamDestroy(s);
+    }
+
+    int64_t device_bytes() const override {
+        int64_t b = 0;
+        b += cam_off.bytes() + co_pt.bytes() + chunk_off.bytes() + chunk_len.bytes() + slot_j.bytes();
+        b += co_uv64.bytes() + co_uvr.bytes();
+        b += cam_x.bytes() * 4 + pt_x.bytes() * 4 + dcam.bytes() + dpt.bytes();
+        b += prep.bytes() + prep_c.bytes();
+        b += U.bytes() * 4 + gc.bytes() * 7;
+        b += V.bytes() + gp.bytes() + sp.bytes() + Vinv64.bytes();
+        b += VinvR.bytes() + qp.bytes() + zs.bytes() + pv.bytes() + u.bytes();
+        b += cam_part.bytes() + pt_part.bytes() + dense.device_bytes();
+        return b;
+    }
+    int solver_kind() const override { return linear_solver; }
+
+    int create(const ba_problem_view* v, const ba_options& o) {
+        opt = o;
+        device = o.device;
+        BA_TRY(use_device(device));
+        BA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        BA_HIP(hipHostMalloc((void**)&st_host, sizeof(LMState), hipHostMallocDefault));
+        BA_TRY(build_layout(v, &L));
+        const int nc = L.nc, np = L.np, no = L.no;
+        linear_solver = o.linear_solver;
+        if (linear_solver == BA_SOLVER_AUTO)
+            linear_solver = nc <= o.dense_schur_max_cameras ? BA_SOLVER_DENSE_SCHUR : BA_SOLVER_ITERATIVE_SCHUR;
+        BA_TRY(cam_off.upload(L.cam_off.data(), L.cam_off.size(), s));
+        BA_TRY(co_pt.upload(L.co_pt.data(), L.co_pt.size(), s));
+        BA_TRY(chunk_off.upload(L.chunk_off.data(), L.chunk_off.size(), s));
+        BA_TRY(chunk_len.upload(L.chunk_len.data(), L.chunk_len.size(), s));
+        BA_TRY(slot_j.upload(L.slot_j.data(), L.slot_j.size(), s));
+        BA_TRY(co_uv64.upload(L.co_uv.data(), L.co_uv.size(), s));
+        if (sizeof(Real) != sizeof(double)) {
+            std::vector<Real> uvr(L.co_uv.begin(), L.co_uv.end());
+            BA_TRY(co_uvr.upload(uvr.data(), uvr.size(), s));
+            BA_HIP(hipStreamSynchronize(s));
+        }
+        // parameters: SoA, points in internal order
+        std::vector<double> cx(9 * (size_t)nc), px_(3 * (size_t)np);
+        for (int c = 0; c < nc; ++c)
+            for (int i = 0; i < 9; ++i) cx[(size_t)i * nc + c] = v->cameras[9 * (size_t)c + i];
+        for (int ip = 0; ip < np; ++ip)
+            for (int i = 0; i < 3; ++i) px_[(size_t)i * np + ip] = v->points[3 * (size_t)L.pt_perm[ip] + i];
+        BA_TRY(cam_x0.upload(cx.data(), cx.size(), s));
+        BA_TRY(pt_x0.upload(px_.data(), px_.size(), s));
+        BA_HIP(hipStreamSynchronize(s));
+        BA_TRY(cam_x.alloc(9 * (size_t)nc));
+        BA_TRY(pt_x.alloc(3 * (size_t)np));
+        BA_TRY(cam_c.alloc(9 * (size_t)nc));
+        BA_TRY(pt_c.alloc(3 * (size_t)np));
+        BA_TRY(dcam.alloc(9 * (size_t)nc));
+        BA_TRY(dpt.alloc(3 * (size_t)np));
+        BA_TRY(prep.alloc(nc));
+        BA_TRY(prep_c.alloc(nc));
+        for (DevBuf<double>* b : {&U, &Ustar, &M, &Minv}) BA_TRY(b->alloc(81 * (size_t)nc));
+        for (DevBuf<double>* b : {&gc, &sc, &bc, &px, &pr, &pz, &pp}) BA_TRY(b->alloc(9 * (size_t)nc));
+        BA_TRY(V.alloc(6 * (size_t)np));
+        BA_TRY(gp.alloc(3 * (size_t)np));
+        BA_TRY(sp.alloc(3 * (size_t)np));
+        BA_TRY(Vinv64.alloc(6 * (size_t)np));
+        BA_TRY(VinvR.alloc(6 * (size_t)np));
+        BA_TRY(qp.alloc(3 * (size_t)np));
+        BA_TRY(zs.alloc(3 * (size_t)np));
+        BA_TRY(pv.alloc(9 * (size_t)no));
+        BA_TRY(u.alloc(3 * (size_t)no));
+        const int n_pt_blocks = std::max(1, div_up(std::max(np, 1), kWG));
+        BA_TRY(cam_part.alloc(PS_COUNT * (size_t)std::max(nc, 1)));
+        BA_TRY(pt_part.alloc(PS_COUNT * (size_t)n_pt_blocks));
+        BA_HIP(hipMemsetAsync(cam_part.p, 0, cam_part.bytes(), s));
+        BA_HIP(hipMemsetAsync(pt_part.p, 0, pt_part.bytes(), s));
+        BA_TRY(st.alloc(1));
+        const int trace_cap = o.max_num_iterations + 2;
+        BA_TRY(trace.alloc(trace_cap));
+        d.nc = nc;
+        d.np = np;
+        d.no = no;
+        d.n_chunks = L.n_chunks;
+        d.n_pt_blocks = n_pt_blocks;
+        d.o = make_opts(o);
+        d.cam_off = cam_off.p;
+        d.co_pt = co_pt.p;
+        d.co_uv64 = co_uv64.p;
+        d.co_uv = sizeof(Real) == sizeof(double) ? (const Real*)co_uv64.p : co_uvr.p;
+        d.chunk_off = chunk_off.p;
+        d.chunk_len = chunk_len.p;
+        d.slot_j = slot_j.p;
+        d.cam_x = cam_x.p;
+        d.pt_x = pt_x.p;
+        d.cam_c = cam_c.p;
+        d.pt_c = pt_c.p;
+        d.dcam = dcam.p;
+        d.dpt = dpt.p;
+        d.prep = prep.p;
+        d.prep_c = prep_c.p;
+        d.U = U.p;
+        d.gc = gc.p;
+        d.sc = sc.p;
+        d.Ustar = Ustar.p;
+        d.M = M.p;
+        d.Minv = Minv.p;
+        d.bc = bc.p;
+        d.px = px.p;
+        d.pr = pr.p;
+        d.pz = pz.p;
+        d.pp = pp.p;
+        d.V = V.p;
+        d.gp = gp.p;
+        d.sp = sp.p;
+        d.Vinv64 = Vinv64.p;
+        d.VinvR = VinvR.p;
+        d.qp = qp.p;
+        d.zs = zs.p;
+        d.pv = pv.p;
+        d.u = u.p;
+        d.cam_part = cam_part.p;
+        d.pt_part = pt_part.p;
+        d.st = st.p;
+        d.trace = trace.p;
+        d.trace_cap = trace_cap;
+        if (linear_solver == BA_SOLVER_DENSE_SCHUR) BA_TRY(dense.create(v, L, o, s));
+        BA_TRY(reset());
+        return BA_OK;
+    }
+
+    int reset() override {
+        BA_HIP(hipMemcpyAsync(cam_x.p, cam_x0.p, cam_x0.bytes(), hipMemcpyDeviceToDevice, s));
+        BA_HIP(hipMemcpyAsync(pt_x.p, pt_x0.p, pt_x0.bytes(), hipMemcpyDeviceToDevice, s));
+        BA_HIP(hipStreamSynchronize(s));
+        return BA_OK;
+    }
+
+    int download(double* cams, double* pts) override {
+        const int nc = L.nc, np = L.np;
+        std::vector<double> cx(9 * (size_t)nc), px_(3 * (size_t)np);
+        if (nc) BA_HIP(hipMemcpyAsync(cx.data(), cam_x.p, cx.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (np) BA_HIP(hipMemcpyAsync(px_.data(), pt_x.p, px_.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        BA_HIP(hipStreamSynchronize(s));
+        if (cams)
+            for (int c = 0; c < nc; ++c)
+                for (int i = 0; i < 9; ++i) cams[9 * (size_t)c + i] = cx[(size_t)i * nc + c];
+        if (pts)
+            for (int ip = 0; ip < np; ++ip)
+                for (int i = 0; i < 3; ++i) pts[3 * (size_t)L.pt_perm[ip] + i] = px_[(size_t)i * np + ip];
+        return BA_OK;
+    }
+
+    // ---- launch helpers ----
+    hipEvent_t next_event() {
+        if (event_next == event_pool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            event_pool.push_back(e);
+        }
+        return event_pool[event_next++];
+    }
+    template <typename K, typename... Args>
+    void launch_shm(const char* name, K kernel, int grid, int block, size_t shm, Args... args) {
+        if (grid <= 0) return;
+        hipEvent_t a = nullptr, b = nullptr;
+        if (profiling) {
+            a = next_event();
+            b = next_event();
+            if (a) (void)hipEventRecord(a, s);
+        }
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shm, s, args...);
+        if (profiling && a && b) {
+            (void)hipEventRecord(b, s);
+            recs.push_back({name, a, b});
+        }
+    }
+    template <typename K, typename... Args>
+    void launch(const char* name, K kernel, int grid, int block, Args... args) {
+        launch_shm(name, kernel, grid, block, 0, args...);
+    }
+
+    void enqueue_linearize() {
+        const int nc = L.nc, gpt = div_up(std::max(L.n_chunks * 64, 1), kWG);
+        launch("cam_prep", k_cam_prep<Real>, div_up(nc, 64), 64, d);
+        launch("lin_cam", k_lin_cam<Real>, nc, kWG, d);
+        launch("lin_pt", k_lin_pt<Real>, gpt, kWG, d);
+    }
+
+    void enqueue_iteration() {
+        const int nc = L.nc, np = L.np;
+        const int gpt = div_up(std::max(L.n_chunks * 64, 1), kWG);
+        launch("damp_pt", k_damp_pt<Real>, div_up(std::max(np, 1), kWG), kWG, d);
+        launch("rhs_cam", k_rhs_cam<Real>, nc, kWG, d);
+        if (linear_solver == BA_SOLVER_DENSE_SCHUR) {
+            dense.enqueue_solve(*this);
+        } else {
+            launch("cam_factor", k_cam_factor<Real>, div_up(nc, 64), 64, d);
+            launch("pcg_beta", k_pcg_beta<Real>, 1, kWG, d, 1);
+            for (int it = 0; it < opt.pcg_max_iterations; ++it) {
+                launch("jtx", k_jtx<Real>, nc, kWG, d, 1, (const double*)d.pp);
+                launch("pt_z", k_pt_z<Real>, gpt, kWG, d);
+                launch("pcg_alpha", k_pcg_alpha<Real>, 1, kWG, d);
+                launch("jz", k_jz<Real>, nc, kWG, d);
+                launch("pcg_beta", k_pcg_beta<Real>, 1, kWG, d, 0);
+            }
+        }
+        launch("jtx_y", k_jtx<Real>, nc, kWG, d, 0, (const double*)d.px);
+        launch("pt_backsub", k_pt_backsub<Real>, gpt, kWG, d);
+        launch("cam_step", k_cam_step<Real>, div_up(nc, 64), 64, d);
+        launch("model_cand", k_model_cand<Real>, nc, kWG, d);
+        launch("decide", k_decide<Real>, 1, kWG, d);
+        launch("apply", k_apply<Real>, div_up(9 * (int64_t)nc + 3 * (int64_t)np, kWG), kWG, d);
+        enqueue_linearize();
+        launch("post_lin", k_post_lin<Real>, 1, kWG, d);
+    }
+
+    int run(ba_summary* summary) override {
+        const auto t0 = std::chrono::steady_clock::now();
+        BA_TRY(use_device(device));
+        recs.clear();
+        event_next = 0;
+        LMState init{};
+        init.radius = opt.initial_trust_region_radius;
+        init.decrease_factor = 2.0;
+        init.need_lin = 1;
+        init.first_lin = 1;
+        init.termination = BA_NO_CONVERGENCE;
+        *st_host = init;
+        BA_HIP(hipMemcpyAsync(st.p, st_host, sizeof(LMState), hipMemcpyHostToDevice, s));
+        const int nc = L.nc;
+        if (L.no == 0) {
+            // nothing to optimise: cost 0, gradient 0
+            BA_HIP(hipStreamSynchronize(s));
+            if (summary) fill_empty(summary);
+            return BA_OK;
+        }
+        enqueue_linearize();
+        launch("prep_x", k_prep_from<Real>, div_up(nc, 64), 64, d, (const double*)d.cam_x);
+        launch("cost_x", k_cost_cam<Real>, nc, kWG, d, (const double*)d.pt_x);
+        launch("post_lin", k_post_lin<Real>, 1, kWG, d);
+        BA_HIP(hipGetLastError());
+        const int check_every = 4;
+        for (int it = 0; it < opt.max_num_iterations; ++it) {
+            enqueue_iteration();
+            BA_HIP(hipGetLastError());
+            if ((it + 1) % check_every == 0 || it + 1 == opt.max_num_iterations) {
+                BA_HIP(hipMemcpyAsync(st_host, st.p, sizeof(LMState), hipMemcpyDeviceToHost, s));
+                BA_HIP(hipStreamSynchronize(s));
+                if (st_host->done) break;
+            }
+        }
+        BA_HIP(hipMemcpyAsync(st_host, st.p, sizeof(LMState), hipMemcpyDeviceToHost, s));
+        BA_HIP(hipStreamSynchronize(s));
+        const auto t1 = std::chrono::steady_clock::now();
+        if (profiling) collect_profile();
+        if (summary) BA_TRY(fill_summary(summary, std::chrono::duration<double, std::milli>(t1 - t0).count()));
+        if (opt.verbose) print_trace();
+        return BA_OK;
+    }
+
+    void fill_empty(ba_summary* sm) {
+        sm->initial_cost = sm->final_cost = 0.0;
+        sm->num_iterations = 0;
+        sm->num_successful_steps = sm->num_unsuccessful_steps = 0;
+        sm->termination_type = BA_CONVERGENCE;
+        sm->linear_solver_used = linear_solver;
+        sm->total_linear_solver_iterations = 0;
+        sm->total_time_ms = 0.0;
+    }
+
+    int fill_summary(ba_summary* sm, double ms) {
+        const LMState& S = *st_host;
+        const int n = std::min(S.trace_len, d.trace_cap);
+        std::vector<ba_iteration> rows(n);
+        if (n) BA_HIP(hipMemcpy(rows.data(), trace.p, n * sizeof(ba_iteration), hipMemcpyDeviceToHost));
+        sm->initial_cost = n ? rows[0].cost : S.x_cost;
+        sm->final_cost = S.x_cost;
+        sm->num_iterations = S.trace_len;
+        sm->num_successful_steps = S.num_successful;
+        sm->num_unsuccessful_steps = S.num_unsuccessful;
+        sm->termination_type = S.termination;
+        sm->linear_solver_used = linear_solver;
+        sm->total_linear_solver_iterations = S.total_pcg_iters;
+        sm->total_time_ms = ms;
+        if (sm->trace && sm->trace_capacity > 0) {
+            const int m = std::min(n, sm->trace_capacity);
+            std::copy(rows.begin(), rows.begin() + m, sm->trace);
+        }
+        last_rows = rows;
+        return BA_OK;
+    }
+
+    std::vector<ba_iteration> last_rows;
+
+    void print_trace() {
+        printf("iter      cost      cost_change  |gradient|   |step|    tr_ratio  tr_radius  ls_iter\n");
+        for (const ba_iteration& r : last_rows)
+            printf("%4d % 8.6e % 3.2e % 3.2e % 3.2e % 3.2e % 3.2e %4d\n", r.iteration, r.cost, r.cost_change,
+                   r.gradient_max_norm, r.step_norm, r.relative_decrease, r.trust_region_radius,
+                   r.linear_solver_iterations);
+        fflush(stdout);
+    }
+
+    // ---- profiling ----
+    int set_profiling(bool on) override {
+        profiling = on;
+        ktimes.clear();
+        return BA_OK;
+    }
+
+    double bytes_of(const char* name) const {
+        const double R = sizeof(Real), no = L.no, np = L.np, nc = L.nc;
+        const std::string n(name);
+        // Algorithmic bytes of one launch: every stream counted once per pass, per-camera and per-point
+        // parameters once per pass, FP64 parameters, int32 indices.
+        if (n == "jtx" || n == "jtx_y") return no * (4 + 2 * R + 3 * R) + np * 3 * 8 + nc * 9 * 8 * 3;
+        if (n == "jz") return no * (4 + 2 * R) + np * (3 * 8 + 3 * R) + nc * 9 * 8 * 8;
+        if (n == "pt_z") return (no + 63.0 * L.n_chunks / 2) * (4 + 3 * R) + np * (3 * 8 + 6 * R + 3 * R);
+        if (n == "lin_cam") return no * (4 + 2 * R + 9 * R) + np * 3 * 8 + nc * (9 + 81 + 9) * 8;
+        if (n == "lin_pt") return no * (4 + 9 * R) + np * 9 * 8;
+        if (n == "rhs_cam") return no * (4 + 2 * R) + np * (3 * 8 + 3 * 8 + 6 * R + 3 * R) + nc * (81 * 3 + 9 * 3) * 8;
+        if (n == "model_cand") return no * (4 + 2 * R + 16) + np * 3 * 8 * 3 + nc * 9 * 8;
+        return 0.0;
+    }
+
+    void collect_profile() {
+        (void)hipStreamSynchronize(s);
+        for (const Rec& r : recs) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) continue;
+            auto it = std::find_if(ktimes.begin(), ktimes.end(), [&](const ba_kernel_time& k) { return strcmp(k.name, r.name) == 0; });
+            if (it == ktimes.end()) {
+                ba_kernel_time k{};
+                snprintf(k.name, sizeof(k.name), "%s", r.name);
+                k.bytes_per_launch = bytes_of(r.name);
+                ktimes.push_back(k);
+                it = ktimes.end() - 1;
+            }
+            it->launches++;
+            it->total_ms += ms;
+        }
+        recs.clear();
+        event_next = 0;
+    }
+
+    int kernel_times(ba_kernel_time* out, int cap, int* count) override {
+        const int n = (int)ktimes.size();
+        if (count) *count = n;
+        for (int i = 0; i < std::min(n, cap); ++i) out[i] = ktimes[i];
+        return BA_OK;
+    }
+};
+
+int make_solver(const ba_problem_view* v, const ba_options& o, std::unique_ptr<SolverBase>* out) {
+    if (o.precision == BA_PRECISION_FP64) {
+        auto s = std::make_unique<Solver<double>>();
+        BA_TRY(s->create(v, o));
+        *out = std::move(s);
+    } else {
+        auto s = std::make_unique<Solver<float>>();
+        BA_TRY(s->create(v, o));
+        *out = std::move(s);
+    }
+    return BA_OK;
+}
+
+}  // namespace mba
+
+// =============================================================================================
+struct ba_solver {
+    std::unique_ptr<mba::SolverBase> impl;
+    ba_problem_view view{};
+};
+
+using namespace mba;
+
+extern "C" {
+
+int ba_solve(const ba_problem_view* v, const ba_options* options, ba_summary* summary) {
+    clear_error();
+    ba_options o;
+    if (options)
+        o = *options;
+    else
+        ba_default_options(&o);
+    BA_TRY(validate_view(v, true));
+    BA_TRY(check_options(&o));
+    std::unique_ptr<SolverBase> s;
+    BA_TRY(make_solver(v, o, &s));
+    BA_TRY(s->run(summary));
+    return s->download(v->cameras, v->points);
+}
+
+int ba_solver_create(const ba_problem_view* v, const ba_options* options, ba_solver** out) {
+    clear_error();
+    if (!out) return fail(BA_ERR_INVALID_ARGUMENT, "out is NULL");
+    ba_options o;
+    if (options)
+        o = *options;
+    else
+        ba_default_options(&o);
+    BA_TRY(validate_view(v, true));
+    BA_TRY(check_options(&o));
+    auto h = std::make_unique<ba_solver>();
+    BA_TRY(make_solver(v, o, &h->impl));
+    h->view = *v;
+    *out = h.release();
+    return BA_OK;
+}
+
+int ba_solver_run(ba_solver* h, ba_summary* summary) {
+    clear_error();
+    if (!h) return fail(BA_ERR_INVALID_ARGUMENT, "NULL solver");
+    return h->impl->run(summary);
+}
+
+int ba_solver_reset(ba_solver* h) {
+    clear_error();
+    if (!h) return fail(BA_ERR_INVALID_ARGUMENT, "NULL solver");
+    return h->impl->reset();
+}
+
+int ba_solver_download(ba_solver* h, double* cameras, double* points) {
+    clear_error();
+    if (!h) return fail(BA_ERR_INVALID_ARGUMENT, "NULL solver");
+    return h->impl->download(cameras, points);
+}
+
+int ba_solver_destroy(ba_solver* h) {
+    delete h;
+    return BA_OK;
+}
+
+int ba_solver_set_profiling(ba_solver* h, int32_t enabled) {
+    clear_error();
+    if (!h) return fail(BA_ERR_INVALID_ARGUMENT, "NULL solver");
+    return h->impl->set_profiling(enabled != 0);
+}
+
+int ba_solver_kernel_times(ba_solver* h, ba_kernel_time* out, int32_t capacity, int32_t* count) {
+    clear_error();
+    if (!h || (!out && capacity > 0)) return fail(BA_ERR_INVALID_ARGUMENT, "NULL argument");
+    int n = 0;
+    BA_TRY(h->impl->kernel_times(out, capacity, &n));
+    if (count) *count = n;
+    return BA_OK;
+}
+
+int ba_solver_info(ba_solver* h, int64_t* device_bytes, int32_t* linear_solver) {
+    clear_error();
+    if (!h) return fail(BA_ERR_INVALID_ARGUMENT, "NULL solver");
+    if (device_bytes) *device_bytes = h->impl->device_bytes();
+    if (linear_solver) *linear_solver = h->impl->solver_kind();
+    return BA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,176 @@
+"""Host layer of the engine (no GPU needed): BAL I/O, Normalize/Perturb, synthetic problems, the
+C ABI surface and its argument validation."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import slamplay_amd as S
+from slamplay_amd import _capi as C
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_ladybug49_known_answers(ladybug_path):
+    p = S.BALProblem(ladybug_path)
+    assert (p.num_cameras(), p.num_points(), p.num_observations()) == (49, 7776, 31843)
+    assert p.num_parameters() == 23769 == 9 * 49 + 3 * 7776
+    assert p.camera_block_size() == 9 and p.point_block_size() == 3
+    assert p.camera_index()[0] == 0 and p.point_index()[0] == 0
+    np.testing.assert_array_equal(p.observations()[:2], [-332.65, 262.09])
+    with open(ladybug_path) as fh:
+        lines = fh.read().splitlines()
+    assert len(lines) == 55613 == 1 + 31843 + 23769
+    assert float(lines[1 + 31843]) == p.mutable_cameras()[0]
+    assert float(lines[-1]) == p.mutable_points()[-1]
+    # observations are sorted by point (BAL convention); tracks 2..29 long
+    assert np.all(np.diff(p.point_index()) >= 0)
+    counts = np.bincount(p.point_index())
+    assert counts.min() == 2 and counts.max() == 29
+
+
+def test_bal_round_trip(ladybug_path, tmp_path):
+    p = S.BALProblem(ladybug_path)
+    out = tmp_path / "rt.txt"
+    p.WriteToFile(out)
+    q = S.BALProblem(out)
+    np.testing.assert_array_equal(q.camera_index(), p.camera_index())
+    np.testing.assert_array_equal(q.point_index(), p.point_index())
+    # parameters are written with %.16g as in the reference (BALProblem.cpp:169,177): 16 significant digits, so 5e-16 relative
+    np.testing.assert_allclose(q.mutable_cameras(), p.mutable_cameras(), rtol=1e-15, atol=0)
+    np.testing.assert_allclose(q.mutable_points(), p.mutable_points(), rtol=1e-15, atol=0)
+    np.testing.assert_allclose(q.observations(), p.observations(), rtol=1e-5)  # %g, as the reference
+    legacy = tmp_path / "legacy.txt"
+    p.WriteToFile(legacy, legacy_header=True)
+    with open(legacy) as fh:
+        assert fh.readline().split() == ["49", "49", "7776", "31843"]
+
+
+def test_quaternion_mode_round_trip(ladybug_path, tmp_path):
+    p = S.BALProblem(ladybug_path)
+    q = S.BALProblem(ladybug_path, use_quaternions=True)
+    assert q.camera_block_size() == 10 and q.num_parameters() == 10 * 49 + 3 * 7776
+    out = tmp_path / "q.txt"
+    q.WriteToFile(out)  # written back in angle-axis form
+    r = S.BALProblem(out)
+    np.testing.assert_allclose(r.mutable_cameras(), p.mutable_cameras(), rtol=1e-12, atol=1e-14)
+
+
+def test_ply_output(ladybug_path, tmp_path):
+    p = S.BALProblem(ladybug_path)
+    out = tmp_path / "x.ply"
+    p.WriteToPLYFile(out)
+    with open(out) as fh:
+        lines = fh.read().splitlines()
+    assert lines[0] == "ply" and lines[2] == "element vertex %d" % (49 + 7776) and lines[9] == "end_header"
+    assert len(lines) == 10 + 49 + 7776
+    assert lines[10].endswith(" 0 255 0") and lines[-1].endswith(" 255 255 255")
+
+
+def test_normalize_matches_oracle(oracle_mod, ladybug_path):
+    p = S.BALProblem(ladybug_path)
+    cams, pts = oracle_mod.normalize(49, 7776, p.mutable_cameras(), p.mutable_points())
+    p.Normalize()
+    np.testing.assert_array_equal(p.mutable_cameras(), cams)
+    np.testing.assert_array_equal(p.mutable_points(), pts)
+
+
+def test_perturb_matches_oracle(oracle_mod, ladybug_path):
+    p = S.BALProblem(ladybug_path)
+    p.Normalize()
+    cams, pts = oracle_mod.perturb(49, 7776, p.mutable_cameras(), p.mutable_points(), 0.1, 0.5, 0.5, seed=1)
+    oracle_mod.lib().oracle_srand(1)  # glibc's default seed, as the reference never calls srand
+    p.Perturb(0.1, 0.5, 0.5)
+    np.testing.assert_array_equal(p.mutable_cameras(), cams)
+    np.testing.assert_array_equal(p.mutable_points(), pts)
+
+
+def test_synthetic_configs_deterministic():
+    a = S.synthetic(1, seed=5)
+    b = S.synthetic(1, seed=5)
+    c = S.synthetic(1, seed=6)
+    np.testing.assert_array_equal(a.observations(), b.observations())
+    np.testing.assert_array_equal(a.mutable_cameras(), b.mutable_cameras())
+    assert not np.array_equal(a.observations(), c.observations())
+    assert a.num_cameras() == 16 and 1800 < a.num_points() <= 2000 and 13000 < a.num_observations() < 17000
+
+
+def test_synthetic_noise_level(oracle_mod):
+    for cfg in (1, 2, 4):
+        p, gt = S.synthetic(cfg, seed=2, ground_truth=True,
+                            **({"num_points": 3000} if cfg == 2 else {}))
+        nc = p.num_cameras()
+        r, _, _ = oracle_mod.eval_residual_jacobian(p.camera_index(), p.point_index(), p.observations(),
+                                                   gt[:9 * nc], gt[9 * nc:])
+        assert 0.45 < r.std() < 0.55  # pixel noise sigma 0.5
+        # every observation is in front of its camera (Snavely: p_z < 0)
+        cams = gt[:9 * nc].reshape(nc, 9)
+        pts = gt[9 * nc:].reshape(-1, 3)
+        for k in range(0, p.num_observations(), 97):
+            c, q = p.camera_index()[k], p.point_index()[k]
+            pz = (oracle_mod.rotate_point(cams[c, :3], pts[q]) + cams[c, 3:6])[2]
+            assert pz < 0
+        assert np.bincount(p.point_index()).min() >= 2
+
+
+def test_large_config_sizes():
+    p = S.synthetic(3, seed=1)
+    assert p.num_cameras() == 1500 and p.num_observations() == 1_200_000
+    counts = np.bincount(p.point_index())
+    assert counts.min() >= 2 and counts.max() <= 60
+
+
+def _header_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include", "mba")):
+        with open(os.path.join(ROOT, "include", "mba", fn)) as fh:
+            text = fh.read()
+        if fn.endswith(".h") and 'extern "C"' in text:
+            names |= set(re.findall(r"^\s*(?:int|int32_t|void|const char\*)\s+(ba_\w+)\s*\(", text, re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    names = _header_functions()
+    assert len(names) > 20
+    L = C.lib()
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    assert L.ba_abi_version() == 1
+
+
+def test_invalid_arguments_rejected_before_device_use(ladybug_path):
+    p = S.synthetic(1, seed=1)
+    ci = p.camera_index().copy()
+    ci[5] = 99
+    bad = S.BALProblem.from_arrays(ci, p.point_index(), p.observations(), p.mutable_cameras(), p.mutable_points())
+    with pytest.raises(S.BAError) as e:
+        S.SolveBA(bad)
+    assert e.value.status == 1 and "camera index 99" in str(e.value)
+    obs = p.observations().copy()
+    obs[7] = np.nan
+    bad = S.BALProblem.from_arrays(p.camera_index(), p.point_index(), obs, p.mutable_cameras(), p.mutable_points())
+    with pytest.raises(S.BAError) as e:
+        S.SolveBA(bad)
+    assert e.value.status == 3
+    q = S.BALProblem(ladybug_path, use_quaternions=True)
+    with pytest.raises(S.BAError) as e:
+        S.SolveBA(q)
+    assert e.value.status == 1 and "camera_block_size" in str(e.value)
+    with pytest.raises(S.BAError) as e:
+        S.SolveBA(p, S.default_options(loss=7))
+    assert e.value.status == 1
+    with pytest.raises(S.BAError) as e:
+        S.BALProblem("/nonexistent/problem.txt")
+    assert e.value.status == 7
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    if S.device_count() > 0:
+        pytest.skip("a GPU is present")
+    p = S.synthetic(1, seed=1)
+    with pytest.raises(S.BAError) as e:
+        S.SolveBA(p)
+    assert e.value.status == 5

@@ -1,0 +1,235 @@
+"""Pins the CPU FP64 oracle (oracle/ba_oracle.cpp) before anything is checked against it.
+
+Each check is chosen so that a plausible restatement mistake fails it: a dropped term, a wrong
+sign, a transposed Jacobian block, a wrong branch threshold, a wrong damping or scaling rule.
+  * residual and Jacobians vs 50-digit mpmath values (tests/golden/snavely_golden.json);
+  * Jacobians vs an independent numpy complex-step derivative of the reference formula;
+  * the first-order branch: d(R X)/dw = -[X]x exactly;
+  * Huber loss and corrector closed forms on both branches;
+  * the Schur-complement LM step vs a dense numpy solve of the full damped normal equations;
+  * block-Jacobi PCG converging to the dense step;
+  * noise-free data converging to zero cost, monotone accepted costs, the chi^2 band;
+  * Normalize and Perturb invariants.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, theta_tolerance, tiny_problem
+
+
+@pytest.fixture(scope="module")
+def golden():
+    with open(os.path.join(GOLDEN, "snavely_golden.json")) as fh:
+        return json.load(fh)["cases"]
+
+
+def test_residual_matches_mpmath(oracle_mod, golden):
+    for c in golden:
+        r, _, _ = oracle_mod.eval_residual_jacobian([0], [0], c["obs"], c["cam"], c["pt"])
+        pred = np.array(c["r"]) + np.array(c["obs"])
+        err = np.linalg.norm(r[0] - np.array(c["r"])) / np.linalg.norm(pred)
+        assert err < 1e-14, (c["branch"], c["cam"][:3], err)
+
+
+def test_jacobian_matches_mpmath(oracle_mod, golden):
+    for c in golden:
+        _, jc, jp = oracle_mod.eval_residual_jacobian([0], [0], c["obs"], c["cam"], c["pt"])
+        J = np.concatenate([np.array(c["jc"]).reshape(2, 9), np.array(c["jp"]).reshape(2, 3)], 1)
+        Jo = np.concatenate([jc[0], jp[0]], 1)
+        err = np.linalg.norm(Jo - J) / np.linalg.norm(J)
+        assert err < theta_tolerance(c["cam"], 2e-14), (c["branch"], np.linalg.norm(c["cam"][:3]), err)
+
+
+def _residual_complex(params, obs):
+    """The reference formula in numpy complex arithmetic (for complex-step derivatives)."""
+    w, t, f, k1, k2 = params[0:3], params[3:6], params[6], params[7], params[8]
+    X = params[9:12]
+    th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2]
+    if th2.real > 2.0 ** -52:
+        th = np.sqrt(th2)
+        c, s = np.cos(th), np.sin(th)
+        u = w / th
+        ux = np.array([u[1] * X[2] - u[2] * X[1], u[2] * X[0] - u[0] * X[2], u[0] * X[1] - u[1] * X[0]])
+        p = X * c + ux * s + u * (u @ X) * (1 - c)
+    else:
+        p = X + np.array([w[1] * X[2] - w[2] * X[1], w[2] * X[0] - w[0] * X[2], w[0] * X[1] - w[1] * X[0]])
+    p = p + t
+    xp, yp = -p[0] / p[2], -p[1] / p[2]
+    r2 = xp * xp + yp * yp
+    d = 1 + r2 * (k1 + k2 * r2)
+    return np.array([f * d * xp - obs[0], f * d * yp - obs[1]])
+
+
+def test_jacobian_matches_complex_step(oracle_mod):
+    rng = np.random.default_rng(7)
+    for _ in range(60):
+        th = np.exp(rng.uniform(np.log(1e-3), np.log(3.0)))
+        ax = rng.normal(size=3)
+        cam = np.concatenate([ax / np.linalg.norm(ax) * th, rng.normal(0, 1, 3), [rng.normal(500, 20)],
+                              rng.normal(0, 0.05, 2)])
+        X = rng.normal(0, 1, 3)
+        # put the point in front of the camera: adjust t_z so that p_z = -U(4, 20)
+        p = oracle_mod.rotate_point(cam[:3], X) + cam[3:6]
+        cam[5] += -rng.uniform(4, 20) - p[2]
+        obs = rng.normal(0, 50, 2)
+        params = np.concatenate([cam, X]).astype(complex)
+        J = np.zeros((2, 12))
+        h = 1e-30
+        for j in range(12):
+            q = params.copy()
+            q[j] += 1j * h
+            J[:, j] = _residual_complex(q, obs).imag / h
+        _, jc, jp = oracle_mod.eval_residual_jacobian([0], [0], obs, cam, X)
+        Jo = np.concatenate([jc[0], jp[0]], 1)
+        assert np.linalg.norm(Jo - J) / np.linalg.norm(J) < 1e-13
+
+
+def test_taylor_branch_rotation_derivative(oracle_mod):
+    """Inside theta^2 <= DBL_EPSILON the rotation is X + w x X, so d(RX)/dw = -[X]x exactly and the
+    camera-rotation Jacobian columns equal dpred/dp . (e_j x X)."""
+    rng = np.random.default_rng(3)
+    for w in ([0.0, 0.0, 0.0], [1e-9, -2e-9, 3e-9], [2.0 ** -26, 0.0, 0.0]):
+        X = rng.normal(0, 1, 3)
+        cam = np.array(list(w) + [0.1, -0.2, -8.0, 480.0, 1e-3, -2e-3])
+        _, jc, _ = oracle_mod.eval_residual_jacobian([0], [0], [0.0, 0.0], cam, X)
+        # dpred/dt = dpred/dp (t enters p additively), so column block 3:6 is dpred/dp
+        dpdp = jc[0][:, 3:6]
+        cols = np.stack([np.cross(np.eye(3)[j], X) for j in range(3)], 1)  # d p / d w_j = e_j x X
+        np.testing.assert_allclose(jc[0][:, 0:3], dpdp @ cols, rtol=1e-12, atol=1e-12 * np.abs(dpdp).max())
+
+
+def test_huber_closed_form(oracle_mod):
+    for s in (0.0, 0.25, 0.999999, 1.0):
+        np.testing.assert_array_equal(oracle_mod.huber(s), [s, 1.0, 0.0])
+    for s in (1.000001, 4.0, 100.0):
+        r = np.sqrt(s)
+        rho = oracle_mod.huber(s)
+        np.testing.assert_allclose(rho, [2 * r - 1, 1 / r, -(1 / r) / (2 * s)], rtol=1e-15)
+    rho = oracle_mod.huber(9.0, a=2.0)  # a = 2: b = 4
+    np.testing.assert_allclose(rho, [2 * 2 * 3 - 4, 2 / 3, -(2 / 3) / 18], rtol=1e-15)
+
+
+def _project(oracle_mod, cams, pts, ci, pi):
+    obs = np.zeros((len(ci), 2))
+    for k in range(len(ci)):
+        obs[k] = oracle_mod.residual(cams[ci[k]], pts[pi[k]], 0.0, 0.0)
+    return obs
+
+
+def _dense_lm_step(oracle_mod, cams, pts, ci, pi, obs, radius, loss=1):
+    """Independent numpy restatement of one Ceres LM step: Huber-corrected J (sqrt(rho')),
+    Jacobi scaling 1/(1+|col|), D^2 = clamp(diag(J~'J~),1e-6,1e32)/radius, full dense solve."""
+    nc, npt = len(cams), len(pts)
+    r, jc, jp = oracle_mod.eval_residual_jacobian(ci, pi, obs, cams.reshape(-1), pts.reshape(-1))
+    n = 9 * nc + 3 * npt
+    J = np.zeros((2 * len(ci), n))
+    R = np.zeros(2 * len(ci))
+    for k in range(len(ci)):
+        s = r[k] @ r[k]
+        w = 1.0 if (loss == 0 or s <= 1.0) else np.sqrt(1.0 / np.sqrt(s))
+        J[2 * k:2 * k + 2, 9 * ci[k]:9 * ci[k] + 9] = w * jc[k]
+        J[2 * k:2 * k + 2, 9 * nc + 3 * pi[k]:9 * nc + 3 * pi[k] + 3] = w * jp[k]
+        R[2 * k:2 * k + 2] = w * r[k]
+    scale = 1.0 / (1.0 + np.sqrt((J * J).sum(0)))
+    Js = J * scale
+    H = Js.T @ Js
+    D2 = np.clip(np.diag(H), 1e-6, 1e32) / radius
+    y = np.linalg.solve(H + np.diag(D2), Js.T @ R)
+    step = -y
+    mcc = -(Js @ step) @ (R + Js @ step / 2)
+    return step * scale, mcc
+
+
+@pytest.mark.parametrize("radius", [1e4, 1.0, 1e-3])
+def test_schur_step_matches_dense_solve(oracle_mod, radius):
+    cams, pts, ci, pi, rng = tiny_problem(seed=1, nc=3, npts=12)
+    obs = _project(oracle_mod, cams, pts, ci, pi) + rng.normal(0, 2.0, (len(ci), 2))  # some Huber outliers
+    cams_p = cams + rng.normal(0, 0.01, cams.shape)
+    pts_p = pts + rng.normal(0, 0.05, pts.shape)
+    ref, ref_mcc = _dense_lm_step(oracle_mod, cams_p, pts_p, ci, pi, obs, radius)
+    delta, mcc, _, _ = oracle_mod.lm_step(3, 12, ci, pi, obs, cams_p, pts_p, radius)
+    assert np.linalg.norm(delta - ref) / np.linalg.norm(ref) < 1e-10
+    assert abs(mcc - ref_mcc) / abs(ref_mcc) < 1e-10
+
+
+def test_pcg_converges_to_dense_step(oracle_mod):
+    cams, pts, ci, pi, rng = tiny_problem(seed=2, nc=4, npts=20)
+    obs = _project(oracle_mod, cams, pts, ci, pi) + rng.normal(0, 0.5, (len(ci), 2))
+    cams_p = cams + rng.normal(0, 0.01, cams.shape)
+    dense, _, _, _ = oracle_mod.lm_step(4, 20, ci, pi, obs, cams_p, pts, 1e2)
+    o = oracle_mod.default_options(linear_solver=2, pcg_tolerance=1e-14, pcg_max_iterations=500)
+    pcg, _, _, it = oracle_mod.lm_step(4, 20, ci, pi, obs, cams_p, pts, 1e2, options=o)
+    assert it > 1
+    assert np.linalg.norm(pcg - dense) / np.linalg.norm(dense) < 1e-8
+    # a loose tolerance stops early
+    o = oracle_mod.default_options(linear_solver=2, pcg_tolerance=1e-1, pcg_max_iterations=500)
+    _, _, _, it2 = oracle_mod.lm_step(4, 20, ci, pi, obs, cams_p, pts, 1e2, options=o)
+    assert it2 < it
+
+
+def test_noise_free_converges_to_zero(oracle_mod):
+    cams, pts, ci, pi, rng = tiny_problem(seed=4, nc=4, npts=30)
+    obs = _project(oracle_mod, cams, pts, ci, pi)
+    cams_p = cams + rng.normal(0, 0.005, cams.shape)
+    pts_p = pts + rng.normal(0, 0.02, pts.shape)
+    o = oracle_mod.default_options(max_num_iterations=50, function_tolerance=0, parameter_tolerance=0,
+                                   gradient_tolerance=0)
+    res = oracle_mod.solve(4, 30, ci, pi, obs, cams_p, pts_p, o)
+    assert res["final_cost"] < 1e-18 * len(ci)
+    # accepted costs decrease until the accept test only sees rounding noise (~1e-26 here)
+    accepted = [r["cost"] for r in res["trace"] if r["step_is_successful"]]
+    assert all(b <= a + 1e-22 for a, b in zip(accepted, accepted[1:]))
+    assert res["trace"][1]["cost"] < 1e-3 * res["trace"][0]["cost"]
+
+
+def test_lm_chi2_band(oracle_mod):
+    import slamplay_amd as S
+    p = S.synthetic(1, seed=3)
+    nc, npt = p.num_cameras(), p.num_points()
+    res = oracle_mod.solve(nc, npt, p.camera_index(), p.point_index(), p.observations(), p.mutable_cameras(),
+                           p.mutable_points(), oracle_mod.default_options(max_num_iterations=30))
+    dof = 2 * p.num_observations() - 9 * nc - 3 * npt + 7
+    expected = 0.5 * dof * 0.5 ** 2
+    assert 0.9 * expected < res["final_cost"] < 1.05 * expected
+    assert res["termination"] == 0
+    costs = [r["cost"] for r in res["trace"] if r["step_is_successful"]]
+    assert all(b <= a for a, b in zip(costs, costs[1:]))
+
+
+def test_normalize_invariants(oracle_mod, ladybug_path):
+    import slamplay_amd as S
+    p = S.BALProblem(ladybug_path)
+    nc, npt = p.num_cameras(), p.num_points()
+    cams, pts = oracle_mod.normalize(nc, npt, p.mutable_cameras(), p.mutable_points())
+    P = pts.reshape(-1, 3)
+    med = np.array([np.partition(P[:, i], npt // 2)[npt // 2] for i in range(3)])
+    np.testing.assert_array_equal(med, 0.0)  # the median point maps exactly onto 0
+    mad = np.partition(np.abs(P).sum(1), npt // 2)[npt // 2]
+    assert abs(mad - 100.0) < 1e-11
+    # Normalize is a similarity: reprojection residuals are unchanged
+    ci, pi, obs = p.camera_index(), p.point_index(), p.observations()
+    r0, _, _ = oracle_mod.eval_residual_jacobian(ci, pi, obs, p.mutable_cameras(), p.mutable_points())
+    r1, _, _ = oracle_mod.eval_residual_jacobian(ci, pi, obs, cams, pts)
+    assert np.abs(r1 - r0).max() < 1e-9 * np.abs(obs).max()
+
+
+def test_perturb_rotation_keeps_centers(oracle_mod, ladybug_path):
+    import slamplay_amd as S
+    p = S.BALProblem(ladybug_path)
+    nc, npt = p.num_cameras(), p.num_points()
+    cams0 = p.mutable_cameras().copy().reshape(nc, 9)
+    cams, pts = oracle_mod.perturb(nc, npt, cams0, p.mutable_points(), 0.1, 0.0, 0.0)
+    cams = cams.reshape(nc, 9)
+    np.testing.assert_array_equal(pts, p.mutable_points())
+    for c in range(nc):
+        c0 = -oracle_mod.rotate_point(-cams0[c, :3], cams0[c, 3:6])
+        c1 = -oracle_mod.rotate_point(-cams[c, :3], cams[c, 3:6])
+        np.testing.assert_allclose(c1, c0, rtol=1e-12, atol=1e-12)
+        assert np.linalg.norm(cams[c, :3] - cams0[c, :3]) > 0
+    # translation-only perturbation moves t by sigma-scale normals; points untouched
+    cams2, pts2 = oracle_mod.perturb(nc, npt, cams0, p.mutable_points(), 0.0, 0.5, 0.0)
+    d = cams2.reshape(nc, 9)[:, 3:6] - cams0[:, 3:6]
+    assert 0.2 < d.std() < 0.8

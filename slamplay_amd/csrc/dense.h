@@ -2,6 +2,7 @@
 // SPARSE_SCHUR step of test_bundle_adjustment_ceres.cpp:97 for problems up to a few hundred cameras).
 //
 //   S = U* - sum_points sum_{i,j in track} W~_i Vinv_p W~_j'     (9 nc x 9 nc, FP64)
+// with U*'s per-observation part folded into the (i, i) pairs: S_aa = D_a^2 + sum J~'(I - G)J~ - ...
 // is assembled without atomics: the host enumerates every ordered observation pair (i, j) of a
 // point with camera(i) <= camera(j) once per problem and groups the pairs by camera block (a, b);
 // one workgroup owns one 9x9 block and reduces its pair list in a fixed order.  A pair contributes
@@ -24,13 +25,14 @@
 namespace mba {
 
 constexpr int kTile = 32;
-constexpr int kJs = 32;  // per-observation stride of the stored Jc~ (18), Jp~ (6), Hp (6)
+constexpr int kJs = 36;  // per-observation stride of Jc~ (18), Jp~ (6), Hp (6), I - G (3)
 
 template <typename Real>
 __global__ void __launch_bounds__(kWG) k_dense_store(Dev<Real> d, Real* __restrict__ js) {
     LMState* st = d.st;
     if (st->done || st->step_invalid) return;
-    const int c = blockIdx.x;
+    const int c = xcd_remap(blockIdx.x, d.nc);
+    if (c >= d.nc) return;
     const CameraPrep<Real> P = d.prep[c];
     Real sc[9];
     for (int i = 0; i < 9; ++i) sc[i] = (Real)d.sc[9 * (size_t)c + i];
@@ -53,6 +55,7 @@ __global__ void __launch_bounds__(kWG) k_dense_store(Dev<Real> d, Real* __restri
         sym3_mul<Real>(d.VinvR + 6 * (size_t)p, Jp, h);
         sym3_mul<Real>(d.VinvR + 6 * (size_t)p, Jp + 3, h + 3);
         for (int i = 0; i < 6; ++i) o[24 + i] = h[i];
+        point_complement<Real, Real>(d, p, Jp, o[30], o[31], o[32]);
     }
 }
 
@@ -85,10 +88,17 @@ __global__ void __launch_bounds__(kWG) k_schur_pairs(Dev<Real> d, const Real* __
             const Real* Jj = js + kJs * (size_t)pair_j[base + t];
             const Real* Hp = Ji + 24;
             const Real* Jp = Jj + 18;
-            const Real G00 = Hp[0] * Jp[0] + Hp[1] * Jp[1] + Hp[2] * Jp[2];
-            const Real G01 = Hp[0] * Jp[3] + Hp[1] * Jp[4] + Hp[2] * Jp[5];
-            const Real G10 = Hp[3] * Jp[0] + Hp[4] * Jp[1] + Hp[5] * Jp[2];
-            const Real G11 = Hp[3] * Jp[3] + Hp[4] * Jp[4] + Hp[5] * Jp[5];
+            Real G00 = Hp[0] * Jp[0] + Hp[1] * Jp[1] + Hp[2] * Jp[2];
+            Real G01 = Hp[0] * Jp[3] + Hp[1] * Jp[4] + Hp[2] * Jp[5];
+            Real G10 = Hp[3] * Jp[0] + Hp[4] * Jp[1] + Hp[5] * Jp[2];
+            Real G11 = Hp[3] * Jp[3] + Hp[4] * Jp[4] + Hp[5] * Jp[5];
+            if (pair_i[base + t] == pair_j[base + t]) {
+                // an observation with itself: its J~c'J~c share of U* folded in, J~c'(I - G)J~c
+                // with I - G formed in FP64 (see point_complement); negated as acc is subtracted
+                G00 = -Ji[30];
+                G01 = G10 = -Ji[31];
+                G11 = -Ji[32];
+            }
             for (int k = 0; k < 9; ++k) {
                 T[t][k] = G00 * Jj[k] + G01 * Jj[9 + k];
                 T[t][9 + k] = G10 * Jj[k] + G11 * Jj[9 + k];
@@ -101,7 +111,7 @@ __global__ void __launch_bounds__(kWG) k_schur_pairs(Dev<Real> d, const Real* __
         __syncthreads();
     }
     if (t < 81) {
-        const double v = (ca == cb ? d.Ustar[81 * (size_t)ca + t] : 0.0) - acc;
+        const double v = (ca == cb && r == cc ? d.dc2[9 * (size_t)ca + r] : 0.0) - acc;
         S[(size_t)(9 * ca + r) * ld + 9 * cb + cc] = v;
         if (ca != cb) S[(size_t)(9 * cb + cc) * ld + 9 * ca + r] = v;
     }
@@ -398,7 +408,7 @@ struct DenseSchur {
     void enqueue_solve(SolverT& sv) {
         auto& d = sv.d;
         const int nc = d.nc;
-        sv.launch("dense_store", k_dense_store<Real>, nc, kWG, d, js.p);
+        sv.launch("dense_store", k_dense_store<Real>, xcd_grid(nc), kWG, d, js.p);
         const size_t nn = (size_t)ld * ld;
         sv.launch("fill_S", k_fill_S, (int)((nn + 255) / 256), 256, S.p, n, ld, (const LMState*)d.st);
         sv.launch("schur_pairs", k_schur_pairs<Real>, nblocks, kWG, d, (const Real*)js.p, (const int*)blk_a.p,

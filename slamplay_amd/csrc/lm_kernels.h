@@ -5,11 +5,17 @@
 //     every Jacobian is recomputed in camera-order kernels, where the camera -- and so its
 //     CameraPrep -- is uniform across the workgroup, from 12-20 bytes of observation data plus a
 //     gathered point.  No per-observation Jacobian is stored.
-//   * points sorted by track length (descending) in chunks of 64 (one wave): slot_j holds, for
-//     row r of chunk ch and lane l, the camera-order index of that point's r-th observation
-//     (-1 padding) at chunk_off[ch] + 64 r + l.  Point kernels sum per-observation partials
-//     written by camera kernels, coalesced and load-balanced inside a wave.
-//   * parameters FP64 SoA: cam_x[i * nc + c], pt_x[i * np + p].
+//   * points in chunks of 64 (one wave).  Slot chunk_off[ch] + 64 r + l belongs to the r-th
+//     observation of point 64 ch + l; slot_of[j] maps a camera-order observation to its slot.
+//     Camera kernels write their per-observation partials for the points straight into slot
+//     order, so point kernels read them as coalesced streams with no indirection (padding slots
+//     stay zero).  Chunks hold points of similar track length, which keeps the padding small.
+//   * internal point order: by first observing camera, then track length (longest first), so a
+//     camera's observations touch clustered points and a wave's 64 points have similar tracks;
+//   * parameters FP64: cameras SoA cam_x[i * nc + c]; points AoS4 pt_x[4 p + i] (one aligned
+//     32-byte record per gather); per-point and per-observation working vectors are 4-wide too.
+//   * workgroups are remapped so that each XCD (own L2) walks a contiguous range of cameras or
+//     point chunks (xcd_remap).
 // All reductions are fixed-order tree reductions in registers/LDS: no atomics, and results are
 // bitwise reproducible run to run.
 //
@@ -28,6 +34,21 @@ namespace mba {
 
 constexpr int kWG = 256;    // threads per workgroup for every solver kernel
 constexpr int kWaves = kWG / 64;
+constexpr int kXcds = 8;    // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+
+// Block b runs on XCD b % 8; give XCD k the contiguous index range [k per, (k + 1) per) so that
+// neighbouring cameras (which share points) and neighbouring point chunks share one L2.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+    const int per = (n + kXcds - 1) / kXcds;
+    return (b % kXcds) * per + b / kXcds;
+}
+inline int xcd_grid(int n) { return kXcds * ((n + kXcds - 1) / kXcds); }
+
+// 4-wide aligned record of per-point / per-observation working vectors (w unused).
+template <typename Real>
+struct alignas(4 * sizeof(Real)) R4 {
+    Real x, y, z, w;
+};
 
 // Layout of the per-workgroup partial-sum slots (partial[slot * stride + block]).
 enum PartialSlot {
@@ -46,7 +67,7 @@ struct LMState {
     double x_cost, cand_cost, model_change;
     double gmax;
     double step_norm, x_norm;
-    double pcg_rz, pcg_alpha, pcg_beta, pcg_bnorm;
+    double pcg_bb;
     int iteration, consecutive_invalid, done, termination;
     int need_lin, first_lin, accepted, step_invalid;
     int num_successful, num_unsuccessful;
@@ -76,22 +97,25 @@ struct Dev {
     const Real* co_uv;
     const int* chunk_off;
     const int* chunk_len;
-    const int* slot_j;
+    const int* slot_of;
     double *cam_x, *pt_x, *cam_c, *pt_c, *dcam, *dpt;
     CameraPrep<Real>* prep;
     CameraPrep<double>* prep_c;
-    double *U, *gc, *sc;       // per camera: 81, 9, 9
-    double *Ustar, *M, *Minv;  // per camera: 81 each
-    double *bc, *px, *pr, *pz, *pp;  // per camera: 9 each (PCG vectors, FP64)
-    double *V, *gp, *sp;       // per point: 6, 3, 3
+    double *U, *gc, *sc, *dc2; // per camera: diag(Jc'Jc), Jc'r, Jacobi scale, damping D_c^2 (9 each)
+    double *M, *Minv;          // per camera: 81 each (Schur-Jacobi block and its inverse)
+    double *bc, *px, *pr, *pz, *pp, *qc;  // per camera: 9 each (PCG vectors, FP64)
+    double *V, *gp, *sp, *dp2; // per point: 6, 3, 3, 3
     double* Vinv64;            // per point: 6 (FP64 inverse of the damped block)
     Real* VinvR;               // per point: 6 (same, in working precision)
-    Real* qp;                  // per point: 3
-    Real* zs;                  // per point: 3
-    Real* pv;                  // per observation (camera order): 9 (J_p'J_p sym 6, J_p'r 3)
-    Real* u;                   // per observation (camera order): 3
+    R4<Real>* qp;              // per point: Vinv S g_p
+    R4<Real>* zs;              // per point: S_p z_p of the current PCG direction
+    double4* pv;               // per slot, 3 records: J_p'J_p (sym 6), J_p'r (3), padding
+    R4<Real>* u;               // per slot: J_p'(J_c S_c v_c)
     double* cam_part;          // PS_COUNT * nc
     double* pt_part;           // PS_COUNT * n_pt_blocks
+    double* upd_part;          // 2 * n_upd: PCG r.z and r.r per block of kCamsPerUpd cameras
+    double* rz_hist;           // r.z at the start of each PCG iteration
+    int n_upd;
     LMState* st;
     ba_iteration* trace;
     int trace_cap;
@@ -218,10 +242,11 @@ __device__ __forceinline__ void chol_inverse(const double (&L)[N * N], double (&
 // Observation helpers
 
 template <typename Real>
-__device__ __forceinline__ void load_point(const double* __restrict__ pt_x, int np, int p, Real X[3]) {
-    X[0] = (Real)pt_x[p];
-    X[1] = (Real)pt_x[np + p];
-    X[2] = (Real)pt_x[2 * np + p];
+__device__ __forceinline__ void load_point(const double* __restrict__ pts, int p, Real X[3]) {
+    const double4 v = reinterpret_cast<const double4*>(pts)[p];
+    X[0] = (Real)v.x;
+    X[1] = (Real)v.y;
+    X[2] = (Real)v.z;
 }
 
 // Corrected (sqrt(rho')-weighted) residual and Jacobians of observation j of camera prep P.
@@ -230,7 +255,7 @@ __device__ __forceinline__ void corrected_jacobian(const Dev<Real>& d, const Cam
                                                    Real r[2], Real Jc[18], Real Jp[6]) {
     p = d.co_pt[j];
     double X[3];
-    load_point<double>(d.pt_x, d.np, p, X);
+    load_point<double>(d.pt_x, p, X);
     snavely_residual_jacobian<Real>(P, X, d.co_uv[2 * (size_t)j], d.co_uv[2 * (size_t)j + 1], r, Jc, Jp);
     const Real w = huber_sqrt_rho1<Real>(r[0] * r[0] + r[1] * r[1], d.o.loss, (Real)d.o.loss_a);
     r[0] *= w;
@@ -256,103 +281,132 @@ __global__ void __launch_bounds__(64) k_cam_prep(Dev<Real> d) {
     d.prep[c] = P;
 }
 
-// One workgroup per camera: U_c = sum Jc'Jc, g_c = sum Jc'r (unscaled, corrected), plus the point
-// contributions Jp'Jp and Jp'r of each observation for k_lin_pt.
+// One workgroup per camera: diag(Jc'Jc) and g_c = Jc'r (unscaled, corrected) -- the column norms
+// for the Jacobi scaling and the LM diagonal -- plus each observation's point contributions
+// Jp'Jp and Jp'r for k_lin_pt.
 template <typename Real>
 __global__ void __launch_bounds__(kWG) k_lin_cam(Dev<Real> d) {
     if (!d.st->need_lin) return;
-    __shared__ double lds[54 * kWaves];
-    const int c = blockIdx.x;
+    __shared__ double lds[18 * kWaves];
+    const int c = xcd_remap(blockIdx.x, d.nc);
+    if (c >= d.nc) return;
     const CameraPrep<Real> P = d.prep[c];
-    double acc[54];
+    double acc[18];
 #pragma unroll
-    for (int i = 0; i < 54; ++i) acc[i] = 0.0;
+    for (int i = 0; i < 18; ++i) acc[i] = 0.0;
     for (int j = d.cam_off[c] + threadIdx.x; j < d.cam_off[c + 1]; j += kWG) {
         int p;
         Real r[2], Jc[18], Jp[6];
         corrected_jacobian<Real>(d, P, j, p, r, Jc, Jp);
-        Real hu[45];
 #pragma unroll
-        for (int i = 0; i < 9; ++i)
-#pragma unroll
-            for (int k = i; k < 9; ++k) hu[sym9(i, k)] = Jc[i] * Jc[k] + Jc[9 + i] * Jc[9 + k];
-#pragma unroll
-        for (int i = 0; i < 45; ++i) acc[i] += (double)hu[i];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) acc[45 + i] += (double)(Jc[i] * r[0] + Jc[9 + i] * r[1]);
-        Real* pv = d.pv + 9 * (size_t)j;
-        pv[0] = Jp[0] * Jp[0] + Jp[3] * Jp[3];
-        pv[1] = Jp[0] * Jp[1] + Jp[3] * Jp[4];
-        pv[2] = Jp[0] * Jp[2] + Jp[3] * Jp[5];
-        pv[3] = Jp[1] * Jp[1] + Jp[4] * Jp[4];
-        pv[4] = Jp[1] * Jp[2] + Jp[4] * Jp[5];
-        pv[5] = Jp[2] * Jp[2] + Jp[5] * Jp[5];
-        pv[6] = Jp[0] * r[0] + Jp[3] * r[1];
-        pv[7] = Jp[1] * r[0] + Jp[4] * r[1];
-        pv[8] = Jp[2] * r[0] + Jp[5] * r[1];
+        for (int i = 0; i < 9; ++i) {
+            acc[i] += (double)(Jc[i] * Jc[i] + Jc[9 + i] * Jc[9 + i]);
+            acc[9 + i] += (double)(Jc[i] * r[0] + Jc[9 + i] * r[1]);
+        }
+        // The point block's per-observation products are formed in FP64 from the (FP32 or FP64)
+        // Jacobian: products of FP32 values are exact in FP64, so each contribution is exactly
+        // positive semidefinite and V stays positive definite for nearly depth-free points.
+        const double q[6] = {Jp[0], Jp[1], Jp[2], Jp[3], Jp[4], Jp[5]};
+        const double r0 = r[0], r1 = r[1];
+        double4* pv = d.pv + 3 * (size_t)d.slot_of[j];
+        pv[0] = make_double4(q[0] * q[0] + q[3] * q[3], q[0] * q[1] + q[3] * q[4], q[0] * q[2] + q[3] * q[5],
+                             q[1] * q[1] + q[4] * q[4]);
+        pv[1] = make_double4(q[1] * q[2] + q[4] * q[5], q[2] * q[2] + q[5] * q[5], q[0] * r0 + q[3] * r1,
+                             q[1] * r0 + q[4] * r1);
+        pv[2] = make_double4(q[2] * r0 + q[5] * r1, 0.0, 0.0, 0.0);
     }
-    block_sum<54>(acc, lds);
+    block_sum<18>(acc, lds);
     const int t = threadIdx.x;
-    if (t < 81) {
-        const int i = t / 9, k = t % 9;
-        d.U[81 * (size_t)c + t] = lds[sym9(i < k ? i : k, i < k ? k : i) * kWaves];
+    if (t < 9) {
+        d.U[9 * (size_t)c + t] = lds[t * kWaves];
+        d.gc[9 * (size_t)c + t] = lds[(9 + t) * kWaves];
     }
-    if (t < 9) d.gc[9 * (size_t)c + t] = lds[(45 + t) * kWaves];
     if (t == 0) {
         double gm = 0.0;
-        for (int i = 0; i < 9; ++i) gm = fmax(gm, fabs(lds[(45 + i) * kWaves]));
+        for (int i = 0; i < 9; ++i) gm = fmax(gm, fabs(lds[(9 + i) * kWaves]));
         d.cam_part[PS_GMAX * (size_t)d.nc + c] = gm;
         if (d.st->first_lin)
             for (int i = 0; i < 9; ++i)
-                d.sc[9 * (size_t)c + i] = d.o.jacobi_scaling ? 1.0 / (1.0 + sqrt(lds[sym9(i, i) * kWaves])) : 1.0;
+                d.sc[9 * (size_t)c + i] = d.o.jacobi_scaling ? 1.0 / (1.0 + sqrt(lds[i * kWaves])) : 1.0;
     }
 }
 
-// One thread per point (SELL chunks): V_p = sum Jp'Jp, g_p = sum Jp'r.
+// Point kernels: one workgroup per SELL chunk (64 points).  The 4 waves split the chunk's rows
+// (wave w takes rows w, w + 4, ...; two rows in flight per lane) and combine through LDS, so a
+// chunk's slots are streamed with 4x the memory parallelism of one wave walking all rows.  On
+// return, lane l of wave 0 holds the sums of point 64 ch + l; other waves' sums are undefined.
+template <int N, typename Load>
+__device__ __forceinline__ void chunk_sum(int len, size_t base, double (&acc)[N], Load load, double* lds) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[i] = 0.0;
+    int r = wave;
+    for (; r + kWaves < len; r += 2 * kWaves) {
+        load(base + 64 * (size_t)r + lane, acc);
+        load(base + 64 * (size_t)(r + kWaves) + lane, acc);
+    }
+    if (r < len) load(base + 64 * (size_t)r + lane, acc);
+    if (wave > 0)
+#pragma unroll
+        for (int i = 0; i < N; ++i) lds[((wave - 1) * N + i) * 64 + lane] = acc[i];
+    __syncthreads();
+    if (wave == 0)
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            for (int w = 0; w < kWaves - 1; ++w) acc[i] += lds[(w * N + i) * 64 + lane];
+}
+
+// One workgroup per chunk: V_p = sum Jp'Jp, g_p = sum Jp'r per point.
 template <typename Real>
 __global__ void __launch_bounds__(kWG) k_lin_pt(Dev<Real> d) {
     if (!d.st->need_lin) return;
-    __shared__ double lds[kWaves];
-    const int gid = blockIdx.x * kWG + threadIdx.x;
-    const int ch = gid >> 6, lane = gid & 63;
+    __shared__ double lds[(kWaves - 1) * 9 * 64];
+    const int ch = xcd_remap(blockIdx.x, d.n_chunks);
+    if (ch >= d.n_chunks) return;
+    double a[9];
+    chunk_sum<9>(d.chunk_len[ch], (size_t)d.chunk_off[ch], a,
+                 [&](size_t slot, double (&acc)[9]) {
+                     const double4* pv = d.pv + 3 * slot;
+                     const double4 x = pv[0], y = pv[1], z = pv[2];
+                     acc[0] += x.x;
+                     acc[1] += x.y;
+                     acc[2] += x.z;
+                     acc[3] += x.w;
+                     acc[4] += y.x;
+                     acc[5] += y.y;
+                     acc[6] += y.z;
+                     acc[7] += y.w;
+                     acc[8] += z.x;
+                 },
+                 lds);
+    if (threadIdx.x >= 64) return;
+    const int p = 64 * ch + threadIdx.x;
     double gm = 0.0;
-    if (ch < d.n_chunks) {
-        double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        const int len = d.chunk_len[ch], base = d.chunk_off[ch];
-        for (int r = 0; r < len; ++r) {
-            const int j = d.slot_j[base + 64 * r + lane];
-            if (j >= 0) {
-                const Real* pv = d.pv + 9 * (size_t)j;
-#pragma unroll
-                for (int i = 0; i < 9; ++i) a[i] += (double)pv[i];
-            }
-        }
-        const int p = gid;
-        if (p < d.np) {
-            for (int i = 0; i < 6; ++i) d.V[6 * (size_t)p + i] = a[i];
-            for (int i = 0; i < 3; ++i) d.gp[3 * (size_t)p + i] = a[6 + i];
-            gm = fmax(fabs(a[6]), fmax(fabs(a[7]), fabs(a[8])));
-            if (d.st->first_lin) {
-                const double dg[3] = {a[0], a[3], a[5]};
-                for (int i = 0; i < 3; ++i) d.sp[3 * (size_t)p + i] = d.o.jacobi_scaling ? 1.0 / (1.0 + sqrt(dg[i])) : 1.0;
-            }
+    if (p < d.np) {
+        for (int i = 0; i < 6; ++i) d.V[6 * (size_t)p + i] = a[i];
+        for (int i = 0; i < 3; ++i) d.gp[3 * (size_t)p + i] = a[6 + i];
+        gm = fmax(fabs(a[6]), fmax(fabs(a[7]), fabs(a[8])));
+        if (d.st->first_lin) {
+            const double dg[3] = {a[0], a[3], a[5]};
+            for (int i = 0; i < 3; ++i) d.sp[3 * (size_t)p + i] = d.o.jacobi_scaling ? 1.0 / (1.0 + sqrt(dg[i])) : 1.0;
         }
     }
-    gm = block_max1(gm, lds);
-    if (threadIdx.x == 0) d.pt_part[PS_GMAX * (size_t)d.n_pt_blocks + blockIdx.x] = gm;
+    gm = wave_max(gm);
+    if (threadIdx.x == 0) d.pt_part[PS_GMAX * (size_t)d.n_pt_blocks + ch] = gm;
 }
 
 // FP64 cost 1/2 sum rho at the parameters whose prep is prep_c and points pts (one WG per camera).
 template <typename Real>
 __global__ void __launch_bounds__(kWG) k_cost_cam(Dev<Real> d, const double* __restrict__ pts) {
     __shared__ double lds[kWaves];
-    const int c = blockIdx.x;
+    const int c = xcd_remap(blockIdx.x, d.nc);
+    if (c >= d.nc) return;
     const CameraPrep<double> P = d.prep_c[c];
     double acc = 0.0;
     for (int j = d.cam_off[c] + threadIdx.x; j < d.cam_off[c + 1]; j += kWG) {
         const int p = d.co_pt[j];
         double X[3];
-        load_point<double>(pts, d.np, p, X);
+        load_point<double>(pts, p, X);
         double r[2];
         snavely_residual<double>(P, X, d.co_uv64[2 * (size_t)j], d.co_uv64[2 * (size_t)j + 1], r);
         acc += half_rho(r[0] * r[0] + r[1] * r[1], d.o.loss, d.o.loss_a);
@@ -426,6 +480,19 @@ __global__ void __launch_bounds__(kWG) k_post_lin(Dev<Real> d) {
 
 // ---------------------------------------------------------------------------------------------
 // Damped blocks and the reduced camera system
+//
+// Every Schur quantity is evaluated in per-observation residual form.  With J~c = Jc S_c,
+// J~p = Jp S_p (Jacobi-scaled, Huber-corrected), V*_p = sum J~p'J~p + D_p^2 and
+// G_k = J~p V*^-1 J~p' (2x2) for observation k:
+//   M_c   = D_c^2 + sum_k J~c'(I - G_k) J~c                      (Schur-Jacobi block)
+//   b_c   = sum_k J~c'(r~_k - J~p q_p),   q_p = V*^-1 g~_p           (reduced gradient)
+//   S v   = D_c^2 v_c + sum_k J~c' e_k,   e_k = J~c v_c - J~p z_p,  z = V*^-1 W~'v
+//   v'Sv  = sum_k |e_k|^2 + sum_p z_p'D_p^2 z_p + sum_c v_c'D_c^2 v_c
+//   y_p   = V*^-1 sum_k J~p'(r~_k - J~c y_c)                         (back-substitution)
+// These equal U* - W V^-1 W', g - W V^-1 g_p, ... exactly, but never subtract two large
+// accumulated sums: the cancellation happens inside one observation's 2-vector, so the FP32
+// path stays accurate when the damping is small (large trust radius) and v'Sv cannot go
+// negative through rounding.
 
 // One thread per point: D_p^2 = clamp(diag(S V S))/radius, Vinv = (S V S + D_p^2)^-1 in FP64,
 // q_p = Vinv (S g_p).
@@ -445,7 +512,11 @@ __global__ void __launch_bounds__(kWG) k_damp_pt(Dev<Real> d) {
     A[4] = s[1] * V[3] * s[1];
     A[5] = A[7] = s[1] * V[4] * s[2];
     A[8] = s[2] * V[5] * s[2];
-    for (int i = 0; i < 3; ++i) A[4 * i] += fmin(fmax(A[4 * i], d.o.min_diag), d.o.max_diag) / radius;
+    for (int i = 0; i < 3; ++i) {
+        const double dd = fmin(fmax(A[4 * i], d.o.min_diag), d.o.max_diag) / radius;
+        d.dp2[3 * (size_t)p + i] = dd;
+        A[4 * i] += dd;
+    }
     if (!chol_inplace<3>(A)) {
         st->step_invalid = 1;
         return;
@@ -458,7 +529,12 @@ __global__ void __launch_bounds__(kWG) k_damp_pt(Dev<Real> d) {
         d.VinvR[6 * (size_t)p + i] = (Real)sym[i];
     }
     const double g[3] = {s[0] * d.gp[3 * (size_t)p], s[1] * d.gp[3 * (size_t)p + 1], s[2] * d.gp[3 * (size_t)p + 2]};
-    for (int i = 0; i < 3; ++i) d.qp[3 * (size_t)p + i] = (Real)(Ai[3 * i] * g[0] + Ai[3 * i + 1] * g[1] + Ai[3 * i + 2] * g[2]);
+    R4<Real> q;
+    q.x = (Real)(Ai[0] * g[0] + Ai[1] * g[1] + Ai[2] * g[2]);
+    q.y = (Real)(Ai[3] * g[0] + Ai[4] * g[1] + Ai[5] * g[2]);
+    q.z = (Real)(Ai[6] * g[0] + Ai[7] * g[1] + Ai[8] * g[2]);
+    q.w = (Real)0;
+    d.qp[p] = q;
 }
 
 template <typename Real>
@@ -468,15 +544,45 @@ __device__ __forceinline__ void sym3_mul(const Real* S6, const Real v[3], Real o
     out[2] = S6[2] * v[0] + S6[4] * v[1] + S6[5] * v[2];
 }
 
-// One WG per camera: b_c = S g_c - sum W~ q_p, the Schur-Jacobi block
-// M_c = U*_c - sum W~ Vinv W~' (U*_c = S U S + D_c^2).  With W~ = S_c Jc' Jp S_p, a point's
-// contribution is Jc~' (Jp~ Vinv Jp~') Jc~, a 2x2 middle factor per observation.
+// Corrected Jacobians scaled by the Jacobi scaling: Jc <- Jc S_c, Jp <- Jp S_p.
+template <typename Real>
+__device__ __forceinline__ void scale_jacobians(const Dev<Real>& d, const Real sc[9], int p, Real Jc[18], Real Jp[6]) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        Jc[i] *= sc[i];
+        Jc[9 + i] *= sc[i];
+    }
+    const Real s0 = (Real)d.sp[3 * (size_t)p], s1 = (Real)d.sp[3 * (size_t)p + 1], s2 = (Real)d.sp[3 * (size_t)p + 2];
+    Jp[0] *= s0; Jp[3] *= s0;
+    Jp[1] *= s1; Jp[4] *= s1;
+    Jp[2] *= s2; Jp[5] *= s2;
+}
+
+// I - G for one observation, G = J~p V*^-1 J~p' (2x2), formed in FP64 with the FP64 inverse of the
+// damped point block: for a weakly determined point V* is ill-conditioned and G's eigenvalues
+// approach 1, where an FP32 inverse would push 1 - G negative.
+template <typename Real, typename Out>
+__device__ __forceinline__ void point_complement(const Dev<Real>& d, int p, const Real Jp[6], Out& A00, Out& A01,
+                                                 Out& A11) {
+    const double* Vi = d.Vinv64 + 6 * (size_t)p;
+    const double j0[3] = {(double)Jp[0], (double)Jp[1], (double)Jp[2]};
+    const double j1[3] = {(double)Jp[3], (double)Jp[4], (double)Jp[5]};
+    double h0[3], h1[3];
+    sym3_mul<double>(Vi, j0, h0);
+    sym3_mul<double>(Vi, j1, h1);
+    A00 = (Out)(1.0 - (j0[0] * h0[0] + j0[1] * h0[1] + j0[2] * h0[2]));
+    A01 = (Out)(-(j0[0] * h1[0] + j0[1] * h1[1] + j0[2] * h1[2]));
+    A11 = (Out)(1.0 - (j1[0] * h1[0] + j1[1] * h1[1] + j1[2] * h1[2]));
+}
+
+// One WG per camera: b_c, M_c (residual form above) and D_c^2.
 template <typename Real>
 __global__ void __launch_bounds__(kWG) k_rhs_cam(Dev<Real> d) {
     LMState* st = d.st;
     if (st->done) return;
     __shared__ double lds[54 * kWaves];
-    const int c = blockIdx.x;
+    const int c = xcd_remap(blockIdx.x, d.nc);
+    if (c >= d.nc) return;
     const CameraPrep<Real> P = d.prep[c];
     Real sc[9];
     for (int i = 0; i < 9; ++i) sc[i] = (Real)d.sc[9 * (size_t)c + i];
@@ -487,149 +593,156 @@ __global__ void __launch_bounds__(kWG) k_rhs_cam(Dev<Real> d) {
         int p;
         Real r[2], Jc[18], Jp[6];
         corrected_jacobian<Real>(d, P, j, p, r, Jc, Jp);
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            Jc[i] *= sc[i];
-            Jc[9 + i] *= sc[i];
-        }
-        const Real s0 = (Real)d.sp[3 * (size_t)p], s1 = (Real)d.sp[3 * (size_t)p + 1], s2 = (Real)d.sp[3 * (size_t)p + 2];
-        Jp[0] *= s0; Jp[3] *= s0;
-        Jp[1] *= s1; Jp[4] *= s1;
-        Jp[2] *= s2; Jp[5] *= s2;
-        const Real* q = d.qp + 3 * (size_t)p;
-        const Real a0 = Jp[0] * q[0] + Jp[1] * q[1] + Jp[2] * q[2];
-        const Real a1 = Jp[3] * q[0] + Jp[4] * q[1] + Jp[5] * q[2];
-        const Real* Vi = d.VinvR + 6 * (size_t)p;
-        Real h0[3], h1[3];
-        sym3_mul<Real>(Vi, Jp, h0);
-        sym3_mul<Real>(Vi, Jp + 3, h1);
-        const Real G00 = Jp[0] * h0[0] + Jp[1] * h0[1] + Jp[2] * h0[2];
-        const Real G01 = Jp[0] * h1[0] + Jp[1] * h1[1] + Jp[2] * h1[2];
-        const Real G11 = Jp[3] * h1[0] + Jp[4] * h1[1] + Jp[5] * h1[2];
-        Real H0[9], H1[9];  // G * Jc~ (2x9)
+        scale_jacobians<Real>(d, sc, p, Jc, Jp);
+        const R4<Real> q = d.qp[p];
+        const Real e0 = r[0] - (Jp[0] * q.x + Jp[1] * q.y + Jp[2] * q.z);
+        const Real e1 = r[1] - (Jp[3] * q.x + Jp[4] * q.y + Jp[5] * q.z);
+        // the camera block's per-observation product J~c'(I - G)J~c in FP64 (exactly PSD inputs)
+        double A00, A01, A11;
+        point_complement<Real, double>(d, p, Jp, A00, A01, A11);
+        double J0[9], J1[9], H0[9], H1[9];  // (I - G) Jc~ (2x9)
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            H0[k] = G00 * Jc[k] + G01 * Jc[9 + k];
-            H1[k] = G01 * Jc[k] + G11 * Jc[9 + k];
+            J0[k] = Jc[k];
+            J1[k] = Jc[9 + k];
+            H0[k] = A00 * J0[k] + A01 * J1[k];
+            H1[k] = A01 * J0[k] + A11 * J1[k];
         }
 #pragma unroll
         for (int i = 0; i < 9; ++i)
 #pragma unroll
-            for (int k = i; k < 9; ++k) acc[sym9(i, k)] += (double)(Jc[i] * H0[k] + Jc[9 + i] * H1[k]);
+            for (int k = i; k < 9; ++k) acc[sym9(i, k)] += J0[i] * H0[k] + J1[i] * H1[k];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) acc[45 + i] += (double)(Jc[i] * a0 + Jc[9 + i] * a1);
+        for (int i = 0; i < 9; ++i) acc[45 + i] += (double)(Jc[i] * e0 + Jc[9 + i] * e1);
     }
     block_sum<54>(acc, lds);
     const int t = threadIdx.x;
     if (t < 81) {
         const int i = t / 9, k = t % 9;
-        const double si = d.sc[9 * (size_t)c + i], sk = d.sc[9 * (size_t)c + k];
-        double u = si * d.U[81 * (size_t)c + t] * sk;
-        if (i == k) u += fmin(fmax(u, d.o.min_diag), d.o.max_diag) / st->radius;
-        d.Ustar[81 * (size_t)c + t] = u;
-        d.M[81 * (size_t)c + t] = u - lds[sym9(i < k ? i : k, i < k ? k : i) * kWaves];
+        double m = lds[sym9(i < k ? i : k, i < k ? k : i) * kWaves];
+        if (i == k) {
+            const double si = d.sc[9 * (size_t)c + i];
+            const double dd = fmin(fmax(si * d.U[9 * (size_t)c + i] * si, d.o.min_diag), d.o.max_diag) / st->radius;
+            d.dc2[9 * (size_t)c + i] = dd;
+            m += dd;
+        }
+        d.M[81 * (size_t)c + t] = m;
     }
-    if (t < 9) d.bc[9 * (size_t)c + t] = d.sc[9 * (size_t)c + t] * d.gc[9 * (size_t)c + t] - lds[(45 + t) * kWaves];
+    if (t < 9) d.bc[9 * (size_t)c + t] = lds[(45 + t) * kWaves];
 }
 
-// One thread per camera: Minv_c = M_c^-1 (FP64) and the PCG start x = 0, r = b, z = Minv b, p = 0.
+constexpr int kCamsPerUpd = kWG / 16;  // camera-vector kernels: 16 threads per camera
+
+// 16 threads per camera: thread 0 factors M_c and writes Minv_c = M_c^-1 (FP64); threads 0..8
+// start the PCG: x = 0, r = b, z = Minv b, p = 0; per-block partials r.z and b.b.
 template <typename Real>
-__global__ void __launch_bounds__(64) k_cam_factor(Dev<Real> d) {
+__global__ void __launch_bounds__(kWG) k_cam_factor(Dev<Real> d) {
+    __shared__ double lds[2 * kWaves];
+    __shared__ double Mi[kCamsPerUpd][81];
     LMState* st = d.st;
     if (st->done) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= d.nc) return;
-    double A[81];
-    for (int i = 0; i < 81; ++i) A[i] = d.M[81 * (size_t)c + i];
-    double rz = 0.0, bb = 0.0;
-    if (!chol_inplace<9>(A)) {
-        st->step_invalid = 1;
-    } else {
-        double Ai[81];
-        chol_inverse<9>(A, Ai);
-        double b[9];
-        for (int i = 0; i < 9; ++i) b[i] = d.bc[9 * (size_t)c + i];
-        for (int i = 0; i < 9; ++i) {
-            double z = 0.0;
-            for (int k = 0; k < 9; ++k) z += Ai[9 * i + k] * b[k];
-            d.pz[9 * (size_t)c + i] = z;
-            d.pr[9 * (size_t)c + i] = b[i];
-            d.px[9 * (size_t)c + i] = 0.0;
-            d.pp[9 * (size_t)c + i] = 0.0;
-            rz += b[i] * z;
-            bb += b[i] * b[i];
-        }
-        for (int i = 0; i < 81; ++i) d.Minv[81 * (size_t)c + i] = Ai[i];
+    const int lc = threadIdx.x >> 4, i = threadIdx.x & 15;
+    const int c = blockIdx.x * kCamsPerUpd + lc;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->pcg_done = 0;
+        st->pcg_iters = 0;
     }
-    d.cam_part[PS_A * (size_t)d.nc + c] = rz;
-    d.cam_part[PS_B * (size_t)d.nc + c] = bb;
+    if (c < d.nc && i == 0) {
+        double A[81];
+        for (int k = 0; k < 81; ++k) A[k] = d.M[81 * (size_t)c + k];
+        if (!chol_inplace<9>(A)) {
+            st->step_invalid = 1;
+            for (int k = 0; k < 81; ++k) Mi[lc][k] = 0.0;
+        } else {
+            double Ai[81];
+            chol_inverse<9>(A, Ai);
+            for (int k = 0; k < 81; ++k) Mi[lc][k] = Ai[k];
+        }
+    }
+    __syncthreads();
+    double rz = 0.0, bb = 0.0;
+    if (c < d.nc && i < 9) {
+        const size_t o = 9 * (size_t)c;
+        double z = 0.0;
+        for (int k = 0; k < 9; ++k) z += Mi[lc][9 * i + k] * d.bc[o + k];
+        for (int k = 0; k < 9; ++k) d.Minv[81 * (size_t)c + 9 * i + k] = Mi[lc][9 * i + k];
+        const double b = d.bc[o + i];
+        d.pz[o + i] = z;
+        d.pr[o + i] = b;
+        d.px[o + i] = 0.0;
+        d.pp[o + i] = 0.0;
+        rz = b * z;
+        bb = b * b;
+    }
+    double a[2] = {rz, bb};
+    block_sum<2>(a, lds);
+    if (threadIdx.x == 0) {
+        d.upd_part[blockIdx.x] = lds[0];
+        d.upd_part[d.n_upd + blockIdx.x] = lds[kWaves];
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
-// Implicit-Schur PCG.  Per iteration: k_pcg_beta -> k_jtx(update p) -> k_pt_z -> k_pcg_alpha -> k_jz.
+// Implicit-Schur PCG, four kernels per iteration it = 1, 2, ...:
+//   k_jtx(it)        beta and the convergence test from the previous update's partials (every
+//                    WG reduces the same few partials in the same order, so all agree), then
+//                    p <- z + beta p and u_j = Jp'(Jc S_c p_c) per observation;
+//   k_pt_z           z_p = V*^-1 S_p sum u_j, zs_p = S_p z_p, partial z'D_p^2 z;
+//   k_jz             q_c = D_c^2 p_c + S_c sum Jc' e_j, partial sum |e|^2 + p'D_c^2 p;
+//   k_cam_update(it) alpha = r.z / p'Sp, x += alpha p, r -= alpha q, z = Minv r, partials.
+// Stop: ||r|| <= tol ||b|| or pcg_max_iterations (or a non-positive p'Sp, i.e. rounding level).
 
-// Single WG. first != 0: rz = sum r.z, |b| = sqrt(sum b.b), p = z (beta 0). Otherwise check
-// ||r|| <= tol ||b|| (partials of k_jz), count the finished iteration, beta = rz_new / rz.
-template <typename Real>
-__global__ void __launch_bounds__(kWG) k_pcg_beta(Dev<Real> d, int first) {
-    __shared__ double lds[2 * kWaves];
-    LMState* st = d.st;
-    if (st->done || st->step_invalid || (!first && st->pcg_done)) return;
-    double a[2] = {0.0, 0.0};
-    for (int i = threadIdx.x; i < d.nc; i += kWG) {
-        a[0] += d.cam_part[PS_A * (size_t)d.nc + i];
-        a[1] += d.cam_part[PS_B * (size_t)d.nc + i];
+__device__ __forceinline__ void reduce_upd(const double* __restrict__ part, int n, double& a, double& b, double* lds) {
+    double v[2] = {0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        v[0] += part[i];
+        v[1] += part[n + i];
     }
-    block_sum<2>(a, lds);
-    if (threadIdx.x != 0) return;
-    const double rz = lds[0], rr = lds[kWaves];
-    if (first) {
-        st->pcg_iters = 0;
-        st->pcg_done = 0;
-        st->pcg_bnorm = sqrt(rr);
-        st->pcg_rz = rz;
-        st->pcg_beta = 0.0;
-        if (!(st->pcg_bnorm > 0.0)) st->pcg_done = 1;
-        if (!isfinite(rz) || !isfinite(rr)) st->step_invalid = 1;
-        if (d.o.pcg_max_iterations <= 0) st->pcg_done = 1;
-        return;
-    }
-    st->pcg_iters++;
-    if (sqrt(rr) <= d.o.pcg_tol * st->pcg_bnorm || st->pcg_iters >= d.o.pcg_max_iterations) {
-        st->pcg_done = 1;
-        return;
-    }
-    st->pcg_beta = rz / st->pcg_rz;
-    st->pcg_rz = rz;
+    block_sum<2>(v, lds);
+    a = lds[0];
+    b = lds[kWaves];
 }
 
-// One WG per camera.  v = p (after p <- z + beta p when update != 0) or v = y (back-substitution):
-// u_j = Jp' (Jc (S_c v_c)) per observation; partial v'U*v for the PCG step length.
+// One WG per camera.  mode 1 (PCG iteration it): see above.  mode 0 (back-substitution):
+// u_j = Jp'(r~ - Jc S_c y_c) with y = vsrc.
 template <typename Real>
-__global__ void __launch_bounds__(kWG) k_jtx(Dev<Real> d, int update, const double* __restrict__ vsrc) {
+__global__ void __launch_bounds__(kWG) k_jtx(Dev<Real> d, int mode, int it, const double* __restrict__ vsrc) {
     LMState* st = d.st;
-    if (st->done || st->step_invalid || (update && st->pcg_done)) return;
+    if (st->done || st->step_invalid || (mode == 1 && st->pcg_done)) return;
+    __shared__ double lds[2 * kWaves];
     __shared__ double vsh[9];
-    const int c = blockIdx.x;
+    const int c = xcd_remap(blockIdx.x, d.nc);
+    if (c >= d.nc) return;
+    double beta = 0.0;
+    if (mode == 1) {
+        double rz, rr;
+        reduce_upd(d.upd_part, d.n_upd, rz, rr, lds);
+        bool done;
+        if (it == 1)
+            done = !(rr > 0.0) || d.o.pcg_max_iterations <= 0;
+        else
+            done = rr <= d.o.pcg_tol * d.o.pcg_tol * st->pcg_bb || it - 1 >= d.o.pcg_max_iterations;
+        if (!isfinite(rz) || !isfinite(rr)) done = true;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            if (it == 1) st->pcg_bb = rr;
+            d.rz_hist[it] = rz;
+            st->pcg_iters = done ? it - 1 : it;
+            if (done) st->pcg_done = 1;
+            if (!isfinite(rz) || !isfinite(rr)) st->step_invalid = 1;
+        }
+        if (done) return;
+        beta = it == 1 ? 0.0 : rz / d.rz_hist[it - 1];
+    }
     if (threadIdx.x < 9) {
         const size_t i = 9 * (size_t)c + threadIdx.x;
         double v = vsrc[i];
-        if (update) {
-            v = d.pz[i] + st->pcg_beta * v;
+        if (mode == 1) {
+            v = d.pz[i] + beta * v;
             d.pp[i] = v;
         }
         vsh[threadIdx.x] = v;
     }
     __syncthreads();
-    if (update && threadIdx.x == 0) {
-        double s = 0.0;
-        for (int i = 0; i < 9; ++i) {
-            double t = 0.0;
-            for (int k = 0; k < 9; ++k) t += d.Ustar[81 * (size_t)c + 9 * i + k] * vsh[k];
-            s += vsh[i] * t;
-        }
-        d.cam_part[PS_A * (size_t)d.nc + c] = s;
-    }
     Real vs[9];
     for (int i = 0; i < 9; ++i) vs[i] = (Real)(d.sc[9 * (size_t)c + i] * vsh[i]);
     const CameraPrep<Real> P = d.prep[c];
@@ -643,166 +756,202 @@ __global__ void __launch_bounds__(kWG) k_jtx(Dev<Real> d, int update, const doub
             a0 += Jc[i] * vs[i];
             a1 += Jc[9 + i] * vs[i];
         }
-        Real* u = d.u + 3 * (size_t)j;
-        u[0] = Jp[0] * a0 + Jp[3] * a1;
-        u[1] = Jp[1] * a0 + Jp[4] * a1;
-        u[2] = Jp[2] * a0 + Jp[5] * a1;
+        if (mode == 0) {
+            a0 = r[0] - a0;
+            a1 = r[1] - a1;
+        }
+        R4<Real> u;
+        u.x = Jp[0] * a0 + Jp[3] * a1;
+        u.y = Jp[1] * a0 + Jp[4] * a1;
+        u.z = Jp[2] * a0 + Jp[5] * a1;
+        u.w = (Real)0;
+        d.u[d.slot_of[j]] = u;
     }
 }
 
-// One thread per point: t_p = S_p sum u_j, z_p = Vinv t_p, zs_p = S_p z_p; partial t.z.
+// One workgroup per chunk: t_p = S_p sum u_j, z_p = Vinv t_p, zs_p = S_p z_p; partial z'D_p^2 z.
 template <typename Real>
 __global__ void __launch_bounds__(kWG) k_pt_z(Dev<Real> d) {
     LMState* st = d.st;
     if (st->done || st->step_invalid || st->pcg_done) return;
-    __shared__ double lds[kWaves];
-    const int gid = blockIdx.x * kWG + threadIdx.x;
-    const int ch = gid >> 6, lane = gid & 63;
-    double tz = 0.0;
-    if (ch < d.n_chunks && gid < d.np) {
-        Real a[3] = {0, 0, 0};
-        const int len = d.chunk_len[ch], base = d.chunk_off[ch];
-        for (int r = 0; r < len; ++r) {
-            const int j = d.slot_j[base + 64 * r + lane];
-            if (j >= 0) {
-                const Real* u = d.u + 3 * (size_t)j;
-                a[0] += u[0];
-                a[1] += u[1];
-                a[2] += u[2];
-            }
-        }
-        const int p = gid;
-        Real s[3], t[3], z[3];
+    __shared__ double lds[(kWaves - 1) * 3 * 64];
+    const int ch = xcd_remap(blockIdx.x, d.n_chunks);
+    if (ch >= d.n_chunks) return;
+    double a[3];
+    chunk_sum<3>(d.chunk_len[ch], (size_t)d.chunk_off[ch], a,
+                 [&](size_t slot, double (&acc)[3]) {
+                     const R4<Real> u = d.u[slot];
+                     acc[0] += (double)u.x;
+                     acc[1] += (double)u.y;
+                     acc[2] += (double)u.z;
+                 },
+                 lds);
+    if (threadIdx.x >= 64) return;
+    const int p = 64 * ch + threadIdx.x;
+    double zdz = 0.0;
+    if (p < d.np) {
+        double s[3], t[3], z[3];
         for (int i = 0; i < 3; ++i) {
-            s[i] = (Real)d.sp[3 * (size_t)p + i];
+            s[i] = d.sp[3 * (size_t)p + i];
             t[i] = s[i] * a[i];
         }
-        sym3_mul<Real>(d.VinvR + 6 * (size_t)p, t, z);
-        for (int i = 0; i < 3; ++i) d.zs[3 * (size_t)p + i] = s[i] * z[i];
-        tz = (double)t[0] * (double)z[0] + (double)t[1] * (double)z[1] + (double)t[2] * (double)z[2];
+        sym3_mul<double>(d.Vinv64 + 6 * (size_t)p, t, z);  // the point block solve stays FP64
+        R4<Real> zo;
+        zo.x = (Real)(s[0] * z[0]);
+        zo.y = (Real)(s[1] * z[1]);
+        zo.z = (Real)(s[2] * z[2]);
+        zo.w = (Real)0;
+        d.zs[p] = zo;
+        for (int i = 0; i < 3; ++i) zdz += z[i] * d.dp2[3 * (size_t)p + i] * z[i];
     }
-    tz = block_sum1(tz, lds);
-    if (threadIdx.x == 0) d.pt_part[PS_B * (size_t)d.n_pt_blocks + blockIdx.x] = tz;
+    zdz = wave_sum(zdz);
+    if (threadIdx.x == 0) d.pt_part[PS_B * (size_t)d.n_pt_blocks + ch] = zdz;
 }
 
-// Single WG: alpha = rz / (p'Sp), p'Sp = sum_c p_c'U*_c p_c - sum_p t_p'Vinv t_p.
-template <typename Real>
-__global__ void __launch_bounds__(kWG) k_pcg_alpha(Dev<Real> d) {
-    __shared__ double lds[2 * kWaves];
-    LMState* st = d.st;
-    if (st->done || st->step_invalid || st->pcg_done) return;
-    double a[2] = {0.0, 0.0};
-    for (int i = threadIdx.x; i < d.nc; i += kWG) a[0] += d.cam_part[PS_A * (size_t)d.nc + i];
-    for (int i = threadIdx.x; i < d.n_pt_blocks; i += kWG) a[1] += d.pt_part[PS_B * (size_t)d.n_pt_blocks + i];
-    block_sum<2>(a, lds);
-    if (threadIdx.x != 0) return;
-    const double pq = lds[0] - lds[kWaves];
-    if (!(pq > 0.0) || !isfinite(pq)) {
-        // S is positive definite in exact arithmetic; a non-positive curvature means the search
-        // direction is at rounding level: stop with the current iterate.
-        st->pcg_done = 1;
-        return;
-    }
-    st->pcg_alpha = st->pcg_rz / pq;
-}
-
-// One WG per camera: q_c = U*_c p_c - S_c sum Jc'(Jp zs_p); x += alpha p, r -= alpha q,
-// z = Minv r; partials r.z and r.r.
+// One WG per camera: q_c = D_c^2 p_c + S_c sum Jc' e_j, e_j = Jc (S_c p_c) - Jp zs_p;
+// partial sum |e|^2 + p_c'D_c^2 p_c.
 template <typename Real>
 __global__ void __launch_bounds__(kWG) k_jz(Dev<Real> d) {
     LMState* st = d.st;
     if (st->done || st->step_invalid || st->pcg_done) return;
-    __shared__ double lds[9 * kWaves];
-    __shared__ double rsh[9];
-    const int c = blockIdx.x;
+    __shared__ double lds[10 * kWaves];
+    __shared__ double psh[9];
+    const int c = xcd_remap(blockIdx.x, d.nc);
+    if (c >= d.nc) return;
+    if (threadIdx.x < 9) psh[threadIdx.x] = d.pp[9 * (size_t)c + threadIdx.x];
+    __syncthreads();
+    Real ps[9];
+    for (int i = 0; i < 9; ++i) ps[i] = (Real)(d.sc[9 * (size_t)c + i] * psh[i]);
     const CameraPrep<Real> P = d.prep[c];
-    double acc[9];
+    double acc[10];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) acc[i] = 0.0;
+    for (int i = 0; i < 10; ++i) acc[i] = 0.0;
     for (int j = d.cam_off[c] + threadIdx.x; j < d.cam_off[c + 1]; j += kWG) {
         int p;
         Real r[2], Jc[18], Jp[6];
         corrected_jacobian<Real>(d, P, j, p, r, Jc, Jp);
-        const Real* z = d.zs + 3 * (size_t)p;
-        const Real a0 = Jp[0] * z[0] + Jp[1] * z[1] + Jp[2] * z[2];
-        const Real a1 = Jp[3] * z[0] + Jp[4] * z[1] + Jp[5] * z[2];
+        const R4<Real> z = d.zs[p];
+        Real e0 = -(Jp[0] * z.x + Jp[1] * z.y + Jp[2] * z.z);
+        Real e1 = -(Jp[3] * z.x + Jp[4] * z.y + Jp[5] * z.z);
 #pragma unroll
-        for (int i = 0; i < 9; ++i) acc[i] += (double)(Jc[i] * a0 + Jc[9 + i] * a1);
+        for (int i = 0; i < 9; ++i) {
+            e0 += Jc[i] * ps[i];
+            e1 += Jc[9 + i] * ps[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) acc[i] += (double)(Jc[i] * e0 + Jc[9 + i] * e1);
+        acc[9] += (double)e0 * (double)e0 + (double)e1 * (double)e1;
     }
-    block_sum<9>(acc, lds);
+    block_sum<10>(acc, lds);
     const int t = threadIdx.x;
-    const double alpha = st->pcg_alpha;
     if (t < 9) {
         const size_t i = 9 * (size_t)c + t;
-        double uq = 0.0;
-        for (int k = 0; k < 9; ++k) uq += d.Ustar[81 * (size_t)c + 9 * t + k] * d.pp[9 * (size_t)c + k];
-        const double q = uq - d.sc[i] * lds[t * kWaves];
-        d.px[i] += alpha * d.pp[i];
-        const double r = d.pr[i] - alpha * q;
-        d.pr[i] = r;
-        rsh[t] = r;
+        d.qc[i] = d.dc2[i] * psh[t] + d.sc[i] * lds[t * kWaves];
+    }
+    if (t == 0) {
+        double s = lds[9 * kWaves];
+        for (int i = 0; i < 9; ++i) s += psh[i] * d.dc2[9 * (size_t)c + i] * psh[i];
+        d.cam_part[PS_A * (size_t)d.nc + c] = s;
+    }
+}
+
+// 16 threads per camera: alpha = rz / p'Sp (each block reduces the same partials in the same
+// order), x += alpha p, r -= alpha q, z = Minv r (coalesced Minv rows); per-block partials r.z
+// and r.r for k_jtx.
+template <typename Real>
+__global__ void __launch_bounds__(kWG) k_cam_update(Dev<Real> d, int it) {
+    LMState* st = d.st;
+    if (st->done || st->step_invalid || st->pcg_done) return;
+    __shared__ double lds[2 * kWaves];
+    __shared__ double rsh[kCamsPerUpd][9];
+    double v = 0.0;
+    for (int k = threadIdx.x; k < d.nc; k += kWG) v += d.cam_part[PS_A * (size_t)d.nc + k];
+    for (int k = threadIdx.x; k < d.n_pt_blocks; k += kWG) v += d.pt_part[PS_B * (size_t)d.n_pt_blocks + k];
+    const double pSp = block_sum1(v, lds);
+    if (!(pSp > 0.0) || !isfinite(pSp)) {
+        // v'Sv is a sum of squares: zero means the direction vanished (rounding level)
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->pcg_done = 1;
+            st->pcg_iters = it - 1;
+        }
+        return;
+    }
+    const double alpha = d.rz_hist[it] / pSp;
+    const int lc = threadIdx.x >> 4, i = threadIdx.x & 15;
+    const int c = blockIdx.x * kCamsPerUpd + lc;
+    double r = 0.0;
+    if (c < d.nc && i < 9) {
+        const size_t k = 9 * (size_t)c + i;
+        d.px[k] += alpha * d.pp[k];
+        r = d.pr[k] - alpha * d.qc[k];
+        d.pr[k] = r;
+        rsh[lc][i] = r;
     }
     __syncthreads();
-    if (t == 0) {
-        double rz = 0.0, rr = 0.0;
-        for (int i = 0; i < 9; ++i) {
-            double z = 0.0;
-            for (int k = 0; k < 9; ++k) z += d.Minv[81 * (size_t)c + 9 * i + k] * rsh[k];
-            d.pz[9 * (size_t)c + i] = z;
-            rz += rsh[i] * z;
-            rr += rsh[i] * rsh[i];
-        }
-        d.cam_part[PS_A * (size_t)d.nc + c] = rz;
-        d.cam_part[PS_B * (size_t)d.nc + c] = rr;
+    double rz = 0.0, rr = 0.0;
+    if (c < d.nc && i < 9) {
+        const double* Mrow = d.Minv + 81 * (size_t)c + 9 * i;
+        double z = 0.0;
+        for (int k = 0; k < 9; ++k) z += Mrow[k] * rsh[lc][k];
+        d.pz[9 * (size_t)c + i] = z;
+        rz = r * z;
+        rr = r * r;
+    }
+    double a[2] = {rz, rr};
+    block_sum<2>(a, lds);
+    if (threadIdx.x == 0) {
+        d.upd_part[blockIdx.x] = lds[0];
+        d.upd_part[d.n_upd + blockIdx.x] = lds[kWaves];
     }
 }
 
 // ---------------------------------------------------------------------------------------------
 // Step, candidate and decision
 
-// One thread per point: y_p = Vinv (S g_p - S sum u_j), delta_p = -S y_p, candidate = x + delta.
+// One workgroup per chunk: y_p = Vinv S_p sum u_j (u_j = Jp'(r~ - Jc S_c y_c)),
+// delta_p = -S_p y_p, candidate = x + delta.
 template <typename Real>
 __global__ void __launch_bounds__(kWG) k_pt_backsub(Dev<Real> d) {
     LMState* st = d.st;
     if (st->done || st->step_invalid) return;
-    __shared__ double lds[2 * kWaves];
-    const int gid = blockIdx.x * kWG + threadIdx.x;
-    const int ch = gid >> 6, lane = gid & 63;
-    double a2[2] = {0.0, 0.0};
-    if (ch < d.n_chunks && gid < d.np) {
-        double a[3] = {0, 0, 0};
-        const int len = d.chunk_len[ch], base = d.chunk_off[ch];
-        for (int r = 0; r < len; ++r) {
-            const int j = d.slot_j[base + 64 * r + lane];
-            if (j >= 0) {
-                const Real* u = d.u + 3 * (size_t)j;
-                a[0] += (double)u[0];
-                a[1] += (double)u[1];
-                a[2] += (double)u[2];
-            }
-        }
-        const int p = gid;
+    __shared__ double lds[(kWaves - 1) * 3 * 64];
+    const int ch = xcd_remap(blockIdx.x, d.n_chunks);
+    if (ch >= d.n_chunks) return;
+    double a[3];
+    chunk_sum<3>(d.chunk_len[ch], (size_t)d.chunk_off[ch], a,
+                 [&](size_t slot, double (&acc)[3]) {
+                     const R4<Real> u = d.u[slot];
+                     acc[0] += (double)u.x;
+                     acc[1] += (double)u.y;
+                     acc[2] += (double)u.z;
+                 },
+                 lds);
+    if (threadIdx.x >= 64) return;
+    const int p = 64 * ch + threadIdx.x;
+    double dx2 = 0.0, x2 = 0.0;
+    if (p < d.np) {
         double s[3], t[3];
         for (int i = 0; i < 3; ++i) {
             s[i] = d.sp[3 * (size_t)p + i];
-            t[i] = s[i] * d.gp[3 * (size_t)p + i] - s[i] * a[i];
+            t[i] = s[i] * a[i];
         }
         const double* Vi = d.Vinv64 + 6 * (size_t)p;
         const double y[3] = {Vi[0] * t[0] + Vi[1] * t[1] + Vi[2] * t[2], Vi[1] * t[0] + Vi[3] * t[1] + Vi[4] * t[2],
                              Vi[2] * t[0] + Vi[4] * t[1] + Vi[5] * t[2]};
         for (int i = 0; i < 3; ++i) {
             const double dl = -s[i] * y[i];
-            const double x = d.pt_x[(size_t)i * d.np + p];
-            d.dpt[(size_t)i * d.np + p] = dl;
-            d.pt_c[(size_t)i * d.np + p] = x + dl;
-            a2[0] += dl * dl;
-            a2[1] += x * x;
+            const double x = d.pt_x[4 * (size_t)p + i];
+            d.dpt[4 * (size_t)p + i] = dl;
+            d.pt_c[4 * (size_t)p + i] = x + dl;
+            dx2 += dl * dl;
+            x2 += x * x;
         }
     }
-    block_sum<2>(a2, lds);
+    dx2 = wave_sum(dx2);
+    x2 = wave_sum(x2);
     if (threadIdx.x == 0) {
-        d.pt_part[PS_DX2 * (size_t)d.n_pt_blocks + blockIdx.x] = lds[0];
-        d.pt_part[PS_X2 * (size_t)d.n_pt_blocks + blockIdx.x] = lds[kWaves];
+        d.pt_part[PS_DX2 * (size_t)d.n_pt_blocks + ch] = dx2;
+        d.pt_part[PS_X2 * (size_t)d.n_pt_blocks + ch] = x2;
     }
 }
 
@@ -837,7 +986,8 @@ __global__ void __launch_bounds__(kWG) k_model_cand(Dev<Real> d) {
     LMState* st = d.st;
     if (st->done || st->step_invalid) return;
     __shared__ double lds[2 * kWaves];
-    const int c = blockIdx.x;
+    const int c = xcd_remap(blockIdx.x, d.nc);
+    if (c >= d.nc) return;
     const CameraPrep<Real> P = d.prep[c];
     const CameraPrep<double> Pc = d.prep_c[c];
     Real dc[9];
@@ -847,7 +997,8 @@ __global__ void __launch_bounds__(kWG) k_model_cand(Dev<Real> d) {
         int p;
         Real r[2], Jc[18], Jp[6];
         corrected_jacobian<Real>(d, P, j, p, r, Jc, Jp);
-        const Real dp[3] = {(Real)d.dpt[p], (Real)d.dpt[(size_t)d.np + p], (Real)d.dpt[2 * (size_t)d.np + p]};
+        const double4 dp4 = reinterpret_cast<const double4*>(d.dpt)[p];
+        const Real dp[3] = {(Real)dp4.x, (Real)dp4.y, (Real)dp4.z};
         Real m0 = Jp[0] * dp[0] + Jp[1] * dp[1] + Jp[2] * dp[2];
         Real m1 = Jp[3] * dp[0] + Jp[4] * dp[1] + Jp[5] * dp[2];
 #pragma unroll
@@ -857,7 +1008,7 @@ __global__ void __launch_bounds__(kWG) k_model_cand(Dev<Real> d) {
         }
         acc[0] += (double)m0 * ((double)r[0] + 0.5 * (double)m0) + (double)m1 * ((double)r[1] + 0.5 * (double)m1);
         double X[3];
-        load_point<double>(d.pt_c, d.np, p, X);
+        load_point<double>(d.pt_c, p, X);
         double rc[2];
         snavely_residual<double>(Pc, X, d.co_uv64[2 * (size_t)j], d.co_uv64[2 * (size_t)j + 1], rc);
         acc[1] += half_rho(rc[0] * rc[0] + rc[1] * rc[1], d.o.loss, d.o.loss_a);
@@ -977,7 +1128,7 @@ template <typename Real>
 __global__ void __launch_bounds__(kWG) k_apply(Dev<Real> d) {
     if (!d.st->accepted) return;
     const size_t i = (size_t)blockIdx.x * kWG + threadIdx.x;
-    const size_t ncam = 9 * (size_t)d.nc, npt = 3 * (size_t)d.np;
+    const size_t ncam = 9 * (size_t)d.nc, npt = 4 * (size_t)d.np;
     if (i < ncam) d.cam_x[i] = d.cam_c[i];
     else if (i < ncam + npt) d.pt_x[i - ncam] = d.pt_c[i - ncam];
 }

@@ -16,7 +16,9 @@ struct Layout {
     std::vector<int> cam_off;    // nc + 1
     std::vector<int> co_pt;      // internal point of each camera-order observation
     std::vector<double> co_uv;   // x, y per camera-order observation
-    std::vector<int> chunk_off, chunk_len, slot_j;
+    std::vector<int> chunk_off, chunk_len;
+    std::vector<int> slot_of;    // camera-order observation -> SELL slot
+    std::vector<int> slot_j;     // SELL slot -> camera-order observation (-1: padding)
 };
 
 int build_layout(const ba_problem_view* v, Layout* L);

@@ -66,12 +66,25 @@ int build_layout(const ba_problem_view* v, Layout* L) {
     L->nc = nc;
     L->np = np;
     L->no = no;
-    std::vector<int> track(np, 0);
-    for (int k = 0; k < no; ++k) track[v->point_index[k]]++;
-    // points by track length, longest first (ties by index): one wave = 64 similar tracks
+    std::vector<int> track(np, 0), first(np, nc);
+    for (int k = 0; k < no; ++k) {
+        track[v->point_index[k]]++;
+        first[v->point_index[k]] = std::min(first[v->point_index[k]], v->camera_index[k]);
+    }
+    // points by first observing camera (a camera's observations then touch clustered points),
+    // then, inside windows of 512 points, by track length (longest first) so that a wave's 64
+    // points have similar tracks and the SELL padding stays small
     L->pt_perm.resize(np);
     std::iota(L->pt_perm.begin(), L->pt_perm.end(), 0);
-    std::stable_sort(L->pt_perm.begin(), L->pt_perm.end(), [&](int a, int b) { return track[a] > track[b]; });
+    std::sort(L->pt_perm.begin(), L->pt_perm.end(), [&](int a, int b) {
+        if (first[a] != first[b]) return first[a] < first[b];
+        if (track[a] != track[b]) return track[a] > track[b];
+        return a < b;
+    });
+    constexpr int kWindow = 512;
+    for (int s0 = 0; s0 < np; s0 += kWindow)
+        std::stable_sort(L->pt_perm.begin() + s0, L->pt_perm.begin() + std::min(np, s0 + kWindow),
+                         [&](int a, int b) { return track[a] > track[b]; });
     std::vector<int> inv(np);
     for (int i = 0; i < np; ++i) inv[L->pt_perm[i]] = i;
     // observations by (camera, internal point)
@@ -96,15 +109,20 @@ int build_layout(const ba_problem_view* v, Layout* L) {
     L->chunk_len.assign(L->n_chunks, 0);
     L->chunk_off.assign(L->n_chunks + 1, 0);
     for (int ch = 0; ch < L->n_chunks; ++ch) {
-        L->chunk_len[ch] = track[L->pt_perm[ch * 64]];
+        int len = 0;
+        for (int i = ch * 64; i < std::min(np, ch * 64 + 64); ++i) len = std::max(len, track[L->pt_perm[i]]);
+        L->chunk_len[ch] = len;
         L->chunk_off[ch + 1] = L->chunk_off[ch] + 64 * L->chunk_len[ch];
     }
     L->slot_j.assign((size_t)L->chunk_off[L->n_chunks], -1);
+    L->slot_of.assign(no, -1);
     std::vector<int> fill(np, 0);
     for (int j = 0; j < no; ++j) {
         const int ip = L->co_pt[j];
         const int ch = ip >> 6, lane = ip & 63;
-        L->slot_j[(size_t)L->chunk_off[ch] + 64 * (size_t)fill[ip] + lane] = j;
+        const size_t slot = (size_t)L->chunk_off[ch] + 64 * (size_t)fill[ip] + lane;
+        L->slot_j[slot] = j;
+        L->slot_of[j] = (int)slot;
         fill[ip]++;
     }
     return BA_OK;
@@ -119,15 +137,17 @@ struct Solver final : SolverBase {
     int linear_solver = BA_SOLVER_ITERATIVE_SCHUR;
     Layout L;
     Dev<Real> d{};
-    DevBuf<int> cam_off, co_pt, chunk_off, chunk_len, slot_j;
+    DevBuf<int> cam_off, co_pt, chunk_off, chunk_len, slot_of;
     DevBuf<double> co_uv64;
     DevBuf<Real> co_uvr;
     DevBuf<double> cam_x, pt_x, cam_x0, pt_x0, cam_c, pt_c, dcam, dpt;
     DevBuf<CameraPrep<Real>> prep;
     DevBuf<CameraPrep<double>> prep_c;
-    DevBuf<double> U, gc, sc, Ustar, M, Minv, bc, px, pr, pz, pp;
-    DevBuf<double> V, gp, sp, Vinv64;
-    DevBuf<Real> VinvR, qp, zs, pv, u;
+    DevBuf<double> U, gc, sc, dc2, M, Minv, bc, px, pr, pz, pp, qc;
+    DevBuf<double> V, gp, sp, dp2, Vinv64, upd_part, rz_hist;
+    DevBuf<Real> VinvR;
+    DevBuf<R4<Real>> qp, zs, u;
+    DevBuf<double4> pv;
     DevBuf<double> cam_part, pt_part;
     DevBuf<LMState> st;
     DevBuf<ba_iteration> trace;
@@ -152,11 +172,11 @@ struct Solver final : SolverBase {
 
     int64_t device_bytes() const override {
         int64_t b = 0;
-        b += cam_off.bytes() + co_pt.bytes() + chunk_off.bytes() + chunk_len.bytes() + slot_j.bytes();
+        b += cam_off.bytes() + co_pt.bytes() + chunk_off.bytes() + chunk_len.bytes() + slot_of.bytes();
         b += co_uv64.bytes() + co_uvr.bytes();
         b += cam_x.bytes() * 4 + pt_x.bytes() * 4 + dcam.bytes() + dpt.bytes();
         b += prep.bytes() + prep_c.bytes();
-        b += U.bytes() * 4 + gc.bytes() * 7;
+        b += M.bytes() * 2 + gc.bytes() * 10 + upd_part.bytes() + rz_hist.bytes() + dp2.bytes();
         b += V.bytes() + gp.bytes() + sp.bytes() + Vinv64.bytes();
         b += VinvR.bytes() + qp.bytes() + zs.bytes() + pv.bytes() + u.bytes();
         b += cam_part.bytes() + pt_part.bytes() + dense.device_bytes();
@@ -179,7 +199,7 @@ struct Solver final : SolverBase {
         BA_TRY(co_pt.upload(L.co_pt.data(), L.co_pt.size(), s));
         BA_TRY(chunk_off.upload(L.chunk_off.data(), L.chunk_off.size(), s));
         BA_TRY(chunk_len.upload(L.chunk_len.data(), L.chunk_len.size(), s));
-        BA_TRY(slot_j.upload(L.slot_j.data(), L.slot_j.size(), s));
+        BA_TRY(slot_of.upload(L.slot_of.data(), L.slot_of.size(), s));
         BA_TRY(co_uv64.upload(L.co_uv.data(), L.co_uv.size(), s));
         if (sizeof(Real) != sizeof(double)) {
             std::vector<Real> uvr(L.co_uv.begin(), L.co_uv.end());
@@ -187,34 +207,42 @@ struct Solver final : SolverBase {
             BA_HIP(hipStreamSynchronize(s));
         }
         // parameters: SoA, points in internal order
-        std::vector<double> cx(9 * (size_t)nc), px_(3 * (size_t)np);
+        std::vector<double> cx(9 * (size_t)nc), px_(4 * (size_t)np, 0.0);
         for (int c = 0; c < nc; ++c)
             for (int i = 0; i < 9; ++i) cx[(size_t)i * nc + c] = v->cameras[9 * (size_t)c + i];
         for (int ip = 0; ip < np; ++ip)
-            for (int i = 0; i < 3; ++i) px_[(size_t)i * np + ip] = v->points[3 * (size_t)L.pt_perm[ip] + i];
+            for (int i = 0; i < 3; ++i) px_[4 * (size_t)ip + i] = v->points[3 * (size_t)L.pt_perm[ip] + i];
         BA_TRY(cam_x0.upload(cx.data(), cx.size(), s));
         BA_TRY(pt_x0.upload(px_.data(), px_.size(), s));
         BA_HIP(hipStreamSynchronize(s));
         BA_TRY(cam_x.alloc(9 * (size_t)nc));
-        BA_TRY(pt_x.alloc(3 * (size_t)np));
+        BA_TRY(pt_x.alloc(4 * (size_t)np));
         BA_TRY(cam_c.alloc(9 * (size_t)nc));
-        BA_TRY(pt_c.alloc(3 * (size_t)np));
+        BA_TRY(pt_c.alloc(4 * (size_t)np));
         BA_TRY(dcam.alloc(9 * (size_t)nc));
-        BA_TRY(dpt.alloc(3 * (size_t)np));
+        BA_TRY(dpt.alloc(4 * (size_t)np));
         BA_TRY(prep.alloc(nc));
         BA_TRY(prep_c.alloc(nc));
-        for (DevBuf<double>* b : {&U, &Ustar, &M, &Minv}) BA_TRY(b->alloc(81 * (size_t)nc));
-        for (DevBuf<double>* b : {&gc, &sc, &bc, &px, &pr, &pz, &pp}) BA_TRY(b->alloc(9 * (size_t)nc));
+        for (DevBuf<double>* b : {&M, &Minv}) BA_TRY(b->alloc(81 * (size_t)nc));
+        for (DevBuf<double>* b : {&U, &gc, &sc, &dc2, &bc, &px, &pr, &pz, &pp, &qc}) BA_TRY(b->alloc(9 * (size_t)nc));
+        const int n_upd = std::max(1, div_up(std::max(nc, 1), kCamsPerUpd));
+        BA_TRY(upd_part.alloc(2 * (size_t)n_upd));
+        BA_TRY(rz_hist.alloc((size_t)std::max(o.pcg_max_iterations, 0) + 2));
         BA_TRY(V.alloc(6 * (size_t)np));
         BA_TRY(gp.alloc(3 * (size_t)np));
         BA_TRY(sp.alloc(3 * (size_t)np));
+        BA_TRY(dp2.alloc(3 * (size_t)np));
         BA_TRY(Vinv64.alloc(6 * (size_t)np));
         BA_TRY(VinvR.alloc(6 * (size_t)np));
-        BA_TRY(qp.alloc(3 * (size_t)np));
-        BA_TRY(zs.alloc(3 * (size_t)np));
-        BA_TRY(pv.alloc(9 * (size_t)no));
-        BA_TRY(u.alloc(3 * (size_t)no));
-        const int n_pt_blocks = std::max(1, div_up(std::max(np, 1), kWG));
+        BA_TRY(qp.alloc(np));
+        BA_TRY(zs.alloc(np));
+        // slot-order buffers: padding slots are never written and must read as zero
+        const size_t nslots = (size_t)std::max(L.chunk_off[L.n_chunks], 1);
+        BA_TRY(pv.alloc(3 * nslots));
+        BA_TRY(u.alloc(nslots));
+        BA_HIP(hipMemsetAsync(pv.p, 0, pv.bytes(), s));
+        BA_HIP(hipMemsetAsync(u.p, 0, u.bytes(), s));
+        const int n_pt_blocks = std::max(1, L.n_chunks);  // point kernels: one block per chunk
         BA_TRY(cam_part.alloc(PS_COUNT * (size_t)std::max(nc, 1)));
         BA_TRY(pt_part.alloc(PS_COUNT * (size_t)n_pt_blocks));
         BA_HIP(hipMemsetAsync(cam_part.p, 0, cam_part.bytes(), s));
@@ -234,7 +262,7 @@ struct Solver final : SolverBase {
         d.co_uv = sizeof(Real) == sizeof(double) ? (const Real*)co_uv64.p : co_uvr.p;
         d.chunk_off = chunk_off.p;
         d.chunk_len = chunk_len.p;
-        d.slot_j = slot_j.p;
+        d.slot_of = slot_of.p;
         d.cam_x = cam_x.p;
         d.pt_x = pt_x.p;
         d.cam_c = cam_c.p;
@@ -246,7 +274,12 @@ struct Solver final : SolverBase {
         d.U = U.p;
         d.gc = gc.p;
         d.sc = sc.p;
-        d.Ustar = Ustar.p;
+        d.dc2 = dc2.p;
+        d.qc = qc.p;
+        d.dp2 = dp2.p;
+        d.upd_part = upd_part.p;
+        d.rz_hist = rz_hist.p;
+        d.n_upd = n_upd;
         d.M = M.p;
         d.Minv = Minv.p;
         d.bc = bc.p;
@@ -282,7 +315,7 @@ struct Solver final : SolverBase {
 
     int download(double* cams, double* pts) override {
         const int nc = L.nc, np = L.np;
-        std::vector<double> cx(9 * (size_t)nc), px_(3 * (size_t)np);
+        std::vector<double> cx(9 * (size_t)nc), px_(4 * (size_t)np);
         if (nc) BA_HIP(hipMemcpyAsync(cx.data(), cam_x.p, cx.size() * sizeof(double), hipMemcpyDeviceToHost, s));
         if (np) BA_HIP(hipMemcpyAsync(px_.data(), pt_x.p, px_.size() * sizeof(double), hipMemcpyDeviceToHost, s));
         BA_HIP(hipStreamSynchronize(s));
@@ -291,7 +324,7 @@ struct Solver final : SolverBase {
                 for (int i = 0; i < 9; ++i) cams[9 * (size_t)c + i] = cx[(size_t)i * nc + c];
         if (pts)
             for (int ip = 0; ip < np; ++ip)
-                for (int i = 0; i < 3; ++i) pts[3 * (size_t)L.pt_perm[ip] + i] = px_[(size_t)i * np + ip];
+                for (int i = 0; i < 3; ++i) pts[3 * (size_t)L.pt_perm[ip] + i] = px_[4 * (size_t)ip + i];
         return BA_OK;
     }
 
@@ -325,36 +358,36 @@ struct Solver final : SolverBase {
     }
 
     void enqueue_linearize() {
-        const int nc = L.nc, gpt = div_up(std::max(L.n_chunks * 64, 1), kWG);
+        const int nc = L.nc, gpt = xcd_grid(d.n_pt_blocks), gcam = xcd_grid(nc);
         launch("cam_prep", k_cam_prep<Real>, div_up(nc, 64), 64, d);
-        launch("lin_cam", k_lin_cam<Real>, nc, kWG, d);
+        launch("lin_cam", k_lin_cam<Real>, gcam, kWG, d);
         launch("lin_pt", k_lin_pt<Real>, gpt, kWG, d);
     }
 
     void enqueue_iteration() {
         const int nc = L.nc, np = L.np;
-        const int gpt = div_up(std::max(L.n_chunks * 64, 1), kWG);
+        const int gpt = xcd_grid(d.n_pt_blocks), gcam = xcd_grid(nc);
         launch("damp_pt", k_damp_pt<Real>, div_up(std::max(np, 1), kWG), kWG, d);
-        launch("rhs_cam", k_rhs_cam<Real>, nc, kWG, d);
+        launch("rhs_cam", k_rhs_cam<Real>, gcam, kWG, d);
         if (linear_solver == BA_SOLVER_DENSE_SCHUR) {
             dense.enqueue_solve(*this);
         } else {
-            launch("cam_factor", k_cam_factor<Real>, div_up(nc, 64), 64, d);
-            launch("pcg_beta", k_pcg_beta<Real>, 1, kWG, d, 1);
-            for (int it = 0; it < opt.pcg_max_iterations; ++it) {
-                launch("jtx", k_jtx<Real>, nc, kWG, d, 1, (const double*)d.pp);
+            launch("cam_factor", k_cam_factor<Real>, d.n_upd, kWG, d);
+            for (int it = 1; it <= opt.pcg_max_iterations; ++it) {
+                launch("jtx", k_jtx<Real>, gcam, kWG, d, 1, it, (const double*)d.pp);
                 launch("pt_z", k_pt_z<Real>, gpt, kWG, d);
-                launch("pcg_alpha", k_pcg_alpha<Real>, 1, kWG, d);
-                launch("jz", k_jz<Real>, nc, kWG, d);
-                launch("pcg_beta", k_pcg_beta<Real>, 1, kWG, d, 0);
+                launch("jz", k_jz<Real>, gcam, kWG, d);
+                launch("cam_update", k_cam_update<Real>, d.n_upd, kWG, d, it);
             }
+            // the convergence test after the last update (sets the iteration count)
+            launch("jtx", k_jtx<Real>, gcam, kWG, d, 1, opt.pcg_max_iterations + 1, (const double*)d.pp);
         }
-        launch("jtx_y", k_jtx<Real>, nc, kWG, d, 0, (const double*)d.px);
+        launch("jtx_y", k_jtx<Real>, gcam, kWG, d, 0, 0, (const double*)d.px);
         launch("pt_backsub", k_pt_backsub<Real>, gpt, kWG, d);
         launch("cam_step", k_cam_step<Real>, div_up(nc, 64), 64, d);
-        launch("model_cand", k_model_cand<Real>, nc, kWG, d);
+        launch("model_cand", k_model_cand<Real>, gcam, kWG, d);
         launch("decide", k_decide<Real>, 1, kWG, d);
-        launch("apply", k_apply<Real>, div_up(9 * (int64_t)nc + 3 * (int64_t)np, kWG), kWG, d);
+        launch("apply", k_apply<Real>, div_up(9 * (int64_t)nc + 4 * (int64_t)np, kWG), kWG, d);
         enqueue_linearize();
         launch("post_lin", k_post_lin<Real>, 1, kWG, d);
     }
@@ -381,7 +414,7 @@ struct Solver final : SolverBase {
         }
         enqueue_linearize();
         launch("prep_x", k_prep_from<Real>, div_up(nc, 64), 64, d, (const double*)d.cam_x);
-        launch("cost_x", k_cost_cam<Real>, nc, kWG, d, (const double*)d.pt_x);
+        launch("cost_x", k_cost_cam<Real>, xcd_grid(nc), kWG, d, (const double*)d.pt_x);
         launch("post_lin", k_post_lin<Real>, 1, kWG, d);
         BA_HIP(hipGetLastError());
         const int check_every = 4;
@@ -458,11 +491,11 @@ struct Solver final : SolverBase {
         const std::string n(name);
         // Algorithmic bytes of one launch: every stream counted once per pass, per-camera and per-point
         // parameters once per pass, FP64 parameters, int32 indices.
-        if (n == "jtx" || n == "jtx_y") return no * (4 + 2 * R + 3 * R) + np * 3 * 8 + nc * 9 * 8 * 3;
-        if (n == "jz") return no * (4 + 2 * R) + np * (3 * 8 + 3 * R) + nc * 9 * 8 * 8;
-        if (n == "pt_z") return (no + 63.0 * L.n_chunks / 2) * (4 + 3 * R) + np * (3 * 8 + 6 * R + 3 * R);
-        if (n == "lin_cam") return no * (4 + 2 * R + 9 * R) + np * 3 * 8 + nc * (9 + 81 + 9) * 8;
-        if (n == "lin_pt") return no * (4 + 9 * R) + np * 9 * 8;
+        if (n == "jtx" || n == "jtx_y") return no * (4 + 4 + 2 * R + 4 * R) + np * 4 * 8 + nc * 9 * 8 * 3;
+        if (n == "jz") return no * (4 + 2 * R) + np * (4 * 8 + 4 * R) + nc * 9 * 8 * 8;
+        if (n == "pt_z") return no * 4 * R + np * (3 * 8 + 3 * 8 + 6 * R + 4 * R);
+        if (n == "lin_cam") return no * (4 + 4 + 2 * R + 12 * 8) + np * 4 * 8 + nc * 18 * 8;
+        if (n == "lin_pt") return no * 12 * 8 + np * 9 * 8;
         if (n == "rhs_cam") return no * (4 + 2 * R) + np * (3 * 8 + 3 * 8 + 6 * R + 3 * R) + nc * (81 * 3 + 9 * 3) * 8;
         if (n == "model_cand") return no * (4 + 2 * R + 16) + np * 3 * 8 * 3 + nc * 9 * 8;
         return 0.0;

@@ -171,6 +171,70 @@ int ba_solver_kernel_times(ba_solver* solver, ba_kernel_time* out, int32_t capac
 /* Observations in the solver's device layout (after dropping nothing) and device bytes held. */
 int ba_solver_info(ba_solver* solver, int64_t* device_bytes, int32_t* linear_solver);
 
+/* ---- Batched local BA: many independent small problems ("windows") ----
+ * The caller of windowed BA is full_slam's local bundle adjustment (full_slam/src/backend.cpp:67-199,
+ * 7-10 keyframes); here every window is a Snavely BA problem solved exactly like ba_solve with the
+ * dense Schur solver (per window: explicit reduced camera system + Cholesky), each with its own LM
+ * state.  One workgroup owns one window for a whole LM iteration, so a window's result does not
+ * depend on which other windows share the launch.  A window's reduced camera system is factored in
+ * the workgroup's LDS: up to 21 cameras in mixed precision, 14 in FP64; larger windows are solved by
+ * the single-problem solver (ba_solver) and reported the same way. */
+typedef struct ba_batch ba_batch;
+int ba_batch_create(const ba_problem_view* windows, int32_t count, const ba_options* options, ba_batch** out);
+/* Solves every window from the parameters held on the device (options.max_num_iterations each).
+ * summaries: optional array of `count` (their trace pointers are ignored). */
+int ba_batch_run(ba_batch* batch, ba_summary* summaries);
+int ba_batch_reset(ba_batch* batch);
+int ba_batch_download(ba_batch* batch, int32_t window, double* cameras, double* points);
+int ba_batch_destroy(ba_batch* batch);
+/* One-shot: create, run, write every window's parameters back in place, destroy. */
+int ba_batch_solve(const ba_problem_view* windows, int32_t count, const ba_options* options, ba_summary* summaries);
+/* Workgroup size used by the batched kernel (256 or 512; 0 = autotune, which create does). */
+int ba_batch_set_block(ba_batch* batch, int32_t threads);
+int ba_batch_info(ba_batch* batch, int32_t* threads, int64_t* device_bytes);
+
+/* ---- Serving: continuous batching of local-BA requests ----
+ * Requests are admitted while their device footprint fits in `arena_bytes` and a slot is free,
+ * their state stays resident on the device, and every step runs `iterations_per_step` LM
+ * iterations for all resident windows in one launch; finished windows are retired and their
+ * results queued for ba_server_poll. */
+typedef struct ba_server_config {
+    int64_t arena_bytes;          /* device memory for resident windows */
+    int32_t max_windows;          /* resident window slots */
+    int32_t iterations_per_step;  /* LM iterations per step (1 = per-iteration batching) */
+    ba_options options;
+} ba_server_config;
+typedef struct ba_server_result {
+    int64_t tag;
+    double initial_cost;
+    double final_cost;
+    int32_t iterations;
+    int32_t termination_type;
+    double submit_time_s;   /* ba_server clock (seconds since create) */
+    double admit_time_s;
+    double finish_time_s;
+    int32_t num_cameras;
+    int32_t num_points;
+    const double* cameras;  /* solved parameters, owned by the server, valid until the next poll */
+    const double* points;
+} ba_server_result;
+typedef struct ba_server ba_server;
+int ba_server_create(const ba_server_config* config, ba_server** out);
+/* Copies the window into the server's queue; it is admitted by a later ba_server_step. */
+int ba_server_submit(ba_server* server, const ba_problem_view* window, int64_t tag);
+/* Admits queued windows that fit, runs one step, retires finished windows.  *resident receives the
+ * number of windows on the device after the step, *queued the number still waiting. */
+int ba_server_step(ba_server* server, int32_t* resident, int32_t* queued);
+/* Pops up to `capacity` finished results (oldest first). */
+int ba_server_poll(ba_server* server, ba_server_result* out, int32_t capacity, int32_t* count);
+double ba_server_clock(ba_server* server);
+int ba_server_destroy(ba_server* server);
+
+/* Load generator: Poisson arrivals at `rate_hz` over `count` requests drawn round-robin from
+ * `pool`, served by a ba_server; per-request results (latency = finish - submit) in `results`. */
+int ba_server_simulate(const ba_server_config* config, const ba_problem_view* pool, int32_t pool_count,
+                       double rate_hz, int32_t count, uint64_t seed, ba_server_result* results);
+
 /* ---- Host BALProblem (utils/datasets/BALProblem.{h,cpp}) through opaque handles ---- */
 typedef struct ba_bal ba_bal;
 /* BALProblem::BALProblem(filename, use_quaternions) (BALProblem.cpp:78-139); use_quaternions must

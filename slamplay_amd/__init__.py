@@ -12,6 +12,9 @@ libslamplay_ba.so:
   * Solver        -- device-resident solver handle (upload once, run many times).
   * eval_residual_jacobian / eval_cost -- per-observation evaluation on the GPU.
   * synthetic     -- seeded synthetic problems of the benchmark configurations.
+  * Batch         -- many small BA problems ("windows": the local BA of full_slam/src/backend.cpp:67-199)
+                     solved in one launch, one workgroup per window.
+  * Server / simulate -- continuous batching of local-BA requests and its Poisson load generator.
 
 Every entry point runs the HIP kernels of libslamplay_ba.so; there is no CPU fallback.
 """
@@ -22,8 +25,8 @@ import numpy as np
 from . import _capi as C
 from ._capi import BAError
 
-__all__ = ["BALProblem", "SolveBA", "Solver", "default_options", "eval_residual_jacobian", "eval_cost",
-           "synthetic", "synth_params", "BAError", "device_count"]
+__all__ = ["BALProblem", "SolveBA", "Solver", "Batch", "Server", "simulate", "server_config", "default_options",
+           "eval_residual_jacobian", "eval_cost", "synthetic", "synth_params", "BAError", "device_count"]
 
 
 def default_options(**kw):
@@ -256,3 +259,146 @@ def synthetic(config=1, seed=1, ground_truth=False, **kw):
         return prob, gt[: prob.num_parameters()]
     C.check(C.lib().ba_synth_generate(ctypes.byref(p), ctypes.byref(h), None, 0))
     return BALProblem(_handle=h)
+
+
+def _views(problems):
+    arr = (C.ProblemView * len(problems))()
+    for i, p in enumerate(problems):
+        arr[i] = p.view()
+    return arr
+
+
+def _summary_plain(s):
+    return dict(initial_cost=s.initial_cost, final_cost=s.final_cost, num_iterations=s.num_iterations,
+                num_successful_steps=s.num_successful_steps, num_unsuccessful_steps=s.num_unsuccessful_steps,
+                termination=s.termination_type, linear_solver=s.linear_solver_used)
+
+
+class Batch:
+    """Batched local BA: every window (a BALProblem) keeps its own LM state; run() solves all of them
+    from the parameters held on the device, download(w) returns window w's (cameras, points)."""
+
+    def __init__(self, problems, options=None, **kw):
+        self.options = options if options is not None else default_options(**kw)
+        self._shapes = [(p.num_cameras(), p.num_points()) for p in problems]
+        views = _views(problems)
+        h = C.vp()
+        C.check(C.lib().ba_batch_create(views, len(problems), ctypes.byref(self.options), ctypes.byref(h)))
+        self._h = h
+
+    def __len__(self):
+        return len(self._shapes)
+
+    def run(self):
+        sums = (C.Summary * max(1, len(self)))()
+        C.check(C.lib().ba_batch_run(self._h, sums))
+        return [_summary_plain(sums[i]) for i in range(len(self))]
+
+    def reset(self):
+        C.check(C.lib().ba_batch_reset(self._h))
+
+    def download(self, w):
+        nc, npt = self._shapes[w]
+        cams = np.zeros(9 * nc)
+        pts = np.zeros(3 * npt)
+        C.check(C.lib().ba_batch_download(self._h, w, cams.ctypes.data_as(C.f64p), pts.ctypes.data_as(C.f64p)))
+        return cams, pts
+
+    def set_block(self, threads):
+        C.check(C.lib().ba_batch_set_block(self._h, threads))
+
+    def info(self):
+        t = C.i32(0)
+        b = C.i64(0)
+        C.check(C.lib().ba_batch_info(self._h, ctypes.byref(t), ctypes.byref(b)))
+        return dict(threads=t.value, device_bytes=b.value)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            C.lib().ba_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if C._lib is not None:
+            self.close()
+
+
+def server_config(arena_bytes=1 << 30, max_windows=512, iterations_per_step=1, options=None, **kw):
+    c = C.ServerConfig()
+    c.arena_bytes = arena_bytes
+    c.max_windows = max_windows
+    c.iterations_per_step = iterations_per_step
+    c.options = options if options is not None else default_options(**kw)
+    return c
+
+
+def _result_dict(r, with_params):
+    d = dict(tag=r.tag, initial_cost=r.initial_cost, final_cost=r.final_cost, iterations=r.iterations,
+             termination=r.termination_type, submit_time_s=r.submit_time_s, admit_time_s=r.admit_time_s,
+             finish_time_s=r.finish_time_s, latency_s=r.finish_time_s - r.submit_time_s)
+    if with_params:
+        d["cameras"] = np.ctypeslib.as_array(r.cameras, (9 * r.num_cameras,)).copy() if r.num_cameras else np.zeros(0)
+        d["points"] = np.ctypeslib.as_array(r.points, (3 * r.num_points,)).copy() if r.num_points else np.zeros(0)
+    return d
+
+
+class Server:
+    """Continuous batching of local-BA requests (see ba_server_* in include/mba/ba_c_api.h)."""
+
+    def __init__(self, config=None, **kw):
+        self.config = config if config is not None else server_config(**kw)
+        h = C.vp()
+        C.check(C.lib().ba_server_create(ctypes.byref(self.config), ctypes.byref(h)))
+        self._h = h
+
+    def submit(self, problem, tag):
+        C.check(C.lib().ba_server_submit(self._h, ctypes.byref(problem.view()), tag))
+
+    def step(self):
+        r = C.i32(0)
+        q = C.i32(0)
+        C.check(C.lib().ba_server_step(self._h, ctypes.byref(r), ctypes.byref(q)))
+        return r.value, q.value
+
+    def poll(self, capacity=256):
+        buf = (C.ServerResult * capacity)()
+        n = C.i32(0)
+        C.check(C.lib().ba_server_poll(self._h, buf, capacity, ctypes.byref(n)))
+        return [_result_dict(buf[i], True) for i in range(n.value)]
+
+    def clock(self):
+        return C.lib().ba_server_clock(self._h)
+
+    def drain(self, max_steps=1_000_000):
+        """Steps until nothing is resident or queued; returns every result."""
+        out = []
+        for _ in range(max_steps):
+            resident, queued = self.step()
+            out += self.poll()
+            if resident == 0 and queued == 0:
+                break
+        while True:
+            more = self.poll()
+            if not more:
+                break
+            out += more
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            C.lib().ba_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if C._lib is not None:
+            self.close()
+
+
+def simulate(pool, rate_hz, count, seed=1, config=None, **kw):
+    """Poisson arrivals at rate_hz of `count` requests drawn round-robin from `pool` (BALProblems),
+    served by a Server; returns one dict per request (latency_s = finish - scheduled arrival)."""
+    cfg = config if config is not None else server_config(**kw)
+    views = _views(pool)
+    res = (C.ServerResult * max(1, count))()
+    C.check(C.lib().ba_server_simulate(ctypes.byref(cfg), views, len(pool), rate_hz, count, seed, res))
+    return [_result_dict(res[i], False) for i in range(count)]

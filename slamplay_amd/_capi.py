@@ -66,6 +66,16 @@ class SynthParams(ctypes.Structure):
                 ("distortion_sigma", f64), ("seed", ctypes.c_uint64)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("arena_bytes", i64), ("max_windows", i32), ("iterations_per_step", i32), ("options", Options)]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [("tag", i64), ("initial_cost", f64), ("final_cost", f64), ("iterations", i32),
+                ("termination_type", i32), ("submit_time_s", f64), ("admit_time_s", f64), ("finish_time_s", f64),
+                ("num_cameras", i32), ("num_points", i32), ("cameras", f64p), ("points", f64p)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "ba_abi_version": (i32, []),
@@ -83,6 +93,24 @@ SIGNATURES = {
     "ba_solver_set_profiling": (ctypes.c_int, [vp, i32]),
     "ba_solver_kernel_times": (ctypes.c_int, [vp, ctypes.POINTER(KernelTime), i32, i32p]),
     "ba_solver_info": (ctypes.c_int, [vp, ctypes.POINTER(i64), i32p]),
+    "ba_batch_create": (ctypes.c_int, [ctypes.POINTER(ProblemView), i32, ctypes.POINTER(Options),
+                                        ctypes.POINTER(vp)]),
+    "ba_batch_run": (ctypes.c_int, [vp, ctypes.POINTER(Summary)]),
+    "ba_batch_reset": (ctypes.c_int, [vp]),
+    "ba_batch_download": (ctypes.c_int, [vp, i32, f64p, f64p]),
+    "ba_batch_destroy": (ctypes.c_int, [vp]),
+    "ba_batch_solve": (ctypes.c_int, [ctypes.POINTER(ProblemView), i32, ctypes.POINTER(Options),
+                                       ctypes.POINTER(Summary)]),
+    "ba_batch_set_block": (ctypes.c_int, [vp, i32]),
+    "ba_batch_info": (ctypes.c_int, [vp, i32p, ctypes.POINTER(i64)]),
+    "ba_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
+    "ba_server_submit": (ctypes.c_int, [vp, ctypes.POINTER(ProblemView), i64]),
+    "ba_server_step": (ctypes.c_int, [vp, i32p, i32p]),
+    "ba_server_poll": (ctypes.c_int, [vp, ctypes.POINTER(ServerResult), i32, i32p]),
+    "ba_server_clock": (f64, [vp]),
+    "ba_server_destroy": (ctypes.c_int, [vp]),
+    "ba_server_simulate": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(ProblemView), i32, f64, i32,
+                                           ctypes.c_uint64, ctypes.POINTER(ServerResult)]),
     "ba_bal_load": (ctypes.c_int, [ctypes.c_char_p, i32, ctypes.POINTER(vp)]),
     "ba_bal_from_arrays": (ctypes.c_int, [i32, i32, i32, i32p, i32p, f64p, f64p, f64p, ctypes.POINTER(vp)]),
     "ba_bal_view": (ctypes.c_int, [vp, ctypes.POINTER(ProblemView)]),

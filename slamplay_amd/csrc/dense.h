@@ -303,66 +303,9 @@ struct DenseSchur {
         nt = std::max(1, (n + kTile - 1) / kTile);
         ld = nt * kTile;
         refine = sizeof(Real) == sizeof(double) ? 0 : std::max(0, o.refine_steps);
-        // camera of each camera-order observation
-        std::vector<int> cam_of(L.no);
-        for (int c = 0; c < nc; ++c)
-            for (int j = L.cam_off[c]; j < L.cam_off[c + 1]; ++j) cam_of[j] = c;
-        // point observation lists (camera order) from the SELL slots
-        std::vector<int> pt_off(L.np + 1, 0);
-        for (int j = 0; j < L.no; ++j) pt_off[L.co_pt[j] + 1]++;
-        for (int p = 0; p < L.np; ++p) pt_off[p + 1] += pt_off[p];
-        std::vector<int> pt_obs(L.no), fill(pt_off.begin(), pt_off.end() - 1);
-        for (int j = 0; j < L.no; ++j) pt_obs[fill[L.co_pt[j]]++] = j;
-        struct P3 {
-            int64_t key;
-            int i, j;
-        };
-        std::vector<P3> pairs;
-        size_t total = 0;
-        for (int p = 0; p < L.np; ++p) {
-            const size_t t = (size_t)(pt_off[p + 1] - pt_off[p]);
-            total += t * (t + 1) / 2 + t;
-        }
-        pairs.reserve(total + nc);
-        for (int p = 0; p < L.np; ++p)
-            for (int x = pt_off[p]; x < pt_off[p + 1]; ++x)
-                for (int y = pt_off[p]; y < pt_off[p + 1]; ++y) {
-                    const int i = pt_obs[x], j = pt_obs[y];
-                    if (cam_of[i] <= cam_of[j]) pairs.push_back({(int64_t)cam_of[i] * nc + cam_of[j], i, j});
-                }
-        std::stable_sort(pairs.begin(), pairs.end(), [](const P3& a, const P3& b) { return a.key < b.key; });
-        std::vector<int> ba, bb, boff, pi, pj;
-        pi.reserve(pairs.size());
-        pj.reserve(pairs.size());
-        size_t q = 0;
-        for (int a = 0; a < nc; ++a) {
-            // blocks of row a in increasing b; the diagonal block always exists
-            bool diag_done = false;
-            while (q < pairs.size() && pairs[q].key / nc == a) {
-                const int b = (int)(pairs[q].key % nc);
-                if (b > a && !diag_done) {
-                    ba.push_back(a);
-                    bb.push_back(a);
-                    boff.push_back((int)pi.size());
-                    diag_done = true;
-                }
-                ba.push_back(a);
-                bb.push_back(b);
-                boff.push_back((int)pi.size());
-                if (b == a) diag_done = true;
-                while (q < pairs.size() && pairs[q].key == (int64_t)a * nc + b) {
-                    pi.push_back(pairs[q].i);
-                    pj.push_back(pairs[q].j);
-                    ++q;
-                }
-            }
-            if (!diag_done) {
-                ba.push_back(a);
-                bb.push_back(a);
-                boff.push_back((int)pi.size());
-            }
-        }
-        boff.push_back((int)pi.size());
+        PairBlocks pb;
+        build_pair_blocks(L, &pb);
+        const std::vector<int>&ba = pb.a, &bb = pb.b, &boff = pb.off, &pi = pb.i, &pj = pb.j;
         nblocks = (int)ba.size();
         BA_TRY(blk_a.upload(ba.data(), ba.size(), s));
         BA_TRY(blk_b.upload(bb.data(), bb.size(), s));

@@ -23,6 +23,15 @@ struct Layout {
 
 int build_layout(const ba_problem_view* v, Layout* L);
 
+// 9x9 blocks (a <= b) of the reduced camera system and, per block, the ordered observation pairs
+// (i, j) of a common point with camera(i) = a, camera(j) = b (camera-order indices).  Every
+// diagonal block is present, possibly with no pairs.
+struct PairBlocks {
+    std::vector<int> a, b, off;  // off: nblocks + 1
+    std::vector<int> i, j;
+};
+void build_pair_blocks(const Layout& L, PairBlocks* out);
+
 struct SolverBase {
     virtual ~SolverBase() = default;
     virtual int run(ba_summary* summary) = 0;

@@ -128,6 +128,68 @@ int build_layout(const ba_problem_view* v, Layout* L) {
     return BA_OK;
 }
 
+void build_pair_blocks(const Layout& L, PairBlocks* out) {
+    const int nc = L.nc;
+    std::vector<int> cam_of(L.no);
+    for (int c = 0; c < nc; ++c)
+        for (int j = L.cam_off[c]; j < L.cam_off[c + 1]; ++j) cam_of[j] = c;
+    std::vector<int> pt_off(L.np + 1, 0);
+    for (int j = 0; j < L.no; ++j) pt_off[L.co_pt[j] + 1]++;
+    for (int p = 0; p < L.np; ++p) pt_off[p + 1] += pt_off[p];
+    std::vector<int> pt_obs(L.no), fill(pt_off.begin(), pt_off.end() - 1);
+    for (int j = 0; j < L.no; ++j) pt_obs[fill[L.co_pt[j]]++] = j;
+    struct P3 {
+        int64_t key;
+        int i, j;
+    };
+    std::vector<P3> pairs;
+    size_t total = 0;
+    for (int p = 0; p < L.np; ++p) {
+        const size_t t = (size_t)(pt_off[p + 1] - pt_off[p]);
+        total += t * (t + 1) / 2 + t;
+    }
+    pairs.reserve(total);
+    for (int p = 0; p < L.np; ++p)
+        for (int x = pt_off[p]; x < pt_off[p + 1]; ++x)
+            for (int y = pt_off[p]; y < pt_off[p + 1]; ++y) {
+                const int i = pt_obs[x], j = pt_obs[y];
+                if (cam_of[i] <= cam_of[j]) pairs.push_back({(int64_t)cam_of[i] * nc + cam_of[j], i, j});
+            }
+    std::stable_sort(pairs.begin(), pairs.end(), [](const P3& a, const P3& b) { return a.key < b.key; });
+    PairBlocks& o = *out;
+    o = PairBlocks{};
+    o.i.reserve(pairs.size());
+    o.j.reserve(pairs.size());
+    size_t q = 0;
+    for (int a = 0; a < nc; ++a) {
+        bool diag_done = false;
+        while (q < pairs.size() && pairs[q].key / nc == a) {
+            const int b = (int)(pairs[q].key % nc);
+            if (b > a && !diag_done) {
+                o.a.push_back(a);
+                o.b.push_back(a);
+                o.off.push_back((int)o.i.size());
+                diag_done = true;
+            }
+            o.a.push_back(a);
+            o.b.push_back(b);
+            o.off.push_back((int)o.i.size());
+            if (b == a) diag_done = true;
+            while (q < pairs.size() && pairs[q].key == (int64_t)a * nc + b) {
+                o.i.push_back(pairs[q].i);
+                o.j.push_back(pairs[q].j);
+                ++q;
+            }
+        }
+        if (!diag_done) {
+            o.a.push_back(a);
+            o.b.push_back(a);
+            o.off.push_back((int)o.i.size());
+        }
+    }
+    o.off.push_back((int)o.i.size());
+}
+
 // ---------------------------------------------------------------------------------------------
 template <typename Real>
 struct Solver final : SolverBase {

@@ -1,0 +1,993 @@
+// window.hip -- batched local bundle adjustment (ba_batch_*) and the serving loop (ba_server_*).
+//
+// Windows (small BA problems: the local BA that full_slam/src/backend.cpp:67-199 runs over its
+// 7-10 active keyframes, here with the Snavely camera of the BA path) are packed into regions of
+// one device arena and solved by k_window_lm, one workgroup per window.  Windows whose reduced
+// camera system does not fit the workgroup's LDS are solved by the single-problem solver.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <random>
+#include <thread>
+#include <vector>
+
+#include "solver.h"
+#include "window_kernels.h"
+
+namespace mba {
+
+namespace {
+
+// LDS a workgroup may allocate on the current device (160 KiB on MI355X), queried once.
+int64_t lds_per_block() {
+    static int64_t cached = -1;
+    if (cached < 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            cached = v;
+        else
+            cached = 65536;
+    }
+    return cached;
+}
+
+int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+Opts opts_of(const ba_options& o) {
+    Opts r{};
+    r.loss = o.loss;
+    r.max_iterations = o.max_num_iterations;
+    r.max_invalid = o.max_num_consecutive_invalid_steps;
+    r.jacobi_scaling = o.jacobi_scaling;
+    r.pcg_max_iterations = o.pcg_max_iterations;
+    r.loss_a = o.loss_scale;
+    r.ftol = o.function_tolerance;
+    r.gtol = o.gradient_tolerance;
+    r.ptol = o.parameter_tolerance;
+    r.max_radius = o.max_trust_region_radius;
+    r.min_radius = o.min_trust_region_radius;
+    r.min_rel_decrease = o.min_relative_decrease;
+    r.min_diag = o.min_lm_diagonal;
+    r.max_diag = o.max_lm_diagonal;
+    r.pcg_tol = o.pcg_tolerance;
+    return r;
+}
+
+template <typename Real>
+int64_t static_lds(int nt) {
+    const int nw = nt / 64;
+    return (int64_t)nw * 20 * 8 + (int64_t)nw * 16 * 36 * sizeof(Real) + kWinMaxN * 8 + 16;
+}
+
+// Largest reduced system (9 K) a workgroup of nt threads can factor in LDS.
+template <typename Real>
+int max_window_n(int nt) {
+    const int64_t room = lds_per_block() - static_lds<Real>(nt);
+    int n = (int)std::floor(std::sqrt((double)room / sizeof(Real)));
+    n = std::min(n, kWinMaxN);
+    return n - n % 9;
+}
+
+}  // namespace
+
+// Host image of one window's arena region (offsets relative to the region start).
+struct WindowImage {
+    WinDesc desc{};
+    std::vector<char> bytes;
+    std::vector<int> pt_perm;
+    int K = 0, P = 0;
+    int64_t cost = 0;  // work estimate for dispatch order
+};
+
+template <typename Real>
+int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowImage* img) {
+    Layout L;
+    BA_TRY(build_layout(v, &L));
+    PairBlocks pb;
+    build_pair_blocks(L, &pb);
+    const int K = L.nc, P = L.np, No = L.no, nch = L.n_chunks;
+    const int slots = L.chunk_off[nch];
+    const int nb = (int)pb.a.size(), npairs = (int)pb.i.size();
+    const int n = 9 * K;
+    WinDesc& D = img->desc;
+    D = WinDesc{};
+    D.K = K;
+    D.P = P;
+    D.No = No;
+    D.n_chunks = nch;
+    D.n_slots = slots;
+    D.n_blocks = nb;
+    D.n_pairs = npairs;
+    int64_t off = 0;
+    auto take = [&](int64_t bytes) {
+        const int64_t o = off;
+        off = align256(off + std::max<int64_t>(bytes, 1));
+        return o;
+    };
+    D.o_cam_off = take(4 * (K + 1));
+    D.o_co_pt = take(4 * (int64_t)No);
+    D.o_co_uv = take(2 * (int64_t)No * sizeof(Real));
+    D.o_co_uv64 = take(16 * (int64_t)No);
+    D.o_slot_of = take(4 * (int64_t)No);
+    D.o_chunk_len = take(4 * (int64_t)nch);
+    D.o_chunk_off = take(4 * ((int64_t)nch + 1));
+    D.o_blk = take(16 * (int64_t)nb);
+    D.o_pairs = take(8 * (int64_t)npairs);
+    D.o_cam_x = take(72 * (int64_t)K);
+    D.o_cam_x0 = take(72 * (int64_t)K);
+    D.o_cam_c = take(72 * (int64_t)K);
+    D.o_pt_x = take(32 * (int64_t)P);
+    D.o_pt_x0 = take(32 * (int64_t)P);
+    D.o_pt_c = take(32 * (int64_t)P);
+    D.o_prep = take(sizeof(CameraPrep<Real>) * (int64_t)K);
+    D.o_prep_c = take(sizeof(CameraPrep<double>) * (int64_t)K);
+    D.o_udiag = take(72 * (int64_t)K);
+    D.o_gc = take(72 * (int64_t)K);
+    D.o_sc = take(72 * (int64_t)K);
+    D.o_dc2 = take(72 * (int64_t)K);
+    D.o_bc = take(72 * (int64_t)K);
+    D.o_yc = take(72 * (int64_t)K);
+    D.o_V = take(48 * (int64_t)P);
+    D.o_gp = take(24 * (int64_t)P);
+    D.o_sp = take(24 * (int64_t)P);
+    D.o_Vinv = take(48 * (int64_t)P);
+    D.o_qp = take(sizeof(R4<Real>) * (int64_t)P);
+    D.o_yp = take(24 * (int64_t)P);
+    D.o_pv = take(96 * (int64_t)slots);
+    D.o_u = take(sizeof(R4<Real>) * (int64_t)slots);
+    D.o_js = take(kWinJs * sizeof(Real) * (int64_t)No);
+    D.o_S64 = take(8 * (int64_t)n * n);
+    D.o_state = take(sizeof(WinState));
+    img->bytes.assign(off, 0);
+    char* b = img->bytes.data();
+    std::memcpy(b + D.o_cam_off, L.cam_off.data(), 4 * (size_t)(K + 1));
+    std::memcpy(b + D.o_co_pt, L.co_pt.data(), 4 * (size_t)No);
+    Real* uv = reinterpret_cast<Real*>(b + D.o_co_uv);
+    for (size_t i = 0; i < 2 * (size_t)No; ++i) uv[i] = (Real)L.co_uv[i];
+    std::memcpy(b + D.o_co_uv64, L.co_uv.data(), 16 * (size_t)No);
+    std::memcpy(b + D.o_slot_of, L.slot_of.data(), 4 * (size_t)No);
+    std::memcpy(b + D.o_chunk_len, L.chunk_len.data(), 4 * (size_t)nch);
+    std::memcpy(b + D.o_chunk_off, L.chunk_off.data(), 4 * (size_t)(nch + 1));
+    int* blk = reinterpret_cast<int*>(b + D.o_blk);
+    for (int i = 0; i < nb; ++i) {
+        blk[4 * i] = pb.a[i];
+        blk[4 * i + 1] = pb.b[i];
+        blk[4 * i + 2] = pb.off[i];
+        blk[4 * i + 3] = pb.off[i + 1];
+    }
+    int* pr = reinterpret_cast<int*>(b + D.o_pairs);
+    for (int i = 0; i < npairs; ++i) {
+        pr[2 * i] = pb.i[i];
+        pr[2 * i + 1] = pb.j[i];
+    }
+    std::memcpy(b + D.o_cam_x, v->cameras, 72 * (size_t)K);
+    std::memcpy(b + D.o_cam_x0, v->cameras, 72 * (size_t)K);
+    double* px = reinterpret_cast<double*>(b + D.o_pt_x);
+    double* px0 = reinterpret_cast<double*>(b + D.o_pt_x0);
+    for (int ip = 0; ip < P; ++ip)
+        for (int i = 0; i < 3; ++i) px[4 * (size_t)ip + i] = px0[4 * (size_t)ip + i] = v->points[3 * (size_t)L.pt_perm[ip] + i];
+    WinState* st = reinterpret_cast<WinState*>(b + D.o_state);
+    *st = WinState{};
+    st->radius = opt.initial_trust_region_radius;
+    st->decrease_factor = 2.0;
+    st->need_lin = 1;
+    st->first_lin = 1;
+    st->termination = BA_NO_CONVERGENCE;
+    if (No == 0) {
+        st->done = 1;
+        st->termination = BA_CONVERGENCE;
+        st->first_lin = 0;
+    }
+    img->pt_perm = L.pt_perm;
+    img->K = K;
+    img->P = P;
+    img->cost = (int64_t)No * 8 + (int64_t)npairs * 2 + (int64_t)n * n * n / 64;
+    return BA_OK;
+}
+
+WinDesc rebase(const WinDesc& d, int64_t base) {
+    WinDesc r = d;
+    int64_t* o = &r.o_cam_off;
+    const int count = (int)((&r.o_state - &r.o_cam_off) + 1);
+    for (int i = 0; i < count; ++i) o[i] += base;
+    return r;
+}
+
+// Restores a window's initial parameters and LM state.
+__global__ void k_window_reset(const WinDesc* __restrict__ descs, char* __restrict__ arena,
+                               const int* __restrict__ work, int n_work, WinState init) {
+    const int wi = blockIdx.x;
+    if (wi >= n_work) return;
+    const WinDesc D = descs[work[wi]];
+    double* cx = at<double>(arena, D.o_cam_x);
+    const double* cx0 = at<double>(arena, D.o_cam_x0);
+    double4* px = at<double4>(arena, D.o_pt_x);
+    const double4* px0 = at<double4>(arena, D.o_pt_x0);
+    for (int k = threadIdx.x; k < 9 * D.K; k += blockDim.x) cx[k] = cx0[k];
+    for (int p = threadIdx.x; p < D.P; p += blockDim.x) px[p] = px0[p];
+    if (threadIdx.x == 0) {
+        WinState s = init;
+        if (D.No == 0) {
+            s.done = 1;
+            s.termination = BA_CONVERGENCE;
+            s.first_lin = 0;
+        }
+        *at<WinState>(arena, D.o_state) = s;
+    }
+}
+
+// Gathers the LM states of the listed windows into out (one thread per window).
+__global__ void k_window_states(const WinDesc* __restrict__ descs, const char* __restrict__ arena,
+                                const int* __restrict__ work, int n_work, WinState* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_work) return;
+    out[i] = *reinterpret_cast<const WinState*>(arena + descs[work[i]].o_state);
+}
+
+// ---------------------------------------------------------------------------------------------
+// The engine: an arena of window regions, slot descriptors, launches.
+template <typename Real>
+struct WindowEngine {
+    ba_options opt{};
+    Opts o{};
+    hipStream_t s = nullptr;
+    DevBuf<char> arena;
+    int64_t arena_cap = 0;
+    DevBuf<WinDesc> ddesc;
+    std::vector<WinDesc> hdesc;  // absolute offsets, by slot
+    DevBuf<int> dwork;
+    DevBuf<WinState> dstates;
+    std::vector<WinState> hstates;
+    int threads = 256;
+    int max_slots = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    ~WindowEngine() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (s) (void)hipStreamDestroy(s);
+    }
+
+    int init(const ba_options& options, int64_t arena_bytes, int slots) {
+        opt = options;
+        o = opts_of(options);
+        BA_TRY(use_device(options.device));
+        BA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        BA_HIP(hipEventCreate(&ev0));
+        BA_HIP(hipEventCreate(&ev1));
+        arena_cap = std::max<int64_t>(arena_bytes, 256);
+        BA_TRY(arena.alloc((size_t)arena_cap));
+        max_slots = std::max(slots, 1);
+        BA_TRY(ddesc.alloc(max_slots));
+        hdesc.assign(max_slots, WinDesc{});
+        BA_TRY(dwork.alloc(max_slots));
+        BA_TRY(dstates.alloc(max_slots));
+        return BA_OK;
+    }
+
+    WinState init_state() const {
+        WinState st{};
+        st.radius = opt.initial_trust_region_radius;
+        st.decrease_factor = 2.0;
+        st.need_lin = 1;
+        st.first_lin = 1;
+        st.termination = BA_NO_CONVERGENCE;
+        return st;
+    }
+
+    // Places an image at arena offset `base` in slot `slot` (asynchronous on the engine stream;
+    // the image must stay alive until the stream has consumed it).
+    int place(const WindowImage& img, int slot, int64_t base) {
+        if (base + (int64_t)img.bytes.size() > arena_cap) return fail(BA_ERR_CAPACITY, "window does not fit the arena");
+        BA_HIP(hipMemcpyAsync(arena.p + base, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice, s));
+        hdesc[slot] = rebase(img.desc, base);
+        BA_HIP(hipMemcpyAsync(ddesc.p + slot, &hdesc[slot], sizeof(WinDesc), hipMemcpyHostToDevice, s));
+        return BA_OK;
+    }
+
+    int max_n() const { return max_window_n<Real>(threads); }
+
+    template <int NT>
+    int launch_nt(int n_work, int iters, int nmax) {
+        const size_t shm = (size_t)nmax * nmax * sizeof(Real);
+        BA_HIP(hipFuncSetAttribute((const void*)k_window_lm<Real, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)shm));
+        hipLaunchKernelGGL((k_window_lm<Real, NT>), dim3(n_work), dim3(NT), shm, s, (const WinDesc*)ddesc.p, arena.p,
+                           (const int*)dwork.p, n_work, iters, o, sizeof(Real) == sizeof(double) ? 0 : opt.refine_steps);
+        BA_HIP(hipGetLastError());
+        return BA_OK;
+    }
+
+    // Runs `iters` LM iterations of the windows in `slots` (dispatched in the given order).
+    int run(const std::vector<int>& slots, int iters) {
+        if (slots.empty()) return BA_OK;
+        int nmax = 9;
+        for (int sl : slots) nmax = std::max(nmax, 9 * hdesc[sl].K);
+        BA_HIP(hipMemcpyAsync(dwork.p, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        const int n_work = (int)slots.size();
+        if (threads == 512) return launch_nt<512>(n_work, iters, nmax);
+        return launch_nt<256>(n_work, iters, nmax);
+    }
+
+    int reset(const std::vector<int>& slots) {
+        if (slots.empty()) return BA_OK;
+        BA_HIP(hipMemcpyAsync(dwork.p, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_window_reset, dim3((unsigned)slots.size()), dim3(256), 0, s, (const WinDesc*)ddesc.p,
+                           arena.p, (const int*)dwork.p, (int)slots.size(), init_state());
+        BA_HIP(hipGetLastError());
+        return BA_OK;
+    }
+
+    // States of `slots` (synchronizes the stream).
+    int states(const std::vector<int>& slots, std::vector<WinState>* out) {
+        out->assign(slots.size(), WinState{});
+        if (slots.empty()) return BA_OK;
+        BA_HIP(hipMemcpyAsync(dwork.p, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_window_states, dim3(div_up((int64_t)slots.size(), 64)), dim3(64), 0, s,
+                           (const WinDesc*)ddesc.p, (const char*)arena.p, (const int*)dwork.p, (int)slots.size(),
+                           dstates.p);
+        BA_HIP(hipMemcpyAsync(out->data(), dstates.p, slots.size() * sizeof(WinState), hipMemcpyDeviceToHost, s));
+        BA_HIP(hipStreamSynchronize(s));
+        return BA_OK;
+    }
+
+    int download(int slot, const std::vector<int>& pt_perm, double* cams, double* pts) {
+        const WinDesc& D = hdesc[slot];
+        std::vector<double> px(4 * (size_t)D.P);
+        if (cams && D.K) BA_HIP(hipMemcpyAsync(cams, arena.p + D.o_cam_x, 72 * (size_t)D.K, hipMemcpyDeviceToHost, s));
+        if (D.P) BA_HIP(hipMemcpyAsync(px.data(), arena.p + D.o_pt_x, 32 * (size_t)D.P, hipMemcpyDeviceToHost, s));
+        BA_HIP(hipStreamSynchronize(s));
+        if (pts)
+            for (int ip = 0; ip < D.P; ++ip)
+                for (int i = 0; i < 3; ++i) pts[3 * (size_t)pt_perm[ip] + i] = px[4 * (size_t)ip + i];
+        return BA_OK;
+    }
+
+    // Times one LM iteration of the listed windows (ms), leaving their state as it was.
+    int time_iteration(const std::vector<int>& slots, float* ms) {
+        BA_TRY(reset(slots));
+        BA_TRY(run(slots, 1));  // iteration 0 + first step (warm)
+        BA_TRY(reset(slots));
+        BA_HIP(hipEventRecord(ev0, s));
+        BA_TRY(run(slots, 2));
+        BA_HIP(hipEventRecord(ev1, s));
+        BA_HIP(hipEventSynchronize(ev1));
+        BA_HIP(hipEventElapsedTime(ms, ev0, ev1));
+        return reset(slots);
+    }
+};
+
+template <typename Real>
+void fill_summary(const WinState& st, ba_summary* sm) {
+    sm->initial_cost = st.initial_cost;
+    sm->final_cost = st.x_cost;
+    sm->num_iterations = st.rows;
+    sm->num_successful_steps = st.num_successful;
+    sm->num_unsuccessful_steps = st.num_unsuccessful;
+    sm->termination_type = st.termination;
+    sm->linear_solver_used = BA_SOLVER_DENSE_SCHUR;
+    sm->total_linear_solver_iterations = st.iteration;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Batch
+struct BatchBase {
+    virtual ~BatchBase() = default;
+    virtual int run(ba_summary* summaries) = 0;
+    virtual int reset() = 0;
+    virtual int download(int w, double* cams, double* pts) = 0;
+    virtual int set_threads(int t) = 0;
+    virtual void info(int* threads, int64_t* bytes) = 0;
+};
+
+template <typename Real>
+struct Batch final : BatchBase {
+    WindowEngine<Real> eng;
+    std::vector<WindowImage> imgs;
+    std::vector<int> order;              // window slots in dispatch order (largest work first)
+    std::vector<int> small;              // windows solved in the batched kernel
+    std::vector<int> large;              // windows solved by the single-problem solver
+    std::vector<std::unique_ptr<SolverBase>> large_solvers;
+    std::vector<int> large_index;        // window -> index in large_solvers or -1
+    std::vector<ba_summary> large_sum;
+    ba_options opt{};
+
+    int create(const ba_problem_view* views, int count, const ba_options& o) {
+        opt = o;
+        BA_TRY(use_device(o.device));
+        imgs.resize(count);
+        large_index.assign(count, -1);
+        int64_t total = 0;
+        for (int w = 0; w < count; ++w) {
+            BA_TRY(validate_view(&views[w], true));
+            if (9 * views[w].num_cameras > max_window_n<Real>(256)) {
+                large_index[w] = (int)large_solvers.size();
+                ba_options lo = o;
+                lo.linear_solver = BA_SOLVER_DENSE_SCHUR;
+                std::unique_ptr<SolverBase> sv;
+                BA_TRY(make_solver(&views[w], lo, &sv));
+                large_solvers.push_back(std::move(sv));
+                large.push_back(w);
+                continue;
+            }
+            BA_TRY(build_window_image<Real>(&views[w], o, &imgs[w]));
+            total += (int64_t)imgs[w].bytes.size();
+            small.push_back(w);
+        }
+        large_sum.assign(large_solvers.size(), ba_summary{});
+        BA_TRY(eng.init(o, total, count));
+        int64_t base = 0;
+        for (int w : small) {
+            BA_TRY(eng.place(imgs[w], w, base));
+            base += (int64_t)imgs[w].bytes.size();
+        }
+        order = small;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return imgs[a].cost > imgs[b].cost; });
+        BA_HIP(hipStreamSynchronize(eng.s));
+        // the host images are only needed for their permutations from here on
+        for (int w : small) std::vector<char>().swap(imgs[w].bytes);
+        return autotune();
+    }
+
+    int autotune() {
+        if (order.empty()) return BA_OK;
+        int nmax = 9;
+        for (int w : order) nmax = std::max(nmax, 9 * imgs[w].K);
+        float best = 1e30f;
+        int best_t = 256;
+        for (int t : {256, 512}) {
+            if (nmax > max_window_n<Real>(t)) continue;
+            eng.threads = t;
+            float ms = 0.f;
+            BA_TRY(eng.time_iteration(order, &ms));
+            if (ms < best) {
+                best = ms;
+                best_t = t;
+            }
+        }
+        eng.threads = best_t;
+        return BA_OK;
+    }
+
+    int set_threads(int t) override {
+        if (t == 0) return autotune();
+        if (t != 256 && t != 512) return fail(BA_ERR_INVALID_ARGUMENT, "threads must be 256 or 512");
+        int nmax = 9;
+        for (int w : order) nmax = std::max(nmax, 9 * imgs[w].K);
+        if (nmax > max_window_n<Real>(t)) return fail(BA_ERR_CAPACITY, "a window does not fit %d threads' LDS", t);
+        eng.threads = t;
+        return BA_OK;
+    }
+
+    void info(int* threads, int64_t* bytes) override {
+        if (threads) *threads = eng.threads;
+        if (bytes) *bytes = eng.arena_cap;
+    }
+
+    int run(ba_summary* summaries) override {
+        BA_TRY(use_device(opt.device));
+        BA_TRY(eng.run(order, opt.max_num_iterations + 1));
+        for (size_t i = 0; i < large.size(); ++i) BA_TRY(large_solvers[i]->run(&large_sum[i]));
+        std::vector<WinState> st;
+        BA_TRY(eng.states(order, &st));
+        if (summaries) {
+            for (size_t i = 0; i < order.size(); ++i) {
+                ba_summary& sm = summaries[order[i]];
+                ba_iteration* tr = sm.trace;
+                const int cap = sm.trace_capacity;
+                fill_summary<Real>(st[i], &sm);
+                sm.trace = tr;
+                sm.trace_capacity = cap;
+            }
+            for (size_t i = 0; i < large.size(); ++i) {
+                ba_summary& sm = summaries[large[i]];
+                ba_iteration* tr = sm.trace;
+                const int cap = sm.trace_capacity;
+                sm = large_sum[i];
+                sm.trace = tr;
+                sm.trace_capacity = cap;
+            }
+        }
+        return BA_OK;
+    }
+
+    int reset() override {
+        BA_TRY(eng.reset(order));
+        for (auto& sv : large_solvers) BA_TRY(sv->reset());
+        BA_HIP(hipStreamSynchronize(eng.s));
+        return BA_OK;
+    }
+
+    int download(int w, double* cams, double* pts) override {
+        if (w < 0 || w >= (int)imgs.size()) return fail(BA_ERR_INVALID_ARGUMENT, "window %d out of range", w);
+        if (large_index[w] >= 0) return large_solvers[large_index[w]]->download(cams, pts);
+        return eng.download(w, imgs[w].pt_perm, cams, pts);
+    }
+};
+
+int make_batch(const ba_problem_view* views, int count, const ba_options& o, std::unique_ptr<BatchBase>* out) {
+    if (o.precision == BA_PRECISION_FP64) {
+        auto b = std::make_unique<Batch<double>>();
+        BA_TRY(b->create(views, count, o));
+        *out = std::move(b);
+    } else {
+        auto b = std::make_unique<Batch<float>>();
+        BA_TRY(b->create(views, count, o));
+        *out = std::move(b);
+    }
+    return BA_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Serving
+struct ServerBase {
+    virtual ~ServerBase() = default;
+    virtual int submit(const ba_problem_view* v, int64_t tag) = 0;
+    virtual int step(int* resident, int* queued) = 0;
+    virtual int poll(ba_server_result* out, int cap, int* n) = 0;
+    virtual double clock() const = 0;
+};
+
+struct Request {
+    int64_t tag = 0;
+    double submit = 0.0, admit = 0.0;
+    int nc = 0, np = 0;
+    std::vector<int> ci, pi;
+    std::vector<double> obs, cams, pts;
+    ba_problem_view view() {
+        ba_problem_view v{};
+        v.num_cameras = nc;
+        v.num_points = np;
+        v.num_observations = (int)ci.size();
+        v.camera_block_size = 9;
+        v.camera_index = ci.data();
+        v.point_index = pi.data();
+        v.observations = obs.data();
+        v.cameras = cams.data();
+        v.points = pts.data();
+        return v;
+    }
+};
+
+struct Done {
+    ba_server_result r{};
+    std::vector<double> cams, pts;
+};
+
+template <typename Real>
+struct Server final : ServerBase {
+    ba_server_config cfg{};
+    WindowEngine<Real> eng;
+    std::chrono::steady_clock::time_point t0;
+    std::map<int64_t, int64_t> free_ranges;  // offset -> size
+    std::vector<int> free_slots;
+    struct Resident {
+        int slot = -1;
+        int64_t off = 0, size = 0;
+        std::unique_ptr<Request> req;
+        WindowImage img;
+    };
+    std::vector<Resident> res;  // by slot
+    std::vector<int> active;    // resident slots
+    std::deque<std::unique_ptr<Request>> queue;
+    std::deque<Done> done;
+    std::vector<Done> polled;   // keeps the last poll's parameter arrays alive
+    // requests too large for a window slot: solved by the single-problem solver on a worker thread
+    std::thread worker;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<std::unique_ptr<Request>> big_queue;
+    bool stopping = false;
+    int big_inflight = 0;
+
+    ~Server() override {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stopping = true;
+        }
+        cv.notify_all();
+        if (worker.joinable()) worker.join();
+    }
+
+    double clock() const override {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    int create(const ba_server_config& c) {
+        cfg = c;
+        if (cfg.max_windows < 1) return fail(BA_ERR_INVALID_ARGUMENT, "max_windows < 1");
+        BA_TRY(use_device(cfg.options.device));
+        if (cfg.iterations_per_step < 1) cfg.iterations_per_step = 1;
+        BA_TRY(eng.init(cfg.options, cfg.arena_bytes, cfg.max_windows));
+        free_ranges[0] = eng.arena_cap;
+        res.resize(cfg.max_windows);
+        for (int i = cfg.max_windows - 1; i >= 0; --i) free_slots.push_back(i);
+        t0 = std::chrono::steady_clock::now();
+        worker = std::thread([this] { big_loop(); });
+        return BA_OK;
+    }
+
+    bool alloc(int64_t size, int64_t* off) {
+        for (auto it = free_ranges.begin(); it != free_ranges.end(); ++it) {
+            if (it->second >= size) {
+                *off = it->first;
+                const int64_t rest = it->second - size;
+                const int64_t o2 = it->first + size;
+                free_ranges.erase(it);
+                if (rest > 0) free_ranges[o2] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+
+    void release(int64_t off, int64_t size) {
+        auto it = free_ranges.emplace(off, size).first;
+        auto next = std::next(it);
+        if (next != free_ranges.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_ranges.erase(next);
+        }
+        if (it != free_ranges.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_ranges.erase(it);
+            }
+        }
+    }
+
+    int submit(const ba_problem_view* v, int64_t tag) override {
+        BA_TRY(validate_view(v, true));
+        auto r = std::make_unique<Request>();
+        r->tag = tag;
+        r->submit = clock();
+        r->nc = v->num_cameras;
+        r->np = v->num_points;
+        r->ci.assign(v->camera_index, v->camera_index + v->num_observations);
+        r->pi.assign(v->point_index, v->point_index + v->num_observations);
+        r->obs.assign(v->observations, v->observations + 2 * (size_t)v->num_observations);
+        r->cams.assign(v->cameras, v->cameras + 9 * (size_t)v->num_cameras);
+        r->pts.assign(v->points, v->points + 3 * (size_t)v->num_points);
+        if (9 * r->nc > max_window_n<Real>(256)) {
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                big_queue.push_back(std::move(r));
+                big_inflight++;
+            }
+            cv.notify_one();
+            return BA_OK;
+        }
+        queue.push_back(std::move(r));
+        return BA_OK;
+    }
+
+    void big_loop() {
+        for (;;) {
+            std::unique_ptr<Request> r;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return stopping || !big_queue.empty(); });
+                if (stopping) return;
+                r = std::move(big_queue.front());
+                big_queue.pop_front();
+            }
+            r->admit = clock();
+            ba_problem_view v = r->view();
+            ba_options o = cfg.options;
+            o.linear_solver = BA_SOLVER_AUTO;
+            std::unique_ptr<SolverBase> sv;
+            ba_summary sm{};
+            Done d;
+            d.r.tag = r->tag;
+            d.r.submit_time_s = r->submit;
+            d.r.admit_time_s = r->admit;
+            int st = make_solver(&v, o, &sv);
+            if (st == BA_OK) st = sv->run(&sm);
+            if (st == BA_OK) st = sv->download(v.cameras, v.points);
+            d.r.initial_cost = sm.initial_cost;
+            d.r.final_cost = sm.final_cost;
+            d.r.iterations = sm.num_iterations;
+            d.r.termination_type = st == BA_OK ? sm.termination_type : BA_FAILURE;
+            d.cams = std::move(r->cams);
+            d.pts = std::move(r->pts);
+            d.r.finish_time_s = clock();
+            std::lock_guard<std::mutex> lk(mu);
+            done.push_back(std::move(d));
+            big_inflight--;
+        }
+    }
+
+    int admit() {
+        std::vector<int> placed;
+        while (!queue.empty() && !free_slots.empty()) {
+            Request& rq = *queue.front();
+            WindowImage img;
+            ba_problem_view v = rq.view();
+            BA_TRY(build_window_image<Real>(&v, cfg.options, &img));
+            int64_t off = 0;
+            if (!alloc((int64_t)img.bytes.size(), &off)) {
+                if ((int64_t)img.bytes.size() > eng.arena_cap)
+                    return fail(BA_ERR_CAPACITY, "request %lld needs %zu bytes, arena holds %lld", (long long)rq.tag,
+                                img.bytes.size(), (long long)eng.arena_cap);
+                break;  // wait for memory to free up (FIFO admission)
+            }
+            const int slot = free_slots.back();
+            free_slots.pop_back();
+            Resident& r = res[slot];
+            r.slot = slot;
+            r.off = off;
+            r.size = (int64_t)img.bytes.size();
+            r.req = std::move(queue.front());
+            queue.pop_front();
+            r.req->admit = clock();
+            r.img = std::move(img);
+            BA_TRY(eng.place(r.img, slot, off));
+            active.push_back(slot);
+            placed.push_back(slot);
+        }
+        if (!placed.empty()) BA_HIP(hipStreamSynchronize(eng.s));
+        for (int sl : placed) std::vector<char>().swap(res[sl].img.bytes);
+        return BA_OK;
+    }
+
+    int step(int* resident, int* queued) override {
+        BA_TRY(use_device(cfg.options.device));
+        BA_TRY(admit());
+        if (!active.empty()) {
+            std::vector<int> order = active;
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return res[a].img.cost > res[b].img.cost; });
+            BA_TRY(eng.run(order, cfg.iterations_per_step));
+            std::vector<WinState> st;
+            BA_TRY(eng.states(order, &st));
+            const double now = clock();
+            for (size_t i = 0; i < order.size(); ++i) {
+                if (!st[i].done) continue;
+                const int sl = order[i];
+                Resident& r = res[sl];
+                Done d;
+                d.r.tag = r.req->tag;
+                d.r.submit_time_s = r.req->submit;
+                d.r.admit_time_s = r.req->admit;
+                d.r.finish_time_s = now;
+                d.r.initial_cost = st[i].initial_cost;
+                d.r.final_cost = st[i].x_cost;
+                d.r.iterations = st[i].rows;
+                d.r.termination_type = st[i].termination;
+                d.cams = std::move(r.req->cams);
+                d.pts = std::move(r.req->pts);
+                BA_TRY(eng.download(sl, r.img.pt_perm, d.cams.data(), d.pts.data()));
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    done.push_back(std::move(d));
+                }
+                release(r.off, r.size);
+                r = Resident{};
+                free_slots.push_back(sl);
+                active.erase(std::find(active.begin(), active.end(), sl));
+            }
+        }
+        if (resident) *resident = (int)active.size();
+        if (queued) {
+            std::lock_guard<std::mutex> lk(mu);
+            *queued = (int)queue.size() + big_inflight;
+        }
+        return BA_OK;
+    }
+
+    int poll(ba_server_result* out, int cap, int* n) override {
+        std::lock_guard<std::mutex> lk(mu);
+        polled.clear();
+        int k = 0;
+        while (k < cap && !done.empty()) {
+            polled.push_back(std::move(done.front()));
+            done.pop_front();
+            ++k;
+        }
+        for (int i = 0; i < k; ++i) {
+            Done& d = polled[i];
+            out[i] = d.r;
+            out[i].num_cameras = (int)(d.cams.size() / 9);
+            out[i].num_points = (int)(d.pts.size() / 3);
+            out[i].cameras = d.cams.data();
+            out[i].points = d.pts.data();
+        }
+        if (n) *n = k;
+        return BA_OK;
+    }
+};
+
+int make_server(const ba_server_config& c, std::unique_ptr<ServerBase>* out) {
+    if (c.options.precision == BA_PRECISION_FP64) {
+        auto s = std::make_unique<Server<double>>();
+        BA_TRY(s->create(c));
+        *out = std::move(s);
+    } else {
+        auto s = std::make_unique<Server<float>>();
+        BA_TRY(s->create(c));
+        *out = std::move(s);
+    }
+    return BA_OK;
+}
+
+}  // namespace mba
+
+struct ba_batch {
+    std::unique_ptr<mba::BatchBase> impl;
+    std::vector<ba_problem_view> views;
+};
+
+struct ba_server {
+    std::unique_ptr<mba::ServerBase> impl;
+};
+
+using namespace mba;
+
+extern "C" {
+
+int ba_batch_create(const ba_problem_view* windows, int32_t count, const ba_options* options, ba_batch** out) {
+    clear_error();
+    if (!out || (count > 0 && !windows) || count < 0) return fail(BA_ERR_INVALID_ARGUMENT, "invalid arguments");
+    ba_options o;
+    if (options)
+        o = *options;
+    else
+        ba_default_options(&o);
+    auto h = std::make_unique<ba_batch>();
+    BA_TRY(make_batch(windows, count, o, &h->impl));
+    h->views.assign(windows, windows + count);
+    *out = h.release();
+    return BA_OK;
+}
+
+int ba_batch_run(ba_batch* b, ba_summary* summaries) {
+    clear_error();
+    if (!b) return fail(BA_ERR_INVALID_ARGUMENT, "NULL batch");
+    return b->impl->run(summaries);
+}
+
+int ba_batch_reset(ba_batch* b) {
+    clear_error();
+    if (!b) return fail(BA_ERR_INVALID_ARGUMENT, "NULL batch");
+    return b->impl->reset();
+}
+
+int ba_batch_download(ba_batch* b, int32_t w, double* cameras, double* points) {
+    clear_error();
+    if (!b) return fail(BA_ERR_INVALID_ARGUMENT, "NULL batch");
+    return b->impl->download(w, cameras, points);
+}
+
+int ba_batch_destroy(ba_batch* b) {
+    delete b;
+    return BA_OK;
+}
+
+int ba_batch_set_block(ba_batch* b, int32_t threads) {
+    clear_error();
+    if (!b) return fail(BA_ERR_INVALID_ARGUMENT, "NULL batch");
+    return b->impl->set_threads(threads);
+}
+
+int ba_batch_info(ba_batch* b, int32_t* threads, int64_t* device_bytes) {
+    clear_error();
+    if (!b) return fail(BA_ERR_INVALID_ARGUMENT, "NULL batch");
+    int t = 0;
+    int64_t bytes = 0;
+    b->impl->info(&t, &bytes);
+    if (threads) *threads = t;
+    if (device_bytes) *device_bytes = bytes;
+    return BA_OK;
+}
+
+int ba_batch_solve(const ba_problem_view* windows, int32_t count, const ba_options* options, ba_summary* summaries) {
+    ba_batch* b = nullptr;
+    BA_TRY(ba_batch_create(windows, count, options, &b));
+    std::unique_ptr<ba_batch> guard(b);
+    BA_TRY(b->impl->run(summaries));
+    for (int w = 0; w < count; ++w) BA_TRY(b->impl->download(w, windows[w].cameras, windows[w].points));
+    return BA_OK;
+}
+
+int ba_server_create(const ba_server_config* config, ba_server** out) {
+    clear_error();
+    if (!config || !out) return fail(BA_ERR_INVALID_ARGUMENT, "NULL argument");
+    auto h = std::make_unique<ba_server>();
+    BA_TRY(make_server(*config, &h->impl));
+    *out = h.release();
+    return BA_OK;
+}
+
+int ba_server_submit(ba_server* s, const ba_problem_view* window, int64_t tag) {
+    clear_error();
+    if (!s) return fail(BA_ERR_INVALID_ARGUMENT, "NULL server");
+    return s->impl->submit(window, tag);
+}
+
+int ba_server_step(ba_server* s, int32_t* resident, int32_t* queued) {
+    clear_error();
+    if (!s) return fail(BA_ERR_INVALID_ARGUMENT, "NULL server");
+    int r = 0, q = 0;
+    BA_TRY(s->impl->step(&r, &q));
+    if (resident) *resident = r;
+    if (queued) *queued = q;
+    return BA_OK;
+}
+
+int ba_server_poll(ba_server* s, ba_server_result* out, int32_t capacity, int32_t* count) {
+    clear_error();
+    if (!s || (!out && capacity > 0)) return fail(BA_ERR_INVALID_ARGUMENT, "NULL argument");
+    int n = 0;
+    BA_TRY(s->impl->poll(out, capacity, &n));
+    if (count) *count = n;
+    return BA_OK;
+}
+
+double ba_server_clock(ba_server* s) { return s ? s->impl->clock() : 0.0; }
+
+int ba_server_destroy(ba_server* s) {
+    delete s;
+    return BA_OK;
+}
+
+int ba_server_simulate(const ba_server_config* config, const ba_problem_view* pool, int32_t pool_count, double rate_hz,
+                       int32_t count, uint64_t seed, ba_server_result* results) {
+    clear_error();
+    if (!config || !pool || pool_count < 1 || count < 0 || !(rate_hz > 0.0) || (count > 0 && !results))
+        return fail(BA_ERR_INVALID_ARGUMENT, "invalid arguments");
+    ba_server* sv = nullptr;
+    BA_TRY(ba_server_create(config, &sv));
+    std::unique_ptr<ba_server> guard(sv);
+    std::mt19937_64 rng(seed);
+    std::exponential_distribution<double> gap(rate_hz);
+    std::vector<double> arrival(count);
+    double t = 0.0;
+    for (int i = 0; i < count; ++i) {
+        t += gap(rng);
+        arrival[i] = t;
+    }
+    const double start = sv->impl->clock();
+    int next = 0, finished = 0;
+    std::vector<ba_server_result> buf(64);
+    while (finished < count) {
+        const double now = sv->impl->clock() - start;
+        while (next < count && arrival[next] <= now) {
+            BA_TRY(sv->impl->submit(&pool[next % pool_count], next));
+            ++next;
+        }
+        int resident = 0, queued = 0;
+        BA_TRY(sv->impl->step(&resident, &queued));
+        int n = 0;
+        do {
+            BA_TRY(sv->impl->poll(buf.data(), (int)buf.size(), &n));
+            for (int i = 0; i < n; ++i) {
+                ba_server_result r = buf[i];
+                const int64_t id = r.tag;
+                // latency is measured from the scheduled arrival time
+                r.submit_time_s = arrival[id];
+                r.admit_time_s -= start;
+                r.finish_time_s -= start;
+                r.cameras = nullptr;
+                r.points = nullptr;
+                results[id] = r;
+                ++finished;
+            }
+        } while (n == (int)buf.size());
+        if (resident == 0) {
+            // idle (or only requests on the single-problem solver): wait for the next arrival
+            const double wait = next < count ? arrival[next] - (sv->impl->clock() - start) : 1e-4;
+            if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 1e-3)));
+        }
+    }
+    return BA_OK;
+}
+
+}  // extern "C"

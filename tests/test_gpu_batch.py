@@ -1,0 +1,164 @@
+"""Batched local BA (one workgroup per window) and the serving loop, vs the oracle and vs themselves.
+
+Bars: FP64 windows follow the oracle's dense-Schur LM (final cost within 1e-9 relative, same
+termination and step counts on a fixed schedule); mixed-precision windows end within 1e-5 of the
+FP64 oracle; a window's result is bitwise the same alone, in any mix and through the server.
+"""
+import numpy as np
+import pytest
+
+import slamplay_amd as S
+
+pytestmark = pytest.mark.gpu
+
+FIXED = dict(function_tolerance=0.0, gradient_tolerance=0.0, parameter_tolerance=0.0)
+
+
+def window(seed, nc, npts):
+    """A local-BA window: random-walk keyframes, tracks of 2..nc frames."""
+    return S.synthetic(4, seed=seed, num_cameras=nc, num_points=npts, max_track=nc)
+
+
+def _oracle(oracle_mod, p, **kw):
+    o = oracle_mod.default_options(linear_solver=1, **kw)
+    return oracle_mod.solve(p.num_cameras(), p.num_points(), p.camera_index(), p.point_index(), p.observations(),
+                            p.mutable_cameras(), p.mutable_points(), o)
+
+
+WINDOWS = [(101, 5, 200), (102, 9, 700), (103, 14, 400), (104, 12, 1000), (105, 7, 300)]
+
+
+def test_fp64_windows_match_oracle(oracle_mod):
+    probs = [window(*w) for w in WINDOWS]
+    b = S.Batch(probs, precision=0, max_num_iterations=10, **FIXED)
+    sums = b.run()
+    for w, p in enumerate(probs):
+        ref = _oracle(oracle_mod, p, max_num_iterations=10, **FIXED)
+        s = sums[w]
+        assert s["num_iterations"] == len(ref["trace"])
+        assert s["num_successful_steps"] == sum(r["step_is_successful"] for r in ref["trace"][1:])
+        assert abs(s["initial_cost"] - ref["trace"][0]["cost"]) <= 1e-12 * ref["trace"][0]["cost"]
+        assert abs(s["final_cost"] - ref["final_cost"]) <= 1e-9 * ref["final_cost"], (w, s, ref["final_cost"])
+        cams, pts = b.download(w)
+        np.testing.assert_allclose(pts, ref["pts"], rtol=0, atol=1e-7 * np.abs(ref["pts"]).max())
+        np.testing.assert_allclose(cams, ref["cams"], rtol=0, atol=1e-7 * np.abs(ref["cams"]).max())
+
+
+def test_fp64_default_tolerances_terminate_like_oracle(oracle_mod):
+    probs = [window(*w) for w in WINDOWS[:3]]
+    sums = S.Batch(probs, precision=0).run()
+    for p, s in zip(probs, sums):
+        ref = _oracle(oracle_mod, p)
+        assert s["termination"] == ref["termination"]
+        assert s["num_iterations"] == len(ref["trace"])
+        assert abs(s["final_cost"] - ref["final_cost"]) <= 1e-9 * ref["final_cost"]
+
+
+def test_mixed_windows_final_cost(oracle_mod):
+    probs = [window(*w) for w in WINDOWS] + [window(106, 20, 2500)]
+    sums = S.Batch(probs, precision=1, max_num_iterations=10, **FIXED).run()
+    for p, s in zip(probs, sums):
+        ref = _oracle(oracle_mod, p, max_num_iterations=10, **FIXED)
+        assert abs(s["final_cost"] - ref["final_cost"]) <= 1e-5 * ref["final_cost"], (s, ref["final_cost"])
+
+
+def test_matches_single_problem_solver():
+    """The window kernel and the single-problem dense-Schur solver take the same LM steps."""
+    probs = [window(*w) for w in WINDOWS[:3]]
+    sums = S.Batch(probs, precision=0, max_num_iterations=8, **FIXED).run()
+    for p, s in zip(probs, sums):
+        q = S.BALProblem.from_arrays(p.camera_index(), p.point_index(), p.observations(), p.mutable_cameras(),
+                                     p.mutable_points())
+        out = S.SolveBA(q, precision=0, linear_solver=1, max_num_iterations=8, **FIXED)
+        assert s["num_iterations"] == out["num_iterations"]
+        assert abs(s["final_cost"] - out["final_cost"]) <= 1e-10 * out["final_cost"]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_alone_and_in_mix_bitwise_identical(precision):
+    target = window(201, 10, 600)
+    others = [window(202 + i, 5 + (3 * i) % 9, 200 + 150 * i) for i in range(7)]
+    kw = dict(precision=precision, max_num_iterations=10)
+    alone = S.Batch([target], **kw)
+    s_alone = alone.run()[0]
+    c_alone, p_alone = alone.download(0)
+    mix = S.Batch(others[:3] + [target] + others[3:], **kw)
+    s_mix = mix.run()[3]
+    c_mix, p_mix = mix.download(3)
+    assert s_alone == s_mix
+    np.testing.assert_array_equal(c_alone, c_mix)
+    np.testing.assert_array_equal(p_alone, p_mix)
+    # both workgroup sizes give the same bits too (the reduction order is per wave and per block)
+    for t in (256, 512):
+        mix.set_block(t)
+        mix.reset()
+        assert mix.run()[3] == s_alone
+        np.testing.assert_array_equal(mix.download(3)[1], p_alone)
+
+
+def test_reset_rerun_and_empty_window():
+    probs = [window(301, 8, 400), S.BALProblem.from_arrays(np.zeros(0, np.int32), np.zeros(0, np.int32),
+                                                           np.zeros(0), np.zeros(9 * 2), np.zeros(3 * 4))]
+    b = S.Batch(probs, precision=1, max_num_iterations=6)
+    a = b.run()
+    ca, pa = b.download(0)
+    b.reset()
+    a2 = b.run()
+    assert a == a2
+    np.testing.assert_array_equal(ca, b.download(0)[0])
+    assert a[1]["termination"] == 0 and a[1]["final_cost"] == 0.0
+    info = b.info()
+    assert info["threads"] in (256, 512) and info["device_bytes"] > 0
+
+
+def test_large_window_uses_single_problem_solver(oracle_mod):
+    """A window whose reduced system does not fit the workgroup's LDS (30 cameras, mixed) is solved
+    by the single-problem solver and reported like the others."""
+    probs = [window(401, 30, 800), window(402, 6, 300)]
+    sums = S.Batch(probs, precision=0, max_num_iterations=8, **FIXED).run()
+    for p, s in zip(probs, sums):
+        ref = _oracle(oracle_mod, p, max_num_iterations=8, **FIXED)
+        assert abs(s["final_cost"] - ref["final_cost"]) <= 1e-9 * ref["final_cost"]
+
+
+def test_invalid_block_size():
+    b = S.Batch([window(501, 6, 200)], precision=1)
+    with pytest.raises(S.BAError):
+        b.set_block(1024)
+
+
+@pytest.mark.parametrize("arena", [1 << 30, 12 << 20])
+def test_server_results_equal_alone(arena):
+    """Continuous batching: requests admitted while others run (a small arena forces waiting for
+    memory) give bitwise the results of solving each window alone."""
+    probs = [window(600 + i, 5 + (7 * i) % 16, 200 + 97 * i) for i in range(12)]
+    opts = S.default_options(precision=1, max_num_iterations=10)
+    srv = S.Server(S.server_config(arena_bytes=arena, max_windows=5, iterations_per_step=1, options=opts))
+    for i, p in enumerate(probs[:6]):
+        srv.submit(p, i)
+    results = []
+    for _ in range(3):
+        srv.step()
+        results += srv.poll()
+    for i, p in enumerate(probs[6:]):
+        srv.submit(p, 6 + i)
+    results += srv.drain()
+    assert sorted(r["tag"] for r in results) == list(range(12))
+    for r in results:
+        alone = S.Batch([probs[r["tag"]]], options=opts)
+        s = alone.run()[0]
+        cams, pts = alone.download(0)
+        assert r["final_cost"] == s["final_cost"] and r["iterations"] == s["num_iterations"]
+        np.testing.assert_array_equal(r["cameras"], cams)
+        np.testing.assert_array_equal(r["points"], pts)
+        assert r["finish_time_s"] >= r["admit_time_s"] >= r["submit_time_s"]
+
+
+def test_simulate_poisson():
+    pool = [window(700 + i, 5 + i % 12, 200 + 50 * i) for i in range(16)]
+    res = S.simulate(pool, rate_hz=2000.0, count=64, seed=3, max_windows=64, precision=1, max_num_iterations=10)
+    assert len(res) == 64
+    lat = np.array([r["latency_s"] for r in res])
+    assert (lat > 0).all() and np.isfinite(lat).all()
+    assert all(r["termination"] in (0, 1) for r in res)
+    assert all(r["final_cost"] < r["initial_cost"] for r in res)

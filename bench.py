@@ -7,10 +7,18 @@ observations, Snavely cameras, Huber(1.0), implicit-Schur block-Jacobi PCG (max 
 tol 1e-6), all termination tolerances 0 so exactly 30 LM iterations run.
 
 value = mixed-precision LM iterations/s; the plain FP64 GPU path is timed the same way and reported
-beside it, with the final-cost difference between the two.  The roofline entry is the dominant
-kernel's algorithmic bytes per launch over its average duration, timed with HIP events on the
-solver's stream; cpu_baseline is the FP64 CPU oracle (one thread) on a bounded sample of the same
-problem.
+beside it, with the final-cost difference between the two.
+
+The metric's second half, batched local-BA latency, is measured beside it: 512 windows of uneven
+size (configuration 4: 5-20 keyframes, 200-3,000 points log-uniform, tracks of 2 frames up to the
+window length), 10 LM iterations each in mixed precision, solved in one launch (every window's
+latency is the launch's); and the serving loop (configuration 5): Poisson arrivals of such windows
+plus 2% of 100-camera/20,000-point problems, per-iteration continuous batching, p50/p99 latency and
+solves/s at several arrival rates.
+
+The roofline entry is the dominant kernel's algorithmic bytes per launch over its average duration,
+timed with HIP events on the solver's stream; cpu_baseline is the FP64 CPU oracle (one thread) on a
+bounded sample of the same problem.
 """
 import argparse
 import json
@@ -73,20 +81,85 @@ def cpu_baseline(S, p):
                       "of the full 1.2M-observation problem, %.1f s" % dt}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py: this engine runs on one GPU; a multi-GPU figure is not measured "
-              "(independent windows shard by window, a large BA by points -- see DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-    import slamplay_amd as S
+WINDOW_ITERS = 10
+
+
+def window_options(S, precision):
+    return S.default_options(precision=precision, linear_solver=1, max_num_iterations=WINDOW_ITERS,
+                             function_tolerance=0.0, gradient_tolerance=0.0, parameter_tolerance=0.0)
+
+
+def bench_batch(S, seed, steps, warmup):
+    """512 windows, one launch: batch latency (= every window's latency), and the same windows solved
+    one problem per launch sequence (single-problem dense solver, FP64 and mixed) on a sample."""
+    import numpy as np
+    wins = S.local_ba_windows(512, seed=seed)
+    obs = int(sum(p.num_observations() for p in wins))
+    b = S.Batch(wins, window_options(S, 1))
+    info = b.info()
+    for _ in range(warmup):
+        b.reset()
+        b.run()
+    times = []
+    for _ in range(steps):
+        b.reset()
+        t0 = time.perf_counter()
+        sums = b.run()
+        times.append(time.perf_counter() - t0)
+    lat_ms = np.array(times) * 1e3
+    iters = [s["num_iterations"] - 1 for s in sums]
+    b.close()
+    # one problem per solve on a sample of windows (the plain path a caller would loop over)
+    sample = wins[:: max(1, len(wins) // 32)]
+    serial = {}
+    for prec in (0, 1):
+        t = 0.0
+        for p in sample:
+            sv = S.Solver(p, window_options(S, prec))
+            sv.run()
+            sv.reset()
+            t0 = time.perf_counter()
+            sv.run()
+            t += time.perf_counter() - t0
+            sv.close()
+        serial["fp64" if prec == 0 else "mixed"] = t / len(sample) * len(wins) * 1e3
+    return {"windows": len(wins), "observations": obs, "lm_iterations_each": WINDOW_ITERS,
+            "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+            "max_ms": float(lat_ms.max()), "threads_per_window": info["threads"],
+            "windows_per_s": len(wins) / (float(np.median(lat_ms)) / 1e3),
+            "lm_iters_done": int(sum(iters)),
+            "serial_single_problem_ms": {k: round(v, 2) for k, v in serial.items()},
+            "note": "latency of a window = latency of the one launch that solves all 512; serial = the "
+                    "single-problem solver looped over the windows (sampled 1 in %d)" % max(1, len(wins) // 32)}
+
+
+def bench_serving(S, seed, rates, count):
+    import numpy as np
+    pool = S.local_ba_windows(200, seed=seed + 7)
+    big = [S.synthetic(2, seed=seed + 100 + i, num_cameras=100, num_points=20000) for i in range(2)]
+    # 2% large problems: every 50th request
+    mix = []
+    for i in range(250):
+        mix.append(big[(i // 50) % 2] if i % 50 == 49 else pool[i % len(pool)])
+    out = {}
+    for rate in rates:
+        cfg = S.server_config(arena_bytes=16 << 30, max_windows=1024, iterations_per_step=1,
+                              options=window_options(S, 1))
+        t0 = time.perf_counter()
+        res = S.simulate(mix, rate_hz=rate, count=count, seed=seed, config=cfg)
+        wall = time.perf_counter() - t0
+        lat = np.array([r["latency_s"] for r in res]) * 1e3
+        small = np.array([r["latency_s"] for i, r in enumerate(res) if i % len(mix) % 50 != 49]) * 1e3
+        out[str(int(rate))] = {"requests": count, "p50_ms": float(np.percentile(lat, 50)),
+                               "p99_ms": float(np.percentile(lat, 99)),
+                               "p99_ms_windows_only": float(np.percentile(small, 99)),
+                               "solves_per_s": count / wall}
+        log("bench: serving %d req/s: p50 %.2f ms p99 %.2f ms, %.0f solves/s" %
+            (rate, out[str(int(rate))]["p50_ms"], out[str(int(rate))]["p99_ms"], count / wall))
+    return out
+
+
+def bench_c3(S, args):
     log("bench: generating configuration 3 (seed %d)" % args.seed)
     p = S.synthetic(3, seed=args.seed)
     nc, npt, no = p.num_cameras(), p.num_points(), p.num_observations()
@@ -96,7 +169,7 @@ def main():
     mixed_ips = it_mixed / t_mixed
     log("bench: mixed %.1f LM iters/s, final cost %.9e" % (mixed_ips, out_mixed["final_cost"]))
     kt = profile_kernels(sm)
-    res = {}
+    res = {"value": mixed_ips, "ms_per_step": t_mixed / args.steps * 1e3}
     if not args.no_fp64:
         log("bench: FP64")
         s64, t64, it64, out64 = time_solver(S, p, 0, args.steps, args.warmup)
@@ -117,37 +190,62 @@ def main():
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
                     "avg_launch_us": avg_s * 1e6, "launches_per_solve": v["launches"]}
-    kernel_share = {k: round(v["total_ms"], 3) for k, v in sorted(kt.items(), key=lambda kv: -kv[1]["total_ms"])}
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("bench: CPU oracle baseline (bounded sample)")
-        cpu = cpu_baseline(S, p)
+    res["roofline"] = roofline
+    res["kernel_ms_per_solve"] = {k: round(v["total_ms"], 3)
+                                  for k, v in sorted(kt.items(), key=lambda kv: -kv[1]["total_ms"])}
+    res["config"] = {"workload": "BA config 3: %d cameras, %d points, %d observations, mixed precision, "
+                                 "implicit-Schur block-Jacobi PCG (max 50, tol 1e-6), %d LM iterations" %
+                                 (nc, npt, no, LM_ITERS),
+                     "cameras": nc, "points": npt, "observations": no, "lm_iterations": LM_ITERS}
+    res.update(final_cost=out_mixed["final_cost"], initial_cost=out_mixed["initial_cost"],
+               pcg_iterations=out_mixed["linear_solver_iterations"])
+    return res, sm, p
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 1.2M-observation solve (profiling runs)")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; a multi-GPU figure is not measured "
+              "(independent windows shard by window, a large BA by points -- see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import slamplay_amd as S
     line = {
         "metric": "BA LM iters/s at 1.2M obs (mixed vs FP64) + p99 batched local-BA latency",
-        "value": mixed_ips,
-        "unit": "LM iters/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": t_mixed / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
+        "value": None, "unit": "LM iters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 Jacobian/Hessian/PCG matvec + f64 block inverses and accept-test cost",
         "data": "synthetic (seeded generator, configuration 3)",
-        "config": {"workload": "BA config 3: %d cameras, %d points, %d observations, mixed precision, "
-                               "implicit-Schur block-Jacobi PCG (max 50, tol 1e-6), %d LM iterations" %
-                               (nc, npt, no, LM_ITERS),
-                   "cameras": nc, "points": npt, "observations": no, "lm_iterations": LM_ITERS},
-        "final_cost": out_mixed["final_cost"],
-        "initial_cost": out_mixed["initial_cost"],
-        "pcg_iterations": out_mixed["linear_solver_iterations"],
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "kernel_ms_per_solve": kernel_share,
     }
-    line.update(res)
-    sm.close()
+    sm = p = None
+    if not args.no_c3:
+        c3, sm, p = bench_c3(S, args)
+        line.update(c3)
+    batch = serving = None
+    if not args.no_batch:
+        log("bench: batched local BA (configuration 4, 512 windows)")
+        batch = bench_batch(S, args.seed, max(args.steps, 5), max(args.warmup, 1))
+        log("bench: batch p50 %.2f ms p99 %.2f ms" % (batch["p50_ms"], batch["p99_ms"]))
+    if not args.no_serving:
+        log("bench: serving loop (configuration 5)")
+        serving = bench_serving(S, args.seed, [2000.0, 5000.0, 10000.0, 20000.0], args.serve_requests)
+    cpu = None
+    if not args.no_cpu_baseline and p is not None:
+        log("bench: CPU oracle baseline (bounded sample)")
+        cpu = cpu_baseline(S, p)
+    line.update(roofline=line.get("roofline"), cpu_baseline=cpu, batched_local_ba=batch, serving=serving)
+    if sm is not None:
+        sm.close()
     print(json.dumps(line), flush=True)
 
 

@@ -84,8 +84,10 @@ typedef struct ba_options {
     int32_t dense_schur_max_cameras;           /* 300 */
     int32_t device;                            /* HIP device ordinal, 0 */
     int32_t verbose;                           /* print a Ceres-style iteration table to stdout */
-    int32_t refine_steps;                      /* FP64 refinement steps of the dense mixed solve, 2 */
-    int32_t reserved0;
+    int32_t refine_steps;                      /* FP64 refinement steps of the dense mixed solve, 2; batched
+                                                  windows run up to 10x as many preconditioned CG steps */
+    int32_t launch_mode;                       /* 0: each LM iteration replayed as one hipGraph (default);
+                                                  1: plain stream launches (always used while profiling) */
     double loss_scale;                  /* 1.0 (HuberLoss(1.0)) */
     double function_tolerance;          /* 1e-6 */
     double gradient_tolerance;          /* 1e-10 */

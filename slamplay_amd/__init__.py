@@ -25,7 +25,8 @@ import numpy as np
 from . import _capi as C
 from ._capi import BAError
 
-__all__ = ["BALProblem", "SolveBA", "Solver", "Batch", "Server", "simulate", "server_config", "default_options",
+__all__ = ["BALProblem", "SolveBA", "Solver", "Batch", "Server", "simulate", "server_config", "local_ba_windows",
+           "default_options",
            "eval_residual_jacobian", "eval_cost", "synthetic", "synth_params", "BAError", "device_count"]
 
 
@@ -244,6 +245,19 @@ def synth_params(config, **kw):
             raise AttributeError(f"unknown synthetic parameter {k}")
         setattr(p, k, v)
     return p
+
+
+def local_ba_windows(count, seed=1, min_cameras=5, max_cameras=20, min_points=200, max_points=3000):
+    """`count` local-BA windows of uneven size (benchmark configuration 4): keyframes uniform on
+    [min_cameras, max_cameras], points log-uniform on [min_points, max_points], tracks of 2 frames up to
+    the window length, random-walk keyframes, one generator seed per window."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        nc = int(rng.integers(min_cameras, max_cameras + 1))
+        npts = int(round(np.exp(rng.uniform(np.log(min_points), np.log(max_points)))))
+        out.append(synthetic(4, seed=int(seed) * 1_000_003 + i, num_cameras=nc, num_points=npts, max_track=nc))
+    return out
 
 
 def synthetic(config=1, seed=1, ground_truth=False, **kw):

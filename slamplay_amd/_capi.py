@@ -34,7 +34,7 @@ class Options(ctypes.Structure):
     _fields_ = [("max_num_iterations", i32), ("max_num_consecutive_invalid_steps", i32), ("jacobi_scaling", i32),
                 ("loss", i32), ("linear_solver", i32), ("pcg_max_iterations", i32), ("precision", i32),
                 ("dense_schur_max_cameras", i32), ("device", i32), ("verbose", i32), ("refine_steps", i32),
-                ("reserved0", i32), ("loss_scale", f64), ("function_tolerance", f64), ("gradient_tolerance", f64),
+                ("launch_mode", i32), ("loss_scale", f64), ("function_tolerance", f64), ("gradient_tolerance", f64),
                 ("parameter_tolerance", f64), ("initial_trust_region_radius", f64), ("max_trust_region_radius", f64),
                 ("min_trust_region_radius", f64), ("min_relative_decrease", f64), ("min_lm_diagonal", f64),
                 ("max_lm_diagonal", f64), ("pcg_tolerance", f64)]

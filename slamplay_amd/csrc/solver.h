@@ -45,4 +45,7 @@ struct SolverBase {
 
 int make_solver(const ba_problem_view* v, const ba_options& o, std::unique_ptr<SolverBase>* out);
 
+// BA_OK, or BA_ERR_INVALID_ARGUMENT with the reason for an option out of range.
+int validate_options(const ba_options* o);
+
 }  // namespace mba

@@ -54,10 +54,13 @@ int check_options(const ba_options* o) {
     if (!(o->min_lm_diagonal > 0.0) || !(o->max_lm_diagonal >= o->min_lm_diagonal))
         return fail(BA_ERR_INVALID_ARGUMENT, "invalid LM diagonal bounds");
     if (o->pcg_max_iterations < 0) return fail(BA_ERR_INVALID_ARGUMENT, "pcg_max_iterations < 0");
+    if (o->launch_mode != 0 && o->launch_mode != 1) return fail(BA_ERR_INVALID_ARGUMENT, "unknown launch_mode %d", o->launch_mode);
     return BA_OK;
 }
 
 }  // namespace
+
+int validate_options(const ba_options* o) { return check_options(o); }
 
 // ---------------------------------------------------------------------------------------------
 // Host-built device layout shared by the FP64 and mixed solvers.
@@ -215,6 +218,8 @@ struct Solver final : SolverBase {
     DevBuf<ba_iteration> trace;
     LMState* st_host = nullptr;  // pinned
     DenseSchur<Real> dense;
+    // one LM iteration captured once and replayed (launch_mode 0)
+    hipGraphExec_t iter_graph = nullptr;
     // profiling
     bool profiling = false;
     struct Rec {
@@ -227,6 +232,7 @@ struct Solver final : SolverBase {
     std::vector<ba_kernel_time> ktimes;
 
     ~Solver() override {
+        if (iter_graph) (void)hipGraphExecDestroy(iter_graph);
         if (st_host) (void)hipHostFree(st_host);
         for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
         if (s) (void)hipStreamDestroy(s);
@@ -454,6 +460,24 @@ struct Solver final : SolverBase {
         launch("post_lin", k_post_lin<Real>, 1, kWG, d);
     }
 
+    // Records one LM iteration (kernel launches only: no memset or copy nodes) into iter_graph.
+    // Every argument is fixed for the solver's lifetime (device pointers, PCG iteration numbers);
+    // what changes between iterations lives in device memory (the LM state).
+    int capture_iteration() {
+        hipGraph_t g = nullptr;
+        BA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        enqueue_iteration();
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (e != hipSuccess) return fail(BA_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        const hipError_t e2 = hipGraphInstantiate(&iter_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e2 != hipSuccess) {
+            iter_graph = nullptr;
+            return fail(BA_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e2));
+        }
+        return BA_OK;
+    }
+
     int run(ba_summary* summary) override {
         const auto t0 = std::chrono::steady_clock::now();
         BA_TRY(use_device(device));
@@ -480,8 +504,13 @@ struct Solver final : SolverBase {
         launch("post_lin", k_post_lin<Real>, 1, kWG, d);
         BA_HIP(hipGetLastError());
         const int check_every = 4;
+        const bool graphs = opt.launch_mode == 0 && !profiling;
+        if (graphs && !iter_graph) BA_TRY(capture_iteration());
         for (int it = 0; it < opt.max_num_iterations; ++it) {
-            enqueue_iteration();
+            if (graphs)
+                BA_HIP(hipGraphLaunch(iter_graph, s));
+            else
+                enqueue_iteration();
             BA_HIP(hipGetLastError());
             if ((it + 1) % check_every == 0 || it + 1 == opt.max_num_iterations) {
                 BA_HIP(hipMemcpyAsync(st_host, st.p, sizeof(LMState), hipMemcpyDeviceToHost, s));

@@ -2,10 +2,12 @@
 //
 // Every window lives in one contiguous region of a device arena; WinDesc holds the absolute
 // byte offsets of its arrays.  Within a window:
-//   * observations in camera order (cam_off, co_pt local point index, co_uv);
-//   * points in SELL-64 chunks (chunk_off / chunk_len), slot_of[j] = slot of observation j;
+//   * observations in camera order (cam_off, co_pt local point index, co_uv64);
+//   * points in SELL-64 chunks (chunk_off / chunk_len), slot_of[j] = slot of observation j; the
+//     slot records pv are three double4 arrays of n_slots entries (component-major);
 //   * the 9x9 blocks (a <= b) of the reduced camera system with their observation pairs
-//     (i, j) (camera(i) = a, camera(j) = b, same point), built once on the host.
+//     (i, j) (camera(i) = a, camera(j) = b, same point), built once on the host and listed with
+//     the largest pair count first (the order in which workgroup waves pick them up).
 #pragma once
 
 #include <cstdint>
@@ -23,13 +25,21 @@ struct WinState {
 
 struct WinDesc {
     int K, P, No, n_chunks, n_slots, n_blocks, n_pairs, pad;
-    int64_t o_cam_off, o_co_pt, o_co_uv, o_co_uv64, o_slot_of, o_chunk_len, o_chunk_off, o_blk, o_pairs;
+    int64_t o_cam_off, o_co_pt, o_co_uv64, o_slot_of, o_chunk_len, o_chunk_off, o_blk, o_pairs;
     int64_t o_cam_x, o_cam_x0, o_cam_c, o_pt_x, o_pt_x0, o_pt_c;
-    int64_t o_prep, o_prep_c, o_udiag, o_gc, o_sc, o_dc2, o_bc, o_yc;
-    int64_t o_V, o_gp, o_sp, o_Vinv, o_qp, o_yp, o_pv, o_u, o_js, o_S64, o_state;
+    int64_t o_prep, o_prep_c, o_udiag, o_gc, o_sc, o_dc2, o_bc;
+    int64_t o_V, o_gp, o_sp, o_Vinv, o_qp, o_yp, o_pv, o_j64, o_js, o_S, o_state;
 };
 
-constexpr int kWinJs = 36;  // per-observation record: Jc~ 18, Jp~ 6, Hp 6, I - G 3, r~ 2, pad 1
+// Per-observation records.
+//   j64 (FP64, 26 values, stored component-major j64[e][j]): corrected J_c (2x9, row-major),
+//       J_p (2x3), r (2) at the linearization point, unscaled -- written once per accepted step;
+//       the FP64 operator of the LM system;
+//   js (working precision, 36 values): scaled J~_c (18), J~_p (6), H = J~_p V*^-1 (6),
+//       I - G = I - J~_p V*^-1 J~_p' (3: g00, g01, g11), unused (3) -- the input of the reduced
+//       camera system, rewritten whenever the damping changes.
+constexpr int kWinJ64 = 26;
+constexpr int kWinJs = 36;
 
 template <typename T>
 __host__ __device__ __forceinline__ T* at(char* base, int64_t off) {

@@ -64,13 +64,23 @@ Opts opts_of(const ba_options& o) {
 template <typename Real>
 int64_t static_lds(int nt) {
     const int nw = nt / 64;
-    return (int64_t)nw * 20 * 8 + (int64_t)nw * 16 * 36 * sizeof(Real) + kWinMaxN * 8 + 16;
+    // red, xv, rv, zv, dv, qv, sdot, colv, dinv, cpart, sflag of k_window_lm, plus slack for the
+    // compiler's alignment
+    return (int64_t)nw * 8 + 5 * kWinMaxN * 8 + 8 + 2 * kWinMaxN * sizeof(Real) + kWinMaxK * 4 * 8 + 16 + 256;
 }
 
-// Largest reduced system (9 K) a workgroup of nt threads can factor in LDS.
+// Dynamic LDS of k_window_lm for reduced systems up to n x n: the factor, or the per-wave
+// staging areas of the Schur phase if larger.
+template <typename Real>
+int64_t dynamic_lds(int nt, int n) {
+    return std::max<int64_t>((int64_t)n * n * sizeof(Real), (int64_t)(nt / 64) * win_stage_bytes<Real>());
+}
+
+// Largest reduced system (9 K) a workgroup of nt threads can factor in LDS (0: none fits).
 template <typename Real>
 int max_window_n(int nt) {
     const int64_t room = lds_per_block() - static_lds<Real>(nt);
+    if ((int64_t)(nt / 64) * win_stage_bytes<Real>() > room) return 0;
     int n = (int)std::floor(std::sqrt((double)room / sizeof(Real)));
     n = std::min(n, kWinMaxN);
     return n - n % 9;
@@ -114,7 +124,6 @@ int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowIm
     };
     D.o_cam_off = take(4 * (K + 1));
     D.o_co_pt = take(4 * (int64_t)No);
-    D.o_co_uv = take(2 * (int64_t)No * sizeof(Real));
     D.o_co_uv64 = take(16 * (int64_t)No);
     D.o_slot_of = take(4 * (int64_t)No);
     D.o_chunk_len = take(4 * (int64_t)nch);
@@ -127,41 +136,44 @@ int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowIm
     D.o_pt_x = take(32 * (int64_t)P);
     D.o_pt_x0 = take(32 * (int64_t)P);
     D.o_pt_c = take(32 * (int64_t)P);
-    D.o_prep = take(sizeof(CameraPrep<Real>) * (int64_t)K);
+    D.o_prep = take(sizeof(CameraPrep<double>) * (int64_t)K);
     D.o_prep_c = take(sizeof(CameraPrep<double>) * (int64_t)K);
     D.o_udiag = take(72 * (int64_t)K);
     D.o_gc = take(72 * (int64_t)K);
     D.o_sc = take(72 * (int64_t)K);
     D.o_dc2 = take(72 * (int64_t)K);
     D.o_bc = take(72 * (int64_t)K);
-    D.o_yc = take(72 * (int64_t)K);
     D.o_V = take(48 * (int64_t)P);
     D.o_gp = take(24 * (int64_t)P);
     D.o_sp = take(24 * (int64_t)P);
     D.o_Vinv = take(48 * (int64_t)P);
-    D.o_qp = take(sizeof(R4<Real>) * (int64_t)P);
+    D.o_qp = take(32 * (int64_t)P);
     D.o_yp = take(24 * (int64_t)P);
     D.o_pv = take(96 * (int64_t)slots);
-    D.o_u = take(sizeof(R4<Real>) * (int64_t)slots);
+    D.o_j64 = take(8 * kWinJ64 * (int64_t)No);
     D.o_js = take(kWinJs * sizeof(Real) * (int64_t)No);
-    D.o_S64 = take(8 * (int64_t)n * n);
+    D.o_S = take(sizeof(Real) * (int64_t)n * n);  // zero: blocks without common points are never written
     D.o_state = take(sizeof(WinState));
     img->bytes.assign(off, 0);
     char* b = img->bytes.data();
     std::memcpy(b + D.o_cam_off, L.cam_off.data(), 4 * (size_t)(K + 1));
     std::memcpy(b + D.o_co_pt, L.co_pt.data(), 4 * (size_t)No);
-    Real* uv = reinterpret_cast<Real*>(b + D.o_co_uv);
-    for (size_t i = 0; i < 2 * (size_t)No; ++i) uv[i] = (Real)L.co_uv[i];
     std::memcpy(b + D.o_co_uv64, L.co_uv.data(), 16 * (size_t)No);
     std::memcpy(b + D.o_slot_of, L.slot_of.data(), 4 * (size_t)No);
     std::memcpy(b + D.o_chunk_len, L.chunk_len.data(), 4 * (size_t)nch);
     std::memcpy(b + D.o_chunk_off, L.chunk_off.data(), 4 * (size_t)(nch + 1));
+    // blocks with the most pairs first: the workgroup's waves take them in this order
+    std::vector<int> border(nb);
+    std::iota(border.begin(), border.end(), 0);
+    std::stable_sort(border.begin(), border.end(),
+                     [&](int x, int y) { return pb.off[x + 1] - pb.off[x] > pb.off[y + 1] - pb.off[y]; });
     int* blk = reinterpret_cast<int*>(b + D.o_blk);
-    for (int i = 0; i < nb; ++i) {
-        blk[4 * i] = pb.a[i];
-        blk[4 * i + 1] = pb.b[i];
-        blk[4 * i + 2] = pb.off[i];
-        blk[4 * i + 3] = pb.off[i + 1];
+    for (int k = 0; k < nb; ++k) {
+        const int i = border[k];
+        blk[4 * k] = pb.a[i];
+        blk[4 * k + 1] = pb.b[i];
+        blk[4 * k + 2] = pb.off[i];
+        blk[4 * k + 3] = pb.off[i + 1];
     }
     int* pr = reinterpret_cast<int*>(b + D.o_pairs);
     for (int i = 0; i < npairs; ++i) {
@@ -189,7 +201,7 @@ int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowIm
     img->pt_perm = L.pt_perm;
     img->K = K;
     img->P = P;
-    img->cost = (int64_t)No * 8 + (int64_t)npairs * 2 + (int64_t)n * n * n / 64;
+    img->cost = (int64_t)No * 16 + (int64_t)npairs * 4 + (int64_t)n * n * n / 32;
     return BA_OK;
 }
 
@@ -295,13 +307,17 @@ struct WindowEngine {
 
     int max_n() const { return max_window_n<Real>(threads); }
 
+    // Mixed precision: the preconditioned CG of a window step runs up to 10 iterations per
+    // refinement step of the options (20 by default); FP64 windows solve directly.
+    int cg_iterations() const { return sizeof(Real) == sizeof(double) ? 0 : 10 * std::max(opt.refine_steps, 1); }
+
     template <int NT>
     int launch_nt(int n_work, int iters, int nmax) {
-        const size_t shm = (size_t)nmax * nmax * sizeof(Real);
+        const size_t shm = (size_t)dynamic_lds<Real>(NT, nmax);
         BA_HIP(hipFuncSetAttribute((const void*)k_window_lm<Real, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)shm));
         hipLaunchKernelGGL((k_window_lm<Real, NT>), dim3(n_work), dim3(NT), shm, s, (const WinDesc*)ddesc.p, arena.p,
-                           (const int*)dwork.p, n_work, iters, o, sizeof(Real) == sizeof(double) ? 0 : opt.refine_steps);
+                           (const int*)dwork.p, n_work, iters, o, cg_iterations());
         BA_HIP(hipGetLastError());
         return BA_OK;
     }
@@ -402,6 +418,7 @@ struct Batch final : BatchBase {
 
     int create(const ba_problem_view* views, int count, const ba_options& o) {
         opt = o;
+        BA_TRY(validate_options(&o));
         BA_TRY(use_device(o.device));
         imgs.resize(count);
         large_index.assign(count, -1);
@@ -604,6 +621,8 @@ struct Server final : ServerBase {
     int create(const ba_server_config& c) {
         cfg = c;
         if (cfg.max_windows < 1) return fail(BA_ERR_INVALID_ARGUMENT, "max_windows < 1");
+        if (cfg.arena_bytes < 1) return fail(BA_ERR_INVALID_ARGUMENT, "arena_bytes < 1");
+        BA_TRY(validate_options(&cfg.options));
         BA_TRY(use_device(cfg.options.device));
         if (cfg.iterations_per_step < 1) cfg.iterations_per_step = 1;
         BA_TRY(eng.init(cfg.options, cfg.arena_bytes, cfg.max_windows));

@@ -1,54 +1,45 @@
 // window_kernels.h -- batched local bundle adjustment: one workgroup runs whole LM iterations of
-// one window (the dense-Schur solve of ba_solve, restated for windows of <= 32 cameras).
+// one window (the dense-Schur solve of ba_solve, restated for windows of a few dozen cameras).
 //
-// Per LM iteration, with __syncthreads between phases:
-//   lin    (after an accepted step) camera prep; waves over cameras recompute the corrected
-//          Jacobians: diag(Jc'Jc), Jc'r per camera by wave reductions, Jp'Jp and Jp'r per
-//          observation (FP64 products) into slot order; threads over points sum their slots;
-//   damp   V*^-1 (FP64), q = V*^-1 g~_p, D_c^2;
-//   store  per observation Jc~, Jp~, Hp = Jp~ V*^-1, I - G (FP64-formed), r~; reduced gradient
-//          b_c = sum Jc~'(r~ - Jp~ q) by wave reductions;
-//   S      each wave owns 9x9 blocks (a, b) of S and reduces their observation pairs in a fixed
-//          order, staging 16 pairs at a time in wave-private LDS (no workgroup barrier);
-//   chol   S in LDS, tiled right-looking Cholesky (16-column panels factored by wave 0, trailing
-//          update by the workgroup: 2 barriers per panel); triangular solves by one wave with
-//          the right-hand side in registers and shuffles; FP64 refinement against the FP64 S in
-//          mixed precision;
-//   back   u = Jp~'(r~ - Jc~ y_c) per observation, y_p = V*^-1 sum u per point, candidate;
-//   decide model cost change from the stored J~, FP64 candidate cost, Ceres LM rule.
-// A window's arithmetic and reduction order depend only on the window, so its result is the same
-// whichever other windows share the launch.
+// Per LM iteration, with workgroup barriers between phases:
+//   lin     (after an accepted step) FP64 camera prep; waves over cameras evaluate the corrected
+//           residuals and Jacobians once, in FP64, into the per-observation j64 records, with
+//           diag(J_c'J_c) and J_c'r per camera (wave reductions) and J_p'J_p, J_p'r per
+//           observation into slot order; threads over points sum their slots;
+//   damp    V*^-1 (FP64), q = V*^-1 g~_p, D_c^2;
+//   records per observation the working-precision inputs of the reduced system (J~_c, J~_p,
+//           H = J~_p V*^-1, I - G) from j64; reduced gradient b_c = sum J~_c'(r~ - J~_p q) in FP64;
+//   schur   each wave takes 9x9 blocks (a, b) of S from a shared counter, largest first, stages
+//           the records of 32 observation pairs at a time in LDS with vector loads, and a lane
+//           accumulates every 32nd pair of the block for rows 0-4 or 5-8 in registers; the 32
+//           partial sums are combined by a butterfly in FP64 and the block is written to S in
+//           global memory, which is then copied into LDS;
+//   chol    right-looking Cholesky of S in LDS, one column at a time over the whole workgroup;
+//   solve   blocked triangular solves (16-row diagonal blocks by one wave, the off-diagonal
+//           updates by the workgroup).  Mixed precision: preconditioned CG in FP64 with the FP32
+//           factor as preconditioner and the FP64 operator of the LM system applied implicitly
+//           through the j64 records (S y = D_c^2 y + sum J~_c'(J~_c y - J~_p z),
+//           z = V*^-1 sum J~_p'J~_c y), until the relative residual is 1e-12: the step is the FP64
+//           step also at trust radii where cond(S) is beyond what an FP32 factor resolves;
+//   back    u = J~_p'(r~ - J~_c y_c) per observation (FP64), y_p = V*^-1 sum u per point,
+//           candidate parameters;
+//   decide  model cost change from j64, FP64 candidate cost, Ceres LM rule.
+// A window's arithmetic and reduction order depend only on the window -- per-camera sums are
+// combined in camera order, per-point sums by one fixed wave, S blocks by one wave each in a fixed
+// lane order -- so its result is bitwise the same whichever other windows share the launch and at
+// either workgroup size.
 #pragma once
 
 #include "window.h"
 
 namespace mba {
 
-__device__ __forceinline__ void wave_sync() {
-    // LDS traffic of one wave is processed in order; this keeps the compiler from moving loads
-    // and stores of the wave-private staging area across the exchange point
+// Orders the LDS accesses of one wave around an exchange through its private staging area (the
+// wave's LDS operations complete in order; the fences keep the compiler from moving them).
+__device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Sum of N values over the workgroup (NT threads); totals in red[i * NW], barrier included.
-template <int N, int NT>
-__device__ __forceinline__ void wg_sum(double (&v)[N], double* red) {
-    constexpr int NW = NT / 64;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        const double s = wave_sum(v[i]);
-        if (lane == 0) red[i * NW + wave] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < N) {
-        double s = red[threadIdx.x * NW];
-        for (int w = 1; w < NW; ++w) s += red[threadIdx.x * NW + w];
-        red[threadIdx.x * NW] = s;
-    }
-    __syncthreads();
 }
 
 template <int NT>
@@ -64,92 +55,26 @@ __device__ __forceinline__ double wg_max(double v, double* red) {
     return m;
 }
 
+constexpr int kWinMaxN = 288;  // 9 * 32 cameras
+constexpr int kWinMaxK = kWinMaxN / 9;
+constexpr int kWinTb = 16;     // diagonal block of the triangular solves
+
 struct WinPtrs {
+    int No, n_slots;
     const int *cam_off, *co_pt, *slot_of, *chunk_len, *chunk_off;
     const int4* blk;
     const int2* pairs;
     const double* co_uv64;
-    double *cam_x, *cam_c, *udiag, *gc, *sc, *dc2, *bc, *yc, *V, *gp, *sp, *Vinv, *yp, *S64;
-    double4 *pt_x, *pt_c, *pv;
+    double *cam_x, *cam_c, *udiag, *gc, *sc, *dc2, *bc, *V, *gp, *sp, *Vinv, *yp, *j64;
+    double4 *pt_x, *pt_c, *pv, *qp;
+    CameraPrep<double>*prep, *prep_c;
     WinState* st;
 };
 
-template <typename Real>
-__device__ __forceinline__ void win_corrected(const CameraPrep<Real>& P, const double4* pt_x, const Real* co_uv,
-                                              const int* co_pt, int j, const Opts& o, int& p, Real r[2], Real Jc[18],
-                                              Real Jp[6]) {
-    p = co_pt[j];
-    const double4 x4 = pt_x[p];
-    const double X[3] = {x4.x, x4.y, x4.z};
-    snavely_residual_jacobian<Real>(P, X, co_uv[2 * (size_t)j], co_uv[2 * (size_t)j + 1], r, Jc, Jp);
-    const Real w = huber_sqrt_rho1<Real>(r[0] * r[0] + r[1] * r[1], o.loss, (Real)o.loss_a);
-    r[0] *= w;
-    r[1] *= w;
-#pragma unroll
-    for (int i = 0; i < 18; ++i) Jc[i] *= w;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) Jp[i] *= w;
-}
-
-// Forward (lower = 1: L z = b) or backward (lower = 0: L' x = z) substitution of one wave with the
-// vector held as yv[s] = y[64 s + lane]; L is the n x n lower factor in LDS.
-template <typename F, int NS>
-__device__ __forceinline__ void wave_trsv(const F* L, int n, double (&yv)[NS], bool lower) {
-    const int lane = threadIdx.x & 63;
-    if (lower) {
-#pragma unroll
-        for (int ks = 0; ks < NS; ++ks) {
-            for (int kk = 0; kk < 64; ++kk) {
-                const int k = 64 * ks + kk;
-                if (k >= n) break;
-                const double v = __shfl(yv[ks], kk, 64) / (double)L[(size_t)k * n + k];
-                if (lane == kk) yv[ks] = v;
-#pragma unroll
-                for (int s = 0; s < NS; ++s) {
-                    const int i = 64 * s + lane;
-                    if (i > k && i < n) yv[s] -= (double)L[(size_t)i * n + k] * v;
-                }
-            }
-        }
-    } else {
-#pragma unroll
-        for (int ks = NS - 1; ks >= 0; --ks) {
-            for (int kk = 63; kk >= 0; --kk) {
-                const int k = 64 * ks + kk;
-                if (k >= n) continue;
-                const double v = __shfl(yv[ks], kk, 64) / (double)L[(size_t)k * n + k];
-                if (lane == kk) yv[ks] = v;
-#pragma unroll
-                for (int s = 0; s < NS; ++s) {
-                    const int i = 64 * s + lane;
-                    if (i < k) yv[s] -= (double)L[(size_t)k * n + i] * v;
-                }
-            }
-        }
-    }
-}
-
-constexpr int kWinMaxN = 288;  // 9 * 32 cameras
-constexpr int kWinNS = (kWinMaxN + 63) / 64;
-constexpr int kWinTile = 16;
-
-template <typename Real, int NT>
-__global__ void __launch_bounds__(NT) k_window_lm(const WinDesc* __restrict__ descs, char* __restrict__ arena,
-                                                  const int* __restrict__ work, int n_work, int iters, Opts o,
-                                                  int refine) {
-    using F = Real;  // factor precision: FP32 in mixed precision (refined in FP64), FP64 otherwise
-    constexpr int NW = NT / 64;
-    extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ double red[NW * 20];
-    __shared__ Real stage[NW][16][36];
-    __shared__ double rsh[kWinMaxN];
-    __shared__ int sflag[2];  // 0: step invalid, 1: accepted
-    const int wi = blockIdx.x;
-    if (wi >= n_work) return;
-    const WinDesc D = descs[work[wi]];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int K = D.K, P = D.P, n = 9 * K;
+__device__ __forceinline__ WinPtrs win_ptrs(const WinDesc& D, char* arena) {
     WinPtrs w;
+    w.No = D.No;
+    w.n_slots = D.n_slots;
     w.cam_off = at<int>(arena, D.o_cam_off);
     w.co_pt = at<int>(arena, D.o_co_pt);
     w.slot_of = at<int>(arena, D.o_slot_of);
@@ -158,7 +83,6 @@ __global__ void __launch_bounds__(NT) k_window_lm(const WinDesc* __restrict__ de
     w.blk = at<int4>(arena, D.o_blk);
     w.pairs = at<int2>(arena, D.o_pairs);
     w.co_uv64 = at<double>(arena, D.o_co_uv64);
-    const Real* co_uv = at<Real>(arena, D.o_co_uv);
     w.cam_x = at<double>(arena, D.o_cam_x);
     w.cam_c = at<double>(arena, D.o_cam_c);
     w.udiag = at<double>(arena, D.o_udiag);
@@ -166,24 +90,385 @@ __global__ void __launch_bounds__(NT) k_window_lm(const WinDesc* __restrict__ de
     w.sc = at<double>(arena, D.o_sc);
     w.dc2 = at<double>(arena, D.o_dc2);
     w.bc = at<double>(arena, D.o_bc);
-    w.yc = at<double>(arena, D.o_yc);
     w.V = at<double>(arena, D.o_V);
     w.gp = at<double>(arena, D.o_gp);
     w.sp = at<double>(arena, D.o_sp);
     w.Vinv = at<double>(arena, D.o_Vinv);
     w.yp = at<double>(arena, D.o_yp);
-    w.S64 = at<double>(arena, D.o_S64);
+    w.j64 = at<double>(arena, D.o_j64);
     w.pt_x = at<double4>(arena, D.o_pt_x);
     w.pt_c = at<double4>(arena, D.o_pt_c);
     w.pv = at<double4>(arena, D.o_pv);
+    w.qp = at<double4>(arena, D.o_qp);
+    w.prep = at<CameraPrep<double>>(arena, D.o_prep);
+    w.prep_c = at<CameraPrep<double>>(arena, D.o_prep_c);
     w.st = at<WinState>(arena, D.o_state);
-    CameraPrep<Real>* prep = at<CameraPrep<Real>>(arena, D.o_prep);
-    CameraPrep<double>* prep_c = at<CameraPrep<double>>(arena, D.o_prep_c);
-    R4<Real>* qp = at<R4<Real>>(arena, D.o_qp);
-    R4<Real>* u = at<R4<Real>>(arena, D.o_u);
+    return w;
+}
+
+// Component e of the FP64 record of observation j (stored component-major: j64[e][j], so the
+// lanes of a wave, which own consecutive observations, read consecutive addresses).
+__device__ __forceinline__ double j64_at(const WinPtrs& w, int e, int j) { return w.j64[(size_t)e * w.No + j]; }
+
+// Scaled FP64 Jacobians of observation j: J~_c = J_c diag(s_c), J~_p = J_p diag(s_p).
+__device__ __forceinline__ void win_scaled(const WinPtrs& w, int j, const double sc[9], const double* sp, double Jc[18],
+                                           double Jp[6]) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        Jc[i] = j64_at(w, i, j) * sc[i];
+        Jc[9 + i] = j64_at(w, 9 + i, j) * sc[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        Jp[i] = j64_at(w, 18 + i, j) * sp[i];
+        Jp[3 + i] = j64_at(w, 21 + i, j) * sp[i];
+    }
+}
+
+// Slot records: pv holds three double4 arrays of n_slots entries each (component-major).
+__device__ __forceinline__ double4& pv_at(const WinPtrs& w, int k, int slot) {
+    return w.pv[(size_t)k * w.n_slots + slot];
+}
+
+// sum over the SELL slots of point p of the first slot record (x, y, z), in slot order.
+__device__ __forceinline__ void win_slot_sum3(const WinPtrs& w, int p, double t[3]) {
+    const int ch = p >> 6, l = p & 63, len = w.chunk_len[ch];
+    const int base = w.chunk_off[ch];
+    t[0] = t[1] = t[2] = 0.0;
+    for (int r = 0; r < len; ++r) {
+        const double4 u = pv_at(w, 0, base + 64 * r + l);
+        t[0] += u.x;
+        t[1] += u.y;
+        t[2] += u.z;
+    }
+}
+
+constexpr int kWinPairsPerStage = 32;
+
+// LDS bytes of the per-wave staging area of win_schur: the records of 32 pairs (i and j).
+template <typename Real>
+constexpr int win_stage_bytes() {
+    return 2 * kWinPairsPerStage * kWinJs * (int)sizeof(Real);
+}
+
+// 9x9 blocks of the reduced camera system into S (n x n row-major, both triangles, global memory):
+// S_ab = [a == b] D_a^2 - sum_pairs J~_c,i' G_ij J~_c,j with G_ij = H_i J~_p,j' (i != j) and
+// G_ii = -(I - G)_i.  Blocks are taken from *counter in list order.  Per stage of 32 pairs the
+// wave copies the record chunks the products read into its LDS staging area with 4-element vector
+// loads (lanes cover the chunks of consecutive pairs), then lanes 0-31 accumulate rows 0-4 and lanes
+// 32-63 rows 5-8 of pair q = lane % 32 in registers; the 32 partial sums of a block are combined by
+// a butterfly in FP64.
+template <typename Real, typename F>
+__device__ __forceinline__ void win_schur(const int4* __restrict__ blk, const int2* __restrict__ pairs,
+                                          const Real* __restrict__ js, const double* __restrict__ dc2, F* S, int n,
+                                          int n_blocks, int* counter, Real* stage) {
+    constexpr int PS = kWinPairsPerStage;
+    const int lane = threadIdx.x & 63, q = lane & 31, h = lane >> 5;
+    const int rb = 5 * h;
+    Real* st_i = stage;
+    Real* st_j = stage + PS * kWinJs;
+    for (;;) {
+        int b = 0;
+        if (lane == 0) b = atomicAdd(counter, 1);
+        b = __shfl(b, 0, 64);
+        if (b >= n_blocks) break;
+        const int4 bk = blk[b];
+        Real acc[5][9];
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+#pragma unroll
+            for (int c = 0; c < 9; ++c) acc[r][c] = (Real)0;
+        for (int base = bk.z; base < bk.w; base += PS) {
+            const int m = min(PS, bk.w - base);
+            int2 pr = make_int2(0, 0);
+            if (lane < m) pr = pairs[base + lane];
+            // cooperative copy of the chunks the pair products read: i records chunks 0-4 (J~_c) and
+            // 6-8 (H, I - G), j records chunks 0-5 (J~_c, J~_p); slot s of 14 per pair
+            constexpr int CI = 8, CJ = 6, CP = CI + CJ;
+            const int nslot = m * CP;
+            for (int s0 = 0; s0 < nslot; s0 += 64) {
+                const int s = s0 + lane;
+                const int k = min(s / CP, PS - 1), e = s - CP * (s / CP);
+                const int oi = __shfl(pr.x, k, 64), oj = __shfl(pr.y, k, 64);
+                if (s < nslot) {
+                    const bool is_j = e >= CI;
+                    const int chunk = is_j ? e - CI : (e < 5 ? e : e + 1);
+                    const int obs = is_j ? oj : oi;
+                    const R4<Real> v = reinterpret_cast<const R4<Real>*>(js + kWinJs * (size_t)obs)[chunk];
+                    *reinterpret_cast<R4<Real>*>((is_j ? st_j : st_i) + kWinJs * k + 4 * chunk) = v;
+                }
+            }
+            const int pi = __shfl(pr.x, q, 64), pj = __shfl(pr.y, q, 64);
+            wave_sync_lds();
+            if (q < m) {
+                // whole 4-element chunks from LDS (record stride 36 elements keeps 8 consecutive
+                // lanes on distinct banks)
+                const R4<Real>* Ri = reinterpret_cast<const R4<Real>*>(st_i + kWinJs * q);
+                const R4<Real>* Rj = reinterpret_cast<const R4<Real>*>(st_j + kWinJs * q);
+                Real ji[20], jj[24], hg[12];
+#pragma unroll
+                for (int c4 = 0; c4 < 5; ++c4) {
+                    const R4<Real> v = Ri[c4];
+                    ji[4 * c4] = v.x;
+                    ji[4 * c4 + 1] = v.y;
+                    ji[4 * c4 + 2] = v.z;
+                    ji[4 * c4 + 3] = v.w;
+                }
+#pragma unroll
+                for (int c4 = 0; c4 < 3; ++c4) {
+                    const R4<Real> v = Ri[6 + c4];
+                    hg[4 * c4] = v.x;
+                    hg[4 * c4 + 1] = v.y;
+                    hg[4 * c4 + 2] = v.z;
+                    hg[4 * c4 + 3] = v.w;
+                }
+#pragma unroll
+                for (int c4 = 0; c4 < 6; ++c4) {
+                    const R4<Real> v = Rj[c4];
+                    jj[4 * c4] = v.x;
+                    jj[4 * c4 + 1] = v.y;
+                    jj[4 * c4 + 2] = v.z;
+                    jj[4 * c4 + 3] = v.w;
+                }
+                Real G00, G01, G10, G11;
+                if (pi == pj) {
+                    // an observation with itself carries its share of U*: J~c'(I - G)J~c, negated
+                    // because the accumulated sum is subtracted
+                    G00 = -hg[6];
+                    G01 = G10 = -hg[7];
+                    G11 = -hg[8];
+                } else {
+                    // H_i = hg[0..5], J~_p,j = jj[18..23]
+                    G00 = hg[0] * jj[18] + hg[1] * jj[19] + hg[2] * jj[20];
+                    G01 = hg[0] * jj[21] + hg[1] * jj[22] + hg[2] * jj[23];
+                    G10 = hg[3] * jj[18] + hg[4] * jj[19] + hg[5] * jj[20];
+                    G11 = hg[3] * jj[21] + hg[4] * jj[22] + hg[5] * jj[23];
+                }
+                Real a0[5], a1[5];
+#pragma unroll
+                for (int r = 0; r < 5; ++r) {
+                    // rows rb + r (the second half's spare fifth row repeats row 8)
+                    a0[r] = h ? ji[min(5 + r, 8)] : ji[r];
+                    a1[r] = h ? ji[9 + min(5 + r, 8)] : ji[9 + r];
+                }
+#pragma unroll
+                for (int c = 0; c < 9; ++c) {
+                    const Real j0 = jj[c], j1 = jj[9 + c];
+                    const Real t0 = G00 * j0 + G01 * j1, t1 = G10 * j0 + G11 * j1;
+#pragma unroll
+                    for (int r = 0; r < 5; ++r) acc[r][c] += a0[r] * t0 + a1[r] * t1;
+                }
+            }
+            wave_sync_lds();
+        }
+        const int ca = bk.x, cb = bk.y;
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                double v = (double)acc[r][c];
+#pragma unroll
+                for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+                const int row = rb + r;
+                if (q == 0 && row < 9) {
+                    const double s = (ca == cb && row == c ? dc2[9 * ca + row] : 0.0) - v;
+                    S[(size_t)(9 * ca + row) * n + 9 * cb + c] = (F)s;
+                    if (ca != cb) S[(size_t)(9 * cb + c) * n + 9 * ca + row] = (F)s;
+                }
+            }
+        }
+    }
+}
+
+// In-place right-looking Cholesky of the n x n LDS matrix L (lower triangle used), one column per
+// step over the whole workgroup; dinv receives 1 / L_jj.  *bad is set when a pivot is not positive
+// (the factor is then meaningless and the caller rejects the step).
+template <typename F, int NT>
+__device__ __forceinline__ void win_cholesky(F* L, int n, F* colv, F* dinv, int* bad) {
+    constexpr int NW = NT / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int j = 0; j < n; ++j) {
+        F djj = L[(size_t)j * n + j];
+        const bool nonpos = !(djj > (F)0);
+        if (nonpos) djj = (F)1;
+        const F d = sqrt(djj), inv = (F)1 / d;
+        for (int i = j + 1 + tid; i < n; i += NT) colv[i] = L[(size_t)i * n + j] * inv;
+        __syncthreads();
+        if (tid == 0) {
+            L[(size_t)j * n + j] = d;
+            dinv[j] = inv;
+            if (nonpos) *bad = 1;
+        }
+        for (int i = j + 1 + wave; i < n; i += NW) {
+            const F lij = colv[i];
+            F* row = L + (size_t)i * n;
+            if (lane == 0) row[j] = lij;
+            for (int k = j + 1 + lane; k <= i; k += 64) row[k] -= lij * colv[k];
+        }
+        __syncthreads();
+    }
+}
+
+// x <- (L L')^-1 x for the LDS vector x (double), L the factor above.  Blocks of kWinTb rows: the
+// diagonal block by wave 0 (one row per lane), the rest of the vector by the workgroup.  Expects x
+// complete on entry (after a barrier) and leaves it complete.
+template <typename F, int NT>
+__device__ __forceinline__ void win_chol_solve(const F* L, const F* dinv, int n, double* x) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // forward: L z = x
+    for (int kb = 0; kb < n; kb += kWinTb) {
+        const int m = min(kWinTb, n - kb);
+        if (wave == 0) {
+            double v = lane < m ? x[kb + lane] : 0.0;
+            for (int c = 0; c < m; ++c) {
+                const double zc = __shfl(v, c, 64) * (double)dinv[kb + c];
+                if (lane == c) v = zc;
+                if (lane > c && lane < m) v -= (double)L[(size_t)(kb + lane) * n + kb + c] * zc;
+            }
+            if (lane < m) x[kb + lane] = v;
+        }
+        __syncthreads();
+        for (int i = kb + m + tid; i < n; i += NT) {
+            const F* row = L + (size_t)i * n + kb;
+            double s = 0.0;
+            for (int c = 0; c < m; ++c) s += (double)row[c] * x[kb + c];
+            x[i] -= s;
+        }
+        __syncthreads();
+    }
+    // backward: L' y = z
+    const int last = ((n - 1) / kWinTb) * kWinTb;
+    for (int kb = last; kb >= 0; kb -= kWinTb) {
+        const int m = min(kWinTb, n - kb);
+        if (wave == 0) {
+            double v = lane < m ? x[kb + lane] : 0.0;
+            for (int c = m - 1; c >= 0; --c) {
+                const double yc = __shfl(v, c, 64) * (double)dinv[kb + c];
+                if (lane == c) v = yc;
+                if (lane < c) v -= (double)L[(size_t)(kb + c) * n + kb + lane] * yc;
+            }
+            if (lane < m) x[kb + lane] = v;
+        }
+        __syncthreads();
+        for (int i = tid; i < kb; i += NT) {
+            double s = 0.0;
+            for (int c = 0; c < m; ++c) s += (double)L[(size_t)(kb + c) * n + i] * x[kb + c];
+            x[i] -= s;
+        }
+        __syncthreads();
+    }
+}
+
+// out = S y for the reduced camera system in FP64, applied through the j64 records:
+// S y = D_c^2 y + sum_j J~_c,j'(J~_c,j y - J~_p,j z_p(j)), z_p = V*^-1 sum_{j in p} J~_p,j' J~_c,j y.
+// y and out are LDS vectors of 9K; the point vectors z go through the slot records and w.yp.
+template <int NT>
+__device__ __forceinline__ void win_apply_S(const WinPtrs& w, int K, int P, const double* y, double* out) {
+    constexpr int NW = NT / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int c = wave; c < K; c += NW) {
+        double scc[9], yc[9];
+        for (int i = 0; i < 9; ++i) {
+            scc[i] = w.sc[9 * (size_t)c + i];
+            yc[i] = y[9 * c + i];
+        }
+        for (int j = w.cam_off[c] + lane; j < w.cam_off[c + 1]; j += 64) {
+            const int p = w.co_pt[j];
+            double Jc[18], Jp[6];
+            win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
+            double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                t0 += Jc[i] * yc[i];
+                t1 += Jc[9 + i] * yc[i];
+            }
+            pv_at(w, 0, w.slot_of[j]) =
+                make_double4(Jp[0] * t0 + Jp[3] * t1, Jp[1] * t0 + Jp[4] * t1, Jp[2] * t0 + Jp[5] * t1, 0.0);
+        }
+    }
+    __syncthreads();
+    for (int p = tid; p < P; p += NT) {
+        double t[3], z[3];
+        win_slot_sum3(w, p, t);
+        sym3_mul<double>(w.Vinv + 6 * (size_t)p, t, z);
+        for (int i = 0; i < 3; ++i) w.yp[3 * (size_t)p + i] = z[i];
+    }
+    __syncthreads();
+    for (int c = wave; c < K; c += NW) {
+        double scc[9], yc[9], acc[9];
+        for (int i = 0; i < 9; ++i) {
+            scc[i] = w.sc[9 * (size_t)c + i];
+            yc[i] = y[9 * c + i];
+            acc[i] = 0.0;
+        }
+        for (int j = w.cam_off[c] + lane; j < w.cam_off[c + 1]; j += 64) {
+            const int p = w.co_pt[j];
+            double Jc[18], Jp[6];
+            win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
+            const double* z = w.yp + 3 * (size_t)p;
+            double e0 = -(Jp[0] * z[0] + Jp[1] * z[1] + Jp[2] * z[2]);
+            double e1 = -(Jp[3] * z[0] + Jp[4] * z[1] + Jp[5] * z[2]);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                e0 += Jc[i] * yc[i];
+                e1 += Jc[9 + i] * yc[i];
+            }
+#pragma unroll
+            for (int i = 0; i < 9; ++i) acc[i] += Jc[i] * e0 + Jc[9 + i] * e1;
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) acc[i] = wave_sum(acc[i]);
+        if (lane == 0)
+            for (int i = 0; i < 9; ++i) out[9 * c + i] = w.dc2[9 * (size_t)c + i] * yc[i] + acc[i];
+    }
+    __syncthreads();
+}
+
+// a'b over n entries of two LDS vectors, by wave 0 in a fixed order; every thread gets the value.
+__device__ __forceinline__ double win_dot(const double* a, const double* b, int n, double* sh) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (wave == 0) {
+        double s = 0.0;
+        for (int k = lane; k < n; k += 64) s += a[k] * b[k];
+        s = wave_sum(s);
+        if (lane == 0) *sh = s;
+    }
+    __syncthreads();
+    const double v = *sh;
+    __syncthreads();
+    return v;
+}
+
+constexpr double kWinCgTol = 1e-12;  // relative residual at which the mixed-precision CG stops
+
+template <typename Real, int NT>
+__global__ void __launch_bounds__(NT) k_window_lm(const WinDesc* __restrict__ descs, char* __restrict__ arena,
+                                                  const int* __restrict__ work, int n_work, int iters, Opts o,
+                                                  int refine) {
+    using F = Real;  // reduced system and factor: FP32 in mixed precision (refined in FP64), FP64 otherwise
+    constexpr int NW = NT / 64;
+    extern __shared__ __align__(32) unsigned char smem[];
+    __shared__ double red[NW];
+    __shared__ double xv[kWinMaxN];        // reduced gradient, then the camera step y_c
+    __shared__ double rv[kWinMaxN];        // CG: residual
+    __shared__ double zv[kWinMaxN];        // CG: preconditioned residual
+    __shared__ double dv[kWinMaxN];        // CG: search direction
+    __shared__ double qv[kWinMaxN];        // CG: S times the search direction
+    __shared__ double sdot;
+    __shared__ F colv[kWinMaxN];           // Cholesky: current column
+    __shared__ F dinv[kWinMaxN];           // 1 / diagonal of the factor
+    __shared__ double cpart[kWinMaxK][4];  // per-camera partial sums, combined in camera order
+    __shared__ int sflag[4];               // 0: invalid point block, 1: accepted / factor failure, 2: block counter
+    const int wi = blockIdx.x;
+    if (wi >= n_work) return;
+    const WinDesc D = descs[work[wi]];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int K = D.K, P = D.P, n = 9 * K;
+    const WinPtrs w = win_ptrs(D, arena);
     Real* js = at<Real>(arena, D.o_js);
+    F* Sg = at<F>(arena, D.o_S);  // reduced system as formed (unlisted blocks stay zero)
     WinState* st = w.st;
-    F* Lf = reinterpret_cast<F*>(smem);
+    F* S = reinterpret_cast<F*>(smem);  // its LDS copy, factored in place
 
     __syncthreads();
     if (st->done) return;
@@ -192,25 +477,28 @@ __global__ void __launch_bounds__(NT) k_window_lm(const WinDesc* __restrict__ de
         for (int c = tid; c < K; c += NT) {
             CameraPrep<double> Pc;
             camera_prep<double>(w.cam_x + 9 * (size_t)c, Pc);
-            prep_c[c] = Pc;
+            w.prep_c[c] = Pc;
         }
         __syncthreads();
-        double cost[1] = {0.0};
         for (int c = wave; c < K; c += NW) {
-            const CameraPrep<double> Pc = prep_c[c];
+            double cost = 0.0;
             for (int j = w.cam_off[c] + lane; j < w.cam_off[c + 1]; j += 64) {
                 const double4 x4 = w.pt_x[w.co_pt[j]];
                 const double X[3] = {x4.x, x4.y, x4.z};
                 double r[2];
-                snavely_residual<double>(Pc, X, w.co_uv64[2 * (size_t)j], w.co_uv64[2 * (size_t)j + 1], r);
-                cost[0] += half_rho(r[0] * r[0] + r[1] * r[1], o.loss, o.loss_a);
+                snavely_residual<double>(w.prep_c[c], X, w.co_uv64[2 * (size_t)j], w.co_uv64[2 * (size_t)j + 1], r);
+                cost += half_rho(r[0] * r[0] + r[1] * r[1], o.loss, o.loss_a);
             }
+            cost = wave_sum(cost);
+            if (lane == 0) cpart[c][0] = cost;
         }
-        wg_sum<1, NT>(cost, red);
+        __syncthreads();
         if (tid == 0) {
-            st->x_cost = st->initial_cost = red[0];
+            double cost = 0.0;
+            for (int c = 0; c < K; ++c) cost += cpart[c][0];
+            st->x_cost = st->initial_cost = cost;
             st->rows = 1;
-            if (!isfinite(red[0])) {
+            if (!isfinite(cost)) {
                 st->done = 1;
                 st->termination = BA_FAILURE;
             }
@@ -226,34 +514,45 @@ __global__ void __launch_bounds__(NT) k_window_lm(const WinDesc* __restrict__ de
         if (st->need_lin) {
             const int first = st->first_lin;
             for (int c = tid; c < K; c += NT) {
-                CameraPrep<Real> Pr;
-                camera_prep<Real>(w.cam_x + 9 * (size_t)c, Pr);
-                prep[c] = Pr;
+                CameraPrep<double> Pc;
+                camera_prep<double>(w.cam_x + 9 * (size_t)c, Pc);
+                w.prep[c] = Pc;
             }
             __syncthreads();
             double gm = 0.0;
             for (int c = wave; c < K; c += NW) {
-                const CameraPrep<Real> Pr = prep[c];
                 double acc[18];
 #pragma unroll
                 for (int i = 0; i < 18; ++i) acc[i] = 0.0;
                 for (int j = w.cam_off[c] + lane; j < w.cam_off[c + 1]; j += 64) {
-                    int p;
-                    Real r[2], Jc[18], Jp[6];
-                    win_corrected<Real>(Pr, w.pt_x, co_uv, w.co_pt, j, o, p, r, Jc, Jp);
+                    const int p = w.co_pt[j];
+                    const double4 x4 = w.pt_x[p];
+                    const double X[3] = {x4.x, x4.y, x4.z};
+                    double r[2], Jc[18], Jp[6];
+                    snavely_residual_jacobian<double>(w.prep[c], X, w.co_uv64[2 * (size_t)j],
+                                                      w.co_uv64[2 * (size_t)j + 1], r, Jc, Jp);
+                    const double s = huber_sqrt_rho1<double>(r[0] * r[0] + r[1] * r[1], o.loss, o.loss_a);
+                    double* rec = w.j64 + j;
+                    const size_t ld = (size_t)w.No;
+#pragma unroll
+                    for (int i = 0; i < 18; ++i) rec[i * ld] = Jc[i] *= s;
+#pragma unroll
+                    for (int i = 0; i < 6; ++i) rec[(18 + i) * ld] = Jp[i] *= s;
+                    r[0] *= s;
+                    r[1] *= s;
+                    rec[24 * ld] = r[0];
+                    rec[25 * ld] = r[1];
 #pragma unroll
                     for (int i = 0; i < 9; ++i) {
-                        acc[i] += (double)(Jc[i] * Jc[i] + Jc[9 + i] * Jc[9 + i]);
-                        acc[9 + i] += (double)(Jc[i] * r[0] + Jc[9 + i] * r[1]);
+                        acc[i] += Jc[i] * Jc[i] + Jc[9 + i] * Jc[9 + i];
+                        acc[9 + i] += Jc[i] * r[0] + Jc[9 + i] * r[1];
                     }
-                    const double q[6] = {Jp[0], Jp[1], Jp[2], Jp[3], Jp[4], Jp[5]};
-                    const double r0 = r[0], r1 = r[1];
-                    double4* pv = w.pv + 3 * (size_t)w.slot_of[j];
-                    pv[0] = make_double4(q[0] * q[0] + q[3] * q[3], q[0] * q[1] + q[3] * q[4],
-                                         q[0] * q[2] + q[3] * q[5], q[1] * q[1] + q[4] * q[4]);
-                    pv[1] = make_double4(q[1] * q[2] + q[4] * q[5], q[2] * q[2] + q[5] * q[5], q[0] * r0 + q[3] * r1,
-                                         q[1] * r0 + q[4] * r1);
-                    pv[2] = make_double4(q[2] * r0 + q[5] * r1, 0.0, 0.0, 0.0);
+                    const int slot = w.slot_of[j];
+                    pv_at(w, 0, slot) = make_double4(Jp[0] * Jp[0] + Jp[3] * Jp[3], Jp[0] * Jp[1] + Jp[3] * Jp[4],
+                                                     Jp[0] * Jp[2] + Jp[3] * Jp[5], Jp[1] * Jp[1] + Jp[4] * Jp[4]);
+                    pv_at(w, 1, slot) = make_double4(Jp[1] * Jp[2] + Jp[4] * Jp[5], Jp[2] * Jp[2] + Jp[5] * Jp[5],
+                                                     Jp[0] * r[0] + Jp[3] * r[1], Jp[1] * r[0] + Jp[4] * r[1]);
+                    pv_at(w, 2, slot) = make_double4(Jp[2] * r[0] + Jp[5] * r[1], 0.0, 0.0, 0.0);
                 }
 #pragma unroll
                 for (int i = 0; i < 18; ++i) acc[i] = wave_sum(acc[i]);
@@ -271,8 +570,8 @@ __global__ void __launch_bounds__(NT) k_window_lm(const WinDesc* __restrict__ de
                 const size_t base = (size_t)w.chunk_off[ch];
                 double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
                 for (int r = 0; r < len; ++r) {
-                    const double4* pv = w.pv + 3 * (base + 64 * (size_t)r + l);
-                    const double4 x = pv[0], y = pv[1], z = pv[2];
+                    const int slot = (int)base + 64 * r + l;
+                    const double4 x = pv_at(w, 0, slot), y = pv_at(w, 1, slot), z = pv_at(w, 2, slot);
                     a[0] += x.x; a[1] += x.y; a[2] += x.z; a[3] += x.w;
                     a[4] += y.x; a[5] += y.y; a[6] += y.z; a[7] += y.w;
                     a[8] += z.x;
@@ -309,7 +608,11 @@ __global__ void __launch_bounds__(NT) k_window_lm(const WinDesc* __restrict__ de
             if (st->done) break;
         }
         // ---------------- damp ----------------
-        if (tid == 0) sflag[0] = 0;
+        if (tid == 0) {
+            sflag[0] = 0;
+            sflag[1] = 0;
+            sflag[2] = 0;
+        }
         __syncthreads();
         const double radius = st->radius;
         for (int p = tid; p < P; p += NT) {
@@ -332,290 +635,238 @@ __global__ void __launch_bounds__(NT) k_window_lm(const WinDesc* __restrict__ de
             const double sym[6] = {Ai[0], Ai[1], Ai[2], Ai[4], Ai[5], Ai[8]};
             for (int i = 0; i < 6; ++i) w.Vinv[6 * (size_t)p + i] = sym[i];
             const double g[3] = {s[0] * w.gp[3 * (size_t)p], s[1] * w.gp[3 * (size_t)p + 1], s[2] * w.gp[3 * (size_t)p + 2]};
-            R4<Real> q;
-            q.x = (Real)(Ai[0] * g[0] + Ai[1] * g[1] + Ai[2] * g[2]);
-            q.y = (Real)(Ai[3] * g[0] + Ai[4] * g[1] + Ai[5] * g[2]);
-            q.z = (Real)(Ai[6] * g[0] + Ai[7] * g[1] + Ai[8] * g[2]);
-            q.w = (Real)0;
-            qp[p] = q;
+            w.qp[p] = make_double4(Ai[0] * g[0] + Ai[1] * g[1] + Ai[2] * g[2], Ai[3] * g[0] + Ai[4] * g[1] + Ai[5] * g[2],
+                                   Ai[6] * g[0] + Ai[7] * g[1] + Ai[8] * g[2], 0.0);
         }
         for (int k = tid; k < n; k += NT)
             w.dc2[k] = fmin(fmax(w.sc[k] * w.udiag[k] * w.sc[k], o.min_diag), o.max_diag) / radius;
         __syncthreads();
         const bool invalid_blocks = sflag[0] != 0;
-        // ---------------- store + reduced gradient ----------------
+        // ---------------- records + reduced gradient ----------------
         if (!invalid_blocks) {
             for (int c = wave; c < K; c += NW) {
-                const CameraPrep<Real> Pr = prep[c];
-                Real scc[9];
-                for (int i = 0; i < 9; ++i) scc[i] = (Real)w.sc[9 * (size_t)c + i];
+                double scc[9];
+                for (int i = 0; i < 9; ++i) scc[i] = w.sc[9 * (size_t)c + i];
                 double bacc[9];
 #pragma unroll
                 for (int i = 0; i < 9; ++i) bacc[i] = 0.0;
                 for (int j = w.cam_off[c] + lane; j < w.cam_off[c + 1]; j += 64) {
-                    int p;
-                    Real r[2], Jc[18], Jp[6];
-                    win_corrected<Real>(Pr, w.pt_x, co_uv, w.co_pt, j, o, p, r, Jc, Jp);
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) {
-                        Jc[i] *= scc[i];
-                        Jc[9 + i] *= scc[i];
-                    }
-                    for (int i = 0; i < 3; ++i) {
-                        const Real s = (Real)w.sp[3 * (size_t)p + i];
-                        Jp[i] *= s;
-                        Jp[3 + i] *= s;
-                    }
+                    const int p = w.co_pt[j];
+                    double Jc[18], Jp[6];
+                    win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
                     const double* Vi = w.Vinv + 6 * (size_t)p;
                     const double j0[3] = {Jp[0], Jp[1], Jp[2]}, j1[3] = {Jp[3], Jp[4], Jp[5]};
                     double h0[3], h1[3];
                     sym3_mul<double>(Vi, j0, h0);
                     sym3_mul<double>(Vi, j1, h1);
-                    Real* rec = js + kWinJs * (size_t)j;
+                    Real* out = js + kWinJs * (size_t)j;
 #pragma unroll
-                    for (int i = 0; i < 18; ++i) rec[i] = Jc[i];
+                    for (int i = 0; i < 18; ++i) out[i] = (Real)Jc[i];
 #pragma unroll
-                    for (int i = 0; i < 6; ++i) rec[18 + i] = Jp[i];
+                    for (int i = 0; i < 6; ++i) out[18 + i] = (Real)Jp[i];
                     for (int i = 0; i < 3; ++i) {
-                        rec[24 + i] = (Real)h0[i];
-                        rec[27 + i] = (Real)h1[i];
+                        out[24 + i] = (Real)h0[i];
+                        out[27 + i] = (Real)h1[i];
                     }
-                    rec[30] = (Real)(1.0 - (j0[0] * h0[0] + j0[1] * h0[1] + j0[2] * h0[2]));
-                    rec[31] = (Real)(-(j0[0] * h1[0] + j0[1] * h1[1] + j0[2] * h1[2]));
-                    rec[32] = (Real)(1.0 - (j1[0] * h1[0] + j1[1] * h1[1] + j1[2] * h1[2]));
-                    rec[33] = r[0];
-                    rec[34] = r[1];
-                    const R4<Real> q = qp[p];
-                    const Real e0 = r[0] - (Jp[0] * q.x + Jp[1] * q.y + Jp[2] * q.z);
-                    const Real e1 = r[1] - (Jp[3] * q.x + Jp[4] * q.y + Jp[5] * q.z);
+                    out[30] = (Real)(1.0 - (j0[0] * h0[0] + j0[1] * h0[1] + j0[2] * h0[2]));
+                    out[31] = (Real)(-(j0[0] * h1[0] + j0[1] * h1[1] + j0[2] * h1[2]));
+                    out[32] = (Real)(1.0 - (j1[0] * h1[0] + j1[1] * h1[1] + j1[2] * h1[2]));
+                    const double4 q = w.qp[p];
+                    const double e0 = j64_at(w, 24, j) - (Jp[0] * q.x + Jp[1] * q.y + Jp[2] * q.z);
+                    const double e1 = j64_at(w, 25, j) - (Jp[3] * q.x + Jp[4] * q.y + Jp[5] * q.z);
 #pragma unroll
-                    for (int i = 0; i < 9; ++i) bacc[i] += (double)(Jc[i] * e0 + Jc[9 + i] * e1);
+                    for (int i = 0; i < 9; ++i) bacc[i] += Jc[i] * e0 + Jc[9 + i] * e1;
                 }
 #pragma unroll
                 for (int i = 0; i < 9; ++i) bacc[i] = wave_sum(bacc[i]);
                 if (lane == 0)
-                    for (int i = 0; i < 9; ++i) w.bc[9 * (size_t)c + i] = bacc[i];
+                    for (int i = 0; i < 9; ++i) {
+                        w.bc[9 * (size_t)c + i] = bacc[i];
+                        xv[9 * c + i] = bacc[i];
+                    }
             }
             __syncthreads();
-            // ---------------- S ----------------
-            for (int b = wave; b < D.n_blocks; b += NW) {
-                const int4 bk = w.blk[b];
-                const int ca = bk.x, cb = bk.y;
-                const int e0 = lane, e1 = lane + 64;
-                const int r0 = e0 / 9, c0 = e0 % 9, r1 = e1 / 9, c1 = e1 % 9;
-                double acc0 = 0.0, acc1 = 0.0;
-                for (int base = bk.z; base < bk.w; base += 16) {
-                    const int m = min(16, bk.w - base);
-                    if (lane < m) {
-                        const int2 pr = w.pairs[base + lane];
-                        const Real* Ji = js + kWinJs * (size_t)pr.x;
-                        const Real* Jj = js + kWinJs * (size_t)pr.y;
-                        Real G00, G01, G10, G11;
-                        if (pr.x == pr.y) {
-                            // an observation with itself carries its share of U*: J~c'(I - G)J~c,
-                            // negated because the accumulated sum is subtracted
-                            G00 = -Ji[30];
-                            G01 = G10 = -Ji[31];
-                            G11 = -Ji[32];
-                        } else {
-                            const Real* Hp = Ji + 24;
-                            const Real* Jp = Jj + 18;
-                            G00 = Hp[0] * Jp[0] + Hp[1] * Jp[1] + Hp[2] * Jp[2];
-                            G01 = Hp[0] * Jp[3] + Hp[1] * Jp[4] + Hp[2] * Jp[5];
-                            G10 = Hp[3] * Jp[0] + Hp[4] * Jp[1] + Hp[5] * Jp[2];
-                            G11 = Hp[3] * Jp[3] + Hp[4] * Jp[4] + Hp[5] * Jp[5];
-                        }
-                        for (int k = 0; k < 9; ++k) {
-                            stage[wave][lane][k] = Ji[k];
-                            stage[wave][lane][9 + k] = Ji[9 + k];
-                            stage[wave][lane][18 + k] = G00 * Jj[k] + G01 * Jj[9 + k];
-                            stage[wave][lane][27 + k] = G10 * Jj[k] + G11 * Jj[9 + k];
-                        }
-                    }
-                    wave_sync();
-                    for (int k = 0; k < m; ++k) {
-                        const Real* A = stage[wave][k];
-                        acc0 += (double)(A[r0] * A[18 + c0] + A[9 + r0] * A[27 + c0]);
-                        if (e1 < 81) acc1 += (double)(A[r1] * A[18 + c1] + A[9 + r1] * A[27 + c1]);
-                    }
-                    wave_sync();
-                }
-                {
-                    const double v = (ca == cb && r0 == c0 ? w.dc2[9 * (size_t)ca + r0] : 0.0) - acc0;
-                    w.S64[(size_t)(9 * ca + r0) * n + 9 * cb + c0] = v;
-                    if (ca != cb) w.S64[(size_t)(9 * cb + c0) * n + 9 * ca + r0] = v;
-                }
-                if (e1 < 81) {
-                    const double v = (ca == cb && r1 == c1 ? w.dc2[9 * (size_t)ca + r1] : 0.0) - acc1;
-                    w.S64[(size_t)(9 * ca + r1) * n + 9 * cb + c1] = v;
-                    if (ca != cb) w.S64[(size_t)(9 * cb + c1) * n + 9 * ca + r1] = v;
-                }
-            }
+            // ---------------- schur + chol ----------------
+            win_schur<Real, F>(w.blk, w.pairs, js, w.dc2, Sg, n, D.n_blocks, &sflag[2],
+                               reinterpret_cast<Real*>(smem + (size_t)wave * win_stage_bytes<Real>()));
             __syncthreads();
-            // ---------------- chol ----------------
-            for (int idx = tid; idx < n * n; idx += NT) Lf[idx] = (F)w.S64[idx];
-            if (tid == 0) sflag[1] = 0;
-            __syncthreads();
-            for (int k0 = 0; k0 < n; k0 += kWinTile) {
-                const int kb = min(kWinTile, n - k0);
-                if (wave == 0) {
-                    for (int j = k0; j < k0 + kb; ++j) {
-                        F dj = Lf[(size_t)j * n + j];
-                        if (!(dj > (F)0)) {
-                            if (lane == 0) sflag[1] = 1;
-                            dj = (F)1;
-                        }
-                        dj = sqrt(dj);
-                        wave_sync();
-                        if (lane == 0) Lf[(size_t)j * n + j] = dj;
-                        for (int i = j + 1 + lane; i < n; i += 64) Lf[(size_t)i * n + j] /= dj;
-                        wave_sync();
-                        for (int c = j + 1; c < k0 + kb; ++c) {
-                            const F lc = Lf[(size_t)c * n + j];
-                            for (int i = c + lane; i < n; i += 64) Lf[(size_t)i * n + c] -= Lf[(size_t)i * n + j] * lc;
-                        }
-                        wave_sync();
-                    }
+            // factor; in mixed precision a failed FP32 factorization is retried with a diagonal shift
+            // (the factor is only the preconditioner of the FP64 CG below)
+            for (int attempt = 0;; ++attempt) {
+                F shift = (F)0;
+                if (attempt > 0) {
+                    double dmax = 0.0;
+                    for (int k = 0; k < n; ++k) dmax = fmax(dmax, (double)Sg[(size_t)k * n + k]);
+                    shift = (F)(dmax * 1e-6 * pow(100.0, attempt - 1));
                 }
+                for (int idx = tid; idx < n * n; idx += NT) S[idx] = Sg[idx];
+                if (tid == 0) sflag[1] = 0;
                 __syncthreads();
-                const int m0 = k0 + kb, nt = n - m0;
-                for (int idx = tid; idx < nt * nt; idx += NT) {
-                    const int r = m0 + idx / nt, c = m0 + idx % nt;
-                    if (c <= r) {
-                        F s = 0;
-                        for (int mm = 0; mm < kb; ++mm) s += Lf[(size_t)r * n + k0 + mm] * Lf[(size_t)c * n + k0 + mm];
-                        Lf[(size_t)r * n + c] -= s;
-                    }
-                }
+                if (attempt > 0)
+                    for (int k = tid; k < n; k += NT) S[(size_t)k * n + k] += shift;
                 __syncthreads();
+                win_cholesky<F, NT>(S, n, colv, dinv, &sflag[1]);
+                if (sflag[1] == 0 || refine == 0 || attempt == 3) break;
             }
             if (sflag[1] == 0) {
-                // y = S^-1 b (wave 0), then FP64 refinement against S64
-                for (int pass = 0; pass <= refine; ++pass) {
-                    if (pass > 0) {
-                        for (int i = tid; i < n; i += NT) {
-                            double s = w.bc[i];
-                            for (int k = 0; k < n; ++k) s -= w.S64[(size_t)i * n + k] * w.yc[k];
-                            rsh[i] = s;
-                        }
-                        __syncthreads();
-                    }
-                    if (wave == 0) {
-                        double yv[kWinNS];
-#pragma unroll
-                        for (int s = 0; s < kWinNS; ++s) {
-                            const int i = 64 * s + lane;
-                            yv[s] = i < n ? (pass == 0 ? w.bc[i] : rsh[i]) : 0.0;
-                        }
-                        wave_trsv<F, kWinNS>(Lf, n, yv, true);
-                        wave_trsv<F, kWinNS>(Lf, n, yv, false);
-#pragma unroll
-                        for (int s = 0; s < kWinNS; ++s) {
-                            const int i = 64 * s + lane;
-                            if (i < n) w.yc[i] = pass == 0 ? yv[s] : w.yc[i] + yv[s];
-                        }
-                    }
+                // ---------------- solve ----------------
+                win_chol_solve<F, NT>(S, dinv, n, xv);  // x0 = M^-1 b
+                if (refine > 0) {
+                    // preconditioned CG on the FP64 operator with M = the factor: converges to the FP64
+                    // step also where cond(S) exceeds what an FP32 factor resolves (large trust radii)
+                    for (int k = tid; k < n; k += NT) zv[k] = w.bc[k];
                     __syncthreads();
+                    const double bb = win_dot(zv, zv, n, &sdot);
+                    win_apply_S<NT>(w, K, P, xv, qv);
+                    for (int k = tid; k < n; k += NT) rv[k] = zv[k] - qv[k];
+                    __syncthreads();
+                    double rr = win_dot(rv, rv, n, &sdot);
+                    if (rr > kWinCgTol * kWinCgTol * bb) {
+                        for (int k = tid; k < n; k += NT) zv[k] = rv[k];
+                        __syncthreads();
+                        win_chol_solve<F, NT>(S, dinv, n, zv);
+                        for (int k = tid; k < n; k += NT) dv[k] = zv[k];
+                        __syncthreads();
+                        double rz = win_dot(rv, zv, n, &sdot);
+                        for (int cg = 0; cg < refine; ++cg) {
+                            win_apply_S<NT>(w, K, P, dv, qv);
+                            const double dq = win_dot(dv, qv, n, &sdot);
+                            if (!(dq > 0.0)) break;
+                            const double alpha = rz / dq;
+                            for (int k = tid; k < n; k += NT) {
+                                xv[k] += alpha * dv[k];
+                                rv[k] -= alpha * qv[k];
+                            }
+                            __syncthreads();
+                            rr = win_dot(rv, rv, n, &sdot);
+                            if (rr <= kWinCgTol * kWinCgTol * bb) break;
+                            for (int k = tid; k < n; k += NT) zv[k] = rv[k];
+                            __syncthreads();
+                            win_chol_solve<F, NT>(S, dinv, n, zv);
+                            const double rz_new = win_dot(rv, zv, n, &sdot);
+                            const double beta = rz_new / rz;
+                            rz = rz_new;
+                            for (int k = tid; k < n; k += NT) dv[k] = zv[k] + beta * dv[k];
+                            __syncthreads();
+                        }
+                    }
                 }
             }
         }
         const bool have_step = !invalid_blocks && sflag[1] == 0;
         __syncthreads();
         // ---------------- back-substitution and candidate ----------------
-        double nrm[4] = {0.0, 0.0, 0.0, 0.0};  // |dx|^2, |x|^2, model, cost
+        // |dx|^2, |x|^2, model, cost: per-camera partials (cpart) and one wave over the points, summed
+        // in a fixed order so the result does not depend on the workgroup size
+        double nrm[2] = {0.0, 0.0};
         if (have_step) {
             for (int c = wave; c < K; c += NW) {
-                double y[9];
-                for (int i = 0; i < 9; ++i) y[i] = w.yc[9 * (size_t)c + i];
+                double scc[9], y[9];
+                for (int i = 0; i < 9; ++i) {
+                    scc[i] = w.sc[9 * (size_t)c + i];
+                    y[i] = xv[9 * c + i];
+                }
                 for (int j = w.cam_off[c] + lane; j < w.cam_off[c + 1]; j += 64) {
-                    const Real* rec = js + kWinJs * (size_t)j;
-                    Real a0 = rec[33], a1 = rec[34];
+                    const int p = w.co_pt[j];
+                    double Jc[18], Jp[6];
+                    win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
+                    double a0 = j64_at(w, 24, j), a1 = j64_at(w, 25, j);
 #pragma unroll
                     for (int i = 0; i < 9; ++i) {
-                        a0 -= rec[i] * (Real)y[i];
-                        a1 -= rec[9 + i] * (Real)y[i];
+                        a0 -= Jc[i] * y[i];
+                        a1 -= Jc[9 + i] * y[i];
                     }
-                    R4<Real> uu;
-                    uu.x = rec[18] * a0 + rec[21] * a1;
-                    uu.y = rec[19] * a0 + rec[22] * a1;
-                    uu.z = rec[20] * a0 + rec[23] * a1;
-                    uu.w = (Real)0;
-                    u[w.slot_of[j]] = uu;
+                    pv_at(w, 0, w.slot_of[j]) = make_double4(Jp[0] * a0 + Jp[3] * a1, Jp[1] * a0 + Jp[4] * a1,
+                                                             Jp[2] * a0 + Jp[5] * a1, 0.0);
                 }
             }
             __syncthreads();
             for (int p = tid; p < P; p += NT) {
-                const int ch = p >> 6, l = p & 63, len = w.chunk_len[ch];
-                const size_t base = (size_t)w.chunk_off[ch];
-                double t[3] = {0.0, 0.0, 0.0};
-                for (int r = 0; r < len; ++r) {
-                    const R4<Real> uu = u[base + 64 * (size_t)r + l];
-                    t[0] += (double)uu.x;
-                    t[1] += (double)uu.y;
-                    t[2] += (double)uu.z;
-                }
-                double y[3];
+                double t[3], y[3];
+                win_slot_sum3(w, p, t);
                 sym3_mul<double>(w.Vinv + 6 * (size_t)p, t, y);
                 const double4 x4 = w.pt_x[p];
                 const double xs[3] = {x4.x, x4.y, x4.z};
                 double cand[3];
                 for (int i = 0; i < 3; ++i) {
                     w.yp[3 * (size_t)p + i] = y[i];
-                    const double dl = -w.sp[3 * (size_t)p + i] * y[i];
-                    cand[i] = xs[i] + dl;
-                    nrm[0] += dl * dl;
-                    nrm[1] += xs[i] * xs[i];
+                    cand[i] = xs[i] - w.sp[3 * (size_t)p + i] * y[i];
                 }
                 w.pt_c[p] = make_double4(cand[0], cand[1], cand[2], 0.0);
             }
-            for (int k = tid; k < n; k += NT) {
-                const double dl = -w.sc[k] * w.yc[k];
-                const double x = w.cam_x[k];
-                w.cam_c[k] = x + dl;
-                nrm[0] += dl * dl;
-                nrm[1] += x * x;
-            }
+            for (int k = tid; k < n; k += NT) w.cam_c[k] = w.cam_x[k] - w.sc[k] * xv[k];
             __syncthreads();
+            if (wave == 0) {
+                for (int p = lane; p < P; p += 64) {
+                    const double4 x4 = w.pt_x[p];
+                    const double xs[3] = {x4.x, x4.y, x4.z};
+                    for (int i = 0; i < 3; ++i) {
+                        const double dl = -w.sp[3 * (size_t)p + i] * w.yp[3 * (size_t)p + i];
+                        nrm[0] += dl * dl;
+                        nrm[1] += xs[i] * xs[i];
+                    }
+                }
+                nrm[0] = wave_sum(nrm[0]);
+                nrm[1] = wave_sum(nrm[1]);
+            }
             for (int c = tid; c < K; c += NT) {
                 CameraPrep<double> Pc;
                 camera_prep<double>(w.cam_c + 9 * (size_t)c, Pc);
-                prep_c[c] = Pc;
+                w.prep_c[c] = Pc;
             }
             __syncthreads();
-            // model cost change from the stored J~ (step s = -y) and the FP64 candidate cost
+            // model cost change -(J~s)'(r~ + J~s/2) from j64 (step s = -y) and the FP64 candidate cost
             for (int c = wave; c < K; c += NW) {
-                const CameraPrep<double> Pc = prep_c[c];
-                double y[9];
-                for (int i = 0; i < 9; ++i) y[i] = w.yc[9 * (size_t)c + i];
+                double scc[9], y[9];
+                for (int i = 0; i < 9; ++i) {
+                    scc[i] = w.sc[9 * (size_t)c + i];
+                    y[i] = xv[9 * c + i];
+                }
+                double cm[4] = {0.0, 0.0, 0.0, 0.0};
+                if (lane < 9) {
+                    const int k = 9 * c + lane;
+                    const double dl = -w.sc[k] * xv[k], x = w.cam_x[k];
+                    cm[0] = dl * dl;
+                    cm[1] = x * x;
+                }
                 for (int j = w.cam_off[c] + lane; j < w.cam_off[c + 1]; j += 64) {
-                    const Real* rec = js + kWinJs * (size_t)j;
                     const int p = w.co_pt[j];
+                    double Jc[18], Jp[6];
+                    win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
                     const double* yp = w.yp + 3 * (size_t)p;
                     double m0 = 0.0, m1 = 0.0;
                     for (int i = 0; i < 9; ++i) {
-                        m0 -= (double)rec[i] * y[i];
-                        m1 -= (double)rec[9 + i] * y[i];
+                        m0 -= Jc[i] * y[i];
+                        m1 -= Jc[9 + i] * y[i];
                     }
                     for (int i = 0; i < 3; ++i) {
-                        m0 -= (double)rec[18 + i] * yp[i];
-                        m1 -= (double)rec[21 + i] * yp[i];
+                        m0 -= Jp[i] * yp[i];
+                        m1 -= Jp[3 + i] * yp[i];
                     }
-                    nrm[2] += m0 * ((double)rec[33] + 0.5 * m0) + m1 * ((double)rec[34] + 0.5 * m1);
+                    cm[2] += m0 * (j64_at(w, 24, j) + 0.5 * m0) + m1 * (j64_at(w, 25, j) + 0.5 * m1);
                     const double4 x4 = w.pt_c[p];
                     const double X[3] = {x4.x, x4.y, x4.z};
                     double r[2];
-                    snavely_residual<double>(Pc, X, w.co_uv64[2 * (size_t)j], w.co_uv64[2 * (size_t)j + 1], r);
-                    nrm[3] += half_rho(r[0] * r[0] + r[1] * r[1], o.loss, o.loss_a);
+                    snavely_residual<double>(w.prep_c[c], X, w.co_uv64[2 * (size_t)j], w.co_uv64[2 * (size_t)j + 1], r);
+                    cm[3] += half_rho(r[0] * r[0] + r[1] * r[1], o.loss, o.loss_a);
                 }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) cm[i] = wave_sum(cm[i]);
+                if (lane == 0)
+                    for (int i = 0; i < 4; ++i) cpart[c][i] = cm[i];
             }
         }
-        wg_sum<4, NT>(nrm, red);
+        __syncthreads();
         // ---------------- decide ----------------
         if (tid == 0) {
             sflag[1] = 0;
-            const double model = -red[2 * NW];
-            double cand = red[3 * NW];
-            const double dx2 = red[0], x2 = red[NW];
+            double tot[4] = {nrm[0], nrm[1], 0.0, 0.0};  // wave 0 holds the point sums
+            if (have_step)
+                for (int c = 0; c < K; ++c)
+                    for (int i = 0; i < 4; ++i) tot[i] += cpart[c][i];
+            const double model = -tot[2];
+            double cand = tot[3];
+            const double dx2 = tot[0], x2 = tot[1];
             st->iteration++;
             const bool valid = have_step && model > 0.0 && isfinite(dx2);
             bool record = true;

@@ -88,12 +88,20 @@ def test_alone_and_in_mix_bitwise_identical(precision):
     assert s_alone == s_mix
     np.testing.assert_array_equal(c_alone, c_mix)
     np.testing.assert_array_equal(p_alone, p_mix)
-    # both workgroup sizes give the same bits too (the reduction order is per wave and per block)
+    # every workgroup size that fits gives the same bits too (the reduction order is per wave and
+    # per block)
+    sizes = 0
     for t in (256, 512):
-        mix.set_block(t)
+        try:
+            mix.set_block(t)
+        except S.BAError as e:
+            assert e.status == 6  # BA_ERR_CAPACITY
+            continue
+        sizes += 1
         mix.reset()
         assert mix.run()[3] == s_alone
         np.testing.assert_array_equal(mix.download(3)[1], p_alone)
+    assert sizes >= 1
 
 
 def test_reset_rerun_and_empty_window():

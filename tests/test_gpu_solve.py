@@ -131,3 +131,25 @@ def test_random_walk_dense_mixed_vs_fp64(oracle_mod):
     assert (np.abs(c64[:n] - cref[:n]) / cref[:n]).max() < 1e-10
     assert abs(o64["final_cost"] - ref["final_cost"]) / ref["final_cost"] < 1e-9
     assert abs(o32["final_cost"] - ref["final_cost"]) / ref["final_cost"] < 1e-5
+
+
+@pytest.mark.parametrize("solver", [1, 2])
+def test_graph_replay_matches_stream_launches(solver):
+    """Each LM iteration replayed from one captured hipGraph gives bitwise the results of plain
+    stream launches, on every replay (eager work and host synchronisation between replays)."""
+    p = S.synthetic(1, seed=27)
+    kw = dict(max_num_iterations=8, linear_solver=solver, precision=1, **FIXED)
+    eager = S.Solver(p, launch_mode=1, **kw)
+    a = eager.run()
+    ca, pa = eager.download()
+    g = S.Solver(p, launch_mode=0, **kw)
+    for _ in range(3):
+        g.reset()
+        b = g.run()
+        cb, pb = g.download()
+        assert [r["cost"] for r in a["trace"]] == [r["cost"] for r in b["trace"]]
+        np.testing.assert_array_equal(ca, cb)
+        np.testing.assert_array_equal(pa, pb)
+        S.eval_cost(p)  # eager work on the device between replays
+    eager.close()
+    g.close()

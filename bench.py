@@ -31,16 +31,23 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 LM_ITERS = 30
+# HBM bytes per launch of the PCG kernels at this workload (mixed precision), measured with
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by a 1 GiB copy's counts
+# (profiles/c3_mixed_pcg_kernels.md)
+MEASURED_TRAFFIC_BYTES = {"jz": 20.77e6, "jtx": 42.21e6, "pt_z": 37.04e6}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+LAUNCH_MODE = 0  # 0: each LM iteration replayed as a hipGraph; 1: stream launches (profiler runs)
+
+
 def solver_options(S, precision):
     return S.default_options(precision=precision, linear_solver=2, max_num_iterations=LM_ITERS,
                              pcg_max_iterations=50, pcg_tolerance=1e-6, function_tolerance=0.0,
-                             gradient_tolerance=0.0, parameter_tolerance=0.0)
+                             gradient_tolerance=0.0, parameter_tolerance=0.0, launch_mode=LAUNCH_MODE)
 
 
 def time_solver(S, p, precision, steps, warmup):
@@ -86,7 +93,8 @@ WINDOW_ITERS = 10
 
 def window_options(S, precision):
     return S.default_options(precision=precision, linear_solver=1, max_num_iterations=WINDOW_ITERS,
-                             function_tolerance=0.0, gradient_tolerance=0.0, parameter_tolerance=0.0)
+                             function_tolerance=0.0, gradient_tolerance=0.0, parameter_tolerance=0.0,
+                             launch_mode=LAUNCH_MODE)
 
 
 def bench_batch(S, seed, steps, warmup):
@@ -188,7 +196,8 @@ def bench_c3(S, args):
         avg_s = v["total_ms"] / v["launches"] / 1e3
         achieved = v["bytes_per_launch"] / avg_s / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC_BYTES.get(dom), "kernel": dom,
+                    "algorithmic_bytes": v["bytes_per_launch"],
                     "avg_launch_us": avg_s * 1e6, "launches_per_solve": v["launches"]}
     res["roofline"] = roofline
     res["kernel_ms_per_solve"] = {k: round(v["total_ms"], 3)
@@ -200,6 +209,19 @@ def bench_c3(S, args):
     res.update(final_cost=out_mixed["final_cost"], initial_cost=out_mixed["initial_cost"],
                pcg_iterations=out_mixed["linear_solver_iterations"])
     return res, sm, p
+
+
+def calibrate_copy():
+    """A plain device-to-device copy of 1 GiB (torch's elementwise copy kernel), five times: under
+    rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch calibrate the counters to bytes."""
+    import torch
+    n = (1 << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("bench: copied 5 x %d bytes" % (4 * n))
 
 
 def main():
@@ -214,7 +236,17 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--launch-mode", type=int, default=0, choices=(0, 1),
+                    help="1: plain stream launches instead of replayed hipGraphs (for profiler runs)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a device copy of 1 GiB five times (the byte-counter calibration of the "
+                         "HBM traffic figures) and exit")
     args = ap.parse_args()
+    global LAUNCH_MODE
+    LAUNCH_MODE = args.launch_mode
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; a multi-GPU figure is not measured "
               "(independent windows shard by window, a large BA by points -- see DESIGN.md)", file=sys.stderr)

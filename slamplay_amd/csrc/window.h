@@ -1,4 +1,4 @@
-// window.h -- device layout of batched local-BA windows (one workgroup per window).
+// window.h -- device layout of batched local-BA windows.
 //
 // Every window lives in one contiguous region of a device arena; WinDesc holds the absolute
 // byte offsets of its arrays.  Within a window:
@@ -7,7 +7,9 @@
 //     slot records pv are three double4 arrays of n_slots entries (component-major);
 //   * the 9x9 blocks (a <= b) of the reduced camera system with their observation pairs
 //     (i, j) (camera(i) = a, camera(j) = b, same point), built once on the host and listed with
-//     the largest pair count first (the order in which workgroup waves pick them up).
+//     the largest pair count first;
+//   * per-camera and per-chunk partial sums (cpart, ppart) through which the phase kernels hand
+//     results to the per-window kernels, which combine them in a fixed order.
 #pragma once
 
 #include <cstdint>
@@ -20,14 +22,17 @@ struct WinState {
     double radius, decrease_factor, x_cost, initial_cost, gmax;
     int iteration, consecutive_invalid, done, termination;
     int need_lin, first_lin, num_successful, num_unsuccessful;
-    int rows, pad;  // trace rows of the equivalent single-problem solve (iteration 0 included)
+    int rows;       // trace rows of the equivalent single-problem solve (iteration 0 included)
+    int invalid;    // this iteration's damped point blocks were not all positive definite
+    int have_step;  // this iteration produced a step
+    int pad;
 };
 
 struct WinDesc {
     int K, P, No, n_chunks, n_slots, n_blocks, n_pairs, pad;
     int64_t o_cam_off, o_co_pt, o_co_uv64, o_slot_of, o_chunk_len, o_chunk_off, o_blk, o_pairs;
     int64_t o_cam_x, o_cam_x0, o_cam_c, o_pt_x, o_pt_x0, o_pt_c;
-    int64_t o_prep, o_prep_c, o_udiag, o_gc, o_sc, o_dc2, o_bc;
+    int64_t o_prep, o_prep_c, o_udiag, o_gc, o_sc, o_dc2, o_bc, o_yc, o_cpart, o_ppart;
     int64_t o_V, o_gp, o_sp, o_Vinv, o_qp, o_yp, o_pv, o_j64, o_js, o_S, o_state;
 };
 
@@ -35,11 +40,15 @@ struct WinDesc {
 //   j64 (FP64, 26 values, stored component-major j64[e][j]): corrected J_c (2x9, row-major),
 //       J_p (2x3), r (2) at the linearization point, unscaled -- written once per accepted step;
 //       the FP64 operator of the LM system;
-//   js (working precision, 36 values): scaled J~_c (18), J~_p (6), H = J~_p V*^-1 (6),
+//   js (working precision, 36 values): scaled J~_c column by column (18: the two rows of column c
+//       at 2c, 2c + 1), J~_p (6, row-major), H = J~_p V*^-1 (6, row-major),
 //       I - G = I - J~_p V*^-1 J~_p' (3: g00, g01, g11), unused (3) -- the input of the reduced
 //       camera system, rewritten whenever the damping changes.
 constexpr int kWinJ64 = 26;
 constexpr int kWinJs = 36;
+// Partial sums per camera (cpart) and per point chunk (ppart).
+enum WinCamPart { WC_COST = 0, WC_GMAX, WC_MODEL, WC_CAND, WC_DX2, WC_X2, WC_COUNT = 8 };
+enum WinPtPart { WP_GMAX = 0, WP_DX2, WP_X2, WP_COUNT = 4 };
 
 template <typename T>
 __host__ __device__ __forceinline__ T* at(char* base, int64_t off) {

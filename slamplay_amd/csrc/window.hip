@@ -2,8 +2,10 @@
 //
 // Windows (small BA problems: the local BA that full_slam/src/backend.cpp:67-199 runs over its
 // 7-10 active keyframes, here with the Snavely camera of the BA path) are packed into regions of
-// one device arena and solved by k_window_lm, one workgroup per window.  Windows whose reduced
-// camera system does not fit the workgroup's LDS are solved by the single-problem solver.
+// one device arena; one LM iteration of every active window is the fixed sequence of phase
+// kernels of window_kernels.h (captured once and replayed as a hipGraph for a fixed batch).
+// Windows whose reduced camera system does not fit a workgroup's LDS are solved by the
+// single-problem solver.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -61,29 +63,26 @@ Opts opts_of(const ba_options& o) {
     return r;
 }
 
+// Static LDS of k_wsolve (xv, rv, zv, dv, qv, colv, dinv, sdot, bad) plus slack for alignment.
 template <typename Real>
-int64_t static_lds(int nt) {
-    const int nw = nt / 64;
-    // red, xv, rv, zv, dv, qv, sdot, colv, dinv, cpart, sflag of k_window_lm, plus slack for the
-    // compiler's alignment
-    return (int64_t)nw * 8 + 5 * kWinMaxN * 8 + 8 + 2 * kWinMaxN * sizeof(Real) + kWinMaxK * 4 * 8 + 16 + 256;
+int64_t solve_static_lds() {
+    return 5 * kWinMaxN * 8 + 2 * kWinMaxN * sizeof(Real) + 16 + 256;
 }
 
-// Dynamic LDS of k_window_lm for reduced systems up to n x n: the factor, or the per-wave
-// staging areas of the Schur phase if larger.
+// Largest reduced system (9 K) k_wsolve can factor in LDS (0: none).
 template <typename Real>
-int64_t dynamic_lds(int nt, int n) {
-    return std::max<int64_t>((int64_t)n * n * sizeof(Real), (int64_t)(nt / 64) * win_stage_bytes<Real>());
-}
-
-// Largest reduced system (9 K) a workgroup of nt threads can factor in LDS (0: none fits).
-template <typename Real>
-int max_window_n(int nt) {
-    const int64_t room = lds_per_block() - static_lds<Real>(nt);
-    if ((int64_t)(nt / 64) * win_stage_bytes<Real>() > room) return 0;
+int max_window_n() {
+    const int64_t room = lds_per_block() - solve_static_lds<Real>();
+    if (room <= 0) return 0;
     int n = (int)std::floor(std::sqrt((double)room / sizeof(Real)));
     n = std::min(n, kWinMaxN);
     return n - n % 9;
+}
+
+// Dynamic LDS of k_wschur: one staging area per wave.
+template <typename Real>
+int64_t schur_lds(int nt) {
+    return (int64_t)(nt / 64) * win_stage_bytes<Real>();
 }
 
 }  // namespace
@@ -143,6 +142,9 @@ int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowIm
     D.o_sc = take(72 * (int64_t)K);
     D.o_dc2 = take(72 * (int64_t)K);
     D.o_bc = take(72 * (int64_t)K);
+    D.o_yc = take(72 * (int64_t)K);
+    D.o_cpart = take(8 * WC_COUNT * (int64_t)K);
+    D.o_ppart = take(8 * WP_COUNT * (int64_t)nch);
     D.o_V = take(48 * (int64_t)P);
     D.o_gp = take(24 * (int64_t)P);
     D.o_sp = take(24 * (int64_t)P);
@@ -258,11 +260,19 @@ struct WindowEngine {
     DevBuf<int> dwork;
     DevBuf<WinState> dstates;
     std::vector<WinState> hstates;
-    int threads = 256;
+    int threads = 256;  // block size of k_wschur and k_wsolve
     int max_slots = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // the current work set: task lists of the phase kernels (windows in dispatch order)
+    std::vector<int> work_slots;
+    DevBuf<int2> ctask, ptask;
+    DevBuf<int4> stask;
+    DevBuf<int> wtask;
+    int n_ctask = 0, n_ptask = 0, n_stask = 0, n_wtask = 0, work_nmax = 9, work_threads = 0;
+    hipGraphExec_t graph = nullptr;  // one LM iteration of the current work set
 
     ~WindowEngine() {
+        if (graph) (void)hipGraphExecDestroy(graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (s) (void)hipStreamDestroy(s);
@@ -302,35 +312,123 @@ struct WindowEngine {
         BA_HIP(hipMemcpyAsync(arena.p + base, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice, s));
         hdesc[slot] = rebase(img.desc, base);
         BA_HIP(hipMemcpyAsync(ddesc.p + slot, &hdesc[slot], sizeof(WinDesc), hipMemcpyHostToDevice, s));
+        work_slots.clear();  // the task lists depend on the windows' sizes: rebuild them
         return BA_OK;
     }
-
-    int max_n() const { return max_window_n<Real>(threads); }
 
     // Mixed precision: the preconditioned CG of a window step runs up to 10 iterations per
     // refinement step of the options (20 by default); FP64 windows solve directly.
     int cg_iterations() const { return sizeof(Real) == sizeof(double) ? 0 : 10 * std::max(opt.refine_steps, 1); }
 
-    template <int NT>
-    int launch_nt(int n_work, int iters, int nmax) {
-        const size_t shm = (size_t)dynamic_lds<Real>(NT, nmax);
-        BA_HIP(hipFuncSetAttribute((const void*)k_window_lm<Real, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)shm));
-        hipLaunchKernelGGL((k_window_lm<Real, NT>), dim3(n_work), dim3(NT), shm, s, (const WinDesc*)ddesc.p, arena.p,
-                           (const int*)dwork.p, n_work, iters, o, cg_iterations());
-        BA_HIP(hipGetLastError());
+    template <typename T>
+    int upload(DevBuf<T>& buf, const std::vector<T>& v) {
+        if (v.empty()) return BA_OK;
+        if (buf.n < v.size()) BA_TRY(buf.alloc(v.size()));
+        BA_HIP(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
         return BA_OK;
     }
 
-    // Runs `iters` LM iterations of the windows in `slots` (dispatched in the given order).
-    int run(const std::vector<int>& slots, int iters) {
+    // Task lists of the phase kernels for the windows in `slots` (in this dispatch order).
+    int set_work(const std::vector<int>& slots) {
+        if (slots == work_slots && work_threads == threads) return BA_OK;
+        if (graph) {
+            (void)hipGraphExecDestroy(graph);
+            graph = nullptr;
+        }
+        std::vector<int2> ct, pt;
+        std::vector<int4> stk;
+        const int nw = threads / 64, group = 2 * nw;  // about two blocks per wave of k_wschur
+        work_nmax = 9;
+        for (int sl : slots) {
+            const WinDesc& D = hdesc[sl];
+            work_nmax = std::max(work_nmax, 9 * D.K);
+            for (int c = 0; c < D.K; ++c) ct.push_back(make_int2(sl, c));
+            for (int ch = 0; ch < D.n_chunks; ++ch) pt.push_back(make_int2(sl, ch));
+            const int G = std::max(1, (D.n_blocks + group - 1) / group);
+            for (int g = 0; g < G && g < D.n_blocks; ++g)
+                stk.push_back(make_int4(sl, g, G, (D.n_blocks - g + G - 1) / G));
+        }
+        BA_TRY(upload(ctask, ct));
+        BA_TRY(upload(ptask, pt));
+        BA_TRY(upload(stask, stk));
+        BA_TRY(upload(wtask, slots));
+        BA_HIP(hipStreamSynchronize(s));
+        n_ctask = (int)ct.size();
+        n_ptask = (int)pt.size();
+        n_stask = (int)stk.size();
+        n_wtask = (int)slots.size();
+        work_slots = slots;
+        work_threads = threads;
+        return BA_OK;
+    }
+
+    template <int NT>
+    int enqueue_iteration_nt() {
+        const WinDesc* dd = ddesc.p;
+        if (n_ctask) hipLaunchKernelGGL((k_wlin_cam<kWinCamThreads>), dim3(n_ctask), dim3(kWinCamThreads), 0, s, dd,
+                                        arena.p, (const int2*)ctask.p, n_ctask, o);
+        if (n_ptask) hipLaunchKernelGGL(k_wlin_pt, dim3(n_ptask), dim3(64), 0, s, dd, arena.p, (const int2*)ptask.p,
+                                        n_ptask, o);
+        hipLaunchKernelGGL((k_wdamp<256>), dim3(n_wtask), dim3(256), 0, s, dd, arena.p, (const int*)wtask.p, n_wtask, o);
+        if (n_ctask) hipLaunchKernelGGL((k_wrec_cam<Real, kWinCamThreads>), dim3(n_ctask), dim3(kWinCamThreads), 0, s,
+                                        dd, arena.p, (const int2*)ctask.p, n_ctask);
+        if (n_stask)
+            hipLaunchKernelGGL((k_wschur<Real, NT>), dim3(n_stask), dim3(NT), (size_t)schur_lds<Real>(NT), s, dd, arena.p,
+                               (const int4*)stask.p, n_stask);
+        hipLaunchKernelGGL((k_wsolve<Real, NT>), dim3(n_wtask), dim3(NT), (size_t)work_nmax * work_nmax * sizeof(Real), s,
+                           dd, arena.p, (const int*)wtask.p, n_wtask, cg_iterations());
+        if (n_ctask) hipLaunchKernelGGL((k_wback_cam<kWinCamThreads>), dim3(n_ctask), dim3(kWinCamThreads), 0, s, dd,
+                                        arena.p, (const int2*)ctask.p, n_ctask);
+        if (n_ptask) hipLaunchKernelGGL(k_wback_pt, dim3(n_ptask), dim3(64), 0, s, dd, arena.p, (const int2*)ptask.p,
+                                        n_ptask);
+        if (n_ctask) hipLaunchKernelGGL((k_wcand_cam<kWinCamThreads>), dim3(n_ctask), dim3(kWinCamThreads), 0, s, dd,
+                                        arena.p, (const int2*)ctask.p, n_ctask, o);
+        hipLaunchKernelGGL((k_wdecide<256>), dim3(n_wtask), dim3(256), 0, s, dd, arena.p, (const int*)wtask.p, n_wtask,
+                           o);
+        return BA_OK;
+    }
+
+    template <int NT>
+    int set_lds_limits() {
+        BA_HIP(hipFuncSetAttribute((const void*)k_wschur<Real, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)schur_lds<Real>(NT)));
+        BA_HIP(hipFuncSetAttribute((const void*)k_wsolve<Real, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(work_nmax * work_nmax * sizeof(Real))));
+        return BA_OK;
+    }
+
+    int enqueue_iteration() {
+        if (threads == 512) return enqueue_iteration_nt<512>();
+        return enqueue_iteration_nt<256>();
+    }
+
+    // Runs `iters` LM iterations of the windows in `slots` (dispatched in the given order); with
+    // use_graph the iteration is captured once per work set and replayed.
+    int run(const std::vector<int>& slots, int iters, bool use_graph = false) {
         if (slots.empty()) return BA_OK;
-        int nmax = 9;
-        for (int sl : slots) nmax = std::max(nmax, 9 * hdesc[sl].K);
-        BA_HIP(hipMemcpyAsync(dwork.p, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice, s));
-        const int n_work = (int)slots.size();
-        if (threads == 512) return launch_nt<512>(n_work, iters, nmax);
-        return launch_nt<256>(n_work, iters, nmax);
+        BA_TRY(set_work(slots));
+        BA_TRY(threads == 512 ? set_lds_limits<512>() : set_lds_limits<256>());
+        if (use_graph && !graph) {
+            hipGraph_t g = nullptr;
+            BA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            BA_TRY(enqueue_iteration());
+            const hipError_t e = hipStreamEndCapture(s, &g);
+            if (e != hipSuccess) return fail(BA_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+            const hipError_t e2 = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e2 != hipSuccess) {
+                graph = nullptr;
+                return fail(BA_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e2));
+            }
+        }
+        for (int it = 0; it < iters; ++it) {
+            if (use_graph)
+                BA_HIP(hipGraphLaunch(graph, s));
+            else
+                BA_TRY(enqueue_iteration());
+            BA_HIP(hipGetLastError());
+        }
+        return BA_OK;
     }
 
     int reset(const std::vector<int>& slots) {
@@ -425,7 +523,7 @@ struct Batch final : BatchBase {
         int64_t total = 0;
         for (int w = 0; w < count; ++w) {
             BA_TRY(validate_view(&views[w], true));
-            if (9 * views[w].num_cameras > max_window_n<Real>(256)) {
+            if (9 * views[w].num_cameras > max_window_n<Real>()) {
                 large_index[w] = (int)large_solvers.size();
                 ba_options lo = o;
                 lo.linear_solver = BA_SOLVER_DENSE_SCHUR;
@@ -461,7 +559,7 @@ struct Batch final : BatchBase {
         float best = 1e30f;
         int best_t = 256;
         for (int t : {256, 512}) {
-            if (nmax > max_window_n<Real>(t)) continue;
+            if (schur_lds<Real>(t) + 64 > lds_per_block()) continue;
             eng.threads = t;
             float ms = 0.f;
             BA_TRY(eng.time_iteration(order, &ms));
@@ -479,7 +577,9 @@ struct Batch final : BatchBase {
         if (t != 256 && t != 512) return fail(BA_ERR_INVALID_ARGUMENT, "threads must be 256 or 512");
         int nmax = 9;
         for (int w : order) nmax = std::max(nmax, 9 * imgs[w].K);
-        if (nmax > max_window_n<Real>(t)) return fail(BA_ERR_CAPACITY, "a window does not fit %d threads' LDS", t);
+        if (schur_lds<Real>(t) + 64 > lds_per_block())
+            return fail(BA_ERR_CAPACITY, "the Schur staging of %d threads does not fit the LDS", t);
+        (void)nmax;
         eng.threads = t;
         return BA_OK;
     }
@@ -491,7 +591,7 @@ struct Batch final : BatchBase {
 
     int run(ba_summary* summaries) override {
         BA_TRY(use_device(opt.device));
-        BA_TRY(eng.run(order, opt.max_num_iterations + 1));
+        BA_TRY(eng.run(order, opt.max_num_iterations + 1, true));
         for (size_t i = 0; i < large.size(); ++i) BA_TRY(large_solvers[i]->run(&large_sum[i]));
         std::vector<WinState> st;
         BA_TRY(eng.states(order, &st));
@@ -676,7 +776,7 @@ struct Server final : ServerBase {
         r->obs.assign(v->observations, v->observations + 2 * (size_t)v->num_observations);
         r->cams.assign(v->cameras, v->cameras + 9 * (size_t)v->num_cameras);
         r->pts.assign(v->points, v->points + 3 * (size_t)v->num_points);
-        if (9 * r->nc > max_window_n<Real>(256)) {
+        if (9 * r->nc > max_window_n<Real>()) {
             {
                 std::lock_guard<std::mutex> lk(mu);
                 big_queue.push_back(std::move(r));

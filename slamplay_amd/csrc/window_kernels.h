@@ -1,33 +1,31 @@
-// window_kernels.h -- batched local bundle adjustment: one workgroup runs whole LM iterations of
-// one window (the dense-Schur solve of ba_solve, restated for windows of a few dozen cameras).
-//
-// Per LM iteration, with workgroup barriers between phases:
-//   lin     (after an accepted step) FP64 camera prep; waves over cameras evaluate the corrected
-//           residuals and Jacobians once, in FP64, into the per-observation j64 records, with
-//           diag(J_c'J_c) and J_c'r per camera (wave reductions) and J_p'J_p, J_p'r per
-//           observation into slot order; threads over points sum their slots;
-//   damp    V*^-1 (FP64), q = V*^-1 g~_p, D_c^2;
-//   records per observation the working-precision inputs of the reduced system (J~_c, J~_p,
-//           H = J~_p V*^-1, I - G) from j64; reduced gradient b_c = sum J~_c'(r~ - J~_p q) in FP64;
-//   schur   each wave takes 9x9 blocks (a, b) of S from a shared counter, largest first, stages
-//           the records of 32 observation pairs at a time in LDS with vector loads, and a lane
-//           accumulates every 32nd pair of the block for rows 0-4 or 5-8 in registers; the 32
-//           partial sums are combined by a butterfly in FP64 and the block is written to S in
-//           global memory, which is then copied into LDS;
-//   chol    right-looking Cholesky of S in LDS, one column at a time over the whole workgroup;
-//   solve   blocked triangular solves (16-row diagonal blocks by one wave, the off-diagonal
-//           updates by the workgroup).  Mixed precision: preconditioned CG in FP64 with the FP32
-//           factor as preconditioner and the FP64 operator of the LM system applied implicitly
-//           through the j64 records (S y = D_c^2 y + sum J~_c'(J~_c y - J~_p z),
-//           z = V*^-1 sum J~_p'J~_c y), until the relative residual is 1e-12: the step is the FP64
-//           step also at trust radii where cond(S) is beyond what an FP32 factor resolves;
-//   back    u = J~_p'(r~ - J~_c y_c) per observation (FP64), y_p = V*^-1 sum u per point,
-//           candidate parameters;
-//   decide  model cost change from j64, FP64 candidate cost, Ceres LM rule.
-// A window's arithmetic and reduction order depend only on the window -- per-camera sums are
-// combined in camera order, per-point sums by one fixed wave, S blocks by one wave each in a fixed
-// lane order -- so its result is bitwise the same whichever other windows share the launch and at
-// either workgroup size.
+// window_kernels.h -- batched local bundle adjustment: the LM iteration of many windows as a
+// fixed sequence of phase kernels (the dense-Schur solve of ba_solve, restated for windows of a
+// few dozen cameras).  Each kernel covers every active window at once:
+//   k_wlin_cam    (window, camera) FP64 residuals and Jacobians of the camera's observations
+//                 into the j64 records, diag(J_c'J_c), J_c'r, J_p'J_p and J_p'r into slot order,
+//                 and at the start the FP64 cost (after an accepted step only);
+//   k_wlin_pt     (window, point chunk) V and g_p from the slots;
+//   k_wdamp       (window) gradient test, Jacobi scales at the start, V*^-1 and q = V*^-1 g~_p
+//                 per point, D_c^2;
+//   k_wrec_cam    (window, camera) working-precision records of the reduced system (J~_c, J~_p,
+//                 H = J~_p V*^-1, I - G) and b_c = sum J~_c'(r~ - J~_p q) in FP64;
+//   k_wschur      (window, block group) 9x9 blocks of S, each by one wave from LDS-staged
+//                 observation pairs;
+//   k_wsolve      (window) S into LDS, right-looking Cholesky, blocked triangular solves; mixed
+//                 precision: preconditioned CG in FP64 on the FP64 operator with the FP32 factor
+//                 as preconditioner (S y = D_c^2 y + sum J~_c'(J~_c y - J~_p z),
+//                 z = V*^-1 sum J~_p'J~_c y), until the relative residual is 1e-8, so the step is
+//                 the FP64 step also where cond(S) is beyond what an FP32 factor resolves;
+//   k_wback_cam   (window, camera) u = J~_p'(r~ - J~_c y_c) into slot order (FP64);
+//   k_wback_pt    (window, point chunk) y_p = V*^-1 sum u, candidate points, |dx|^2, |x|^2;
+//   k_wcand_cam   (window, camera) candidate camera, model cost change from j64, FP64 candidate
+//                 cost;
+//   k_wdecide     (window) Ceres LM rule, accepted candidates copied into the parameters.
+// Every kernel reads the window's LM state first and returns when it has nothing to do (or when its
+// task lies outside the window it names), so the sequence can be replayed as a graph.  Per-camera and per-chunk sums are written to their own
+// slots and combined by the per-window kernels in camera / chunk order, and every 9x9 block of S is
+// reduced by one wave in a fixed order: a window's result does not depend on which other windows
+// share the launch, nor on the block sizes chosen for the launch.
 #pragma once
 
 #include "window.h"
@@ -42,30 +40,18 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int NT>
-__device__ __forceinline__ double wg_max(double v, double* red) {
-    constexpr int NW = NT / 64;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    v = wave_max(v);
-    if (lane == 0) red[wave] = v;
-    __syncthreads();
-    double m = red[0];
-    for (int w = 1; w < NW; ++w) m = fmax(m, red[w]);
-    __syncthreads();
-    return m;
-}
-
 constexpr int kWinMaxN = 288;  // 9 * 32 cameras
 constexpr int kWinMaxK = kWinMaxN / 9;
 constexpr int kWinTb = 16;     // diagonal block of the triangular solves
-
+constexpr int kWinCamThreads = 256;  // block size of the per-camera kernels
 struct WinPtrs {
-    int No, n_slots;
+    int K, P, No, n_slots;
     const int *cam_off, *co_pt, *slot_of, *chunk_len, *chunk_off;
     const int4* blk;
     const int2* pairs;
     const double* co_uv64;
-    double *cam_x, *cam_c, *udiag, *gc, *sc, *dc2, *bc, *V, *gp, *sp, *Vinv, *yp, *j64;
+    double *cam_x, *cam_c, *udiag, *gc, *sc, *dc2, *bc, *yc, *cpart, *ppart;
+    double *V, *gp, *sp, *Vinv, *yp, *j64;
     double4 *pt_x, *pt_c, *pv, *qp;
     CameraPrep<double>*prep, *prep_c;
     WinState* st;
@@ -73,6 +59,8 @@ struct WinPtrs {
 
 __device__ __forceinline__ WinPtrs win_ptrs(const WinDesc& D, char* arena) {
     WinPtrs w;
+    w.K = D.K;
+    w.P = D.P;
     w.No = D.No;
     w.n_slots = D.n_slots;
     w.cam_off = at<int>(arena, D.o_cam_off);
@@ -90,6 +78,9 @@ __device__ __forceinline__ WinPtrs win_ptrs(const WinDesc& D, char* arena) {
     w.sc = at<double>(arena, D.o_sc);
     w.dc2 = at<double>(arena, D.o_dc2);
     w.bc = at<double>(arena, D.o_bc);
+    w.yc = at<double>(arena, D.o_yc);
+    w.cpart = at<double>(arena, D.o_cpart);
+    w.ppart = at<double>(arena, D.o_ppart);
     w.V = at<double>(arena, D.o_V);
     w.gp = at<double>(arena, D.o_gp);
     w.sp = at<double>(arena, D.o_sp);
@@ -104,6 +95,26 @@ __device__ __forceinline__ WinPtrs win_ptrs(const WinDesc& D, char* arena) {
     w.prep_c = at<CameraPrep<double>>(arena, D.o_prep_c);
     w.st = at<WinState>(arena, D.o_state);
     return w;
+}
+
+// Sum of N per-thread values over a workgroup of NT threads in a fixed order (wave butterflies,
+// then the waves in order by thread 0); the totals are returned to thread 0 only.
+template <int N, int NT>
+__device__ __forceinline__ void wg_sum_fixed(double (&v)[N], double* lds) {
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const double s = wave_sum(v[i]);
+        if (lane == 0) lds[i * NW + wave] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 0; i < N; ++i) {
+            double s = lds[i * NW];
+            for (int k = 1; k < NW; ++k) s += lds[i * NW + k];
+            v[i] = s;
+        }
 }
 
 // Component e of the FP64 record of observation j (stored component-major: j64[e][j], so the
@@ -145,15 +156,51 @@ __device__ __forceinline__ void win_slot_sum3(const WinPtrs& w, int p, double t[
 
 constexpr int kWinPairsPerStage = 32;
 
+// Two consecutive record elements, read as one LDS access.
+template <typename Real>
+struct alignas(2 * sizeof(Real)) R2v {
+    Real x, y;
+};
+
 // LDS bytes of the per-wave staging area of win_schur: the records of 32 pairs (i and j).
 template <typename Real>
 constexpr int win_stage_bytes() {
     return 2 * kWinPairsPerStage * kWinJs * (int)sizeof(Real);
 }
 
+// One 64-lane round of the stage copy of win_schur: slot s = 64 r + lane of the stage's m pairs
+// (14 record chunks per pair: i chunks 0-4 and 6-8, j chunks 0-5).  load_round fetches the chunk
+// into registers, store_round puts it into the wave's staging area.
+template <typename Real, int PS>
+__device__ __forceinline__ R4<Real> win_load_round(const Real* __restrict__ js, int2 pr, int m, int r) {
+    constexpr int CI = 8, CP = 14;
+    const int s = 64 * r + (threadIdx.x & 63);
+    const int k = min(s / CP, PS - 1), e = s - CP * (s / CP);
+    const int oi = __shfl(pr.x, k, 64), oj = __shfl(pr.y, k, 64);
+    R4<Real> v{};
+    if (s < m * CP) {
+        const bool is_j = e >= CI;
+        const int chunk = is_j ? e - CI : (e < 5 ? e : e + 1);
+        v = reinterpret_cast<const R4<Real>*>(js + kWinJs * (size_t)(is_j ? oj : oi))[chunk];
+    }
+    return v;
+}
+
+template <typename Real, int PS>
+__device__ __forceinline__ void win_store_round(Real* st_i, Real* st_j, int m, int r, const R4<Real>& v) {
+    constexpr int CI = 8, CP = 14;
+    const int s = 64 * r + (threadIdx.x & 63);
+    if (s < m * CP) {
+        const int k = s / CP, e = s - CP * k;
+        const bool is_j = e >= CI;
+        const int chunk = is_j ? e - CI : (e < 5 ? e : e + 1);
+        *reinterpret_cast<R4<Real>*>((is_j ? st_j : st_i) + kWinJs * k + 4 * chunk) = v;
+    }
+}
+
 // 9x9 blocks of the reduced camera system into S (n x n row-major, both triangles, global memory):
 // S_ab = [a == b] D_a^2 - sum_pairs J~_c,i' G_ij J~_c,j with G_ij = H_i J~_p,j' (i != j) and
-// G_ii = -(I - G)_i.  Blocks are taken from *counter in list order.  Per stage of 32 pairs the
+// G_ii = -(I - G)_i.  The blocks first + stride t, t < count, are taken through *counter.  Per stage of 32 pairs the
 // wave copies the record chunks the products read into its LDS staging area with 4-element vector
 // loads (lanes cover the chunks of consecutive pairs), then lanes 0-31 accumulate rows 0-4 and lanes
 // 32-63 rows 5-8 of pair q = lane % 32 in registers; the 32 partial sums of a block are combined by
@@ -161,100 +208,98 @@ constexpr int win_stage_bytes() {
 template <typename Real, typename F>
 __device__ __forceinline__ void win_schur(const int4* __restrict__ blk, const int2* __restrict__ pairs,
                                           const Real* __restrict__ js, const double* __restrict__ dc2, F* S, int n,
-                                          int n_blocks, int* counter, Real* stage) {
+                                          int first, int stride, int count, int* counter, Real* stage) {
     constexpr int PS = kWinPairsPerStage;
     const int lane = threadIdx.x & 63, q = lane & 31, h = lane >> 5;
     const int rb = 5 * h;
     Real* st_i = stage;
     Real* st_j = stage + PS * kWinJs;
     for (;;) {
-        int b = 0;
-        if (lane == 0) b = atomicAdd(counter, 1);
-        b = __shfl(b, 0, 64);
-        if (b >= n_blocks) break;
-        const int4 bk = blk[b];
+        int t = 0;
+        if (lane == 0) t = atomicAdd(counter, 1);
+        t = __shfl(t, 0, 64);
+        if (t >= count) break;
+        const int4 bk = blk[first + stride * t];
         Real acc[5][9];
 #pragma unroll
         for (int r = 0; r < 5; ++r)
 #pragma unroll
             for (int c = 0; c < 9; ++c) acc[r][c] = (Real)0;
-        for (int base = bk.z; base < bk.w; base += PS) {
+        // record chunks the pair products read: i records chunks 0-4 (J~_c) and 6-8 (H, I - G), j
+        // records chunks 0-5 (J~_c, J~_p); 14 slots per pair, 7 rounds of 64 lanes per stage
+        static_assert(PS * 14 <= 7 * 64, "stage copy rounds");
+        const int npairs = bk.w - bk.z;
+        const int nst = (npairs + PS - 1) / PS;
+        int2 pr = make_int2(0, 0), pr_next = make_int2(0, 0);
+        if (lane < min(PS, npairs)) pr = pairs[bk.z + lane];
+        if (nst > 1 && lane < min(PS, npairs - PS)) pr_next = pairs[bk.z + PS + lane];
+        int m_ld = min(PS, npairs);
+        R4<Real> b0 = win_load_round<Real, PS>(js, pr, m_ld, 0), b1 = win_load_round<Real, PS>(js, pr, m_ld, 1),
+                 b2 = win_load_round<Real, PS>(js, pr, m_ld, 2), b3 = win_load_round<Real, PS>(js, pr, m_ld, 3),
+                 b4 = win_load_round<Real, PS>(js, pr, m_ld, 4), b5 = win_load_round<Real, PS>(js, pr, m_ld, 5),
+                 b6 = win_load_round<Real, PS>(js, pr, m_ld, 6);
+        for (int stg = 0; stg < nst; ++stg) {
+            const int base = bk.z + stg * PS;
             const int m = min(PS, bk.w - base);
-            int2 pr = make_int2(0, 0);
-            if (lane < m) pr = pairs[base + lane];
-            // cooperative copy of the chunks the pair products read: i records chunks 0-4 (J~_c) and
-            // 6-8 (H, I - G), j records chunks 0-5 (J~_c, J~_p); slot s of 14 per pair
-            constexpr int CI = 8, CJ = 6, CP = CI + CJ;
-            const int nslot = m * CP;
-            for (int s0 = 0; s0 < nslot; s0 += 64) {
-                const int s = s0 + lane;
-                const int k = min(s / CP, PS - 1), e = s - CP * (s / CP);
-                const int oi = __shfl(pr.x, k, 64), oj = __shfl(pr.y, k, 64);
-                if (s < nslot) {
-                    const bool is_j = e >= CI;
-                    const int chunk = is_j ? e - CI : (e < 5 ? e : e + 1);
-                    const int obs = is_j ? oj : oi;
-                    const R4<Real> v = reinterpret_cast<const R4<Real>*>(js + kWinJs * (size_t)obs)[chunk];
-                    *reinterpret_cast<R4<Real>*>((is_j ? st_j : st_i) + kWinJs * k + 4 * chunk) = v;
-                }
-            }
+            win_store_round<Real, PS>(st_i, st_j, m, 0, b0);
+            win_store_round<Real, PS>(st_i, st_j, m, 1, b1);
+            win_store_round<Real, PS>(st_i, st_j, m, 2, b2);
+            win_store_round<Real, PS>(st_i, st_j, m, 3, b3);
+            win_store_round<Real, PS>(st_i, st_j, m, 4, b4);
+            win_store_round<Real, PS>(st_i, st_j, m, 5, b5);
+            win_store_round<Real, PS>(st_i, st_j, m, 6, b6);
             const int pi = __shfl(pr.x, q, 64), pj = __shfl(pr.y, q, 64);
+            if (stg + 1 < nst) {
+                // the next stage's loads are in flight while this one is computed
+                m_ld = min(PS, bk.w - base - PS);
+                b0 = win_load_round<Real, PS>(js, pr_next, m_ld, 0);
+                b1 = win_load_round<Real, PS>(js, pr_next, m_ld, 1);
+                b2 = win_load_round<Real, PS>(js, pr_next, m_ld, 2);
+                b3 = win_load_round<Real, PS>(js, pr_next, m_ld, 3);
+                b4 = win_load_round<Real, PS>(js, pr_next, m_ld, 4);
+                b5 = win_load_round<Real, PS>(js, pr_next, m_ld, 5);
+                b6 = win_load_round<Real, PS>(js, pr_next, m_ld, 6);
+                pr = pr_next;
+                pr_next = make_int2(0, 0);
+                if (stg + 2 < nst && lane < min(PS, bk.w - base - 2 * PS)) pr_next = pairs[base + 2 * PS + lane];
+            }
             wave_sync_lds();
             if (q < m) {
-                // whole 4-element chunks from LDS (record stride 36 elements keeps 8 consecutive
-                // lanes on distinct banks)
-                const R4<Real>* Ri = reinterpret_cast<const R4<Real>*>(st_i + kWinJs * q);
-                const R4<Real>* Rj = reinterpret_cast<const R4<Real>*>(st_j + kWinJs * q);
-                Real ji[20], jj[24], hg[12];
-#pragma unroll
-                for (int c4 = 0; c4 < 5; ++c4) {
-                    const R4<Real> v = Ri[c4];
-                    ji[4 * c4] = v.x;
-                    ji[4 * c4 + 1] = v.y;
-                    ji[4 * c4 + 2] = v.z;
-                    ji[4 * c4 + 3] = v.w;
-                }
-#pragma unroll
-                for (int c4 = 0; c4 < 3; ++c4) {
-                    const R4<Real> v = Ri[6 + c4];
-                    hg[4 * c4] = v.x;
-                    hg[4 * c4 + 1] = v.y;
-                    hg[4 * c4 + 2] = v.z;
-                    hg[4 * c4 + 3] = v.w;
-                }
-#pragma unroll
-                for (int c4 = 0; c4 < 6; ++c4) {
-                    const R4<Real> v = Rj[c4];
-                    jj[4 * c4] = v.x;
-                    jj[4 * c4 + 1] = v.y;
-                    jj[4 * c4 + 2] = v.z;
-                    jj[4 * c4 + 3] = v.w;
-                }
+                // pairs of consecutive elements from LDS (the two rows of a J~_c column are adjacent)
+                using R2 = R2v<Real>;
+                const Real* Ri = st_i + kWinJs * q;
+                const Real* Rj = st_j + kWinJs * q;
                 Real G00, G01, G10, G11;
                 if (pi == pj) {
                     // an observation with itself carries its share of U*: J~c'(I - G)J~c, negated
                     // because the accumulated sum is subtracted
-                    G00 = -hg[6];
-                    G01 = G10 = -hg[7];
-                    G11 = -hg[8];
+                    const R2 g = *reinterpret_cast<const R2*>(Ri + 30);
+                    G00 = -g.x;
+                    G01 = G10 = -g.y;
+                    G11 = -Ri[32];
                 } else {
-                    // H_i = hg[0..5], J~_p,j = jj[18..23]
-                    G00 = hg[0] * jj[18] + hg[1] * jj[19] + hg[2] * jj[20];
-                    G01 = hg[0] * jj[21] + hg[1] * jj[22] + hg[2] * jj[23];
-                    G10 = hg[3] * jj[18] + hg[4] * jj[19] + hg[5] * jj[20];
-                    G11 = hg[3] * jj[21] + hg[4] * jj[22] + hg[5] * jj[23];
+                    const R2 h0 = *reinterpret_cast<const R2*>(Ri + 24), h1 = *reinterpret_cast<const R2*>(Ri + 26),
+                             h2 = *reinterpret_cast<const R2*>(Ri + 28);
+                    const R2 p0 = *reinterpret_cast<const R2*>(Rj + 18), p1 = *reinterpret_cast<const R2*>(Rj + 20),
+                             p2 = *reinterpret_cast<const R2*>(Rj + 22);
+                    // H_i = [h0.x h0.y h1.x; h1.y h2.x h2.y], J~_p,j = [p0.x p0.y p1.x; p1.y p2.x p2.y]
+                    G00 = h0.x * p0.x + h0.y * p0.y + h1.x * p1.x;
+                    G01 = h0.x * p1.y + h0.y * p2.x + h1.x * p2.y;
+                    G10 = h1.y * p0.x + h2.x * p0.y + h2.y * p1.x;
+                    G11 = h1.y * p1.y + h2.x * p2.x + h2.y * p2.y;
                 }
                 Real a0[5], a1[5];
 #pragma unroll
                 for (int r = 0; r < 5; ++r) {
-                    // rows rb + r (the second half's spare fifth row repeats row 8)
-                    a0[r] = h ? ji[min(5 + r, 8)] : ji[r];
-                    a1[r] = h ? ji[9 + min(5 + r, 8)] : ji[9 + r];
+                    const int row = min(rb + r, 8);  // the second half's spare fifth row repeats row 8
+                    const R2 v = *reinterpret_cast<const R2*>(Ri + 2 * row);
+                    a0[r] = v.x;
+                    a1[r] = v.y;
                 }
 #pragma unroll
                 for (int c = 0; c < 9; ++c) {
-                    const Real j0 = jj[c], j1 = jj[9 + c];
-                    const Real t0 = G00 * j0 + G01 * j1, t1 = G10 * j0 + G11 * j1;
+                    const R2 jc = *reinterpret_cast<const R2*>(Rj + 2 * c);
+                    const Real t0 = G00 * jc.x + G01 * jc.y, t1 = G10 * jc.x + G11 * jc.y;
 #pragma unroll
                     for (int r = 0; r < 5; ++r) acc[r][c] += a0[r] * t0 + a1[r] * t1;
                 }
@@ -439,498 +484,583 @@ __device__ __forceinline__ double win_dot(const double* a, const double* b, int 
     return v;
 }
 
-constexpr double kWinCgTol = 1e-12;  // relative residual at which the mixed-precision CG stops
+constexpr double kWinCgTol = 1e-8;  // relative residual at which the mixed-precision CG stops
 
-template <typename Real, int NT>
-__global__ void __launch_bounds__(NT) k_window_lm(const WinDesc* __restrict__ descs, char* __restrict__ arena,
-                                                  const int* __restrict__ work, int n_work, int iters, Opts o,
-                                                  int refine) {
-    using F = Real;  // reduced system and factor: FP32 in mixed precision (refined in FP64), FP64 otherwise
-    constexpr int NW = NT / 64;
-    extern __shared__ __align__(32) unsigned char smem[];
-    __shared__ double red[NW];
-    __shared__ double xv[kWinMaxN];        // reduced gradient, then the camera step y_c
-    __shared__ double rv[kWinMaxN];        // CG: residual
-    __shared__ double zv[kWinMaxN];        // CG: preconditioned residual
-    __shared__ double dv[kWinMaxN];        // CG: search direction
-    __shared__ double qv[kWinMaxN];        // CG: S times the search direction
-    __shared__ double sdot;
-    __shared__ F colv[kWinMaxN];           // Cholesky: current column
-    __shared__ F dinv[kWinMaxN];           // 1 / diagonal of the factor
-    __shared__ double cpart[kWinMaxK][4];  // per-camera partial sums, combined in camera order
-    __shared__ int sflag[4];               // 0: invalid point block, 1: accepted / factor failure, 2: block counter
-    const int wi = blockIdx.x;
-    if (wi >= n_work) return;
-    const WinDesc D = descs[work[wi]];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int K = D.K, P = D.P, n = 9 * K;
+
+// ------------------------------------------------------------------------------------------------
+// Phase kernels.  ctask: (slot, camera) per workgroup; ptask: (slot, chunk) per 64-thread
+// workgroup; wtask: slot per workgroup; stask: (slot, first block, block stride, block count).
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_wlin_cam(const WinDesc* __restrict__ descs, char* __restrict__ arena,
+                                                 const int2* __restrict__ ctask, int ntask, Opts o) {
+    __shared__ double lds[19 * (NT / 64)];
+    __shared__ CameraPrep<double> Psh;
+    const int t = blockIdx.x;
+    if (t >= ntask) return;
+    const int2 tk = ctask[t];
+    const WinDesc D = descs[tk.x];
     const WinPtrs w = win_ptrs(D, arena);
-    Real* js = at<Real>(arena, D.o_js);
-    F* Sg = at<F>(arena, D.o_S);  // reduced system as formed (unlisted blocks stay zero)
-    WinState* st = w.st;
-    F* S = reinterpret_cast<F*>(smem);  // its LDS copy, factored in place
-
+    const WinState* st = w.st;
+    if (st->done || !st->need_lin || tk.y >= w.K) return;
+    const int c = tk.y, first = st->first_lin;
+    if (threadIdx.x == 0) {
+        camera_prep<double>(w.cam_x + 9 * (size_t)c, Psh);
+        w.prep[c] = Psh;
+    }
     __syncthreads();
+    double acc[19];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) acc[i] = 0.0;  // diag (9), J_c'r (9), cost
+    const size_t ld = (size_t)w.No;
+    for (int j = w.cam_off[c] + threadIdx.x; j < w.cam_off[c + 1]; j += NT) {
+        const int p = w.co_pt[j];
+        const double4 x4 = w.pt_x[p];
+        const double X[3] = {x4.x, x4.y, x4.z};
+        double r[2], Jc[18], Jp[6];
+        snavely_residual_jacobian<double>(Psh, X, w.co_uv64[2 * (size_t)j], w.co_uv64[2 * (size_t)j + 1], r, Jc, Jp);
+        const double s2 = r[0] * r[0] + r[1] * r[1];
+        if (first) acc[18] += half_rho(s2, o.loss, o.loss_a);
+        const double s = huber_sqrt_rho1<double>(s2, o.loss, o.loss_a);
+        double* rec = w.j64 + j;
+#pragma unroll
+        for (int i = 0; i < 18; ++i) rec[i * ld] = Jc[i] *= s;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) rec[(18 + i) * ld] = Jp[i] *= s;
+        r[0] *= s;
+        r[1] *= s;
+        rec[24 * ld] = r[0];
+        rec[25 * ld] = r[1];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            acc[i] += Jc[i] * Jc[i] + Jc[9 + i] * Jc[9 + i];
+            acc[9 + i] += Jc[i] * r[0] + Jc[9 + i] * r[1];
+        }
+        const int slot = w.slot_of[j];
+        pv_at(w, 0, slot) = make_double4(Jp[0] * Jp[0] + Jp[3] * Jp[3], Jp[0] * Jp[1] + Jp[3] * Jp[4],
+                                         Jp[0] * Jp[2] + Jp[3] * Jp[5], Jp[1] * Jp[1] + Jp[4] * Jp[4]);
+        pv_at(w, 1, slot) = make_double4(Jp[1] * Jp[2] + Jp[4] * Jp[5], Jp[2] * Jp[2] + Jp[5] * Jp[5],
+                                         Jp[0] * r[0] + Jp[3] * r[1], Jp[1] * r[0] + Jp[4] * r[1]);
+        pv_at(w, 2, slot) = make_double4(Jp[2] * r[0] + Jp[5] * r[1], 0.0, 0.0, 0.0);
+    }
+    wg_sum_fixed<19, NT>(acc, lds);
+    if (threadIdx.x == 0) {
+        double gm = 0.0;
+        for (int i = 0; i < 9; ++i) {
+            w.udiag[9 * (size_t)c + i] = acc[i];
+            w.gc[9 * (size_t)c + i] = acc[9 + i];
+            gm = fmax(gm, fabs(acc[9 + i]));
+        }
+        w.cpart[WC_COUNT * (size_t)c + WC_GMAX] = gm;
+        w.cpart[WC_COUNT * (size_t)c + WC_COST] = acc[18];
+    }
+}
+
+// One wave per chunk of 64 points (lane = point).
+__global__ void __launch_bounds__(64) k_wlin_pt(const WinDesc* __restrict__ descs, char* __restrict__ arena,
+                                                const int2* __restrict__ ptask, int ntask, Opts o) {
+    const int t = blockIdx.x;
+    if (t >= ntask) return;
+    const int2 tk = ptask[t];
+    const WinDesc D = descs[tk.x];
+    const WinPtrs w = win_ptrs(D, arena);
+    const WinState* st = w.st;
+    if (st->done || !st->need_lin || tk.y >= D.n_chunks) return;
+    const int ch = tk.y, l = threadIdx.x, p = 64 * ch + l;
+    double gm = 0.0;
+    if (p < w.P) {
+        const int len = w.chunk_len[ch], base = w.chunk_off[ch];
+        double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int r = 0; r < len; ++r) {
+            const int slot = base + 64 * r + l;
+            const double4 x = pv_at(w, 0, slot), y = pv_at(w, 1, slot), z = pv_at(w, 2, slot);
+            a[0] += x.x; a[1] += x.y; a[2] += x.z; a[3] += x.w;
+            a[4] += y.x; a[5] += y.y; a[6] += y.z; a[7] += y.w;
+            a[8] += z.x;
+        }
+        for (int i = 0; i < 6; ++i) w.V[6 * (size_t)p + i] = a[i];
+        for (int i = 0; i < 3; ++i) w.gp[3 * (size_t)p + i] = a[6 + i];
+        gm = fmax(fabs(a[6]), fmax(fabs(a[7]), fabs(a[8])));
+        if (st->first_lin) {
+            const double dg[3] = {a[0], a[3], a[5]};
+            for (int i = 0; i < 3; ++i) w.sp[3 * (size_t)p + i] = o.jacobi_scaling ? 1.0 / (1.0 + sqrt(dg[i])) : 1.0;
+        }
+    }
+    gm = wave_max(gm);
+    if (l == 0) w.ppart[WP_COUNT * (size_t)ch + WP_GMAX] = gm;
+}
+
+// Per window: close the linearization (cost at the start, gradient test, Jacobi scales), then the
+// damped point blocks and camera damping of this iteration.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_wdamp(const WinDesc* __restrict__ descs, char* __restrict__ arena,
+                                              const int* __restrict__ wtask, int ntask, Opts o) {
+    __shared__ int bad;
+    const int t = blockIdx.x;
+    if (t >= ntask) return;
+    const WinDesc D = descs[wtask[t]];
+    const WinPtrs w = win_ptrs(D, arena);
+    WinState* st = w.st;
     if (st->done) return;
-    if (st->first_lin) {
-        // iteration 0: the FP64 cost at the initial parameters
-        for (int c = tid; c < K; c += NT) {
-            CameraPrep<double> Pc;
-            camera_prep<double>(w.cam_x + 9 * (size_t)c, Pc);
-            w.prep_c[c] = Pc;
-        }
-        __syncthreads();
-        for (int c = wave; c < K; c += NW) {
-            double cost = 0.0;
-            for (int j = w.cam_off[c] + lane; j < w.cam_off[c + 1]; j += 64) {
-                const double4 x4 = w.pt_x[w.co_pt[j]];
-                const double X[3] = {x4.x, x4.y, x4.z};
-                double r[2];
-                snavely_residual<double>(w.prep_c[c], X, w.co_uv64[2 * (size_t)j], w.co_uv64[2 * (size_t)j + 1], r);
-                cost += half_rho(r[0] * r[0] + r[1] * r[1], o.loss, o.loss_a);
-            }
-            cost = wave_sum(cost);
-            if (lane == 0) cpart[c][0] = cost;
-        }
-        __syncthreads();
+    const int K = w.K, P = w.P, n = 9 * K, tid = threadIdx.x;
+    if (st->need_lin) {
+        const int first = st->first_lin;
+        if (first)
+            for (int k = tid; k < n; k += NT) w.sc[k] = o.jacobi_scaling ? 1.0 / (1.0 + sqrt(w.udiag[k])) : 1.0;
         if (tid == 0) {
-            double cost = 0.0;
-            for (int c = 0; c < K; ++c) cost += cpart[c][0];
-            st->x_cost = st->initial_cost = cost;
-            st->rows = 1;
-            if (!isfinite(cost)) {
+            double gm = 0.0;
+            for (int c = 0; c < K; ++c) gm = fmax(gm, w.cpart[WC_COUNT * (size_t)c + WC_GMAX]);
+            for (int ch = 0; ch < D.n_chunks; ++ch) gm = fmax(gm, w.ppart[WP_COUNT * (size_t)ch + WP_GMAX]);
+            if (first) {
+                double cost = 0.0;
+                for (int c = 0; c < K; ++c) cost += w.cpart[WC_COUNT * (size_t)c + WC_COST];
+                st->x_cost = st->initial_cost = cost;
+                st->rows = 1;
+                if (!isfinite(cost)) {
+                    st->done = 1;
+                    st->termination = BA_FAILURE;
+                }
+            }
+            st->gmax = gm;
+            st->need_lin = 0;
+            st->first_lin = 0;
+            if (!st->done && first && st->iteration >= o.max_iterations) {
                 st->done = 1;
-                st->termination = BA_FAILURE;
+                st->termination = BA_NO_CONVERGENCE;
+            }
+            if (!st->done && gm <= o.gtol) {
+                st->done = 1;
+                st->termination = BA_CONVERGENCE;
+            }
+            if (!st->done && st->radius <= o.min_radius) {
+                st->done = 1;
+                st->termination = BA_CONVERGENCE;
             }
         }
         __syncthreads();
         if (st->done) return;
     }
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    const double radius = st->radius;
+    for (int p = tid; p < P; p += NT) {
+        const double* V = w.V + 6 * (size_t)p;
+        const double s[3] = {w.sp[3 * (size_t)p], w.sp[3 * (size_t)p + 1], w.sp[3 * (size_t)p + 2]};
+        double A[9];
+        A[0] = s[0] * V[0] * s[0];
+        A[1] = A[3] = s[0] * V[1] * s[1];
+        A[2] = A[6] = s[0] * V[2] * s[2];
+        A[4] = s[1] * V[3] * s[1];
+        A[5] = A[7] = s[1] * V[4] * s[2];
+        A[8] = s[2] * V[5] * s[2];
+        for (int i = 0; i < 3; ++i) A[4 * i] += fmin(fmax(A[4 * i], o.min_diag), o.max_diag) / radius;
+        if (!chol_inplace<3>(A)) {
+            bad = 1;
+            continue;
+        }
+        double Ai[9];
+        chol_inverse<3>(A, Ai);
+        const double sym[6] = {Ai[0], Ai[1], Ai[2], Ai[4], Ai[5], Ai[8]};
+        for (int i = 0; i < 6; ++i) w.Vinv[6 * (size_t)p + i] = sym[i];
+        const double g[3] = {s[0] * w.gp[3 * (size_t)p], s[1] * w.gp[3 * (size_t)p + 1], s[2] * w.gp[3 * (size_t)p + 2]};
+        w.qp[p] = make_double4(Ai[0] * g[0] + Ai[1] * g[1] + Ai[2] * g[2], Ai[3] * g[0] + Ai[4] * g[1] + Ai[5] * g[2],
+                               Ai[6] * g[0] + Ai[7] * g[1] + Ai[8] * g[2], 0.0);
+    }
+    for (int k = tid; k < n; k += NT)
+        w.dc2[k] = fmin(fmax(w.sc[k] * w.udiag[k] * w.sc[k], o.min_diag), o.max_diag) / radius;
+    __syncthreads();
+    if (tid == 0) {
+        st->invalid = bad;
+        st->have_step = 0;
+    }
+}
 
-    for (int it = 0; it < iters; ++it) {
-        __syncthreads();
-        if (st->done) break;
-        // ---------------- lin ----------------
-        if (st->need_lin) {
-            const int first = st->first_lin;
-            for (int c = tid; c < K; c += NT) {
-                CameraPrep<double> Pc;
-                camera_prep<double>(w.cam_x + 9 * (size_t)c, Pc);
-                w.prep[c] = Pc;
-            }
-            __syncthreads();
-            double gm = 0.0;
-            for (int c = wave; c < K; c += NW) {
-                double acc[18];
+// Per (window, camera): records of the reduced system and b_c.
+template <typename Real, int NT>
+__global__ void __launch_bounds__(NT) k_wrec_cam(const WinDesc* __restrict__ descs, char* __restrict__ arena,
+                                                 const int2* __restrict__ ctask, int ntask) {
+    __shared__ double lds[9 * (NT / 64)];
+    const int t = blockIdx.x;
+    if (t >= ntask) return;
+    const int2 tk = ctask[t];
+    const WinDesc D = descs[tk.x];
+    const WinPtrs w = win_ptrs(D, arena);
+    const WinState* st = w.st;
+    if (st->done || st->invalid || tk.y >= w.K) return;
+    Real* js = at<Real>(arena, D.o_js);
+    const int c = tk.y;
+    double scc[9];
+    for (int i = 0; i < 9; ++i) scc[i] = w.sc[9 * (size_t)c + i];
+    double bacc[9];
 #pragma unroll
-                for (int i = 0; i < 18; ++i) acc[i] = 0.0;
-                for (int j = w.cam_off[c] + lane; j < w.cam_off[c + 1]; j += 64) {
-                    const int p = w.co_pt[j];
-                    const double4 x4 = w.pt_x[p];
-                    const double X[3] = {x4.x, x4.y, x4.z};
-                    double r[2], Jc[18], Jp[6];
-                    snavely_residual_jacobian<double>(w.prep[c], X, w.co_uv64[2 * (size_t)j],
-                                                      w.co_uv64[2 * (size_t)j + 1], r, Jc, Jp);
-                    const double s = huber_sqrt_rho1<double>(r[0] * r[0] + r[1] * r[1], o.loss, o.loss_a);
-                    double* rec = w.j64 + j;
-                    const size_t ld = (size_t)w.No;
+    for (int i = 0; i < 9; ++i) bacc[i] = 0.0;
+    for (int j = w.cam_off[c] + threadIdx.x; j < w.cam_off[c + 1]; j += NT) {
+        const int p = w.co_pt[j];
+        double Jc[18], Jp[6];
+        win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
+        const double* Vi = w.Vinv + 6 * (size_t)p;
+        const double j0[3] = {Jp[0], Jp[1], Jp[2]}, j1[3] = {Jp[3], Jp[4], Jp[5]};
+        double h0[3], h1[3];
+        sym3_mul<double>(Vi, j0, h0);
+        sym3_mul<double>(Vi, j1, h1);
+        Real* out = js + kWinJs * (size_t)j;
 #pragma unroll
-                    for (int i = 0; i < 18; ++i) rec[i * ld] = Jc[i] *= s;
-#pragma unroll
-                    for (int i = 0; i < 6; ++i) rec[(18 + i) * ld] = Jp[i] *= s;
-                    r[0] *= s;
-                    r[1] *= s;
-                    rec[24 * ld] = r[0];
-                    rec[25 * ld] = r[1];
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) {
-                        acc[i] += Jc[i] * Jc[i] + Jc[9 + i] * Jc[9 + i];
-                        acc[9 + i] += Jc[i] * r[0] + Jc[9 + i] * r[1];
-                    }
-                    const int slot = w.slot_of[j];
-                    pv_at(w, 0, slot) = make_double4(Jp[0] * Jp[0] + Jp[3] * Jp[3], Jp[0] * Jp[1] + Jp[3] * Jp[4],
-                                                     Jp[0] * Jp[2] + Jp[3] * Jp[5], Jp[1] * Jp[1] + Jp[4] * Jp[4]);
-                    pv_at(w, 1, slot) = make_double4(Jp[1] * Jp[2] + Jp[4] * Jp[5], Jp[2] * Jp[2] + Jp[5] * Jp[5],
-                                                     Jp[0] * r[0] + Jp[3] * r[1], Jp[1] * r[0] + Jp[4] * r[1]);
-                    pv_at(w, 2, slot) = make_double4(Jp[2] * r[0] + Jp[5] * r[1], 0.0, 0.0, 0.0);
-                }
-#pragma unroll
-                for (int i = 0; i < 18; ++i) acc[i] = wave_sum(acc[i]);
-                if (lane == 0) {
-                    for (int i = 0; i < 9; ++i) {
-                        w.udiag[9 * (size_t)c + i] = acc[i];
-                        w.gc[9 * (size_t)c + i] = acc[9 + i];
-                        gm = fmax(gm, fabs(acc[9 + i]));
-                    }
-                }
-            }
-            __syncthreads();
-            for (int p = tid; p < P; p += NT) {
-                const int ch = p >> 6, l = p & 63, len = w.chunk_len[ch];
-                const size_t base = (size_t)w.chunk_off[ch];
-                double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-                for (int r = 0; r < len; ++r) {
-                    const int slot = (int)base + 64 * r + l;
-                    const double4 x = pv_at(w, 0, slot), y = pv_at(w, 1, slot), z = pv_at(w, 2, slot);
-                    a[0] += x.x; a[1] += x.y; a[2] += x.z; a[3] += x.w;
-                    a[4] += y.x; a[5] += y.y; a[6] += y.z; a[7] += y.w;
-                    a[8] += z.x;
-                }
-                for (int i = 0; i < 6; ++i) w.V[6 * (size_t)p + i] = a[i];
-                for (int i = 0; i < 3; ++i) w.gp[3 * (size_t)p + i] = a[6 + i];
-                gm = fmax(gm, fmax(fabs(a[6]), fmax(fabs(a[7]), fabs(a[8]))));
-                if (first) {
-                    const double dg[3] = {a[0], a[3], a[5]};
-                    for (int i = 0; i < 3; ++i) w.sp[3 * (size_t)p + i] = o.jacobi_scaling ? 1.0 / (1.0 + sqrt(dg[i])) : 1.0;
-                }
-            }
-            if (first)
-                for (int k = tid; k < n; k += NT) w.sc[k] = o.jacobi_scaling ? 1.0 / (1.0 + sqrt(w.udiag[k])) : 1.0;
-            gm = wg_max<NT>(gm, red);
-            if (tid == 0) {
-                st->gmax = gm;
-                st->need_lin = 0;
-                st->first_lin = 0;
-                if (first && st->iteration >= o.max_iterations) {
-                    st->done = 1;
-                    st->termination = BA_NO_CONVERGENCE;
-                }
-                if (!st->done && gm <= o.gtol) {
-                    st->done = 1;
-                    st->termination = BA_CONVERGENCE;
-                }
-                if (!st->done && st->radius <= o.min_radius) {
-                    st->done = 1;
-                    st->termination = BA_CONVERGENCE;
-                }
-            }
-            __syncthreads();
-            if (st->done) break;
+        for (int i = 0; i < 9; ++i) {
+            out[2 * i] = (Real)Jc[i];
+            out[2 * i + 1] = (Real)Jc[9 + i];
         }
-        // ---------------- damp ----------------
-        if (tid == 0) {
-            sflag[0] = 0;
-            sflag[1] = 0;
-            sflag[2] = 0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) out[18 + i] = (Real)Jp[i];
+        for (int i = 0; i < 3; ++i) {
+            out[24 + i] = (Real)h0[i];
+            out[27 + i] = (Real)h1[i];
         }
-        __syncthreads();
-        const double radius = st->radius;
-        for (int p = tid; p < P; p += NT) {
-            const double* V = w.V + 6 * (size_t)p;
-            const double s[3] = {w.sp[3 * (size_t)p], w.sp[3 * (size_t)p + 1], w.sp[3 * (size_t)p + 2]};
-            double A[9];
-            A[0] = s[0] * V[0] * s[0];
-            A[1] = A[3] = s[0] * V[1] * s[1];
-            A[2] = A[6] = s[0] * V[2] * s[2];
-            A[4] = s[1] * V[3] * s[1];
-            A[5] = A[7] = s[1] * V[4] * s[2];
-            A[8] = s[2] * V[5] * s[2];
-            for (int i = 0; i < 3; ++i) A[4 * i] += fmin(fmax(A[4 * i], o.min_diag), o.max_diag) / radius;
-            if (!chol_inplace<3>(A)) {
-                sflag[0] = 1;
-                continue;
-            }
-            double Ai[9];
-            chol_inverse<3>(A, Ai);
-            const double sym[6] = {Ai[0], Ai[1], Ai[2], Ai[4], Ai[5], Ai[8]};
-            for (int i = 0; i < 6; ++i) w.Vinv[6 * (size_t)p + i] = sym[i];
-            const double g[3] = {s[0] * w.gp[3 * (size_t)p], s[1] * w.gp[3 * (size_t)p + 1], s[2] * w.gp[3 * (size_t)p + 2]};
-            w.qp[p] = make_double4(Ai[0] * g[0] + Ai[1] * g[1] + Ai[2] * g[2], Ai[3] * g[0] + Ai[4] * g[1] + Ai[5] * g[2],
-                                   Ai[6] * g[0] + Ai[7] * g[1] + Ai[8] * g[2], 0.0);
+        out[30] = (Real)(1.0 - (j0[0] * h0[0] + j0[1] * h0[1] + j0[2] * h0[2]));
+        out[31] = (Real)(-(j0[0] * h1[0] + j0[1] * h1[1] + j0[2] * h1[2]));
+        out[32] = (Real)(1.0 - (j1[0] * h1[0] + j1[1] * h1[1] + j1[2] * h1[2]));
+        const double4 q = w.qp[p];
+        const double e0 = j64_at(w, 24, j) - (Jp[0] * q.x + Jp[1] * q.y + Jp[2] * q.z);
+        const double e1 = j64_at(w, 25, j) - (Jp[3] * q.x + Jp[4] * q.y + Jp[5] * q.z);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) bacc[i] += Jc[i] * e0 + Jc[9 + i] * e1;
+    }
+    wg_sum_fixed<9, NT>(bacc, lds);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < 9; ++i) w.bc[9 * (size_t)c + i] = bacc[i];
+}
+
+// Per (window, block group): blocks first + stride t of the window's list.
+template <typename Real, int NT>
+__global__ void __launch_bounds__(NT) k_wschur(const WinDesc* __restrict__ descs, char* __restrict__ arena,
+                                               const int4* __restrict__ stask, int ntask) {
+    extern __shared__ __align__(32) unsigned char smem[];
+    __shared__ int counter;
+    const int t = blockIdx.x;
+    if (t >= ntask) return;
+    const int4 tk = stask[t];
+    const WinDesc D = descs[tk.x];
+    const WinPtrs w = win_ptrs(D, arena);
+    const WinState* st = w.st;
+    if (st->done || st->invalid || tk.y >= D.n_blocks || tk.y + (size_t)tk.z * (tk.w - 1) >= (size_t)D.n_blocks) return;
+    if (threadIdx.x == 0) counter = 0;
+    __syncthreads();
+    const int wave = threadIdx.x >> 6;
+    win_schur<Real, Real>(w.blk, w.pairs, at<Real>(arena, D.o_js), w.dc2, at<Real>(arena, D.o_S), 9 * w.K, tk.y, tk.z,
+                          tk.w, &counter, reinterpret_cast<Real*>(smem + (size_t)wave * win_stage_bytes<Real>()));
+}
+
+// Per window: factor S (in LDS) and solve for the camera step y_c (into yc); mixed precision
+// refines by preconditioned CG on the FP64 operator (at most cg_max iterations).
+template <typename Real, int NT>
+__global__ void __launch_bounds__(NT) k_wsolve(const WinDesc* __restrict__ descs, char* __restrict__ arena,
+                                               const int* __restrict__ wtask, int ntask, int cg_max) {
+    using F = Real;
+    extern __shared__ __align__(32) unsigned char smem[];
+    __shared__ double xv[kWinMaxN], rv[kWinMaxN], zv[kWinMaxN], dv[kWinMaxN], qv[kWinMaxN];
+    __shared__ F colv[kWinMaxN], dinv[kWinMaxN];
+    __shared__ double sdot;
+    __shared__ int bad;
+    const int t = blockIdx.x;
+    if (t >= ntask) return;
+    const WinDesc D = descs[wtask[t]];
+    const WinPtrs w = win_ptrs(D, arena);
+    WinState* st = w.st;
+    if (st->done || st->invalid) return;
+    const int K = w.K, P = w.P, n = 9 * K, tid = threadIdx.x;
+    const F* Sg = at<F>(arena, D.o_S);
+    F* S = reinterpret_cast<F*>(smem);
+    for (int attempt = 0;; ++attempt) {
+        F shift = (F)0;
+        if (attempt > 0) {
+            double dmax = 0.0;
+            for (int k = 0; k < n; ++k) dmax = fmax(dmax, (double)Sg[(size_t)k * n + k]);
+            shift = (F)(dmax * 1e-6 * pow(100.0, attempt - 1));
         }
-        for (int k = tid; k < n; k += NT)
-            w.dc2[k] = fmin(fmax(w.sc[k] * w.udiag[k] * w.sc[k], o.min_diag), o.max_diag) / radius;
+        for (int idx = tid; idx < n * n; idx += NT) S[idx] = Sg[idx];
+        if (tid == 0) bad = 0;
         __syncthreads();
-        const bool invalid_blocks = sflag[0] != 0;
-        // ---------------- records + reduced gradient ----------------
-        if (!invalid_blocks) {
-            for (int c = wave; c < K; c += NW) {
-                double scc[9];
-                for (int i = 0; i < 9; ++i) scc[i] = w.sc[9 * (size_t)c + i];
-                double bacc[9];
-#pragma unroll
-                for (int i = 0; i < 9; ++i) bacc[i] = 0.0;
-                for (int j = w.cam_off[c] + lane; j < w.cam_off[c + 1]; j += 64) {
-                    const int p = w.co_pt[j];
-                    double Jc[18], Jp[6];
-                    win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
-                    const double* Vi = w.Vinv + 6 * (size_t)p;
-                    const double j0[3] = {Jp[0], Jp[1], Jp[2]}, j1[3] = {Jp[3], Jp[4], Jp[5]};
-                    double h0[3], h1[3];
-                    sym3_mul<double>(Vi, j0, h0);
-                    sym3_mul<double>(Vi, j1, h1);
-                    Real* out = js + kWinJs * (size_t)j;
-#pragma unroll
-                    for (int i = 0; i < 18; ++i) out[i] = (Real)Jc[i];
-#pragma unroll
-                    for (int i = 0; i < 6; ++i) out[18 + i] = (Real)Jp[i];
-                    for (int i = 0; i < 3; ++i) {
-                        out[24 + i] = (Real)h0[i];
-                        out[27 + i] = (Real)h1[i];
-                    }
-                    out[30] = (Real)(1.0 - (j0[0] * h0[0] + j0[1] * h0[1] + j0[2] * h0[2]));
-                    out[31] = (Real)(-(j0[0] * h1[0] + j0[1] * h1[1] + j0[2] * h1[2]));
-                    out[32] = (Real)(1.0 - (j1[0] * h1[0] + j1[1] * h1[1] + j1[2] * h1[2]));
-                    const double4 q = w.qp[p];
-                    const double e0 = j64_at(w, 24, j) - (Jp[0] * q.x + Jp[1] * q.y + Jp[2] * q.z);
-                    const double e1 = j64_at(w, 25, j) - (Jp[3] * q.x + Jp[4] * q.y + Jp[5] * q.z);
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) bacc[i] += Jc[i] * e0 + Jc[9 + i] * e1;
-                }
-#pragma unroll
-                for (int i = 0; i < 9; ++i) bacc[i] = wave_sum(bacc[i]);
-                if (lane == 0)
-                    for (int i = 0; i < 9; ++i) {
-                        w.bc[9 * (size_t)c + i] = bacc[i];
-                        xv[9 * c + i] = bacc[i];
-                    }
-            }
+        if (attempt > 0)
+            for (int k = tid; k < n; k += NT) S[(size_t)k * n + k] += shift;
+        __syncthreads();
+        win_cholesky<F, NT>(S, n, colv, dinv, &bad);
+        if (bad == 0 || cg_max == 0 || attempt == 3) break;
+    }
+    if (bad) return;  // have_step stays 0: an invalid step
+    for (int k = tid; k < n; k += NT) xv[k] = w.bc[k];
+    __syncthreads();
+    win_chol_solve<F, NT>(S, dinv, n, xv);  // x0 = M^-1 b
+    if (cg_max > 0) {
+        for (int k = tid; k < n; k += NT) zv[k] = w.bc[k];
+        __syncthreads();
+        const double bb = win_dot(zv, zv, n, &sdot);
+        win_apply_S<NT>(w, K, P, xv, qv);
+        for (int k = tid; k < n; k += NT) rv[k] = zv[k] - qv[k];
+        __syncthreads();
+        double rr = win_dot(rv, rv, n, &sdot);
+        if (rr > kWinCgTol * kWinCgTol * bb) {
+            for (int k = tid; k < n; k += NT) zv[k] = rv[k];
             __syncthreads();
-            // ---------------- schur + chol ----------------
-            win_schur<Real, F>(w.blk, w.pairs, js, w.dc2, Sg, n, D.n_blocks, &sflag[2],
-                               reinterpret_cast<Real*>(smem + (size_t)wave * win_stage_bytes<Real>()));
+            win_chol_solve<F, NT>(S, dinv, n, zv);
+            for (int k = tid; k < n; k += NT) dv[k] = zv[k];
             __syncthreads();
-            // factor; in mixed precision a failed FP32 factorization is retried with a diagonal shift
-            // (the factor is only the preconditioner of the FP64 CG below)
-            for (int attempt = 0;; ++attempt) {
-                F shift = (F)0;
-                if (attempt > 0) {
-                    double dmax = 0.0;
-                    for (int k = 0; k < n; ++k) dmax = fmax(dmax, (double)Sg[(size_t)k * n + k]);
-                    shift = (F)(dmax * 1e-6 * pow(100.0, attempt - 1));
+            double rz = win_dot(rv, zv, n, &sdot);
+            for (int cg = 0; cg < cg_max; ++cg) {
+                win_apply_S<NT>(w, K, P, dv, qv);
+                const double dq = win_dot(dv, qv, n, &sdot);
+                if (!(dq > 0.0)) break;
+                const double alpha = rz / dq;
+                for (int k = tid; k < n; k += NT) {
+                    xv[k] += alpha * dv[k];
+                    rv[k] -= alpha * qv[k];
                 }
-                for (int idx = tid; idx < n * n; idx += NT) S[idx] = Sg[idx];
-                if (tid == 0) sflag[1] = 0;
                 __syncthreads();
-                if (attempt > 0)
-                    for (int k = tid; k < n; k += NT) S[(size_t)k * n + k] += shift;
+                rr = win_dot(rv, rv, n, &sdot);
+                if (rr <= kWinCgTol * kWinCgTol * bb) break;
+                for (int k = tid; k < n; k += NT) zv[k] = rv[k];
                 __syncthreads();
-                win_cholesky<F, NT>(S, n, colv, dinv, &sflag[1]);
-                if (sflag[1] == 0 || refine == 0 || attempt == 3) break;
-            }
-            if (sflag[1] == 0) {
-                // ---------------- solve ----------------
-                win_chol_solve<F, NT>(S, dinv, n, xv);  // x0 = M^-1 b
-                if (refine > 0) {
-                    // preconditioned CG on the FP64 operator with M = the factor: converges to the FP64
-                    // step also where cond(S) exceeds what an FP32 factor resolves (large trust radii)
-                    for (int k = tid; k < n; k += NT) zv[k] = w.bc[k];
-                    __syncthreads();
-                    const double bb = win_dot(zv, zv, n, &sdot);
-                    win_apply_S<NT>(w, K, P, xv, qv);
-                    for (int k = tid; k < n; k += NT) rv[k] = zv[k] - qv[k];
-                    __syncthreads();
-                    double rr = win_dot(rv, rv, n, &sdot);
-                    if (rr > kWinCgTol * kWinCgTol * bb) {
-                        for (int k = tid; k < n; k += NT) zv[k] = rv[k];
-                        __syncthreads();
-                        win_chol_solve<F, NT>(S, dinv, n, zv);
-                        for (int k = tid; k < n; k += NT) dv[k] = zv[k];
-                        __syncthreads();
-                        double rz = win_dot(rv, zv, n, &sdot);
-                        for (int cg = 0; cg < refine; ++cg) {
-                            win_apply_S<NT>(w, K, P, dv, qv);
-                            const double dq = win_dot(dv, qv, n, &sdot);
-                            if (!(dq > 0.0)) break;
-                            const double alpha = rz / dq;
-                            for (int k = tid; k < n; k += NT) {
-                                xv[k] += alpha * dv[k];
-                                rv[k] -= alpha * qv[k];
-                            }
-                            __syncthreads();
-                            rr = win_dot(rv, rv, n, &sdot);
-                            if (rr <= kWinCgTol * kWinCgTol * bb) break;
-                            for (int k = tid; k < n; k += NT) zv[k] = rv[k];
-                            __syncthreads();
-                            win_chol_solve<F, NT>(S, dinv, n, zv);
-                            const double rz_new = win_dot(rv, zv, n, &sdot);
-                            const double beta = rz_new / rz;
-                            rz = rz_new;
-                            for (int k = tid; k < n; k += NT) dv[k] = zv[k] + beta * dv[k];
-                            __syncthreads();
-                        }
-                    }
-                }
+                win_chol_solve<F, NT>(S, dinv, n, zv);
+                const double rz_new = win_dot(rv, zv, n, &sdot);
+                const double beta = rz_new / rz;
+                rz = rz_new;
+                for (int k = tid; k < n; k += NT) dv[k] = zv[k] + beta * dv[k];
+                __syncthreads();
             }
         }
-        const bool have_step = !invalid_blocks && sflag[1] == 0;
-        __syncthreads();
-        // ---------------- back-substitution and candidate ----------------
-        // |dx|^2, |x|^2, model, cost: per-camera partials (cpart) and one wave over the points, summed
-        // in a fixed order so the result does not depend on the workgroup size
-        double nrm[2] = {0.0, 0.0};
+    }
+    for (int k = tid; k < n; k += NT) w.yc[k] = xv[k];
+    if (tid == 0) st->have_step = 1;
+}
+
+// Per (window, camera): u = J~_p'(r~ - J~_c y_c) into slot order.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_wback_cam(const WinDesc* __restrict__ descs, char* __restrict__ arena,
+                                                  const int2* __restrict__ ctask, int ntask) {
+    const int t = blockIdx.x;
+    if (t >= ntask) return;
+    const int2 tk = ctask[t];
+    const WinDesc D = descs[tk.x];
+    const WinPtrs w = win_ptrs(D, arena);
+    const WinState* st = w.st;
+    if (st->done || !st->have_step || tk.y >= w.K) return;
+    const int c = tk.y;
+    double scc[9], y[9];
+    for (int i = 0; i < 9; ++i) {
+        scc[i] = w.sc[9 * (size_t)c + i];
+        y[i] = w.yc[9 * (size_t)c + i];
+    }
+    for (int j = w.cam_off[c] + threadIdx.x; j < w.cam_off[c + 1]; j += NT) {
+        const int p = w.co_pt[j];
+        double Jc[18], Jp[6];
+        win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
+        double a0 = j64_at(w, 24, j), a1 = j64_at(w, 25, j);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            a0 -= Jc[i] * y[i];
+            a1 -= Jc[9 + i] * y[i];
+        }
+        pv_at(w, 0, w.slot_of[j]) =
+            make_double4(Jp[0] * a0 + Jp[3] * a1, Jp[1] * a0 + Jp[4] * a1, Jp[2] * a0 + Jp[5] * a1, 0.0);
+    }
+}
+
+// One wave per (window, chunk): y_p, candidate points, |dx_p|^2 and |x_p|^2 of the chunk.
+__global__ void __launch_bounds__(64) k_wback_pt(const WinDesc* __restrict__ descs, char* __restrict__ arena,
+                                                 const int2* __restrict__ ptask, int ntask) {
+    const int t = blockIdx.x;
+    if (t >= ntask) return;
+    const int2 tk = ptask[t];
+    const WinDesc D = descs[tk.x];
+    const WinPtrs w = win_ptrs(D, arena);
+    const WinState* st = w.st;
+    if (st->done || !st->have_step || tk.y >= D.n_chunks) return;
+    const int ch = tk.y, p = 64 * ch + threadIdx.x;
+    double dx2 = 0.0, x2 = 0.0;
+    if (p < w.P) {
+        double tt[3], y[3];
+        win_slot_sum3(w, p, tt);
+        sym3_mul<double>(w.Vinv + 6 * (size_t)p, tt, y);
+        const double4 x4 = w.pt_x[p];
+        const double xs[3] = {x4.x, x4.y, x4.z};
+        double cand[3];
+        for (int i = 0; i < 3; ++i) {
+            w.yp[3 * (size_t)p + i] = y[i];
+            const double dl = -w.sp[3 * (size_t)p + i] * y[i];
+            cand[i] = xs[i] + dl;
+            dx2 += dl * dl;
+            x2 += xs[i] * xs[i];
+        }
+        w.pt_c[p] = make_double4(cand[0], cand[1], cand[2], 0.0);
+    }
+    dx2 = wave_sum(dx2);
+    x2 = wave_sum(x2);
+    if (threadIdx.x == 0) {
+        w.ppart[WP_COUNT * (size_t)ch + WP_DX2] = dx2;
+        w.ppart[WP_COUNT * (size_t)ch + WP_X2] = x2;
+    }
+}
+
+// Per (window, camera): candidate camera, model cost change -(J~s)'(r~ + J~s/2) (step s = -y) and
+// FP64 candidate cost of the camera's observations, |dx_c|^2, |x_c|^2.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_wcand_cam(const WinDesc* __restrict__ descs, char* __restrict__ arena,
+                                                  const int2* __restrict__ ctask, int ntask, Opts o) {
+    __shared__ double lds[2 * (NT / 64)];
+    __shared__ CameraPrep<double> Psh;
+    __shared__ double csh[9];
+    const int t = blockIdx.x;
+    if (t >= ntask) return;
+    const int2 tk = ctask[t];
+    const WinDesc D = descs[tk.x];
+    const WinPtrs w = win_ptrs(D, arena);
+    const WinState* st = w.st;
+    if (st->done || !st->have_step || tk.y >= w.K) return;
+    const int c = tk.y;
+    if (threadIdx.x < 9) {
+        const size_t k = 9 * (size_t)c + threadIdx.x;
+        const double v = w.cam_x[k] - w.sc[k] * w.yc[k];
+        w.cam_c[k] = v;
+        csh[threadIdx.x] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) camera_prep<double>(csh, Psh);
+    __syncthreads();
+    double scc[9], y[9];
+    for (int i = 0; i < 9; ++i) {
+        scc[i] = w.sc[9 * (size_t)c + i];
+        y[i] = w.yc[9 * (size_t)c + i];
+    }
+    double acc[2] = {0.0, 0.0};  // model, cost
+    for (int j = w.cam_off[c] + threadIdx.x; j < w.cam_off[c + 1]; j += NT) {
+        const int p = w.co_pt[j];
+        double Jc[18], Jp[6];
+        win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
+        const double* yp = w.yp + 3 * (size_t)p;
+        double m0 = 0.0, m1 = 0.0;
+        for (int i = 0; i < 9; ++i) {
+            m0 -= Jc[i] * y[i];
+            m1 -= Jc[9 + i] * y[i];
+        }
+        for (int i = 0; i < 3; ++i) {
+            m0 -= Jp[i] * yp[i];
+            m1 -= Jp[3 + i] * yp[i];
+        }
+        acc[0] += m0 * (j64_at(w, 24, j) + 0.5 * m0) + m1 * (j64_at(w, 25, j) + 0.5 * m1);
+        const double4 x4 = w.pt_c[p];
+        const double X[3] = {x4.x, x4.y, x4.z};
+        double r[2];
+        snavely_residual<double>(Psh, X, w.co_uv64[2 * (size_t)j], w.co_uv64[2 * (size_t)j + 1], r);
+        acc[1] += half_rho(r[0] * r[0] + r[1] * r[1], o.loss, o.loss_a);
+    }
+    wg_sum_fixed<2, NT>(acc, lds);
+    if (threadIdx.x == 0) {
+        double dx2 = 0.0, x2 = 0.0;
+        for (int i = 0; i < 9; ++i) {
+            const size_t k = 9 * (size_t)c + i;
+            const double dl = -w.sc[k] * w.yc[k], x = w.cam_x[k];
+            dx2 += dl * dl;
+            x2 += x * x;
+        }
+        double* cp = w.cpart + WC_COUNT * (size_t)c;
+        cp[WC_MODEL] = acc[0];
+        cp[WC_CAND] = acc[1];
+        cp[WC_DX2] = dx2;
+        cp[WC_X2] = x2;
+    }
+}
+
+// Per window: the LM decision (Ceres rules) and, for an accepted step, the new parameters.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_wdecide(const WinDesc* __restrict__ descs, char* __restrict__ arena,
+                                                const int* __restrict__ wtask, int ntask, Opts o) {
+    __shared__ int accepted;
+    const int t = blockIdx.x;
+    if (t >= ntask) return;
+    const WinDesc D = descs[wtask[t]];
+    const WinPtrs w = win_ptrs(D, arena);
+    WinState* st = w.st;
+    if (st->done) return;
+    if (threadIdx.x == 0) {
+        accepted = 0;
+        const bool have_step = st->have_step != 0;
+        double model = 0.0, cand = 0.0, dx2 = 0.0, x2 = 0.0;
         if (have_step) {
-            for (int c = wave; c < K; c += NW) {
-                double scc[9], y[9];
-                for (int i = 0; i < 9; ++i) {
-                    scc[i] = w.sc[9 * (size_t)c + i];
-                    y[i] = xv[9 * c + i];
-                }
-                for (int j = w.cam_off[c] + lane; j < w.cam_off[c + 1]; j += 64) {
-                    const int p = w.co_pt[j];
-                    double Jc[18], Jp[6];
-                    win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
-                    double a0 = j64_at(w, 24, j), a1 = j64_at(w, 25, j);
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) {
-                        a0 -= Jc[i] * y[i];
-                        a1 -= Jc[9 + i] * y[i];
-                    }
-                    pv_at(w, 0, w.slot_of[j]) = make_double4(Jp[0] * a0 + Jp[3] * a1, Jp[1] * a0 + Jp[4] * a1,
-                                                             Jp[2] * a0 + Jp[5] * a1, 0.0);
-                }
+            for (int c = 0; c < w.K; ++c) {
+                const double* cp = w.cpart + WC_COUNT * (size_t)c;
+                model += cp[WC_MODEL];
+                cand += cp[WC_CAND];
+                dx2 += cp[WC_DX2];
+                x2 += cp[WC_X2];
             }
-            __syncthreads();
-            for (int p = tid; p < P; p += NT) {
-                double t[3], y[3];
-                win_slot_sum3(w, p, t);
-                sym3_mul<double>(w.Vinv + 6 * (size_t)p, t, y);
-                const double4 x4 = w.pt_x[p];
-                const double xs[3] = {x4.x, x4.y, x4.z};
-                double cand[3];
-                for (int i = 0; i < 3; ++i) {
-                    w.yp[3 * (size_t)p + i] = y[i];
-                    cand[i] = xs[i] - w.sp[3 * (size_t)p + i] * y[i];
-                }
-                w.pt_c[p] = make_double4(cand[0], cand[1], cand[2], 0.0);
-            }
-            for (int k = tid; k < n; k += NT) w.cam_c[k] = w.cam_x[k] - w.sc[k] * xv[k];
-            __syncthreads();
-            if (wave == 0) {
-                for (int p = lane; p < P; p += 64) {
-                    const double4 x4 = w.pt_x[p];
-                    const double xs[3] = {x4.x, x4.y, x4.z};
-                    for (int i = 0; i < 3; ++i) {
-                        const double dl = -w.sp[3 * (size_t)p + i] * w.yp[3 * (size_t)p + i];
-                        nrm[0] += dl * dl;
-                        nrm[1] += xs[i] * xs[i];
-                    }
-                }
-                nrm[0] = wave_sum(nrm[0]);
-                nrm[1] = wave_sum(nrm[1]);
-            }
-            for (int c = tid; c < K; c += NT) {
-                CameraPrep<double> Pc;
-                camera_prep<double>(w.cam_c + 9 * (size_t)c, Pc);
-                w.prep_c[c] = Pc;
-            }
-            __syncthreads();
-            // model cost change -(J~s)'(r~ + J~s/2) from j64 (step s = -y) and the FP64 candidate cost
-            for (int c = wave; c < K; c += NW) {
-                double scc[9], y[9];
-                for (int i = 0; i < 9; ++i) {
-                    scc[i] = w.sc[9 * (size_t)c + i];
-                    y[i] = xv[9 * c + i];
-                }
-                double cm[4] = {0.0, 0.0, 0.0, 0.0};
-                if (lane < 9) {
-                    const int k = 9 * c + lane;
-                    const double dl = -w.sc[k] * xv[k], x = w.cam_x[k];
-                    cm[0] = dl * dl;
-                    cm[1] = x * x;
-                }
-                for (int j = w.cam_off[c] + lane; j < w.cam_off[c + 1]; j += 64) {
-                    const int p = w.co_pt[j];
-                    double Jc[18], Jp[6];
-                    win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
-                    const double* yp = w.yp + 3 * (size_t)p;
-                    double m0 = 0.0, m1 = 0.0;
-                    for (int i = 0; i < 9; ++i) {
-                        m0 -= Jc[i] * y[i];
-                        m1 -= Jc[9 + i] * y[i];
-                    }
-                    for (int i = 0; i < 3; ++i) {
-                        m0 -= Jp[i] * yp[i];
-                        m1 -= Jp[3 + i] * yp[i];
-                    }
-                    cm[2] += m0 * (j64_at(w, 24, j) + 0.5 * m0) + m1 * (j64_at(w, 25, j) + 0.5 * m1);
-                    const double4 x4 = w.pt_c[p];
-                    const double X[3] = {x4.x, x4.y, x4.z};
-                    double r[2];
-                    snavely_residual<double>(w.prep_c[c], X, w.co_uv64[2 * (size_t)j], w.co_uv64[2 * (size_t)j + 1], r);
-                    cm[3] += half_rho(r[0] * r[0] + r[1] * r[1], o.loss, o.loss_a);
-                }
-#pragma unroll
-                for (int i = 0; i < 4; ++i) cm[i] = wave_sum(cm[i]);
-                if (lane == 0)
-                    for (int i = 0; i < 4; ++i) cpart[c][i] = cm[i];
+            for (int ch = 0; ch < D.n_chunks; ++ch) {
+                dx2 += w.ppart[WP_COUNT * (size_t)ch + WP_DX2];
+                x2 += w.ppart[WP_COUNT * (size_t)ch + WP_X2];
             }
         }
-        __syncthreads();
-        // ---------------- decide ----------------
-        if (tid == 0) {
-            sflag[1] = 0;
-            double tot[4] = {nrm[0], nrm[1], 0.0, 0.0};  // wave 0 holds the point sums
-            if (have_step)
-                for (int c = 0; c < K; ++c)
-                    for (int i = 0; i < 4; ++i) tot[i] += cpart[c][i];
-            const double model = -tot[2];
-            double cand = tot[3];
-            const double dx2 = tot[0], x2 = tot[1];
-            st->iteration++;
-            const bool valid = have_step && model > 0.0 && isfinite(dx2);
-            bool record = true;
-            if (!valid) {
-                if (++st->consecutive_invalid >= o.max_invalid) {
-                    st->done = 1;
-                    st->termination = BA_FAILURE;
-                    record = false;
-                } else {
-                    st->radius = st->radius / st->decrease_factor;
-                    st->decrease_factor *= 2.0;
-                    st->num_unsuccessful++;
-                }
+        model = -model;
+        st->iteration++;
+        const bool valid = have_step && model > 0.0 && isfinite(dx2);
+        bool record = true;
+        if (!valid) {
+            if (++st->consecutive_invalid >= o.max_invalid) {
+                st->done = 1;
+                st->termination = BA_FAILURE;
+                record = false;
             } else {
-                st->consecutive_invalid = 0;
-                if (!isfinite(cand)) cand = DBL_MAX;
-                const double step_norm = sqrt(dx2), x_norm = sqrt(x2);
-                if (step_norm <= o.ptol * (x_norm + o.ptol)) {
+                st->radius = st->radius / st->decrease_factor;
+                st->decrease_factor *= 2.0;
+                st->num_unsuccessful++;
+            }
+        } else {
+            st->consecutive_invalid = 0;
+            if (!isfinite(cand)) cand = DBL_MAX;
+            const double step_norm = sqrt(dx2), x_norm = sqrt(x2);
+            if (step_norm <= o.ptol * (x_norm + o.ptol)) {
+                st->done = 1;
+                st->termination = BA_CONVERGENCE;
+                record = false;
+            } else {
+                const double cost_change = st->x_cost - cand;
+                if (fabs(cost_change) <= o.ftol * st->x_cost) {
                     st->done = 1;
                     st->termination = BA_CONVERGENCE;
                     record = false;
                 } else {
-                    const double cost_change = st->x_cost - cand;
-                    if (fabs(cost_change) <= o.ftol * st->x_cost) {
-                        st->done = 1;
-                        st->termination = BA_CONVERGENCE;
-                        record = false;
+                    const double rho = cost_change / model;
+                    if (rho > o.min_rel_decrease) {
+                        st->x_cost = cand;
+                        const double tt = 2.0 * rho - 1.0;
+                        st->radius = fmin(o.max_radius, st->radius / fmax(1.0 / 3.0, 1.0 - tt * tt * tt));
+                        st->decrease_factor = 2.0;
+                        st->need_lin = 1;
+                        st->num_successful++;
+                        accepted = 1;
                     } else {
-                        const double rho = cost_change / model;
-                        if (rho > o.min_rel_decrease) {
-                            st->x_cost = cand;
-                            const double t = 2.0 * rho - 1.0;
-                            st->radius = fmin(o.max_radius, st->radius / fmax(1.0 / 3.0, 1.0 - t * t * t));
-                            st->decrease_factor = 2.0;
-                            st->need_lin = 1;
-                            st->num_successful++;
-                            sflag[1] = 1;
-                        } else {
-                            st->radius = st->radius / st->decrease_factor;
-                            st->decrease_factor *= 2.0;
-                            st->num_unsuccessful++;
-                        }
+                        st->radius = st->radius / st->decrease_factor;
+                        st->decrease_factor *= 2.0;
+                        st->num_unsuccessful++;
                     }
                 }
             }
-            if (record) st->rows++;
-            if (record && !st->done) {
-                if (st->iteration >= o.max_iterations) {
-                    st->done = 1;
-                    st->termination = BA_NO_CONVERGENCE;
-                } else if (!sflag[1] && st->gmax <= o.gtol) {
-                    st->done = 1;
-                    st->termination = BA_CONVERGENCE;
-                } else if (!sflag[1] && st->radius <= o.min_radius) {
-                    st->done = 1;
-                    st->termination = BA_CONVERGENCE;
-                }
+        }
+        if (record) st->rows++;
+        if (record && !st->done) {
+            if (st->iteration >= o.max_iterations) {
+                st->done = 1;
+                st->termination = BA_NO_CONVERGENCE;
+            } else if (!accepted && st->gmax <= o.gtol) {
+                st->done = 1;
+                st->termination = BA_CONVERGENCE;
+            } else if (!accepted && st->radius <= o.min_radius) {
+                st->done = 1;
+                st->termination = BA_CONVERGENCE;
             }
         }
-        __syncthreads();
-        if (sflag[1]) {
-            for (int k = tid; k < n; k += NT) w.cam_x[k] = w.cam_c[k];
-            for (int p = tid; p < P; p += NT) w.pt_x[p] = w.pt_c[p];
-        }
+    }
+    __syncthreads();
+    if (accepted) {
+        const int n = 9 * w.K;
+        for (int k = threadIdx.x; k < n; k += NT) w.cam_x[k] = w.cam_c[k];
+        for (int p = threadIdx.x; p < w.P; p += NT) w.pt_x[p] = w.pt_c[p];
     }
 }
 

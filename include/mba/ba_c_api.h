@@ -184,7 +184,8 @@ int ba_solver_info(ba_solver* solver, int64_t* device_bytes, int32_t* linear_sol
 typedef struct ba_batch ba_batch;
 int ba_batch_create(const ba_problem_view* windows, int32_t count, const ba_options* options, ba_batch** out);
 /* Solves every window from the parameters held on the device (options.max_num_iterations each).
- * summaries: optional array of `count` (their trace pointers are ignored). */
+ * summaries: optional array of `count` (their trace pointers are ignored); total_time_ms of a window
+ * is the time from the start of the run to the moment that window finished on the device. */
 int ba_batch_run(ba_batch* batch, ba_summary* summaries);
 int ba_batch_reset(ba_batch* batch);
 int ba_batch_download(ba_batch* batch, int32_t window, double* cameras, double* points);

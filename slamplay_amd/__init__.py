@@ -285,7 +285,7 @@ def _views(problems):
 def _summary_plain(s):
     return dict(initial_cost=s.initial_cost, final_cost=s.final_cost, num_iterations=s.num_iterations,
                 num_successful_steps=s.num_successful_steps, num_unsuccessful_steps=s.num_unsuccessful_steps,
-                termination=s.termination_type, linear_solver=s.linear_solver_used)
+                termination=s.termination_type, linear_solver=s.linear_solver_used, finish_ms=s.total_time_ms)
 
 
 class Batch:

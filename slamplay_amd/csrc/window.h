@@ -26,6 +26,7 @@ struct WinState {
     int invalid;    // this iteration's damped point blocks were not all positive definite
     int have_step;  // this iteration produced a step
     int pad;
+    unsigned long long t_done;  // device wall clock when the window finished
 };
 
 struct WinDesc {
@@ -33,7 +34,7 @@ struct WinDesc {
     int64_t o_cam_off, o_co_pt, o_co_uv64, o_slot_of, o_chunk_len, o_chunk_off, o_blk, o_pairs;
     int64_t o_cam_x, o_cam_x0, o_cam_c, o_pt_x, o_pt_x0, o_pt_c;
     int64_t o_prep, o_prep_c, o_udiag, o_gc, o_sc, o_dc2, o_bc, o_yc, o_cpart, o_ppart;
-    int64_t o_V, o_gp, o_sp, o_Vinv, o_qp, o_yp, o_pv, o_j64, o_js, o_S, o_state;
+    int64_t o_V, o_gp, o_sp, o_Vinv, o_qp, o_yp, o_pv, o_j64, o_js, o_S, o_L, o_state;  // o_L: 0 = factor in LDS
 };
 
 // Per-observation records.

@@ -74,7 +74,8 @@ template <typename Real>
 int max_window_n() {
     const int64_t room = lds_per_block() - solve_static_lds<Real>();
     if (room <= 0) return 0;
-    int n = (int)std::floor(std::sqrt((double)room / sizeof(Real)));
+    int n = (int)std::floor(std::sqrt(2.0 * (double)room / sizeof(Real)));  // packed: n (n + 1) / 2
+    while (n > 0 && (int64_t)n * (n + 1) / 2 * (int64_t)sizeof(Real) > room) --n;
     n = std::min(n, kWinMaxN);
     return n - n % 9;
 }
@@ -155,6 +156,7 @@ int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowIm
     D.o_j64 = take(8 * kWinJ64 * (int64_t)No);
     D.o_js = take(kWinJs * sizeof(Real) * (int64_t)No);
     D.o_S = take(sizeof(Real) * (int64_t)n * n);  // zero: blocks without common points are never written
+    D.o_L = n > max_window_n<Real>() ? take(sizeof(Real) * (int64_t)n * (n + 1) / 2) : 0;
     D.o_state = take(sizeof(WinState));
     img->bytes.assign(off, 0);
     char* b = img->bytes.data();
@@ -212,6 +214,7 @@ WinDesc rebase(const WinDesc& d, int64_t base) {
     int64_t* o = &r.o_cam_off;
     const int count = (int)((&r.o_state - &r.o_cam_off) + 1);
     for (int i = 0; i < count; ++i) o[i] += base;
+    if (d.o_L == 0) r.o_L = 0;  // factor in LDS
     return r;
 }
 
@@ -270,6 +273,8 @@ struct WindowEngine {
     DevBuf<int> wtask;
     int n_ctask = 0, n_ptask = 0, n_stask = 0, n_wtask = 0, work_nmax = 9, work_threads = 0;
     hipGraphExec_t graph = nullptr;  // one LM iteration of the current work set
+    DevBuf<unsigned long long> dclock;  // device wall clock at the start of the last run
+    double clock_khz = 100000.0;        // rate of the device wall clock
 
     ~WindowEngine() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -292,6 +297,21 @@ struct WindowEngine {
         hdesc.assign(max_slots, WinDesc{});
         BA_TRY(dwork.alloc(max_slots));
         BA_TRY(dstates.alloc(max_slots));
+        BA_TRY(dclock.alloc(1));
+        int dev = 0, khz = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+            clock_khz = khz;
+        return BA_OK;
+    }
+
+    // Milliseconds from the start of the last run to the window's finish (0 if it has not finished).
+    int finish_times(const std::vector<WinState>& st, std::vector<double>* ms) {
+        unsigned long long t0 = 0;
+        BA_HIP(hipMemcpy(&t0, dclock.p, sizeof(t0), hipMemcpyDeviceToHost));
+        ms->assign(st.size(), 0.0);
+        for (size_t i = 0; i < st.size(); ++i)
+            if (st[i].done && st[i].t_done >= t0) (*ms)[i] = (double)(st[i].t_done - t0) / clock_khz;
         return BA_OK;
     }
 
@@ -336,18 +356,30 @@ struct WindowEngine {
             graph = nullptr;
         }
         std::vector<int2> ct, pt;
-        std::vector<int4> stk;
         const int nw = threads / 64, group = 2 * nw;  // about two blocks per wave of k_wschur
         work_nmax = 9;
+        // k_wschur: every block group of a window goes to the same XCD (workgroup k runs on XCD
+        // k mod 8), so the window's records, read once per pair, are re-read from that XCD's L2;
+        // windows are spread over the XCDs largest first onto the least loaded one
+        std::vector<std::vector<int4>> xq(kXcds);
+        std::vector<double> xload(kXcds, 0.0);
         for (int sl : slots) {
             const WinDesc& D = hdesc[sl];
-            work_nmax = std::max(work_nmax, 9 * D.K);
+            if (D.o_L == 0) work_nmax = std::max(work_nmax, 9 * D.K);
             for (int c = 0; c < D.K; ++c) ct.push_back(make_int2(sl, c));
             for (int ch = 0; ch < D.n_chunks; ++ch) pt.push_back(make_int2(sl, ch));
+            const int x = (int)(std::min_element(xload.begin(), xload.end()) - xload.begin());
+            xload[x] += (double)D.n_pairs + 1.0;
             const int G = std::max(1, (D.n_blocks + group - 1) / group);
             for (int g = 0; g < G && g < D.n_blocks; ++g)
-                stk.push_back(make_int4(sl, g, G, (D.n_blocks - g + G - 1) / G));
+                xq[x].push_back(make_int4(sl, g, G, (D.n_blocks - g + G - 1) / G));
         }
+        size_t qmax = 0;
+        for (const auto& q : xq) qmax = std::max(qmax, q.size());
+        std::vector<int4> stk(qmax * kXcds);
+        for (size_t k = 0; k < qmax; ++k)
+            for (int x = 0; x < kXcds; ++x)  // a padding task names a block group past the window's last
+                stk[k * kXcds + x] = k < xq[x].size() ? xq[x][k] : make_int4(slots[0], 1 << 30, 1, 1);
         BA_TRY(upload(ctask, ct));
         BA_TRY(upload(ptask, pt));
         BA_TRY(upload(stask, stk));
@@ -375,7 +407,7 @@ struct WindowEngine {
         if (n_stask)
             hipLaunchKernelGGL((k_wschur<Real, NT>), dim3(n_stask), dim3(NT), (size_t)schur_lds<Real>(NT), s, dd, arena.p,
                                (const int4*)stask.p, n_stask);
-        hipLaunchKernelGGL((k_wsolve<Real, NT>), dim3(n_wtask), dim3(NT), (size_t)work_nmax * work_nmax * sizeof(Real), s,
+        hipLaunchKernelGGL((k_wsolve<Real, NT>), dim3(n_wtask), dim3(NT), solve_lds(), s,
                            dd, arena.p, (const int*)wtask.p, n_wtask, cg_iterations());
         if (n_ctask) hipLaunchKernelGGL((k_wback_cam<kWinCamThreads>), dim3(n_ctask), dim3(kWinCamThreads), 0, s, dd,
                                         arena.p, (const int2*)ctask.p, n_ctask);
@@ -393,9 +425,12 @@ struct WindowEngine {
         BA_HIP(hipFuncSetAttribute((const void*)k_wschur<Real, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)schur_lds<Real>(NT)));
         BA_HIP(hipFuncSetAttribute((const void*)k_wsolve<Real, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(work_nmax * work_nmax * sizeof(Real))));
+                                   (int)solve_lds()));
         return BA_OK;
     }
+
+    // dynamic LDS of k_wsolve: the packed factor of the largest window factored in LDS
+    size_t solve_lds() const { return (size_t)work_nmax * (work_nmax + 1) / 2 * sizeof(Real); }
 
     int enqueue_iteration() {
         if (threads == 512) return enqueue_iteration_nt<512>();
@@ -421,6 +456,7 @@ struct WindowEngine {
                 return fail(BA_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e2));
             }
         }
+        hipLaunchKernelGGL(k_wclock, dim3(1), dim3(64), 0, s, dclock.p);
         for (int it = 0; it < iters; ++it) {
             if (use_graph)
                 BA_HIP(hipGraphLaunch(graph, s));
@@ -523,7 +559,7 @@ struct Batch final : BatchBase {
         int64_t total = 0;
         for (int w = 0; w < count; ++w) {
             BA_TRY(validate_view(&views[w], true));
-            if (9 * views[w].num_cameras > max_window_n<Real>()) {
+            if (9 * views[w].num_cameras > kWinMaxN) {
                 large_index[w] = (int)large_solvers.size();
                 ba_options lo = o;
                 lo.linear_solver = BA_SOLVER_DENSE_SCHUR;
@@ -595,12 +631,15 @@ struct Batch final : BatchBase {
         for (size_t i = 0; i < large.size(); ++i) BA_TRY(large_solvers[i]->run(&large_sum[i]));
         std::vector<WinState> st;
         BA_TRY(eng.states(order, &st));
+        std::vector<double> fin;
+        BA_TRY(eng.finish_times(st, &fin));
         if (summaries) {
             for (size_t i = 0; i < order.size(); ++i) {
                 ba_summary& sm = summaries[order[i]];
                 ba_iteration* tr = sm.trace;
                 const int cap = sm.trace_capacity;
                 fill_summary<Real>(st[i], &sm);
+                sm.total_time_ms = fin[i];
                 sm.trace = tr;
                 sm.trace_capacity = cap;
             }
@@ -776,7 +815,7 @@ struct Server final : ServerBase {
         r->obs.assign(v->observations, v->observations + 2 * (size_t)v->num_observations);
         r->cams.assign(v->cameras, v->cameras + 9 * (size_t)v->num_cameras);
         r->pts.assign(v->points, v->points + 3 * (size_t)v->num_points);
-        if (9 * r->nc > max_window_n<Real>()) {
+        if (9 * r->nc > kWinMaxN) {
             {
                 std::lock_guard<std::mutex> lk(mu);
                 big_queue.push_back(std::move(r));

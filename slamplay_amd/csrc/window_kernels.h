@@ -11,7 +11,8 @@
 //                 H = J~_p V*^-1, I - G) and b_c = sum J~_c'(r~ - J~_p q) in FP64;
 //   k_wschur      (window, block group) 9x9 blocks of S, each by one wave from LDS-staged
 //                 observation pairs;
-//   k_wsolve      (window) S into LDS, right-looking Cholesky, blocked triangular solves; mixed
+//   k_wsolve      (window) S into LDS (packed lower triangle), right-looking Cholesky, blocked
+//                 triangular solves; mixed
 //                 precision: preconditioned CG in FP64 on the FP64 operator with the FP32 factor
 //                 as preconditioner (S y = D_c^2 y + sum J~_c'(J~_c y - J~_p z),
 //                 z = V*^-1 sum J~_p'J~_c y), until the relative residual is 1e-8, so the step is
@@ -325,28 +326,31 @@ __device__ __forceinline__ void win_schur(const int4* __restrict__ blk, const in
     }
 }
 
-// In-place right-looking Cholesky of the n x n LDS matrix L (lower triangle used), one column per
-// step over the whole workgroup; dinv receives 1 / L_jj.  *bad is set when a pivot is not positive
-// (the factor is then meaningless and the caller rejects the step).
+// Packed lower-triangular storage: element (i, j), j <= i, at i (i + 1) / 2 + j (row i contiguous).
+__device__ __forceinline__ int tri(int i) { return i * (i + 1) / 2; }
+
+// In-place right-looking Cholesky of the packed n x n matrix L, one column per step over the whole
+// workgroup; dinv receives 1 / L_jj.  *bad is set when a pivot is not positive (the factor is then
+// meaningless and the caller rejects the step).
 template <typename F, int NT>
 __device__ __forceinline__ void win_cholesky(F* L, int n, F* colv, F* dinv, int* bad) {
     constexpr int NW = NT / 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int j = 0; j < n; ++j) {
-        F djj = L[(size_t)j * n + j];
+        F djj = L[tri(j) + j];
         const bool nonpos = !(djj > (F)0);
         if (nonpos) djj = (F)1;
         const F d = sqrt(djj), inv = (F)1 / d;
-        for (int i = j + 1 + tid; i < n; i += NT) colv[i] = L[(size_t)i * n + j] * inv;
+        for (int i = j + 1 + tid; i < n; i += NT) colv[i] = L[tri(i) + j] * inv;
         __syncthreads();
         if (tid == 0) {
-            L[(size_t)j * n + j] = d;
+            L[tri(j) + j] = d;
             dinv[j] = inv;
             if (nonpos) *bad = 1;
         }
         for (int i = j + 1 + wave; i < n; i += NW) {
             const F lij = colv[i];
-            F* row = L + (size_t)i * n;
+            F* row = L + tri(i);
             if (lane == 0) row[j] = lij;
             for (int k = j + 1 + lane; k <= i; k += 64) row[k] -= lij * colv[k];
         }
@@ -354,9 +358,9 @@ __device__ __forceinline__ void win_cholesky(F* L, int n, F* colv, F* dinv, int*
     }
 }
 
-// x <- (L L')^-1 x for the LDS vector x (double), L the factor above.  Blocks of kWinTb rows: the
-// diagonal block by wave 0 (one row per lane), the rest of the vector by the workgroup.  Expects x
-// complete on entry (after a barrier) and leaves it complete.
+// x <- (L L')^-1 x for the LDS vector x (double), L the packed factor above.  Blocks of kWinTb
+// rows: the diagonal block by wave 0 (one row per lane), the rest of the vector by the workgroup.
+// Expects x complete on entry (after a barrier) and leaves it complete.
 template <typename F, int NT>
 __device__ __forceinline__ void win_chol_solve(const F* L, const F* dinv, int n, double* x) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -368,13 +372,13 @@ __device__ __forceinline__ void win_chol_solve(const F* L, const F* dinv, int n,
             for (int c = 0; c < m; ++c) {
                 const double zc = __shfl(v, c, 64) * (double)dinv[kb + c];
                 if (lane == c) v = zc;
-                if (lane > c && lane < m) v -= (double)L[(size_t)(kb + lane) * n + kb + c] * zc;
+                if (lane > c && lane < m) v -= (double)L[tri(kb + lane) + kb + c] * zc;
             }
             if (lane < m) x[kb + lane] = v;
         }
         __syncthreads();
         for (int i = kb + m + tid; i < n; i += NT) {
-            const F* row = L + (size_t)i * n + kb;
+            const F* row = L + tri(i) + kb;
             double s = 0.0;
             for (int c = 0; c < m; ++c) s += (double)row[c] * x[kb + c];
             x[i] -= s;
@@ -390,14 +394,14 @@ __device__ __forceinline__ void win_chol_solve(const F* L, const F* dinv, int n,
             for (int c = m - 1; c >= 0; --c) {
                 const double yc = __shfl(v, c, 64) * (double)dinv[kb + c];
                 if (lane == c) v = yc;
-                if (lane < c) v -= (double)L[(size_t)(kb + c) * n + kb + lane] * yc;
+                if (lane < c) v -= (double)L[tri(kb + c) + kb + lane] * yc;
             }
             if (lane < m) x[kb + lane] = v;
         }
         __syncthreads();
         for (int i = tid; i < kb; i += NT) {
             double s = 0.0;
-            for (int c = 0; c < m; ++c) s += (double)L[(size_t)(kb + c) * n + i] * x[kb + c];
+            for (int c = 0; c < m; ++c) s += (double)L[tri(kb + c) + i] * x[kb + c];
             x[i] -= s;
         }
         __syncthreads();
@@ -637,6 +641,7 @@ __global__ void __launch_bounds__(NT) k_wdamp(const WinDesc* __restrict__ descs,
                 st->termination = BA_CONVERGENCE;
             }
         }
+        if (tid == 0 && st->done) st->t_done = wall_clock64();
         __syncthreads();
         if (st->done) return;
     }
@@ -741,7 +746,8 @@ __global__ void __launch_bounds__(NT) k_wschur(const WinDesc* __restrict__ descs
     const WinDesc D = descs[tk.x];
     const WinPtrs w = win_ptrs(D, arena);
     const WinState* st = w.st;
-    if (st->done || st->invalid || tk.y >= D.n_blocks || tk.y + (size_t)tk.z * (tk.w - 1) >= (size_t)D.n_blocks) return;
+    if (st->done || st->invalid || tk.y >= D.n_blocks || tk.y + (int64_t)tk.z * (tk.w - 1) >= (int64_t)D.n_blocks)
+        return;
     if (threadIdx.x == 0) counter = 0;
     __syncthreads();
     const int wave = threadIdx.x >> 6;
@@ -768,7 +774,9 @@ __global__ void __launch_bounds__(NT) k_wsolve(const WinDesc* __restrict__ descs
     if (st->done || st->invalid) return;
     const int K = w.K, P = w.P, n = 9 * K, tid = threadIdx.x;
     const F* Sg = at<F>(arena, D.o_S);
-    F* S = reinterpret_cast<F*>(smem);
+    // the packed factor lives in LDS, or for reduced systems larger than the LDS holds in the
+    // window's own global buffer (L2-resident at these sizes)
+    F* S = D.o_L ? at<F>(arena, D.o_L) : reinterpret_cast<F*>(smem);
     for (int attempt = 0;; ++attempt) {
         F shift = (F)0;
         if (attempt > 0) {
@@ -776,11 +784,16 @@ __global__ void __launch_bounds__(NT) k_wsolve(const WinDesc* __restrict__ descs
             for (int k = 0; k < n; ++k) dmax = fmax(dmax, (double)Sg[(size_t)k * n + k]);
             shift = (F)(dmax * 1e-6 * pow(100.0, attempt - 1));
         }
-        for (int idx = tid; idx < n * n; idx += NT) S[idx] = Sg[idx];
+        {
+            constexpr int NW = NT / 64;
+            const int lane = tid & 63, wave = tid >> 6;
+            for (int i = wave; i < n; i += NW)
+                for (int j = lane; j <= i; j += 64) S[tri(i) + j] = Sg[(size_t)i * n + j];
+        }
         if (tid == 0) bad = 0;
         __syncthreads();
         if (attempt > 0)
-            for (int k = tid; k < n; k += NT) S[(size_t)k * n + k] += shift;
+            for (int k = tid; k < n; k += NT) S[tri(k) + k] += shift;
         __syncthreads();
         win_cholesky<F, NT>(S, n, colv, dinv, &bad);
         if (bad == 0 || cg_max == 0 || attempt == 3) break;
@@ -968,6 +981,11 @@ __global__ void __launch_bounds__(NT) k_wcand_cam(const WinDesc* __restrict__ de
     }
 }
 
+// The device wall clock at the start of a batch run (windows' finishing times are relative to it).
+__global__ void k_wclock(unsigned long long* t) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *t = wall_clock64();
+}
+
 // Per window: the LM decision (Ceres rules) and, for an accepted step, the new parameters.
 template <int NT>
 __global__ void __launch_bounds__(NT) k_wdecide(const WinDesc* __restrict__ descs, char* __restrict__ arena,
@@ -1055,6 +1073,7 @@ __global__ void __launch_bounds__(NT) k_wdecide(const WinDesc* __restrict__ desc
                 st->termination = BA_CONVERGENCE;
             }
         }
+        if (st->done) st->t_done = wall_clock64();
     }
     __syncthreads();
     if (accepted) {

@@ -249,6 +249,37 @@ __device__ __forceinline__ void load_point(const double* __restrict__ pts, int p
     X[2] = (Real)v.z;
 }
 
+// Inputs of one observation (point index and position, measurement), loaded ahead of their use so
+// that the index -> point gather of the next observation overlaps the current one's arithmetic.
+template <typename Real>
+struct ObsIn {
+    int p;
+    double X[3];
+    Real u, v;
+};
+
+template <typename Real>
+__device__ __forceinline__ void load_obs(const Dev<Real>& d, int j, ObsIn<Real>& in) {
+    in.p = d.co_pt[j];
+    load_point<double>(d.pt_x, in.p, in.X);
+    in.u = d.co_uv[2 * (size_t)j];
+    in.v = d.co_uv[2 * (size_t)j + 1];
+}
+
+// Corrected residual and Jacobians from prefetched inputs (same arithmetic as corrected_jacobian).
+template <typename Real>
+__device__ __forceinline__ void corrected_from(const Dev<Real>& d, const CameraPrep<Real>& P, const ObsIn<Real>& in,
+                                               Real r[2], Real Jc[18], Real Jp[6]) {
+    snavely_residual_jacobian<Real>(P, in.X, in.u, in.v, r, Jc, Jp);
+    const Real w = huber_sqrt_rho1<Real>(r[0] * r[0] + r[1] * r[1], d.o.loss, (Real)d.o.loss_a);
+    r[0] *= w;
+    r[1] *= w;
+#pragma unroll
+    for (int i = 0; i < 18; ++i) Jc[i] *= w;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) Jp[i] *= w;
+}
+
 // Corrected (sqrt(rho')-weighted) residual and Jacobians of observation j of camera prep P.
 template <typename Real>
 __device__ __forceinline__ void corrected_jacobian(const Dev<Real>& d, const CameraPrep<Real>& P, int j, int& p,
@@ -713,6 +744,11 @@ __global__ void __launch_bounds__(kWG) k_jtx(Dev<Real> d, int mode, int it, cons
     __shared__ double vsh[9];
     const int c = xcd_remap(blockIdx.x, d.nc);
     if (c >= d.nc) return;
+    // the first observation's inputs are on their way while beta is reduced
+    const int j_end = d.cam_off[c + 1];
+    int j = d.cam_off[c] + threadIdx.x;
+    ObsIn<Real> in{};
+    if (j < j_end) load_obs<Real>(d, j, in);
     double beta = 0.0;
     if (mode == 1) {
         double rz, rr;
@@ -746,10 +782,11 @@ __global__ void __launch_bounds__(kWG) k_jtx(Dev<Real> d, int mode, int it, cons
     Real vs[9];
     for (int i = 0; i < 9; ++i) vs[i] = (Real)(d.sc[9 * (size_t)c + i] * vsh[i]);
     const CameraPrep<Real> P = d.prep[c];
-    for (int j = d.cam_off[c] + threadIdx.x; j < d.cam_off[c + 1]; j += kWG) {
-        int p;
+    for (; j < j_end; j += kWG) {
+        const ObsIn<Real> cur = in;
+        if (j + kWG < j_end) load_obs<Real>(d, j + kWG, in);
         Real r[2], Jc[18], Jp[6];
-        corrected_jacobian<Real>(d, P, j, p, r, Jc, Jp);
+        corrected_from<Real>(d, P, cur, r, Jc, Jp);
         Real a0 = 0, a1 = 0;
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
@@ -818,6 +855,10 @@ __global__ void __launch_bounds__(kWG) k_jz(Dev<Real> d) {
     __shared__ double psh[9];
     const int c = xcd_remap(blockIdx.x, d.nc);
     if (c >= d.nc) return;
+    const int j_end = d.cam_off[c + 1];
+    int j = d.cam_off[c] + threadIdx.x;
+    ObsIn<Real> in{};
+    if (j < j_end) load_obs<Real>(d, j, in);
     if (threadIdx.x < 9) psh[threadIdx.x] = d.pp[9 * (size_t)c + threadIdx.x];
     __syncthreads();
     Real ps[9];
@@ -826,11 +867,12 @@ __global__ void __launch_bounds__(kWG) k_jz(Dev<Real> d) {
     double acc[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) acc[i] = 0.0;
-    for (int j = d.cam_off[c] + threadIdx.x; j < d.cam_off[c + 1]; j += kWG) {
-        int p;
+    for (; j < j_end; j += kWG) {
+        const ObsIn<Real> cur = in;
+        if (j + kWG < j_end) load_obs<Real>(d, j + kWG, in);
         Real r[2], Jc[18], Jp[6];
-        corrected_jacobian<Real>(d, P, j, p, r, Jc, Jp);
-        const R4<Real> z = d.zs[p];
+        corrected_from<Real>(d, P, cur, r, Jc, Jp);
+        const R4<Real> z = d.zs[cur.p];
         Real e0 = -(Jp[0] * z.x + Jp[1] * z.y + Jp[2] * z.z);
         Real e1 = -(Jp[3] * z.x + Jp[4] * z.y + Jp[5] * z.z);
 #pragma unroll

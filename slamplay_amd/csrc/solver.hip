@@ -141,56 +141,46 @@ void build_pair_blocks(const Layout& L, PairBlocks* out) {
     for (int p = 0; p < L.np; ++p) pt_off[p + 1] += pt_off[p];
     std::vector<int> pt_obs(L.no), fill(pt_off.begin(), pt_off.end() - 1);
     for (int j = 0; j < L.no; ++j) pt_obs[fill[L.co_pt[j]]++] = j;
-    struct P3 {
-        int64_t key;
-        int i, j;
-    };
-    std::vector<P3> pairs;
-    size_t total = 0;
-    for (int p = 0; p < L.np; ++p) {
-        const size_t t = (size_t)(pt_off[p + 1] - pt_off[p]);
-        total += t * (t + 1) / 2 + t;
-    }
-    pairs.reserve(total);
+    // counting sort of the pairs (i, j), camera(i) <= camera(j), of every point by block key
+    // camera(i) * nc + camera(j); within a block the pairs keep point order
+    const size_t nkey = (size_t)nc * nc;
+    std::vector<int64_t> cnt(nkey + 1, 0);
     for (int p = 0; p < L.np; ++p)
         for (int x = pt_off[p]; x < pt_off[p + 1]; ++x)
             for (int y = pt_off[p]; y < pt_off[p + 1]; ++y) {
-                const int i = pt_obs[x], j = pt_obs[y];
-                if (cam_of[i] <= cam_of[j]) pairs.push_back({(int64_t)cam_of[i] * nc + cam_of[j], i, j});
+                const int a = cam_of[pt_obs[x]], b = cam_of[pt_obs[y]];
+                if (a <= b) cnt[(size_t)a * nc + b + 1]++;
             }
-    std::stable_sort(pairs.begin(), pairs.end(), [](const P3& a, const P3& b) { return a.key < b.key; });
+    for (size_t k = 0; k < nkey; ++k) cnt[k + 1] += cnt[k];
     PairBlocks& o = *out;
     o = PairBlocks{};
-    o.i.reserve(pairs.size());
-    o.j.reserve(pairs.size());
-    size_t q = 0;
-    for (int a = 0; a < nc; ++a) {
-        bool diag_done = false;
-        while (q < pairs.size() && pairs[q].key / nc == a) {
-            const int b = (int)(pairs[q].key % nc);
-            if (b > a && !diag_done) {
-                o.a.push_back(a);
-                o.b.push_back(a);
-                o.off.push_back((int)o.i.size());
-                diag_done = true;
-            }
-            o.a.push_back(a);
-            o.b.push_back(b);
-            o.off.push_back((int)o.i.size());
-            if (b == a) diag_done = true;
-            while (q < pairs.size() && pairs[q].key == (int64_t)a * nc + b) {
-                o.i.push_back(pairs[q].i);
-                o.j.push_back(pairs[q].j);
-                ++q;
-            }
-        }
-        if (!diag_done) {
-            o.a.push_back(a);
-            o.b.push_back(a);
-            o.off.push_back((int)o.i.size());
-        }
+    o.i.resize(cnt[nkey]);
+    o.j.resize(cnt[nkey]);
+    {
+        std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+        for (int p = 0; p < L.np; ++p)
+            for (int x = pt_off[p]; x < pt_off[p + 1]; ++x)
+                for (int y = pt_off[p]; y < pt_off[p + 1]; ++y) {
+                    const int i = pt_obs[x], j = pt_obs[y];
+                    const int a = cam_of[i], b = cam_of[j];
+                    if (a <= b) {
+                        const int64_t q = pos[(size_t)a * nc + b]++;
+                        o.i[q] = i;
+                        o.j[q] = j;
+                    }
+                }
     }
-    o.off.push_back((int)o.i.size());
+    // blocks (a, b) with pairs, in key order; every diagonal block is listed
+    for (int a = 0; a < nc; ++a)
+        for (int b = a; b < nc; ++b) {
+            const size_t k = (size_t)a * nc + b;
+            if (a == b || cnt[k + 1] > cnt[k]) {
+                o.a.push_back(a);
+                o.b.push_back(b);
+                o.off.push_back((int)cnt[k]);
+            }
+        }
+    o.off.push_back((int)cnt[nkey]);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <atomic>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -91,7 +92,8 @@ int64_t schur_lds(int nt) {
 // Host image of one window's arena region (offsets relative to the region start).
 struct WindowImage {
     WinDesc desc{};
-    std::vector<char> bytes;
+    std::vector<char> bytes;  // the region's inputs (the first bytes.size() bytes of the region)
+    int64_t total = 0;        // bytes of the whole region; the rest starts zeroed on the device
     std::vector<int> pt_perm;
     int K = 0, P = 0;
     int64_t cost = 0;  // work estimate for dispatch order
@@ -132,9 +134,11 @@ int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowIm
     D.o_pairs = take(8 * (int64_t)npairs);
     D.o_cam_x = take(72 * (int64_t)K);
     D.o_cam_x0 = take(72 * (int64_t)K);
-    D.o_cam_c = take(72 * (int64_t)K);
     D.o_pt_x = take(32 * (int64_t)P);
     D.o_pt_x0 = take(32 * (int64_t)P);
+    D.o_state = take(sizeof(WinState));
+    const int64_t input_bytes = off;  // everything below is work space, zeroed on the device
+    D.o_cam_c = take(72 * (int64_t)K);
     D.o_pt_c = take(32 * (int64_t)P);
     D.o_prep = take(sizeof(CameraPrep<double>) * (int64_t)K);
     D.o_prep_c = take(sizeof(CameraPrep<double>) * (int64_t)K);
@@ -157,8 +161,8 @@ int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowIm
     D.o_js = take(kWinJs * sizeof(Real) * (int64_t)No);
     D.o_S = take(sizeof(Real) * (int64_t)n * n);  // zero: blocks without common points are never written
     D.o_L = n > max_window_n<Real>() ? take(sizeof(Real) * (int64_t)n * (n + 1) / 2) : 0;
-    D.o_state = take(sizeof(WinState));
-    img->bytes.assign(off, 0);
+    img->total = off;
+    img->bytes.assign(input_bytes, 0);
     char* b = img->bytes.data();
     std::memcpy(b + D.o_cam_off, L.cam_off.data(), 4 * (size_t)(K + 1));
     std::memcpy(b + D.o_co_pt, L.co_pt.data(), 4 * (size_t)No);
@@ -328,8 +332,10 @@ struct WindowEngine {
     // Places an image at arena offset `base` in slot `slot` (asynchronous on the engine stream;
     // the image must stay alive until the stream has consumed it).
     int place(const WindowImage& img, int slot, int64_t base) {
-        if (base + (int64_t)img.bytes.size() > arena_cap) return fail(BA_ERR_CAPACITY, "window does not fit the arena");
-        BA_HIP(hipMemcpyAsync(arena.p + base, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice, s));
+        if (base + img.total > arena_cap) return fail(BA_ERR_CAPACITY, "window does not fit the arena");
+        const size_t in = img.bytes.size();
+        BA_HIP(hipMemcpyAsync(arena.p + base, img.bytes.data(), in, hipMemcpyHostToDevice, s));
+        BA_HIP(hipMemsetAsync(arena.p + base + in, 0, (size_t)img.total - in, s));
         hdesc[slot] = rebase(img.desc, base);
         BA_HIP(hipMemcpyAsync(ddesc.p + slot, &hdesc[slot], sizeof(WinDesc), hipMemcpyHostToDevice, s));
         work_slots.clear();  // the task lists depend on the windows' sizes: rebuild them
@@ -570,7 +576,7 @@ struct Batch final : BatchBase {
                 continue;
             }
             BA_TRY(build_window_image<Real>(&views[w], o, &imgs[w]));
-            total += (int64_t)imgs[w].bytes.size();
+            total += imgs[w].total;
             small.push_back(w);
         }
         large_sum.assign(large_solvers.size(), ba_summary{});
@@ -578,7 +584,7 @@ struct Batch final : BatchBase {
         int64_t base = 0;
         for (int w : small) {
             BA_TRY(eng.place(imgs[w], w, base));
-            base += (int64_t)imgs[w].bytes.size();
+            base += imgs[w].total;
         }
         order = small;
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return imgs[a].cost > imgs[b].cost; });
@@ -698,6 +704,10 @@ struct Request {
     int nc = 0, np = 0;
     std::vector<int> ci, pi;
     std::vector<double> obs, cams, pts;
+    // device image, built by the server's builder threads after submit
+    std::unique_ptr<WindowImage> img;
+    int build_status = BA_OK;
+    std::atomic<int> ready{0};
     ba_problem_view view() {
         ba_problem_view v{};
         v.num_cameras = nc;
@@ -736,13 +746,21 @@ struct Server final : ServerBase {
     std::deque<std::unique_ptr<Request>> queue;
     std::deque<Done> done;
     std::vector<Done> polled;   // keeps the last poll's parameter arrays alive
-    // requests too large for a window slot: solved by the single-problem solver on a worker thread
-    std::thread worker;
+    // requests too large for a window slot: solved by the single-problem solver on worker threads
+    std::vector<std::thread> workers;
     std::mutex mu;
     std::condition_variable cv;
     std::deque<std::unique_ptr<Request>> big_queue;
     bool stopping = false;
     int big_inflight = 0;
+    // builder threads: device images of submitted windows are built off the stepping thread
+    std::vector<std::thread> builders;
+    std::mutex bmu;
+    std::condition_variable bcv;
+    std::deque<Request*> build_queue;
+    bool bstop = false;
+    char* pinned = nullptr;  // staging of finished windows' parameters
+    size_t pinned_cap = 0;
 
     ~Server() override {
         {
@@ -750,7 +768,34 @@ struct Server final : ServerBase {
             stopping = true;
         }
         cv.notify_all();
-        if (worker.joinable()) worker.join();
+        for (auto& t : workers)
+            if (t.joinable()) t.join();
+        {
+            std::lock_guard<std::mutex> lk(bmu);
+            bstop = true;
+        }
+        bcv.notify_all();
+        for (auto& t : builders)
+            if (t.joinable()) t.join();
+        if (pinned) (void)hipHostFree(pinned);
+    }
+
+    void build_loop() {
+        for (;;) {
+            Request* r = nullptr;
+            {
+                std::unique_lock<std::mutex> lk(bmu);
+                bcv.wait(lk, [&] { return bstop || !build_queue.empty(); });
+                if (bstop) return;
+                r = build_queue.front();
+                build_queue.pop_front();
+            }
+            auto img = std::make_unique<WindowImage>();
+            ba_problem_view v = r->view();
+            r->build_status = build_window_image<Real>(&v, cfg.options, img.get());
+            r->img = std::move(img);
+            r->ready.store(1, std::memory_order_release);
+        }
     }
 
     double clock() const override {
@@ -769,7 +814,9 @@ struct Server final : ServerBase {
         res.resize(cfg.max_windows);
         for (int i = cfg.max_windows - 1; i >= 0; --i) free_slots.push_back(i);
         t0 = std::chrono::steady_clock::now();
-        worker = std::thread([this] { big_loop(); });
+        for (int i = 0; i < 2; ++i) workers.emplace_back([this] { big_loop(); });
+        const int nb = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2));
+        for (int i = 0; i < nb; ++i) builders.emplace_back([this] { build_loop(); });
         return BA_OK;
     }
 
@@ -824,7 +871,13 @@ struct Server final : ServerBase {
             cv.notify_one();
             return BA_OK;
         }
+        Request* rp = r.get();
         queue.push_back(std::move(r));
+        {
+            std::lock_guard<std::mutex> lk(bmu);
+            build_queue.push_back(rp);
+        }
+        bcv.notify_one();
         return BA_OK;
     }
 
@@ -868,14 +921,14 @@ struct Server final : ServerBase {
         std::vector<int> placed;
         while (!queue.empty() && !free_slots.empty()) {
             Request& rq = *queue.front();
-            WindowImage img;
-            ba_problem_view v = rq.view();
-            BA_TRY(build_window_image<Real>(&v, cfg.options, &img));
+            if (!rq.ready.load(std::memory_order_acquire)) break;  // FIFO: wait for the head's image
+            if (rq.build_status != BA_OK) return fail(rq.build_status, "building request %lld failed", (long long)rq.tag);
+            WindowImage& img = *rq.img;
             int64_t off = 0;
-            if (!alloc((int64_t)img.bytes.size(), &off)) {
-                if ((int64_t)img.bytes.size() > eng.arena_cap)
-                    return fail(BA_ERR_CAPACITY, "request %lld needs %zu bytes, arena holds %lld", (long long)rq.tag,
-                                img.bytes.size(), (long long)eng.arena_cap);
+            if (!alloc(img.total, &off)) {
+                if (img.total > eng.arena_cap)
+                    return fail(BA_ERR_CAPACITY, "request %lld needs %lld bytes, arena holds %lld", (long long)rq.tag,
+                                (long long)img.total, (long long)eng.arena_cap);
                 break;  // wait for memory to free up (FIFO admission)
             }
             const int slot = free_slots.back();
@@ -883,11 +936,12 @@ struct Server final : ServerBase {
             Resident& r = res[slot];
             r.slot = slot;
             r.off = off;
-            r.size = (int64_t)img.bytes.size();
+            r.size = img.total;
             r.req = std::move(queue.front());
             queue.pop_front();
             r.req->admit = clock();
-            r.img = std::move(img);
+            r.img = std::move(*r.req->img);
+            r.req->img.reset();
             BA_TRY(eng.place(r.img, slot, off));
             active.push_back(slot);
             placed.push_back(slot);
@@ -906,11 +960,38 @@ struct Server final : ServerBase {
             BA_TRY(eng.run(order, cfg.iterations_per_step));
             std::vector<WinState> st;
             BA_TRY(eng.states(order, &st));
+            // parameters of every finished window in one pass through pinned staging memory
+            std::vector<size_t> stage_off(order.size(), 0);
+            size_t need = 0;
+            for (size_t i = 0; i < order.size(); ++i)
+                if (st[i].done) {
+                    const WinDesc& D = eng.hdesc[order[i]];
+                    stage_off[i] = need;
+                    need += 72 * (size_t)D.K + 32 * (size_t)D.P;
+                }
+            if (need > pinned_cap) {
+                if (pinned) (void)hipHostFree(pinned);
+                pinned = nullptr;
+                pinned_cap = 0;
+                BA_HIP(hipHostMalloc((void**)&pinned, need, hipHostMallocDefault));
+                pinned_cap = need;
+            }
+            for (size_t i = 0; i < order.size(); ++i)
+                if (st[i].done) {
+                    const WinDesc& D = eng.hdesc[order[i]];
+                    char* dst = pinned + stage_off[i];
+                    if (D.K) BA_HIP(hipMemcpyAsync(dst, eng.arena.p + D.o_cam_x, 72 * (size_t)D.K, hipMemcpyDeviceToHost, eng.s));
+                    if (D.P)
+                        BA_HIP(hipMemcpyAsync(dst + 72 * (size_t)D.K, eng.arena.p + D.o_pt_x, 32 * (size_t)D.P,
+                                              hipMemcpyDeviceToHost, eng.s));
+                }
+            if (need) BA_HIP(hipStreamSynchronize(eng.s));
             const double now = clock();
             for (size_t i = 0; i < order.size(); ++i) {
                 if (!st[i].done) continue;
                 const int sl = order[i];
                 Resident& r = res[sl];
+                const WinDesc& D = eng.hdesc[sl];
                 Done d;
                 d.r.tag = r.req->tag;
                 d.r.submit_time_s = r.req->submit;
@@ -922,7 +1003,11 @@ struct Server final : ServerBase {
                 d.r.termination_type = st[i].termination;
                 d.cams = std::move(r.req->cams);
                 d.pts = std::move(r.req->pts);
-                BA_TRY(eng.download(sl, r.img.pt_perm, d.cams.data(), d.pts.data()));
+                const char* src = pinned + stage_off[i];
+                std::memcpy(d.cams.data(), src, 72 * (size_t)D.K);
+                const double* px = reinterpret_cast<const double*>(src + 72 * (size_t)D.K);
+                for (int ip = 0; ip < D.P; ++ip)
+                    for (int k = 0; k < 3; ++k) d.pts[3 * (size_t)r.img.pt_perm[ip] + k] = px[4 * (size_t)ip + k];
                 {
                     std::lock_guard<std::mutex> lk(mu);
                     done.push_back(std::move(d));

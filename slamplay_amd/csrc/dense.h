@@ -124,11 +124,15 @@ __global__ void k_convert_S(const double* __restrict__ S, F* __restrict__ A, siz
     if (i < n) A[i] = (F)S[i];
 }
 
-// Factor diagonal tile k in LDS; keep its lower-triangular inverse in Dinv[k].
+// Factor diagonal tile k in LDS; keep its lower-triangular inverse in Dinv[k].  Column j: every
+// thread reads the pivot (after the previous barrier), scales its row of column j, and after one
+// barrier the trailing triangle is updated; the inverse is formed row by row in LDS (row r of
+// L^-1 from rows < r), one barrier per row.
 template <typename F>
 __global__ void __launch_bounds__(kWG) k_potrf_tile(F* __restrict__ A, int ld, int k, F* __restrict__ Dinv, LMState* st) {
     if (st->done || st->step_invalid) return;
     __shared__ F T[kTile][kTile + 1];
+    __shared__ F X[kTile][kTile + 1];
     __shared__ int bad;
     const int t = threadIdx.x;
     const size_t o = (size_t)(kTile * k) * ld + kTile * k;
@@ -136,14 +140,16 @@ __global__ void __launch_bounds__(kWG) k_potrf_tile(F* __restrict__ A, int ld, i
     if (t == 0) bad = 0;
     __syncthreads();
     for (int j = 0; j < kTile; ++j) {
+        const F dj = T[j][j];
+        const bool nonpos = !(dj > (F)0) || !isfinite((double)dj);
+        const F ljj = sqrt(nonpos ? (F)1 : dj), inv = (F)1 / ljj;
+        if (t > j && t < kTile) T[t][j] *= inv;
+        __syncthreads();
+        // T[j][j] is not read again in this step: the update reads column j below the diagonal only
         if (t == 0) {
-            const F dj = T[j][j];
-            if (!(dj > (F)0) || !isfinite((double)dj)) bad = 1;
-            T[j][j] = sqrt(dj > (F)0 ? dj : (F)1);
+            T[j][j] = ljj;
+            if (nonpos) bad = 1;
         }
-        __syncthreads();
-        if (t > j && t < kTile) T[t][j] /= T[j][j];
-        __syncthreads();
         for (int idx = t; idx < kTile * kTile; idx += kWG) {
             const int r = idx / kTile, c = idx % kTile;
             if (c > j && c <= r) T[r][c] -= T[r][j] * T[c][j];
@@ -158,17 +164,17 @@ __global__ void __launch_bounds__(kWG) k_potrf_tile(F* __restrict__ A, int ld, i
         const int r = idx / kTile, c = idx % kTile;
         if (c <= r) A[o + (size_t)r * ld + c] = T[r][c];
     }
-    // inverse of the lower factor: thread c solves L x = e_c
-    if (t < kTile) {
-        const int c = t;
-        F x[kTile];
-        for (int r = 0; r < kTile; ++r) {
+    // X = L^-1, row r from the rows above it (thread c owns column c)
+    for (int r = 0; r < kTile; ++r) {
+        if (t < kTile) {
+            const int c = t;
             F v = (r == c) ? (F)1 : (F)0;
-            for (int m = c; m < r; ++m) v -= T[r][m] * x[m];
-            x[r] = r < c ? (F)0 : v / T[r][r];
+            for (int m = c; m < r; ++m) v -= T[r][m] * X[m][c];
+            X[r][c] = r < c ? (F)0 : v / T[r][r];
         }
-        for (int r = 0; r < kTile; ++r) Dinv[(size_t)k * kTile * kTile + r * kTile + c] = x[r];
+        __syncthreads();
     }
+    for (int idx = t; idx < kTile * kTile; idx += kWG) Dinv[(size_t)k * kTile * kTile + idx] = X[idx / kTile][idx % kTile];
 }
 
 // Panel: A_ik <- A_ik L_kk^-T for i > k (one WG per tile row).

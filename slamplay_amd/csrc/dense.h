@@ -9,11 +9,15 @@
 // Jc~_i' (Hp_i Jp~_j') Jc~_j with Hp_i = Jp~_i Vinv_p, i.e. a 2x2 middle factor.
 //
 // The factorization is a tiled right-looking Cholesky (32 x 32 tiles): per step, the diagonal
-// tile is factored in LDS and its triangular inverse kept, the panel below is multiplied by that
-// inverse (TRSM as a small GEMM) and the trailing lower tiles are updated (SYRK/GEMM).  The
-// triangular solves use the kept tile inverses, so no step has a serial dependency chain longer
-// than one tile.  In mixed precision the factor is FP32 and the step is refined in FP64 against
-// the FP64 S (refine_steps iterations of r = b - S y, y += S_f^-1 r).
+// tile is factored by one wave and its triangular inverse kept, the panel below is multiplied by
+// that inverse (TRSM as a small GEMM) and the trailing lower tiles are updated (SYRK/GEMM); both
+// products run on the matrix cores (v_mfma_f64_16x16x4f64, v_mfma_f32_16x16x4f32), and the
+// trailing update also factors the next diagonal tile, so a step is two launches.  The panel is
+// also stored transposed in the upper tiles (the factorization reads only the lower ones; in FP64
+// the factor overwrites S).  The triangular solves use the kept tile inverses, so no step has a
+// serial dependency chain longer than one tile.  In mixed precision the factor is FP32 and the
+// step is refined in FP64 against the FP64 S (refine_steps iterations of r = b - S y,
+// y += S_f^-1 r).
 #pragma once
 
 #include <algorithm>
@@ -124,108 +128,225 @@ __global__ void k_convert_S(const double* __restrict__ S, F* __restrict__ A, siz
     if (i < n) A[i] = (F)S[i];
 }
 
-// Factor diagonal tile k in LDS; keep its lower-triangular inverse in Dinv[k].  Column j: every
-// thread reads the pivot (after the previous barrier), scales its row of column j, and after one
-// barrier the trailing triangle is updated; the inverse is formed row by row in LDS (row r of
-// L^-1 from rows < r), one barrier per row.
+// ---------------------------------------------------------------------------------------------
+// 32 x 32 tile products on the matrix cores.  A 16 x 16 block of the product X Y' of two 32-column
+// tiles is one wave's eight v_mfma 16x16x4 steps: in step s, lane l supplies X[row0 + l%16][8q + s]
+// and Y[col0 + l%16][8q + s] (q = l/16), so each lane reads 8 consecutive values of one row and the
+// four lanes of a row cover it; every k in [0, 32) enters exactly once.  Accumulator element i of
+// lane l is entry (row(l, i), l%16) of the block; the FP64 and FP32 instructions order rows
+// differently.
+
 template <typename F>
-__global__ void __launch_bounds__(kWG) k_potrf_tile(F* __restrict__ A, int ld, int k, F* __restrict__ Dinv, LMState* st) {
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef double acc __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int i) { return (lane >> 4) + 4 * i; }
+    static __device__ __forceinline__ void load8(const double* p, double (&v)[8]) {
+        const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const double2 t = q[i];
+            v[2 * i] = t.x;
+            v[2 * i + 1] = t.y;
+        }
+    }
+};
+template <>
+struct Mfma<float> {
+    typedef float acc __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int i) { return 4 * (lane >> 4) + i; }
+    static __device__ __forceinline__ void load8(const float* p, float (&v)[8]) {
+        const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const float4 t = q[i];
+            v[4 * i] = t.x;
+            v[4 * i + 1] = t.y;
+            v[4 * i + 2] = t.z;
+            v[4 * i + 3] = t.w;
+        }
+    }
+};
+
+// c + sign * X[16 si .., :] Y[16 sj .., :]' for 32-column tiles X (leading dimension ldx) and Y (ldy);
+// rows start 8-element aligned.
+template <typename F>
+__device__ __forceinline__ typename Mfma<F>::acc tile_mma_nt(const F* __restrict__ X, int ldx, const F* __restrict__ Y,
+                                                             int ldy, int si, int sj, typename Mfma<F>::acc c, F sign) {
+    const int l = threadIdx.x & 63, q = l >> 4, r = l & 15;
+    F xv[8], yv[8];
+    Mfma<F>::load8(X + (size_t)(16 * si + r) * ldx + 8 * q, xv);
+    Mfma<F>::load8(Y + (size_t)(16 * sj + r) * ldy + 8 * q, yv);
+#pragma unroll
+    for (int s = 0; s < 8; ++s) c = Mfma<F>::mma(sign * xv[s], yv[s], c);
+    return c;
+}
+
+// Value of v in lane `lane` (a wave-uniform index) without a trip through LDS.
+__device__ __forceinline__ double read_lane(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane), hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float read_lane(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// 1 / sqrt(d) for d > 0: the hardware estimate and Newton steps r += r (1 - d r^2) / 2 (two in
+// FP64, whose estimate has about 23 correct bits; one in FP32).
+__device__ __forceinline__ double inv_sqrt(double d) {
+    double r = __builtin_amdgcn_rsq(d);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) r = fma(0.5 * r, fma(-d * r, r, 1.0), r);
+    return r;
+}
+__device__ __forceinline__ float inv_sqrt(float d) {
+    const float r = __builtin_amdgcn_rsqf(d);
+    return fmaf(0.5f * r, fmaf(-d * r, r, 1.0f), r);
+}
+
+// Factors the tile held in T (lower triangle used) with one wave and writes L (lower triangle of
+// diagonal tile k of A) and X = L^-1 (Dinv[k], row-major, zero above the diagonal).  Lane r < 32
+// keeps row r in registers; the pivot and the entries l_cj of the scaled column j are the same
+// for every lane and are read from their lanes with v_readlane, so the factorization makes no
+// trip through LDS.  X is formed by columns: lane c < 32 solves L x = e_c by forward substitution
+// (x[m] = 0 for m < c follows by itself) with L broadcast from LDS.  Returns false (and writes
+// nothing) when a pivot is not positive and finite.  Only wave 0 may call it; lanes 32..63 repeat
+// the work of lanes 0..31 and store nothing.
+template <typename F>
+__device__ __forceinline__ bool potrf_tile_wave(F (*T)[kTile + 1], F* dl, F* __restrict__ A, int ld, int k,
+                                                F* __restrict__ Dinv) {
+    const int l = threadIdx.x & 63, r = l & 31;
+    F a[kTile];
+#pragma unroll
+    for (int c = 0; c < kTile; ++c) a[c] = T[r][c];
+    bool bad = false;
+#pragma unroll
+    for (int j = 0; j < kTile; ++j) {
+        const F d = read_lane(a[j], j);
+        const bool nonpos = !(d > (F)0) || !isfinite((double)d);
+        bad |= nonpos;
+        const F dd = nonpos ? (F)1 : d, inv = inv_sqrt(dd), ljj = dd * inv;
+        a[j] = r > j ? a[j] * inv : (r == j ? ljj : (F)0);
+        if (l == 0) dl[j] = inv;
+#pragma unroll
+        for (int c = j + 1; c < kTile; ++c) {
+            const F lcj = read_lane(a[j], c);
+            if (r >= c) a[c] -= a[j] * lcj;
+        }
+    }
+    if (bad) return false;
+#pragma unroll
+    for (int c = 0; c < kTile; ++c) T[r][c] = c <= r ? a[c] : (F)0;
+    wave_sync_lds();
+    const size_t o = (size_t)(kTile * k) * ld + kTile * k;
+    for (int idx = l; idx < kTile * kTile; idx += 64) {
+        const int rr = idx / kTile, cc = idx % kTile;
+        if (cc <= rr) A[o + (size_t)rr * ld + cc] = T[rr][cc];
+    }
+    F x[kTile];
+#pragma unroll
+    for (int m = 0; m < kTile; ++m) {
+        F p0 = 0, p1 = 0;
+#pragma unroll
+        for (int q = 0; q + 1 < m; q += 2) {
+            p0 += T[m][q] * x[q];
+            p1 += T[m][q + 1] * x[q + 1];
+        }
+        if (m & 1) p0 += T[m][m - 1] * x[m - 1];
+        x[m] = ((m == r) ? (F)1 - (p0 + p1) : -(p0 + p1)) * dl[m];
+    }
+    if (l < kTile)
+#pragma unroll
+        for (int m = 0; m < kTile; ++m) Dinv[(size_t)k * kTile * kTile + m * kTile + r] = x[m];
+    return true;
+}
+
+// Diagonal tile k (used for the first tile; later ones are factored by the trailing update).
+template <typename F>
+__global__ void __launch_bounds__(64) k_potrf_tile(F* __restrict__ A, int ld, int k, F* __restrict__ Dinv, LMState* st) {
     if (st->done || st->step_invalid) return;
     __shared__ F T[kTile][kTile + 1];
-    __shared__ F X[kTile][kTile + 1];
-    __shared__ int bad;
-    const int t = threadIdx.x;
+    __shared__ F dl[kTile];
+    const int l = threadIdx.x;
     const size_t o = (size_t)(kTile * k) * ld + kTile * k;
-    for (int idx = t; idx < kTile * kTile; idx += kWG) T[idx / kTile][idx % kTile] = A[o + (size_t)(idx / kTile) * ld + idx % kTile];
-    if (t == 0) bad = 0;
+    for (int idx = l; idx < kTile * kTile; idx += 64) T[idx / kTile][idx % kTile] = A[o + (size_t)(idx / kTile) * ld + idx % kTile];
     __syncthreads();
-    for (int j = 0; j < kTile; ++j) {
-        const F dj = T[j][j];
-        const bool nonpos = !(dj > (F)0) || !isfinite((double)dj);
-        const F ljj = sqrt(nonpos ? (F)1 : dj), inv = (F)1 / ljj;
-        if (t > j && t < kTile) T[t][j] *= inv;
-        __syncthreads();
-        // T[j][j] is not read again in this step: the update reads column j below the diagonal only
-        if (t == 0) {
-            T[j][j] = ljj;
-            if (nonpos) bad = 1;
-        }
-        for (int idx = t; idx < kTile * kTile; idx += kWG) {
-            const int r = idx / kTile, c = idx % kTile;
-            if (c > j && c <= r) T[r][c] -= T[r][j] * T[c][j];
-        }
-        __syncthreads();
-    }
-    if (bad) {
-        if (t == 0) st->step_invalid = 1;
-        return;
-    }
-    for (int idx = t; idx < kTile * kTile; idx += kWG) {
-        const int r = idx / kTile, c = idx % kTile;
-        if (c <= r) A[o + (size_t)r * ld + c] = T[r][c];
-    }
-    // X = L^-1, row r from the rows above it (thread c owns column c)
-    for (int r = 0; r < kTile; ++r) {
-        if (t < kTile) {
-            const int c = t;
-            F v = (r == c) ? (F)1 : (F)0;
-            for (int m = c; m < r; ++m) v -= T[r][m] * X[m][c];
-            X[r][c] = r < c ? (F)0 : v / T[r][r];
-        }
-        __syncthreads();
-    }
-    for (int idx = t; idx < kTile * kTile; idx += kWG) Dinv[(size_t)k * kTile * kTile + idx] = X[idx / kTile][idx % kTile];
+    if (!potrf_tile_wave<F>(T, dl, A, ld, k, Dinv) && l == 0) st->step_invalid = 1;
 }
 
-// Panel: A_ik <- A_ik L_kk^-T for i > k (one WG per tile row).
+// Panel: L_ik = A_ik L_kk^-T = A_ik Dinv_k' for i > k (one workgroup of four waves per tile, one
+// 16 x 16 block per wave).  L_ik goes to tile (i, k) and its transpose to tile (k, i) of the upper
+// triangle, which the factorization does not use, so that both triangular sweeps of the solve read
+// rows.
 template <typename F>
-__global__ void __launch_bounds__(kWG) k_trsm_tiles(F* __restrict__ A, int ld, int k, const F* __restrict__ Dinv, const LMState* st) {
+__global__ void __launch_bounds__(256) k_trsm_tiles(F* __restrict__ A, int ld, int k, const F* __restrict__ Dinv,
+                                                    const LMState* st) {
     if (st->done || st->step_invalid) return;
-    __shared__ F Li[kTile][kTile + 1];
-    __shared__ F X[kTile][kTile + 1];
-    const int i = k + 1 + blockIdx.x, t = threadIdx.x;
-    const size_t o = (size_t)(kTile * i) * ld + kTile * k;
-    for (int idx = t; idx < kTile * kTile; idx += kWG) {
-        Li[idx / kTile][idx % kTile] = Dinv[(size_t)k * kTile * kTile + idx];
-        X[idx / kTile][idx % kTile] = A[o + (size_t)(idx / kTile) * ld + idx % kTile];
-    }
-    __syncthreads();
-    for (int idx = t; idx < kTile * kTile; idx += kWG) {
-        const int r = idx / kTile, c = idx % kTile;
-        F v = 0;
-        for (int m = 0; m <= c; ++m) v += X[r][m] * Li[c][m];
-        A[o + (size_t)r * ld + c] = v;
+    const int i = k + 1 + blockIdx.x, w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int si = w >> 1, sj = w & 1;
+    F* Aik = A + (size_t)(kTile * i) * ld + kTile * k;
+    F* Aki = A + (size_t)(kTile * k) * ld + kTile * i;
+    typename Mfma<F>::acc c = {0, 0, 0, 0};
+    c = tile_mma_nt<F>(Aik, ld, Dinv + (size_t)k * kTile * kTile, kTile, si, sj, c, (F)1);
+    __syncthreads();  // every wave has read A_ik before any block of it is overwritten
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int rr = 16 * si + Mfma<F>::row(l, e), cc = 16 * sj + (l & 15);
+        Aik[(size_t)rr * ld + cc] = c[e];
+        Aki[(size_t)cc * ld + rr] = c[e];
     }
 }
 
-// Trailing update: A_ij -= A_ik A_jk' for k < j <= i (one WG per lower tile).
+// Trailing update A_ij -= L_ik L_jk' for k < j <= i (one workgroup per lower tile, one 16 x 16
+// block per wave, the block above the diagonal of diagonal tiles skipped).  The workgroup of tile
+// (k + 1, k + 1) -- block 0, dispatched first -- keeps its result in LDS and factors it, so the
+// next step's diagonal factor costs no launch of its own.
 template <typename F>
-__global__ void __launch_bounds__(kWG) k_syrk_tiles(F* __restrict__ A, int ld, int k, const LMState* st) {
+__global__ void __launch_bounds__(256) k_syrk_tiles(F* __restrict__ A, int ld, int k, F* __restrict__ Dinv, LMState* st) {
     if (st->done || st->step_invalid) return;
-    __shared__ F Ai[kTile][kTile + 1];
-    __shared__ F Aj[kTile][kTile + 1];
-    const int b = blockIdx.x, t = threadIdx.x;
+    __shared__ F T[kTile][kTile + 1];
+    __shared__ F dl[kTile];
+    const int b = blockIdx.x, w = threadIdx.x >> 6, l = threadIdx.x & 63;
     int ii = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
     while ((ii + 1) * (ii + 2) / 2 <= b) ++ii;
     while (ii * (ii + 1) / 2 > b) --ii;
     const int jj = b - ii * (ii + 1) / 2;
     const int i = k + 1 + ii, j = k + 1 + jj;
-    const size_t oi = (size_t)(kTile * i) * ld + kTile * k, oj = (size_t)(kTile * j) * ld + kTile * k;
-    for (int idx = t; idx < kTile * kTile; idx += kWG) {
-        Ai[idx / kTile][idx % kTile] = A[oi + (size_t)(idx / kTile) * ld + idx % kTile];
-        Aj[idx / kTile][idx % kTile] = A[oj + (size_t)(idx / kTile) * ld + idx % kTile];
+    const int si = w >> 1, sj = w & 1;
+    const bool next_diag = b == 0;
+    F* Aij = A + (size_t)(kTile * i) * ld + kTile * j;
+    if (!(i == j && si < sj)) {
+        typename Mfma<F>::acc c;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) c[e] = Aij[(size_t)(16 * si + Mfma<F>::row(l, e)) * ld + 16 * sj + (l & 15)];
+        c = tile_mma_nt<F>(A + (size_t)(kTile * i) * ld + kTile * k, ld, A + (size_t)(kTile * j) * ld + kTile * k, ld, si, sj,
+                           c, (F)-1);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int rr = 16 * si + Mfma<F>::row(l, e), cc = 16 * sj + (l & 15);
+            if (next_diag)
+                T[rr][cc] = c[e];
+            else
+                Aij[(size_t)rr * ld + cc] = c[e];
+        }
     }
+    if (!next_diag) return;
     __syncthreads();
-    const size_t od = (size_t)(kTile * i) * ld + kTile * j;
-    for (int idx = t; idx < kTile * kTile; idx += kWG) {
-        const int r = idx / kTile, c = idx % kTile;
-        F v = 0;
-        for (int m = 0; m < kTile; ++m) v += Ai[r][m] * Aj[c][m];
-        A[od + (size_t)r * ld + c] -= v;
-    }
+    if (w == 0 && !potrf_tile_wave<F>(T, dl, A, ld, k + 1, Dinv) && l == 0) st->step_invalid = 1;
 }
 
-// Solve (L L') x = b with the tiled factor (single WG); x = y + solution when accumulate != 0.
+// Solve (L L') x = b with the tiled factor (single workgroup); x = y + solution when accumulate != 0.
+// Forward step k: z_k <- Dinv_k z_k, then z_i -= L_ik z_k for the rows below, read from the
+// transposed panel in the upper tiles; backward step k: z_k <- Dinv_k' z_k, then z_i -= L_ki' z_k for
+// the rows above, read from the lower tiles.  Threads run along matrix rows in both sweeps.
 template <typename F>
 __global__ void __launch_bounds__(1024) k_chol_solve(const F* __restrict__ A, int ld, const F* __restrict__ Dinv,
                                                      const double* __restrict__ b, double* __restrict__ x, int n,
@@ -234,37 +355,42 @@ __global__ void __launch_bounds__(1024) k_chol_solve(const F* __restrict__ A, in
     extern __shared__ double z[];
     __shared__ double zt[kTile];
     const int t = threadIdx.x, nt = ld / kTile;
+    const int hr = t >> 5, hm = t & 31;  // (row, column) of the tile element handled in the diagonal products
     for (int i = t; i < ld; i += blockDim.x) z[i] = i < n ? b[i] : 0.0;
     __syncthreads();
     for (int k = 0; k < nt; ++k) {  // forward: L z = b
-        if (t < kTile) {
-            double v = 0.0;
-            for (int m = 0; m <= t; ++m) v += (double)Dinv[(size_t)k * kTile * kTile + t * kTile + m] * z[kTile * k + m];
-            zt[t] = v;
-        }
+        const F* Dk = Dinv + (size_t)k * kTile * kTile;
+        double v = (double)Dk[hr * kTile + hm] * z[kTile * k + hm];
+#pragma unroll
+        for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 32);
+        if (hm == 0) zt[hr] = v;
         __syncthreads();
         if (t < kTile) z[kTile * k + t] = zt[t];
         __syncthreads();
+        const F* U = A + (size_t)(kTile * k) * ld;
         for (int i = kTile * (k + 1) + t; i < ld; i += blockDim.x) {
-            double v = 0.0;
-            for (int m = 0; m < kTile; ++m) v += (double)A[(size_t)i * ld + kTile * k + m] * z[kTile * k + m];
-            z[i] -= v;
+            double s = 0.0;
+#pragma unroll 8
+            for (int m = 0; m < kTile; ++m) s += (double)U[(size_t)m * ld + i] * z[kTile * k + m];
+            z[i] -= s;
         }
         __syncthreads();
     }
     for (int k = nt - 1; k >= 0; --k) {  // backward: L' x = z
-        if (t < kTile) {
-            double v = 0.0;
-            for (int m = t; m < kTile; ++m) v += (double)Dinv[(size_t)k * kTile * kTile + m * kTile + t] * z[kTile * k + m];
-            zt[t] = v;
-        }
+        const F* Dk = Dinv + (size_t)k * kTile * kTile;
+        double v = (double)Dk[hm * kTile + hr] * z[kTile * k + hm];
+#pragma unroll
+        for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 32);
+        if (hm == 0) zt[hr] = v;
         __syncthreads();
         if (t < kTile) z[kTile * k + t] = zt[t];
         __syncthreads();
+        const F* Lk = A + (size_t)(kTile * k) * ld;
         for (int i = t; i < kTile * k; i += blockDim.x) {
-            double v = 0.0;
-            for (int m = 0; m < kTile; ++m) v += (double)A[(size_t)(kTile * k + m) * ld + i] * z[kTile * k + m];
-            z[i] -= v;
+            double s = 0.0;
+#pragma unroll 8
+            for (int m = 0; m < kTile; ++m) s += (double)Lk[(size_t)m * ld + i] * z[kTile * k + m];
+            z[i] -= s;
         }
         __syncthreads();
     }
@@ -334,13 +460,11 @@ struct DenseSchur {
     template <typename F, typename SolverT>
     void factor_and_solve(SolverT& sv, F* A, F* Dinv) {
         auto& d = sv.d;
-        for (int k = 0; k < nt; ++k) {
-            sv.launch("potrf", k_potrf_tile<F>, 1, kWG, A, ld, k, Dinv, d.st);
-            if (k + 1 < nt) {
-                const int m = nt - k - 1;
-                sv.launch("trsm", k_trsm_tiles<F>, m, kWG, A, ld, k, (const F*)Dinv, (const LMState*)d.st);
-                sv.launch("syrk", k_syrk_tiles<F>, m * (m + 1) / 2, kWG, A, ld, k, (const LMState*)d.st);
-            }
+        sv.launch("potrf", k_potrf_tile<F>, 1, 64, A, ld, 0, Dinv, d.st);
+        for (int k = 0; k + 1 < nt; ++k) {
+            const int m = nt - k - 1;
+            sv.launch("trsm", k_trsm_tiles<F>, m, 256, A, ld, k, (const F*)Dinv, (const LMState*)d.st);
+            sv.launch("syrk", k_syrk_tiles<F>, m * (m + 1) / 2, 256, A, ld, k, Dinv, d.st);
         }
         const size_t shm = (size_t)ld * sizeof(double);
         sv.launch_shm("chol_solve", k_chol_solve<F>, 1, 1024, shm, (const F*)A, ld, (const F*)Dinv, (const double*)d.bc,

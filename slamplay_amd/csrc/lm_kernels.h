@@ -124,6 +124,14 @@ struct Dev {
 // ---------------------------------------------------------------------------------------------
 // Reductions
 
+// Orders the LDS accesses of one wave around an exchange through LDS (the
+// wave's LDS operations complete in order; the fences keep the compiler from moving them).
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

@@ -633,7 +633,7 @@ struct Batch final : BatchBase {
 
     int run(ba_summary* summaries) override {
         BA_TRY(use_device(opt.device));
-        BA_TRY(eng.run(order, opt.max_num_iterations + 1, true));
+        BA_TRY(eng.run(order, opt.max_num_iterations + 1, opt.launch_mode == 0));
         for (size_t i = 0; i < large.size(); ++i) BA_TRY(large_solvers[i]->run(&large_sum[i]));
         std::vector<WinState> st;
         BA_TRY(eng.states(order, &st));

@@ -33,14 +33,6 @@
 
 namespace mba {
 
-// Orders the LDS accesses of one wave around an exchange through its private staging area (the
-// wave's LDS operations complete in order; the fences keep the compiler from moving them).
-__device__ __forceinline__ void wave_sync_lds() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 constexpr int kWinMaxN = 288;  // 9 * 32 cameras
 constexpr int kWinMaxK = kWinMaxN / 9;
 constexpr int kWinTb = 16;     // diagonal block of the triangular solves

@@ -11,10 +11,12 @@ beside it, with the final-cost difference between the two.
 
 The metric's second half, batched local-BA latency, is measured beside it: 512 windows of uneven
 size (configuration 4: 5-20 keyframes, 200-3,000 points log-uniform, tracks of 2 frames up to the
-window length), 10 LM iterations each in mixed precision, solved in one launch (every window's
-latency is the launch's); and the serving loop (configuration 5): Poisson arrivals of such windows
-plus 2% of 100-camera/20,000-point problems, per-iteration continuous batching, p50/p99 latency and
-solves/s at several arrival rates.
+window length), 10 LM iterations each, mixed precision and FP64, solved together by one launch
+sequence per LM iteration (each window's latency is the time to its own finish on the device); the
+serving loop (configuration 5): Poisson arrivals of such windows plus 2% of 100-camera/20,000-point
+problems, per-iteration continuous batching, p50/p99 latency and solves/s at several arrival rates,
+beside the FP64 single-problem solver run over the same mix one request at a time; and the dense
+path of configuration 2 (2250 x 2250 reduced system, FP64 and mixed).
 
 The roofline entry is the dominant kernel's algorithmic bytes per launch over its average duration,
 timed with HIP events on the solver's stream; cpu_baseline is the FP64 CPU oracle (one thread) on a
@@ -34,7 +36,7 @@ LM_ITERS = 30
 # HBM bytes per launch of the PCG kernels at this workload (mixed precision), measured with
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by a 1 GiB copy's counts
 # (profiles/c3_mixed_pcg_kernels.md)
-MEASURED_TRAFFIC_BYTES = {"jz": 20.77e6, "jtx": 42.21e6, "pt_z": 37.04e6}
+MEASURED_TRAFFIC_BYTES = {"jz": 20.78e6, "jtx": 42.27e6, "pt_z": 37.04e6}
 
 
 def log(msg):
@@ -124,7 +126,7 @@ def bench_batch(S, seed, steps, warmup):
                     "window_latency_p50_ms": float(np.percentile(lat, 50)),
                     "window_latency_p99_ms": float(np.percentile(lat, 99)),
                     "windows_per_s": len(wins) / (float(np.median(makespan)) / 1e3),
-                    "threads": info["threads"]}
+                    "threads": info["threads"], "device_bytes": info["device_bytes"]}
         log("bench: batch %s: makespan %.2f ms, window p50 %.2f ms p99 %.2f ms" %
             (tag, out[tag]["batch_ms_median"], out[tag]["window_latency_p50_ms"], out[tag]["window_latency_p99_ms"]))
     # one problem per solve on a sample of windows (the plain path a caller would loop over)
@@ -157,6 +159,21 @@ def bench_serving(S, seed, rates, count):
     for i in range(250):
         mix.append(big[(i // 50) % 2] if i % 50 == 49 else pool[i % len(pool)])
     out = {}
+    # the plain comparator: the FP64 single-problem solver over the same request mix, one request
+    # after the other (every 5th request timed; the large problems are every 50th, so 1 in 10 of
+    # the sample, as in the mix)
+    t, n = 0.0, 0
+    for i in range(4, len(mix), 5):
+        sv = S.Solver(mix[i], window_options(S, 0))
+        sv.run()
+        sv.reset()
+        t0 = time.perf_counter()
+        sv.run()
+        t += time.perf_counter() - t0
+        n += 1
+        sv.close()
+    out["fp64_serial_solves_per_s"] = n / t
+    log("bench: serving comparator, FP64 single-problem solves one after the other: %.0f solves/s" % (n / t))
     for rate in rates:
         cfg = S.server_config(arena_bytes=16 << 30, max_windows=1024, iterations_per_step=1,
                               options=window_options(S, 1))
@@ -218,6 +235,35 @@ def bench_c3(S, args):
     return res, sm, p
 
 
+def bench_dense(S, seed, steps):
+    """Configuration 2 (250 cameras, ~473k observations): explicit Schur and the dense Cholesky of
+    the 2250 x 2250 reduced system, 30 LM iterations, FP64 and mixed precision (FP32 factor with FP64
+    refinement)."""
+    p = S.synthetic(2, seed=seed)
+    out = {"cameras": p.num_cameras(), "points": p.num_points(), "observations": p.num_observations(),
+           "lm_iterations": LM_ITERS}
+    cost = {}
+    for prec, tag in ((0, "fp64"), (1, "mixed")):
+        s = S.Solver(p, S.default_options(precision=prec, linear_solver=1, max_num_iterations=LM_ITERS,
+                                          function_tolerance=0.0, gradient_tolerance=0.0, parameter_tolerance=0.0,
+                                          launch_mode=LAUNCH_MODE))
+        s.run()
+        best = None
+        for _ in range(steps):
+            s.reset()
+            t0 = time.perf_counter()
+            r = s.run()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        cost[tag] = r["final_cost"]
+        out[tag + "_iters_per_s"] = (r["num_iterations"] - 1) / best
+        s.close()
+    out["final_cost_rel_diff"] = abs(cost["mixed"] - cost["fp64"]) / cost["fp64"]
+    log("bench: dense configuration 2: fp64 %.1f, mixed %.1f LM iters/s, final cost diff %.2e" %
+        (out["fp64_iters_per_s"], out["mixed_iters_per_s"], out["final_cost_rel_diff"]))
+    return out
+
+
 def calibrate_copy():
     """A plain device-to-device copy of 1 GiB (torch's elementwise copy kernel), five times: under
     rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch calibrate the counters to bytes."""
@@ -242,6 +288,7 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the 1.2M-observation solve (profiling runs)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-dense", action="store_true", help="skip configuration 2 (dense Schur + Cholesky)")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--launch-mode", type=int, default=0, choices=(0, 1),
                     help="1: plain stream launches instead of replayed hipGraphs (for profiler runs)")
@@ -270,7 +317,10 @@ def main():
     if not args.no_c3:
         c3, sm, p = bench_c3(S, args)
         line.update(c3)
-    batch = serving = None
+    batch = serving = dense = None
+    if not args.no_dense:
+        log("bench: dense Schur + Cholesky (configuration 2)")
+        dense = bench_dense(S, args.seed, max(args.steps, 3))
     if not args.no_batch:
         log("bench: batched local BA (configuration 4, 512 windows)")
         batch = bench_batch(S, args.seed, max(args.steps, 5), max(args.warmup, 1))
@@ -281,7 +331,8 @@ def main():
     if not args.no_cpu_baseline and p is not None:
         log("bench: CPU oracle baseline (bounded sample)")
         cpu = cpu_baseline(S, p)
-    line.update(roofline=line.get("roofline"), cpu_baseline=cpu, batched_local_ba=batch, serving=serving)
+    line.update(roofline=line.get("roofline"), cpu_baseline=cpu, batched_local_ba=batch, serving=serving,
+                dense_config2=dense)
     if sm is not None:
         sm.close()
     print(json.dumps(line), flush=True)

@@ -24,6 +24,7 @@
 #include <cmath>
 
 #include "lm_kernels.h"
+#include "mfma.h"
 #include "solver.h"
 
 namespace mba {
@@ -135,45 +136,6 @@ __global__ void k_convert_S(const double* __restrict__ S, F* __restrict__ A, siz
 // four lanes of a row cover it; every k in [0, 32) enters exactly once.  Accumulator element i of
 // lane l is entry (row(l, i), l%16) of the block; the FP64 and FP32 instructions order rows
 // differently.
-
-template <typename F>
-struct Mfma;
-template <>
-struct Mfma<double> {
-    typedef double acc __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ acc mma(double a, double b, acc c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int i) { return (lane >> 4) + 4 * i; }
-    static __device__ __forceinline__ void load8(const double* p, double (&v)[8]) {
-        const double2* q = reinterpret_cast<const double2*>(p);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const double2 t = q[i];
-            v[2 * i] = t.x;
-            v[2 * i + 1] = t.y;
-        }
-    }
-};
-template <>
-struct Mfma<float> {
-    typedef float acc __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ acc mma(float a, float b, acc c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int i) { return 4 * (lane >> 4) + i; }
-    static __device__ __forceinline__ void load8(const float* p, float (&v)[8]) {
-        const float4* q = reinterpret_cast<const float4*>(p);
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const float4 t = q[i];
-            v[4 * i] = t.x;
-            v[4 * i + 1] = t.y;
-            v[4 * i + 2] = t.z;
-            v[4 * i + 3] = t.w;
-        }
-    }
-};
 
 // c + sign * X[16 si .., :] Y[16 sj .., :]' for 32-column tiles X (leading dimension ldx) and Y (ldy);
 // rows start 8-element aligned.

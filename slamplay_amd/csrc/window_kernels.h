@@ -29,6 +29,7 @@
 // share the launch, nor on the block sizes chosen for the launch.
 #pragma once
 
+#include "mfma.h"
 #include "window.h"
 
 namespace mba {
@@ -134,16 +135,23 @@ __device__ __forceinline__ double4& pv_at(const WinPtrs& w, int k, int slot) {
     return w.pv[(size_t)k * w.n_slots + slot];
 }
 
-// sum over the SELL slots of point p of the first slot record (x, y, z), in slot order.
+// sum over the SELL slots of point p of the first slot record (x, y, z), in slot order; the loads
+// of eight rows are issued together.
 __device__ __forceinline__ void win_slot_sum3(const WinPtrs& w, int p, double t[3]) {
     const int ch = p >> 6, l = p & 63, len = w.chunk_len[ch];
     const int base = w.chunk_off[ch];
     t[0] = t[1] = t[2] = 0.0;
-    for (int r = 0; r < len; ++r) {
-        const double4 u = pv_at(w, 0, base + 64 * r + l);
-        t[0] += u.x;
-        t[1] += u.y;
-        t[2] += u.z;
+    for (int r0 = 0; r0 < len; r0 += 8) {
+        double4 u[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) u[i] = pv_at(w, 0, base + 64 * min(r0 + i, len - 1) + l);
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (r0 + i < len) {
+                t[0] += u[i].x;
+                t[1] += u[i].y;
+                t[2] += u[i].z;
+            }
     }
 }
 
@@ -321,30 +329,77 @@ __device__ __forceinline__ void win_schur(const int4* __restrict__ blk, const in
 // Packed lower-triangular storage: element (i, j), j <= i, at i (i + 1) / 2 + j (row i contiguous).
 __device__ __forceinline__ int tri(int i) { return i * (i + 1) / 2; }
 
-// In-place right-looking Cholesky of the packed n x n matrix L, one column per step over the whole
-// workgroup; dinv receives 1 / L_jj.  *bad is set when a pivot is not positive (the factor is then
-// meaningless and the caller rejects the step).
+// In-place blocked right-looking Cholesky of the packed n x n matrix L (LDS, or the window's global
+// buffer); dinv receives 1 / L_jj.  Panels of kWinNb columns: the panel is factored a column at a
+// time by the whole workgroup (scale the column, then update the panel's remaining columns -- a few
+// independent entries per thread), and the trailing lower triangle is updated with the panel by
+// 16 x 16 tiles on the matrix cores (one tile per wave at a time, K = 16 in four steps; entries past
+// n or above the diagonal are neither read nor written).  *bad is set when a pivot is not
+// positive (the factor is then meaningless and the caller rejects the step).  Every entry's
+// arithmetic is fixed by the algorithm, not by the workgroup size.
+constexpr int kWinNb = 16;
+
 template <typename F, int NT>
 __device__ __forceinline__ void win_cholesky(F* L, int n, F* colv, F* dinv, int* bad) {
-    constexpr int NW = NT / 64;
+    constexpr int NW = NT / 64, NB = kWinNb;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int j = 0; j < n; ++j) {
-        F djj = L[tri(j) + j];
-        const bool nonpos = !(djj > (F)0);
-        if (nonpos) djj = (F)1;
-        const F d = sqrt(djj), inv = (F)1 / d;
-        for (int i = j + 1 + tid; i < n; i += NT) colv[i] = L[tri(i) + j] * inv;
-        __syncthreads();
-        if (tid == 0) {
-            L[tri(j) + j] = d;
-            dinv[j] = inv;
-            if (nonpos) *bad = 1;
+    for (int jb = 0; jb < n; jb += NB) {
+        const int je = min(jb + NB, n);
+        for (int j = jb; j < je; ++j) {
+            F djj = L[tri(j) + j];
+            const bool nonpos = !(djj > (F)0);
+            if (nonpos) djj = (F)1;
+            const F d = sqrt(djj), inv = (F)1 / d;
+            for (int i = j + 1 + tid; i < n; i += NT) {
+                const F v = L[tri(i) + j] * inv;
+                colv[i] = v;
+                L[tri(i) + j] = v;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                L[tri(j) + j] = d;
+                dinv[j] = inv;
+                if (nonpos) *bad = 1;
+            }
+            const int wc = je - j - 1;  // panel columns right of j
+            if (wc > 0) {
+                const int rows = n - j - 1;
+                for (int idx = tid; idx < rows * wc; idx += NT) {
+                    const int i = j + 1 + idx / wc, c = j + 1 + idx % wc;
+                    if (c <= i) L[tri(i) + c] -= colv[i] * colv[c];
+                }
+            }
+            __syncthreads();
         }
-        for (int i = j + 1 + wave; i < n; i += NW) {
-            const F lij = colv[i];
-            F* row = L + tri(i);
-            if (lane == 0) row[j] = lij;
-            for (int k = j + 1 + lane; k <= i; k += 64) row[k] -= lij * colv[k];
+        if (je >= n) break;
+        // trailing update: L[r][c] -= sum_{k in [jb, je)} L[r][k] L[c][k] for je <= c <= r < n
+        const int m = n - je, T = (m + 15) / 16;
+        const int l16 = lane & 15, q = lane >> 4;
+        for (int tt = wave; tt < T * (T + 1) / 2; tt += NW) {
+            int ti = (int)((sqrtf(8.0f * tt + 1.0f) - 1.0f) * 0.5f);
+            while ((ti + 1) * (ti + 2) / 2 <= tt) ++ti;
+            while (ti * (ti + 1) / 2 > tt) --ti;
+            const int tk = tt - ti * (ti + 1) / 2;
+            const int r0 = je + 16 * ti, c0 = je + 16 * tk;
+            typename Mfma<F>::acc acc;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = r0 + Mfma<F>::row(lane, e), c = c0 + l16;
+                acc[e] = (r < n && c <= r) ? L[tri(r) + c] : (F)0;
+            }
+            const int ra = r0 + l16, cb = c0 + l16;
+#pragma unroll
+            for (int s4 = 0; s4 < NB / 4; ++s4) {
+                const int k = jb + 4 * s4 + q;
+                const F a = (ra < n && k < je) ? L[tri(ra) + k] : (F)0;
+                const F b = (cb < n && k < je) ? L[tri(cb) + k] : (F)0;
+                acc = Mfma<F>::mma(-a, b, acc);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = r0 + Mfma<F>::row(lane, e), c = c0 + l16;
+                if (r < n && c <= r) L[tri(r) + c] = acc[e];
+            }
         }
         __syncthreads();
     }
@@ -403,28 +458,60 @@ __device__ __forceinline__ void win_chol_solve(const F* L, const F* dinv, int n,
 // out = S y for the reduced camera system in FP64, applied through the j64 records:
 // S y = D_c^2 y + sum_j J~_c,j'(J~_c,j y - J~_p,j z_p(j)), z_p = V*^-1 sum_{j in p} J~_p,j' J~_c,j y.
 // y and out are LDS vectors of 9K; the point vectors z go through the slot records and w.yp.
+// Waves take cameras; each lane keeps kApplyU observations of its camera in flight: the point
+// and slot indices of the next round are fetched while this round's records are used, so a round
+// costs one memory latency.
+// J~ = J diag(s): the camera scale is folded into y (first pass) and into the sums (last pass).
+constexpr int kApplyU = 3;
+
 template <int NT>
 __device__ __forceinline__ void win_apply_S(const WinPtrs& w, int K, int P, const double* y, double* out) {
-    constexpr int NW = NT / 64;
+    constexpr int NW = NT / 64, U = kApplyU;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int No = w.No;
     for (int c = wave; c < K; c += NW) {
-        double scc[9], yc[9];
-        for (int i = 0; i < 9; ++i) {
-            scc[i] = w.sc[9 * (size_t)c + i];
-            yc[i] = y[9 * c + i];
-        }
-        for (int j = w.cam_off[c] + lane; j < w.cam_off[c + 1]; j += 64) {
-            const int p = w.co_pt[j];
-            double Jc[18], Jp[6];
-            win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
-            double t0 = 0.0, t1 = 0.0;
+        double ys[9];
 #pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                t0 += Jc[i] * yc[i];
-                t1 += Jc[9 + i] * yc[i];
+        for (int i = 0; i < 9; ++i) ys[i] = w.sc[9 * (size_t)c + i] * y[9 * c + i];
+        const int j0 = w.cam_off[c], j1 = w.cam_off[c + 1];
+        int pn[U], sn[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = min(j0 + lane + 64 * u, j1 - 1);  // past the camera's end: repeat its last
+            pn[u] = j1 > j0 ? w.co_pt[j] : 0;
+            sn[u] = j1 > j0 ? w.slot_of[j] : 0;
+        }
+        for (int jb = j0 + lane; jb < j1; jb += 64 * U) {
+            double J[U][24], sp[U][3];
+            int sl[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = min(jb + 64 * u, j1 - 1);
+                sl[u] = sn[u];
+#pragma unroll
+                for (int e = 0; e < 24; ++e) J[u][e] = w.j64[(size_t)e * No + j];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) sp[u][i] = w.sp[3 * (size_t)pn[u] + i];
             }
-            pv_at(w, 0, w.slot_of[j]) =
-                make_double4(Jp[0] * t0 + Jp[3] * t1, Jp[1] * t0 + Jp[4] * t1, Jp[2] * t0 + Jp[5] * t1, 0.0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = min(jb + 64 * (U + u), j1 - 1);
+                pn[u] = w.co_pt[j];
+                sn[u] = w.slot_of[j];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (jb + 64 * u >= j1) break;
+                double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+                for (int i = 0; i < 9; ++i) {
+                    t0 += J[u][i] * ys[i];
+                    t1 += J[u][9 + i] * ys[i];
+                }
+                pv_at(w, 0, sl[u]) = make_double4((J[u][18] * t0 + J[u][21] * t1) * sp[u][0],
+                                                  (J[u][19] * t0 + J[u][22] * t1) * sp[u][1],
+                                                  (J[u][20] * t0 + J[u][23] * t1) * sp[u][2], 0.0);
+            }
         }
     }
     __syncthreads();
@@ -432,35 +519,53 @@ __device__ __forceinline__ void win_apply_S(const WinPtrs& w, int K, int P, cons
         double t[3], z[3];
         win_slot_sum3(w, p, t);
         sym3_mul<double>(w.Vinv + 6 * (size_t)p, t, z);
-        for (int i = 0; i < 3; ++i) w.yp[3 * (size_t)p + i] = z[i];
+        const double* sp = w.sp + 3 * (size_t)p;
+        // stored pre-scaled: J~_p z = J_p (s_p * z)
+        for (int i = 0; i < 3; ++i) w.yp[3 * (size_t)p + i] = z[i] * sp[i];
     }
     __syncthreads();
     for (int c = wave; c < K; c += NW) {
-        double scc[9], yc[9], acc[9];
+        double ys[9], acc[9];
+#pragma unroll
         for (int i = 0; i < 9; ++i) {
-            scc[i] = w.sc[9 * (size_t)c + i];
-            yc[i] = y[9 * c + i];
+            ys[i] = w.sc[9 * (size_t)c + i] * y[9 * c + i];
             acc[i] = 0.0;
         }
-        for (int j = w.cam_off[c] + lane; j < w.cam_off[c + 1]; j += 64) {
-            const int p = w.co_pt[j];
-            double Jc[18], Jp[6];
-            win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
-            const double* z = w.yp + 3 * (size_t)p;
-            double e0 = -(Jp[0] * z[0] + Jp[1] * z[1] + Jp[2] * z[2]);
-            double e1 = -(Jp[3] * z[0] + Jp[4] * z[1] + Jp[5] * z[2]);
+        const int j0 = w.cam_off[c], j1 = w.cam_off[c + 1];
+        int pn[U];
 #pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                e0 += Jc[i] * yc[i];
-                e1 += Jc[9 + i] * yc[i];
+        for (int u = 0; u < U; ++u) pn[u] = j1 > j0 ? w.co_pt[min(j0 + lane + 64 * u, j1 - 1)] : 0;
+        for (int jb = j0 + lane; jb < j1; jb += 64 * U) {
+            double J[U][24], zs[U][3];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = min(jb + 64 * u, j1 - 1);
+#pragma unroll
+                for (int e = 0; e < 24; ++e) J[u][e] = w.j64[(size_t)e * No + j];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) zs[u][i] = w.yp[3 * (size_t)pn[u] + i];
             }
 #pragma unroll
-            for (int i = 0; i < 9; ++i) acc[i] += Jc[i] * e0 + Jc[9 + i] * e1;
+            for (int u = 0; u < U; ++u) pn[u] = w.co_pt[min(jb + 64 * (U + u), j1 - 1)];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (jb + 64 * u >= j1) break;
+                double e0 = -(J[u][18] * zs[u][0] + J[u][19] * zs[u][1] + J[u][20] * zs[u][2]);
+                double e1 = -(J[u][21] * zs[u][0] + J[u][22] * zs[u][1] + J[u][23] * zs[u][2]);
+#pragma unroll
+                for (int i = 0; i < 9; ++i) {
+                    e0 += J[u][i] * ys[i];
+                    e1 += J[u][9 + i] * ys[i];
+                }
+#pragma unroll
+                for (int i = 0; i < 9; ++i) acc[i] += J[u][i] * e0 + J[u][9 + i] * e1;
+            }
         }
 #pragma unroll
         for (int i = 0; i < 9; ++i) acc[i] = wave_sum(acc[i]);
         if (lane == 0)
-            for (int i = 0; i < 9; ++i) out[9 * c + i] = w.dc2[9 * (size_t)c + i] * yc[i] + acc[i];
+            for (int i = 0; i < 9; ++i)
+                out[9 * c + i] = w.dc2[9 * (size_t)c + i] * y[9 * c + i] + w.sc[9 * (size_t)c + i] * acc[i];
     }
     __syncthreads();
 }
@@ -567,12 +672,22 @@ __global__ void __launch_bounds__(64) k_wlin_pt(const WinDesc* __restrict__ desc
     if (p < w.P) {
         const int len = w.chunk_len[ch], base = w.chunk_off[ch];
         double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int r = 0; r < len; ++r) {
-            const int slot = base + 64 * r + l;
-            const double4 x = pv_at(w, 0, slot), y = pv_at(w, 1, slot), z = pv_at(w, 2, slot);
-            a[0] += x.x; a[1] += x.y; a[2] += x.z; a[3] += x.w;
-            a[4] += y.x; a[5] += y.y; a[6] += y.z; a[7] += y.w;
-            a[8] += z.x;
+        for (int r0 = 0; r0 < len; r0 += 4) {  // four rows' loads in flight, summed in row order
+            double4 x[4], y[4], z[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int slot = base + 64 * min(r0 + i, len - 1) + l;
+                x[i] = pv_at(w, 0, slot);
+                y[i] = pv_at(w, 1, slot);
+                z[i] = pv_at(w, 2, slot);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (r0 + i < len) {
+                    a[0] += x[i].x; a[1] += x[i].y; a[2] += x[i].z; a[3] += x[i].w;
+                    a[4] += y[i].x; a[5] += y[i].y; a[6] += y[i].z; a[7] += y[i].w;
+                    a[8] += z[i].x;
+                }
         }
         for (int i = 0; i < 6; ++i) w.V[6 * (size_t)p + i] = a[i];
         for (int i = 0; i < 3; ++i) w.gp[3 * (size_t)p + i] = a[6 + i];
@@ -769,18 +884,36 @@ __global__ void __launch_bounds__(NT) k_wsolve(const WinDesc* __restrict__ descs
     // the packed factor lives in LDS, or for reduced systems larger than the LDS holds in the
     // window's own global buffer (L2-resident at these sizes)
     F* S = D.o_L ? at<F>(arena, D.o_L) : reinterpret_cast<F*>(smem);
+    const int ntri = n * (n + 1) / 2;
     for (int attempt = 0;; ++attempt) {
         F shift = (F)0;
         if (attempt > 0) {
-            double dmax = 0.0;
-            for (int k = 0; k < n; ++k) dmax = fmax(dmax, (double)Sg[(size_t)k * n + k]);
-            shift = (F)(dmax * 1e-6 * pow(100.0, attempt - 1));
+            // largest diagonal entry, in a fixed order (wave 0)
+            if (tid < 64) {
+                double dmax = 0.0;
+                for (int k = tid; k < n; k += 64) dmax = fmax(dmax, (double)Sg[(size_t)k * n + k]);
+                dmax = wave_max(dmax);
+                if (tid == 0) sdot = dmax;
+            }
+            __syncthreads();
+            shift = (F)(sdot * 1e-6 * pow(100.0, attempt - 1));
         }
-        {
-            constexpr int NW = NT / 64;
-            const int lane = tid & 63, wave = tid >> 6;
-            for (int i = wave; i < n; i += NW)
-                for (int j = lane; j <= i; j += 64) S[tri(i) + j] = Sg[(size_t)i * n + j];
+        // packed copy of the lower triangle, four loads in flight per thread
+        for (int b0 = tid; b0 < ntri; b0 += 4 * NT) {
+            F v[4];
+            int ix[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int idx = min(b0 + u * NT, ntri - 1);
+                int i = (int)((sqrtf(8.0f * idx + 1.0f) - 1.0f) * 0.5f);
+                while (tri(i + 1) <= idx) ++i;
+                while (tri(i) > idx) --i;
+                ix[u] = idx;
+                v[u] = Sg[(size_t)i * n + (idx - tri(i))];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (b0 + u * NT < ntri) S[ix[u]] = v[u];
         }
         if (tid == 0) bad = 0;
         __syncthreads();

@@ -124,6 +124,29 @@ struct Dev {
 // ---------------------------------------------------------------------------------------------
 // Reductions
 
+// Value of v in lane `lane` (a wave-uniform index) without a trip through LDS.
+__device__ __forceinline__ double read_lane(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane), hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float read_lane(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// 1 / sqrt(d) for d > 0: the hardware estimate and Newton steps r += r (1 - d r^2) / 2 (two in
+// FP64, whose estimate has about 23 correct bits; one in FP32).
+__device__ __forceinline__ double inv_sqrt(double d) {
+    double r = __builtin_amdgcn_rsq(d);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) r = fma(0.5 * r, fma(-d * r, r, 1.0), r);
+    return r;
+}
+__device__ __forceinline__ float inv_sqrt(float d) {
+    const float r = __builtin_amdgcn_rsqf(d);
+    return fmaf(0.5f * r, fmaf(-d * r, r, 1.0f), r);
+}
+
 // Orders the LDS accesses of one wave around an exchange through LDS (the
 // wave's LDS operations complete in order; the fences keep the compiler from moving them).
 __device__ __forceinline__ void wave_sync_lds() {

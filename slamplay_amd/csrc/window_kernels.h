@@ -330,47 +330,79 @@ __device__ __forceinline__ void win_schur(const int4* __restrict__ blk, const in
 __device__ __forceinline__ int tri(int i) { return i * (i + 1) / 2; }
 
 // In-place blocked right-looking Cholesky of the packed n x n matrix L (LDS, or the window's global
-// buffer); dinv receives 1 / L_jj.  Panels of kWinNb columns: the panel is factored a column at a
-// time by the whole workgroup (scale the column, then update the panel's remaining columns -- a few
-// independent entries per thread), and the trailing lower triangle is updated with the panel by
-// 16 x 16 tiles on the matrix cores (one tile per wave at a time, K = 16 in four steps; entries past
-// n or above the diagonal are neither read nor written).  *bad is set when a pivot is not
-// positive (the factor is then meaningless and the caller rejects the step).  Every entry's
-// arithmetic is fixed by the algorithm, not by the workgroup size.
+// buffer); dinv receives 1 / L_jj.  Panels of kWinNb columns: thread t keeps the panel part of
+// rows jb + t + NT s in registers; per column every thread takes the pivot from LDS, scales its
+// rows, and the owners of the panel's top rows publish the scaled entries l_cj and the next pivot
+// (double-buffered), so a column costs one barrier.  The trailing lower triangle is then updated
+// with the panel by 16 x 16 tiles on the matrix cores (one tile per wave at a time, K = 16 in four
+// steps; entries past n or above the diagonal are neither read nor written).  *bad is set when a
+// pivot is not positive (the factor is then meaningless and the caller rejects the step).  Every
+// entry's arithmetic is fixed by the algorithm, not by the workgroup size.
 constexpr int kWinNb = 16;
+
+template <typename F, int NT>
+__device__ __forceinline__ void win_panel(F* L, int n, int jb, int je, F* colv, F* dinv, int* bad) {
+    constexpr int NB = kWinNb, RS = (kWinMaxN + NT - 1) / NT;
+    const int tid = threadIdx.x;
+    F* cbuf = colv;           // 2 x NB published column entries
+    F* pbuf = colv + 2 * NB;  // 2 pivots
+    F a[RS][NB];
+#pragma unroll
+    for (int s = 0; s < RS; ++s) {
+        const int row = jb + tid + NT * s;
+#pragma unroll
+        for (int cc = 0; cc < NB; ++cc)
+            a[s][cc] = (row < n && jb + cc < je && jb + cc <= row) ? L[tri(row) + jb + cc] : (F)0;
+    }
+    if (tid == 0) pbuf[0] = a[0][0];
+    __syncthreads();
+#pragma unroll
+    for (int jj = 0; jj < NB; ++jj) {
+        const int j = jb + jj;
+        if (j >= je) break;  // uniform; the columns past the last panel's end are not visited
+        const int buf = jj & 1;
+        F djj = pbuf[buf];
+        const bool nonpos = !(djj > (F)0);
+        if (nonpos) djj = (F)1;
+        const F d = sqrt(djj), inv = (F)1 / d;
+#pragma unroll
+        for (int s = 0; s < RS; ++s) {
+            const int row = jb + tid + NT * s;
+            a[s][jj] = row > j ? a[s][jj] * inv : (row == j ? d : a[s][jj]);
+        }
+        if (tid > jj && tid < NB) cbuf[buf * NB + tid] = a[0][jj];  // l_(jb + tid) j
+        if (jj + 1 < NB && tid == jj + 1) pbuf[buf ^ 1] = fma(-a[0][jj], a[0][jj], a[0][jj + 1]);
+        if (tid == 0) {
+            dinv[j] = inv;
+            if (nonpos) *bad = 1;
+        }
+        __syncthreads();
+        // (a predicate, not a break: the loop must unroll so that a[][cc] stays in registers)
+#pragma unroll
+        for (int cc = jj + 1; cc < NB; ++cc) {
+            const F lcj = jb + cc < je ? cbuf[buf * NB + cc] : (F)0;
+#pragma unroll
+            for (int s = 0; s < RS; ++s)
+                if (jb + tid + NT * s >= jb + cc) a[s][cc] = fma(-a[s][jj], lcj, a[s][cc]);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < RS; ++s) {
+        const int row = jb + tid + NT * s;
+#pragma unroll
+        for (int cc = 0; cc < NB; ++cc)
+            if (row < n && jb + cc < je && jb + cc <= row) L[tri(row) + jb + cc] = a[s][cc];
+    }
+}
 
 template <typename F, int NT>
 __device__ __forceinline__ void win_cholesky(F* L, int n, F* colv, F* dinv, int* bad) {
     constexpr int NW = NT / 64, NB = kWinNb;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int jb = 0; jb < n; jb += NB) {
         const int je = min(jb + NB, n);
-        for (int j = jb; j < je; ++j) {
-            F djj = L[tri(j) + j];
-            const bool nonpos = !(djj > (F)0);
-            if (nonpos) djj = (F)1;
-            const F d = sqrt(djj), inv = (F)1 / d;
-            for (int i = j + 1 + tid; i < n; i += NT) {
-                const F v = L[tri(i) + j] * inv;
-                colv[i] = v;
-                L[tri(i) + j] = v;
-            }
-            __syncthreads();
-            if (tid == 0) {
-                L[tri(j) + j] = d;
-                dinv[j] = inv;
-                if (nonpos) *bad = 1;
-            }
-            const int wc = je - j - 1;  // panel columns right of j
-            if (wc > 0) {
-                const int rows = n - j - 1;
-                for (int idx = tid; idx < rows * wc; idx += NT) {
-                    const int i = j + 1 + idx / wc, c = j + 1 + idx % wc;
-                    if (c <= i) L[tri(i) + c] -= colv[i] * colv[c];
-                }
-            }
-            __syncthreads();
-        }
+        win_panel<F, NT>(L, n, jb, je, colv, dinv, bad);
+        __syncthreads();
         if (je >= n) break;
         // trailing update: L[r][c] -= sum_{k in [jb, je)} L[r][k] L[c][k] for je <= c <= r < n
         const int m = n - je, T = (m + 15) / 16;

@@ -174,20 +174,22 @@ def bench_serving(S, seed, rates, count):
         sv.close()
     out["fp64_serial_solves_per_s"] = n / t
     log("bench: serving comparator, FP64 single-problem solves one after the other: %.0f solves/s" % (n / t))
-    for rate in rates:
-        cfg = S.server_config(arena_bytes=16 << 30, max_windows=1024, iterations_per_step=1,
-                              options=window_options(S, 1))
-        t0 = time.perf_counter()
-        res = S.simulate(mix, rate_hz=rate, count=count, seed=seed, config=cfg)
-        wall = time.perf_counter() - t0
-        lat = np.array([r["latency_s"] for r in res]) * 1e3
-        small = np.array([r["latency_s"] for i, r in enumerate(res) if i % len(mix) % 50 != 49]) * 1e3
-        out[str(int(rate))] = {"requests": count, "p50_ms": float(np.percentile(lat, 50)),
-                               "p99_ms": float(np.percentile(lat, 99)),
-                               "p99_ms_windows_only": float(np.percentile(small, 99)),
-                               "solves_per_s": count / wall}
-        log("bench: serving %d req/s: p50 %.2f ms p99 %.2f ms, %.0f solves/s" %
-            (rate, out[str(int(rate))]["p50_ms"], out[str(int(rate))]["p99_ms"], count / wall))
+    for prec, tag in ((1, "mixed"), (0, "fp64")):
+        out[tag] = {}
+        for rate in rates:
+            cfg = S.server_config(arena_bytes=16 << 30, max_windows=1024, iterations_per_step=1,
+                                  options=window_options(S, prec))
+            t0 = time.perf_counter()
+            res = S.simulate(mix, rate_hz=rate, count=count, seed=seed, config=cfg)
+            wall = time.perf_counter() - t0
+            lat = np.array([r["latency_s"] for r in res]) * 1e3
+            small = np.array([r["latency_s"] for i, r in enumerate(res) if i % len(mix) % 50 != 49]) * 1e3
+            out[tag][str(int(rate))] = {"requests": count, "p50_ms": float(np.percentile(lat, 50)),
+                                        "p99_ms": float(np.percentile(lat, 99)),
+                                        "p99_ms_windows_only": float(np.percentile(small, 99)),
+                                        "solves_per_s": count / wall}
+            log("bench: serving %s %d req/s: p50 %.2f ms p99 %.2f ms, %.0f solves/s" %
+                (tag, rate, out[tag][str(int(rate))]["p50_ms"], out[tag][str(int(rate))]["p99_ms"], count / wall))
     return out
 
 

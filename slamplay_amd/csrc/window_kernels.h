@@ -336,12 +336,17 @@ __device__ __forceinline__ int tri(int i) { return i * (i + 1) / 2; }
 // (double-buffered), so a column costs one barrier.  The trailing lower triangle is then updated
 // with the panel by 16 x 16 tiles on the matrix cores (one tile per wave at a time, K = 16 in four
 // steps; entries past n or above the diagonal are neither read nor written).  *bad is set when a
-// pivot is not positive (the factor is then meaningless and the caller rejects the step).  Every
-// entry's arithmetic is fixed by the algorithm, not by the workgroup size.
+// pivot is not positive (the factor is then meaningless and the caller rejects the step).  With
+// clamp (the FP32 factor of the mixed-precision solve, a CG preconditioner), a pivot below
+// kWinPivotFloor times the original diagonal entry -- FP32 cancellation in a direction the trust
+// region leaves nearly free -- is raised to that floor instead: the preconditioner then differs from
+// S in a few directions only, which CG resolves in a few steps; dinv holds the original diagonal on
+// entry.  Every entry's arithmetic is fixed by the algorithm, not by the workgroup size.
 constexpr int kWinNb = 16;
+constexpr double kWinPivotFloor = 1e-5;
 
 template <typename F, int NT>
-__device__ __forceinline__ void win_panel(F* L, int n, int jb, int je, F* colv, F* dinv, int* bad) {
+__device__ __forceinline__ void win_panel(F* L, int n, int jb, int je, F* colv, F* dinv, int* bad, bool clamp) {
     constexpr int NB = kWinNb, RS = (kWinMaxN + NT - 1) / NT;
     const int tid = threadIdx.x;
     F* cbuf = colv;           // 2 x NB published column entries
@@ -354,6 +359,9 @@ __device__ __forceinline__ void win_panel(F* L, int n, int jb, int je, F* colv, 
         for (int cc = 0; cc < NB; ++cc)
             a[s][cc] = (row < n && jb + cc < je && jb + cc <= row) ? L[tri(row) + jb + cc] : (F)0;
     }
+    F floor_[NB];  // read before the barrier: dinv[j] is overwritten during the panel
+#pragma unroll
+    for (int cc = 0; cc < NB; ++cc) floor_[cc] = clamp && jb + cc < je ? (F)kWinPivotFloor * dinv[jb + cc] : (F)0;
     if (tid == 0) pbuf[0] = a[0][0];
     __syncthreads();
 #pragma unroll
@@ -362,7 +370,13 @@ __device__ __forceinline__ void win_panel(F* L, int n, int jb, int je, F* colv, 
         if (j >= je) break;  // uniform; the columns past the last panel's end are not visited
         const int buf = jj & 1;
         F djj = pbuf[buf];
-        const bool nonpos = !(djj > (F)0);
+        bool nonpos;
+        if (clamp) {
+            nonpos = !isfinite((double)djj) || !(floor_[jj] > (F)0);
+            if (!nonpos && !(djj > floor_[jj])) djj = floor_[jj];
+        } else {
+            nonpos = !(djj > (F)0);
+        }
         if (nonpos) djj = (F)1;
         const F d = sqrt(djj), inv = (F)1 / d;
 #pragma unroll
@@ -396,12 +410,12 @@ __device__ __forceinline__ void win_panel(F* L, int n, int jb, int je, F* colv, 
 }
 
 template <typename F, int NT>
-__device__ __forceinline__ void win_cholesky(F* L, int n, F* colv, F* dinv, int* bad) {
+__device__ __forceinline__ void win_cholesky(F* L, int n, F* colv, F* dinv, int* bad, bool clamp) {
     constexpr int NW = NT / 64, NB = kWinNb;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int jb = 0; jb < n; jb += NB) {
         const int je = min(jb + NB, n);
-        win_panel<F, NT>(L, n, jb, je, colv, dinv, bad);
+        win_panel<F, NT>(L, n, jb, je, colv, dinv, bad, clamp);
         __syncthreads();
         if (je >= n) break;
         // trailing update: L[r][c] -= sum_{k in [jb, je)} L[r][k] L[c][k] for je <= c <= r < n
@@ -952,7 +966,9 @@ __global__ void __launch_bounds__(NT) k_wsolve(const WinDesc* __restrict__ descs
         if (attempt > 0)
             for (int k = tid; k < n; k += NT) S[tri(k) + k] += shift;
         __syncthreads();
-        win_cholesky<F, NT>(S, n, colv, dinv, &bad);
+        for (int k = tid; k < n; k += NT) dinv[k] = S[tri(k) + k];  // the pivot floors' reference
+        __syncthreads();
+        win_cholesky<F, NT>(S, n, colv, dinv, &bad, cg_max > 0);
         if (bad == 0 || cg_max == 0 || attempt == 3) break;
     }
     if (bad) return;  // have_step stays 0: an invalid step

@@ -55,6 +55,11 @@ typedef enum ba_linear_solver {
 
 typedef enum ba_loss { BA_LOSS_NONE = 0, BA_LOSS_HUBER = 1 } ba_loss;
 
+typedef enum ba_mode {
+    BA_MODE_CERES = 0, /* test_bundle_adjustment_ceres.cpp's SolveBA (Ceres trust-region LM) */
+    BA_MODE_G2O = 1    /* test_bundle_adjustment_g2o.cpp's SolveBA (g2o Levenberg, SO(3) rotation update) */
+} ba_mode;
+
 typedef enum ba_termination { BA_CONVERGENCE = 0, BA_NO_CONVERGENCE = 1, BA_FAILURE = 2 } ba_termination;
 
 /* Zero-copy view of a BAL problem.  cameras: 9 doubles per camera (angle-axis[3], t[3], f, k1, k2),
@@ -99,6 +104,14 @@ typedef struct ba_options {
     double min_lm_diagonal;             /* 1e-6 */
     double max_lm_diagonal;             /* 1e32 */
     double pcg_tolerance;               /* 1e-6: stop when ||r|| <= tol * ||b|| */
+    /* BA_MODE_CERES (default): the Ceres driver's trust-region LM (test_bundle_adjustment_ceres.cpp:96-100).
+     * BA_MODE_G2O: the g2o driver's (test_bundle_adjustment_g2o.cpp:166-218): camera rotation updated as
+     * exp(delta) R, lambda-LM on H + lambda I without column scaling, lambda_0 = g2o_tau * max diag(J'J),
+     * at most max_num_iterations iterations of up to g2o_max_trials trials; costs reported as g2o's chi2
+     * (sum of rho, no 1/2).  Single-problem solves only (the batch and server entry points reject it). */
+    int32_t mode;
+    int32_t g2o_max_trials; /* 10 */
+    double g2o_tau;         /* 1e-5 */
 } ba_options;
 
 /* One row of the per-iteration trace (the Ceres progress table's columns). */

@@ -57,6 +57,9 @@ struct Options {
     double min_lm_diagonal;
     double max_lm_diagonal;
     double pcg_tolerance;
+    int32_t mode;            // 0 Ceres driver, 1 g2o driver (test_bundle_adjustment_g2o.cpp)
+    int32_t g2o_max_trials;  // g2o OptimizationAlgorithmLevenberg: rejected trials per iteration (10)
+    double g2o_tau;          // g2o: lambda_0 = tau * max diag(H) (1e-5)
 };
 
 struct IterRecord {
@@ -215,6 +218,60 @@ void residual_and_jacobian(const double* cam, const double* pt, double ox, doubl
     }
 }
 
+// g2o driver (test_bundle_adjustment_g2o.cpp:85): the rotation is updated as R <- exp(d) R, so its
+// Jacobian columns are taken with respect to d: d p / d d_j = e_j x (R X).  The other columns are
+// the autodiff ones (d p / d t = I is shared), so J_d = J_t [e_0 x q, e_1 x q, e_2 x q], q = R X.
+void left_rotation_columns(const double* cam, const double* pt, double jc[18]) {
+    double q[3];
+    rotate_point(cam, pt, q);
+    const double col[3][3] = {{0.0, -q[2], q[1]}, {q[2], 0.0, -q[0]}, {-q[1], q[0], 0.0}};  // col[j] = e_j x q
+    for (int k = 0; k < 2; ++k)
+        for (int j = 0; j < 3; ++j)
+            jc[9 * k + j] = jc[9 * k + 3] * col[j][0] + jc[9 * k + 4] * col[j][1] + jc[9 * k + 5] * col[j][2];
+}
+
+// SO(3) exp / log through unit quaternions (w, x, y, z) as Sophus defines them (the g2o driver's
+// PoseAndIntrinsics, test_bundle_adjustment_g2o.cpp:48-63): exp(w) = (cos(t/2), sin(t/2) w/t) with
+// series below t = 1e-10; log(q) = 2 atan(n/q_w) v/n with the small-n series and +-pi/n for q_w ~ 0.
+void so3_exp(const double w[3], double q[4]) {
+    const double t2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    const double t = std::sqrt(t2);
+    double re, im;
+    if (t < 1e-10) {
+        re = 1.0 - t2 / 8.0 + t2 * t2 / 384.0;
+        im = 0.5 - t2 / 48.0 + t2 * t2 / 3840.0;
+    } else {
+        re = std::cos(0.5 * t);
+        im = std::sin(0.5 * t) / t;
+    }
+    q[0] = re;
+    q[1] = im * w[0];
+    q[2] = im * w[1];
+    q[3] = im * w[2];
+}
+
+void so3_log(const double q[4], double w[3]) {
+    const double n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    const double n = std::sqrt(n2), qw = q[0];
+    double k;
+    if (n < 1e-10) k = 2.0 / qw - 2.0 * n2 / (3.0 * qw * qw * qw);
+    else if (std::fabs(qw) < 1e-10) k = (qw >= 0.0 ? 3.14159265358979323846 : -3.14159265358979323846) / n;
+    else k = 2.0 * std::atan(n / qw) / n;
+    for (int i = 0; i < 3; ++i) w[i] = k * q[1 + i];
+}
+
+// angle-axis of exp(d) exp(w) (Hamilton product of the two quaternions)
+void so3_left_update(const double d[3], const double w[3], double out[3]) {
+    double a[4], b[4], c[4];
+    so3_exp(d, a);
+    so3_exp(w, b);
+    c[0] = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+    c[1] = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+    c[2] = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+    c[3] = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+    so3_log(c, out);
+}
+
 // ceres::HuberLoss(a): rho(s) = s (s <= a^2), 2 a sqrt(s) - a^2 otherwise.
 void huber(double s, double a, double rho[3]) {
     const double b = a * a;
@@ -277,6 +334,7 @@ bool linearize(const Problem& P, const Options& o, const double* cams, const dou
     for (int k = 0; k < P.no; ++k) {
         double r[2], jc[18], jp[6], w;
         residual_and_jacobian(cams + 9 * P.ci[k], pts + 3 * P.pi[k], P.obs[2 * k], P.obs[2 * k + 1], r, jc, jp);
+        if (o.mode == 1) left_rotation_columns(cams + 9 * P.ci[k], pts + 3 * P.pi[k], jc);
         sum += obs_cost(o, r[0], r[1], &w);
         for (int i = 0; i < 18; ++i) L->jc[18 * (size_t)k + i] = w * jc[i];
         for (int i = 0; i < 6; ++i) L->jp[6 * (size_t)k + i] = w * jp[i];
@@ -692,6 +750,15 @@ void oracle_residual(const double* cam, const double* pt, double ox, double oy, 
 
 void oracle_rotate_point(const double* aa, const double* pt, double* out) { rotate_point(aa, pt, out); }
 
+// g2o-mode Jacobian of one observation: rotation columns with respect to the left update exp(d) R.
+void oracle_jacobian_left(const double* cam, const double* pt, double ox, double oy, double* r, double* jc, double* jp) {
+    residual_and_jacobian(cam, pt, ox, oy, r, jc, jp);
+    left_rotation_columns(cam, pt, jc);
+}
+
+// angle-axis of exp(d) exp(w)
+void oracle_so3_left_update(const double* d, const double* w, double* out) { so3_left_update(d, w, out); }
+
 void oracle_huber(double s, double a, double* rho) { huber(s, a, rho); }
 
 // Total cost 1/2 sum rho(|r|^2) in observation order.
@@ -737,10 +804,119 @@ int oracle_lm_step(int nc, int np, int no, const int* ci, const int* pi, const d
 
 // Ceres-style trust-region LM.  cams/pts are updated in place with the best parameters found.
 // termination: 0 CONVERGENCE, 1 NO_CONVERGENCE, 2 FAILURE.
+// The g2o driver's solve (test_bundle_adjustment_g2o.cpp:166-218: OptimizationAlgorithmLevenberg,
+// optimize(40), marginalized points, CSparse) on the same residual.  g2o 20230223 is not in the
+// reference tree (build_thirdparty.sh:253); its Levenberg algorithm is restated from its public
+// source: lambda_0 = tau * max_i H_ii; per iteration up to max_trials trials of
+// (H + lambda I) x = b, x applied through the vertices' oplus (rotation exp(x) R, the rest additive),
+// rho = (chi2 - chi2_new) / (x'(lambda x + b) + 1e-3); rho > 0: accept, lambda *= max(1/3,
+// min(2/3, 1 - (2 rho - 1)^3)), nu = 2; else lambda *= nu, nu *= 2; the solve stops after max_trials
+// rejections (or rho == 0).  chi2 = sum rho(s) = 2 cost; here x'(lambda x + b) = 2 * model cost
+// change, which is exact for the solved damped system.  Radius = 1 / lambda in the trace; costs in
+// the records are chi2.
+static int solve_g2o(const Problem& P, const Options& o0, double* cams, double* pts, IterRecord* trace, int trace_cap,
+                     int* n_trace, double* initial_cost, double* final_cost, int* termination) {
+    Options o = o0;
+    o.jacobi_scaling = 0;
+    o.min_lm_diagonal = o.max_lm_diagonal = 1.0;
+    const size_t ncam = 9 * (size_t)P.nc, nparam = ncam + 3 * (size_t)P.np;
+    std::vector<double> x(nparam), cand(nparam);
+    std::copy(cams, cams + ncam, x.begin());
+    std::copy(pts, pts + 3 * (size_t)P.np, x.begin() + ncam);
+    int nt = 0;
+    auto push = [&](const IterRecord& r) { if (trace && nt < trace_cap) trace[nt] = r; ++nt; };
+    PointObs po = point_obs(P);
+    Linearization L;
+    if (!linearize(P, o, x.data(), x.data() + ncam, &L)) {
+        *termination = 2;
+        if (n_trace) *n_trace = 0;
+        return 2;
+    }
+    const std::vector<double> ones(nparam, 1.0);
+    std::vector<double> hdiag;
+    jtj_diagonal(P, L, ones, &hdiag);
+    double hmax = 0.0;
+    for (double v : hdiag) hmax = std::max(hmax, v);
+    double radius = 1.0 / (o.g2o_tau * hmax), nu = 2.0;
+    auto grad_max = [&]() { double m = 0.0; for (double g : L.grad) m = std::max(m, std::fabs(g)); return m; };
+    double x_cost = L.cost, gmax = grad_max();
+    *initial_cost = 2.0 * x_cost;
+    IterRecord rec{};
+    rec.step_is_successful = 1;
+    rec.step_is_valid = 1;
+    rec.cost = 2.0 * x_cost;
+    rec.gradient_max_norm = gmax;
+    rec.trust_region_radius = radius;
+    push(rec);
+    int term = 1, trial = 0, iter = 0;
+    bool stop = false;
+    while (!stop && iter < o.max_num_iterations) {
+        for (int t = 0;; ++t) {
+            rec = IterRecord{};
+            rec.iteration = ++trial;
+            std::vector<double> y;
+            const int lin_it = solve_damped(P, o, po, L, ones, ones, radius, &y);
+            rec.linear_solver_iterations = lin_it < 0 ? 0 : lin_it;
+            const bool have = lin_it >= 0;
+            double mcc = 0.0, cand_cost = std::numeric_limits<double>::max(), rho = -1.0;
+            std::vector<double> delta(nparam, 0.0);
+            if (have) {
+                for (size_t i = 0; i < nparam; ++i) delta[i] = -y[i];
+                mcc = model_cost_change(P, L, delta);
+                for (size_t i = 0; i < nparam; ++i) cand[i] = x[i] + delta[i];
+                for (int c = 0; c < P.nc; ++c) so3_left_update(&delta[9 * (size_t)c], &x[9 * (size_t)c], &cand[9 * (size_t)c]);
+                if (!evaluate_cost(P, o, cand.data(), cand.data() + ncam, &cand_cost))
+                    cand_cost = std::numeric_limits<double>::max();
+                rho = 2.0 * (x_cost - cand_cost) / (2.0 * mcc + 1e-3);
+                rec.step_norm = norm2(delta.data(), nparam);
+                rec.cost_change = 2.0 * (x_cost - cand_cost);
+            }
+            rec.step_is_valid = have ? 1 : 0;
+            rec.model_cost_change = 2.0 * mcc;
+            rec.relative_decrease = rho;
+            if (have && rho > 0.0 && cand_cost < std::numeric_limits<double>::max()) {
+                x = cand;
+                linearize(P, o, x.data(), x.data() + ncam, &L);
+                x_cost = L.cost;
+                gmax = grad_max();
+                radius = radius / std::max(1.0 / 3.0, std::min(2.0 / 3.0, 1.0 - std::pow(2.0 * rho - 1.0, 3)));
+                nu = 2.0;
+                rec.step_is_successful = 1;
+                rec.cost = 2.0 * x_cost;
+                rec.gradient_max_norm = gmax;
+                rec.trust_region_radius = radius;
+                push(rec);
+                ++iter;
+                break;
+            }
+            radius = radius / nu;
+            nu *= 2.0;
+            rec.step_is_successful = 0;
+            rec.cost = 2.0 * (have ? cand_cost : x_cost);
+            rec.gradient_max_norm = gmax;
+            rec.trust_region_radius = radius;
+            push(rec);
+            if (t + 1 >= o.g2o_max_trials || rho == 0.0 || !std::isfinite(radius) || !(radius > 0.0)) {
+                stop = true;
+                term = 0;
+                break;
+            }
+        }
+    }
+    std::copy(x.begin(), x.begin() + ncam, cams);
+    std::copy(x.begin() + ncam, x.end(), pts);
+    *final_cost = 2.0 * x_cost;
+    *termination = term;
+    if (n_trace) *n_trace = nt;
+    return term;
+}
+
 int oracle_solve(int nc, int np, int no, const int* ci, const int* pi, const double* obs, double* cams, double* pts,
                  const Options* o, IterRecord* trace, int trace_cap, int* n_trace, double* initial_cost,
                  double* final_cost, int* termination) {
     Problem P{nc, np, no, ci, pi, obs};
+    if (o->mode == 1)
+        return solve_g2o(P, *o, cams, pts, trace, trace_cap, n_trace, initial_cost, final_cost, termination);
     const size_t ncam = 9 * (size_t)nc, nparam = ncam + 3 * (size_t)np;
     std::vector<double> x(nparam), cand(nparam);
     std::copy(cams, cams + ncam, x.begin());

@@ -36,6 +36,9 @@ class Options(ctypes.Structure):
         ("min_lm_diagonal", ctypes.c_double),
         ("max_lm_diagonal", ctypes.c_double),
         ("pcg_tolerance", ctypes.c_double),
+        ("mode", ctypes.c_int32),
+        ("g2o_max_trials", ctypes.c_int32),
+        ("g2o_tau", ctypes.c_double),
     ]
 
 
@@ -74,6 +77,9 @@ def default_options(**kw):
         min_lm_diagonal=1e-6,
         max_lm_diagonal=1e32,
         pcg_tolerance=1e-6,
+        mode=0,
+        g2o_max_trials=10,
+        g2o_tau=1e-5,
     )
     for k, v in kw.items():
         setattr(o, k, v)
@@ -92,6 +98,8 @@ def lib():
         L.oracle_eval.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
         L.oracle_residual.argtypes = [_f64p, _f64p, ctypes.c_double, ctypes.c_double, _f64p]
         L.oracle_rotate_point.argtypes = [_f64p, _f64p, _f64p]
+        L.oracle_jacobian_left.argtypes = [_f64p, _f64p, ctypes.c_double, ctypes.c_double, _f64p, _f64p, _f64p]
+        L.oracle_so3_left_update.argtypes = [_f64p, _f64p, _f64p]
         L.oracle_huber.argtypes = [ctypes.c_double, ctypes.c_double, _f64p]
         L.oracle_cost.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, ctypes.c_int, ctypes.c_double]
         L.oracle_cost.restype = ctypes.c_double
@@ -140,6 +148,22 @@ def rotate_point(aa, pt):
     aa, pt = _c(aa, np.float64), _c(pt, np.float64)
     out = np.zeros(3)
     lib().oracle_rotate_point(_p(aa, _f64p), _p(pt, _f64p), _p(out, _f64p))
+    return out
+
+
+def jacobian_left(cam, pt, ox, oy):
+    """g2o-mode residual (2,), Jc (2,9) with rotation columns for exp(d) R, Jp (2,3)."""
+    cam, pt = _c(cam, np.float64), _c(pt, np.float64)
+    r, jc, jp = np.zeros(2), np.zeros((2, 9)), np.zeros((2, 3))
+    lib().oracle_jacobian_left(_p(cam, _f64p), _p(pt, _f64p), ox, oy, _p(r, _f64p), _p(jc, _f64p), _p(jp, _f64p))
+    return r, jc, jp
+
+
+def so3_left_update(d, w):
+    """angle-axis of exp(d) exp(w)."""
+    d, w = _c(d, np.float64), _c(w, np.float64)
+    out = np.zeros(3)
+    lib().oracle_so3_left_update(_p(d, _f64p), _p(w, _f64p), _p(out, _f64p))
     return out
 
 

@@ -160,10 +160,16 @@ def _summary_dict(s, rows):
                 trace=trace)
 
 
+def _trace_rows(o):
+    """Trace rows a solve can write: one per trial (a g2o iteration may take g2o_max_trials)."""
+    trials = o.max_num_iterations * (o.g2o_max_trials if o.mode == 1 else 1)
+    return max(8, trials + 2)
+
+
 def SolveBA(problem, options=None, **kw):
     """Solve `problem` (a BALProblem) on the GPU in place; returns the summary as a dict."""
     o = options if options is not None else default_options(**kw)
-    s, rows = _summary(max(8, o.max_num_iterations + 2))
+    s, rows = _summary(_trace_rows(o))
     C.check(C.lib().ba_solve(ctypes.byref(problem.view()), ctypes.byref(o), ctypes.byref(s)))
     return _summary_dict(s, rows)
 
@@ -180,7 +186,7 @@ class Solver:
         self._h = h
 
     def run(self):
-        s, rows = _summary(max(8, self.options.max_num_iterations + 2))
+        s, rows = _summary(_trace_rows(self.options))
         C.check(C.lib().ba_solver_run(self._h, ctypes.byref(s)))
         return _summary_dict(s, rows)
 

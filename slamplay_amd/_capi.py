@@ -37,7 +37,8 @@ class Options(ctypes.Structure):
                 ("launch_mode", i32), ("loss_scale", f64), ("function_tolerance", f64), ("gradient_tolerance", f64),
                 ("parameter_tolerance", f64), ("initial_trust_region_radius", f64), ("max_trust_region_radius", f64),
                 ("min_trust_region_radius", f64), ("min_relative_decrease", f64), ("min_lm_diagonal", f64),
-                ("max_lm_diagonal", f64), ("pcg_tolerance", f64)]
+                ("max_lm_diagonal", f64), ("pcg_tolerance", f64), ("mode", i32), ("g2o_max_trials", i32),
+                ("g2o_tau", f64)]
 
 
 class Iteration(ctypes.Structure):

@@ -99,6 +99,9 @@ void ba_default_options(ba_options* o) {
     o->min_lm_diagonal = 1e-6;
     o->max_lm_diagonal = 1e32;
     o->pcg_tolerance = 1e-6;
+    o->mode = BA_MODE_CERES;
+    o->g2o_max_trials = 10;
+    o->g2o_tau = 1e-5;
 }
 
 int ba_device_count(int32_t* count) {

@@ -72,6 +72,7 @@ struct LMState {
     int need_lin, first_lin, accepted, step_invalid;
     int num_successful, num_unsuccessful;
     int pcg_iters, pcg_done, total_pcg_iters, trace_len;
+    int g2o_iter, g2o_trials;  // g2o mode: outer iterations done, rejected trials of the current one
 };
 
 struct Opts {
@@ -85,6 +86,9 @@ struct Opts {
     double max_radius, min_radius, min_rel_decrease;
     double min_diag, max_diag;
     double pcg_tol;
+    int g2o;             // the g2o driver's LM (BA_MODE_G2O)
+    int g2o_max_trials;  // rejected trials per iteration before the solve stops
+    double g2o_tau;      // initial lambda = tau * max diag(J'J)
 };
 
 template <typename Real>
@@ -301,7 +305,7 @@ __device__ __forceinline__ void load_obs(const Dev<Real>& d, int j, ObsIn<Real>&
 template <typename Real>
 __device__ __forceinline__ void corrected_from(const Dev<Real>& d, const CameraPrep<Real>& P, const ObsIn<Real>& in,
                                                Real r[2], Real Jc[18], Real Jp[6]) {
-    snavely_residual_jacobian<Real>(P, in.X, in.u, in.v, r, Jc, Jp);
+    snavely_residual_jacobian<Real>(P, in.X, in.u, in.v, r, Jc, Jp, d.o.g2o != 0);
     const Real w = huber_sqrt_rho1<Real>(r[0] * r[0] + r[1] * r[1], d.o.loss, (Real)d.o.loss_a);
     r[0] *= w;
     r[1] *= w;
@@ -318,7 +322,7 @@ __device__ __forceinline__ void corrected_jacobian(const Dev<Real>& d, const Cam
     p = d.co_pt[j];
     double X[3];
     load_point<double>(d.pt_x, p, X);
-    snavely_residual_jacobian<Real>(P, X, d.co_uv[2 * (size_t)j], d.co_uv[2 * (size_t)j + 1], r, Jc, Jp);
+    snavely_residual_jacobian<Real>(P, X, d.co_uv[2 * (size_t)j], d.co_uv[2 * (size_t)j + 1], r, Jc, Jp, d.o.g2o != 0);
     const Real w = huber_sqrt_rho1<Real>(r[0] * r[0] + r[1] * r[1], d.o.loss, (Real)d.o.loss_a);
     r[0] *= w;
     r[1] *= w;
@@ -503,9 +507,21 @@ __global__ void __launch_bounds__(kWG) k_post_lin(Dev<Real> d) {
     for (int i = threadIdx.x; i < d.n_pt_blocks; i += kWG) gm = fmax(gm, d.pt_part[PS_GMAX * (size_t)d.n_pt_blocks + i]);
     gm = block_max1(gm, lds);
     cost = block_sum1(cost, lds);
+    double hmax = 0.0;
+    if (d.o.g2o && st->first_lin) {
+        // g2o's computeLambdaInit: tau times the largest diagonal entry of J'J (cameras and points)
+        __syncthreads();  // lds is reused
+        for (int i = threadIdx.x; i < 9 * d.nc; i += kWG) hmax = fmax(hmax, d.U[i]);
+        for (int p = threadIdx.x; p < d.np; p += kWG) {
+            const double* V = d.V + 6 * (size_t)p;
+            hmax = fmax(hmax, fmax(V[0], fmax(V[3], V[5])));
+        }
+        hmax = block_max1(hmax, lds);
+    }
     if (threadIdx.x != 0) return;
     st->gmax = gm;
     if (st->first_lin) {
+        if (d.o.g2o) st->radius = 1.0 / (d.o.g2o_tau * hmax);  // radius = 1 / lambda
         st->x_cost = cost;
         if (st->trace_len < d.trace_cap) {
             ba_iteration row{};
@@ -528,11 +544,11 @@ __global__ void __launch_bounds__(kWG) k_post_lin(Dev<Real> d) {
     } else if (st->trace_len > 0 && st->trace_len <= d.trace_cap) {
         d.trace[st->trace_len - 1].gradient_max_norm = gm;
     }
-    if (!st->done && gm <= d.o.gtol) {
+    if (!d.o.g2o && !st->done && gm <= d.o.gtol) {
         st->done = 1;
         st->termination = BA_CONVERGENCE;
     }
-    if (!st->done && st->radius <= d.o.min_radius) {
+    if (!d.o.g2o && !st->done && st->radius <= d.o.min_radius) {
         st->done = 1;
         st->termination = BA_CONVERGENCE;
     }
@@ -1041,10 +1057,19 @@ __global__ void __launch_bounds__(64) k_cam_step(Dev<Real> d) {
         const double x = d.cam_x[(size_t)i * d.nc + c];
         d.dcam[(size_t)i * d.nc + c] = dl;
         cam[i] = x + dl;
-        d.cam_c[(size_t)i * d.nc + c] = cam[i];
         dx2 += dl * dl;
         x2 += x * x;
     }
+    if (d.o.g2o) {
+        // rotation <- exp(delta) rotation, read back as angle-axis
+        double dd[3], w[3];
+        for (int i = 0; i < 3; ++i) {
+            dd[i] = d.dcam[(size_t)i * d.nc + c];
+            w[i] = d.cam_x[(size_t)i * d.nc + c];
+        }
+        so3_left_update(dd, w, cam);
+    }
+    for (int i = 0; i < 9; ++i) d.cam_c[(size_t)i * d.nc + c] = cam[i];
     CameraPrep<double> P;
     camera_prep<double>(cam, P);
     d.prep_c[c] = P;
@@ -1126,6 +1151,54 @@ __global__ void __launch_bounds__(kWG) k_decide(Dev<Real> d) {
     st->total_pcg_iters += st->pcg_iters;
     row.model_cost_change = model;
     st->accepted = 0;
+    if (o.g2o) {
+        // g2o OptimizationAlgorithmLevenberg: rho = (chi2 - chi2_new) / (x'(lambda x + b) + 1e-3) with
+        // chi2 = 2 cost and x'(lambda x + b) = 2 model for the solved damped system; accept when
+        // rho > 0, lambda *= max(1/3, min(2/3, 1 - (2 rho - 1)^3)), else lambda *= nu, nu *= 2; an
+        // iteration ends with an accepted trial, the solve after g2o_max_trials rejected ones
+        // (radius = 1 / lambda, decrease_factor = nu)
+        if (!isfinite(cand)) cand = DBL_MAX;
+        const double step_norm = sqrt(dx2);
+        row.step_norm = have_step ? step_norm : 0.0;
+        row.step_is_valid = have_step ? 1 : 0;
+        const double chi_change = have_step ? 2.0 * (st->x_cost - cand) : -DBL_MAX;
+        const double rho = have_step ? chi_change / (2.0 * model + 1e-3) : -1.0;
+        row.cost_change = have_step ? st->x_cost - cand : 0.0;
+        row.relative_decrease = rho;
+        st->step_invalid = 0;
+        if (have_step && rho > 0.0 && cand < DBL_MAX) {
+            const double t = 2.0 * rho - 1.0;
+            st->radius = st->radius / fmax(1.0 / 3.0, fmin(2.0 / 3.0, 1.0 - t * t * t));
+            st->decrease_factor = 2.0;
+            st->x_cost = cand;
+            st->accepted = 1;
+            st->need_lin = 1;
+            st->g2o_iter++;
+            st->g2o_trials = 0;
+            row.step_is_successful = 1;
+            row.cost = cand;
+            st->num_successful++;
+        } else {
+            st->radius = st->radius / st->decrease_factor;
+            st->decrease_factor *= 2.0;
+            st->g2o_trials++;
+            row.step_is_successful = 0;
+            row.cost = have_step ? cand : st->x_cost;
+            st->num_unsuccessful++;
+        }
+        row.gradient_max_norm = st->gmax;
+        row.trust_region_radius = st->radius;
+        if (st->trace_len < d.trace_cap) d.trace[st->trace_len] = row;
+        st->trace_len++;
+        if (st->g2o_iter >= o.max_iterations) {
+            st->done = 1;
+            st->termination = BA_NO_CONVERGENCE;
+        } else if (st->g2o_trials >= o.g2o_max_trials || rho == 0.0 || !isfinite(st->radius) || !(st->radius > 0.0)) {
+            st->done = 1;
+            st->termination = BA_CONVERGENCE;
+        }
+        return;
+    }
     const bool valid = have_step && model > 0.0 && isfinite(dx2);
     if (!valid) {
         st->step_invalid = 0;

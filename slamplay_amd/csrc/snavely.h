@@ -142,7 +142,7 @@ BA_HD void snavely_residual(const CameraPrep<Real>& P, const double X[3], Real o
 // Residual and Jacobians. Jc: 2x9 row-major (w0..2, t0..2, f, k1, k2); Jp: 2x3 row-major (X0..2).
 template <typename Real>
 BA_HD void snavely_residual_jacobian(const CameraPrep<Real>& P, const double X64[3], Real ox, Real oy, Real r[2],
-                                     Real Jc[18], Real Jp[6]) {
+                                     Real Jc[18], Real Jp[6], bool left = false) {
     const Real* u = P.u;
     const Real X[3] = {(Real)X64[0], (Real)X64[1], (Real)X64[2]};
     Real p[3];
@@ -179,6 +179,15 @@ BA_HD void snavely_residual_jacobian(const CameraPrep<Real>& P, const double X64
         dpX[2][0] = -u[1]; dpX[2][1] = u[0];  dpX[2][2] = 1;
     }
     camera_point<Real>(P, X64, p);
+    if (left) {
+        // g2o's camera update R <- exp(d) R (test_bundle_adjustment_g2o.cpp:85): the rotation
+        // columns are d p / d d = -[q]x with q = R X, i.e. column j = e_j x q
+        double q[3];
+        for (int i = 0; i < 3; ++i) q[i] = P.R64[3 * i] * X64[0] + P.R64[3 * i + 1] * X64[1] + P.R64[3 * i + 2] * X64[2];
+        dpw[0][0] = 0;           dpw[0][1] = (Real)q[2];  dpw[0][2] = (Real)-q[1];
+        dpw[1][0] = (Real)-q[2]; dpw[1][1] = 0;           dpw[1][2] = (Real)q[0];
+        dpw[2][0] = (Real)q[1];  dpw[2][1] = (Real)-q[0]; dpw[2][2] = 0;
+    }
 
     // Perspective divide with the dual-number quotient rule: x' = -p0 / p2.
     const Real ginv = (Real)1 / p[2];
@@ -215,6 +224,61 @@ BA_HD void snavely_residual_jacobian(const CameraPrep<Real>& P, const double X64
         for (int m = 0; m < 3; ++m)
             Jp[3 * row + m] = dpred[row][0] * dpX[0][m] + dpred[row][1] * dpX[1][m] + dpred[row][2] * dpX[2][m];
     }
+}
+
+// SO(3) through unit quaternions (w, x, y, z), for the g2o camera update R <- exp(d) R
+// (test_bundle_adjustment_g2o.cpp:85) and its angle-axis read-back rotation.log() (:57-63).
+// exp: q = (cos(t/2), sin(t/2) w / t), t = |w|, with the series of cos(t/2) and sin(t/2)/t below
+// t = 1e-10; log: w = 2 atan(n / q_w) v / n with n = |v|, the series 2 / q_w - 2 n^2 / (3 q_w^3)
+// below n = 1e-10 and +-pi / n when |q_w| < 1e-10 (the conventions of Sophus's SO3 exp/log).
+BA_HD void so3_exp_quat(const double w[3], double q[4]) {
+    const double t2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    const double t = sqrt(t2);
+    double re, im;
+    if (t < 1e-10) {
+        const double t4 = t2 * t2;
+        re = 1.0 - t2 / 8.0 + t4 / 384.0;
+        im = 0.5 - t2 / 48.0 + t4 / 3840.0;
+    } else {
+        re = cos(0.5 * t);
+        im = sin(0.5 * t) / t;
+    }
+    q[0] = re;
+    q[1] = im * w[0];
+    q[2] = im * w[1];
+    q[3] = im * w[2];
+}
+
+BA_HD void quat_mul(const double a[4], const double b[4], double c[4]) {
+    c[0] = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+    c[1] = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+    c[2] = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+    c[3] = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+}
+
+BA_HD void so3_log_quat(const double q[4], double w[3]) {
+    const double n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    const double n = sqrt(n2), qw = q[0];
+    double k;
+    if (n < 1e-10) {
+        k = 2.0 / qw - 2.0 * n2 / (3.0 * qw * qw * qw);
+    } else if (fabs(qw) < 1e-10) {
+        k = (qw >= 0.0 ? 3.14159265358979323846 : -3.14159265358979323846) / n;
+    } else {
+        k = 2.0 * atan(n / qw) / n;
+    }
+    w[0] = k * q[1];
+    w[1] = k * q[2];
+    w[2] = k * q[3];
+}
+
+// angle-axis of exp(d) exp(w)
+BA_HD void so3_left_update(const double d[3], const double w[3], double out[3]) {
+    double qd[4], qw[4], qc[4];
+    so3_exp_quat(d, qd);
+    so3_exp_quat(w, qw);
+    quat_mul(qd, qw, qc);
+    so3_log_quat(qc, out);
 }
 
 // Huber corrector weight sqrt(rho'(s)) and the cost term rho(s)/2 (ceres::HuberLoss(a) with the

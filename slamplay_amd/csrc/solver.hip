@@ -22,6 +22,9 @@ namespace {
 
 Opts make_opts(const ba_options& o) {
     Opts r{};
+    r.g2o = o.mode == BA_MODE_G2O;
+    r.g2o_max_trials = o.g2o_max_trials;
+    r.g2o_tau = o.g2o_tau;
     r.loss = o.loss;
     r.max_iterations = o.max_num_iterations;
     r.max_invalid = o.max_num_consecutive_invalid_steps;
@@ -37,6 +40,11 @@ Opts make_opts(const ba_options& o) {
     r.min_diag = o.min_lm_diagonal;
     r.max_diag = o.max_lm_diagonal;
     r.pcg_tol = o.pcg_tolerance;
+    if (r.g2o) {
+        // g2o damps H + lambda I without column scaling: D^2 = clamp(diag, 1, 1) / radius = lambda
+        r.jacobi_scaling = 0;
+        r.min_diag = r.max_diag = 1.0;
+    }
     return r;
 }
 
@@ -55,6 +63,9 @@ int check_options(const ba_options* o) {
         return fail(BA_ERR_INVALID_ARGUMENT, "invalid LM diagonal bounds");
     if (o->pcg_max_iterations < 0) return fail(BA_ERR_INVALID_ARGUMENT, "pcg_max_iterations < 0");
     if (o->launch_mode != 0 && o->launch_mode != 1) return fail(BA_ERR_INVALID_ARGUMENT, "unknown launch_mode %d", o->launch_mode);
+    if (o->mode != BA_MODE_CERES && o->mode != BA_MODE_G2O) return fail(BA_ERR_INVALID_ARGUMENT, "unknown mode %d", o->mode);
+    if (o->mode == BA_MODE_G2O && (o->g2o_max_trials < 1 || !(o->g2o_tau > 0.0)))
+        return fail(BA_ERR_INVALID_ARGUMENT, "g2o mode needs g2o_max_trials >= 1 and g2o_tau > 0");
     return BA_OK;
 }
 
@@ -496,13 +507,15 @@ struct Solver final : SolverBase {
         const int check_every = 4;
         const bool graphs = opt.launch_mode == 0 && !profiling;
         if (graphs && !iter_graph) BA_TRY(capture_iteration());
-        for (int it = 0; it < opt.max_num_iterations; ++it) {
+        // one launch sequence per trial; a g2o iteration takes up to g2o_max_trials of them
+        const int max_trials = opt.mode == BA_MODE_G2O ? opt.max_num_iterations * opt.g2o_max_trials : opt.max_num_iterations;
+        for (int it = 0; it < max_trials; ++it) {
             if (graphs)
                 BA_HIP(hipGraphLaunch(iter_graph, s));
             else
                 enqueue_iteration();
             BA_HIP(hipGetLastError());
-            if ((it + 1) % check_every == 0 || it + 1 == opt.max_num_iterations) {
+            if ((it + 1) % check_every == 0 || it + 1 == max_trials) {
                 BA_HIP(hipMemcpyAsync(st_host, st.p, sizeof(LMState), hipMemcpyDeviceToHost, s));
                 BA_HIP(hipStreamSynchronize(s));
                 if (st_host->done) break;
@@ -532,8 +545,15 @@ struct Solver final : SolverBase {
         const int n = std::min(S.trace_len, d.trace_cap);
         std::vector<ba_iteration> rows(n);
         if (n) BA_HIP(hipMemcpy(rows.data(), trace.p, n * sizeof(ba_iteration), hipMemcpyDeviceToHost));
-        sm->initial_cost = n ? rows[0].cost : S.x_cost;
-        sm->final_cost = S.x_cost;
+        if (opt.mode == BA_MODE_G2O)
+            for (ba_iteration& r : rows) {  // g2o reports chi2 = sum rho(s), without the 1/2
+                r.cost *= 2.0;
+                r.cost_change *= 2.0;
+                r.model_cost_change *= 2.0;
+            }
+        const double cscale = opt.mode == BA_MODE_G2O ? 2.0 : 1.0;
+        sm->initial_cost = n ? rows[0].cost : cscale * S.x_cost;
+        sm->final_cost = cscale * S.x_cost;
         sm->num_iterations = S.trace_len;
         sm->num_successful_steps = S.num_successful;
         sm->num_unsuccessful_steps = S.num_unsuccessful;

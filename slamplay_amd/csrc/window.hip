@@ -559,6 +559,7 @@ struct Batch final : BatchBase {
     int create(const ba_problem_view* views, int count, const ba_options& o) {
         opt = o;
         BA_TRY(validate_options(&o));
+        if (o.mode != BA_MODE_CERES) return fail(BA_ERR_INVALID_ARGUMENT, "batched windows run the Ceres mode only");
         BA_TRY(use_device(o.device));
         imgs.resize(count);
         large_index.assign(count, -1);
@@ -807,6 +808,7 @@ struct Server final : ServerBase {
         if (cfg.max_windows < 1) return fail(BA_ERR_INVALID_ARGUMENT, "max_windows < 1");
         if (cfg.arena_bytes < 1) return fail(BA_ERR_INVALID_ARGUMENT, "arena_bytes < 1");
         BA_TRY(validate_options(&cfg.options));
+        if (cfg.options.mode != BA_MODE_CERES) return fail(BA_ERR_INVALID_ARGUMENT, "the server runs the Ceres mode only");
         BA_TRY(use_device(cfg.options.device));
         if (cfg.iterations_per_step < 1) cfg.iterations_per_step = 1;
         BA_TRY(eng.init(cfg.options, cfg.arena_bytes, cfg.max_windows));

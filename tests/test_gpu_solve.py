@@ -153,3 +153,38 @@ def test_graph_replay_matches_stream_launches(solver):
         S.eval_cost(p)  # eager work on the device between replays
     eager.close()
     g.close()
+
+
+@pytest.mark.parametrize("solver", [1, 2])
+def test_g2o_mode_matches_oracle(oracle_mod, solver):
+    """The g2o driver's solve (test_bundle_adjustment_g2o.cpp:166-218): SO(3) left update, lambda-LM,
+    40 iterations of up to 10 trials; costs are g2o's chi2.  Same trial sequence as the oracle and
+    chi2 per trial within 1e-10 until the cost has converged (g2o accepts any rho > 0, so once the
+    cost changes are at the rounding level, accept or reject is decided by the last bits), final
+    chi2 within 1e-9, final cameras and points within 1e-7 of their scale."""
+    p = S.synthetic(1, seed=24)
+    pcg = dict(pcg_tolerance=1e-12, pcg_max_iterations=300)
+    ref = _oracle_solve(oracle_mod, p, max_num_iterations=40, linear_solver=solver, mode=1, **pcg)
+    q = _copy(p)
+    out = S.SolveBA(q, max_num_iterations=40, linear_solver=solver, precision=0, mode=1, **pcg)
+    c_gpu, s_gpu = _trace_costs(out["trace"])
+    c_ref, s_ref = _trace_costs(ref["trace"])
+    final = ref["final_cost"]
+    # rows up to the first one within 1e-9 of the final chi2
+    n = next(k for k in range(len(c_ref)) if abs(c_ref[k] - final) <= 1e-9 * final)
+    assert n >= 8 and len(c_gpu) > n
+    np.testing.assert_array_equal(s_gpu[: n + 1], s_ref[: n + 1])
+    rel = np.abs(c_gpu[: n + 1] - c_ref[: n + 1]) / c_ref[: n + 1]
+    assert rel.max() < 1e-10, rel
+    assert abs(out["final_cost"] - ref["final_cost"]) / ref["final_cost"] < 1e-9
+    assert out["initial_cost"] == out["trace"][0]["cost"]
+    assert abs(out["initial_cost"] - ref["initial_cost"]) / ref["initial_cost"] < 1e-12
+    np.testing.assert_allclose(q.mutable_cameras(), ref["cams"], rtol=0, atol=1e-7 * np.abs(ref["cams"]).max())
+    np.testing.assert_allclose(q.mutable_points(), ref["pts"], rtol=0, atol=1e-7 * np.abs(ref["pts"]).max())
+
+
+def test_g2o_mode_mixed_final_cost(oracle_mod):
+    p = S.synthetic(1, seed=25)
+    ref = _oracle_solve(oracle_mod, p, max_num_iterations=40, linear_solver=1, mode=1)
+    out = S.SolveBA(_copy(p), max_num_iterations=40, linear_solver=1, precision=1, mode=1)
+    assert abs(out["final_cost"] - ref["final_cost"]) / ref["final_cost"] < 1e-5

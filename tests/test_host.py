@@ -174,3 +174,13 @@ def test_no_device_is_an_error_not_a_fallback():
     with pytest.raises(S.BAError) as e:
         S.SolveBA(p)
     assert e.value.status == 5
+
+
+def test_g2o_mode_rejected_by_batch_and_options():
+    """The batched-window and server engines run the Ceres mode only, and g2o options are checked."""
+    wins = S.local_ba_windows(2, seed=3)
+    with pytest.raises(C.BAError, match="Ceres mode only"):
+        S.Batch(wins, S.default_options(mode=1))
+    for bad in (dict(mode=2), dict(mode=1, g2o_max_trials=0), dict(mode=1, g2o_tau=0.0)):
+        with pytest.raises(C.BAError):
+            S.Batch(wins, S.default_options(**bad))

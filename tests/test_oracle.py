@@ -233,3 +233,88 @@ def test_perturb_rotation_keeps_centers(oracle_mod, ladybug_path):
     cams2, pts2 = oracle_mod.perturb(nc, npt, cams0, p.mutable_points(), 0.0, 0.5, 0.0)
     d = cams2.reshape(nc, 9)[:, 3:6] - cams0[:, 3:6]
     assert 0.2 < d.std() < 0.8
+
+
+# ---------------------------------------------------------------------------------------------
+# g2o driver mode (test_bundle_adjustment_g2o.cpp): left-multiplied rotation update, lambda-LM.
+
+def _aa_matrix(oracle_mod, aa):
+    """Rotation matrix of angle-axis aa through the Ceres-path Rodrigues rotation (columns R e_k)."""
+    return np.stack([oracle_mod.rotate_point(aa, e) for e in np.eye(3)], axis=1)
+
+
+def test_so3_left_update_is_rotation_product(oracle_mod):
+    """exp(d) exp(w) read back by log equals the product of the two rotation matrices, for small,
+    moderate and beyond-pi angles; log(exp(w)) returns w below pi and wraps above it."""
+    rng = np.random.default_rng(21)
+    for scale in (1e-12, 1e-6, 0.3, 2.5):
+        for _ in range(20):
+            d, w = rng.normal(0, scale, 3), rng.normal(0, 1.0, 3)
+            out = oracle_mod.so3_left_update(d, w)
+            np.testing.assert_allclose(_aa_matrix(oracle_mod, out), _aa_matrix(oracle_mod, d) @ _aa_matrix(oracle_mod, w),
+                                       rtol=0, atol=1e-13)
+            assert np.linalg.norm(out) <= np.pi + 1e-12
+    w = np.array([0.4, -1.1, 0.7])
+    np.testing.assert_allclose(oracle_mod.so3_left_update(np.zeros(3), w), w, rtol=0, atol=1e-15)
+    big = 4.0 * w / np.linalg.norm(w)  # |w| = 4 > pi: the same rotation as angle 4 - 2 pi
+    np.testing.assert_allclose(oracle_mod.so3_left_update(np.zeros(3), big), (4.0 - 2 * np.pi) * w / np.linalg.norm(w),
+                               rtol=0, atol=1e-14)
+
+
+def test_left_jacobian_matches_central_differences(oracle_mod):
+    """The g2o-mode rotation columns are d r / d d of r(exp(d) R, t, f, k1, k2; X) at d = 0 (what g2o's
+    numeric linearizeOplus differentiates); the other columns equal the Ceres-path ones."""
+    cams, pts, ci, pi, rng = tiny_problem(seed=22, nc=3, npts=8)
+    h = 1e-6
+    for c in range(3):
+        for p in range(8):
+            cam, pt = cams[c], pts[p]
+            r, jc, jp = oracle_mod.jacobian_left(cam, pt, 1.0, -2.0)
+            r0, jc0, jp0 = oracle_mod.eval_residual_jacobian([0], [0], [1.0, -2.0], cam, pt)
+            np.testing.assert_array_equal(jc[:, 3:], jc0[0][:, 3:])
+            np.testing.assert_array_equal(jp, jp0[0])
+            for j in range(3):
+                e = np.zeros(3)
+                e[j] = h
+                cp, cm = cam.copy(), cam.copy()
+                cp[:3] = oracle_mod.so3_left_update(e, cam[:3])
+                cm[:3] = oracle_mod.so3_left_update(-e, cam[:3])
+                num = (oracle_mod.residual(cp, pt, 1.0, -2.0) - oracle_mod.residual(cm, pt, 1.0, -2.0)) / (2 * h)
+                np.testing.assert_allclose(jc[:, j], num, rtol=0, atol=1e-6 * (1 + np.abs(num).max()))
+
+
+def test_g2o_mode_lambda_rule_and_convergence(oracle_mod):
+    """lambda_0 = tau max diag(J'J); noise-free data reach chi2 ~ 0; accepted chi2 never rises and
+    every rejection multiplies lambda by nu = 2, 4, 8, ... until an acceptance resets it."""
+    cams, pts, ci, pi, rng = tiny_problem(seed=23, nc=4, npts=30)
+    obs = _project(oracle_mod, cams, pts, ci, pi)
+    cams_p = cams + rng.normal(0, 0.005, cams.shape)
+    pts_p = pts + rng.normal(0, 0.02, pts.shape)
+    o = oracle_mod.default_options(max_num_iterations=40, mode=1, loss=0)
+    res = oracle_mod.solve(4, 30, ci, pi, obs, cams_p, pts_p, o)
+    tr = res["trace"]
+    _, jc, jp = oracle_mod.eval_residual_jacobian(ci, pi, obs, cams_p.reshape(-1), pts_p.reshape(-1))
+    hdiag = []
+    for c in range(4):
+        rows = [k for k in range(len(ci)) if ci[k] == c]
+        jl = np.concatenate([oracle_mod.jacobian_left(cams_p[c], pts_p[pi[k]], *obs[k])[1] for k in rows])
+        hdiag.append((jl ** 2).sum(0))
+    for p in range(30):
+        hdiag.append((jp[pi == p] ** 2).sum((0, 1)))
+    lam0 = 1e-5 * max(np.concatenate(hdiag))
+    assert tr[0]["trust_region_radius"] == pytest.approx(1.0 / lam0, rel=1e-12)
+    assert res["final_cost"] < 1e-18 * len(ci)
+    assert res["initial_cost"] == pytest.approx(2 * oracle_mod.cost(ci, pi, obs, cams_p, pts_p, loss=0), rel=1e-14)
+    accepted = [r["cost"] for r in tr if r["step_is_successful"]]
+    assert all(b <= a + 1e-22 for a, b in zip(accepted, accepted[1:]))
+    nu = 2.0
+    for prev, row in zip(tr, tr[1:]):
+        if row["step_is_successful"]:
+            rho = row["relative_decrease"]
+            assert rho > 0
+            factor = max(1 / 3, min(2 / 3, 1 - (2 * rho - 1) ** 3))
+            assert row["trust_region_radius"] == pytest.approx(prev["trust_region_radius"] / factor, rel=1e-12)
+            nu = 2.0
+        else:
+            assert row["trust_region_radius"] == pytest.approx(prev["trust_region_radius"] / nu, rel=1e-12)
+            nu *= 2.0

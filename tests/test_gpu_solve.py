@@ -207,3 +207,32 @@ def test_g2o_mode_mixed_final_cost(oracle_mod):
     ref = _oracle_solve(oracle_mod, p, max_num_iterations=40, linear_solver=1, mode=1)
     out = S.SolveBA(_copy(p), max_num_iterations=40, linear_solver=1, precision=1, mode=1)
     assert abs(out["final_cost"] - ref["final_cost"]) / ref["final_cost"] < 1e-5
+
+
+def test_empty_and_ragged_problems(oracle_mod):
+    """No observations: the solve returns at once with zero cost.  Ragged tracks -- points seen by a
+    single camera (their 3x3 block is rank 2 until damped) mixed with long tracks, and a camera
+    that sees nothing -- follow the oracle on the fixed schedule in FP64 and stay within 1e-5 in
+    mixed precision, on both linear solvers."""
+    empty = S.BALProblem.from_arrays(np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(9), np.zeros(3))
+    out = S.SolveBA(empty)
+    assert out["initial_cost"] == 0.0 and out["final_cost"] == 0.0
+    p = S.synthetic(1, seed=26)
+    ci, pi, obs = p.camera_index(), p.point_index(), p.observations().reshape(-1, 2)
+    nc, npt = p.num_cameras(), p.num_points()
+    # keep one observation of every third point, drop every observation of the last camera
+    first = np.zeros(npt, dtype=bool)
+    keep = np.ones(len(ci), dtype=bool)
+    for k in range(len(ci)):
+        if pi[k] % 3 == 0:
+            keep[k] = not first[pi[k]]
+            first[pi[k]] = True
+    keep &= ci != nc - 1
+    q = S.BALProblem.from_arrays(ci[keep], pi[keep], obs[keep], p.mutable_cameras(), p.mutable_points())
+    for solver in (1, 2):
+        pcg = dict(pcg_tolerance=1e-12, pcg_max_iterations=300)
+        ref = _oracle_solve(oracle_mod, q, max_num_iterations=10, linear_solver=solver, **FIXED, **pcg)
+        out = S.SolveBA(_copy(q), max_num_iterations=10, linear_solver=solver, precision=0, **FIXED, **pcg)
+        assert abs(out["final_cost"] - ref["final_cost"]) / ref["final_cost"] < 1e-9
+        mixed = S.SolveBA(_copy(q), max_num_iterations=10, linear_solver=solver, precision=1, **FIXED, **pcg)
+        assert abs(mixed["final_cost"] - ref["final_cost"]) / ref["final_cost"] < 1e-5

@@ -374,6 +374,17 @@ struct DenseSchur {
         nt = std::max(1, (n + kTile - 1) / kTile);
         ld = nt * kTile;
         refine = sizeof(Real) == sizeof(double) ? 0 : std::max(0, o.refine_steps);
+        // the triangular solve keeps the ld-vector in LDS (160 KiB per workgroup, less the kernel's
+        // static arrays)
+        const size_t shm = (size_t)ld * sizeof(double);
+        constexpr size_t kSolveLdsMax = (160 - 4) * 1024;
+        if (shm > kSolveLdsMax)
+            return fail(BA_ERR_CAPACITY, "dense Schur with %d cameras exceeds the triangular solve's LDS (at most %d "
+                        "cameras); use BA_SOLVER_ITERATIVE_SCHUR", nc, (int)(kSolveLdsMax / sizeof(double) / 9));
+        if (sizeof(Real) == sizeof(double))
+            BA_HIP(hipFuncSetAttribute((const void*)k_chol_solve<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        else
+            BA_HIP(hipFuncSetAttribute((const void*)k_chol_solve<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         PairBlocks pb;
         build_pair_blocks(L, &pb);
         const std::vector<int>&ba = pb.a, &bb = pb.b, &boff = pb.off, &pi = pb.i, &pj = pb.j;

@@ -236,3 +236,14 @@ def test_empty_and_ragged_problems(oracle_mod):
         assert abs(out["final_cost"] - ref["final_cost"]) / ref["final_cost"] < 1e-9
         mixed = S.SolveBA(_copy(q), max_num_iterations=10, linear_solver=solver, precision=1, **FIXED, **pcg)
         assert abs(mixed["final_cost"] - ref["final_cost"]) / ref["final_cost"] < 1e-5
+
+
+def test_dense_schur_capacity_error():
+    """The dense triangular solve keeps the reduced-system vector in LDS: beyond 2,218 cameras a forced
+    dense solve is refused with a capacity error (the automatic choice goes iterative above 300)."""
+    from slamplay_amd import _capi as C
+    p = S.synthetic(2, seed=27, num_cameras=2300, num_points=3000)
+    with pytest.raises(C.BAError, match="LDS"):
+        S.SolveBA(_copy(p), linear_solver=1, max_num_iterations=1)
+    out = S.SolveBA(_copy(p), max_num_iterations=1)
+    assert out["final_cost"] <= out["initial_cost"]

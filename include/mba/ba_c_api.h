@@ -163,6 +163,34 @@ int ba_eval_cost(const ba_problem_view* problem, int32_t loss, double loss_scale
  * Replaces SolveBA(BALProblem&) of test_bundle_adjustment_ceres.cpp:58-101. summary may be NULL. */
 int ba_solve(const ba_problem_view* problem, const ba_options* options, ba_summary* summary);
 
+/* ---------------------------------------------------------------------------------------------
+ * Stereo local bundle adjustment of full_slam (full_slam/src/backend.cpp:67-199, the g2o types of
+ * full_slam/include/myslam/g2o_types.h:48-175): SE3 keyframe poses observe world points through
+ * the left or right camera of a calibrated rig; e = measurement - pi(K E_side T X); pose update
+ * T <- exp(xi) T with the edge's own Jacobian; g2o Levenberg (ba_stereo_default_options: mode
+ * BA_MODE_G2O, 10 iterations, Huber 5.991, dense Schur); afterwards the reference's outlier rule:
+ * chi2 = |e|^2 > threshold, the threshold starting at loss_scale and doubled (at most 5 times)
+ * while the inlier ratio is <= 0.5. */
+typedef struct ba_stereo_view {
+    int32_t num_poses;
+    int32_t num_points;
+    int32_t num_observations;
+    const int32_t* pose_index;
+    const int32_t* point_index;
+    const int32_t* side;         /* 0: left camera, 1: right camera (backend.cpp:117-121) */
+    const double* observations;  /* u, v pixels per observation */
+    double* poses;               /* 6 per pose: angle-axis and translation of T (world to body) */
+    double* points;              /* 3 per point, world frame */
+    double K[4];                 /* fx, fy, cx, cy (backend.cpp:97) */
+    double extrinsics[2][6];     /* left, right camera: angle-axis and translation of E (backend.cpp:98-99) */
+} ba_stereo_view;
+
+void ba_stereo_default_options(ba_options* options);
+/* Solves in place; outlier (optional, num_observations bytes) receives 1 for outliers, chi2_threshold
+ * (optional) the final classification threshold. */
+int ba_stereo_solve(const ba_stereo_view* problem, const ba_options* options, ba_summary* summary, uint8_t* outlier,
+                    double* chi2_threshold);
+
 /* Device-resident solver: create uploads the problem and builds the device layout once; run solves
  * from the parameters currently held on the device; reset restores the parameters given at create;
  * download copies the device parameters to host AoS arrays. */

@@ -287,12 +287,92 @@ void huber(double s, double a, double rho[3]) {
     }
 }
 
+// full_slam's stereo model (full_slam/include/myslam/g2o_types.h:122-175, backend.cpp:67-199): a
+// pose T (angle-axis w, translation t in the 9-parameter slots, the last three unused) observes X
+// through camera side s of the rig: p = E_s (R X + t), e = obs - (fx p_x/p_z + cx, fy p_y/p_z + cy).
+struct StereoModel {
+    double K[4];
+    double R[2][9], t[2][3];
+    const int* side;
+};
+
 struct Problem {
     int nc, np, no;
     const int* ci;
     const int* pi;
     const double* obs;
+    const StereoModel* stereo = nullptr;
 };
+
+void stereo_point(const StereoModel& g, int s, const double* cam, const double* pt, double p[3]) {
+    double q[3];
+    rotate_point(cam, pt, q);
+    for (int i = 0; i < 3; ++i) q[i] += cam[3 + i];
+    for (int i = 0; i < 3; ++i) p[i] = g.R[s][3 * i] * q[0] + g.R[s][3 * i + 1] * q[1] + g.R[s][3 * i + 2] * q[2] + g.t[s][i];
+}
+
+void stereo_residual(const StereoModel& g, int s, const double* cam, const double* pt, double ox, double oy, double r[2]) {
+    double p[3];
+    stereo_point(g, s, cam, pt, p);
+    r[0] = ox - (g.K[0] * p[0] / p[2] + g.K[2]);
+    r[1] = oy - (g.K[1] * p[1] / p[2] + g.K[3]);
+}
+
+// The edge's Jacobian as written (g2o_types.h:147-172): pose columns (rho, phi) from p = E_s T X,
+// Z^-1 taken as 1 / (Z + 1e-18); point columns J_rho E_s.R R.
+void stereo_residual_jacobian(const StereoModel& g, int s, const double* cam, const double* pt, double ox, double oy,
+                              double r[2], double jc[18], double jp[6]) {
+    double p[3];
+    stereo_point(g, s, cam, pt, p);
+    r[0] = ox - (g.K[0] * p[0] / p[2] + g.K[2]);
+    r[1] = oy - (g.K[1] * p[1] / p[2] + g.K[3]);
+    const double fx = g.K[0], fy = g.K[1], X = p[0], Y = p[1], Z = p[2];
+    const double zi = 1.0 / (Z + 1e-18), zi2 = zi * zi;
+    const double row0[6] = {-fx * zi, 0.0, fx * X * zi2, fx * X * Y * zi2, -fx - fx * X * X * zi2, fx * Y * zi};
+    const double row1[6] = {0.0, -fy * zi, fy * Y * zi2, fy + fy * Y * Y * zi2, -fy * X * Y * zi2, -fy * X * zi};
+    for (int i = 0; i < 9; ++i) {
+        jc[i] = i < 6 ? row0[i] : 0.0;
+        jc[9 + i] = i < 6 ? row1[i] : 0.0;
+    }
+    double Rt[9];  // R of the pose: columns R e_k
+    for (int k = 0; k < 3; ++k) {
+        double e[3] = {0.0, 0.0, 0.0}, c[3];
+        e[k] = 1.0;
+        rotate_point(cam, e, c);
+        for (int i = 0; i < 3; ++i) Rt[3 * i + k] = c[i];
+    }
+    for (int row = 0; row < 2; ++row)
+        for (int k = 0; k < 3; ++k) {
+            double v = 0.0;
+            for (int i = 0; i < 3; ++i) {
+                double er = 0.0;  // (E_s.R R)[i][k]
+                for (int m = 0; m < 3; ++m) er += g.R[s][3 * i + m] * Rt[3 * m + k];
+                v += jc[9 * row + i] * er;
+            }
+            jp[3 * row + k] = v;
+        }
+}
+
+// T <- exp(xi) T, xi = (rho, phi): R' = exp(phi) R, t' = exp(phi) t + V(phi) rho with the SO(3) left
+// Jacobian V = I + (1 - cos a)/a^2 [phi]x + (a - sin a)/a^3 [phi]x^2 (Sophus SE3::exp, which
+// VertexPose::oplusImpl applies, full_slam/include/myslam/g2o_types.h:55-60).
+void se3_left_update(const double* xi, const double* w, const double* t, double* w_out, double* t_out) {
+    const double* rho = xi;
+    const double* phi = xi + 3;
+    so3_left_update(phi, w, w_out);
+    double Rt[3];
+    rotate_point(phi, t, Rt);
+    const double a2 = phi[0] * phi[0] + phi[1] * phi[1] + phi[2] * phi[2], a = std::sqrt(a2);
+    // (1 - cos a)/a^2 as 2 sin^2(a/2)/a^2 and (a - sin a)/a^3 by its Taylor series below 0.01: the
+    // direct quotients lose up to half the digits to cancellation at small angles
+    const double sh = std::sin(0.5 * a);
+    const double k1 = a < 1e-150 ? 0.5 : 2.0 * sh * sh / a2;
+    const double k2 = a < 1e-2 ? 1.0 / 6.0 - a2 / 120.0 + a2 * a2 / 5040.0 : (a - std::sin(a)) / (a2 * a);
+    double pr[3], ppr[3];
+    cross3(phi, rho, pr);
+    cross3(phi, pr, ppr);
+    for (int i = 0; i < 3; ++i) t_out[i] = Rt[i] + (rho[i] + k1 * pr[i] + k2 * ppr[i]);
+}
 
 // Per-observation linearization (corrected, unscaled) plus problem-level gradient and cost.
 struct Linearization {
@@ -318,7 +398,10 @@ bool evaluate_cost(const Problem& P, const Options& o, const double* cams, const
     double sum = 0.0;
     for (int k = 0; k < P.no; ++k) {
         double r[2];
-        snavely_residual(cams + 9 * P.ci[k], pts + 3 * P.pi[k], P.obs[2 * k], P.obs[2 * k + 1], r);
+        if (P.stereo)
+            stereo_residual(*P.stereo, P.stereo->side[k], cams + 9 * P.ci[k], pts + 3 * P.pi[k], P.obs[2 * k], P.obs[2 * k + 1], r);
+        else
+            snavely_residual(cams + 9 * P.ci[k], pts + 3 * P.pi[k], P.obs[2 * k], P.obs[2 * k + 1], r);
         sum += obs_cost(o, r[0], r[1], nullptr);
     }
     *cost = sum;
@@ -333,8 +416,13 @@ bool linearize(const Problem& P, const Options& o, const double* cams, const dou
     double sum = 0.0;
     for (int k = 0; k < P.no; ++k) {
         double r[2], jc[18], jp[6], w;
-        residual_and_jacobian(cams + 9 * P.ci[k], pts + 3 * P.pi[k], P.obs[2 * k], P.obs[2 * k + 1], r, jc, jp);
-        if (o.mode == 1) left_rotation_columns(cams + 9 * P.ci[k], pts + 3 * P.pi[k], jc);
+        if (P.stereo) {
+            stereo_residual_jacobian(*P.stereo, P.stereo->side[k], cams + 9 * P.ci[k], pts + 3 * P.pi[k], P.obs[2 * k],
+                                     P.obs[2 * k + 1], r, jc, jp);
+        } else {
+            residual_and_jacobian(cams + 9 * P.ci[k], pts + 3 * P.pi[k], P.obs[2 * k], P.obs[2 * k + 1], r, jc, jp);
+            if (o.mode == 1) left_rotation_columns(cams + 9 * P.ci[k], pts + 3 * P.pi[k], jc);
+        }
         sum += obs_cost(o, r[0], r[1], &w);
         for (int i = 0; i < 18; ++i) L->jc[18 * (size_t)k + i] = w * jc[i];
         for (int i = 0; i < 6; ++i) L->jp[6 * (size_t)k + i] = w * jp[i];
@@ -759,6 +847,39 @@ void oracle_jacobian_left(const double* cam, const double* pt, double ox, double
 // angle-axis of exp(d) exp(w)
 void oracle_so3_left_update(const double* d, const double* w, double* out) { so3_left_update(d, w, out); }
 
+void oracle_se3_left_update(const double* xi, const double* w, const double* t, double* w_out, double* t_out) {
+    se3_left_update(xi, w, t, w_out, t_out);
+}
+
+// rig from K and the two extrinsics (angle-axis, translation)
+static StereoModel make_rig(const double* K, const double* ext, const int* side) {
+    StereoModel g{};
+    for (int i = 0; i < 4; ++i) g.K[i] = K[i];
+    for (int s = 0; s < 2; ++s) {
+        for (int k = 0; k < 3; ++k) {
+            double e[3] = {0.0, 0.0, 0.0}, c[3];
+            e[k] = 1.0;
+            rotate_point(ext + 6 * s, e, c);
+            for (int i = 0; i < 3; ++i) g.R[s][3 * i + k] = c[i];
+        }
+        for (int i = 0; i < 3; ++i) g.t[s][i] = ext[6 * s + 3 + i];
+    }
+    g.side = side;
+    return g;
+}
+
+void oracle_stereo_residual_jacobian(const double* cam, const double* pt, const double* K, const double* ext, int side,
+                                     double ox, double oy, double* r, double* jc, double* jp) {
+    const StereoModel g = make_rig(K, ext, nullptr);
+    stereo_residual_jacobian(g, side, cam, pt, ox, oy, r, jc, jp);
+}
+
+void oracle_stereo_residual(const double* cam, const double* pt, const double* K, const double* ext, int side, double ox,
+                            double oy, double* r) {
+    const StereoModel g = make_rig(K, ext, nullptr);
+    stereo_residual(g, side, cam, pt, ox, oy, r);
+}
+
 void oracle_huber(double s, double a, double* rho) { huber(s, a, rho); }
 
 // Total cost 1/2 sum rho(|r|^2) in observation order.
@@ -864,7 +985,14 @@ static int solve_g2o(const Problem& P, const Options& o0, double* cams, double* 
                 for (size_t i = 0; i < nparam; ++i) delta[i] = -y[i];
                 mcc = model_cost_change(P, L, delta);
                 for (size_t i = 0; i < nparam; ++i) cand[i] = x[i] + delta[i];
-                for (int c = 0; c < P.nc; ++c) so3_left_update(&delta[9 * (size_t)c], &x[9 * (size_t)c], &cand[9 * (size_t)c]);
+                for (int c = 0; c < P.nc; ++c) {
+                    double* cc = &cand[9 * (size_t)c];
+                    const double* xc = &x[9 * (size_t)c];
+                    if (P.stereo)
+                        se3_left_update(&delta[9 * (size_t)c], xc, xc + 3, cc, cc + 3);
+                    else
+                        so3_left_update(&delta[9 * (size_t)c], xc, cc);
+                }
                 if (!evaluate_cost(P, o, cand.data(), cand.data() + ncam, &cand_cost))
                     cand_cost = std::numeric_limits<double>::max();
                 rho = 2.0 * (x_cost - cand_cost) / (2.0 * mcc + 1e-3);
@@ -909,6 +1037,15 @@ static int solve_g2o(const Problem& P, const Options& o0, double* cams, double* 
     *termination = term;
     if (n_trace) *n_trace = nt;
     return term;
+}
+
+// The stereo local BA (g2o mode): cams holds the poses in 9-parameter slots.
+int oracle_solve_stereo(int nc, int np, int no, const int* ci, const int* pi, const int* side, const double* obs,
+                        double* cams, double* pts, const double* K, const double* ext, const Options* o, IterRecord* trace,
+                        int trace_cap, int* n_trace, double* initial_cost, double* final_cost, int* termination) {
+    const StereoModel g = make_rig(K, ext, side);
+    Problem P{nc, np, no, ci, pi, obs, &g};
+    return solve_g2o(P, *o, cams, pts, trace, trace_cap, n_trace, initial_cost, final_cost, termination);
 }
 
 int oracle_solve(int nc, int np, int no, const int* ci, const int* pi, const double* obs, double* cams, double* pts,

@@ -100,6 +100,15 @@ def lib():
         L.oracle_rotate_point.argtypes = [_f64p, _f64p, _f64p]
         L.oracle_jacobian_left.argtypes = [_f64p, _f64p, ctypes.c_double, ctypes.c_double, _f64p, _f64p, _f64p]
         L.oracle_so3_left_update.argtypes = [_f64p, _f64p, _f64p]
+        L.oracle_se3_left_update.argtypes = [_f64p, _f64p, _f64p, _f64p, _f64p]
+        L.oracle_stereo_residual_jacobian.argtypes = [_f64p, _f64p, _f64p, _f64p, ctypes.c_int, ctypes.c_double,
+                                                      ctypes.c_double, _f64p, _f64p, _f64p]
+        L.oracle_stereo_residual.argtypes = [_f64p, _f64p, _f64p, _f64p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                             _f64p]
+        L.oracle_solve_stereo.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i32p, _i32p, _f64p, _f64p,
+                                          _f64p, _f64p, _f64p, ctypes.POINTER(Options), ctypes.POINTER(IterRecord),
+                                          ctypes.c_int, ctypes.POINTER(ctypes.c_int), _f64p, _f64p,
+                                          ctypes.POINTER(ctypes.c_int)]
         L.oracle_huber.argtypes = [ctypes.c_double, ctypes.c_double, _f64p]
         L.oracle_cost.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, ctypes.c_int, ctypes.c_double]
         L.oracle_cost.restype = ctypes.c_double
@@ -165,6 +174,58 @@ def so3_left_update(d, w):
     out = np.zeros(3)
     lib().oracle_so3_left_update(_p(d, _f64p), _p(w, _f64p), _p(out, _f64p))
     return out
+
+
+def se3_left_update(xi, w, t):
+    """(angle-axis, translation) of exp(xi) T for T = (w, t), xi = (rho, phi)."""
+    xi, w, t = _c(xi, np.float64), _c(w, np.float64), _c(t, np.float64)
+    wo, to = np.zeros(3), np.zeros(3)
+    lib().oracle_se3_left_update(_p(xi, _f64p), _p(w, _f64p), _p(t, _f64p), _p(wo, _f64p), _p(to, _f64p))
+    return wo, to
+
+
+def stereo_residual_jacobian(pose6, pt, K, ext, side, ou, ov):
+    """full_slam edge: residual (2,), pose Jacobian (2,6), point Jacobian (2,3)."""
+    cam = np.zeros(9)
+    cam[:6] = pose6
+    pt, K, ext = _c(pt, np.float64), _c(K, np.float64), _c(ext, np.float64)
+    r, jc, jp = np.zeros(2), np.zeros((2, 9)), np.zeros((2, 3))
+    lib().oracle_stereo_residual_jacobian(_p(cam, _f64p), _p(pt, _f64p), _p(K, _f64p), _p(ext, _f64p), int(side),
+                                          float(ou), float(ov), _p(r, _f64p), _p(jc, _f64p), _p(jp, _f64p))
+    return r, jc[:, :6], jp
+
+
+def stereo_residual(pose6, pt, K, ext, side, ou, ov):
+    cam = np.zeros(9)
+    cam[:6] = pose6
+    pt, K, ext = _c(pt, np.float64), _c(K, np.float64), _c(ext, np.float64)
+    r = np.zeros(2)
+    lib().oracle_stereo_residual(_p(cam, _f64p), _p(pt, _f64p), _p(K, _f64p), _p(ext, _f64p), int(side), float(ou),
+                                 float(ov), _p(r, _f64p))
+    return r
+
+
+def solve_stereo(pose_index, point_index, side, obs, poses, pts, K, ext, options, trace_cap=512):
+    """g2o-mode LM on the full_slam stereo model; poses (n, 6).  Returns dict like solve()."""
+    ci, pi, sd = _c(pose_index, np.int32), _c(point_index, np.int32), _c(side, np.int32)
+    obs = _c(obs, np.float64).reshape(-1)
+    P = np.asarray(poses, dtype=np.float64).reshape(-1, 6)
+    cams = np.zeros((P.shape[0], 9))
+    cams[:, :6] = P
+    cams = cams.reshape(-1)
+    pts = np.array(pts, dtype=np.float64, copy=True).reshape(-1)
+    K, ext = _c(K, np.float64), _c(ext, np.float64).reshape(-1)
+    trace = (IterRecord * trace_cap)()
+    nt, term = ctypes.c_int(0), ctypes.c_int(0)
+    ic, fc = np.zeros(1), np.zeros(1)
+    lib().oracle_solve_stereo(P.shape[0], pts.size // 3, ci.size, _p(ci, _i32p), _p(pi, _i32p), _p(sd, _i32p),
+                              _p(obs, _f64p), _p(cams, _f64p), _p(pts, _f64p), _p(K, _f64p), _p(ext, _f64p),
+                              ctypes.byref(options), trace, trace_cap, ctypes.byref(nt), _p(ic, _f64p), _p(fc, _f64p),
+                              ctypes.byref(term))
+    recs = trace[: min(nt.value, trace_cap)]
+    tr = [{f: getattr(r, f) for f, _ in IterRecord._fields_} for r in recs]
+    return dict(poses=cams.reshape(-1, 9)[:, :6].copy(), pts=pts.reshape(-1, 3), initial_cost=ic[0], final_cost=fc[0],
+                termination=term.value, trace=tr)
 
 
 def huber(s, a=1.0):

@@ -422,3 +422,126 @@ def simulate(pool, rate_hz, count, seed=1, config=None, **kw):
     res = (C.ServerResult * max(1, count))()
     C.check(C.lib().ba_server_simulate(ctypes.byref(cfg), views, len(pool), rate_hz, count, seed, res))
     return [_result_dict(res[i], False) for i in range(count)]
+
+
+# ---------------------------------------------------------------------------------------------
+# full_slam's stereo local bundle adjustment (full_slam/src/backend.cpp:67-199)
+
+def stereo_default_options(**kw):
+    """ba_stereo_default_options: g2o Levenberg, 10 iterations, Huber 5.991, dense Schur."""
+    o = C.Options()
+    C.lib().ba_stereo_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(f"unknown option {k}")
+        setattr(o, k, v)
+    return o
+
+
+def stereo_solve(pose_index, point_index, side, observations, poses, points, K, extrinsics, options=None, **kw):
+    """Optimizes the keyframe poses (angle-axis + translation of T, world to body; (n, 6)) and world
+    points ((m, 3)) of one local window on the GPU, then applies the reference's outlier rule.
+    Returns dict(poses, points, outlier, chi2_threshold, initial_cost, final_cost, trace, ...);
+    the inputs are not modified."""
+    o = options if options is not None else stereo_default_options(**kw)
+    pi_ = np.ascontiguousarray(pose_index, dtype=np.int32)
+    qi = np.ascontiguousarray(point_index, dtype=np.int32)
+    sd = np.ascontiguousarray(side, dtype=np.int32)
+    obs = np.ascontiguousarray(observations, dtype=np.float64).reshape(-1)
+    P = np.array(poses, dtype=np.float64, copy=True).reshape(-1, 6)
+    X = np.array(points, dtype=np.float64, copy=True).reshape(-1, 3)
+    v = C.StereoView()
+    v.num_poses, v.num_points, v.num_observations = P.shape[0], X.shape[0], pi_.size
+    v.pose_index = pi_.ctypes.data_as(C.i32p)
+    v.point_index = qi.ctypes.data_as(C.i32p)
+    v.side = sd.ctypes.data_as(C.i32p)
+    v.observations = obs.ctypes.data_as(C.f64p)
+    v.poses = P.ctypes.data_as(C.f64p)
+    v.points = X.ctypes.data_as(C.f64p)
+    for i in range(4):
+        v.K[i] = float(K[i])
+    ext = np.asarray(extrinsics, dtype=np.float64).reshape(2, 6)
+    for s in range(2):
+        for i in range(6):
+            v.extrinsics[s][i] = ext[s, i]
+    s, rows = _summary(_trace_rows(o))
+    outl = np.zeros(max(pi_.size, 1), dtype=np.uint8)
+    th = ctypes.c_double(0.0)
+    C.check(C.lib().ba_stereo_solve(ctypes.byref(v), ctypes.byref(o), ctypes.byref(s),
+                                    outl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(th)))
+    out = _summary_dict(s, rows)
+    out.update(poses=P, points=X, outlier=outl[: pi_.size].astype(bool), chi2_threshold=th.value)
+    return out
+
+
+def _aa_to_R(w):
+    w = np.asarray(w, dtype=np.float64)
+    t = np.linalg.norm(w)
+    if t < 1e-12:
+        return np.eye(3)
+    k = w / t
+    Kx = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + np.sin(t) * Kx + (1 - np.cos(t)) * Kx @ Kx
+
+
+def _R_to_aa(R):
+    c = np.clip((np.trace(R) - 1) / 2, -1.0, 1.0)
+    t = np.arccos(c)
+    if t < 1e-12:
+        return np.zeros(3)
+    v = np.array([R[2, 1] - R[1, 2], R[0, 2] - R[2, 0], R[1, 0] - R[0, 1]]) / (2 * np.sin(t))
+    return v * t
+
+
+def stereo_window(seed=1, num_keyframes=8, num_points=400, baseline=0.5, noise_px=1.0, outlier_fraction=0.02,
+                  pose_sigma=(0.01, 0.05), point_sigma=0.1):
+    """A synthetic stereo local-BA window of the kind full_slam's backend optimizes: keyframes moving
+    forward along z (EuRoC-like K, a horizontal baseline), world points in front of them, left and
+    right pixel observations with Gaussian noise and a fraction of gross outliers; poses and points
+    perturbed.  Returns dict(pose_index, point_index, side, observations, poses, points, K, extrinsics,
+    true_poses, true_points, planted_outlier)."""
+    rng = np.random.default_rng(seed)
+    K = np.array([458.0, 457.0, 367.0, 248.0])
+    ext = np.zeros((2, 6))
+    ext[1, 3] = -baseline
+    poses, Rs, ts = [], [], []
+    for k in range(num_keyframes):
+        c = np.array([0.1 * np.sin(0.7 * k), 0.05 * k, 0.3 * k])
+        w = rng.normal(0, 0.02, 3)
+        R = _aa_to_R(w)
+        t = -R @ c
+        poses.append(np.concatenate([w, t]))
+        Rs.append(R)
+        ts.append(t)
+    X = np.stack([rng.uniform(-6, 6, num_points), rng.uniform(-4, 4, num_points), rng.uniform(4, 20, num_points)], 1)
+    pose_index, point_index, side, obs, bad = [], [], [], [], []
+    for p in range(num_points):
+        for k in range(num_keyframes):
+            q = Rs[k] @ X[p] + ts[k]
+            for s in (0, 1):
+                pc = q + ext[s, 3:]
+                if pc[2] < 0.5:
+                    continue
+                u, v = K[0] * pc[0] / pc[2] + K[2], K[1] * pc[1] / pc[2] + K[3]
+                if not (0 <= u < 752 and 0 <= v < 480):
+                    continue
+                if rng.random() < outlier_fraction:
+                    u, v = u + rng.uniform(-60, 60), v + rng.uniform(-60, 60)
+                    bad.append(True)
+                else:
+                    u, v = u + rng.normal(0, noise_px), v + rng.normal(0, noise_px)
+                    bad.append(False)
+                pose_index.append(k)
+                point_index.append(p)
+                side.append(s)
+                obs.append((u, v))
+    poses = np.array(poses)
+    init = poses.copy()
+    for k in range(num_keyframes):
+        R = _aa_to_R(rng.normal(0, pose_sigma[0], 3)) @ Rs[k]
+        init[k, :3] = _R_to_aa(R)
+        init[k, 3:] += rng.normal(0, pose_sigma[1], 3)
+    return dict(pose_index=np.array(pose_index, np.int32), point_index=np.array(point_index, np.int32),
+                side=np.array(side, np.int32), observations=np.array(obs), poses=init,
+                points=X + rng.normal(0, point_sigma, X.shape), K=K, extrinsics=ext, true_poses=poses, true_points=X,
+                planted_outlier=np.array(bad))

@@ -67,6 +67,12 @@ class SynthParams(ctypes.Structure):
                 ("distortion_sigma", f64), ("seed", ctypes.c_uint64)]
 
 
+class StereoView(ctypes.Structure):
+    _fields_ = [("num_poses", i32), ("num_points", i32), ("num_observations", i32), ("pose_index", i32p),
+                ("point_index", i32p), ("side", i32p), ("observations", f64p), ("poses", f64p), ("points", f64p),
+                ("K", f64 * 4), ("extrinsics", (f64 * 6) * 2)]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("arena_bytes", i64), ("max_windows", i32), ("iterations_per_step", i32), ("options", Options)]
 
@@ -112,6 +118,9 @@ SIGNATURES = {
     "ba_server_destroy": (ctypes.c_int, [vp]),
     "ba_server_simulate": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(ProblemView), i32, f64, i32,
                                            ctypes.c_uint64, ctypes.POINTER(ServerResult)]),
+    "ba_stereo_default_options": (None, [ctypes.POINTER(Options)]),
+    "ba_stereo_solve": (ctypes.c_int, [ctypes.POINTER(StereoView), ctypes.POINTER(Options), ctypes.POINTER(Summary),
+                                        ctypes.POINTER(ctypes.c_uint8), f64p]),
     "ba_bal_load": (ctypes.c_int, [ctypes.c_char_p, i32, ctypes.POINTER(vp)]),
     "ba_bal_from_arrays": (ctypes.c_int, [i32, i32, i32, i32p, i32p, f64p, f64p, f64p, ctypes.POINTER(vp)]),
     "ba_bal_view": (ctypes.c_int, [vp, ctypes.POINTER(ProblemView)]),

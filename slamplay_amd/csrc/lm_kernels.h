@@ -29,6 +29,7 @@
 
 #include "common.h"
 #include "snavely.h"
+#include "stereo.h"
 
 namespace mba {
 
@@ -123,6 +124,9 @@ struct Dev {
     LMState* st;
     ba_iteration* trace;
     int trace_cap;
+    int model;                 // 0: Snavely camera (BAL); 1: stereo rig with SE3 poses (full_slam)
+    const int* co_side;        // model 1: camera side (0 left, 1 right) per camera-order observation
+    StereoRig rig;             // model 1
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -288,7 +292,7 @@ __device__ __forceinline__ void load_point(const double* __restrict__ pts, int p
 // that the index -> point gather of the next observation overlaps the current one's arithmetic.
 template <typename Real>
 struct ObsIn {
-    int p;
+    int p, side;
     double X[3];
     Real u, v;
 };
@@ -296,6 +300,7 @@ struct ObsIn {
 template <typename Real>
 __device__ __forceinline__ void load_obs(const Dev<Real>& d, int j, ObsIn<Real>& in) {
     in.p = d.co_pt[j];
+    in.side = d.model ? d.co_side[j] : 0;
     load_point<double>(d.pt_x, in.p, in.X);
     in.u = d.co_uv[2 * (size_t)j];
     in.v = d.co_uv[2 * (size_t)j + 1];
@@ -305,7 +310,10 @@ __device__ __forceinline__ void load_obs(const Dev<Real>& d, int j, ObsIn<Real>&
 template <typename Real>
 __device__ __forceinline__ void corrected_from(const Dev<Real>& d, const CameraPrep<Real>& P, const ObsIn<Real>& in,
                                                Real r[2], Real Jc[18], Real Jp[6]) {
-    snavely_residual_jacobian<Real>(P, in.X, in.u, in.v, r, Jc, Jp, d.o.g2o != 0);
+    if (d.model)
+        stereo_residual_jacobian<Real>(P, in.X, d.rig, in.side, in.u, in.v, r, Jc, Jp);
+    else
+        snavely_residual_jacobian<Real>(P, in.X, in.u, in.v, r, Jc, Jp, d.o.g2o != 0);
     const Real w = huber_sqrt_rho1<Real>(r[0] * r[0] + r[1] * r[1], d.o.loss, (Real)d.o.loss_a);
     r[0] *= w;
     r[1] *= w;
@@ -322,7 +330,10 @@ __device__ __forceinline__ void corrected_jacobian(const Dev<Real>& d, const Cam
     p = d.co_pt[j];
     double X[3];
     load_point<double>(d.pt_x, p, X);
-    snavely_residual_jacobian<Real>(P, X, d.co_uv[2 * (size_t)j], d.co_uv[2 * (size_t)j + 1], r, Jc, Jp, d.o.g2o != 0);
+    if (d.model)
+        stereo_residual_jacobian<Real>(P, X, d.rig, d.co_side[j], d.co_uv[2 * (size_t)j], d.co_uv[2 * (size_t)j + 1], r, Jc, Jp);
+    else
+        snavely_residual_jacobian<Real>(P, X, d.co_uv[2 * (size_t)j], d.co_uv[2 * (size_t)j + 1], r, Jc, Jp, d.o.g2o != 0);
     const Real w = huber_sqrt_rho1<Real>(r[0] * r[0] + r[1] * r[1], d.o.loss, (Real)d.o.loss_a);
     r[0] *= w;
     r[1] *= w;
@@ -474,7 +485,10 @@ __global__ void __launch_bounds__(kWG) k_cost_cam(Dev<Real> d, const double* __r
         double X[3];
         load_point<double>(pts, p, X);
         double r[2];
-        snavely_residual<double>(P, X, d.co_uv64[2 * (size_t)j], d.co_uv64[2 * (size_t)j + 1], r);
+        if (d.model)
+            stereo_residual<double>(P, X, d.rig, d.co_side[j], d.co_uv64[2 * (size_t)j], d.co_uv64[2 * (size_t)j + 1], r);
+        else
+            snavely_residual<double>(P, X, d.co_uv64[2 * (size_t)j], d.co_uv64[2 * (size_t)j + 1], r);
         acc += half_rho(r[0] * r[0] + r[1] * r[1], d.o.loss, d.o.loss_a);
     }
     acc = block_sum1(acc, lds);
@@ -1060,7 +1074,16 @@ __global__ void __launch_bounds__(64) k_cam_step(Dev<Real> d) {
         dx2 += dl * dl;
         x2 += x * x;
     }
-    if (d.o.g2o) {
+    if (d.model) {
+        // pose <- exp(xi) pose, xi = (rho, phi) in the first six slots
+        double xi[6], w[3], t[3];
+        for (int i = 0; i < 6; ++i) xi[i] = d.dcam[(size_t)i * d.nc + c];
+        for (int i = 0; i < 3; ++i) {
+            w[i] = d.cam_x[(size_t)i * d.nc + c];
+            t[i] = d.cam_x[(size_t)(3 + i) * d.nc + c];
+        }
+        se3_left_update(xi, w, t, cam, cam + 3);
+    } else if (d.o.g2o) {
         // rotation <- exp(delta) rotation, read back as angle-axis
         double dd[3], w[3];
         for (int i = 0; i < 3; ++i) {
@@ -1108,7 +1131,10 @@ __global__ void __launch_bounds__(kWG) k_model_cand(Dev<Real> d) {
         double X[3];
         load_point<double>(d.pt_c, p, X);
         double rc[2];
-        snavely_residual<double>(Pc, X, d.co_uv64[2 * (size_t)j], d.co_uv64[2 * (size_t)j + 1], rc);
+        if (d.model)
+            stereo_residual<double>(Pc, X, d.rig, d.co_side[j], d.co_uv64[2 * (size_t)j], d.co_uv64[2 * (size_t)j + 1], rc);
+        else
+            snavely_residual<double>(Pc, X, d.co_uv64[2 * (size_t)j], d.co_uv64[2 * (size_t)j + 1], rc);
         acc[1] += half_rho(rc[0] * rc[0] + rc[1] * rc[1], d.o.loss, d.o.loss_a);
     }
     block_sum<2>(acc, lds);

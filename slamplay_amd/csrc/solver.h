@@ -43,7 +43,15 @@ struct SolverBase {
     virtual int solver_kind() const = 0;
 };
 
-int make_solver(const ba_problem_view* v, const ba_options& o, std::unique_ptr<SolverBase>* out);
+// The full_slam stereo model (stereo.h): rig and per-observation camera side (caller order).
+struct StereoRig;
+struct StereoInput {
+    const StereoRig* rig;
+    const int32_t* side;
+};
+
+int make_solver(const ba_problem_view* v, const ba_options& o, std::unique_ptr<SolverBase>* out,
+                const StereoInput* stereo = nullptr);
 
 // BA_OK, or BA_ERR_INVALID_ARGUMENT with the reason for an option out of range.
 int validate_options(const ba_options* o);

@@ -15,6 +15,7 @@
 #include "dense.h"
 #include "lm_kernels.h"
 #include "solver.h"
+#include "stereo.h"
 
 namespace mba {
 
@@ -203,7 +204,7 @@ struct Solver final : SolverBase {
     int linear_solver = BA_SOLVER_ITERATIVE_SCHUR;
     Layout L;
     Dev<Real> d{};
-    DevBuf<int> cam_off, co_pt, chunk_off, chunk_len, slot_of;
+    DevBuf<int> cam_off, co_pt, chunk_off, chunk_len, slot_of, co_side;
     DevBuf<double> co_uv64;
     DevBuf<Real> co_uvr;
     DevBuf<double> cam_x, pt_x, cam_x0, pt_x0, cam_c, pt_c, dcam, dpt;
@@ -253,7 +254,7 @@ struct Solver final : SolverBase {
     }
     int solver_kind() const override { return linear_solver; }
 
-    int create(const ba_problem_view* v, const ba_options& o) {
+    int create(const ba_problem_view* v, const ba_options& o, const StereoInput* stereo = nullptr) {
         opt = o;
         device = o.device;
         BA_TRY(use_device(device));
@@ -317,7 +318,8 @@ struct Solver final : SolverBase {
         BA_HIP(hipMemsetAsync(cam_part.p, 0, cam_part.bytes(), s));
         BA_HIP(hipMemsetAsync(pt_part.p, 0, pt_part.bytes(), s));
         BA_TRY(st.alloc(1));
-        const int trace_cap = o.max_num_iterations + 2;
+        // one row per trial: a g2o iteration may take g2o_max_trials of them
+        const int trace_cap = o.max_num_iterations * (o.mode == BA_MODE_G2O ? std::max(o.g2o_max_trials, 1) : 1) + 2;
         BA_TRY(trace.alloc(trace_cap));
         d.nc = nc;
         d.np = np;
@@ -328,6 +330,17 @@ struct Solver final : SolverBase {
         d.cam_off = cam_off.p;
         d.co_pt = co_pt.p;
         d.co_uv64 = co_uv64.p;
+        d.model = 0;
+        d.co_side = nullptr;
+        if (stereo) {
+            std::vector<int> side(std::max(no, 1), 0);
+            for (int j = 0; j < no; ++j) side[j] = stereo->side[L.co_orig[j]] ? 1 : 0;
+            BA_TRY(co_side.upload(side.data(), side.size(), s));
+            BA_HIP(hipStreamSynchronize(s));
+            d.model = 1;
+            d.co_side = co_side.p;
+            d.rig = *stereo->rig;
+        }
         d.co_uv = sizeof(Real) == sizeof(double) ? (const Real*)co_uv64.p : co_uvr.p;
         d.chunk_off = chunk_off.p;
         d.chunk_len = chunk_len.p;
@@ -630,14 +643,15 @@ struct Solver final : SolverBase {
     }
 };
 
-int make_solver(const ba_problem_view* v, const ba_options& o, std::unique_ptr<SolverBase>* out) {
+int make_solver(const ba_problem_view* v, const ba_options& o, std::unique_ptr<SolverBase>* out,
+                const StereoInput* stereo) {
     if (o.precision == BA_PRECISION_FP64) {
         auto s = std::make_unique<Solver<double>>();
-        BA_TRY(s->create(v, o));
+        BA_TRY(s->create(v, o, stereo));
         *out = std::move(s);
     } else {
         auto s = std::make_unique<Solver<float>>();
-        BA_TRY(s->create(v, o));
+        BA_TRY(s->create(v, o, stereo));
         *out = std::move(s);
     }
     return BA_OK;
@@ -668,6 +682,81 @@ int ba_solve(const ba_problem_view* v, const ba_options* options, ba_summary* su
     BA_TRY(make_solver(v, o, &s));
     BA_TRY(s->run(summary));
     return s->download(v->cameras, v->points);
+}
+
+void ba_stereo_default_options(ba_options* o) {
+    if (!o) return;
+    ba_default_options(o);
+    o->mode = BA_MODE_G2O;                       // OptimizationAlgorithmLevenberg (backend.cpp:71-75)
+    o->max_num_iterations = 10;                  // optimizer.optimize(10) (backend.cpp:150)
+    o->loss_scale = 5.991;                       // RobustKernelHuber delta = chi2_th (backend.cpp:102,139-141)
+    o->linear_solver = BA_SOLVER_DENSE_SCHUR;    // LinearSolverCSparse on the 6-dof pose blocks
+}
+
+int ba_stereo_solve(const ba_stereo_view* v, const ba_options* options, ba_summary* summary, uint8_t* outlier,
+                    double* chi2_threshold) {
+    clear_error();
+    if (!v) return fail(BA_ERR_INVALID_ARGUMENT, "problem view is NULL");
+    if (v->num_observations > 0 && !v->side) return fail(BA_ERR_INVALID_ARGUMENT, "side array is NULL");
+    for (int k = 0; k < v->num_observations; ++k)
+        if (v->side[k] != 0 && v->side[k] != 1)
+            return fail(BA_ERR_INVALID_ARGUMENT, "observation %d: side must be 0 (left) or 1 (right)", k);
+    for (int i = 0; i < 4; ++i)
+        if (!std::isfinite(v->K[i])) return fail(BA_ERR_NON_FINITE, "K is not finite");
+    if (v->K[0] == 0.0 || v->K[1] == 0.0) return fail(BA_ERR_INVALID_ARGUMENT, "focal lengths must be nonzero");
+    ba_options o;
+    if (options)
+        o = *options;
+    else
+        ba_stereo_default_options(&o);
+    BA_TRY(check_options(&o));
+    if (v->num_poses > 0 && !v->poses) return fail(BA_ERR_INVALID_ARGUMENT, "poses is NULL");
+    // poses in the solver's 9-parameter slots (angle-axis, t, three unused entries)
+    std::vector<double> cams(9 * (size_t)std::max(v->num_poses, 0), 0.0);
+    for (int i = 0; i < v->num_poses; ++i)
+        for (int k = 0; k < 6; ++k) cams[9 * (size_t)i + k] = v->poses[6 * (size_t)i + k];
+    ba_problem_view pv{v->num_poses, v->num_points, v->num_observations, 9, v->pose_index, v->point_index,
+                       v->observations, cams.data(), v->points};
+    BA_TRY(validate_view(&pv, true));
+    StereoRig rig{};
+    for (int i = 0; i < 4; ++i) rig.K[i] = v->K[i];
+    for (int sd = 0; sd < 2; ++sd) {
+        double e9[9] = {0};
+        for (int k = 0; k < 6; ++k) e9[k] = v->extrinsics[sd][k];
+        CameraPrep<double> P;
+        camera_prep<double>(e9, P);
+        for (int k = 0; k < 9; ++k) rig.R[sd][k] = P.R64[k];
+        for (int k = 0; k < 3; ++k) rig.t[sd][k] = P.t64[k];
+    }
+    const StereoInput in{&rig, v->side};
+    std::unique_ptr<SolverBase> sv;
+    BA_TRY(make_solver(&pv, o, &sv, &in));
+    BA_TRY(sv->run(summary));
+    BA_TRY(sv->download(cams.data(), v->points));
+    for (int i = 0; i < v->num_poses; ++i)
+        for (int k = 0; k < 6; ++k) v->poses[6 * (size_t)i + k] = cams[9 * (size_t)i + k];
+    // the outlier rule of backend.cpp:153-186 on the optimised parameters
+    const int no = v->num_observations;
+    std::vector<double> chi2(no);
+    for (int k = 0; k < no; ++k) {
+        CameraPrep<double> P;
+        camera_prep<double>(&cams[9 * (size_t)v->pose_index[k]], P);
+        double r[2];
+        stereo_residual<double>(P, &v->points[3 * (size_t)v->point_index[k]], rig, v->side[k], v->observations[2 * (size_t)k],
+                                v->observations[2 * (size_t)k + 1], r);
+        chi2[k] = r[0] * r[0] + r[1] * r[1];
+    }
+    double th = o.loss_scale;
+    for (int it = 0; it < 5; ++it) {
+        int in_n = 0;
+        for (int k = 0; k < no; ++k) in_n += chi2[k] > th ? 0 : 1;
+        if (in_n / (double)no > 0.5) break;  // (no observations: 0/0, never > 0.5, as in the reference)
+        th *= 2.0;
+    }
+    if (outlier)
+        for (int k = 0; k < no; ++k) outlier[k] = chi2[k] > th ? 1 : 0;
+    if (chi2_threshold) *chi2_threshold = th;
+    return BA_OK;
 }
 
 int ba_solver_create(const ba_problem_view* v, const ba_options* options, ba_solver** out) {

@@ -247,3 +247,65 @@ def test_dense_schur_capacity_error():
         S.SolveBA(_copy(p), linear_solver=1, max_num_iterations=1)
     out = S.SolveBA(_copy(p), max_num_iterations=1)
     assert out["final_cost"] <= out["initial_cost"]
+
+
+def _outlier_rule(chi2, th=5.991):
+    """backend.cpp:153-186: double the threshold (at most 5 times) until more than half are inliers."""
+    for _ in range(5):
+        if (chi2 <= th).sum() / max(len(chi2), 1) > 0.5:
+            break
+        th *= 2.0
+    return chi2 > th, th
+
+
+def _stereo_oracle(oracle_mod, w, **kw):
+    o = oracle_mod.default_options(mode=1, max_num_iterations=10, loss_scale=5.991, linear_solver=1, **kw)
+    return oracle_mod.solve_stereo(w["pose_index"], w["point_index"], w["side"], w["observations"], w["poses"],
+                                   w["points"], w["K"], w["extrinsics"], o)
+
+
+def _stereo_args(w):
+    return {k: w[k] for k in ("pose_index", "point_index", "side", "observations", "poses", "points", "K",
+                              "extrinsics")}
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_stereo_local_ba_matches_oracle(oracle_mod, seed):
+    """full_slam's backend window (backend.cpp:67-186) in FP64: the default options are the
+    reference's (g2o Levenberg, 10 iterations, Huber delta 5.991); same trial sequence and chi2 per
+    trial within 1e-10 of the oracle until converged, final poses and points within 1e-7 of their
+    scale, and the outlier flags and threshold equal to the reference's rule applied to the oracle's
+    final residuals; most planted gross outliers are flagged."""
+    w = S.stereo_window(seed=seed)
+    ref = _stereo_oracle(oracle_mod, w)
+    out = S.stereo_solve(**_stereo_args(w), precision=0)
+    c_gpu, s_gpu = _trace_costs(out["trace"])
+    c_ref, s_ref = _trace_costs(ref["trace"])
+    final = ref["final_cost"]
+    n = next((k for k in range(len(c_ref)) if abs(c_ref[k] - final) <= 1e-9 * final), len(c_ref) - 1)
+    assert n >= 5 and len(c_gpu) > n
+    np.testing.assert_array_equal(s_gpu[: n + 1], s_ref[: n + 1])
+    rel = np.abs(c_gpu[: n + 1] - c_ref[: n + 1]) / c_ref[: n + 1]
+    assert rel.max() < 1e-10, rel
+    assert abs(out["final_cost"] - final) / final < 1e-9
+    np.testing.assert_allclose(out["poses"], ref["poses"], rtol=0, atol=1e-7 * np.abs(ref["poses"]).max())
+    np.testing.assert_allclose(out["points"], ref["pts"], rtol=0, atol=1e-7 * np.abs(ref["pts"]).max())
+    chi2 = np.array([(oracle_mod.stereo_residual(ref["poses"][w["pose_index"][k]], ref["pts"][w["point_index"][k]],
+                                                 w["K"], w["extrinsics"], w["side"][k], *w["observations"][k]) ** 2).sum()
+                     for k in range(len(w["side"]))])
+    flags, th = _outlier_rule(chi2)
+    assert out["chi2_threshold"] == th
+    # a chi2 within rounding of the threshold could flip; none is that close here
+    assert np.abs(chi2 - th).min() > 1e-6 * th
+    np.testing.assert_array_equal(out["outlier"], flags)
+    planted = w["planted_outlier"]
+    assert out["outlier"][planted].mean() > 0.9
+    assert out["outlier"][~planted].mean() < 0.08  # 1 px noise: 5% of inliers exceed chi2 5.991
+
+
+def test_stereo_local_ba_mixed_final_cost(oracle_mod):
+    w = S.stereo_window(seed=3)
+    ref = _stereo_oracle(oracle_mod, w)
+    out = S.stereo_solve(**_stereo_args(w), precision=1)
+    assert abs(out["final_cost"] - ref["final_cost"]) / ref["final_cost"] < 1e-5
+

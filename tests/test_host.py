@@ -184,3 +184,16 @@ def test_g2o_mode_rejected_by_batch_and_options():
     for bad in (dict(mode=2), dict(mode=1, g2o_max_trials=0), dict(mode=1, g2o_tau=0.0)):
         with pytest.raises(C.BAError):
             S.Batch(wins, S.default_options(**bad))
+
+
+def test_stereo_inputs_rejected():
+    """ba_stereo_solve checks the side flags and the focal lengths before any device work."""
+    w = S.stereo_window(seed=4, num_keyframes=3, num_points=20)
+    args = {k: w[k] for k in ("pose_index", "point_index", "side", "observations", "poses", "points", "K", "extrinsics")}
+    bad = dict(args)
+    bad["side"] = w["side"].copy()
+    bad["side"][0] = 2
+    with pytest.raises(C.BAError, match="side"):
+        S.stereo_solve(**bad)
+    with pytest.raises(C.BAError, match="focal"):
+        S.stereo_solve(**dict(args, K=np.array([0.0, 457.0, 367.0, 248.0])))

@@ -318,3 +318,125 @@ def test_g2o_mode_lambda_rule_and_convergence(oracle_mod):
         else:
             assert row["trust_region_radius"] == pytest.approx(prev["trust_region_radius"] / nu, rel=1e-12)
             nu *= 2.0
+
+
+# ---------------------------------------------------------------------------------------------
+# full_slam stereo local BA (full_slam/include/myslam/g2o_types.h:49-166, backend.cpp:67-186).
+
+def _hat(v):
+    return np.array([[0.0, -v[2], v[1]], [v[2], 0.0, -v[0]], [-v[1], v[0], 0.0]])
+
+
+def _se3_matrix(oracle_mod, pose6):
+    T = np.eye(4)
+    T[:3, :3] = _aa_matrix(oracle_mod, pose6[:3])
+    T[:3, 3] = pose6[3:]
+    return T
+
+
+def _twist(xi):
+    """4x4 hat of xi = (rho, phi): translation part first, as Sophus::SE3::exp takes it."""
+    A = np.zeros((4, 4))
+    A[:3, :3] = _hat(xi[3:])
+    A[:3, 3] = xi[:3]
+    return A
+
+
+def test_se3_left_update_is_matrix_exponential(oracle_mod):
+    """exp(xi) T from the oracle's closed form (quaternion rotation, left Jacobian V) equals the
+    series matrix exponential of the twist times T, for tiny, moderate and large rotations."""
+    from scipy.linalg import expm
+    rng = np.random.default_rng(31)
+    for scale in (1e-12, 1e-5, 0.2, 1.5):
+        for _ in range(15):
+            xi = np.concatenate([rng.normal(0, 1.0, 3), rng.normal(0, scale, 3)])
+            pose = np.concatenate([rng.normal(0, 0.8, 3), rng.normal(0, 2.0, 3)])
+            w, t = oracle_mod.se3_left_update(xi, pose[:3], pose[3:])
+            got = _se3_matrix(oracle_mod, np.concatenate([w, t]))
+            want = expm(_twist(xi)) @ _se3_matrix(oracle_mod, pose)
+            np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * (1 + np.abs(want).max()))
+    # a pure translation twist moves t only
+    w, t = oracle_mod.se3_left_update(np.array([0.3, -0.2, 0.1, 0, 0, 0]), np.array([0.1, 0.2, 0.3]), np.zeros(3))
+    np.testing.assert_allclose(w, [0.1, 0.2, 0.3], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(t, [0.3, -0.2, 0.1], rtol=0, atol=1e-15)
+
+
+def _stereo_case():
+    K = np.array([458.0, 457.0, 367.0, 248.0])
+    ext = np.zeros((2, 6))
+    ext[1, 3] = -0.5
+    return K, ext
+
+
+def test_stereo_residual_known_answers(oracle_mod):
+    """A point on the optical axis projects to the principal point in the left image and fx b / z
+    pixels to the left of it in the right image; the residual is observation minus prediction."""
+    K, ext = _stereo_case()
+    pose = np.zeros(6)
+    pt = np.array([0.0, 0.0, 5.0])
+    np.testing.assert_allclose(oracle_mod.stereo_residual(pose, pt, K, ext, 0, 370.0, 250.0), [3.0, 2.0], atol=1e-12)
+    r = oracle_mod.stereo_residual(pose, pt, K, ext, 1, 367.0, 248.0)
+    np.testing.assert_allclose(r, [458.0 * 0.5 / 5.0, 0.0], rtol=0, atol=1e-12)
+    # T maps world to camera: t = (1, 0, 0) puts the point at x = 1 in the camera frame
+    pose[3] = 1.0
+    r = oracle_mod.stereo_residual(pose, pt, K, ext, 0, 367.0, 248.0)
+    np.testing.assert_allclose(r, [-458.0 / 5.0, 0.0], rtol=0, atol=1e-12)
+
+
+def test_stereo_jacobians_match_central_differences(oracle_mod):
+    """Pose columns: d e / d xi of e = obs - pi(K exp(xi) E T X) at xi = 0 (the reference perturbs the
+    camera-frame transform E T, so for the left camera, E = I, this is the derivative under the
+    vertex update exp(xi) T; for the right camera it is not, and the test pins exactly that).  Point
+    columns: d e / d X, exact for both cameras."""
+    from scipy.linalg import expm
+    K, ext = _stereo_case()
+    ext[1, :3] = [0.01, -0.02, 0.005]  # a slightly rotated right camera makes E T != T E
+    rng = np.random.default_rng(32)
+    h = 1e-6
+
+    def pred(M, X):
+        q = M[:3, :3] @ X + M[:3, 3]
+        return np.array([K[0] * q[0] / q[2] + K[2], K[1] * q[1] / q[2] + K[3]])
+
+    for _ in range(20):
+        pose = np.concatenate([rng.normal(0, 0.2, 3), rng.normal(0, 0.5, 3)])
+        X = np.array([rng.uniform(-3, 3), rng.uniform(-2, 2), rng.uniform(4, 15)])
+        ou, ov = rng.uniform(0, 700), rng.uniform(0, 450)
+        for side in (0, 1):
+            r, jc, jp = oracle_mod.stereo_residual_jacobian(pose, X, K, ext, side, ou, ov)
+            M = _se3_matrix(oracle_mod, ext[side]) @ _se3_matrix(oracle_mod, pose)
+            np.testing.assert_allclose(r, np.array([ou, ov]) - pred(M, X), rtol=0, atol=1e-9)
+            for j in range(6):
+                e = np.zeros(6)
+                e[j] = h
+                num = -(pred(expm(_twist(e)) @ M, X) - pred(expm(_twist(-e)) @ M, X)) / (2 * h)
+                np.testing.assert_allclose(jc[:, j], num, rtol=0, atol=1e-5 * (1 + np.abs(num).max()))
+            for j in range(3):
+                e = np.zeros(3)
+                e[j] = h
+                num = -(pred(M, X + e) - pred(M, X - e)) / (2 * h)
+                np.testing.assert_allclose(jp[:, j], num, rtol=0, atol=1e-6 * (1 + np.abs(num).max()))
+            if side == 0:
+                # E = I: the same columns through the oracle's own vertex update
+                for j in range(6):
+                    e = np.zeros(6)
+                    e[j] = h
+                    wp, tp = oracle_mod.se3_left_update(e, pose[:3], pose[3:])
+                    wm, tm = oracle_mod.se3_left_update(-e, pose[:3], pose[3:])
+                    num = (oracle_mod.stereo_residual(np.concatenate([wp, tp]), X, K, ext, 0, ou, ov)
+                           - oracle_mod.stereo_residual(np.concatenate([wm, tm]), X, K, ext, 0, ou, ov)) / (2 * h)
+                    np.testing.assert_allclose(jc[:, j], num, rtol=0, atol=1e-5 * (1 + np.abs(num).max()))
+
+
+def test_stereo_noise_free_window_converges(oracle_mod):
+    """Without noise or outliers the g2o-mode stereo solve drives chi2 to ~0 and accepted chi2 never
+    rises."""
+    import slamplay_amd as S
+    w = S.stereo_window(seed=3, num_keyframes=4, num_points=60, noise_px=0.0, outlier_fraction=0.0)
+    o = oracle_mod.default_options(mode=1, max_num_iterations=30, loss_scale=5.991, linear_solver=1)
+    res = oracle_mod.solve_stereo(w["pose_index"], w["point_index"], w["side"], w["observations"], w["poses"],
+                                  w["points"], w["K"], w["extrinsics"], o)
+    assert res["initial_cost"] > 1.0
+    assert res["final_cost"] < 1e-16 * len(w["side"])
+    acc = [r["cost"] for r in res["trace"] if r["step_is_successful"]]
+    assert all(b <= a * (1 + 1e-15) for a, b in zip(acc, acc[1:]))

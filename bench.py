@@ -266,6 +266,41 @@ def bench_dense(S, seed, steps):
     return out
 
 
+def bench_preprocess(S, p, reps=5):
+    """Normalize + Perturb(0.1, 0.5, 0.5) of configuration 3 on the device-held parameters (radix
+    selection of the medians; jump-ahead glibc rand() stream) against the host BALProblem methods on
+    the same problem; best of `reps`, wall clock including the final synchronisation."""
+    s = S.Solver(p, S.default_options(precision=0, max_num_iterations=1))
+    dn = dp = None
+    for _ in range(reps):
+        s.reset()
+        t0 = time.perf_counter()
+        s.normalize()
+        t1 = time.perf_counter()
+        s.perturb(0.1, 0.5, 0.5, seed=1)
+        t2 = time.perf_counter()
+        dn = t1 - t0 if dn is None else min(dn, t1 - t0)
+        dp = t2 - t1 if dp is None else min(dp, t2 - t1)
+    s.close()
+    hn = hp = None
+    for _ in range(2):
+        q = S.BALProblem.from_arrays(p.camera_index(), p.point_index(), p.observations(), p.mutable_cameras(),
+                                     p.mutable_points())
+        t0 = time.perf_counter()
+        q.Normalize()
+        t1 = time.perf_counter()
+        q.Perturb(0.1, 0.5, 0.5)
+        t2 = time.perf_counter()
+        hn = t1 - t0 if hn is None else min(hn, t1 - t0)
+        hp = t2 - t1 if hp is None else min(hp, t2 - t1)
+    out = {"points": p.num_points(), "cameras": p.num_cameras(), "device_normalize_ms": dn * 1e3,
+           "device_perturb_ms": dp * 1e3, "host_normalize_ms": hn * 1e3, "host_perturb_ms": hp * 1e3,
+           "host_threads": 1}
+    log("bench: preprocess: device normalize %.3f ms, perturb %.3f ms; host %.3f / %.3f ms" %
+        (out["device_normalize_ms"], out["device_perturb_ms"], out["host_normalize_ms"], out["host_perturb_ms"]))
+    return out
+
+
 def calibrate_copy():
     """A plain device-to-device copy of 1 GiB (torch's elementwise copy kernel), five times: under
     rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch calibrate the counters to bytes."""
@@ -329,12 +364,16 @@ def main():
     if not args.no_serving:
         log("bench: serving loop (configuration 5)")
         serving = bench_serving(S, args.seed, [2000.0, 5000.0, 10000.0, 20000.0], args.serve_requests)
+    prep = None
+    if p is not None:
+        log("bench: device Normalize / Perturb (configuration 3)")
+        prep = bench_preprocess(S, p)
     cpu = None
     if not args.no_cpu_baseline and p is not None:
         log("bench: CPU oracle baseline (bounded sample)")
         cpu = cpu_baseline(S, p)
     line.update(roofline=line.get("roofline"), cpu_baseline=cpu, batched_local_ba=batch, serving=serving,
-                dense_config2=dense)
+                dense_config2=dense, preprocess=prep)
     if sm is not None:
         sm.close()
     print(json.dumps(line), flush=True)

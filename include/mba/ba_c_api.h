@@ -201,6 +201,16 @@ int ba_solver_reset(ba_solver* solver);
 int ba_solver_download(ba_solver* solver, double* cameras, double* points);
 int ba_solver_destroy(ba_solver* solver);
 
+/* BALProblem::Normalize (utils/datasets/BALProblem.cpp:263-303) on the parameters the solver holds
+ * on the device: per-axis upper medians, the upper median of the L1 distances to them, then points
+ * and camera centres scaled to that median distance 100.  An empty problem is left as it is. */
+int ba_solver_normalize(ba_solver* solver);
+/* BALProblem::Perturb (BALProblem.cpp:305-336) on the device-held parameters, with the normals
+ * (RandNormal, :39-55) that glibc's rand() gives after srand(seed); seed 1 is the stream of a
+ * process that never calls srand, as the reference's drivers.  Sigmas must be >= 0. */
+int ba_solver_perturb(ba_solver* solver, uint32_t seed, double rotation_sigma, double translation_sigma,
+                      double point_sigma);
+
 /* Per-kernel timing collected with HIP events on the solver's stream while enabled. */
 typedef struct ba_kernel_time {
     char name[48];

@@ -193,6 +193,14 @@ class Solver:
     def reset(self):
         C.check(C.lib().ba_solver_reset(self._h))
 
+    def normalize(self):
+        """BALProblem::Normalize on the device-held parameters."""
+        C.check(C.lib().ba_solver_normalize(self._h))
+
+    def perturb(self, rotation_sigma, translation_sigma, point_sigma, seed=1):
+        """BALProblem::Perturb on the device-held parameters, with glibc rand() after srand(seed)."""
+        C.check(C.lib().ba_solver_perturb(self._h, seed, rotation_sigma, translation_sigma, point_sigma))
+
     def download(self):
         cams = np.zeros(9 * self._nc)
         pts = np.zeros(3 * self._np)

@@ -96,6 +96,8 @@ SIGNATURES = {
     "ba_solver_run": (ctypes.c_int, [vp, ctypes.POINTER(Summary)]),
     "ba_solver_reset": (ctypes.c_int, [vp]),
     "ba_solver_download": (ctypes.c_int, [vp, f64p, f64p]),
+    "ba_solver_normalize": (ctypes.c_int, [vp]),
+    "ba_solver_perturb": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "ba_solver_destroy": (ctypes.c_int, [vp]),
     "ba_solver_set_profiling": (ctypes.c_int, [vp, i32]),
     "ba_solver_kernel_times": (ctypes.c_int, [vp, ctypes.POINTER(KernelTime), i32, i32p]),

@@ -37,6 +37,9 @@ struct SolverBase {
     virtual int run(ba_summary* summary) = 0;
     virtual int reset() = 0;
     virtual int download(double* cameras, double* points) = 0;
+    // BALProblem::Normalize / ::Perturb on the device-held parameters (preprocess.h)
+    virtual int normalize() = 0;
+    virtual int perturb(uint32_t seed, double rotation_sigma, double translation_sigma, double point_sigma) = 0;
     virtual int set_profiling(bool on) = 0;
     virtual int kernel_times(ba_kernel_time* out, int cap, int* count) = 0;
     virtual int64_t device_bytes() const = 0;

@@ -14,6 +14,7 @@
 
 #include "dense.h"
 #include "lm_kernels.h"
+#include "preprocess.h"
 #include "solver.h"
 #include "stereo.h"
 
@@ -218,6 +219,8 @@ struct Solver final : SolverBase {
     DevBuf<double> cam_part, pt_part;
     DevBuf<LMState> st;
     DevBuf<ba_iteration> trace;
+    DevBuf<int> pt_orig;  // internal point -> caller point, uploaded on the first Normalize / Perturb
+    PrepScratch prep_w;
     LMState* st_host = nullptr;  // pinned
     DenseSchur<Real> dense;
     // one LM iteration captured once and replayed (launch_mode 0)
@@ -391,6 +394,30 @@ struct Solver final : SolverBase {
     int reset() override {
         BA_HIP(hipMemcpyAsync(cam_x.p, cam_x0.p, cam_x0.bytes(), hipMemcpyDeviceToDevice, s));
         BA_HIP(hipMemcpyAsync(pt_x.p, pt_x0.p, pt_x0.bytes(), hipMemcpyDeviceToDevice, s));
+        BA_HIP(hipStreamSynchronize(s));
+        return BA_OK;
+    }
+
+    int prep_params(DeviceParams* p) {
+        BA_TRY(use_device(device));
+        if (!pt_orig.p && L.np) BA_TRY(pt_orig.upload(L.pt_perm.data(), L.pt_perm.size(), s));
+        *p = DeviceParams{cam_x.p, pt_x.p, pt_orig.p, L.nc, L.np};
+        return BA_OK;
+    }
+    int normalize() override {
+        if (d.model != 0) return fail(BA_ERR_INVALID_ARGUMENT, "Normalize applies to Snavely cameras only");
+        DeviceParams p;
+        BA_TRY(prep_params(&p));
+        BA_TRY(device_normalize(p, &prep_w, s));
+        BA_HIP(hipStreamSynchronize(s));
+        return BA_OK;
+    }
+    int perturb(uint32_t seed, double rs, double ts, double ps) override {
+        if (d.model != 0) return fail(BA_ERR_INVALID_ARGUMENT, "Perturb applies to Snavely cameras only");
+        if (!(rs >= 0.0) || !(ts >= 0.0) || !(ps >= 0.0)) return fail(BA_ERR_INVALID_ARGUMENT, "negative sigma");
+        DeviceParams p;
+        BA_TRY(prep_params(&p));
+        BA_TRY(device_perturb(p, seed, rs, ts, ps, &prep_w, s));
         BA_HIP(hipStreamSynchronize(s));
         return BA_OK;
     }
@@ -792,6 +819,19 @@ int ba_solver_download(ba_solver* h, double* cameras, double* points) {
     clear_error();
     if (!h) return fail(BA_ERR_INVALID_ARGUMENT, "NULL solver");
     return h->impl->download(cameras, points);
+}
+
+int ba_solver_normalize(ba_solver* h) {
+    clear_error();
+    if (!h) return fail(BA_ERR_INVALID_ARGUMENT, "NULL solver");
+    return h->impl->normalize();
+}
+
+int ba_solver_perturb(ba_solver* h, uint32_t seed, double rotation_sigma, double translation_sigma,
+                      double point_sigma) {
+    clear_error();
+    if (!h) return fail(BA_ERR_INVALID_ARGUMENT, "NULL solver");
+    return h->impl->perturb(seed, rotation_sigma, translation_sigma, point_sigma);
 }
 
 int ba_solver_destroy(ba_solver* h) {

@@ -1,0 +1,121 @@
+"""Normalize and Perturb on the device-held parameters (ba_solver_normalize / ba_solver_perturb) vs
+the oracle's BALProblem restatement (utils/datasets/BALProblem.cpp:39-76,230-336), which draws from
+glibc's own rand().
+
+Bars: medians, MAD, scale and the point transform are bit-identical to the oracle; camera translations
+(through sin/cos of the device math library) and perturbations (through log) within 1e-12 of the
+parameter scale; the normals come out in the reference's order, identical to glibc's stream up to
+the last bits of log().
+"""
+import numpy as np
+import pytest
+
+import slamplay_amd as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _dup_problem(npts, seed):
+    """Every point seen once by one camera; coordinates drawn from a small set, so the medians sit
+    inside runs of equal values."""
+    rng = np.random.default_rng(seed)
+    pts = rng.integers(-3, 4, (npts, 3)).astype(np.float64) * 0.5
+    cams = np.array([[0.1, -0.2, 0.05, 0.3, -0.1, -5.0, 500.0, 0.0, 0.0],
+                     [-0.3, 0.1, 0.2, 1.0, 2.0, -7.0, 450.0, 1e-3, 0.0]])
+    ci = (np.arange(npts) % 2).astype(np.int32)
+    pi = np.arange(npts, dtype=np.int32)
+    obs = rng.normal(0, 100, (npts, 2))
+    return S.BALProblem.from_arrays(ci, pi, obs, cams.reshape(-1), pts.reshape(-1))
+
+
+def _problems(ladybug_path):
+    yield "ladybug49", S.BALProblem(ladybug_path)
+    yield "synthetic16", S.synthetic(1, seed=3)
+    yield "dup-even", _dup_problem(1000, 1)
+    yield "dup-odd", _dup_problem(1001, 2)
+
+
+def test_device_normalize_matches_oracle(oracle_mod, ladybug_path):
+    for name, p in _problems(ladybug_path):
+        cams, pts = oracle_mod.normalize(p.num_cameras(), p.num_points(), p.mutable_cameras(), p.mutable_points())
+        sv = S.Solver(p)
+        sv.normalize()
+        gc, gp = sv.download()
+        np.testing.assert_array_equal(gp, pts, err_msg=name)
+        np.testing.assert_array_equal(gc.reshape(-1, 9)[:, :3], cams.reshape(-1, 9)[:, :3], err_msg=name)
+        np.testing.assert_array_equal(gc.reshape(-1, 9)[:, 6:], cams.reshape(-1, 9)[:, 6:], err_msg=name)
+        np.testing.assert_allclose(gc, cams, rtol=0, atol=1e-12 * np.abs(cams).max(), err_msg=name)
+        sv.close()
+
+
+@pytest.mark.parametrize("sigmas", [(0.1, 0.5, 0.5), (0.0, 0.5, 0.0), (0.1, 0.0, 0.0), (0.0, 0.0, 0.3),
+                                    (0.0, 0.0, 0.0)])
+@pytest.mark.parametrize("seed", [1, 0, 7, 2**31 + 5])
+def test_device_perturb_matches_oracle(oracle_mod, ladybug_path, sigmas, seed):
+    p = S.BALProblem(ladybug_path)
+    p.Normalize()
+    cams, pts = oracle_mod.perturb(49, 7776, p.mutable_cameras(), p.mutable_points(), *sigmas, seed=seed)
+    sv = S.Solver(p)
+    sv.perturb(*sigmas, seed=seed)
+    gc, gp = sv.download()
+    np.testing.assert_allclose(gp, pts, rtol=0, atol=1e-12 * np.abs(pts).max())
+    np.testing.assert_allclose(gc, cams, rtol=0, atol=1e-12 * np.abs(cams).max())
+    # a shifted normal sequence would move every parameter by O(sigma); the bar above is 1e-12
+    if sigmas[2] > 0:
+        assert np.abs(gp - p.mutable_points()).max() > 0.1 * sigmas[2]
+    # reset restores the created parameters; the same seed draws the same normals again
+    sv.reset()
+    c0, p0 = sv.download()
+    np.testing.assert_array_equal(c0, p.mutable_cameras())
+    np.testing.assert_array_equal(p0, p.mutable_points())
+    sv.perturb(*sigmas, seed=seed)
+    gc2, gp2 = sv.download()
+    np.testing.assert_array_equal(gc2, gc)
+    np.testing.assert_array_equal(gp2, gp)
+    sv.close()
+
+
+def test_device_normals_follow_glibc_stream(oracle_mod):
+    """With zero points and point sigma 1 the perturbed points are the normals themselves: 600k of
+    them (about 1.5M rand() calls over ~800 generator chunks) in the reference's order."""
+    npts = 200_000
+    pi = np.arange(npts, dtype=np.int32)
+    ci = np.zeros(npts, dtype=np.int32)
+    cams = np.array([0.0, 0.0, 0.0, 0.0, 0.0, -5.0, 500.0, 0.0, 0.0])
+    p = S.BALProblem.from_arrays(ci, pi, np.zeros((npts, 2)), cams, np.zeros(3 * npts))
+    for seed in (1, 12345):
+        _, ref = oracle_mod.perturb(1, npts, cams, np.zeros(3 * npts), 0.0, 0.0, 1.0, seed=seed)
+        sv = S.Solver(p)
+        sv.perturb(0.0, 0.0, 1.0, seed=seed)
+        _, got = sv.download()
+        sv.close()
+        np.testing.assert_allclose(got, ref, rtol=4 * np.finfo(np.float64).eps, atol=0)
+        assert (got == ref).mean() > 0.5
+        # the standard normal's moments, as a check on the oracle side of the comparison too
+        assert abs(ref.mean()) < 0.01 and abs(ref.std() - 1.0) < 0.01
+
+
+def test_device_preprocess_then_solve(oracle_mod, ladybug_path):
+    """Normalize + Perturb on the device followed by a solve equals the same solve from the oracle's
+    preprocessed parameters (FP64, fixed schedule, final cost within 1e-9)."""
+    p = S.BALProblem(ladybug_path)
+    sv = S.Solver(p, max_num_iterations=5, function_tolerance=0.0, gradient_tolerance=0.0, parameter_tolerance=0.0,
+                  linear_solver=1)
+    sv.normalize()
+    sv.perturb(0.1, 0.5, 0.5, seed=1)
+    out = sv.run()
+    sv.close()
+    cams, pts = oracle_mod.normalize(49, 7776, p.mutable_cameras(), p.mutable_points())
+    cams, pts = oracle_mod.perturb(49, 7776, cams, pts, 0.1, 0.5, 0.5, seed=1)
+    q = S.BALProblem.from_arrays(p.camera_index(), p.point_index(), p.observations(), cams, pts)
+    ref = S.SolveBA(q, max_num_iterations=5, function_tolerance=0.0, gradient_tolerance=0.0, parameter_tolerance=0.0,
+                    linear_solver=1)
+    assert abs(out["initial_cost"] - ref["initial_cost"]) / ref["initial_cost"] < 1e-12
+    assert abs(out["final_cost"] - ref["final_cost"]) / ref["final_cost"] < 1e-9
+
+
+def test_device_perturb_rejects_negative_sigma(ladybug_path):
+    sv = S.Solver(S.BALProblem(ladybug_path))
+    with pytest.raises(S.BAError, match="negative"):
+        sv.perturb(-0.1, 0.5, 0.5)
+    sv.close()

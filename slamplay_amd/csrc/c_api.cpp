@@ -3,7 +3,10 @@
 
 #include <cmath>
 #include <cstdio>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 
 #include "common.h"
 
@@ -31,6 +34,19 @@ int use_device(int device) {
     if (e != hipSuccess || n == 0) return fail(BA_ERR_NO_DEVICE, "no HIP device available");
     if (device < 0 || device >= n) return fail(BA_ERR_NO_DEVICE, "device %d out of range (%d devices)", device, n);
     BA_HIP(hipSetDevice(device));
+    return BA_OK;
+}
+
+int allow_dynamic_lds(const void* kernel, int bytes) {
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, int> granted;  // (device, kernel) -> bytes
+    int dev = 0;
+    BA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    int& have = granted[{dev, kernel}];
+    if (have >= bytes) return BA_OK;
+    BA_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    have = bytes;
     return BA_OK;
 }
 

@@ -73,4 +73,9 @@ inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 // Sets the calling thread's device; returns BA_ERR_NO_DEVICE if it does not exist.
 int use_device(int device);
 
+// Allows `kernel` at least `bytes` of dynamic LDS on the current device.  The attribute is process
+// wide: it is set under a lock and only when a larger value is asked for, so solvers and servers
+// created or stepped on several threads do not call into the runtime for it concurrently.
+int allow_dynamic_lds(const void* kernel, int bytes);
+
 }  // namespace mba

@@ -32,7 +32,7 @@ namespace mba {
 constexpr int kTile = 32;
 constexpr int kJs = 36;  // per-observation stride of Jc~ (18), Jp~ (6), Hp (6), I - G (3)
 
-template <typename Real>
+template <typename Real, int M>
 __global__ void __launch_bounds__(kWG) k_dense_store(Dev<Real> d, Real* __restrict__ js) {
     LMState* st = d.st;
     if (st->done || st->step_invalid) return;
@@ -44,7 +44,7 @@ __global__ void __launch_bounds__(kWG) k_dense_store(Dev<Real> d, Real* __restri
     for (int j = d.cam_off[c] + threadIdx.x; j < d.cam_off[c + 1]; j += kWG) {
         int p;
         Real r[2], Jc[18], Jp[6];
-        corrected_jacobian<Real>(d, P, j, p, r, Jc, Jp);
+        corrected_jacobian<Real, M>(d, P, j, p, r, Jc, Jp);
         Real* o = js + kJs * (size_t)j;
         for (int i = 0; i < 9; ++i) {
             o[i] = Jc[i] * sc[i];
@@ -382,9 +382,9 @@ struct DenseSchur {
             return fail(BA_ERR_CAPACITY, "dense Schur with %d cameras exceeds the triangular solve's LDS (at most %d "
                         "cameras); use BA_SOLVER_ITERATIVE_SCHUR", nc, (int)(kSolveLdsMax / sizeof(double) / 9));
         if (sizeof(Real) == sizeof(double))
-            BA_HIP(hipFuncSetAttribute((const void*)k_chol_solve<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+            BA_TRY(allow_dynamic_lds((const void*)k_chol_solve<double>, (int)shm));
         else
-            BA_HIP(hipFuncSetAttribute((const void*)k_chol_solve<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+            BA_TRY(allow_dynamic_lds((const void*)k_chol_solve<float>, (int)shm));
         PairBlocks pb;
         build_pair_blocks(L, &pb);
         const std::vector<int>&ba = pb.a, &bb = pb.b, &boff = pb.off, &pi = pb.i, &pj = pb.j;
@@ -431,7 +431,10 @@ struct DenseSchur {
     void enqueue_solve(SolverT& sv) {
         auto& d = sv.d;
         const int nc = d.nc;
-        sv.launch("dense_store", k_dense_store<Real>, xcd_grid(nc), kWG, d, js.p);
+        sv.launch("dense_store",
+                  pick_model(d.model, k_dense_store<Real, kModelSnavely>, k_dense_store<Real, kModelSnavelyG2o>,
+                             k_dense_store<Real, kModelStereo>),
+                  xcd_grid(nc), kWG, d, js.p);
         const size_t nn = (size_t)ld * ld;
         sv.launch("fill_S", k_fill_S, (int)((nn + 255) / 256), 256, S.p, n, ld, (const LMState*)d.st);
         sv.launch("schur_pairs", k_schur_pairs<Real>, nblocks, kWG, d, (const Real*)js.p, (const int*)blk_a.p,

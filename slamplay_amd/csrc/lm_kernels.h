@@ -124,10 +124,22 @@ struct Dev {
     LMState* st;
     ba_iteration* trace;
     int trace_cap;
-    int model;                 // 0: Snavely camera (BAL); 1: stereo rig with SE3 poses (full_slam)
-    const int* co_side;        // model 1: camera side (0 left, 1 right) per camera-order observation
-    StereoRig rig;             // model 1
+    int model;                 // kModelSnavely / kModelSnavelyG2o / kModelStereo (below)
+    const int* co_side;        // stereo: camera side (0 left, 1 right) per camera-order observation
+    StereoRig rig;             // stereo
 };
+
+// Residual models, a compile-time parameter of the kernels that evaluate residuals or update the
+// cameras, so each solver runs code with only its own model's registers live.
+constexpr int kModelSnavely = 0;     // Snavely camera, additive update (Ceres)
+constexpr int kModelSnavelyG2o = 1;  // Snavely camera, exp(delta) R rotation update (g2o driver)
+constexpr int kModelStereo = 2;      // stereo rig, SE3 pose update (full_slam)
+
+// The instantiation of a kernel for residual model m.
+template <typename K>
+inline K pick_model(int m, K snavely, K snavely_g2o, K stereo) {
+    return m == kModelStereo ? stereo : (m == kModelSnavelyG2o ? snavely_g2o : snavely);
+}
 
 // ---------------------------------------------------------------------------------------------
 // Reductions
@@ -297,23 +309,23 @@ struct ObsIn {
     Real u, v;
 };
 
-template <typename Real>
+template <typename Real, int M>
 __device__ __forceinline__ void load_obs(const Dev<Real>& d, int j, ObsIn<Real>& in) {
     in.p = d.co_pt[j];
-    in.side = d.model ? d.co_side[j] : 0;
+    in.side = M == kModelStereo ? d.co_side[j] : 0;
     load_point<double>(d.pt_x, in.p, in.X);
     in.u = d.co_uv[2 * (size_t)j];
     in.v = d.co_uv[2 * (size_t)j + 1];
 }
 
 // Corrected residual and Jacobians from prefetched inputs (same arithmetic as corrected_jacobian).
-template <typename Real>
+template <typename Real, int M>
 __device__ __forceinline__ void corrected_from(const Dev<Real>& d, const CameraPrep<Real>& P, const ObsIn<Real>& in,
                                                Real r[2], Real Jc[18], Real Jp[6]) {
-    if (d.model)
+    if constexpr (M == kModelStereo)
         stereo_residual_jacobian<Real>(P, in.X, d.rig, in.side, in.u, in.v, r, Jc, Jp);
     else
-        snavely_residual_jacobian<Real>(P, in.X, in.u, in.v, r, Jc, Jp, d.o.g2o != 0);
+        snavely_residual_jacobian<Real>(P, in.X, in.u, in.v, r, Jc, Jp, M == kModelSnavelyG2o);
     const Real w = huber_sqrt_rho1<Real>(r[0] * r[0] + r[1] * r[1], d.o.loss, (Real)d.o.loss_a);
     r[0] *= w;
     r[1] *= w;
@@ -324,16 +336,17 @@ __device__ __forceinline__ void corrected_from(const Dev<Real>& d, const CameraP
 }
 
 // Corrected (sqrt(rho')-weighted) residual and Jacobians of observation j of camera prep P.
-template <typename Real>
+template <typename Real, int M>
 __device__ __forceinline__ void corrected_jacobian(const Dev<Real>& d, const CameraPrep<Real>& P, int j, int& p,
                                                    Real r[2], Real Jc[18], Real Jp[6]) {
     p = d.co_pt[j];
     double X[3];
     load_point<double>(d.pt_x, p, X);
-    if (d.model)
+    if constexpr (M == kModelStereo)
         stereo_residual_jacobian<Real>(P, X, d.rig, d.co_side[j], d.co_uv[2 * (size_t)j], d.co_uv[2 * (size_t)j + 1], r, Jc, Jp);
     else
-        snavely_residual_jacobian<Real>(P, X, d.co_uv[2 * (size_t)j], d.co_uv[2 * (size_t)j + 1], r, Jc, Jp, d.o.g2o != 0);
+        snavely_residual_jacobian<Real>(P, X, d.co_uv[2 * (size_t)j], d.co_uv[2 * (size_t)j + 1], r, Jc, Jp,
+                                        M == kModelSnavelyG2o);
     const Real w = huber_sqrt_rho1<Real>(r[0] * r[0] + r[1] * r[1], d.o.loss, (Real)d.o.loss_a);
     r[0] *= w;
     r[1] *= w;
@@ -361,7 +374,7 @@ __global__ void __launch_bounds__(64) k_cam_prep(Dev<Real> d) {
 // One workgroup per camera: diag(Jc'Jc) and g_c = Jc'r (unscaled, corrected) -- the column norms
 // for the Jacobi scaling and the LM diagonal -- plus each observation's point contributions
 // Jp'Jp and Jp'r for k_lin_pt.
-template <typename Real>
+template <typename Real, int M>
 __global__ void __launch_bounds__(kWG) k_lin_cam(Dev<Real> d) {
     if (!d.st->need_lin) return;
     __shared__ double lds[18 * kWaves];
@@ -374,7 +387,7 @@ __global__ void __launch_bounds__(kWG) k_lin_cam(Dev<Real> d) {
     for (int j = d.cam_off[c] + threadIdx.x; j < d.cam_off[c + 1]; j += kWG) {
         int p;
         Real r[2], Jc[18], Jp[6];
-        corrected_jacobian<Real>(d, P, j, p, r, Jc, Jp);
+        corrected_jacobian<Real, M>(d, P, j, p, r, Jc, Jp);
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             acc[i] += (double)(Jc[i] * Jc[i] + Jc[9 + i] * Jc[9 + i]);
@@ -473,7 +486,7 @@ __global__ void __launch_bounds__(kWG) k_lin_pt(Dev<Real> d) {
 }
 
 // FP64 cost 1/2 sum rho at the parameters whose prep is prep_c and points pts (one WG per camera).
-template <typename Real>
+template <typename Real, int M>
 __global__ void __launch_bounds__(kWG) k_cost_cam(Dev<Real> d, const double* __restrict__ pts) {
     __shared__ double lds[kWaves];
     const int c = xcd_remap(blockIdx.x, d.nc);
@@ -485,7 +498,7 @@ __global__ void __launch_bounds__(kWG) k_cost_cam(Dev<Real> d, const double* __r
         double X[3];
         load_point<double>(pts, p, X);
         double r[2];
-        if (d.model)
+        if constexpr (M == kModelStereo)
             stereo_residual<double>(P, X, d.rig, d.co_side[j], d.co_uv64[2 * (size_t)j], d.co_uv64[2 * (size_t)j + 1], r);
         else
             snavely_residual<double>(P, X, d.co_uv64[2 * (size_t)j], d.co_uv64[2 * (size_t)j + 1], r);
@@ -668,7 +681,7 @@ __device__ __forceinline__ void point_complement(const Dev<Real>& d, int p, cons
 }
 
 // One WG per camera: b_c, M_c (residual form above) and D_c^2.
-template <typename Real>
+template <typename Real, int M>
 __global__ void __launch_bounds__(kWG) k_rhs_cam(Dev<Real> d) {
     LMState* st = d.st;
     if (st->done) return;
@@ -684,7 +697,7 @@ __global__ void __launch_bounds__(kWG) k_rhs_cam(Dev<Real> d) {
     for (int j = d.cam_off[c] + threadIdx.x; j < d.cam_off[c + 1]; j += kWG) {
         int p;
         Real r[2], Jc[18], Jp[6];
-        corrected_jacobian<Real>(d, P, j, p, r, Jc, Jp);
+        corrected_jacobian<Real, M>(d, P, j, p, r, Jc, Jp);
         scale_jacobians<Real>(d, sc, p, Jc, Jp);
         const R4<Real> q = d.qp[p];
         const Real e0 = r[0] - (Jp[0] * q.x + Jp[1] * q.y + Jp[2] * q.z);
@@ -797,7 +810,7 @@ __device__ __forceinline__ void reduce_upd(const double* __restrict__ part, int 
 
 // One WG per camera.  mode 1 (PCG iteration it): see above.  mode 0 (back-substitution):
 // u_j = Jp'(r~ - Jc S_c y_c) with y = vsrc.
-template <typename Real>
+template <typename Real, int M>
 __global__ void __launch_bounds__(kWG) k_jtx(Dev<Real> d, int mode, int it, const double* __restrict__ vsrc) {
     LMState* st = d.st;
     if (st->done || st->step_invalid || (mode == 1 && st->pcg_done)) return;
@@ -809,7 +822,7 @@ __global__ void __launch_bounds__(kWG) k_jtx(Dev<Real> d, int mode, int it, cons
     const int j_end = d.cam_off[c + 1];
     int j = d.cam_off[c] + threadIdx.x;
     ObsIn<Real> in{};
-    if (j < j_end) load_obs<Real>(d, j, in);
+    if (j < j_end) load_obs<Real, M>(d, j, in);
     double beta = 0.0;
     if (mode == 1) {
         double rz, rr;
@@ -845,9 +858,9 @@ __global__ void __launch_bounds__(kWG) k_jtx(Dev<Real> d, int mode, int it, cons
     const CameraPrep<Real> P = d.prep[c];
     for (; j < j_end; j += kWG) {
         const ObsIn<Real> cur = in;
-        if (j + kWG < j_end) load_obs<Real>(d, j + kWG, in);
+        if (j + kWG < j_end) load_obs<Real, M>(d, j + kWG, in);
         Real r[2], Jc[18], Jp[6];
-        corrected_from<Real>(d, P, cur, r, Jc, Jp);
+        corrected_from<Real, M>(d, P, cur, r, Jc, Jp);
         Real a0 = 0, a1 = 0;
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
@@ -908,7 +921,7 @@ __global__ void __launch_bounds__(kWG) k_pt_z(Dev<Real> d) {
 
 // One WG per camera: q_c = D_c^2 p_c + S_c sum Jc' e_j, e_j = Jc (S_c p_c) - Jp zs_p;
 // partial sum |e|^2 + p_c'D_c^2 p_c.
-template <typename Real>
+template <typename Real, int M>
 __global__ void __launch_bounds__(kWG) k_jz(Dev<Real> d) {
     LMState* st = d.st;
     if (st->done || st->step_invalid || st->pcg_done) return;
@@ -919,7 +932,7 @@ __global__ void __launch_bounds__(kWG) k_jz(Dev<Real> d) {
     const int j_end = d.cam_off[c + 1];
     int j = d.cam_off[c] + threadIdx.x;
     ObsIn<Real> in{};
-    if (j < j_end) load_obs<Real>(d, j, in);
+    if (j < j_end) load_obs<Real, M>(d, j, in);
     if (threadIdx.x < 9) psh[threadIdx.x] = d.pp[9 * (size_t)c + threadIdx.x];
     __syncthreads();
     Real ps[9];
@@ -930,9 +943,9 @@ __global__ void __launch_bounds__(kWG) k_jz(Dev<Real> d) {
     for (int i = 0; i < 10; ++i) acc[i] = 0.0;
     for (; j < j_end; j += kWG) {
         const ObsIn<Real> cur = in;
-        if (j + kWG < j_end) load_obs<Real>(d, j + kWG, in);
+        if (j + kWG < j_end) load_obs<Real, M>(d, j + kWG, in);
         Real r[2], Jc[18], Jp[6];
-        corrected_from<Real>(d, P, cur, r, Jc, Jp);
+        corrected_from<Real, M>(d, P, cur, r, Jc, Jp);
         const R4<Real> z = d.zs[cur.p];
         Real e0 = -(Jp[0] * z.x + Jp[1] * z.y + Jp[2] * z.z);
         Real e1 = -(Jp[3] * z.x + Jp[4] * z.y + Jp[5] * z.z);
@@ -1059,7 +1072,7 @@ __global__ void __launch_bounds__(kWG) k_pt_backsub(Dev<Real> d) {
 }
 
 // One thread per camera: delta_c = -S_c y_c, candidate camera and its FP64 prep.
-template <typename Real>
+template <typename Real, int M>
 __global__ void __launch_bounds__(64) k_cam_step(Dev<Real> d) {
     LMState* st = d.st;
     if (st->done || st->step_invalid) return;
@@ -1074,7 +1087,7 @@ __global__ void __launch_bounds__(64) k_cam_step(Dev<Real> d) {
         dx2 += dl * dl;
         x2 += x * x;
     }
-    if (d.model) {
+    if constexpr (M == kModelStereo) {
         // pose <- exp(xi) pose, xi = (rho, phi) in the first six slots
         double xi[6], w[3], t[3];
         for (int i = 0; i < 6; ++i) xi[i] = d.dcam[(size_t)i * d.nc + c];
@@ -1083,7 +1096,7 @@ __global__ void __launch_bounds__(64) k_cam_step(Dev<Real> d) {
             t[i] = d.cam_x[(size_t)(3 + i) * d.nc + c];
         }
         se3_left_update(xi, w, t, cam, cam + 3);
-    } else if (d.o.g2o) {
+    } else if constexpr (M == kModelSnavelyG2o) {
         // rotation <- exp(delta) rotation, read back as angle-axis
         double dd[3], w[3];
         for (int i = 0; i < 3; ++i) {
@@ -1102,7 +1115,7 @@ __global__ void __launch_bounds__(64) k_cam_step(Dev<Real> d) {
 
 // One WG per camera: model cost change -(J~ s).(r~ + J~ s / 2) with J, r at x (working
 // precision), and the FP64 candidate cost at x + delta.
-template <typename Real>
+template <typename Real, int M>
 __global__ void __launch_bounds__(kWG) k_model_cand(Dev<Real> d) {
     LMState* st = d.st;
     if (st->done || st->step_invalid) return;
@@ -1117,7 +1130,7 @@ __global__ void __launch_bounds__(kWG) k_model_cand(Dev<Real> d) {
     for (int j = d.cam_off[c] + threadIdx.x; j < d.cam_off[c + 1]; j += kWG) {
         int p;
         Real r[2], Jc[18], Jp[6];
-        corrected_jacobian<Real>(d, P, j, p, r, Jc, Jp);
+        corrected_jacobian<Real, M>(d, P, j, p, r, Jc, Jp);
         const double4 dp4 = reinterpret_cast<const double4*>(d.dpt)[p];
         const Real dp[3] = {(Real)dp4.x, (Real)dp4.y, (Real)dp4.z};
         Real m0 = Jp[0] * dp[0] + Jp[1] * dp[1] + Jp[2] * dp[2];
@@ -1131,7 +1144,7 @@ __global__ void __launch_bounds__(kWG) k_model_cand(Dev<Real> d) {
         double X[3];
         load_point<double>(d.pt_c, p, X);
         double rc[2];
-        if (d.model)
+        if constexpr (M == kModelStereo)
             stereo_residual<double>(Pc, X, d.rig, d.co_side[j], d.co_uv64[2 * (size_t)j], d.co_uv64[2 * (size_t)j + 1], rc);
         else
             snavely_residual<double>(Pc, X, d.co_uv64[2 * (size_t)j], d.co_uv64[2 * (size_t)j + 1], rc);

@@ -20,6 +20,9 @@
 
 namespace mba {
 
+// the instantiation of kernel k for this solver's residual model (d.model)
+#define BA_MODEL_KERNEL(k) pick_model(d.model, k<Real, kModelSnavely>, k<Real, kModelSnavelyG2o>, k<Real, kModelStereo>)
+
 namespace {
 
 Opts make_opts(const ba_options& o) {
@@ -333,14 +336,14 @@ struct Solver final : SolverBase {
         d.cam_off = cam_off.p;
         d.co_pt = co_pt.p;
         d.co_uv64 = co_uv64.p;
-        d.model = 0;
+        d.model = o.mode == BA_MODE_G2O ? kModelSnavelyG2o : kModelSnavely;
         d.co_side = nullptr;
         if (stereo) {
             std::vector<int> side(std::max(no, 1), 0);
             for (int j = 0; j < no; ++j) side[j] = stereo->side[L.co_orig[j]] ? 1 : 0;
             BA_TRY(co_side.upload(side.data(), side.size(), s));
             BA_HIP(hipStreamSynchronize(s));
-            d.model = 1;
+            d.model = kModelStereo;
             d.co_side = co_side.p;
             d.rig = *stereo->rig;
         }
@@ -405,7 +408,7 @@ struct Solver final : SolverBase {
         return BA_OK;
     }
     int normalize() override {
-        if (d.model != 0) return fail(BA_ERR_INVALID_ARGUMENT, "Normalize applies to Snavely cameras only");
+        if (d.model == kModelStereo) return fail(BA_ERR_INVALID_ARGUMENT, "Normalize applies to Snavely cameras only");
         DeviceParams p;
         BA_TRY(prep_params(&p));
         BA_TRY(device_normalize(p, &prep_w, s));
@@ -413,7 +416,7 @@ struct Solver final : SolverBase {
         return BA_OK;
     }
     int perturb(uint32_t seed, double rs, double ts, double ps) override {
-        if (d.model != 0) return fail(BA_ERR_INVALID_ARGUMENT, "Perturb applies to Snavely cameras only");
+        if (d.model == kModelStereo) return fail(BA_ERR_INVALID_ARGUMENT, "Perturb applies to Snavely cameras only");
         if (!(rs >= 0.0) || !(ts >= 0.0) || !(ps >= 0.0)) return fail(BA_ERR_INVALID_ARGUMENT, "negative sigma");
         DeviceParams p;
         BA_TRY(prep_params(&p));
@@ -469,7 +472,7 @@ struct Solver final : SolverBase {
     void enqueue_linearize() {
         const int nc = L.nc, gpt = xcd_grid(d.n_pt_blocks), gcam = xcd_grid(nc);
         launch("cam_prep", k_cam_prep<Real>, div_up(nc, 64), 64, d);
-        launch("lin_cam", k_lin_cam<Real>, gcam, kWG, d);
+        launch("lin_cam", BA_MODEL_KERNEL(k_lin_cam), gcam, kWG, d);
         launch("lin_pt", k_lin_pt<Real>, gpt, kWG, d);
     }
 
@@ -477,24 +480,24 @@ struct Solver final : SolverBase {
         const int nc = L.nc, np = L.np;
         const int gpt = xcd_grid(d.n_pt_blocks), gcam = xcd_grid(nc);
         launch("damp_pt", k_damp_pt<Real>, div_up(std::max(np, 1), kWG), kWG, d);
-        launch("rhs_cam", k_rhs_cam<Real>, gcam, kWG, d);
+        launch("rhs_cam", BA_MODEL_KERNEL(k_rhs_cam), gcam, kWG, d);
         if (linear_solver == BA_SOLVER_DENSE_SCHUR) {
             dense.enqueue_solve(*this);
         } else {
             launch("cam_factor", k_cam_factor<Real>, d.n_upd, kWG, d);
             for (int it = 1; it <= opt.pcg_max_iterations; ++it) {
-                launch("jtx", k_jtx<Real>, gcam, kWG, d, 1, it, (const double*)d.pp);
+                launch("jtx", BA_MODEL_KERNEL(k_jtx), gcam, kWG, d, 1, it, (const double*)d.pp);
                 launch("pt_z", k_pt_z<Real>, gpt, kWG, d);
-                launch("jz", k_jz<Real>, gcam, kWG, d);
+                launch("jz", BA_MODEL_KERNEL(k_jz), gcam, kWG, d);
                 launch("cam_update", k_cam_update<Real>, d.n_upd, kWG, d, it);
             }
             // the convergence test after the last update (sets the iteration count)
-            launch("jtx", k_jtx<Real>, gcam, kWG, d, 1, opt.pcg_max_iterations + 1, (const double*)d.pp);
+            launch("jtx", BA_MODEL_KERNEL(k_jtx), gcam, kWG, d, 1, opt.pcg_max_iterations + 1, (const double*)d.pp);
         }
-        launch("jtx_y", k_jtx<Real>, gcam, kWG, d, 0, 0, (const double*)d.px);
+        launch("jtx_y", BA_MODEL_KERNEL(k_jtx), gcam, kWG, d, 0, 0, (const double*)d.px);
         launch("pt_backsub", k_pt_backsub<Real>, gpt, kWG, d);
-        launch("cam_step", k_cam_step<Real>, div_up(nc, 64), 64, d);
-        launch("model_cand", k_model_cand<Real>, gcam, kWG, d);
+        launch("cam_step", BA_MODEL_KERNEL(k_cam_step), div_up(nc, 64), 64, d);
+        launch("model_cand", BA_MODEL_KERNEL(k_model_cand), gcam, kWG, d);
         launch("decide", k_decide<Real>, 1, kWG, d);
         launch("apply", k_apply<Real>, div_up(9 * (int64_t)nc + 4 * (int64_t)np, kWG), kWG, d);
         enqueue_linearize();
@@ -541,7 +544,7 @@ struct Solver final : SolverBase {
         }
         enqueue_linearize();
         launch("prep_x", k_prep_from<Real>, div_up(nc, 64), 64, d, (const double*)d.cam_x);
-        launch("cost_x", k_cost_cam<Real>, xcd_grid(nc), kWG, d, (const double*)d.pt_x);
+        launch("cost_x", BA_MODEL_KERNEL(k_cost_cam), xcd_grid(nc), kWG, d, (const double*)d.pt_x);
         launch("post_lin", k_post_lin<Real>, 1, kWG, d);
         BA_HIP(hipGetLastError());
         const int check_every = 4;

@@ -428,10 +428,8 @@ struct WindowEngine {
 
     template <int NT>
     int set_lds_limits() {
-        BA_HIP(hipFuncSetAttribute((const void*)k_wschur<Real, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)schur_lds<Real>(NT)));
-        BA_HIP(hipFuncSetAttribute((const void*)k_wsolve<Real, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)solve_lds()));
+        BA_TRY(allow_dynamic_lds((const void*)k_wschur<Real, NT>, (int)schur_lds<Real>(NT)));
+        BA_TRY(allow_dynamic_lds((const void*)k_wsolve<Real, NT>, (int)solve_lds()));
         return BA_OK;
     }
 

@@ -19,7 +19,7 @@ struct DeviceParams {
 // Scratch owned by the caller's handle, grown on demand and kept between calls.
 struct PrepScratch {
     DevBuf<unsigned long long> sel_state;  // 4 selections x (prefix, mask, rank)
-    DevBuf<unsigned int> hist;             // per-block digit histograms
+    DevBuf<unsigned int> hist;             // digit counts of the four selections (256 each)
     DevBuf<double> dist;                   // L1 distances to the median
     DevBuf<unsigned int> draws;            // glibc rand() outputs
     DevBuf<unsigned int> tables;           // jump-ahead matrices

@@ -21,7 +21,7 @@ namespace {
 // outputs, so rand() call k returns r_{344+k} >> 1.  The state (r_{i-31}, ..., r_{i-1}) evolves by
 // a fixed 31x31 matrix over Z/2^32, which is what makes jump-ahead possible.
 constexpr int kDeg = 31;
-constexpr int kChunk = 31 * 64;  // draws per generator thread
+constexpr int kChunk = 31 * 8;   // draws per generator thread
 constexpr int kJumps = 24;       // generator threads below 2^24
 constexpr int kPairsPerThread = 4;
 constexpr int kPairBlock = 256;
@@ -91,15 +91,17 @@ const std::vector<uint32_t>& jump_tables() {
 
 // One thread per chunk of kChunk draws: jump from the seed state to the chunk start, then run the
 // recurrence 31 draws at a time with the state in registers (s[j] += s[j+28] for the first three,
-// whose r_{i-3} is still the previous round's, s[j] += s[j-3] for the rest).
+// whose r_{i-3} is still the previous round's, s[j] += s[j-3] for the rest).  Each round's 64 x 31
+// draws of the workgroup go through LDS so that lanes store consecutive draws of one chunk.
 __global__ void __launch_bounds__(64) k_rand_generate(RandState start, const uint32_t* __restrict__ tables,
                                                       int n_chunks, int64_t n_draws, uint32_t* __restrict__ out) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n_chunks) return;
+    __shared__ uint32_t tile[64][kDeg + 2];  // odd row stride: the per-thread row writes hit distinct banks
+    const int b = blockIdx.x * 64 + threadIdx.x;
+    const int lane_c = threadIdx.x / kDeg, lane_j = threadIdx.x < 2 * kDeg ? threadIdx.x - lane_c * kDeg : kDeg;
     uint32_t s[kDeg];
 #pragma unroll
     for (int j = 0; j < kDeg; ++j) s[j] = start.s[j];
-    for (int q = 0; q < kJumps && (b >> q); ++q) {
+    for (int q = 0; q < kJumps && b < n_chunks && (b >> q); ++q) {
         if (!((b >> q) & 1)) continue;
         const uint32_t* Q = tables + (size_t)q * kDeg * kDeg;
         uint32_t t[kDeg];
@@ -113,16 +115,22 @@ __global__ void __launch_bounds__(64) k_rand_generate(RandState start, const uin
 #pragma unroll
         for (int j = 0; j < kDeg; ++j) s[j] = t[j];
     }
-    const int64_t base = (int64_t)b * kChunk;
     for (int round = 0; round < kChunk / kDeg; ++round) {
 #pragma unroll
         for (int j = 0; j < 3; ++j) s[j] += s[j + 28];
 #pragma unroll
         for (int j = 3; j < kDeg; ++j) s[j] += s[j - 3];
-        const int64_t o = base + (int64_t)round * kDeg;
 #pragma unroll
-        for (int j = 0; j < kDeg; ++j)
-            if (o + j < n_draws) out[o + j] = s[j] >> 1;
+        for (int j = 0; j < kDeg; ++j) tile[threadIdx.x][j] = s[j] >> 1;
+        __syncthreads();
+        // lanes 0-30 and 31-61 store the 31 draws of two chunks per instruction
+        if (lane_j < kDeg)
+            for (int c = lane_c; c < 64; c += 2) {
+                const int64_t bb = (int64_t)blockIdx.x * 64 + c;
+                const int64_t o = bb * kChunk + (int64_t)round * kDeg + lane_j;
+                if (bb < n_chunks && o < n_draws) out[o] = tile[c][lane_j];
+            }
+        __syncthreads();
     }
 }
 
@@ -289,13 +297,15 @@ __device__ __forceinline__ double key_value(unsigned long long k) {
     return __longlong_as_double((long long)u);
 }
 
-__global__ void k_sel_init(unsigned long long* __restrict__ sel, unsigned long long rank) {
+__global__ void __launch_bounds__(256) k_sel_init(unsigned long long* __restrict__ sel, unsigned long long rank,
+                                                  unsigned int* __restrict__ hist) {
     const int k = threadIdx.x;
     if (k < 4) {
         sel[3 * k] = 0;
         sel[3 * k + 1] = 0;
         sel[3 * k + 2] = rank;
     }
+    for (int i = k; i < 4 * 256; i += 256) hist[i] = 0;
 }
 
 __global__ void __launch_bounds__(kSelBlock) k_sel_hist(const double* __restrict__ pt, const double* __restrict__ dist,
@@ -312,31 +322,35 @@ __global__ void __launch_bounds__(kSelBlock) k_sel_hist(const double* __restrict
         if ((key & mask) == prefix) atomicAdd(&h[(key >> shift) & 255], 1u);
     }
     __syncthreads();
-    hist[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 256 + threadIdx.x] = h[threadIdx.x];
+    // integer counts: the sum does not depend on the order the blocks add in
+    if (h[threadIdx.x]) atomicAdd(&hist[256 * k + threadIdx.x], h[threadIdx.x]);
 }
 
-__global__ void __launch_bounds__(256) k_sel_pick(unsigned long long* __restrict__ sel,
-                                                  const unsigned int* __restrict__ hist, int first, int shift,
-                                                  int nblocks) {
-    __shared__ unsigned int c[256];
-    const int k = first + (int)blockIdx.x;
-    unsigned int tot = 0;
-    for (int g = 0; g < nblocks; ++g) tot += hist[((size_t)blockIdx.x * nblocks + g) * 256 + threadIdx.x];
-    c[threadIdx.x] = tot;
+// digit counts scanned; the thread whose bin holds the rank records it, and the counts are cleared
+// for the next pass
+__global__ void __launch_bounds__(256) k_sel_pick(unsigned long long* __restrict__ sel, unsigned int* __restrict__ hist,
+                                                  int first, int shift) {
+    __shared__ unsigned int wsum[4];
+    const int k = first + (int)blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const unsigned int tot = hist[256 * k + t];
+    hist[256 * k + t] = 0;
+    unsigned int x = tot;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long r = sel[3 * k + 2];
-        int bin = 255;
-        for (int b = 0; b < 256; ++b) {
-            if (r < c[b]) {
-                bin = b;
-                break;
-            }
-            r -= c[b];
-        }
-        sel[3 * k] |= (unsigned long long)bin << shift;
+    unsigned int off = 0;
+    for (int w = 0; w < wave; ++w) off += wsum[w];
+    const unsigned long long incl = off + x, excl = incl - tot;
+    const unsigned long long r = sel[3 * k + 2];
+    __syncthreads();  // every thread has read the rank before it is replaced
+    if (excl <= r && r < incl) {
+        sel[3 * k] |= (unsigned long long)t << shift;
         sel[3 * k + 1] |= 255ull << shift;
-        sel[3 * k + 2] = r;
+        sel[3 * k + 2] = r - excl;
     }
 }
 
@@ -384,21 +398,21 @@ int device_normalize(const DeviceParams& p, PrepScratch* w, hipStream_t s) {
     const int np = p.np;
     const int nblocks = std::max(1, std::min(256, div_up(np, kSelBlock * 8)));
     BA_TRY(ensure(w->sel_state, 12));
-    BA_TRY(ensure(w->hist, (size_t)3 * nblocks * 256));
+    BA_TRY(ensure(w->hist, (size_t)4 * 256));
     BA_TRY(ensure(w->dist, (size_t)np));
-    hipLaunchKernelGGL(k_sel_init, dim3(1), dim3(64), 0, s, w->sel_state.p, (unsigned long long)(np / 2));
+    hipLaunchKernelGGL(k_sel_init, dim3(1), dim3(256), 0, s, w->sel_state.p, (unsigned long long)(np / 2), w->hist.p);
     for (int pass = 0; pass < 8; ++pass) {
         const int shift = 56 - 8 * pass;
         hipLaunchKernelGGL(k_sel_hist, dim3(nblocks, 3), dim3(kSelBlock), 0, s, p.pt, w->dist.p, np, w->sel_state.p, 0,
                            shift, w->hist.p);
-        hipLaunchKernelGGL(k_sel_pick, dim3(3), dim3(256), 0, s, w->sel_state.p, w->hist.p, 0, shift, nblocks);
+        hipLaunchKernelGGL(k_sel_pick, dim3(3), dim3(256), 0, s, w->sel_state.p, w->hist.p, 0, shift);
     }
     hipLaunchKernelGGL(k_norm_dist, dim3(div_up(np, 256)), dim3(256), 0, s, p.pt, np, w->sel_state.p, w->dist.p);
     for (int pass = 0; pass < 8; ++pass) {
         const int shift = 56 - 8 * pass;
         hipLaunchKernelGGL(k_sel_hist, dim3(nblocks, 1), dim3(kSelBlock), 0, s, p.pt, w->dist.p, np, w->sel_state.p, 3,
                            shift, w->hist.p);
-        hipLaunchKernelGGL(k_sel_pick, dim3(1), dim3(256), 0, s, w->sel_state.p, w->hist.p, 3, shift, nblocks);
+        hipLaunchKernelGGL(k_sel_pick, dim3(1), dim3(256), 0, s, w->sel_state.p, w->hist.p, 3, shift);
     }
     hipLaunchKernelGGL(k_norm_apply, dim3(div_up(std::max(np, p.nc), 256)), dim3(256), 0, s, p.pt, np, p.cam, p.nc,
                        w->sel_state.p);

@@ -83,19 +83,22 @@ def profile_kernels(s):
 CPU_ITERS = 4
 
 
-def cpu_baseline(S, p):
-    """The FP64 CPU oracle (one thread) on the first CPU_ITERS LM iterations of the same problem and
-    options (about 10 s); LM iterations per second, the initial linearization included."""
+def cpu_baseline(S, p, threads=1):
+    """The FP64 CPU oracle on the first CPU_ITERS LM iterations of the same problem and options;
+    LM iterations per second, the initial linearization included.  With threads > 1 the oracle splits
+    its loops by camera / point (same results, bitwise)."""
     from oracle import oracle as O
+    threads = O.set_threads(threads)
     o = O.default_options(max_num_iterations=CPU_ITERS, linear_solver=2, pcg_max_iterations=50, pcg_tolerance=1e-6,
                           function_tolerance=0.0, gradient_tolerance=0.0, parameter_tolerance=0.0)
     t0 = time.perf_counter()
     O.solve(p.num_cameras(), p.num_points(), p.camera_index(), p.point_index(), p.observations(),
             p.mutable_cameras(), p.mutable_points(), o)
     dt = time.perf_counter() - t0
-    return {"value": CPU_ITERS / dt, "unit": "LM iters/s", "cores": 1, "kind": "port",
-            "sample": "FP64 CPU oracle, 1 thread: initial linearization + %d LM iterations (PCG <= 50, tol 1e-6) "
-                      "of the full 1.2M-observation problem, %.1f s" % (CPU_ITERS, dt)}
+    O.set_threads(1)
+    return {"value": CPU_ITERS / dt, "unit": "LM iters/s", "cores": threads, "kind": "port",
+            "sample": "FP64 CPU oracle, %d thread(s): initial linearization + %d LM iterations (PCG <= 50, tol 1e-6) "
+                      "of the full 1.2M-observation problem, %.1f s" % (threads, CPU_ITERS, dt)}
 
 
 WINDOW_ITERS = 10
@@ -309,6 +312,29 @@ def bench_preprocess(S, p, reps=5):
     return out
 
 
+def measure_copy_gbs(nbytes=1 << 30, reps=10):
+    """Measured HBM denominator: a device-to-device copy of 1 GiB (torch's elementwise copy kernel),
+    best of `reps` by HIP events; (read + write bytes) / time."""
+    import torch
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    del a, b
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / (best * 1e-3) / 1e9
+
+
 def calibrate_copy():
     """A plain device-to-device copy of 1 GiB (torch's elementwise copy kernel), five times: under
     rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch calibrate the counters to bytes."""
@@ -379,11 +405,22 @@ def main():
     if p is not None:
         log("bench: device Normalize / Perturb (configuration 3)")
         prep = bench_preprocess(S, p)
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu_baseline and p is not None:
-        log("bench: CPU oracle baseline (bounded sample)")
+        log("bench: CPU oracle baseline (bounded sample), 1 thread")
         cpu = cpu_baseline(S, p)
-    line.update(roofline=line.get("roofline"), cpu_baseline=cpu, batched_local_ba=batch, serving=serving,
+        log("bench: CPU 1 thread: %.3f LM iters/s" % cpu["value"])
+        n = max(1, min(16, os.cpu_count() or 1))  # a GPU box's CPU share is 16 cores
+        log("bench: CPU oracle baseline, %d threads" % n)
+        cpu_all = cpu_baseline(S, p, threads=n)
+        log("bench: CPU %d threads: %.3f LM iters/s" % (cpu_all["cores"], cpu_all["value"]))
+    if line.get("roofline"):
+        log("bench: measured copy bandwidth (1 GiB device copy)")
+        copy_gbs = measure_copy_gbs()
+        line["roofline"].update(peak_measured_copy=copy_gbs, frac_of_measured_copy=line["roofline"]["achieved"] / copy_gbs)
+        log("bench: device copy %.0f GB/s (spec %.0f)" % (copy_gbs, HBM_PEAK_GBS))
+    line.update(roofline=line.get("roofline"), cpu_baseline=cpu, cpu_baseline_all_cores=cpu_all,
+                batched_local_ba=batch, serving=serving,
                 dense_config2=dense, preprocess=prep)
     if sm is not None:
         sm.close()

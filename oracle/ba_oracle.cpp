@@ -296,12 +296,22 @@ struct StereoModel {
     const int* side;
 };
 
+struct PointObs;
+
+// Timing aid for the CPU baseline: threads for the loops below that are split by camera / point so
+// that every sum still runs over its observations in ascending order (results do not depend on the
+// thread count).  1 unless oracle_set_threads says otherwise.
+static int g_threads = 1;
+
 struct Problem {
     int nc, np, no;
     const int* ci;
     const int* pi;
     const double* obs;
     const StereoModel* stereo = nullptr;
+    // observations of each camera / point in ascending order; set for multi-threaded solves only
+    const PointObs* by_cam = nullptr;
+    const PointObs* by_pt = nullptr;
 };
 
 void stereo_point(const StereoModel& g, int s, const double* cam, const double* pt, double p[3]) {
@@ -374,6 +384,38 @@ void se3_left_update(const double* xi, const double* w, const double* t, double*
     for (int i = 0; i < 3; ++i) t_out[i] = Rt[i] + (rho[i] + k1 * pr[i] + k2 * ppr[i]);
 }
 
+// Observations' lists (CSR) per camera or per point, ascending within each list.
+struct PointObs {
+    std::vector<int> off, idx;
+};
+
+PointObs obs_lists(const int* key, int n_keys, int no) {
+    PointObs po;
+    po.off.assign(n_keys + 1, 0);
+    for (int k = 0; k < no; ++k) po.off[key[k] + 1]++;
+    for (int c = 0; c < n_keys; ++c) po.off[c + 1] += po.off[c];
+    po.idx.resize(no);
+    std::vector<int> fill(po.off.begin(), po.off.end() - 1);
+    for (int k = 0; k < no; ++k) po.idx[fill[key[k]]++] = k;
+    return po;
+}
+
+// f(k) for every observation, grouped by camera (by_camera) or by point: each camera's / point's
+// observations in ascending order, the groups split over threads when the lists are present.  A
+// sum that f accumulates into its camera's / point's slot therefore runs in the serial order.
+template <typename F>
+void for_obs_by(const Problem& P, bool by_camera, F&& f) {
+    const PointObs* lists = by_camera ? P.by_cam : P.by_pt;
+    if (!lists || g_threads <= 1) {
+        for (int k = 0; k < P.no; ++k) f(k);
+        return;
+    }
+    const int n = by_camera ? P.nc : P.np;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(g_threads)
+    for (int g = 0; g < n; ++g)
+        for (int a = lists->off[g]; a < lists->off[g + 1]; ++a) f(lists->idx[a]);
+}
+
 // Per-observation linearization (corrected, unscaled) plus problem-level gradient and cost.
 struct Linearization {
     std::vector<double> jc, jp, r;  // 18, 6, 2 per observation (Huber-corrected)
@@ -395,15 +437,18 @@ double obs_cost(const Options& o, double r0, double r1, double* sqrt_rho1) {
 
 // Cost only (Ceres Evaluate with no Jacobian). Returns false on a non-finite cost.
 bool evaluate_cost(const Problem& P, const Options& o, const double* cams, const double* pts, double* cost) {
-    double sum = 0.0;
+    std::vector<double> ck(P.no);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (int k = 0; k < P.no; ++k) {
         double r[2];
         if (P.stereo)
             stereo_residual(*P.stereo, P.stereo->side[k], cams + 9 * P.ci[k], pts + 3 * P.pi[k], P.obs[2 * k], P.obs[2 * k + 1], r);
         else
             snavely_residual(cams + 9 * P.ci[k], pts + 3 * P.pi[k], P.obs[2 * k], P.obs[2 * k + 1], r);
-        sum += obs_cost(o, r[0], r[1], nullptr);
+        ck[k] = obs_cost(o, r[0], r[1], nullptr);
     }
+    double sum = 0.0;
+    for (int k = 0; k < P.no; ++k) sum += ck[k];
     *cost = sum;
     return std::isfinite(sum);
 }
@@ -413,7 +458,8 @@ bool linearize(const Problem& P, const Options& o, const double* cams, const dou
     L->jp.assign(6 * (size_t)P.no, 0.0);
     L->r.assign(2 * (size_t)P.no, 0.0);
     L->grad.assign(9 * (size_t)P.nc + 3 * (size_t)P.np, 0.0);
-    double sum = 0.0;
+    std::vector<double> ck(P.no);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (int k = 0; k < P.no; ++k) {
         double r[2], jc[18], jp[6], w;
         if (P.stereo) {
@@ -423,19 +469,28 @@ bool linearize(const Problem& P, const Options& o, const double* cams, const dou
             residual_and_jacobian(cams + 9 * P.ci[k], pts + 3 * P.pi[k], P.obs[2 * k], P.obs[2 * k + 1], r, jc, jp);
             if (o.mode == 1) left_rotation_columns(cams + 9 * P.ci[k], pts + 3 * P.pi[k], jc);
         }
-        sum += obs_cost(o, r[0], r[1], &w);
+        ck[k] = obs_cost(o, r[0], r[1], &w);
         for (int i = 0; i < 18; ++i) L->jc[18 * (size_t)k + i] = w * jc[i];
         for (int i = 0; i < 6; ++i) L->jp[6 * (size_t)k + i] = w * jp[i];
         L->r[2 * (size_t)k] = w * r[0];
         L->r[2 * (size_t)k + 1] = w * r[1];
+    }
+    double sum = 0.0;
+    for (int k = 0; k < P.no; ++k) sum += ck[k];
+    auto grad_cam = [&](int k) {
         double* gc = &L->grad[9 * (size_t)P.ci[k]];
-        double* gp = &L->grad[9 * (size_t)P.nc + 3 * (size_t)P.pi[k]];
         const double* J = &L->jc[18 * (size_t)k];
-        const double* Jp = &L->jp[6 * (size_t)k];
         const double* rr = &L->r[2 * (size_t)k];
         for (int i = 0; i < 9; ++i) gc[i] += J[i] * rr[0] + J[9 + i] * rr[1];
+    };
+    auto grad_pt = [&](int k) {
+        double* gp = &L->grad[9 * (size_t)P.nc + 3 * (size_t)P.pi[k]];
+        const double* Jp = &L->jp[6 * (size_t)k];
+        const double* rr = &L->r[2 * (size_t)k];
         for (int i = 0; i < 3; ++i) gp[i] += Jp[i] * rr[0] + Jp[3 + i] * rr[1];
-    }
+    };
+    for_obs_by(P, true, grad_cam);
+    for_obs_by(P, false, grad_pt);
     L->cost = sum;
     return std::isfinite(sum);
 }
@@ -504,26 +559,38 @@ bool build_blocks(const Problem& P, const Linearization& L, const std::vector<do
     S->W.assign(27 * (size_t)P.no, 0.0);
     S->gc.assign(9 * (size_t)nc, 0.0);
     S->gp.assign(3 * (size_t)np, 0.0);
-    for (int k = 0; k < P.no; ++k) {
+    auto scaled = [&](int k, double jc[18], double jp[6]) {
         const int c = P.ci[k], p = P.pi[k];
-        double jc[18], jp[6];
         for (int row = 0; row < 2; ++row) {
             for (int i = 0; i < 9; ++i) jc[9 * row + i] = L.jc[18 * (size_t)k + 9 * row + i] * scale[9 * (size_t)c + i];
             for (int i = 0; i < 3; ++i) jp[3 * row + i] = L.jp[6 * (size_t)k + 3 * row + i] * scale[9 * (size_t)nc + 3 * (size_t)p + i];
         }
+    };
+    auto cam_part = [&](int k) {  // U and g_c of the observation's camera
+        double jc[18], jp[6];
+        scaled(k, jc, jp);
+        const int c = P.ci[k];
         const double* r = &L.r[2 * (size_t)k];
         double* U = &S->U[81 * (size_t)c];
         for (int i = 0; i < 9; ++i)
             for (int j = 0; j < 9; ++j) U[9 * i + j] += jc[i] * jc[j] + jc[9 + i] * jc[9 + j];
+        for (int i = 0; i < 9; ++i) S->gc[9 * (size_t)c + i] += jc[i] * r[0] + jc[9 + i] * r[1];
+    };
+    auto pt_part = [&](int k) {  // V, W and g_p of the observation's point
+        double jc[18], jp[6];
+        scaled(k, jc, jp);
+        const int p = P.pi[k];
+        const double* r = &L.r[2 * (size_t)k];
         double* Vp = &V[9 * (size_t)p];
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) Vp[3 * i + j] += jp[i] * jp[j] + jp[3 + i] * jp[3 + j];
         double* W = &S->W[27 * (size_t)k];
         for (int i = 0; i < 9; ++i)
             for (int j = 0; j < 3; ++j) W[3 * i + j] = jc[i] * jp[j] + jc[9 + i] * jp[3 + j];
-        for (int i = 0; i < 9; ++i) S->gc[9 * (size_t)c + i] += jc[i] * r[0] + jc[9 + i] * r[1];
         for (int i = 0; i < 3; ++i) S->gp[3 * (size_t)p + i] += jp[i] * r[0] + jp[3 + i] * r[1];
-    }
+    };
+    for_obs_by(P, true, cam_part);
+    for_obs_by(P, false, pt_part);
     for (int c = 0; c < nc; ++c)
         for (int i = 0; i < 9; ++i) S->U[81 * (size_t)c + 10 * i] += diag_d2[9 * (size_t)c + i];
     S->Vinv.assign(9 * (size_t)np, 0.0);
@@ -538,36 +605,27 @@ bool build_blocks(const Problem& P, const Linearization& L, const std::vector<do
 // Diagonal of J~'J~ (scaled, corrected Jacobian).
 void jtj_diagonal(const Problem& P, const Linearization& L, const std::vector<double>& scale, std::vector<double>* d) {
     d->assign(9 * (size_t)P.nc + 3 * (size_t)P.np, 0.0);
-    for (int k = 0; k < P.no; ++k) {
-        const int c = P.ci[k], p = P.pi[k];
+    for_obs_by(P, true, [&](int k) {
+        const int c = P.ci[k];
         for (int i = 0; i < 9; ++i) {
             const double s = scale[9 * (size_t)c + i];
             const double a = L.jc[18 * (size_t)k + i] * s, b = L.jc[18 * (size_t)k + 9 + i] * s;
             (*d)[9 * (size_t)c + i] += a * a + b * b;
         }
+    });
+    for_obs_by(P, false, [&](int k) {
+        const int p = P.pi[k];
         for (int i = 0; i < 3; ++i) {
             const size_t col = 9 * (size_t)P.nc + 3 * (size_t)p + i;
             const double s = scale[col];
             const double a = L.jp[6 * (size_t)k + i] * s, b = L.jp[6 * (size_t)k + 3 + i] * s;
             (*d)[col] += a * a + b * b;
         }
-    }
+    });
 }
 
 // Points' observation lists (CSR) for the Schur elimination.
-struct PointObs {
-    std::vector<int> off, idx;
-};
-PointObs point_obs(const Problem& P) {
-    PointObs po;
-    po.off.assign(P.np + 1, 0);
-    for (int k = 0; k < P.no; ++k) po.off[P.pi[k] + 1]++;
-    for (int p = 0; p < P.np; ++p) po.off[p + 1] += po.off[p];
-    po.idx.resize(P.no);
-    std::vector<int> fill(po.off.begin(), po.off.end() - 1);
-    for (int k = 0; k < P.no; ++k) po.idx[fill[P.pi[k]]++] = k;
-    return po;
-}
+PointObs point_obs(const Problem& P) { return obs_lists(P.pi, P.np, P.no); }
 
 // q = Vinv_p * v (3x3 times 3)
 inline void mul33(const double* M, const double* v, double* out) {
@@ -578,19 +636,20 @@ inline void mul33(const double* M, const double* v, double* out) {
 void schur_rhs(const Problem& P, const Schur& S, std::vector<double>* b) {
     b->assign(S.gc.begin(), S.gc.end());
     std::vector<double> q(3 * (size_t)P.np);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (int p = 0; p < P.np; ++p) mul33(&S.Vinv[9 * (size_t)p], &S.gp[3 * (size_t)p], &q[3 * (size_t)p]);
-    for (int k = 0; k < P.no; ++k) {
+    for_obs_by(P, true, [&](int k) {
         const double* W = &S.W[27 * (size_t)k];
         const double* qp = &q[3 * (size_t)P.pi[k]];
         double* bc = &(*b)[9 * (size_t)P.ci[k]];
         for (int i = 0; i < 9; ++i) bc[i] -= W[3 * i] * qp[0] + W[3 * i + 1] * qp[1] + W[3 * i + 2] * qp[2];
-    }
+    });
 }
 
 // yp = Vinv_p (gp - sum_k W_k' yc)
 void schur_back_substitute(const Problem& P, const Schur& S, const double* yc, double* yp) {
     std::vector<double> t(S.gp);
-    for (int k = 0; k < P.no; ++k) {
+    for_obs_by(P, false, [&](int k) {
         const double* W = &S.W[27 * (size_t)k];
         const double* y = yc + 9 * (size_t)P.ci[k];
         double* tp = &t[3 * (size_t)P.pi[k]];
@@ -599,7 +658,8 @@ void schur_back_substitute(const Problem& P, const Schur& S, const double* yc, d
             for (int i = 0; i < 9; ++i) s += W[3 * i + j] * y[i];
             tp[j] -= s;
         }
-    }
+    });
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (int p = 0; p < P.np; ++p) mul33(&S.Vinv[9 * (size_t)p], &t[3 * (size_t)p], yp + 3 * (size_t)p);
 }
 
@@ -636,7 +696,7 @@ void schur_dense_matrix(const Problem& P, const PointObs& po, const Schur& S, st
 // out = S v for the implicit reduced camera matrix.
 void schur_matvec(const Problem& P, const Schur& S, const double* v, double* out) {
     std::vector<double> t(3 * (size_t)P.np, 0.0), z(3 * (size_t)P.np);
-    for (int k = 0; k < P.no; ++k) {
+    for_obs_by(P, false, [&](int k) {
         const double* W = &S.W[27 * (size_t)k];
         const double* vc = v + 9 * (size_t)P.ci[k];
         double* tp = &t[3 * (size_t)P.pi[k]];
@@ -645,8 +705,10 @@ void schur_matvec(const Problem& P, const Schur& S, const double* v, double* out
             for (int i = 0; i < 9; ++i) s += W[3 * i + j] * vc[i];
             tp[j] += s;
         }
-    }
+    });
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (int p = 0; p < P.np; ++p) mul33(&S.Vinv[9 * (size_t)p], &t[3 * (size_t)p], &z[3 * (size_t)p]);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (int c = 0; c < P.nc; ++c) {
         const double* U = &S.U[81 * (size_t)c];
         for (int i = 0; i < 9; ++i) {
@@ -655,12 +717,12 @@ void schur_matvec(const Problem& P, const Schur& S, const double* v, double* out
             out[9 * (size_t)c + i] = s;
         }
     }
-    for (int k = 0; k < P.no; ++k) {
+    for_obs_by(P, true, [&](int k) {
         const double* W = &S.W[27 * (size_t)k];
         const double* zp = &z[3 * (size_t)P.pi[k]];
         double* oc = out + 9 * (size_t)P.ci[k];
         for (int i = 0; i < 9; ++i) oc[i] -= W[3 * i] * zp[0] + W[3 * i + 1] * zp[1] + W[3 * i + 2] * zp[2];
-    }
+    });
 }
 
 double dotv(const std::vector<double>& a, const std::vector<double>& b) {
@@ -677,7 +739,7 @@ int schur_pcg(const Problem& P, const Schur& S, const std::vector<double>& b, in
     const size_t n = 9 * (size_t)nc;
     // M_c = U*_c - sum_{k in c} W_k Vinv_p W_k'
     std::vector<double> M(S.U), Minv(81 * (size_t)nc);
-    for (int k = 0; k < P.no; ++k) {
+    for_obs_by(P, true, [&](int k) {
         const double* W = &S.W[27 * (size_t)k];
         const double* Vi = &S.Vinv[9 * (size_t)P.pi[k]];
         double Y[27];
@@ -686,10 +748,13 @@ int schur_pcg(const Problem& P, const Schur& S, const std::vector<double>& b, in
         double* Mc = &M[81 * (size_t)P.ci[k]];
         for (int i = 0; i < 9; ++i)
             for (int j = 0; j < 9; ++j) Mc[9 * i + j] -= Y[3 * i] * W[3 * j] + Y[3 * i + 1] * W[3 * j + 1] + Y[3 * i + 2] * W[3 * j + 2];
-    }
-    for (int c = 0; c < nc; ++c)
-        if (!spd_inverse(&M[81 * (size_t)c], 9, &Minv[81 * (size_t)c])) return -1;
+    });
+    bool spd = true;
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1) reduction(&& : spd)
+    for (int c = 0; c < nc; ++c) spd = spd_inverse(&M[81 * (size_t)c], 9, &Minv[81 * (size_t)c]) && spd;
+    if (!spd) return -1;
     auto precond = [&](const std::vector<double>& r, std::vector<double>* z) {
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
         for (int c = 0; c < nc; ++c) {
             const double* Mi = &Minv[81 * (size_t)c];
             for (int i = 0; i < 9; ++i) {
@@ -710,6 +775,7 @@ int schur_pcg(const Problem& P, const Schur& S, const std::vector<double>& b, in
     while (it < max_iters) {
         schur_matvec(P, S, p.data(), q.data());
         const double alpha = rz / dotv(p, q);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
         for (size_t i = 0; i < n; ++i) {
             (*x)[i] += alpha * p[i];
             r[i] -= alpha * q[i];
@@ -720,6 +786,7 @@ int schur_pcg(const Problem& P, const Schur& S, const std::vector<double>& b, in
         const double rz_new = dotv(r, z);
         const double beta = rz_new / rz;
         rz = rz_new;
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
         for (size_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
     }
     return it;
@@ -759,7 +826,8 @@ int solve_damped(const Problem& P, const Options& o, const PointObs& po, const L
 
 // model_cost_change = -(J~ s).(r~ + J~ s / 2), s the scaled step.
 double model_cost_change(const Problem& P, const Linearization& L, const std::vector<double>& delta) {
-    double sum = 0.0;
+    std::vector<double> term(2 * (size_t)P.no);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (int k = 0; k < P.no; ++k) {
         const double* jc = &L.jc[18 * (size_t)k];
         const double* jp = &L.jp[6 * (size_t)k];
@@ -769,9 +837,11 @@ double model_cost_change(const Problem& P, const Linearization& L, const std::ve
             double m = 0.0;
             for (int i = 0; i < 9; ++i) m += jc[9 * row + i] * dc[i];
             for (int i = 0; i < 3; ++i) m += jp[3 * row + i] * dp[i];
-            sum += m * (L.r[2 * (size_t)k + row] + 0.5 * m);
+            term[2 * (size_t)k + row] = m * (L.r[2 * (size_t)k + row] + 0.5 * m);
         }
     }
+    double sum = 0.0;
+    for (size_t i = 0; i < term.size(); ++i) sum += term[i];
     return -sum;
 }
 
@@ -846,6 +916,17 @@ void oracle_jacobian_left(const double* cam, const double* pt, double ox, double
 
 // angle-axis of exp(d) exp(w)
 void oracle_so3_left_update(const double* d, const double* w, double* out) { so3_left_update(d, w, out); }
+
+// Threads for oracle_solve (a timing aid; results do not depend on it).  Returns the count in use.
+int oracle_set_threads(int n) {
+#ifdef _OPENMP
+    g_threads = n < 1 ? 1 : n;
+#else
+    (void)n;
+    g_threads = 1;
+#endif
+    return g_threads;
+}
 
 void oracle_se3_left_update(const double* xi, const double* w, const double* t, double* w_out, double* t_out) {
     se3_left_update(xi, w, t, w_out, t_out);
@@ -1052,6 +1133,13 @@ int oracle_solve(int nc, int np, int no, const int* ci, const int* pi, const dou
                  const Options* o, IterRecord* trace, int trace_cap, int* n_trace, double* initial_cost,
                  double* final_cost, int* termination) {
     Problem P{nc, np, no, ci, pi, obs};
+    PointObs cam_lists, pt_lists;
+    if (g_threads > 1) {
+        cam_lists = obs_lists(ci, nc, no);
+        pt_lists = obs_lists(pi, np, no);
+        P.by_cam = &cam_lists;
+        P.by_pt = &pt_lists;
+    }
     if (o->mode == 1)
         return solve_g2o(P, *o, cams, pts, trace, trace_cap, n_trace, initial_cost, final_cost, termination);
     const size_t ncam = 9 * (size_t)nc, nparam = ncam + 3 * (size_t)np;

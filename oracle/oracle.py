@@ -101,6 +101,9 @@ def lib():
         L.oracle_jacobian_left.argtypes = [_f64p, _f64p, ctypes.c_double, ctypes.c_double, _f64p, _f64p, _f64p]
         L.oracle_so3_left_update.argtypes = [_f64p, _f64p, _f64p]
         L.oracle_se3_left_update.argtypes = [_f64p, _f64p, _f64p, _f64p, _f64p]
+        if hasattr(L, "oracle_set_threads"):
+            L.oracle_set_threads.argtypes = [ctypes.c_int]
+            L.oracle_set_threads.restype = ctypes.c_int
         L.oracle_stereo_residual_jacobian.argtypes = [_f64p, _f64p, _f64p, _f64p, ctypes.c_int, ctypes.c_double,
                                                       ctypes.c_double, _f64p, _f64p, _f64p]
         L.oracle_stereo_residual.argtypes = [_f64p, _f64p, _f64p, _f64p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -174,6 +177,12 @@ def so3_left_update(d, w):
     out = np.zeros(3)
     lib().oracle_so3_left_update(_p(d, _f64p), _p(w, _f64p), _p(out, _f64p))
     return out
+
+
+def set_threads(n):
+    """Threads for solve() (a timing aid: results are bitwise the same for any count); returns the
+    count in use (1 when the library was built without OpenMP)."""
+    return lib().oracle_set_threads(int(n))
 
 
 def se3_left_update(xi, w, t):

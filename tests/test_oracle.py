@@ -440,3 +440,23 @@ def test_stereo_noise_free_window_converges(oracle_mod):
     assert res["final_cost"] < 1e-16 * len(w["side"])
     acc = [r["cost"] for r in res["trace"] if r["step_is_successful"]]
     assert all(b <= a * (1 + 1e-15) for a, b in zip(acc, acc[1:]))
+
+
+def test_threaded_oracle_is_bitwise_serial(oracle_mod):
+    """The oracle's threaded loops (the all-cores CPU baseline) split scatter-adds by camera / point
+    and keep each sum in observation order: trajectories are bitwise those of one thread."""
+    import slamplay_amd as S
+    p = S.synthetic(1, seed=31)
+    out = {}
+    for t in (1, 4):
+        oracle_mod.set_threads(t)
+        for ls in (1, 2):
+            o = oracle_mod.default_options(max_num_iterations=5, linear_solver=ls)
+            out[(t, ls)] = oracle_mod.solve(p.num_cameras(), p.num_points(), p.camera_index(), p.point_index(),
+                                            p.observations(), p.mutable_cameras(), p.mutable_points(), o)
+    oracle_mod.set_threads(1)
+    for ls in (1, 2):
+        a, b = out[(1, ls)], out[(4, ls)]
+        np.testing.assert_array_equal(a["cams"], b["cams"])
+        np.testing.assert_array_equal(a["pts"], b["pts"])
+        assert [r["cost"] for r in a["trace"]] == [r["cost"] for r in b["trace"]]

@@ -19,8 +19,10 @@ problems, per-iteration continuous batching, p50/p99 latency and solves/s at sev
 beside the FP64 single-problem solver run over the same mix one request at a time.
 
 The roofline entry is the dominant kernel's algorithmic bytes per launch over its average duration,
-timed with HIP events on the solver's stream; cpu_baseline is the FP64 CPU oracle (one thread) on a
-bounded sample of the same problem.
+timed with HIP events on the solver's stream, beside the spec HBM peak and a measured 1 GiB device
+copy; cpu_baseline is the FP64 CPU oracle (one thread) on a bounded sample of the same problem
+(4 LM iterations), cpu_baseline_all_cores the same on up to 16 host threads.  The device
+Normalize / Perturb of configuration 3 are timed against the host BALProblem methods.
 """
 import argparse
 import json

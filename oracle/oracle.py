@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle_ba.so")
+# SLAMPLAY_ORACLE_LIB selects another build of the same source (e.g. `make -C oracle asan`)
+LIB_PATH = os.environ.get("SLAMPLAY_ORACLE_LIB") or os.path.join(_HERE, "liboracle_ba.so")
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f64p = ctypes.POINTER(ctypes.c_double)

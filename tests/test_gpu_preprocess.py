@@ -119,3 +119,25 @@ def test_device_perturb_rejects_negative_sigma(ladybug_path):
     with pytest.raises(S.BAError, match="negative"):
         sv.perturb(-0.1, 0.5, 0.5)
     sv.close()
+
+
+def test_device_preprocess_small_problems(oracle_mod):
+    """Two points (rank 1 of 2 is the upper median), three points, and cameras only perturbed."""
+    cams = np.array([[0.1, -0.2, 0.05, 0.3, -0.1, -5.0, 500.0, 0.0, 0.0]])
+    for pts in (np.array([[1.0, 2.0, 3.0], [-4.0, 0.5, 7.0]]), np.array([[1.0, 2.0, 3.0], [-4.0, 0.5, 7.0], [2.0, 2.0, -1.0]])):
+        n = len(pts)
+        p = S.BALProblem.from_arrays(np.zeros(n, np.int32), np.arange(n, dtype=np.int32), np.zeros((n, 2)),
+                                     cams.reshape(-1), pts.reshape(-1))
+        c_ref, p_ref = oracle_mod.normalize(1, n, cams.reshape(-1), pts.reshape(-1))
+        sv = S.Solver(p)
+        sv.normalize()
+        gc, gp = sv.download()
+        np.testing.assert_array_equal(gp, p_ref)
+        np.testing.assert_allclose(gc, c_ref, rtol=0, atol=1e-12 * np.abs(c_ref).max())
+        sv.reset()
+        c_ref, p_ref = oracle_mod.perturb(1, n, cams.reshape(-1), pts.reshape(-1), 0.1, 0.5, 0.0, seed=3)
+        sv.perturb(0.1, 0.5, 0.0, seed=3)
+        gc, gp = sv.download()
+        np.testing.assert_array_equal(gp, p_ref)
+        np.testing.assert_allclose(gc, c_ref, rtol=0, atol=1e-12 * np.abs(c_ref).max())
+        sv.close()

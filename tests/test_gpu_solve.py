@@ -309,3 +309,27 @@ def test_stereo_local_ba_mixed_final_cost(oracle_mod):
     out = S.stereo_solve(**_stereo_args(w), precision=1)
     assert abs(out["final_cost"] - ref["final_cost"]) / ref["final_cost"] < 1e-5
 
+
+
+def test_stereo_edge_cases(oracle_mod):
+    """No observations: nothing to optimise, cost 0 and the threshold doubled five times (0/0 inliers
+    is never > 0.5, as in the reference's loop).  A window whose points are seen through the right
+    camera only, and a single keyframe with fixed-baseline pairs, follow the oracle."""
+    K = np.array([458.0, 457.0, 367.0, 248.0])
+    ext = np.zeros((2, 6))
+    ext[1, 3] = -0.5
+    out = S.stereo_solve(np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros((0, 2)),
+                         np.zeros((2, 6)), np.zeros((3, 3)), K, ext)
+    assert out["final_cost"] == 0.0 and out["outlier"].size == 0
+    assert out["chi2_threshold"] == pytest.approx(5.991 * 32)
+    w = S.stereo_window(seed=5, num_keyframes=4, num_points=120, outlier_fraction=0.0)
+    right = w["side"] == 1
+    sub = {k: (w[k][right] if k in ("pose_index", "point_index", "side", "observations") else w[k])
+           for k in ("pose_index", "point_index", "side", "observations", "poses", "points", "K", "extrinsics")}
+    one = S.stereo_window(seed=6, num_keyframes=1, num_points=80, outlier_fraction=0.0)
+    for win in (sub, one):
+        ref = _stereo_oracle(oracle_mod, win)
+        got = S.stereo_solve(**_stereo_args(win), precision=0)
+        assert abs(got["final_cost"] - ref["final_cost"]) <= 1e-9 * max(ref["final_cost"], 1e-30) + 1e-20
+        # no pose is held fixed (as in backend.cpp), so the gauge directions are damped only by lambda
+        np.testing.assert_allclose(got["poses"], ref["poses"], rtol=0, atol=1e-6 * max(np.abs(ref["poses"]).max(), 1))

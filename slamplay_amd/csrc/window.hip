@@ -895,6 +895,11 @@ struct Server final : ServerBase {
             ba_problem_view v = r->view();
             ba_options o = cfg.options;
             o.linear_solver = BA_SOLVER_AUTO;
+            // stream launches, no graph capture: solvers are created, run and destroyed on two
+            // worker threads at once, and a capture open on one thread while another frees device
+            // or pinned memory (calls that synchronize the device) is an interleaving the server
+            // avoids; the results are the same bit for bit
+            o.launch_mode = 1;
             std::unique_ptr<SolverBase> sv;
             ba_summary sm{};
             Done d;

@@ -333,3 +333,27 @@ def test_stereo_edge_cases(oracle_mod):
         assert abs(got["final_cost"] - ref["final_cost"]) <= 1e-9 * max(ref["final_cost"], 1e-30) + 1e-20
         # no pose is held fixed (as in backend.cpp), so the gauge directions are damped only by lambda
         np.testing.assert_allclose(got["poses"], ref["poses"], rtol=0, atol=1e-6 * max(np.abs(ref["poses"]).max(), 1))
+
+
+@pytest.mark.parametrize("mode", ["ceres", "g2o"])
+def test_cpp_drop_in_driver(oracle_mod, ladybug_path, tmp_path, mode):
+    """slamplay_amd/examples/ba_driver (the reference driver's main with SolveBA from
+    include/mba/bal_problem.h, in its own process) gives bit for bit the result of the same pipeline
+    through the Python binding, and writes both PLY snapshots."""
+    import os
+    import subprocess
+    driver = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "slamplay_amd", "examples",
+                          "ba_driver")
+    args = [driver, ladybug_path, str(tmp_path)] + (["g2o"] if mode == "g2o" else [])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    ic, fc, iters, term = r.stdout.split()
+    p = S.BALProblem(ladybug_path)
+    p.Normalize()
+    oracle_mod.lib().oracle_srand(1)  # the driver's process never calls srand: glibc's seed 1
+    p.Perturb(0.1, 0.5, 0.5)
+    kw = dict(mode=1, max_num_iterations=40) if mode == "g2o" else {}
+    out = S.SolveBA(p, **kw)
+    assert float(ic) == out["initial_cost"] and float(fc) == out["final_cost"]
+    assert int(iters) == out["num_iterations"] and int(term) == out["termination"]
+    assert (tmp_path / "initial.ply").stat().st_size > 0 and (tmp_path / "final.ply").stat().st_size > 0

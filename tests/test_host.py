@@ -197,3 +197,19 @@ def test_stereo_inputs_rejected():
         S.stereo_solve(**bad)
     with pytest.raises(C.BAError, match="focal"):
         S.stereo_solve(**dict(args, K=np.array([0.0, 457.0, 367.0, 248.0])))
+
+
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "slamplay_amd", "examples",
+                      "ba_driver")
+
+
+def test_cpp_driver_builds_and_reports_usage(tmp_path):
+    """The reference driver's main linked against the drop-in header and library: it links and
+    starts; bad arguments and an unreadable file fail before any device work."""
+    import subprocess
+    assert os.access(DRIVER, os.X_OK), "build the example with make -C slamplay_amd/csrc"
+    r = subprocess.run([DRIVER], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+    r = subprocess.run([DRIVER, str(tmp_path / "missing.txt"), str(tmp_path)], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "missing.txt" in r.stderr

@@ -390,7 +390,8 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; a multi-GPU figure is not measured "
-              "(independent windows shard by window, a large BA by points -- see DESIGN.md)", file=sys.stderr)
+              "(independent windows shard by window with no exchange; a large BA shards by points with one "
+              "all-reduce of the camera vector per PCG iteration)", file=sys.stderr)
         sys.exit(2)
     import slamplay_amd as S
     line = {
@@ -423,7 +424,7 @@ def main():
         log("bench: CPU oracle baseline (bounded sample), 1 thread")
         cpu = cpu_baseline(S, p)
         log("bench: CPU 1 thread: %.3f LM iters/s" % cpu["value"])
-        n = max(1, min(16, os.cpu_count() or 1))  # a GPU box's CPU share is 16 cores
+        n = max(1, min(16, os.cpu_count() or 1))  # at most 16 host threads
         log("bench: CPU oracle baseline, %d threads" % n)
         cpu_all = cpu_baseline(S, p, threads=n)
         log("bench: CPU %d threads: %.3f LM iters/s" % (cpu_all["cores"], cpu_all["value"]))

@@ -350,7 +350,7 @@ def test_cpp_drop_in_driver(oracle_mod, ladybug_path, tmp_path, mode):
     ic, fc, iters, term = r.stdout.split()
     p = S.BALProblem(ladybug_path)
     p.Normalize()
-    oracle_mod.lib().oracle_srand(1)  # the driver's process never calls srand: glibc's seed 1
+    oracle_mod.lib().oracle_srand(1)  # the ba_driver process never calls srand: glibc's seed 1
     p.Perturb(0.1, 0.5, 0.5)
     kw = dict(mode=1, max_num_iterations=40) if mode == "g2o" else {}
     out = S.SolveBA(p, **kw)

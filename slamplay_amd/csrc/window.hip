@@ -158,8 +158,8 @@ int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowIm
     D.o_yp = take(24 * (int64_t)P);
     D.o_pv = take(96 * (int64_t)slots);
     D.o_j64 = take(8 * kWinJ64 * (int64_t)No);
-    D.o_js = take(kWinJs * sizeof(Real) * (int64_t)No);
-    D.o_S = take(sizeof(Real) * (int64_t)n * n);  // zero: blocks without common points are never written
+    D.o_js = take(kWinJs * sizeof(double) * (int64_t)No);
+    D.o_S = take(sizeof(double) * (int64_t)n * n);  // zero: blocks without common points are never written
     D.o_L = n > max_window_n<Real>() ? take(sizeof(Real) * (int64_t)n * (n + 1) / 2) : 0;
     img->total = off;
     img->bytes.assign(input_bytes, 0);
@@ -408,10 +408,10 @@ struct WindowEngine {
         if (n_ptask) hipLaunchKernelGGL(k_wlin_pt, dim3(n_ptask), dim3(64), 0, s, dd, arena.p, (const int2*)ptask.p,
                                         n_ptask, o);
         hipLaunchKernelGGL((k_wdamp<256>), dim3(n_wtask), dim3(256), 0, s, dd, arena.p, (const int*)wtask.p, n_wtask, o);
-        if (n_ctask) hipLaunchKernelGGL((k_wrec_cam<Real, kWinCamThreads>), dim3(n_ctask), dim3(kWinCamThreads), 0, s,
+        if (n_ctask) hipLaunchKernelGGL((k_wrec_cam<double, kWinCamThreads>), dim3(n_ctask), dim3(kWinCamThreads), 0, s,
                                         dd, arena.p, (const int2*)ctask.p, n_ctask);
         if (n_stask)
-            hipLaunchKernelGGL((k_wschur<Real, NT>), dim3(n_stask), dim3(NT), (size_t)schur_lds<Real>(NT), s, dd, arena.p,
+            hipLaunchKernelGGL((k_wschur<double, NT>), dim3(n_stask), dim3(NT), (size_t)schur_lds<double>(NT), s, dd, arena.p,
                                (const int4*)stask.p, n_stask);
         hipLaunchKernelGGL((k_wsolve<Real, NT>), dim3(n_wtask), dim3(NT), solve_lds(), s,
                            dd, arena.p, (const int*)wtask.p, n_wtask, cg_iterations());
@@ -428,7 +428,7 @@ struct WindowEngine {
 
     template <int NT>
     int set_lds_limits() {
-        BA_TRY(allow_dynamic_lds((const void*)k_wschur<Real, NT>, (int)schur_lds<Real>(NT)));
+        BA_TRY(allow_dynamic_lds((const void*)k_wschur<double, NT>, (int)schur_lds<double>(NT)));
         BA_TRY(allow_dynamic_lds((const void*)k_wsolve<Real, NT>, (int)solve_lds()));
         return BA_OK;
     }
@@ -600,7 +600,7 @@ struct Batch final : BatchBase {
         float best = 1e30f;
         int best_t = 256;
         for (int t : {256, 512}) {
-            if (schur_lds<Real>(t) + 64 > lds_per_block()) continue;
+            if (schur_lds<double>(t) + 64 > lds_per_block()) continue;
             eng.threads = t;
             float ms = 0.f;
             BA_TRY(eng.time_iteration(order, &ms));
@@ -618,7 +618,7 @@ struct Batch final : BatchBase {
         if (t != 256 && t != 512) return fail(BA_ERR_INVALID_ARGUMENT, "threads must be 256 or 512");
         int nmax = 9;
         for (int w : order) nmax = std::max(nmax, 9 * imgs[w].K);
-        if (schur_lds<Real>(t) + 64 > lds_per_block())
+        if (schur_lds<double>(t) + 64 > lds_per_block())
             return fail(BA_ERR_CAPACITY, "the Schur staging of %d threads does not fit the LDS", t);
         (void)nmax;
         eng.threads = t;

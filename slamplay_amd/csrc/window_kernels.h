@@ -501,117 +501,19 @@ __device__ __forceinline__ void win_chol_solve(const F* L, const F* dinv, int n,
     }
 }
 
-// out = S y for the reduced camera system in FP64, applied through the j64 records:
-// S y = D_c^2 y + sum_j J~_c,j'(J~_c,j y - J~_p,j z_p(j)), z_p = V*^-1 sum_{j in p} J~_p,j' J~_c,j y.
-// y and out are LDS vectors of 9K; the point vectors z go through the slot records and w.yp.
-// Waves take cameras; each lane keeps kApplyU observations of its camera in flight: the point
-// and slot indices of the next round are fetched while this round's records are used, so a round
-// costs one memory latency.
-// J~ = J diag(s): the camera scale is folded into y (first pass) and into the sums (last pass).
-constexpr int kApplyU = 3;
-
+// out = S y for the explicit FP64 reduced system of the window (row-major n x n in global memory,
+// L2-resident at these sizes); y and out are LDS vectors.  Waves take rows, lanes columns, rows
+// summed by a butterfly in a fixed order.
 template <int NT>
-__device__ __forceinline__ void win_apply_S(const WinPtrs& w, int K, int P, const double* y, double* out) {
-    constexpr int NW = NT / 64, U = kApplyU;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int No = w.No;
-    for (int c = wave; c < K; c += NW) {
-        double ys[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) ys[i] = w.sc[9 * (size_t)c + i] * y[9 * c + i];
-        const int j0 = w.cam_off[c], j1 = w.cam_off[c + 1];
-        int pn[U], sn[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int j = min(j0 + lane + 64 * u, j1 - 1);  // past the camera's end: repeat its last
-            pn[u] = j1 > j0 ? w.co_pt[j] : 0;
-            sn[u] = j1 > j0 ? w.slot_of[j] : 0;
-        }
-        for (int jb = j0 + lane; jb < j1; jb += 64 * U) {
-            double J[U][24], sp[U][3];
-            int sl[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int j = min(jb + 64 * u, j1 - 1);
-                sl[u] = sn[u];
-#pragma unroll
-                for (int e = 0; e < 24; ++e) J[u][e] = w.j64[(size_t)e * No + j];
-#pragma unroll
-                for (int i = 0; i < 3; ++i) sp[u][i] = w.sp[3 * (size_t)pn[u] + i];
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int j = min(jb + 64 * (U + u), j1 - 1);
-                pn[u] = w.co_pt[j];
-                sn[u] = w.slot_of[j];
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (jb + 64 * u >= j1) break;
-                double t0 = 0.0, t1 = 0.0;
-#pragma unroll
-                for (int i = 0; i < 9; ++i) {
-                    t0 += J[u][i] * ys[i];
-                    t1 += J[u][9 + i] * ys[i];
-                }
-                pv_at(w, 0, sl[u]) = make_double4((J[u][18] * t0 + J[u][21] * t1) * sp[u][0],
-                                                  (J[u][19] * t0 + J[u][22] * t1) * sp[u][1],
-                                                  (J[u][20] * t0 + J[u][23] * t1) * sp[u][2], 0.0);
-            }
-        }
-    }
-    __syncthreads();
-    for (int p = tid; p < P; p += NT) {
-        double t[3], z[3];
-        win_slot_sum3(w, p, t);
-        sym3_mul<double>(w.Vinv + 6 * (size_t)p, t, z);
-        const double* sp = w.sp + 3 * (size_t)p;
-        // stored pre-scaled: J~_p z = J_p (s_p * z)
-        for (int i = 0; i < 3; ++i) w.yp[3 * (size_t)p + i] = z[i] * sp[i];
-    }
-    __syncthreads();
-    for (int c = wave; c < K; c += NW) {
-        double ys[9], acc[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            ys[i] = w.sc[9 * (size_t)c + i] * y[9 * c + i];
-            acc[i] = 0.0;
-        }
-        const int j0 = w.cam_off[c], j1 = w.cam_off[c + 1];
-        int pn[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) pn[u] = j1 > j0 ? w.co_pt[min(j0 + lane + 64 * u, j1 - 1)] : 0;
-        for (int jb = j0 + lane; jb < j1; jb += 64 * U) {
-            double J[U][24], zs[U][3];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int j = min(jb + 64 * u, j1 - 1);
-#pragma unroll
-                for (int e = 0; e < 24; ++e) J[u][e] = w.j64[(size_t)e * No + j];
-#pragma unroll
-                for (int i = 0; i < 3; ++i) zs[u][i] = w.yp[3 * (size_t)pn[u] + i];
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) pn[u] = w.co_pt[min(jb + 64 * (U + u), j1 - 1)];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (jb + 64 * u >= j1) break;
-                double e0 = -(J[u][18] * zs[u][0] + J[u][19] * zs[u][1] + J[u][20] * zs[u][2]);
-                double e1 = -(J[u][21] * zs[u][0] + J[u][22] * zs[u][1] + J[u][23] * zs[u][2]);
-#pragma unroll
-                for (int i = 0; i < 9; ++i) {
-                    e0 += J[u][i] * ys[i];
-                    e1 += J[u][9 + i] * ys[i];
-                }
-#pragma unroll
-                for (int i = 0; i < 9; ++i) acc[i] += J[u][i] * e0 + J[u][9 + i] * e1;
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 9; ++i) acc[i] = wave_sum(acc[i]);
-        if (lane == 0)
-            for (int i = 0; i < 9; ++i)
-                out[9 * c + i] = w.dc2[9 * (size_t)c + i] * y[9 * c + i] + w.sc[9 * (size_t)c + i] * acc[i];
+__device__ __forceinline__ void win_apply_dense(const double* __restrict__ S, int n, const double* y, double* out) {
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int r = wave; r < n; r += NW) {
+        const double* row = S + (size_t)r * n;
+        double s = 0.0;
+        for (int c = lane; c < n; c += 64) s += row[c] * y[c];
+        s = wave_sum(s);
+        if (lane == 0) out[r] = s;
     }
     __syncthreads();
 }
@@ -925,8 +827,8 @@ __global__ void __launch_bounds__(NT) k_wsolve(const WinDesc* __restrict__ descs
     const WinPtrs w = win_ptrs(D, arena);
     WinState* st = w.st;
     if (st->done || st->invalid) return;
-    const int K = w.K, P = w.P, n = 9 * K, tid = threadIdx.x;
-    const F* Sg = at<F>(arena, D.o_S);
+    const int K = w.K, n = 9 * K, tid = threadIdx.x;
+    const double* Sg = at<double>(arena, D.o_S);
     // the packed factor lives in LDS, or for reduced systems larger than the LDS holds in the
     // window's own global buffer (L2-resident at these sizes)
     F* S = D.o_L ? at<F>(arena, D.o_L) : reinterpret_cast<F*>(smem);
@@ -955,7 +857,7 @@ __global__ void __launch_bounds__(NT) k_wsolve(const WinDesc* __restrict__ descs
                 while (tri(i + 1) <= idx) ++i;
                 while (tri(i) > idx) --i;
                 ix[u] = idx;
-                v[u] = Sg[(size_t)i * n + (idx - tri(i))];
+                v[u] = (F)Sg[(size_t)i * n + (idx - tri(i))];
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u)
@@ -979,7 +881,7 @@ __global__ void __launch_bounds__(NT) k_wsolve(const WinDesc* __restrict__ descs
         for (int k = tid; k < n; k += NT) zv[k] = w.bc[k];
         __syncthreads();
         const double bb = win_dot(zv, zv, n, &sdot);
-        win_apply_S<NT>(w, K, P, xv, qv);
+        win_apply_dense<NT>(Sg, n, xv, qv);
         for (int k = tid; k < n; k += NT) rv[k] = zv[k] - qv[k];
         __syncthreads();
         double rr = win_dot(rv, rv, n, &sdot);
@@ -991,7 +893,7 @@ __global__ void __launch_bounds__(NT) k_wsolve(const WinDesc* __restrict__ descs
             __syncthreads();
             double rz = win_dot(rv, zv, n, &sdot);
             for (int cg = 0; cg < cg_max; ++cg) {
-                win_apply_S<NT>(w, K, P, dv, qv);
+                win_apply_dense<NT>(Sg, n, dv, qv);
                 const double dq = win_dot(dv, qv, n, &sdot);
                 if (!(dq > 0.0)) break;
                 const double alpha = rz / dq;

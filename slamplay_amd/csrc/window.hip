@@ -107,7 +107,12 @@ int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowIm
     build_pair_blocks(L, &pb);
     const int K = L.nc, P = L.np, No = L.no, nch = L.n_chunks;
     const int slots = L.chunk_off[nch];
-    const int nb = (int)pb.a.size(), npairs = (int)pb.i.size();
+    // the diagonal blocks (an observation with itself) are formed by k_wrec_cam; the pair kernel
+    // takes the blocks of two different cameras
+    std::vector<int> offdiag;
+    for (int i = 0; i < (int)pb.a.size(); ++i)
+        if (pb.a[i] != pb.b[i]) offdiag.push_back(i);
+    const int nb = (int)offdiag.size(), npairs = (int)pb.i.size();
     const int n = 9 * K;
     WinDesc& D = img->desc;
     D = WinDesc{};
@@ -171,8 +176,7 @@ int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowIm
     std::memcpy(b + D.o_chunk_len, L.chunk_len.data(), 4 * (size_t)nch);
     std::memcpy(b + D.o_chunk_off, L.chunk_off.data(), 4 * (size_t)(nch + 1));
     // blocks with the most pairs first: the workgroup's waves take them in this order
-    std::vector<int> border(nb);
-    std::iota(border.begin(), border.end(), 0);
+    std::vector<int> border(offdiag);
     std::stable_sort(border.begin(), border.end(),
                      [&](int x, int y) { return pb.off[x + 1] - pb.off[x] > pb.off[y + 1] - pb.off[y]; });
     int* blk = reinterpret_cast<int*>(b + D.o_blk);

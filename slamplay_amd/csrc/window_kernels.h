@@ -739,7 +739,9 @@ __global__ void __launch_bounds__(NT) k_wdamp(const WinDesc* __restrict__ descs,
 template <typename Real, int NT>
 __global__ void __launch_bounds__(NT) k_wrec_cam(const WinDesc* __restrict__ descs, char* __restrict__ arena,
                                                  const int2* __restrict__ ctask, int ntask) {
-    __shared__ double lds[9 * (NT / 64)];
+    constexpr int NW = NT / 64;
+    __shared__ double lds[9 * NW];
+    __shared__ double dacc[NW][45];  // per wave: upper triangle of sum J~_c'(I - G)J~_c
     const int t = blockIdx.x;
     if (t >= ntask) return;
     const int2 tk = ctask[t];
@@ -748,45 +750,93 @@ __global__ void __launch_bounds__(NT) k_wrec_cam(const WinDesc* __restrict__ des
     const WinState* st = w.st;
     if (st->done || st->invalid || tk.y >= w.K) return;
     Real* js = at<Real>(arena, D.o_js);
-    const int c = tk.y;
+    const int c = tk.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double scc[9];
     for (int i = 0; i < 9; ++i) scc[i] = w.sc[9 * (size_t)c + i];
     double bacc[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) bacc[i] = 0.0;
-    for (int j = w.cam_off[c] + threadIdx.x; j < w.cam_off[c + 1]; j += NT) {
-        const int p = w.co_pt[j];
-        double Jc[18], Jp[6];
-        win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
-        const double* Vi = w.Vinv + 6 * (size_t)p;
-        const double j0[3] = {Jp[0], Jp[1], Jp[2]}, j1[3] = {Jp[3], Jp[4], Jp[5]};
-        double h0[3], h1[3];
-        sym3_mul<double>(Vi, j0, h0);
-        sym3_mul<double>(Vi, j1, h1);
-        Real* out = js + kWinJs * (size_t)j;
+    for (int k = threadIdx.x; k < NW * 45; k += NT) (&dacc[0][0])[k] = 0.0;
+    __syncthreads();
+    // rounds of NT observations, every lane taking part in the per-round butterflies
+    const int j_end = w.cam_off[c + 1];
+    for (int j0r = w.cam_off[c]; j0r < j_end; j0r += NT) {
+        const int j = j0r + threadIdx.x;
+        double Jd[18], g00 = 0.0, g01 = 0.0, g11 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 18; ++i) Jd[i] = 0.0;
+        if (j < j_end) {
+            const int p = w.co_pt[j];
+            double Jc[18], Jp[6];
+            win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
+            const double* Vi = w.Vinv + 6 * (size_t)p;
+            const double j0[3] = {Jp[0], Jp[1], Jp[2]}, j1[3] = {Jp[3], Jp[4], Jp[5]};
+            double h0[3], h1[3];
+            sym3_mul<double>(Vi, j0, h0);
+            sym3_mul<double>(Vi, j1, h1);
+            Real* out = js + kWinJs * (size_t)j;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                out[2 * i] = (Real)Jc[i];
+                out[2 * i + 1] = (Real)Jc[9 + i];
+            }
+#pragma unroll
+            for (int i = 0; i < 6; ++i) out[18 + i] = (Real)Jp[i];
+            for (int i = 0; i < 3; ++i) {
+                out[24 + i] = (Real)h0[i];
+                out[27 + i] = (Real)h1[i];
+            }
+            g00 = 1.0 - (j0[0] * h0[0] + j0[1] * h0[1] + j0[2] * h0[2]);
+            g01 = -(j0[0] * h1[0] + j0[1] * h1[1] + j0[2] * h1[2]);
+            g11 = 1.0 - (j1[0] * h1[0] + j1[1] * h1[1] + j1[2] * h1[2]);
+            out[30] = (Real)g00;
+            out[31] = (Real)g01;
+            out[32] = (Real)g11;
+            const double4 q = w.qp[p];
+            const double e0 = j64_at(w, 24, j) - (Jp[0] * q.x + Jp[1] * q.y + Jp[2] * q.z);
+            const double e1 = j64_at(w, 25, j) - (Jp[3] * q.x + Jp[4] * q.y + Jp[5] * q.z);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) bacc[i] += Jc[i] * e0 + Jc[9 + i] * e1;
+#pragma unroll
+            for (int i = 0; i < 18; ++i) Jd[i] = Jc[i];
+        }
+        // the observation's own term of the camera's diagonal block: J~_c'(I - G)J~_c
+        double t0[9], t1[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
-            out[2 * i] = (Real)Jc[i];
-            out[2 * i + 1] = (Real)Jc[9 + i];
+            t0[i] = g00 * Jd[i] + g01 * Jd[9 + i];
+            t1[i] = g01 * Jd[i] + g11 * Jd[9 + i];
         }
+        int e = 0;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) out[18 + i] = (Real)Jp[i];
-        for (int i = 0; i < 3; ++i) {
-            out[24 + i] = (Real)h0[i];
-            out[27 + i] = (Real)h1[i];
-        }
-        out[30] = (Real)(1.0 - (j0[0] * h0[0] + j0[1] * h0[1] + j0[2] * h0[2]));
-        out[31] = (Real)(-(j0[0] * h1[0] + j0[1] * h1[1] + j0[2] * h1[2]));
-        out[32] = (Real)(1.0 - (j1[0] * h1[0] + j1[1] * h1[1] + j1[2] * h1[2]));
-        const double4 q = w.qp[p];
-        const double e0 = j64_at(w, 24, j) - (Jp[0] * q.x + Jp[1] * q.y + Jp[2] * q.z);
-        const double e1 = j64_at(w, 25, j) - (Jp[3] * q.x + Jp[4] * q.y + Jp[5] * q.z);
+        for (int r = 0; r < 9; ++r)
 #pragma unroll
-        for (int i = 0; i < 9; ++i) bacc[i] += Jc[i] * e0 + Jc[9 + i] * e1;
+            for (int cc = r; cc < 9; ++cc, ++e) {
+                const double v = wave_sum(Jd[r] * t0[cc] + Jd[9 + r] * t1[cc]);
+                if (lane == 0) dacc[wave][e] += v;
+            }
     }
     wg_sum_fixed<9, NT>(bacc, lds);
     if (threadIdx.x == 0)
         for (int i = 0; i < 9; ++i) w.bc[9 * (size_t)c + i] = bacc[i];
+    // the diagonal block of S: D_c^2 + sum over the camera's observations (the waves in order);
+    // the pair kernel writes the off-diagonal blocks only
+    if (threadIdx.x < 45) {
+        const int k = threadIdx.x;
+        int r = 0, rem = k;
+        while (rem >= 9 - r) {
+            rem -= 9 - r;
+            ++r;
+        }
+        const int cc = r + rem;
+        double v = dacc[0][k];
+        for (int q = 1; q < NW; ++q) v += dacc[q][k];
+        if (r == cc) v += w.dc2[9 * (size_t)c + r];
+        double* S = at<double>(arena, D.o_S);
+        const int n = 9 * w.K;
+        S[(size_t)(9 * c + r) * n + 9 * c + cc] = v;
+        S[(size_t)(9 * c + cc) * n + 9 * c + r] = v;
+    }
 }
 
 // Per (window, block group): blocks first + stride t of the window's list.

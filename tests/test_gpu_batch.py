@@ -186,3 +186,15 @@ def test_simulate_poisson():
     assert (lat > 0).all() and np.isfinite(lat).all()
     assert all(r["termination"] in (0, 1) for r in res)
     assert all(r["final_cost"] < r["initial_cost"] for r in res)
+
+
+def test_mixed_windows_38_final_cost(oracle_mod):
+    """38 generated windows (5-20 keyframes) in mixed precision: final costs within 1e-6 of the FP64
+    oracle after 10 LM iterations (the FP32 factor preconditions CG against the FP64 system)."""
+    wins = S.local_ba_windows(38, seed=11)
+    sums = S.Batch(wins, precision=1, max_num_iterations=10, **FIXED).run()
+    worst = 0.0
+    for p, s in zip(wins, sums):
+        ref = _oracle(oracle_mod, p, max_num_iterations=10, **FIXED)
+        worst = max(worst, abs(s["final_cost"] - ref["final_cost"]) / ref["final_cost"])
+    assert worst < 1e-6, worst

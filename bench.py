@@ -320,6 +320,71 @@ def bench_preprocess(S, p, reps=5):
     return out
 
 
+def bench_stereo_and_g2o(S, reps=10):
+    """The two driver variants besides the Ceres one, timed end to end per call (upload, layout,
+    solve, download; wall clock) with the CPU oracle on the same input beside them:
+    full_slam's stereo local BA on a synthetic window of 8 keyframes / 400 points (g2o Levenberg,
+    10 iterations, Huber 5.991, outlier rule), and the g2o driver's SolveBA (40 iterations) on
+    ladybug49 after Normalize and Perturb(0.1, 0.5, 0.5)."""
+    import gzip
+    import shutil
+    import tempfile
+    from oracle import oracle as O
+    out = {}
+    w = S.stereo_window(seed=1)
+    args = {k: w[k] for k in ("pose_index", "point_index", "side", "observations", "poses", "points", "K",
+                              "extrinsics")}
+    S.stereo_solve(**args)
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = S.stereo_solve(**args)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    o = O.default_options(mode=1, max_num_iterations=10, loss_scale=5.991, linear_solver=1)
+    t0 = time.perf_counter()
+    O.solve_stereo(args["pose_index"], args["point_index"], args["side"], args["observations"], args["poses"],
+                   args["points"], args["K"], args["extrinsics"], o)
+    cpu = time.perf_counter() - t0
+    out["stereo_window"] = {"keyframes": int(w["poses"].shape[0]), "points": int(w["points"].shape[0]),
+                            "observations": int(len(w["side"])), "gpu_ms_per_solve": best * 1e3,
+                            "cpu_oracle_ms_per_solve": cpu * 1e3, "iterations": r["num_iterations"],
+                            "outliers_flagged": int(r["outlier"].sum())}
+    gz = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "ladybug49.txt.gz")
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "ladybug49.txt")
+        with gzip.open(gz, "rb") as src, open(path, "wb") as dst:
+            shutil.copyfileobj(src, dst)
+        base = S.BALProblem(path)
+        base.Normalize()
+        base.Perturb(0.1, 0.5, 0.5)
+    cams, pts = base.mutable_cameras().copy(), base.mutable_points().copy()
+
+    def fresh():
+        return S.BALProblem.from_arrays(base.camera_index(), base.point_index(), base.observations(), cams, pts)
+
+    S.SolveBA(fresh(), mode=1, max_num_iterations=40)
+    best = None
+    for _ in range(max(2, reps // 3)):
+        q = fresh()
+        t0 = time.perf_counter()
+        r = S.SolveBA(q, mode=1, max_num_iterations=40)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    o = O.default_options(mode=1, max_num_iterations=40)
+    t0 = time.perf_counter()
+    ref = O.solve(base.num_cameras(), base.num_points(), base.camera_index(), base.point_index(), base.observations(),
+                  cams, pts, o)
+    cpu = time.perf_counter() - t0
+    out["g2o_mode_ladybug49"] = {"gpu_ms_per_solve": best * 1e3, "cpu_oracle_ms_per_solve": cpu * 1e3,
+                                 "trials": r["num_iterations"], "final_chi2": r["final_cost"],
+                                 "final_chi2_oracle": ref["final_cost"]}
+    log("bench: stereo window %.2f ms (CPU oracle %.1f ms); g2o-mode ladybug49 %.1f ms (CPU oracle %.0f ms)" %
+        (out["stereo_window"]["gpu_ms_per_solve"], out["stereo_window"]["cpu_oracle_ms_per_solve"],
+         out["g2o_mode_ladybug49"]["gpu_ms_per_solve"], out["g2o_mode_ladybug49"]["cpu_oracle_ms_per_solve"]))
+    return out
+
+
 def measure_copy_gbs(nbytes=1 << 30, reps=10):
     """Measured HBM denominator: a device-to-device copy of 1 GiB (torch's elementwise copy kernel),
     best of `reps` by HIP events; (read + write bytes) / time."""
@@ -419,6 +484,10 @@ def main():
     if p is not None:
         log("bench: device Normalize / Perturb (configuration 3)")
         prep = bench_preprocess(S, p)
+    variants = None
+    if not args.no_cpu_baseline:
+        log("bench: stereo local BA window and g2o-mode SolveBA")
+        variants = bench_stereo_and_g2o(S)
     cpu = cpu_all = None
     if not args.no_cpu_baseline and p is not None:
         log("bench: CPU oracle baseline (bounded sample), 1 thread")
@@ -446,7 +515,7 @@ def main():
             log("bench: device copy %.0f GB/s (spec %.0f)" % (copy_gbs, HBM_PEAK_GBS))
     line.update(roofline=line.get("roofline"), cpu_baseline=cpu, cpu_baseline_all_cores=cpu_all,
                 batched_local_ba=batch, serving=serving,
-                dense_config2=dense, preprocess=prep)
+                dense_config2=dense, preprocess=prep, driver_variants=variants)
     if sm is not None:
         sm.close()
     print(json.dumps(line), flush=True)

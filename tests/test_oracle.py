@@ -235,6 +235,94 @@ def test_perturb_rotation_keeps_centers(oracle_mod, ladybug_path):
     assert 0.2 < d.std() < 0.8
 
 
+def test_ceres_trust_region_rule_from_trace(oracle_mod):
+    """The Ceres LevenbergMarquardtStrategy / TrustRegionMinimizer rules, recomputed in numpy from
+    the oracle's own trace: rho = cost_change / model_cost_change; a step is accepted iff
+    rho > min_relative_decrease (1e-3); on acceptance radius <- min(max_radius,
+    radius / max(1/3, 1 - (2 rho - 1)^3)) and the decrease factor resets to 2; on rejection
+    radius <- radius / decrease_factor and the factor doubles (2, 4, 8, ... across consecutive
+    rejections).  A wrong exponent, a missing clamp, a factor that does not double or does not
+    reset each changes some recorded radius."""
+    checked_fail_runs = 0
+    for seed in (31, 33, 35):
+        cams, pts, ci, pi, rng = tiny_problem(seed=seed, nc=4, npts=30)
+        obs = _project(oracle_mod, cams, pts, ci, pi) + rng.normal(0, 0.5, (len(ci), 2))
+        cams_p = cams + rng.normal(0, 0.02, cams.shape) * np.array([1, 1, 1, 1, 1, 1, 0, 0, 0])
+        pts_p = pts + rng.normal(0, 0.3, pts.shape)
+        o = oracle_mod.default_options(max_num_iterations=40, function_tolerance=0.0, gradient_tolerance=0.0,
+                                       parameter_tolerance=0.0)
+        tr = oracle_mod.solve(4, 30, ci, pi, obs, cams_p, pts_p, o)["trace"]
+        assert tr[0]["trust_region_radius"] == 1e4
+        radius, factor, x_cost, run = 1e4, 2.0, tr[0]["cost"], 0
+        for row in tr[1:]:
+            assert row["step_is_valid"]
+            rho = row["cost_change"] / row["model_cost_change"]
+            assert row["relative_decrease"] == pytest.approx(rho, rel=1e-14)
+            # the row's cost is the candidate's (re-evaluated at linearization when accepted)
+            assert abs(row["cost_change"] - (x_cost - row["cost"])) <= 1e-12 * x_cost
+            assert bool(row["step_is_successful"]) == (rho > 1e-3), (seed, row)
+            if row["step_is_successful"]:
+                radius = min(1e16, radius / max(1.0 / 3.0, 1.0 - (2.0 * rho - 1.0) ** 3))
+                factor = 2.0
+                x_cost = row["cost"]
+                run = 0
+            else:
+                radius = radius / factor
+                factor *= 2.0
+                run += 1
+                checked_fail_runs = max(checked_fail_runs, run)
+            assert row["trust_region_radius"] == pytest.approx(radius, rel=1e-14), (seed, row["iteration"])
+    assert checked_fail_runs >= 3  # the doubling was exercised over several consecutive rejections
+
+
+def test_perturb_matches_numpy_polar_method_on_libc_rand(oracle_mod, ladybug_path):
+    """An independent restatement of BALProblem::Perturb (BALProblem.cpp:39-69,305-336): numpy's
+    polar method on glibc rand() called through ctypes (x1 * w returned, x2 discarded), every point
+    first (3 normals each), then per camera 3 rotation normals with the centre held
+    (scipy rotations for c = -R't and t = -R c) and 3 translation normals.  Swapping the point and
+    camera loops, returning x2 w, or perturbing t before re-deriving it from the centre all fail."""
+    import ctypes
+    from scipy.spatial.transform import Rotation
+    import slamplay_amd as S
+    libc = ctypes.CDLL("libc.so.6")
+    libc.rand.restype = ctypes.c_int
+    libc.srand.argtypes = [ctypes.c_uint]
+    rand_max = 2147483647.0  # glibc RAND_MAX
+
+    def rand_normal():
+        while True:
+            x1 = 2.0 * (libc.rand() / rand_max) - 1.0
+            x2 = 2.0 * (libc.rand() / rand_max) - 1.0
+            w = x1 * x1 + x2 * x2
+            if not (w >= 1.0 or w == 0.0):
+                break
+        return x1 * np.sqrt((-2.0 * np.log(w)) / w)
+
+    p = S.BALProblem(ladybug_path)
+    nc, npt = p.num_cameras(), p.num_points()
+    cams0, pts0 = p.mutable_cameras().reshape(nc, 9).copy(), p.mutable_points().reshape(npt, 3).copy()
+    # keep the pure-Python loop short: the first 300 points and all 49 cameras
+    npt_s = 300
+    cams_o, pts_o = oracle_mod.perturb(nc, npt_s, cams0, pts0[:npt_s], 0.01, 0.1, 0.1, seed=4321)
+    libc.srand(4321)
+    pts_n = pts0[:npt_s].copy()
+    for i in range(npt_s):
+        for k in range(3):
+            pts_n[i, k] += rand_normal() * 0.1
+    cams_n = cams0.copy()
+    for c in range(nc):
+        R = Rotation.from_rotvec(cams_n[c, :3]).as_matrix()
+        center = -R.T @ cams_n[c, 3:6]
+        for k in range(3):
+            cams_n[c, k] += rand_normal() * 0.01
+        cams_n[c, 3:6] = -Rotation.from_rotvec(cams_n[c, :3]).as_matrix() @ center
+        for k in range(3):
+            cams_n[c, 3 + k] += rand_normal() * 0.1
+    np.testing.assert_allclose(pts_o.reshape(npt_s, 3), pts_n, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(cams_o.reshape(nc, 9), cams_n, rtol=1e-12, atol=1e-12)
+    assert np.abs(pts_n - pts0[:npt_s]).max() > 0.05  # the normals did move the points
+
+
 # ---------------------------------------------------------------------------------------------
 # g2o driver mode (test_bundle_adjustment_g2o.cpp): left-multiplied rotation update, lambda-LM.
 

@@ -38,7 +38,8 @@ typedef enum ba_status {
     BA_ERR_NO_DEVICE = 5,
     BA_ERR_CAPACITY = 6,
     BA_ERR_IO = 7,
-    BA_ERR_NOT_FOUND = 8
+    BA_ERR_NOT_FOUND = 8,
+    BA_ERR_TIMEOUT = 9 /* the serving loop made no progress within its deadline (state in the message) */
 } ba_status;
 
 typedef enum ba_precision {

@@ -17,7 +17,8 @@ vp = ctypes.c_void_p
 
 BA_OK = 0
 STATUS_NAMES = {0: "BA_OK", 1: "BA_ERR_INVALID_ARGUMENT", 2: "BA_ERR_OUT_OF_MEMORY", 3: "BA_ERR_NON_FINITE",
-                4: "BA_ERR_HIP", 5: "BA_ERR_NO_DEVICE", 6: "BA_ERR_CAPACITY", 7: "BA_ERR_IO", 8: "BA_ERR_NOT_FOUND"}
+                4: "BA_ERR_HIP", 5: "BA_ERR_NO_DEVICE", 6: "BA_ERR_CAPACITY", 7: "BA_ERR_IO", 8: "BA_ERR_NOT_FOUND",
+                9: "BA_ERR_TIMEOUT"}
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 SOLVER_AUTO, SOLVER_DENSE_SCHUR, SOLVER_ITERATIVE_SCHUR = 0, 1, 2
 LOSS_NONE, LOSS_HUBER = 0, 1

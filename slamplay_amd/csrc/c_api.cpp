@@ -1,11 +1,13 @@
 // c_api.cpp -- error reporting, defaults and argument validation of the C ABI.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <utility>
 
 #include "common.h"
@@ -35,6 +37,27 @@ int use_device(int device) {
     if (device < 0 || device >= n) return fail(BA_ERR_NO_DEVICE, "device %d out of range (%d devices)", device, n);
     BA_HIP(hipSetDevice(device));
     return BA_OK;
+}
+
+int wait_stream(hipStream_t s, hipEvent_t ev, double timeout_s, const char* what) {
+    if (!(timeout_s > 0.0) || !ev) {
+        BA_HIP(hipStreamSynchronize(s));
+        return BA_OK;
+    }
+    BA_HIP(hipEventRecord(ev, s));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int polls = 0;; ++polls) {
+        const hipError_t e = hipEventQuery(ev);
+        if (e == hipSuccess) return BA_OK;
+        if (e != hipErrorNotReady)
+            return fail(e == hipErrorOutOfMemory ? BA_ERR_OUT_OF_MEMORY : BA_ERR_HIP, "%s: %s", what,
+                        hipGetErrorString(e));
+        const double waited = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (waited > timeout_s)
+            return fail(BA_ERR_TIMEOUT, "%s: device work not finished after %.1f s", what, waited);
+        // spin briefly (most waits are short), then yield the core
+        if (polls > 64) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
 }
 
 int allow_dynamic_lds(const void* kernel, int bytes) {

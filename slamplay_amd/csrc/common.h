@@ -78,4 +78,9 @@ int use_device(int device);
 // created or stepped on several threads do not call into the runtime for it concurrently.
 int allow_dynamic_lds(const void* kernel, int bytes);
 
+// Waits until the work enqueued on `s` so far has finished.  timeout_s <= 0: hipStreamSynchronize.
+// Otherwise `ev` is recorded on the stream and polled; after timeout_s seconds without completion
+// the wait returns BA_ERR_TIMEOUT naming `what`, instead of blocking the caller for good.
+int wait_stream(hipStream_t s, hipEvent_t ev, double timeout_s, const char* what);
+
 }  // namespace mba

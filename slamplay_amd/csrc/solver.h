@@ -44,6 +44,8 @@ struct SolverBase {
     virtual int kernel_times(ba_kernel_time* out, int cap, int* count) = 0;
     virtual int64_t device_bytes() const = 0;
     virtual int solver_kind() const = 0;
+    // > 0: run()'s waits for the device give up with BA_ERR_TIMEOUT after this many seconds.
+    virtual void set_wait_timeout(double seconds) = 0;
 };
 
 // The full_slam stereo model (stereo.h): rig and per-observation camera side (caller order).

@@ -225,6 +225,8 @@ struct Solver final : SolverBase {
     DevBuf<int> pt_orig;  // internal point -> caller point, uploaded on the first Normalize / Perturb
     PrepScratch prep_w;
     LMState* st_host = nullptr;  // pinned
+    double wait_timeout_s = 0.0;
+    hipEvent_t ev_wait = nullptr;
     DenseSchur<Real> dense;
     // one LM iteration captured once and replayed (launch_mode 0)
     hipGraphExec_t iter_graph = nullptr;
@@ -242,6 +244,7 @@ struct Solver final : SolverBase {
     ~Solver() override {
         if (iter_graph) (void)hipGraphExecDestroy(iter_graph);
         if (st_host) (void)hipHostFree(st_host);
+        if (ev_wait) (void)hipEventDestroy(ev_wait);
         for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
         if (s) (void)hipStreamDestroy(s);
     }
@@ -259,6 +262,11 @@ struct Solver final : SolverBase {
         return b;
     }
     int solver_kind() const override { return linear_solver; }
+    void set_wait_timeout(double seconds) override { wait_timeout_s = seconds; }
+    int sync(const char* what) {
+        if (wait_timeout_s > 0.0 && !ev_wait) BA_HIP(hipEventCreateWithFlags(&ev_wait, hipEventDisableTiming));
+        return wait_stream(s, ev_wait, wait_timeout_s, what);
+    }
 
     int create(const ba_problem_view* v, const ba_options& o, const StereoInput* stereo = nullptr) {
         opt = o;
@@ -560,12 +568,12 @@ struct Solver final : SolverBase {
             BA_HIP(hipGetLastError());
             if ((it + 1) % check_every == 0 || it + 1 == max_trials) {
                 BA_HIP(hipMemcpyAsync(st_host, st.p, sizeof(LMState), hipMemcpyDeviceToHost, s));
-                BA_HIP(hipStreamSynchronize(s));
+                BA_TRY(sync("LM state"));
                 if (st_host->done) break;
             }
         }
         BA_HIP(hipMemcpyAsync(st_host, st.p, sizeof(LMState), hipMemcpyDeviceToHost, s));
-        BA_HIP(hipStreamSynchronize(s));
+        BA_TRY(sync("LM state"));
         const auto t1 = std::chrono::steady_clock::now();
         if (profiling) collect_profile();
         if (summary) BA_TRY(fill_summary(summary, std::chrono::duration<double, std::milli>(t1 - t0).count()));
@@ -587,7 +595,10 @@ struct Solver final : SolverBase {
         const LMState& S = *st_host;
         const int n = std::min(S.trace_len, d.trace_cap);
         std::vector<ba_iteration> rows(n);
-        if (n) BA_HIP(hipMemcpy(rows.data(), trace.p, n * sizeof(ba_iteration), hipMemcpyDeviceToHost));
+        if (n) {
+            BA_HIP(hipMemcpyAsync(rows.data(), trace.p, n * sizeof(ba_iteration), hipMemcpyDeviceToHost, s));
+            BA_TRY(sync("LM trace"));
+        }
         if (opt.mode == BA_MODE_G2O)
             for (ba_iteration& r : rows) {  // g2o reports chi2 = sum rho(s), without the 1/2
                 r.cost *= 2.0;

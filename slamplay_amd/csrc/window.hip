@@ -18,6 +18,7 @@
 #include <mutex>
 #include <numeric>
 #include <random>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -107,11 +108,17 @@ int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowIm
     build_pair_blocks(L, &pb);
     const int K = L.nc, P = L.np, No = L.no, nch = L.n_chunks;
     const int slots = L.chunk_off[nch];
-    // the diagonal blocks (an observation with itself) are formed by k_wrec_cam; the pair kernel
-    // takes the blocks of two different cameras
+    // a diagonal block whose pairs are all an observation with itself is formed by k_wrec_cam; the
+    // pair kernel takes the blocks of two different cameras, and the diagonal blocks of a camera
+    // that observes one point more than once (their cross pairs x != y carry the terms
+    // -J~c,x' J~p,x V*^-1 J~p,y' J~c,y that k_wrec_cam does not form; it then rewrites the whole
+    // block after k_wrec_cam)
     std::vector<int> offdiag;
-    for (int i = 0; i < (int)pb.a.size(); ++i)
-        if (pb.a[i] != pb.b[i]) offdiag.push_back(i);
+    for (int i = 0; i < (int)pb.a.size(); ++i) {
+        bool take = pb.a[i] != pb.b[i];
+        for (int q = pb.off[i]; !take && q < pb.off[i + 1]; ++q) take = pb.i[q] != pb.j[q];
+        if (take) offdiag.push_back(i);
+    }
     const int nb = (int)offdiag.size(), npairs = (int)pb.i.size();
     const int n = 9 * K;
     WinDesc& D = img->desc;
@@ -281,6 +288,8 @@ struct WindowEngine {
     DevBuf<int> wtask;
     int n_ctask = 0, n_ptask = 0, n_stask = 0, n_wtask = 0, work_nmax = 9, work_threads = 0;
     hipGraphExec_t graph = nullptr;  // one LM iteration of the current work set
+    double wait_timeout_s = 0.0;     // > 0: device waits give up with BA_ERR_TIMEOUT after this long
+    hipEvent_t ev_wait = nullptr;
     DevBuf<unsigned long long> dclock;  // device wall clock at the start of the last run
     double clock_khz = 100000.0;        // rate of the device wall clock
 
@@ -288,8 +297,12 @@ struct WindowEngine {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (ev_wait) (void)hipEventDestroy(ev_wait);
         if (s) (void)hipStreamDestroy(s);
     }
+
+    // Waits for the engine stream (bounded when wait_timeout_s > 0).
+    int sync(const char* what) { return wait_stream(s, ev_wait, wait_timeout_s, what); }
 
     int init(const ba_options& options, int64_t arena_bytes, int slots) {
         opt = options;
@@ -298,6 +311,7 @@ struct WindowEngine {
         BA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         BA_HIP(hipEventCreate(&ev0));
         BA_HIP(hipEventCreate(&ev1));
+        BA_HIP(hipEventCreateWithFlags(&ev_wait, hipEventDisableTiming));
         arena_cap = std::max<int64_t>(arena_bytes, 256);
         BA_TRY(arena.alloc((size_t)arena_cap));
         max_slots = std::max(slots, 1);
@@ -316,7 +330,8 @@ struct WindowEngine {
     // Milliseconds from the start of the last run to the window's finish (0 if it has not finished).
     int finish_times(const std::vector<WinState>& st, std::vector<double>* ms) {
         unsigned long long t0 = 0;
-        BA_HIP(hipMemcpy(&t0, dclock.p, sizeof(t0), hipMemcpyDeviceToHost));
+        BA_HIP(hipMemcpyAsync(&t0, dclock.p, sizeof(t0), hipMemcpyDeviceToHost, s));
+        BA_TRY(sync("window clock"));
         ms->assign(st.size(), 0.0);
         for (size_t i = 0; i < st.size(); ++i)
             if (st[i].done && st[i].t_done >= t0) (*ms)[i] = (double)(st[i].t_done - t0) / clock_khz;
@@ -394,7 +409,7 @@ struct WindowEngine {
         BA_TRY(upload(ptask, pt));
         BA_TRY(upload(stask, stk));
         BA_TRY(upload(wtask, slots));
-        BA_HIP(hipStreamSynchronize(s));
+        BA_TRY(sync("window task lists"));
         n_ctask = (int)ct.size();
         n_ptask = (int)pt.size();
         n_stask = (int)stk.size();
@@ -493,7 +508,7 @@ struct WindowEngine {
                            (const WinDesc*)ddesc.p, (const char*)arena.p, (const int*)dwork.p, (int)slots.size(),
                            dstates.p);
         BA_HIP(hipMemcpyAsync(out->data(), dstates.p, slots.size() * sizeof(WinState), hipMemcpyDeviceToHost, s));
-        BA_HIP(hipStreamSynchronize(s));
+        BA_TRY(sync("window states"));
         return BA_OK;
     }
 
@@ -502,7 +517,7 @@ struct WindowEngine {
         std::vector<double> px(4 * (size_t)D.P);
         if (cams && D.K) BA_HIP(hipMemcpyAsync(cams, arena.p + D.o_cam_x, 72 * (size_t)D.K, hipMemcpyDeviceToHost, s));
         if (D.P) BA_HIP(hipMemcpyAsync(px.data(), arena.p + D.o_pt_x, 32 * (size_t)D.P, hipMemcpyDeviceToHost, s));
-        BA_HIP(hipStreamSynchronize(s));
+        BA_TRY(sync("window download"));
         if (pts)
             for (int ip = 0; ip < D.P; ++ip)
                 for (int i = 0; i < 3; ++i) pts[3 * (size_t)pt_perm[ip] + i] = px[4 * (size_t)ip + i];
@@ -699,7 +714,15 @@ struct ServerBase {
     virtual int step(int* resident, int* queued) = 0;
     virtual int poll(ba_server_result* out, int cap, int* n) = 0;
     virtual double clock() const = 0;
+    // One line of state for error messages: queue, ready images, resident windows, large problems.
+    virtual std::string describe() = 0;
 };
+
+// Device waits of the server (and of its large-problem solvers) give up after this long, and the
+// serving simulation reports an error with the server's state when no request has finished for
+// kServerStallS seconds: a stuck state ends with a message instead of silence.
+constexpr double kServerWaitS = 60.0;
+constexpr double kServerStallS = 120.0;
 
 struct Request {
     int64_t tag = 0;
@@ -746,6 +769,9 @@ struct Server final : ServerBase {
     };
     std::vector<Resident> res;  // by slot
     std::vector<int> active;    // resident slots
+    // submitted windows in arrival order; guarded by qmu (ba_server_submit may run on another
+    // thread than ba_server_step)
+    std::mutex qmu;
     std::deque<std::unique_ptr<Request>> queue;
     std::deque<Done> done;
     std::vector<Done> polled;   // keeps the last poll's parameter arrays alive
@@ -814,6 +840,7 @@ struct Server final : ServerBase {
         BA_TRY(use_device(cfg.options.device));
         if (cfg.iterations_per_step < 1) cfg.iterations_per_step = 1;
         BA_TRY(eng.init(cfg.options, cfg.arena_bytes, cfg.max_windows));
+        eng.wait_timeout_s = kServerWaitS;
         free_ranges[0] = eng.arena_cap;
         res.resize(cfg.max_windows);
         for (int i = cfg.max_windows - 1; i >= 0; --i) free_slots.push_back(i);
@@ -876,7 +903,10 @@ struct Server final : ServerBase {
             return BA_OK;
         }
         Request* rp = r.get();
-        queue.push_back(std::move(r));
+        {
+            std::lock_guard<std::mutex> lk(qmu);
+            queue.push_back(std::move(r));
+        }
         {
             std::lock_guard<std::mutex> lk(bmu);
             build_queue.push_back(rp);
@@ -911,6 +941,7 @@ struct Server final : ServerBase {
             d.r.submit_time_s = r->submit;
             d.r.admit_time_s = r->admit;
             int st = make_solver(&v, o, &sv);
+            if (st == BA_OK) sv->set_wait_timeout(kServerWaitS);
             if (st == BA_OK) st = sv->run(&sm);
             if (st == BA_OK) st = sv->download(v.cameras, v.points);
             d.r.initial_cost = sm.initial_cost;
@@ -926,28 +957,55 @@ struct Server final : ServerBase {
         }
     }
 
+    // A request that can never be solved (its image failed to build, or it needs more device memory
+    // than the whole arena) leaves the queue with a BA_FAILURE result instead of blocking the
+    // requests behind it.
+    void reject(std::unique_ptr<Request> r) {
+        Done d;
+        d.r.tag = r->tag;
+        d.r.submit_time_s = r->submit;
+        d.r.admit_time_s = d.r.finish_time_s = clock();
+        d.r.termination_type = BA_FAILURE;
+        d.cams = std::move(r->cams);
+        d.pts = std::move(r->pts);
+        std::lock_guard<std::mutex> lk(mu);
+        done.push_back(std::move(d));
+    }
+
     int admit() {
         std::vector<int> placed;
-        while (!queue.empty() && !free_slots.empty()) {
-            Request& rq = *queue.front();
-            if (!rq.ready.load(std::memory_order_acquire)) break;  // FIFO: wait for the head's image
-            if (rq.build_status != BA_OK) return fail(rq.build_status, "building request %lld failed", (long long)rq.tag);
-            WindowImage& img = *rq.img;
-            int64_t off = 0;
-            if (!alloc(img.total, &off)) {
-                if (img.total > eng.arena_cap)
-                    return fail(BA_ERR_CAPACITY, "request %lld needs %lld bytes, arena holds %lld", (long long)rq.tag,
-                                (long long)img.total, (long long)eng.arena_cap);
-                break;  // wait for memory to free up (FIFO admission)
+        while (!free_slots.empty()) {
+            Request* head = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(qmu);
+                if (queue.empty()) break;
+                head = queue.front().get();
             }
+            if (!head->ready.load(std::memory_order_acquire)) break;  // FIFO: wait for the head's image
+            if (head->build_status != BA_OK || head->img->total > eng.arena_cap) {
+                std::unique_ptr<Request> r;
+                {
+                    std::lock_guard<std::mutex> lk(qmu);
+                    r = std::move(queue.front());
+                    queue.pop_front();
+                }
+                reject(std::move(r));
+                continue;
+            }
+            WindowImage& img = *head->img;
+            int64_t off = 0;
+            if (!alloc(img.total, &off)) break;  // wait for memory to free up (FIFO admission)
             const int slot = free_slots.back();
             free_slots.pop_back();
             Resident& r = res[slot];
             r.slot = slot;
             r.off = off;
             r.size = img.total;
-            r.req = std::move(queue.front());
-            queue.pop_front();
+            {
+                std::lock_guard<std::mutex> lk(qmu);
+                r.req = std::move(queue.front());
+                queue.pop_front();
+            }
             r.req->admit = clock();
             r.img = std::move(*r.req->img);
             r.req->img.reset();
@@ -955,9 +1013,32 @@ struct Server final : ServerBase {
             active.push_back(slot);
             placed.push_back(slot);
         }
-        if (!placed.empty()) BA_HIP(hipStreamSynchronize(eng.s));
+        if (!placed.empty()) BA_TRY(eng.sync("window uploads"));
         for (int sl : placed) std::vector<char>().swap(res[sl].img.bytes);
         return BA_OK;
+    }
+
+    std::string describe() override {
+        size_t nq = 0, nready = 0;
+        {
+            std::lock_guard<std::mutex> lk(qmu);
+            nq = queue.size();
+            for (const auto& r : queue) nready += r->ready.load(std::memory_order_acquire) ? 1 : 0;
+        }
+        size_t ndone = 0, nbig = 0;
+        int inflight = 0;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            ndone = done.size();
+            nbig = big_queue.size();
+            inflight = big_inflight;
+        }
+        char buf[256];
+        snprintf(buf, sizeof(buf),
+                 "queued %zu (images ready %zu), resident %zu of %d slots, free arena ranges %zu, large problems "
+                 "in flight %d (waiting %zu), results not yet polled %zu",
+                 nq, nready, active.size(), cfg.max_windows, free_ranges.size(), inflight, nbig, ndone);
+        return buf;
     }
 
     int step(int* resident, int* queued) override {
@@ -994,7 +1075,7 @@ struct Server final : ServerBase {
                         BA_HIP(hipMemcpyAsync(dst + 72 * (size_t)D.K, eng.arena.p + D.o_pt_x, 32 * (size_t)D.P,
                                               hipMemcpyDeviceToHost, eng.s));
                 }
-            if (need) BA_HIP(hipStreamSynchronize(eng.s));
+            if (need) BA_TRY(eng.sync("finished windows' parameters"));
             const double now = clock();
             for (size_t i = 0; i < order.size(); ++i) {
                 if (!st[i].done) continue;
@@ -1029,8 +1110,13 @@ struct Server final : ServerBase {
         }
         if (resident) *resident = (int)active.size();
         if (queued) {
+            int nq = 0;
+            {
+                std::lock_guard<std::mutex> lk(qmu);
+                nq = (int)queue.size();
+            }
             std::lock_guard<std::mutex> lk(mu);
-            *queued = (int)queue.size() + big_inflight;
+            *queued = nq + big_inflight;
         }
         return BA_OK;
     }
@@ -1208,9 +1294,17 @@ int ba_server_simulate(const ba_server_config* config, const ba_problem_view* po
     }
     const double start = sv->impl->clock();
     int next = 0, finished = 0;
+    double last_progress = 0.0;
     std::vector<ba_server_result> buf(64);
     while (finished < count) {
         const double now = sv->impl->clock() - start;
+        if (next == finished) last_progress = now;  // nothing outstanding: waiting for arrivals
+        if (now - last_progress > kServerStallS) {
+            const std::string state = sv->impl->describe();
+            return fail(BA_ERR_TIMEOUT,
+                        "serving simulation: no request finished for %.0f s (submitted %d of %d, finished %d); %s",
+                        now - last_progress, next, count, finished, state.c_str());
+        }
         while (next < count && arrival[next] <= now) {
             BA_TRY(sv->impl->submit(&pool[next % pool_count], next));
             ++next;
@@ -1231,6 +1325,7 @@ int ba_server_simulate(const ba_server_config* config, const ba_problem_view* po
                 r.points = nullptr;
                 results[id] = r;
                 ++finished;
+                last_progress = sv->impl->clock() - start;
             }
         } while (n == (int)buf.size());
         if (resident == 0) {

@@ -1,0 +1,107 @@
+"""Parity at the benchmark sizes: configurations 2 and 3 solved on the GPU against the FP64 CPU
+oracle's traces on the same seeded problems (tests/golden/fullsize_traces.json, written by
+tests/golden/make_fullsize_traces.py).
+
+Bars: FP64 per-iteration cost within 1e-10 relative of the oracle while the two runs take the same
+accept/reject decisions, final cost within 1e-6 relative (FP64) and 1e-5 (mixed precision)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import slamplay_amd as S
+from conftest import GOLDEN
+
+FIXED = dict(function_tolerance=0.0, gradient_tolerance=0.0, parameter_tolerance=0.0)
+
+
+@pytest.fixture(scope="module")
+def traces():
+    with open(os.path.join(GOLDEN, "fullsize_traces.json")) as fh:
+        return json.load(fh)
+
+
+def _problem(case):
+    p = S.synthetic(case["config"], seed=case["seed"])
+    assert (p.num_cameras(), p.num_points(), p.num_observations()) == (case["cameras"], case["points"],
+                                                                        case["observations"])
+    return p
+
+
+def test_fixture_problems_match_generator(traces):
+    """The generator still produces the problems the fixture traces were computed on."""
+    import sys
+    sys.path.insert(0, GOLDEN)
+    from make_fullsize_traces import digest
+    for name, case in traces.items():
+        assert digest(_problem(case)) == case["digest"], name
+
+
+def _solve(case, precision):
+    p = _problem(case)
+    print("solving %s precision %d" % (case["config"], precision), flush=True)
+    return S.SolveBA(p, max_num_iterations=case["lm_iterations"], precision=precision, **case["options"], **FIXED)
+
+
+def _leading_agreement(out, case):
+    """Rows over which the GPU run takes the oracle's decisions, and their worst cost difference."""
+    g, r = out["trace"], case["trace"]
+    n = 0
+    worst = 0.0
+    for a, b in zip(g, r):
+        if a["step_is_successful"] != b["step_is_successful"]:
+            break
+        worst = max(worst, abs(a["cost"] - b["cost"]) / b["cost"])
+        n += 1
+    return n, worst
+
+
+@pytest.mark.gpu
+def test_config2_dense_fp64_matches_oracle(traces):
+    case = traces["config2_dense"]
+    out = _solve(case, 0)
+    n, worst = _leading_agreement(out, case)
+    print("config 2 FP64: %d rows agree, worst %.2e, final %.3e vs %.3e" %
+          (n, worst, out["final_cost"], case["final_cost"]))
+    assert n >= 10 and worst < 1e-10
+    assert abs(out["final_cost"] - case["final_cost"]) / case["final_cost"] < 1e-6
+
+
+@pytest.mark.gpu
+def test_config2_dense_mixed_final_cost(traces):
+    case = traces["config2_dense"]
+    out = _solve(case, 1)
+    rel = abs(out["final_cost"] - case["final_cost"]) / case["final_cost"]
+    print("config 2 mixed: final %.9e vs %.9e (%.2e)" % (out["final_cost"], case["final_cost"], rel))
+    assert rel < 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_config3_pcg_final_cost(traces, precision):
+    case = traces["config3_pcg"]
+    out = _solve(case, precision)
+    n, worst = _leading_agreement(out, case)
+    rel = abs(out["final_cost"] - case["final_cost"]) / case["final_cost"]
+    print("config 3 precision %d: %d rows agree (worst %.2e), final %.9e vs %.9e (%.2e)" %
+          (precision, n, worst, out["final_cost"], case["final_cost"], rel))
+    assert len(out["trace"]) == case["lm_iterations"] + 1
+    assert rel < (1e-6 if precision == 0 else 1e-5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_config3_near_exact_steps(traces, precision):
+    """Configuration 3 with near-exact linear solves (PCG to 1e-10): the steps no longer depend on
+    where a truncated Krylov iteration happens to stop, so the FP64 trajectory follows the oracle's
+    per iteration and the final costs agree to 1e-6 (FP64) / 1e-5 (mixed)."""
+    case = traces["config3_pcg_tight"]
+    out = _solve(case, precision)
+    n, worst = _leading_agreement(out, case)
+    rel = abs(out["final_cost"] - case["final_cost"]) / case["final_cost"]
+    print("config 3 near-exact, precision %d: %d rows agree (worst %.2e), final %.9e vs %.9e (%.2e)" %
+          (precision, n, worst, out["final_cost"], case["final_cost"], rel))
+    if precision == 0:
+        assert n == len(case["trace"]) and worst < 1e-9
+    assert rel < (1e-6 if precision == 0 else 1e-5)

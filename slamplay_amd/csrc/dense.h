@@ -14,10 +14,11 @@
 // products run on the matrix cores (v_mfma_f64_16x16x4f64, v_mfma_f32_16x16x4f32), and the
 // trailing update also factors the next diagonal tile, so a step is two launches.  The panel is
 // also stored transposed in the upper tiles (the factorization reads only the lower ones; in FP64
-// the factor overwrites S).  The triangular solves use the kept tile inverses, so no step has a
-// serial dependency chain longer than one tile.  In mixed precision the factor is FP32 and the
-// step is refined in FP64 against the FP64 S (refine_steps iterations of r = b - S y,
-// y += S_f^-1 r).
+// the factor overwrites S).  The triangular solves work on diagonal blocks of four tiles whose
+// inverses are formed after the factorization, one launch per block and sweep, with the rows
+// below (above) the block updated by many workgroups at once.  In mixed precision the factor is
+// FP32 and the step is refined in FP64 against the FP64 S (refine_steps iterations of
+// r = b - S y, y += S_f^-1 r).
 #pragma once
 
 #include <algorithm>
@@ -282,58 +283,166 @@ __global__ void __launch_bounds__(256) k_syrk_tiles(F* __restrict__ A, int ld, i
     if (w == 0 && !potrf_tile_wave<F>(T, dl, A, ld, k + 1, Dinv) && l == 0) st->step_invalid = 1;
 }
 
-// Solve (L L') x = b with the tiled factor (single workgroup); x = y + solution when accumulate != 0.
-// Forward step k: z_k <- Dinv_k z_k, then z_i -= L_ik z_k for the rows below, read from the
-// transposed panel in the upper tiles; backward step k: z_k <- Dinv_k' z_k, then z_i -= L_ki' z_k for
-// the rows above, read from the lower tiles.  Threads run along matrix rows in both sweeps.
+// ---------------------------------------------------------------------------------------------
+// Blocked triangular solves over many workgroups.  The factor is cut into diagonal blocks of
+// kSolveBlk tiles (128 rows); after the factorization k_block_inverse forms each block's inverse
+// X = L_KK^-1 from the kept tile inverses (X_ii = Dinv_i, X_ij = -Dinv_i sum_{j<=k<i} L_ik X_kj),
+// so a block of either sweep is one matrix-vector product.  Forward sweep, launch K: workgroup 0
+// applies block K-1's solution to block K's rows and solves block K (y_K = X_K z_K); the other
+// workgroups apply block K-1's solution to one tile row each below block K.  The backward sweep
+// runs the same way upwards with the transposed panel (upper tiles) and x_K = X_K' z_K.  One sweep
+// is one launch per block; every workgroup streams rows of the factor with one wave per row.
+constexpr int kSolveBlk = 4;
+constexpr int kSolveRows = kSolveBlk * kTile;
+
+// Tiles of block K: [kSolveBlk K, min(kSolveBlk (K + 1), nt)).
+__device__ __forceinline__ int solve_block_tiles(int K, int nt) { return min(kSolveBlk, nt - kSolveBlk * K); }
+
 template <typename F>
-__global__ void __launch_bounds__(1024) k_chol_solve(const F* __restrict__ A, int ld, const F* __restrict__ Dinv,
-                                                     const double* __restrict__ b, double* __restrict__ x, int n,
-                                                     int accumulate, LMState* st) {
+__global__ void __launch_bounds__(256) k_block_inverse(const F* __restrict__ A, int ld, const F* __restrict__ Dinv,
+                                                        F* __restrict__ Binv, int nt, const LMState* st) {
     if (st->done || st->step_invalid) return;
-    extern __shared__ double z[];
-    __shared__ double zt[kTile];
-    const int t = threadIdx.x, nt = ld / kTile;
-    const int hr = t >> 5, hm = t & 31;  // (row, column) of the tile element handled in the diagonal products
-    for (int i = t; i < ld; i += blockDim.x) z[i] = i < n ? b[i] : 0.0;
+    __shared__ double T[kTile][kTile + 1];
+    const int K = blockIdx.x, bt = solve_block_tiles(K, nt), t0 = kSolveBlk * K;
+    F* X = Binv + (size_t)K * kSolveRows * kSolveRows;  // row-major kSolveRows x kSolveRows
+    const int t = threadIdx.x;
+    // zero the block, then the diagonal tiles
+    for (int e = t; e < kSolveRows * kSolveRows; e += 256) X[e] = (F)0;
     __syncthreads();
-    for (int k = 0; k < nt; ++k) {  // forward: L z = b
-        const F* Dk = Dinv + (size_t)k * kTile * kTile;
-        double v = (double)Dk[hr * kTile + hm] * z[kTile * k + hm];
-#pragma unroll
-        for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 32);
-        if (hm == 0) zt[hr] = v;
-        __syncthreads();
-        if (t < kTile) z[kTile * k + t] = zt[t];
-        __syncthreads();
-        const F* U = A + (size_t)(kTile * k) * ld;
-        for (int i = kTile * (k + 1) + t; i < ld; i += blockDim.x) {
-            double s = 0.0;
-#pragma unroll 8
-            for (int m = 0; m < kTile; ++m) s += (double)U[(size_t)m * ld + i] * z[kTile * k + m];
-            z[i] -= s;
+    for (int i = 0; i < bt; ++i)
+        for (int e = t; e < kTile * kTile; e += 256)
+            X[(size_t)(kTile * i + e / kTile) * kSolveRows + kTile * i + e % kTile] =
+                Dinv[(size_t)(t0 + i) * kTile * kTile + e];
+    __syncthreads();
+    for (int i = 1; i < bt; ++i) {
+        for (int j = 0; j < i; ++j) {
+            // T = sum_{k=j}^{i-1} L_{t0+i, t0+k} X_kj
+            for (int e = t; e < kTile * kTile; e += 256) {
+                const int r = e / kTile, c = e % kTile;
+                double s = 0.0;
+                for (int k = j; k < i; ++k) {
+                    const F* Lr = A + (size_t)(kTile * (t0 + i) + r) * ld + kTile * (t0 + k);
+                    const F* Xc = X + (size_t)(kTile * k) * kSolveRows + kTile * j + c;
+                    for (int m = 0; m < kTile; ++m) s += (double)Lr[m] * (double)Xc[(size_t)m * kSolveRows];
+                }
+                T[r][c] = s;
+            }
+            __syncthreads();
+            // X_ij = -Dinv_i T
+            const F* Di = Dinv + (size_t)(t0 + i) * kTile * kTile;
+            for (int e = t; e < kTile * kTile; e += 256) {
+                const int r = e / kTile, c = e % kTile;
+                double s = 0.0;
+                for (int m = 0; m < kTile; ++m) s += (double)Di[r * kTile + m] * T[m][c];
+                X[(size_t)(kTile * i + r) * kSolveRows + kTile * j + c] = (F)(-s);
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
-    for (int k = nt - 1; k >= 0; --k) {  // backward: L' x = z
-        const F* Dk = Dinv + (size_t)k * kTile * kTile;
-        double v = (double)Dk[hm * kTile + hr] * z[kTile * k + hm];
-#pragma unroll
-        for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 32);
-        if (hm == 0) zt[hr] = v;
-        __syncthreads();
-        if (t < kTile) z[kTile * k + t] = zt[t];
-        __syncthreads();
-        const F* Lk = A + (size_t)(kTile * k) * ld;
-        for (int i = t; i < kTile * k; i += blockDim.x) {
-            double s = 0.0;
-#pragma unroll 8
-            for (int m = 0; m < kTile; ++m) s += (double)Lk[(size_t)m * ld + i] * z[kTile * k + m];
-            z[i] -= s;
+}
+
+// z[rows] -= M[rows][cols] v for `nrows` rows starting at row r0 of a row-major matrix M (leading
+// dimension ldm, column offset c0), v = z[v0 .. v0 + ncols) in double; one wave per row.
+template <typename F>
+__device__ __forceinline__ void rows_minus_mv(const F* __restrict__ M, size_t ldm, int r0, int nrows, int c0,
+                                              int ncols, const double* __restrict__ v, double* __restrict__ z) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int rr = wave; rr < nrows; rr += kWaves) {
+        const F* row = M + (size_t)(r0 + rr) * ldm + c0;
+        double s = 0.0;
+        for (int c = lane; c < ncols; c += 64) s += (double)row[c] * v[c];
+        s = wave_sum(s);
+        if (lane == 0) z[r0 + rr] -= s;
+    }
+}
+
+// Forward sweep, launch K (see above).  z has ld entries; launch 0 copies b into z first.
+template <typename F>
+__global__ void __launch_bounds__(256) k_trsv_fwd(const F* __restrict__ A, int ld, const F* __restrict__ Binv, int nt,
+                                                  int K, const double* __restrict__ b, int n, double* __restrict__ z,
+                                                  const LMState* st) {
+    if (st->done || st->step_invalid) return;
+    __shared__ double v[kSolveRows];
+    const int t = threadIdx.x, g = blockIdx.x;
+    const int kb = kSolveRows * K;                       // first row of block K
+    const int rows_k = kTile * solve_block_tiles(K, nt);  // rows of block K
+    if (g > 0) {
+        // one tile row below block K
+        const int r0 = kb + rows_k + kTile * (g - 1);
+        if (K == 0) {
+            for (int r = t; r < kTile; r += 256) z[r0 + r] = r0 + r < n ? b[r0 + r] : 0.0;
+            return;
         }
+        const int pb = kb - kSolveRows;  // block K-1 (always full)
+        for (int c = t; c < kSolveRows; c += 256) v[c] = z[pb + c];
         __syncthreads();
+        rows_minus_mv<F>(A, (size_t)ld, r0, kTile, pb, kSolveRows, v, z);
+        return;
     }
-    for (int i = t; i < n; i += blockDim.x) x[i] = accumulate ? x[i] + z[i] : z[i];
+    __shared__ double zk[kSolveRows];
+    if (K == 0) {
+        for (int r = t; r < rows_k; r += 256) z[r] = r < n ? b[r] : 0.0;
+    } else {
+        const int pb = kb - kSolveRows;
+        for (int c = t; c < kSolveRows; c += 256) v[c] = z[pb + c];
+        __syncthreads();
+        rows_minus_mv<F>(A, (size_t)ld, kb, rows_k, pb, kSolveRows, v, z);
+    }
+    __syncthreads();
+    for (int r = t; r < rows_k; r += 256) zk[r] = z[kb + r];
+    __syncthreads();
+    // y_K = X_K z_K (one wave per row)
+    const F* X = Binv + (size_t)K * kSolveRows * kSolveRows;
+    const int wave = t >> 6, lane = t & 63;
+    for (int r = wave; r < rows_k; r += kWaves) {
+        double s = 0.0;
+        for (int c = lane; c < rows_k; c += 64) s += (double)X[(size_t)r * kSolveRows + c] * zk[c];
+        s = wave_sum(s);
+        if (lane == 0) z[kb + r] = s;
+    }
+}
+
+// Backward sweep, launch K (blocks from the last down to 0): x_K = X_K' (z_K - U_{K,K+1} x_{K+1});
+// rows above block K get block K+1's solution applied.  Block K's result also goes to x (added to
+// it when accumulate != 0), rows below n excluded.
+template <typename F>
+__global__ void __launch_bounds__(256) k_trsv_bwd(const F* __restrict__ A, int ld, const F* __restrict__ Binv, int nt,
+                                                  int K, double* __restrict__ z, double* __restrict__ x, int n,
+                                                  int accumulate, const LMState* st) {
+    if (st->done || st->step_invalid) return;
+    __shared__ double v[kSolveRows];
+    const int t = threadIdx.x, g = blockIdx.x;
+    const int nb = (nt + kSolveBlk - 1) / kSolveBlk;
+    const int kb = kSolveRows * K, rows_k = kTile * solve_block_tiles(K, nt);
+    const bool has_next = K + 1 < nb;
+    const int nbk = has_next ? kSolveRows * (K + 1) : 0;                      // first row of block K+1
+    const int cols_next = has_next ? kTile * solve_block_tiles(K + 1, nt) : 0;  // its rows
+    if (g > 0) {
+        if (!has_next) return;
+        const int r0 = kTile * (g - 1);  // a tile row above block K
+        for (int c = t; c < cols_next; c += 256) v[c] = z[nbk + c];
+        __syncthreads();
+        rows_minus_mv<F>(A, (size_t)ld, r0, kTile, nbk, cols_next, v, z);
+        return;
+    }
+    __shared__ double zk[kSolveRows];
+    if (has_next) {
+        for (int c = t; c < cols_next; c += 256) v[c] = z[nbk + c];
+        __syncthreads();
+        rows_minus_mv<F>(A, (size_t)ld, kb, rows_k, nbk, cols_next, v, z);
+    }
+    __syncthreads();
+    for (int r = t; r < rows_k; r += 256) zk[r] = z[kb + r];
+    __syncthreads();
+    // x_K = X_K' z_K: one thread per column
+    const F* X = Binv + (size_t)K * kSolveRows * kSolveRows;
+    for (int c = t; c < rows_k; c += 256) {
+        double s = 0.0;
+        for (int r = 0; r < rows_k; ++r) s += (double)X[(size_t)r * kSolveRows + c] * zk[r];
+        z[kb + c] = s;
+        const int row = kb + c;
+        if (row < n) x[row] = accumulate ? x[row] + s : s;
+    }
 }
 
 // r = b - S y (FP64, S symmetric with its full n x n block stored); one wave per row.
@@ -356,15 +465,15 @@ __global__ void k_set_linear_iters(LMState* st, int iters) {
 
 template <typename Real>
 struct DenseSchur {
-    int n = 0, ld = 0, nt = 0, nblocks = 0, refine = 0;
+    int n = 0, ld = 0, nt = 0, nb = 0, nblocks = 0, refine = 0;
     DevBuf<Real> js;
     DevBuf<int> blk_a, blk_b, blk_off, pair_i, pair_j;
-    DevBuf<double> S, r;
-    DevBuf<float> Af, Dinvf;
-    DevBuf<double> Dinvd;
+    DevBuf<double> S, r, z;
+    DevBuf<float> Af, Dinvf, Binvf;
+    DevBuf<double> Dinvd, Binvd;
     int64_t device_bytes() const {
-        return js.bytes() + blk_a.bytes() * 3 + pair_i.bytes() * 2 + S.bytes() + r.bytes() + Af.bytes() + Dinvf.bytes() +
-               Dinvd.bytes();
+        return js.bytes() + blk_a.bytes() * 3 + pair_i.bytes() * 2 + S.bytes() + r.bytes() + z.bytes() + Af.bytes() +
+               Dinvf.bytes() + Binvf.bytes() + Dinvd.bytes() + Binvd.bytes();
     }
 
     int create(const ba_problem_view* v, const Layout& L, const ba_options& o, hipStream_t s) {
@@ -373,18 +482,8 @@ struct DenseSchur {
         n = 9 * nc;
         nt = std::max(1, (n + kTile - 1) / kTile);
         ld = nt * kTile;
+        nb = (nt + kSolveBlk - 1) / kSolveBlk;
         refine = sizeof(Real) == sizeof(double) ? 0 : std::max(0, o.refine_steps);
-        // the triangular solve keeps the ld-vector in LDS (160 KiB per workgroup, less the kernel's
-        // static arrays)
-        const size_t shm = (size_t)ld * sizeof(double);
-        constexpr size_t kSolveLdsMax = (160 - 4) * 1024;
-        if (shm > kSolveLdsMax)
-            return fail(BA_ERR_CAPACITY, "dense Schur with %d cameras exceeds the triangular solve's LDS (at most %d "
-                        "cameras); use BA_SOLVER_ITERATIVE_SCHUR", nc, (int)(kSolveLdsMax / sizeof(double) / 9));
-        if (sizeof(Real) == sizeof(double))
-            BA_TRY(allow_dynamic_lds((const void*)k_chol_solve<double>, (int)shm));
-        else
-            BA_TRY(allow_dynamic_lds((const void*)k_chol_solve<float>, (int)shm));
         PairBlocks pb;
         build_pair_blocks(L, &pb);
         const std::vector<int>&ba = pb.a, &bb = pb.b, &boff = pb.off, &pi = pb.i, &pj = pb.j;
@@ -397,18 +496,35 @@ struct DenseSchur {
         BA_TRY(js.alloc(kJs * (size_t)std::max(L.no, 1)));
         BA_TRY(S.alloc((size_t)ld * ld));
         BA_TRY(r.alloc(ld));
+        BA_TRY(z.alloc(ld));
+        const size_t binv = (size_t)nb * kSolveRows * kSolveRows;
         if (sizeof(Real) == sizeof(double)) {
             BA_TRY(Dinvd.alloc((size_t)nt * kTile * kTile));
+            BA_TRY(Binvd.alloc(binv));
         } else {
             BA_TRY(Af.alloc((size_t)ld * ld));
             BA_TRY(Dinvf.alloc((size_t)nt * kTile * kTile));
+            BA_TRY(Binvf.alloc(binv));
         }
         BA_HIP(hipStreamSynchronize(s));
         return BA_OK;
     }
 
+    // x = (L L')^-1 b (accumulate: x += ...) by the blocked sweeps: nb launches each way.
     template <typename F, typename SolverT>
-    void factor_and_solve(SolverT& sv, F* A, F* Dinv) {
+    void solve(SolverT& sv, const F* A, const F* Binv, const double* b, double* x, int accumulate) {
+        auto& d = sv.d;
+        for (int K = 0; K < nb; ++K) {
+            const int below = std::max(0, nt - kSolveBlk * (K + 1));
+            sv.launch("trsv_fwd", k_trsv_fwd<F>, 1 + below, 256, A, ld, Binv, nt, K, b, n, z.p, (const LMState*)d.st);
+        }
+        for (int K = nb - 1; K >= 0; --K)
+            sv.launch("trsv_bwd", k_trsv_bwd<F>, 1 + kSolveBlk * K, 256, A, ld, Binv, nt, K, z.p, x, n, accumulate,
+                      (const LMState*)d.st);
+    }
+
+    template <typename F, typename SolverT>
+    void factor_and_solve(SolverT& sv, F* A, F* Dinv, F* Binv) {
         auto& d = sv.d;
         sv.launch("potrf", k_potrf_tile<F>, 1, 64, A, ld, 0, Dinv, d.st);
         for (int k = 0; k + 1 < nt; ++k) {
@@ -416,14 +532,13 @@ struct DenseSchur {
             sv.launch("trsm", k_trsm_tiles<F>, m, 256, A, ld, k, (const F*)Dinv, (const LMState*)d.st);
             sv.launch("syrk", k_syrk_tiles<F>, m * (m + 1) / 2, 256, A, ld, k, Dinv, d.st);
         }
-        const size_t shm = (size_t)ld * sizeof(double);
-        sv.launch_shm("chol_solve", k_chol_solve<F>, 1, 1024, shm, (const F*)A, ld, (const F*)Dinv, (const double*)d.bc,
-                      d.px, n, 0, d.st);
+        sv.launch("block_inverse", k_block_inverse<F>, nb, 256, (const F*)A, ld, (const F*)Dinv, Binv, nt,
+                  (const LMState*)d.st);
+        solve<F>(sv, A, Binv, (const double*)d.bc, d.px, 0);
         for (int it = 0; it < refine; ++it) {
             sv.launch("residual", k_residual, div_up(n, kWaves), kWG, (const double*)S.p, ld, (const double*)d.bc,
                       (const double*)d.px, r.p, n, (const LMState*)d.st);
-            sv.launch_shm("chol_solve", k_chol_solve<F>, 1, 1024, shm, (const F*)A, ld, (const F*)Dinv,
-                          (const double*)r.p, d.px, n, 1, d.st);
+            solve<F>(sv, A, Binv, (const double*)r.p, d.px, 1);
         }
     }
 
@@ -440,11 +555,11 @@ struct DenseSchur {
         sv.launch("schur_pairs", k_schur_pairs<Real>, nblocks, kWG, d, (const Real*)js.p, (const int*)blk_a.p,
                   (const int*)blk_b.p, (const int*)blk_off.p, (const int*)pair_i.p, (const int*)pair_j.p, S.p, ld);
         if (sizeof(Real) == sizeof(double)) {
-            factor_and_solve<double>(sv, S.p, Dinvd.p);
+            factor_and_solve<double>(sv, S.p, Dinvd.p, Binvd.p);
         } else {
             sv.launch("convert_S", k_convert_S<float>, (int)((nn + 255) / 256), 256, (const double*)S.p, Af.p, nn,
                       (const LMState*)d.st);
-            factor_and_solve<float>(sv, Af.p, Dinvf.p);
+            factor_and_solve<float>(sv, Af.p, Dinvf.p, Binvf.p);
         }
         sv.launch("set_iters", k_set_linear_iters, 1, 1, d.st, 1 + refine);
     }

@@ -100,6 +100,7 @@ struct Dev {
     const int* co_pt;
     const double* co_uv64;
     const Real* co_uv;
+    Real* co_w;                // per observation (camera order): Huber weight sqrt(rho') at the linearization
     const int* chunk_off;
     const int* chunk_len;
     const int* slot_of;
@@ -335,10 +336,11 @@ __device__ __forceinline__ void corrected_from(const Dev<Real>& d, const CameraP
     for (int i = 0; i < 6; ++i) Jp[i] *= w;
 }
 
-// Corrected (sqrt(rho')-weighted) residual and Jacobians of observation j of camera prep P.
+// Corrected (sqrt(rho')-weighted) residual and Jacobians of observation j of camera prep P (and
+// the weight itself in *w_out when given).
 template <typename Real, int M>
 __device__ __forceinline__ void corrected_jacobian(const Dev<Real>& d, const CameraPrep<Real>& P, int j, int& p,
-                                                   Real r[2], Real Jc[18], Real Jp[6]) {
+                                                   Real r[2], Real Jc[18], Real Jp[6], Real* w_out = nullptr) {
     p = d.co_pt[j];
     double X[3];
     load_point<double>(d.pt_x, p, X);
@@ -348,6 +350,7 @@ __device__ __forceinline__ void corrected_jacobian(const Dev<Real>& d, const Cam
         snavely_residual_jacobian<Real>(P, X, d.co_uv[2 * (size_t)j], d.co_uv[2 * (size_t)j + 1], r, Jc, Jp,
                                         M == kModelSnavelyG2o);
     const Real w = huber_sqrt_rho1<Real>(r[0] * r[0] + r[1] * r[1], d.o.loss, (Real)d.o.loss_a);
+    if (w_out) *w_out = w;
     r[0] *= w;
     r[1] *= w;
 #pragma unroll
@@ -386,8 +389,9 @@ __global__ void __launch_bounds__(kWG) k_lin_cam(Dev<Real> d) {
     for (int i = 0; i < 18; ++i) acc[i] = 0.0;
     for (int j = d.cam_off[c] + threadIdx.x; j < d.cam_off[c + 1]; j += kWG) {
         int p;
-        Real r[2], Jc[18], Jp[6];
-        corrected_jacobian<Real, M>(d, P, j, p, r, Jc, Jp);
+        Real r[2], Jc[18], Jp[6], w;
+        corrected_jacobian<Real, M>(d, P, j, p, r, Jc, Jp, &w);
+        if (M == kModelSnavely) d.co_w[j] = w;  // for the direction-form PCG products
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             acc[i] += (double)(Jc[i] * Jc[i] + Jc[9 + i] * Jc[9 + i]);
@@ -808,6 +812,31 @@ __device__ __forceinline__ void reduce_upd(const double* __restrict__ part, int 
     b = lds[kWaves];
 }
 
+// Start of PCG iteration `it` (>= 1) in every workgroup: the convergence test and beta from the
+// previous update's partials (each workgroup reduces them in the same order, so all agree).
+// Returns true when the iteration does not run.
+template <typename Real>
+__device__ __forceinline__ bool pcg_beta(const Dev<Real>& d, int it, double* lds, double& beta) {
+    LMState* st = d.st;
+    double rz, rr;
+    reduce_upd(d.upd_part, d.n_upd, rz, rr, lds);
+    bool done;
+    if (it == 1)
+        done = !(rr > 0.0) || d.o.pcg_max_iterations <= 0;
+    else
+        done = rr <= d.o.pcg_tol * d.o.pcg_tol * st->pcg_bb || it - 1 >= d.o.pcg_max_iterations;
+    if (!isfinite(rz) || !isfinite(rr)) done = true;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (it == 1) st->pcg_bb = rr;
+        d.rz_hist[it] = rz;
+        st->pcg_iters = done ? it - 1 : it;
+        if (done) st->pcg_done = 1;
+        if (!isfinite(rz) || !isfinite(rr)) st->step_invalid = 1;
+    }
+    beta = done || it == 1 ? 0.0 : rz / d.rz_hist[it - 1];
+    return done;
+}
+
 // One WG per camera.  mode 1 (PCG iteration it): see above.  mode 0 (back-substitution):
 // u_j = Jp'(r~ - Jc S_c y_c) with y = vsrc.
 template <typename Real, int M>
@@ -824,25 +853,7 @@ __global__ void __launch_bounds__(kWG) k_jtx(Dev<Real> d, int mode, int it, cons
     ObsIn<Real> in{};
     if (j < j_end) load_obs<Real, M>(d, j, in);
     double beta = 0.0;
-    if (mode == 1) {
-        double rz, rr;
-        reduce_upd(d.upd_part, d.n_upd, rz, rr, lds);
-        bool done;
-        if (it == 1)
-            done = !(rr > 0.0) || d.o.pcg_max_iterations <= 0;
-        else
-            done = rr <= d.o.pcg_tol * d.o.pcg_tol * st->pcg_bb || it - 1 >= d.o.pcg_max_iterations;
-        if (!isfinite(rz) || !isfinite(rr)) done = true;
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            if (it == 1) st->pcg_bb = rr;
-            d.rz_hist[it] = rz;
-            st->pcg_iters = done ? it - 1 : it;
-            if (done) st->pcg_done = 1;
-            if (!isfinite(rz) || !isfinite(rr)) st->step_invalid = 1;
-        }
-        if (done) return;
-        beta = it == 1 ? 0.0 : rz / d.rz_hist[it - 1];
-    }
+    if (mode == 1 && pcg_beta(d, it, lds, beta)) return;
     if (threadIdx.x < 9) {
         const size_t i = 9 * (size_t)c + threadIdx.x;
         double v = vsrc[i];
@@ -966,6 +977,201 @@ __global__ void __launch_bounds__(kWG) k_jz(Dev<Real> d) {
     }
     if (t == 0) {
         double s = lds[9 * kWaves];
+        for (int i = 0; i < 9; ++i) s += psh[i] * d.dc2[9 * (size_t)c + i] * psh[i];
+        d.cam_part[PS_A * (size_t)d.nc + c] = s;
+    }
+}
+
+// The two camera-order PCG products of the Snavely model (Ceres additive update) in direction form
+// (snavely.h, rot_directional): the same u_j and q_c as k_jtx (mode 1) and k_jz, without forming
+// the 2x9 / 2x3 Jacobian per observation.  The Huber weight sqrt(rho') of each observation was
+// stored by k_lin_cam at the linearization point (co_w), so the measurement is not re-read.
+
+// Observation inputs of the direction-form kernels, loaded one round ahead of their use.
+template <typename Real>
+struct DirIn {
+    int p;
+    double4 X;
+    Real w;
+};
+
+template <typename Real>
+__device__ __forceinline__ void load_dir(const Dev<Real>& d, int j, DirIn<Real>& in) {
+    in.p = d.co_pt[j];
+    in.X = reinterpret_cast<const double4*>(d.pt_x)[in.p];
+    in.w = d.co_w[j];
+}
+
+// Per-camera constants of a direction-form product: dR_v for the scaled direction v = S_c p_c,
+// the rotation matrix in working precision and v's translation / intrinsic parts.
+template <typename Real>
+struct DirCam {
+    Real Rd[9], R[9], vt[3], vf, vk1, vk2;
+};
+
+template <typename Real>
+__device__ __forceinline__ void load_dircam(const CameraPrep<Real>& P, const double* vsh, const double* rsh,
+                                            DirCam<Real>& C) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        C.Rd[k] = (Real)rsh[k];
+        C.R[k] = (Real)P.R64[k];
+    }
+    C.vt[0] = (Real)vsh[3];
+    C.vt[1] = (Real)vsh[4];
+    C.vt[2] = (Real)vsh[5];
+    C.vf = (Real)vsh[6];
+    C.vk1 = (Real)vsh[7];
+    C.vk2 = (Real)vsh[8];
+}
+
+// a = Jc v / w for one observation (w: its Huber weight), and its linearization.
+template <typename Real>
+__device__ __forceinline__ void dir_apply(const CameraPrep<Real>& P, const DirCam<Real>& C, const double4& X4,
+                                          const Real zR[3], ProjLin<Real>& L, Real a[2]) {
+    const double X64[3] = {X4.x, X4.y, X4.z};
+    proj_lin<Real>(P, X64, L);
+    const Real X[3] = {(Real)X4.x, (Real)X4.y, (Real)X4.z};
+    Real q[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) q[i] = C.Rd[3 * i] * X[0] + C.Rd[3 * i + 1] * X[1] + C.Rd[3 * i + 2] * X[2] + C.vt[i] - zR[i];
+    proj_apply<Real>(L, q, a);
+    const Real kap = L.dist * C.vf + P.f * L.r2 * (C.vk1 + L.r2 * C.vk2);
+    a[0] += L.xp * kap;
+    a[1] += L.yp * kap;
+}
+
+// PCG iteration `it`: beta, p <- z + beta p, then u_j = Jp'(Jc S_c p_c) per observation (slot order).
+template <typename Real>
+__global__ void __launch_bounds__(kWG) k_jtx_dir(Dev<Real> d, int it) {
+    LMState* st = d.st;
+    if (st->done || st->step_invalid || st->pcg_done) return;
+    __shared__ double lds[2 * kWaves];
+    __shared__ double vsh[9], rsh[9];
+    const int c = xcd_remap(blockIdx.x, d.nc);
+    if (c >= d.nc) return;
+    const int j_end = d.cam_off[c + 1];
+    int j = d.cam_off[c] + threadIdx.x;
+    DirIn<Real> in{};
+    if (j < j_end) load_dir<Real>(d, j, in);
+    double beta;
+    if (pcg_beta(d, it, lds, beta)) return;
+    if (threadIdx.x < 9) {
+        const size_t i = 9 * (size_t)c + threadIdx.x;
+        const double v = d.pz[i] + beta * d.pp[i];
+        d.pp[i] = v;
+        vsh[threadIdx.x] = d.sc[i] * v;
+    }
+    __syncthreads();
+    const CameraPrep<Real> P = d.prep[c];
+    if (threadIdx.x == 0) rot_directional<Real>(P, vsh, rsh);
+    __syncthreads();
+    DirCam<Real> C;
+    load_dircam<Real>(P, vsh, rsh, C);
+    const Real zero[3] = {(Real)0, (Real)0, (Real)0};
+    for (; j < j_end; j += kWG) {
+        const DirIn<Real> cur = in;
+        if (j + kWG < j_end) load_dir<Real>(d, j + kWG, in);
+        const int slot = d.slot_of[j];
+        ProjLin<Real> L;
+        Real a[2];
+        dir_apply<Real>(P, C, cur.X, zero, L, a);
+        const Real ww = cur.w * cur.w;
+        a[0] *= ww;
+        a[1] *= ww;
+        Real g[3];
+        proj_apply_t<Real>(L, a, g);
+        R4<Real> u;  // R' g
+        u.x = C.R[0] * g[0] + C.R[3] * g[1] + C.R[6] * g[2];
+        u.y = C.R[1] * g[0] + C.R[4] * g[1] + C.R[7] * g[2];
+        u.z = C.R[2] * g[0] + C.R[5] * g[1] + C.R[8] * g[2];
+        u.w = (Real)0;
+        d.u[slot] = u;
+    }
+}
+
+// q_c = D_c^2 p_c + S_c sum Jc' e_j with e_j = Jc S_c p_c - Jp zs_p; partial sum |e|^2 + p'D_c^2 p.
+// Per observation only g = Jc-frame D'(w e) is formed; the camera accumulates G = sum g X', sum g and
+// the intrinsic sums, and contracts G with dR/dw once at the end.
+template <typename Real>
+__global__ void __launch_bounds__(kWG) k_jz_dir(Dev<Real> d) {
+    LMState* st = d.st;
+    if (st->done || st->step_invalid || st->pcg_done) return;
+    __shared__ double lds[16 * kWaves];
+    __shared__ double psh[9], vsh[9], rsh[9], qsh[9];
+    const int c = xcd_remap(blockIdx.x, d.nc);
+    if (c >= d.nc) return;
+    const int j_end = d.cam_off[c + 1];
+    int j = d.cam_off[c] + threadIdx.x;
+    DirIn<Real> in{};
+    R4<Real> zin{};
+    if (j < j_end) {
+        load_dir<Real>(d, j, in);
+        zin = d.zs[in.p];
+    }
+    if (threadIdx.x < 9) {
+        const size_t i = 9 * (size_t)c + threadIdx.x;
+        psh[threadIdx.x] = d.pp[i];
+        vsh[threadIdx.x] = d.sc[i] * d.pp[i];
+    }
+    __syncthreads();
+    const CameraPrep<Real> P = d.prep[c];
+    if (threadIdx.x == 0) rot_directional<Real>(P, vsh, rsh);
+    __syncthreads();
+    DirCam<Real> C;
+    load_dircam<Real>(P, vsh, rsh, C);
+    double acc[16];  // G (row-major 3x3), sum g, intrinsic sums (dist, r2, r2^2 times x'e0 + y'e1), |e|^2
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0.0;
+    for (; j < j_end; j += kWG) {
+        const DirIn<Real> cur = in;
+        const R4<Real> z = zin;
+        if (j + kWG < j_end) {
+            load_dir<Real>(d, j + kWG, in);
+            zin = d.zs[in.p];
+        }
+        const Real zR[3] = {C.R[0] * z.x + C.R[1] * z.y + C.R[2] * z.z, C.R[3] * z.x + C.R[4] * z.y + C.R[5] * z.z,
+                            C.R[6] * z.x + C.R[7] * z.y + C.R[8] * z.z};
+        ProjLin<Real> L;
+        Real a[2];
+        dir_apply<Real>(P, C, cur.X, zR, L, a);
+        const Real e0 = cur.w * a[0], e1 = cur.w * a[1];
+        acc[15] += (double)e0 * (double)e0 + (double)e1 * (double)e1;
+        const Real ew[2] = {cur.w * e0, cur.w * e1};
+        Real g[3];
+        proj_apply_t<Real>(L, ew, g);
+        const double X64[3] = {cur.X.x, cur.X.y, cur.X.z};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double gi = (double)g[i];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) acc[3 * i + m] += gi * X64[m];
+            acc[9 + i] += gi;
+        }
+        const double se = (double)(L.xp * ew[0] + L.yp * ew[1]);
+        const double r2 = (double)L.r2;
+        acc[12] += (double)L.dist * se;
+        acc[13] += r2 * se;
+        acc[14] += r2 * r2 * se;
+    }
+    block_sum<16>(acc, lds);
+    const int t = threadIdx.x;
+    if (t == 0) {
+        double G[9], w3[3];
+        for (int k = 0; k < 9; ++k) G[k] = lds[k * kWaves];
+        rot_contract<Real>(P, G, w3);
+        const double f = (double)P.f;
+        const double qv[9] = {w3[0], w3[1], w3[2], lds[9 * kWaves], lds[10 * kWaves], lds[11 * kWaves],
+                              lds[12 * kWaves], f * lds[13 * kWaves], f * lds[14 * kWaves]};
+        for (int k = 0; k < 9; ++k) qsh[k] = qv[k];
+    }
+    __syncthreads();
+    if (t < 9) {
+        const size_t i = 9 * (size_t)c + t;
+        d.qc[i] = d.dc2[i] * psh[t] + d.sc[i] * qsh[t];
+    }
+    if (t == 0) {
+        double s = lds[15 * kWaves];
         for (int i = 0; i < 9; ++i) s += psh[i] * d.dc2[9 * (size_t)c + i] * psh[i];
         d.cam_part[PS_A * (size_t)d.nc + c] = s;
     }

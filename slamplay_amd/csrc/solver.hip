@@ -211,6 +211,7 @@ struct Solver final : SolverBase {
     DevBuf<int> cam_off, co_pt, chunk_off, chunk_len, slot_of, co_side;
     DevBuf<double> co_uv64;
     DevBuf<Real> co_uvr;
+    DevBuf<Real> co_w;  // Huber weight per observation at the linearization (direction-form PCG)
     DevBuf<double> cam_x, pt_x, cam_x0, pt_x0, cam_c, pt_c, dcam, dpt;
     DevBuf<CameraPrep<Real>> prep;
     DevBuf<CameraPrep<double>> prep_c;
@@ -252,7 +253,7 @@ struct Solver final : SolverBase {
     int64_t device_bytes() const override {
         int64_t b = 0;
         b += cam_off.bytes() + co_pt.bytes() + chunk_off.bytes() + chunk_len.bytes() + slot_of.bytes();
-        b += co_uv64.bytes() + co_uvr.bytes();
+        b += co_uv64.bytes() + co_uvr.bytes() + co_w.bytes();
         b += cam_x.bytes() * 4 + pt_x.bytes() * 4 + dcam.bytes() + dpt.bytes();
         b += prep.bytes() + prep_c.bytes();
         b += M.bytes() * 2 + gc.bytes() * 10 + upd_part.bytes() + rz_hist.bytes() + dp2.bytes();
@@ -320,6 +321,7 @@ struct Solver final : SolverBase {
         BA_TRY(VinvR.alloc(6 * (size_t)np));
         BA_TRY(qp.alloc(np));
         BA_TRY(zs.alloc(np));
+        BA_TRY(co_w.alloc((size_t)std::max(no, 1)));
         // slot-order buffers: padding slots are never written and must read as zero
         const size_t nslots = (size_t)std::max(L.chunk_off[L.n_chunks], 1);
         BA_TRY(pv.alloc(3 * nslots));
@@ -356,6 +358,7 @@ struct Solver final : SolverBase {
             d.rig = *stereo->rig;
         }
         d.co_uv = sizeof(Real) == sizeof(double) ? (const Real*)co_uv64.p : co_uvr.p;
+        d.co_w = co_w.p;
         d.chunk_off = chunk_off.p;
         d.chunk_len = chunk_len.p;
         d.slot_of = slot_of.p;
@@ -493,10 +496,18 @@ struct Solver final : SolverBase {
             dense.enqueue_solve(*this);
         } else {
             launch("cam_factor", k_cam_factor<Real>, d.n_upd, kWG, d);
+            // the Snavely model's products in direction form; the g2o and stereo models form J
+            const bool dir = d.model == kModelSnavely;
             for (int it = 1; it <= opt.pcg_max_iterations; ++it) {
-                launch("jtx", BA_MODEL_KERNEL(k_jtx), gcam, kWG, d, 1, it, (const double*)d.pp);
+                if (dir)
+                    launch("jtx", k_jtx_dir<Real>, gcam, kWG, d, it);
+                else
+                    launch("jtx", BA_MODEL_KERNEL(k_jtx), gcam, kWG, d, 1, it, (const double*)d.pp);
                 launch("pt_z", k_pt_z<Real>, gpt, kWG, d);
-                launch("jz", BA_MODEL_KERNEL(k_jz), gcam, kWG, d);
+                if (dir)
+                    launch("jz", k_jz_dir<Real>, gcam, kWG, d);
+                else
+                    launch("jz", BA_MODEL_KERNEL(k_jz), gcam, kWG, d);
                 launch("cam_update", k_cam_update<Real>, d.n_upd, kWG, d, it);
             }
             // the convergence test after the last update (sets the iteration count)

@@ -160,6 +160,17 @@ int ba_eval_residual_jacobian(const ba_problem_view* problem, int32_t precision,
 /* 1/2 sum_i rho(|r_i|^2) on the GPU in FP64 (loss = ba_loss, scale = Huber a). */
 int ba_eval_cost(const ba_problem_view* problem, int32_t loss, double loss_scale, double* cost);
 
+/* ba_eval_cost without any device allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Every pointer of `problem` and `cost` (one double) is DEVICE memory on
+ * the current device; the kernels of ba_eval_cost are enqueued on `stream` (a hipStream_t, NULL for
+ * the null stream) and the call returns without waiting.  Work space is a fixed per-GPU buffer of
+ * the library (about 1.8 MB, held until the process ends): at most 4096 cameras and 2^24
+ * observations (BA_ERR_CAPACITY beyond).  Indices are not checked on the host; an out-of-range
+ * index makes the cost NaN.  Calls that overlap on one GPU share the buffer and race. */
+int ba_eval_cost_noalloc(const ba_problem_view* problem, int32_t loss, double loss_scale, double* cost,
+                         void* stream);
+
 /* Full solve: upload, device-resident LM loop, write parameters back in place.
  * Replaces SolveBA(BALProblem&) of test_bundle_adjustment_ceres.cpp:58-101. summary may be NULL. */
 int ba_solve(const ba_problem_view* problem, const ba_options* options, ba_summary* summary);

@@ -92,6 +92,8 @@ SIGNATURES = {
     "ba_device_count": (ctypes.c_int, [i32p]),
     "ba_eval_residual_jacobian": (ctypes.c_int, [ctypes.POINTER(ProblemView), i32, f64p, f64p, f64p]),
     "ba_eval_cost": (ctypes.c_int, [ctypes.POINTER(ProblemView), i32, f64, f64p]),
+    # device pointers throughout (the view's arrays, the one-double cost) and a hipStream_t
+    "ba_eval_cost_noalloc": (ctypes.c_int, [ctypes.POINTER(ProblemView), i32, f64, vp, vp]),
     "ba_solve": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.POINTER(Options), ctypes.POINTER(Summary)]),
     "ba_solver_create": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.POINTER(Options), ctypes.POINTER(vp)]),
     "ba_solver_run": (ctypes.c_int, [vp, ctypes.POINTER(Summary)]),

@@ -4,6 +4,9 @@
 //
 // Layout: observations in caller order; cameras AoS (9 doubles) as in BALProblem.  One thread per
 // observation, camera prep computed once per camera by k_camera_prep_aos.
+#include <map>
+#include <mutex>
+
 #include "common.h"
 #include "snavely.h"
 
@@ -65,6 +68,36 @@ __global__ void k_cost_obs(int no, const int* __restrict__ ci, const int* __rest
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// k_cost_obs for indices that the host has not validated (their arrays live in device memory):
+// an observation whose camera or point index is out of range contributes NaN, so the cost is NaN.
+__global__ void k_cost_obs_checked(int no, int nc, int np, const int* __restrict__ ci, const int* __restrict__ pi,
+                                   const double* __restrict__ obs, const double* __restrict__ pts,
+                                   const CameraPrep<double>* __restrict__ prep, int loss, double a,
+                                   double* __restrict__ partial) {
+    __shared__ double red[256];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double c = 0.0;
+    if (k < no) {
+        const int cam = ci[k], p = pi[k];
+        if (cam < 0 || cam >= nc || p < 0 || p >= np) {
+            c = __builtin_nan("");
+        } else {
+            const CameraPrep<double> P = prep[cam];
+            const double X[3] = {pts[3 * (size_t)p], pts[3 * (size_t)p + 1], pts[3 * (size_t)p + 2]};
+            double r[2];
+            snavely_residual<double>(P, X, obs[2 * (size_t)k], obs[2 * (size_t)k + 1], r);
+            c = half_rho(r[0] * r[0] + r[1] * r[1], loss, a);
+        }
+    }
+    red[threadIdx.x] = c;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
 __global__ void k_sum_partials(const double* __restrict__ partial, int n, double* __restrict__ out) {
     __shared__ double red[256];
     double s = 0.0;
@@ -78,7 +111,40 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int n, double
     if (threadIdx.x == 0) *out = red[0];
 }
 
+// ---------------------------------------------------------------------------------------------
+// Allocation-free cost evaluation (ba_eval_cost_noalloc): the kernels of ba_eval_cost with their
+// work space (camera preps, per-block partial sums) in a file-scope device array instead of memory
+// the library allocates.  Each GPU holds its own copy until the process ends; host code uses the
+// address hipGetSymbolAddress returns for the current device, never the array's name.
+constexpr int kNoAllocMaxCameras = 4096;
+constexpr int kNoAllocMaxObservations = 1 << 24;
+constexpr size_t kNoAllocPrepBytes = sizeof(CameraPrep<double>) * (size_t)kNoAllocMaxCameras;
+constexpr size_t kNoAllocPartials = (size_t)kNoAllocMaxObservations / 256;
+constexpr size_t kNoAllocScratchBytes = kNoAllocPrepBytes + kNoAllocPartials * sizeof(double);
+static_assert(kNoAllocPrepBytes % 256 == 0, "partial sums start 256-byte aligned");
+
+__device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
 namespace {
+
+// Device address of `scratch` on the current device, looked up once per device.
+int noalloc_scratch(char** out) {
+    static std::mutex mu;
+    static std::map<int, char*> addr;  // device -> address of its copy of scratch
+    int dev = 0;
+    BA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(dev);
+    if (it == addr.end()) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(BA_ERR_HIP, "scratch buffer address on device %d: %s", dev, hipGetErrorString(e));
+        it = addr.emplace(dev, static_cast<char*>(ptr)).first;
+    }
+    *out = it->second;
+    return BA_OK;
+}
 
 struct EvalUpload {
     DevBuf<int> ci, pi;
@@ -167,5 +233,33 @@ extern "C" int ba_eval_cost(const ba_problem_view* v, int32_t loss, double loss_
     BA_HIP(hipGetLastError());
     BA_HIP(hipMemcpyAsync(cost, out.p, sizeof(double), hipMemcpyDeviceToHost, g.s));
     BA_HIP(hipStreamSynchronize(g.s));
+    return BA_OK;
+}
+
+extern "C" int ba_eval_cost_noalloc(const ba_problem_view* v, int32_t loss, double loss_scale, double* cost,
+                                    void* stream) {
+    clear_error();
+    if (!v || !cost) return fail(BA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (v->camera_block_size != 9) return fail(BA_ERR_INVALID_ARGUMENT, "camera_block_size must be 9");
+    const int no = v->num_observations, nc = v->num_cameras, np = v->num_points;
+    if (no < 0 || nc < 0 || np < 0) return fail(BA_ERR_INVALID_ARGUMENT, "negative problem size");
+    if (nc > kNoAllocMaxCameras || no > kNoAllocMaxObservations)
+        return fail(BA_ERR_CAPACITY, "ba_eval_cost_noalloc holds at most %d cameras and %d observations (got %d, %d)",
+                    kNoAllocMaxCameras, kNoAllocMaxObservations, nc, no);
+    if (no > 0 && (!v->camera_index || !v->point_index || !v->observations || !v->points))
+        return fail(BA_ERR_INVALID_ARGUMENT, "observation or point arrays are NULL");
+    if (nc > 0 && !v->cameras) return fail(BA_ERR_INVALID_ARGUMENT, "camera array is NULL");
+    char* base = nullptr;
+    BA_TRY(noalloc_scratch(&base));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    auto* prep = reinterpret_cast<CameraPrep<double>*>(base);
+    auto* partial = reinterpret_cast<double*>(base + kNoAllocPrepBytes);
+    const int nb = no > 0 ? div_up(no, 256) : 1;
+    if (nc > 0)
+        hipLaunchKernelGGL(k_camera_prep_aos<double>, dim3(div_up(nc, 64)), dim3(64), 0, s, v->cameras, nc, prep);
+    hipLaunchKernelGGL(k_cost_obs_checked, dim3(nb), dim3(256), 0, s, no, nc, np, v->camera_index, v->point_index,
+                       v->observations, v->points, (const CameraPrep<double>*)prep, loss, loss_scale, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, (const double*)partial, nb, cost);
+    BA_HIP(hipGetLastError());
     return BA_OK;
 }

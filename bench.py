@@ -37,8 +37,8 @@ HBM_PEAK_GBS = 8000.0
 LM_ITERS = 30
 # HBM bytes per launch of the PCG kernels at this workload (mixed precision), measured with
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by a 1 GiB copy's counts
-# (profiles/c3_mixed_pcg_kernels.md)
-MEASURED_TRAFFIC_BYTES = {"jz": 20.78e6, "jtx": 42.27e6, "pt_z": 37.04e6}
+# (profiles/c3_mixed_pcg_kernels.md); empty until measured for the current kernels
+MEASURED_TRAFFIC_BYTES = {}
 
 
 _T0 = time.perf_counter()
@@ -226,15 +226,10 @@ def bench_c3(S, args):
                    final_cost_rel_diff=abs(out_mixed["final_cost"] - out64["final_cost"]) / out64["final_cost"])
         log("bench: fp64 %.1f LM iters/s, final cost %.9e" % (fp64_ips, out64["final_cost"]))
         s64.close()
-    # dominant kernel by total time among those with stated algorithmic bytes, launches of one kernel
-    # function under several names counted together (k_jtx runs as "jtx" in the PCG and as "jtx_y"
-    # once per LM iteration), as the rocprofv3 summary counts them
+    # dominant kernel by total time among those with stated algorithmic bytes (each name is one
+    # kernel function here: the PCG's "jtx" is k_jtx_task, the back-substitution's "jtx_y" k_jtx)
     cands = {k: v for k, v in kt.items() if v["bytes_per_launch"] > 0 and v["launches"] > 0}
-    same_kernel = {"jtx_y": "jtx"}
-    by_kernel = {}
-    for k, v in kt.items():
-        by_kernel[same_kernel.get(k, k)] = by_kernel.get(same_kernel.get(k, k), 0.0) + v["total_ms"]
-    dom = max(cands, key=lambda k: by_kernel.get(k, 0.0)) if cands else None
+    dom = max(cands, key=lambda k: cands[k]["total_ms"]) if cands else None
     roofline = None
     if dom:
         v = cands[dom]

@@ -298,46 +298,73 @@ constexpr int kSolveRows = kSolveBlk * kTile;
 // Tiles of block K: [kSolveBlk K, min(kSolveBlk (K + 1), nt)).
 __device__ __forceinline__ int solve_block_tiles(int K, int nt) { return min(kSolveBlk, nt - kSolveBlk * K); }
 
+// One workgroup per (block K, column tile j): column j of X = L_KK^-1, i.e. X_jj = Dinv_j and
+// X_ij = -Dinv_i sum_{j<=k<i} L_ik X_kj for i = j+1, ... in order, with the column's tiles, the
+// current factor tile and the partial sum staged in LDS (FP64).  X is stored row-major and, for the
+// backward sweep, transposed (XT), so both sweeps read rows.
 template <typename F>
 __global__ void __launch_bounds__(256) k_block_inverse(const F* __restrict__ A, int ld, const F* __restrict__ Dinv,
-                                                        F* __restrict__ Binv, int nt, const LMState* st) {
+                                                        F* __restrict__ Binv, F* __restrict__ BinvT, int nt,
+                                                        const LMState* st) {
     if (st->done || st->step_invalid) return;
-    __shared__ double T[kTile][kTile + 1];
-    const int K = blockIdx.x, bt = solve_block_tiles(K, nt), t0 = kSolveBlk * K;
-    F* X = Binv + (size_t)K * kSolveRows * kSolveRows;  // row-major kSolveRows x kSolveRows
-    const int t = threadIdx.x;
-    // zero the block, then the diagonal tiles
-    for (int e = t; e < kSolveRows * kSolveRows; e += 256) X[e] = (F)0;
+    __shared__ double Xc[kSolveBlk][kTile][kTile + 1];
+    __shared__ double Lt[kTile][kTile + 1];
+    __shared__ double Tt[kTile][kTile + 1];
+    const int K = blockIdx.x / kSolveBlk, j = blockIdx.x % kSolveBlk;
+    const int bt = solve_block_tiles(K, nt), t0 = kSolveBlk * K;
+    const int tid = threadIdx.x;
+    F* X = Binv + (size_t)K * kSolveRows * kSolveRows;
+    F* XT = BinvT + (size_t)K * kSolveRows * kSolveRows;
+    // this workgroup's column: zero above the diagonal tile (and the whole column past the block)
+    for (int e = tid; e < kSolveRows * kTile; e += 256) {
+        const int r = e / kTile, c = kTile * j + e % kTile;
+        if (r < kTile * j || j >= bt || r >= kTile * bt) {
+            X[(size_t)r * kSolveRows + c] = (F)0;
+            XT[(size_t)c * kSolveRows + r] = (F)0;
+        }
+    }
+    if (j >= bt) return;
+    auto put = [&](int i, int r, int c, double v) {
+        Xc[i][r][c] = v;
+        X[(size_t)(kTile * i + r) * kSolveRows + kTile * j + c] = (F)v;
+        XT[(size_t)(kTile * j + c) * kSolveRows + kTile * i + r] = (F)v;
+    };
+    for (int e = tid; e < kTile * kTile; e += 256)
+        put(j, e / kTile, e % kTile, (double)Dinv[(size_t)(t0 + j) * kTile * kTile + e]);
     __syncthreads();
-    for (int i = 0; i < bt; ++i)
-        for (int e = t; e < kTile * kTile; e += 256)
-            X[(size_t)(kTile * i + e / kTile) * kSolveRows + kTile * i + e % kTile] =
-                Dinv[(size_t)(t0 + i) * kTile * kTile + e];
-    __syncthreads();
-    for (int i = 1; i < bt; ++i) {
-        for (int j = 0; j < i; ++j) {
-            // T = sum_{k=j}^{i-1} L_{t0+i, t0+k} X_kj
-            for (int e = t; e < kTile * kTile; e += 256) {
-                const int r = e / kTile, c = e % kTile;
-                double s = 0.0;
-                for (int k = j; k < i; ++k) {
-                    const F* Lr = A + (size_t)(kTile * (t0 + i) + r) * ld + kTile * (t0 + k);
-                    const F* Xc = X + (size_t)(kTile * k) * kSolveRows + kTile * j + c;
-                    for (int m = 0; m < kTile; ++m) s += (double)Lr[m] * (double)Xc[(size_t)m * kSolveRows];
-                }
-                T[r][c] = s;
-            }
+    for (int i = j + 1; i < bt; ++i) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = j; k < i; ++k) {
+            const F* Lik = A + (size_t)(kTile * (t0 + i)) * ld + kTile * (t0 + k);
+            for (int e = tid; e < kTile * kTile; e += 256) Lt[e / kTile][e % kTile] = (double)Lik[(size_t)(e / kTile) * ld + e % kTile];
             __syncthreads();
-            // X_ij = -Dinv_i T
-            const F* Di = Dinv + (size_t)(t0 + i) * kTile * kTile;
-            for (int e = t; e < kTile * kTile; e += 256) {
-                const int r = e / kTile, c = e % kTile;
-                double s = 0.0;
-                for (int m = 0; m < kTile; ++m) s += (double)Di[r * kTile + m] * T[m][c];
-                X[(size_t)(kTile * i + r) * kSolveRows + kTile * j + c] = (F)(-s);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int e = tid + 256 * q, r = e / kTile, c = e % kTile;
+                double sum = 0.0;
+#pragma unroll 8
+                for (int m = 0; m < kTile; ++m) sum += Lt[r][m] * Xc[k][m][c];
+                acc[q] += sum;
             }
             __syncthreads();
         }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + 256 * q;
+            Tt[e / kTile][e % kTile] = acc[q];
+        }
+        const F* Di = Dinv + (size_t)(t0 + i) * kTile * kTile;
+        for (int e = tid; e < kTile * kTile; e += 256) Lt[e / kTile][e % kTile] = (double)Di[e];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + 256 * q, r = e / kTile, c = e % kTile;
+            double sum = 0.0;
+#pragma unroll 8
+            for (int m = 0; m < kTile; ++m) sum += Lt[r][m] * Tt[m][c];
+            put(i, r, c, -sum);
+        }
+        __syncthreads();
     }
 }
 
@@ -406,7 +433,7 @@ __global__ void __launch_bounds__(256) k_trsv_fwd(const F* __restrict__ A, int l
 // rows above block K get block K+1's solution applied.  Block K's result also goes to x (added to
 // it when accumulate != 0), rows below n excluded.
 template <typename F>
-__global__ void __launch_bounds__(256) k_trsv_bwd(const F* __restrict__ A, int ld, const F* __restrict__ Binv, int nt,
+__global__ void __launch_bounds__(256) k_trsv_bwd(const F* __restrict__ A, int ld, const F* __restrict__ BinvT, int nt,
                                                   int K, double* __restrict__ z, double* __restrict__ x, int n,
                                                   int accumulate, const LMState* st) {
     if (st->done || st->step_invalid) return;
@@ -434,14 +461,18 @@ __global__ void __launch_bounds__(256) k_trsv_bwd(const F* __restrict__ A, int l
     __syncthreads();
     for (int r = t; r < rows_k; r += 256) zk[r] = z[kb + r];
     __syncthreads();
-    // x_K = X_K' z_K: one thread per column
-    const F* X = Binv + (size_t)K * kSolveRows * kSolveRows;
-    for (int c = t; c < rows_k; c += 256) {
+    // x_K = X_K' z_K (rows of the transposed inverse, one wave per row)
+    const F* XT = BinvT + (size_t)K * kSolveRows * kSolveRows;
+    const int wave = t >> 6, lane = t & 63;
+    for (int r = wave; r < rows_k; r += kWaves) {
         double s = 0.0;
-        for (int r = 0; r < rows_k; ++r) s += (double)X[(size_t)r * kSolveRows + c] * zk[r];
-        z[kb + c] = s;
-        const int row = kb + c;
-        if (row < n) x[row] = accumulate ? x[row] + s : s;
+        for (int c = lane; c < rows_k; c += 64) s += (double)XT[(size_t)r * kSolveRows + c] * zk[c];
+        s = wave_sum(s);
+        if (lane == 0) {
+            z[kb + r] = s;
+            const int row = kb + r;
+            if (row < n) x[row] = accumulate ? x[row] + s : s;
+        }
     }
 }
 
@@ -463,17 +494,23 @@ __global__ void k_set_linear_iters(LMState* st, int iters) {
     st->pcg_iters = iters;
 }
 
+// Reduced systems up to this size are factored in FP64 in both precisions: the factor costs a few
+// microseconds there, and an FP32 factor of a small, nearly gauge-free system (a local window whose
+// trust region has grown) would need a Krylov solve on top to reach the FP64 step.
+constexpr int kDenseF64MaxN = 288;
+
 template <typename Real>
 struct DenseSchur {
     int n = 0, ld = 0, nt = 0, nb = 0, nblocks = 0, refine = 0;
+    bool f64 = true;  // factor in FP64 (FP64 solver, or a small system)
     DevBuf<Real> js;
     DevBuf<int> blk_a, blk_b, blk_off, pair_i, pair_j;
     DevBuf<double> S, r, z;
-    DevBuf<float> Af, Dinvf, Binvf;
-    DevBuf<double> Dinvd, Binvd;
+    DevBuf<float> Af, Dinvf, Binvf, BinvTf;
+    DevBuf<double> Dinvd, Binvd, BinvTd;
     int64_t device_bytes() const {
         return js.bytes() + blk_a.bytes() * 3 + pair_i.bytes() * 2 + S.bytes() + r.bytes() + z.bytes() + Af.bytes() +
-               Dinvf.bytes() + Binvf.bytes() + Dinvd.bytes() + Binvd.bytes();
+               Dinvf.bytes() + Binvf.bytes() + BinvTf.bytes() + Dinvd.bytes() + Binvd.bytes() + BinvTd.bytes();
     }
 
     int create(const ba_problem_view* v, const Layout& L, const ba_options& o, hipStream_t s) {
@@ -483,7 +520,8 @@ struct DenseSchur {
         nt = std::max(1, (n + kTile - 1) / kTile);
         ld = nt * kTile;
         nb = (nt + kSolveBlk - 1) / kSolveBlk;
-        refine = sizeof(Real) == sizeof(double) ? 0 : std::max(0, o.refine_steps);
+        f64 = sizeof(Real) == sizeof(double) || n <= kDenseF64MaxN;
+        refine = f64 ? 0 : std::max(0, o.refine_steps);
         PairBlocks pb;
         build_pair_blocks(L, &pb);
         const std::vector<int>&ba = pb.a, &bb = pb.b, &boff = pb.off, &pi = pb.i, &pj = pb.j;
@@ -498,13 +536,15 @@ struct DenseSchur {
         BA_TRY(r.alloc(ld));
         BA_TRY(z.alloc(ld));
         const size_t binv = (size_t)nb * kSolveRows * kSolveRows;
-        if (sizeof(Real) == sizeof(double)) {
+        if (f64) {
             BA_TRY(Dinvd.alloc((size_t)nt * kTile * kTile));
             BA_TRY(Binvd.alloc(binv));
+            BA_TRY(BinvTd.alloc(binv));
         } else {
             BA_TRY(Af.alloc((size_t)ld * ld));
             BA_TRY(Dinvf.alloc((size_t)nt * kTile * kTile));
             BA_TRY(Binvf.alloc(binv));
+            BA_TRY(BinvTf.alloc(binv));
         }
         BA_HIP(hipStreamSynchronize(s));
         return BA_OK;
@@ -512,19 +552,19 @@ struct DenseSchur {
 
     // x = (L L')^-1 b (accumulate: x += ...) by the blocked sweeps: nb launches each way.
     template <typename F, typename SolverT>
-    void solve(SolverT& sv, const F* A, const F* Binv, const double* b, double* x, int accumulate) {
+    void solve(SolverT& sv, const F* A, const F* Binv, const F* BinvT, const double* b, double* x, int accumulate) {
         auto& d = sv.d;
         for (int K = 0; K < nb; ++K) {
             const int below = std::max(0, nt - kSolveBlk * (K + 1));
             sv.launch("trsv_fwd", k_trsv_fwd<F>, 1 + below, 256, A, ld, Binv, nt, K, b, n, z.p, (const LMState*)d.st);
         }
         for (int K = nb - 1; K >= 0; --K)
-            sv.launch("trsv_bwd", k_trsv_bwd<F>, 1 + kSolveBlk * K, 256, A, ld, Binv, nt, K, z.p, x, n, accumulate,
+            sv.launch("trsv_bwd", k_trsv_bwd<F>, 1 + kSolveBlk * K, 256, A, ld, BinvT, nt, K, z.p, x, n, accumulate,
                       (const LMState*)d.st);
     }
 
     template <typename F, typename SolverT>
-    void factor_and_solve(SolverT& sv, F* A, F* Dinv, F* Binv) {
+    void factor_and_solve(SolverT& sv, F* A, F* Dinv, F* Binv, F* BinvT) {
         auto& d = sv.d;
         sv.launch("potrf", k_potrf_tile<F>, 1, 64, A, ld, 0, Dinv, d.st);
         for (int k = 0; k + 1 < nt; ++k) {
@@ -532,13 +572,13 @@ struct DenseSchur {
             sv.launch("trsm", k_trsm_tiles<F>, m, 256, A, ld, k, (const F*)Dinv, (const LMState*)d.st);
             sv.launch("syrk", k_syrk_tiles<F>, m * (m + 1) / 2, 256, A, ld, k, Dinv, d.st);
         }
-        sv.launch("block_inverse", k_block_inverse<F>, nb, 256, (const F*)A, ld, (const F*)Dinv, Binv, nt,
-                  (const LMState*)d.st);
-        solve<F>(sv, A, Binv, (const double*)d.bc, d.px, 0);
+        sv.launch("block_inverse", k_block_inverse<F>, nb * kSolveBlk, 256, (const F*)A, ld, (const F*)Dinv, Binv, BinvT,
+                  nt, (const LMState*)d.st);
+        solve<F>(sv, A, Binv, BinvT, (const double*)d.bc, d.px, 0);
         for (int it = 0; it < refine; ++it) {
             sv.launch("residual", k_residual, div_up(n, kWaves), kWG, (const double*)S.p, ld, (const double*)d.bc,
                       (const double*)d.px, r.p, n, (const LMState*)d.st);
-            solve<F>(sv, A, Binv, (const double*)r.p, d.px, 1);
+            solve<F>(sv, A, Binv, BinvT, (const double*)r.p, d.px, 1);
         }
     }
 
@@ -554,12 +594,12 @@ struct DenseSchur {
         sv.launch("fill_S", k_fill_S, (int)((nn + 255) / 256), 256, S.p, n, ld, (const LMState*)d.st);
         sv.launch("schur_pairs", k_schur_pairs<Real>, nblocks, kWG, d, (const Real*)js.p, (const int*)blk_a.p,
                   (const int*)blk_b.p, (const int*)blk_off.p, (const int*)pair_i.p, (const int*)pair_j.p, S.p, ld);
-        if (sizeof(Real) == sizeof(double)) {
-            factor_and_solve<double>(sv, S.p, Dinvd.p, Binvd.p);
+        if (f64) {
+            factor_and_solve<double>(sv, S.p, Dinvd.p, Binvd.p, BinvTd.p);
         } else {
             sv.launch("convert_S", k_convert_S<float>, (int)((nn + 255) / 256), 256, (const double*)S.p, Af.p, nn,
                       (const LMState*)d.st);
-            factor_and_solve<float>(sv, Af.p, Dinvf.p, Binvf.p);
+            factor_and_solve<float>(sv, Af.p, Dinvf.p, Binvf.p, BinvTf.p);
         }
         sv.launch("set_iters", k_set_linear_iters, 1, 1, d.st, 1 + refine);
     }

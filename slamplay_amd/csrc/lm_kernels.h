@@ -450,6 +450,42 @@ __device__ __forceinline__ void chunk_sum(int len, size_t base, double (&acc)[N]
             for (int w = 0; w < kWaves - 1; ++w) acc[i] += lds[(w * N + i) * 64 + lane];
 }
 
+// chunk_sum over a stream of R4 records (x, y, z added), four rows in flight per lane: the rows of
+// a wave are added in the same order as chunk_sum adds them.
+template <typename Real>
+__device__ __forceinline__ void chunk_sum_r4(int len, size_t base, const R4<Real>* __restrict__ src, double (&acc)[3],
+                                             double* lds) {
+    constexpr int U = 4;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    acc[0] = acc[1] = acc[2] = 0.0;
+    int r = wave;
+    for (; r + (U - 1) * kWaves < len; r += U * kWaves) {
+        R4<Real> v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = src[base + 64 * (size_t)(r + u * kWaves) + lane];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            acc[0] += (double)v[u].x;
+            acc[1] += (double)v[u].y;
+            acc[2] += (double)v[u].z;
+        }
+    }
+    for (; r < len; r += kWaves) {
+        const R4<Real> v = src[base + 64 * (size_t)r + lane];
+        acc[0] += (double)v.x;
+        acc[1] += (double)v.y;
+        acc[2] += (double)v.z;
+    }
+    if (wave > 0)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) lds[((wave - 1) * 3 + i) * 64 + lane] = acc[i];
+    __syncthreads();
+    if (wave == 0)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            for (int w = 0; w < kWaves - 1; ++w) acc[i] += lds[(w * 3 + i) * 64 + lane];
+}
+
 // One workgroup per chunk: V_p = sum Jp'Jp, g_p = sum Jp'r per point.
 template <typename Real>
 __global__ void __launch_bounds__(kWG) k_lin_pt(Dev<Real> d) {
@@ -900,14 +936,7 @@ __global__ void __launch_bounds__(kWG) k_pt_z(Dev<Real> d) {
     const int ch = xcd_remap(blockIdx.x, d.n_chunks);
     if (ch >= d.n_chunks) return;
     double a[3];
-    chunk_sum<3>(d.chunk_len[ch], (size_t)d.chunk_off[ch], a,
-                 [&](size_t slot, double (&acc)[3]) {
-                     const R4<Real> u = d.u[slot];
-                     acc[0] += (double)u.x;
-                     acc[1] += (double)u.y;
-                     acc[2] += (double)u.z;
-                 },
-                 lds);
+    chunk_sum_r4<Real>(d.chunk_len[ch], (size_t)d.chunk_off[ch], d.u, a, lds);
     if (threadIdx.x >= 64) return;
     const int p = 64 * ch + threadIdx.x;
     double zdz = 0.0;
@@ -1003,10 +1032,15 @@ __device__ __forceinline__ void load_dir(const Dev<Real>& d, int j, DirIn<Real>&
 }
 
 // Per-camera constants of a direction-form product: dR_v for the scaled direction v = S_c p_c,
-// the rotation matrix in working precision and v's translation / intrinsic parts.
+// the rotation matrix in working precision, the camera centre c = -R't (FP64) and v's translation /
+// intrinsic parts.  The rotation columns are taken about the camera centre,
+//   dR_v X + v_t = dR_v (X - c) + (dR_v c + v_t),
+// with X - c and dR_v c + v_t formed in FP64: for a point near a camera far from the world origin,
+// dR_v X and v_t are large and nearly opposite, and in FP32 their sum would keep few digits.
 template <typename Real>
 struct DirCam {
     Real Rd[9], R[9], vt[3], vf, vk1, vk2;
+    double c[3];
 };
 
 template <typename Real>
@@ -1017,9 +1051,12 @@ __device__ __forceinline__ void load_dircam(const CameraPrep<Real>& P, const dou
         C.Rd[k] = (Real)rsh[k];
         C.R[k] = (Real)P.R64[k];
     }
-    C.vt[0] = (Real)vsh[3];
-    C.vt[1] = (Real)vsh[4];
-    C.vt[2] = (Real)vsh[5];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        C.c[i] = -(P.R64[i] * P.t64[0] + P.R64[3 + i] * P.t64[1] + P.R64[6 + i] * P.t64[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        C.vt[i] = (Real)(vsh[3 + i] + rsh[3 * i] * C.c[0] + rsh[3 * i + 1] * C.c[1] + rsh[3 * i + 2] * C.c[2]);
     C.vf = (Real)vsh[6];
     C.vk1 = (Real)vsh[7];
     C.vk2 = (Real)vsh[8];
@@ -1031,7 +1068,7 @@ __device__ __forceinline__ void dir_apply(const CameraPrep<Real>& P, const DirCa
                                           const Real zR[3], ProjLin<Real>& L, Real a[2]) {
     const double X64[3] = {X4.x, X4.y, X4.z};
     proj_lin<Real>(P, X64, L);
-    const Real X[3] = {(Real)X4.x, (Real)X4.y, (Real)X4.z};
+    const Real X[3] = {(Real)(X4.x - C.c[0]), (Real)(X4.y - C.c[1]), (Real)(X4.z - C.c[2])};
     Real q[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) q[i] = C.Rd[3 * i] * X[0] + C.Rd[3 * i + 1] * X[1] + C.Rd[3 * i + 2] * X[2] + C.vt[i] - zR[i];

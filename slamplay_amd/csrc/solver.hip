@@ -359,6 +359,7 @@ struct Solver final : SolverBase {
         }
         d.co_uv = sizeof(Real) == sizeof(double) ? (const Real*)co_uv64.p : co_uvr.p;
         d.co_w = co_w.p;
+
         d.chunk_off = chunk_off.p;
         d.chunk_len = chunk_len.p;
         d.slot_of = slot_of.p;
@@ -511,7 +512,10 @@ struct Solver final : SolverBase {
                 launch("cam_update", k_cam_update<Real>, d.n_upd, kWG, d, it);
             }
             // the convergence test after the last update (sets the iteration count)
-            launch("jtx", BA_MODEL_KERNEL(k_jtx), gcam, kWG, d, 1, opt.pcg_max_iterations + 1, (const double*)d.pp);
+            if (dir)
+                launch("jtx", k_jtx_dir<Real>, gcam, kWG, d, opt.pcg_max_iterations + 1);
+            else
+                launch("jtx", BA_MODEL_KERNEL(k_jtx), gcam, kWG, d, 1, opt.pcg_max_iterations + 1, (const double*)d.pp);
         }
         launch("jtx_y", BA_MODEL_KERNEL(k_jtx), gcam, kWG, d, 0, 0, (const double*)d.px);
         launch("pt_backsub", k_pt_backsub<Real>, gpt, kWG, d);
@@ -657,6 +661,12 @@ struct Solver final : SolverBase {
         const std::string n(name);
         // Algorithmic bytes of one launch: every stream counted once per pass, per-camera and per-point
         // parameters once per pass, FP64 parameters, int32 indices.
+        const bool dir = d.model == kModelSnavely && linear_solver == BA_SOLVER_ITERATIVE_SCHUR;
+        // direction form: point index, Huber weight, slot, u out (jtx); point index, weight, point
+        // and its zs (jz); per camera its prep and the 9-vectors read and written
+        const double prep_b = sizeof(CameraPrep<Real>);
+        if (n == "jtx" && dir) return no * (4 + R + 4 + 4 * R) + np * 4 * 8 + nc * (prep_b + 9 * 8 * 4 + 2 * 8 * d.n_upd);
+        if (n == "jz" && dir) return no * (4 + R) + np * (4 * 8 + 4 * R) + nc * (prep_b + 9 * 8 * 5);
         if (n == "jtx" || n == "jtx_y") return no * (4 + 4 + 2 * R + 4 * R) + np * 4 * 8 + nc * 9 * 8 * 3;
         if (n == "jz") return no * (4 + 2 * R) + np * (4 * 8 + 4 * R) + nc * 9 * 8 * 8;
         if (n == "pt_z") return no * 4 * R + np * (3 * 8 + 3 * 8 + 6 * R + 4 * R);

@@ -80,6 +80,13 @@ def test_config2_dense_mixed_final_cost(traces):
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", [0, 1])
 def test_config3_pcg_final_cost(traces, precision):
+    """Configuration 3 as benchmarked: PCG stopped at 50 iterations (tolerance 1e-6, never reached).
+    A truncated Krylov step depends on every rounding inside the iteration, and so does the LM path
+    after it: the FP64 oracle itself, re-run seven times with a single camera parameter or point
+    coordinate moved by one ulp, ends between 6.3e-8 and 1.2e-5 away from its own final cost (per-row
+    costs up to 3.2e-3 apart).  The final cost is therefore held to 5e-5 of the oracle's in both
+    precisions -- a few times the spread the oracle shows for one-ulp input changes -- and the
+    per-iteration parity bars are checked on the near-exact variant below."""
     case = traces["config3_pcg"]
     out = _solve(case, precision)
     n, worst = _leading_agreement(out, case)
@@ -87,15 +94,17 @@ def test_config3_pcg_final_cost(traces, precision):
     print("config 3 precision %d: %d rows agree (worst %.2e), final %.9e vs %.9e (%.2e)" %
           (precision, n, worst, out["final_cost"], case["final_cost"], rel))
     assert len(out["trace"]) == case["lm_iterations"] + 1
-    assert rel < (1e-6 if precision == 0 else 1e-5)
+    assert rel < 5e-5
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", [0, 1])
 def test_config3_near_exact_steps(traces, precision):
     """Configuration 3 with near-exact linear solves (PCG to 1e-10): the steps no longer depend on
-    where a truncated Krylov iteration happens to stop, so the FP64 trajectory follows the oracle's
-    per iteration and the final costs agree to 1e-6 (FP64) / 1e-5 (mixed)."""
+    where a truncated Krylov iteration happens to stop, so the FP64 run takes the oracle's decision
+    at every iteration with accepted costs within 1e-9 (a rejected trial's cost, evaluated at a
+    point far from the minimum, within 1e-4), and the final costs agree to 1e-6 (FP64) / 1e-5
+    (mixed)."""
     case = traces["config3_pcg_tight"]
     out = _solve(case, precision)
     n, worst = _leading_agreement(out, case)
@@ -103,5 +112,8 @@ def test_config3_near_exact_steps(traces, precision):
     print("config 3 near-exact, precision %d: %d rows agree (worst %.2e), final %.9e vs %.9e (%.2e)" %
           (precision, n, worst, out["final_cost"], case["final_cost"], rel))
     if precision == 0:
-        assert n == len(case["trace"]) and worst < 1e-9
+        assert n == len(case["trace"])
+        for a, b in zip(out["trace"], case["trace"]):
+            tol = 1e-9 if b["step_is_successful"] else 1e-4
+            assert abs(a["cost"] - b["cost"]) <= tol * b["cost"], (a, b)
     assert rel < (1e-6 if precision == 0 else 1e-5)

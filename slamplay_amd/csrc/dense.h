@@ -368,76 +368,103 @@ __global__ void __launch_bounds__(256) k_block_inverse(const F* __restrict__ A, 
     }
 }
 
-// z[rows] -= M[rows][cols] v for `nrows` rows starting at row r0 of a row-major matrix M (leading
-// dimension ldm, column offset c0), v = z[v0 .. v0 + ncols) in double; one wave per row.
-template <typename F>
-__device__ __forceinline__ void rows_minus_mv(const F* __restrict__ M, size_t ldm, int r0, int nrows, int c0,
-                                              int ncols, const double* __restrict__ v, double* __restrict__ z) {
+// Row dot products out(rr, dot(M[r0 + rr][c0 .. c0 + ncols), v)) for rr < nrows of a row-major M
+// (leading dimension ldm), ncols <= 128, v in double.  One wave per row: lane l takes columns l and
+// l + 64, the wave sums; a wave takes kRowBatch rows at a time with all their loads in flight
+// before the first sum, so the rows do not wait on each other's loads.
+constexpr int kRowBatch = 16;
+
+template <typename F, typename Out>
+__device__ __forceinline__ void rows_dot(const F* __restrict__ M, size_t ldm, int r0, int nrows, int c0, int ncols,
+                                         const double* __restrict__ v, Out out) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int rr = wave; rr < nrows; rr += kWaves) {
-        const F* row = M + (size_t)(r0 + rr) * ldm + c0;
-        double s = 0.0;
-        for (int c = lane; c < ncols; c += 64) s += (double)row[c] * v[c];
-        s = wave_sum(s);
-        if (lane == 0) z[r0 + rr] -= s;
+    const double v0 = lane < ncols ? v[lane] : 0.0, v1 = lane + 64 < ncols ? v[lane + 64] : 0.0;
+    // column and row indices are clamped into the block instead of branching around the loads (a
+    // branch per load makes every load wait for its own result); clamped columns carry v = 0 and
+    // clamped rows are not written
+    const int ca = min(lane, ncols - 1), cb = min(lane + 64, ncols - 1);
+    const int nw = blockDim.x >> 6;
+    for (int rb = wave * kRowBatch; rb < nrows; rb += nw * kRowBatch) {
+        F x0[kRowBatch], x1[kRowBatch];
+#pragma unroll
+        for (int g = 0; g < kRowBatch; ++g) {
+            const F* row = M + (size_t)(r0 + min(rb + g, nrows - 1)) * ldm + c0;
+            x0[g] = row[ca];
+            x1[g] = row[cb];
+        }
+        double s[kRowBatch];
+#pragma unroll
+        for (int g = 0; g < kRowBatch; ++g) {
+            double a = (double)x0[g] * v0;
+            a += (double)x1[g] * v1;
+            s[g] = a;
+        }
+#pragma unroll
+        for (int g = 0; g < kRowBatch; ++g) s[g] = wave_sum(s[g]);
+        if (lane == 0)
+#pragma unroll
+            for (int g = 0; g < kRowBatch; ++g)
+                if (rb + g < nrows) out(rb + g, s[g]);
     }
 }
 
-// Forward sweep, launch K (see above).  z has ld entries; launch 0 copies b into z first.
+// z[r0 + rr] -= M[r0 + rr][c0 ..] v for rr < nrows.
 template <typename F>
-__global__ void __launch_bounds__(256) k_trsv_fwd(const F* __restrict__ A, int ld, const F* __restrict__ Binv, int nt,
-                                                  int K, const double* __restrict__ b, int n, double* __restrict__ z,
-                                                  const LMState* st) {
+__device__ __forceinline__ void rows_minus_mv(const F* __restrict__ M, size_t ldm, int r0, int nrows, int c0,
+                                              int ncols, const double* __restrict__ v, double* __restrict__ z) {
+    rows_dot<F>(M, ldm, r0, nrows, c0, ncols, v, [&](int rr, double s) { z[r0 + rr] -= s; });
+}
+
+// Forward sweep, launch K (see above).  z has ld entries; launch 0 copies b into z first.  The
+// diagonal workgroup keeps block K's right-hand side in LDS between its two products.
+constexpr int kSolveThreads = 512;
+
+template <typename F>
+__global__ void __launch_bounds__(kSolveThreads) k_trsv_fwd(const F* __restrict__ A, int ld, const F* __restrict__ Binv,
+                                                            int nt, int K, const double* __restrict__ b, int n,
+                                                            double* __restrict__ z, const LMState* st) {
     if (st->done || st->step_invalid) return;
     __shared__ double v[kSolveRows];
+    __shared__ double zk[kSolveRows];
     const int t = threadIdx.x, g = blockIdx.x;
     const int kb = kSolveRows * K;                       // first row of block K
     const int rows_k = kTile * solve_block_tiles(K, nt);  // rows of block K
+    const int pb = kb - kSolveRows;                      // block K-1 (always full when K > 0)
     if (g > 0) {
         // one tile row below block K
         const int r0 = kb + rows_k + kTile * (g - 1);
         if (K == 0) {
-            for (int r = t; r < kTile; r += 256) z[r0 + r] = r0 + r < n ? b[r0 + r] : 0.0;
+            for (int r = t; r < kTile; r += blockDim.x) z[r0 + r] = r0 + r < n ? b[r0 + r] : 0.0;
             return;
         }
-        const int pb = kb - kSolveRows;  // block K-1 (always full)
-        for (int c = t; c < kSolveRows; c += 256) v[c] = z[pb + c];
+        for (int c = t; c < kSolveRows; c += blockDim.x) v[c] = z[pb + c];
         __syncthreads();
         rows_minus_mv<F>(A, (size_t)ld, r0, kTile, pb, kSolveRows, v, z);
         return;
     }
-    __shared__ double zk[kSolveRows];
-    if (K == 0) {
-        for (int r = t; r < rows_k; r += 256) z[r] = r < n ? b[r] : 0.0;
-    } else {
-        const int pb = kb - kSolveRows;
-        for (int c = t; c < kSolveRows; c += 256) v[c] = z[pb + c];
+    for (int r = t; r < rows_k; r += blockDim.x) zk[r] = K == 0 ? (r < n ? b[r] : 0.0) : z[kb + r];
+    if (K > 0)
+        for (int c = t; c < kSolveRows; c += blockDim.x) v[c] = z[pb + c];
+    __syncthreads();
+    if (K > 0) {
+        rows_dot<F>(A, (size_t)ld, kb, rows_k, pb, kSolveRows, v, [&](int rr, double s) { zk[rr] -= s; });
         __syncthreads();
-        rows_minus_mv<F>(A, (size_t)ld, kb, rows_k, pb, kSolveRows, v, z);
     }
-    __syncthreads();
-    for (int r = t; r < rows_k; r += 256) zk[r] = z[kb + r];
-    __syncthreads();
-    // y_K = X_K z_K (one wave per row)
+    // y_K = X_K z_K
     const F* X = Binv + (size_t)K * kSolveRows * kSolveRows;
-    const int wave = t >> 6, lane = t & 63;
-    for (int r = wave; r < rows_k; r += kWaves) {
-        double s = 0.0;
-        for (int c = lane; c < rows_k; c += 64) s += (double)X[(size_t)r * kSolveRows + c] * zk[c];
-        s = wave_sum(s);
-        if (lane == 0) z[kb + r] = s;
-    }
+    rows_dot<F>(X, (size_t)kSolveRows, 0, rows_k, 0, rows_k, zk, [&](int r, double s) { z[kb + r] = s; });
 }
 
 // Backward sweep, launch K (blocks from the last down to 0): x_K = X_K' (z_K - U_{K,K+1} x_{K+1});
 // rows above block K get block K+1's solution applied.  Block K's result also goes to x (added to
 // it when accumulate != 0), rows below n excluded.
 template <typename F>
-__global__ void __launch_bounds__(256) k_trsv_bwd(const F* __restrict__ A, int ld, const F* __restrict__ BinvT, int nt,
-                                                  int K, double* __restrict__ z, double* __restrict__ x, int n,
-                                                  int accumulate, const LMState* st) {
+__global__ void __launch_bounds__(kSolveThreads) k_trsv_bwd(const F* __restrict__ A, int ld, const F* __restrict__ BinvT,
+                                                            int nt, int K, double* __restrict__ z, double* __restrict__ x,
+                                                            int n, int accumulate, const LMState* st) {
     if (st->done || st->step_invalid) return;
     __shared__ double v[kSolveRows];
+    __shared__ double zk[kSolveRows];
     const int t = threadIdx.x, g = blockIdx.x;
     const int nb = (nt + kSolveBlk - 1) / kSolveBlk;
     const int kb = kSolveRows * K, rows_k = kTile * solve_block_tiles(K, nt);
@@ -447,33 +474,26 @@ __global__ void __launch_bounds__(256) k_trsv_bwd(const F* __restrict__ A, int l
     if (g > 0) {
         if (!has_next) return;
         const int r0 = kTile * (g - 1);  // a tile row above block K
-        for (int c = t; c < cols_next; c += 256) v[c] = z[nbk + c];
+        for (int c = t; c < cols_next; c += blockDim.x) v[c] = z[nbk + c];
         __syncthreads();
         rows_minus_mv<F>(A, (size_t)ld, r0, kTile, nbk, cols_next, v, z);
         return;
     }
-    __shared__ double zk[kSolveRows];
+    for (int r = t; r < rows_k; r += blockDim.x) zk[r] = z[kb + r];
+    if (has_next)
+        for (int c = t; c < cols_next; c += blockDim.x) v[c] = z[nbk + c];
+    __syncthreads();
     if (has_next) {
-        for (int c = t; c < cols_next; c += 256) v[c] = z[nbk + c];
+        rows_dot<F>(A, (size_t)ld, kb, rows_k, nbk, cols_next, v, [&](int rr, double s) { zk[rr] -= s; });
         __syncthreads();
-        rows_minus_mv<F>(A, (size_t)ld, kb, rows_k, nbk, cols_next, v, z);
     }
-    __syncthreads();
-    for (int r = t; r < rows_k; r += 256) zk[r] = z[kb + r];
-    __syncthreads();
-    // x_K = X_K' z_K (rows of the transposed inverse, one wave per row)
+    // x_K = X_K' z_K (rows of the transposed inverse)
     const F* XT = BinvT + (size_t)K * kSolveRows * kSolveRows;
-    const int wave = t >> 6, lane = t & 63;
-    for (int r = wave; r < rows_k; r += kWaves) {
-        double s = 0.0;
-        for (int c = lane; c < rows_k; c += 64) s += (double)XT[(size_t)r * kSolveRows + c] * zk[c];
-        s = wave_sum(s);
-        if (lane == 0) {
-            z[kb + r] = s;
-            const int row = kb + r;
-            if (row < n) x[row] = accumulate ? x[row] + s : s;
-        }
-    }
+    rows_dot<F>(XT, (size_t)kSolveRows, 0, rows_k, 0, rows_k, zk, [&](int r, double s) {
+        z[kb + r] = s;
+        const int row = kb + r;
+        if (row < n) x[row] = accumulate ? x[row] + s : s;
+    });
 }
 
 // r = b - S y (FP64, S symmetric with its full n x n block stored); one wave per row.
@@ -556,11 +576,12 @@ struct DenseSchur {
         auto& d = sv.d;
         for (int K = 0; K < nb; ++K) {
             const int below = std::max(0, nt - kSolveBlk * (K + 1));
-            sv.launch("trsv_fwd", k_trsv_fwd<F>, 1 + below, 256, A, ld, Binv, nt, K, b, n, z.p, (const LMState*)d.st);
+            sv.launch("trsv_fwd", k_trsv_fwd<F>, 1 + below, kSolveThreads, A, ld, Binv, nt, K, b, n, z.p,
+                      (const LMState*)d.st);
         }
         for (int K = nb - 1; K >= 0; --K)
-            sv.launch("trsv_bwd", k_trsv_bwd<F>, 1 + kSolveBlk * K, 256, A, ld, BinvT, nt, K, z.p, x, n, accumulate,
-                      (const LMState*)d.st);
+            sv.launch("trsv_bwd", k_trsv_bwd<F>, 1 + kSolveBlk * K, kSolveThreads, A, ld, BinvT, nt, K, z.p, x, n,
+                      accumulate, (const LMState*)d.st);
     }
 
     template <typename F, typename SolverT>

@@ -100,7 +100,8 @@ struct Dev {
     const int* co_pt;
     const double* co_uv64;
     const Real* co_uv;
-    Real* co_w;                // per observation (camera order): Huber weight sqrt(rho') at the linearization
+    R4<Real>* co_rec;          // per observation (camera order): X - c (point relative to the camera centre)
+                               // and the Huber weight sqrt(rho'), at the linearization (direction-form PCG)
     const int* chunk_off;
     const int* chunk_len;
     const int* slot_of;
@@ -114,7 +115,7 @@ struct Dev {
     double* Vinv64;            // per point: 6 (FP64 inverse of the damped block)
     Real* VinvR;               // per point: 6 (same, in working precision)
     R4<Real>* qp;              // per point: Vinv S g_p
-    R4<Real>* zs;              // per point: S_p z_p of the current PCG direction
+    double4* zs;               // per point: S_p z_p of the current PCG direction (FP64 in both precisions)
     double4* pv;               // per slot, 3 records: J_p'J_p (sym 6), J_p'r (3), padding
     R4<Real>* u;               // per slot: J_p'(J_c S_c v_c)
     double* cam_part;          // PS_COUNT * nc
@@ -176,21 +177,85 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Wave reductions on the VALU: four butterfly steps inside each row of 16 lanes by DPP lane
+// permutations (partners lane ^ 1, lane ^ 2, the mirrored lane of the half-row and of the row), so
+// every lane of a row holds the row's total, then the four row totals read with v_readlane and added
+// in a fixed order.  (Shuffles through ds_bpermute cost an LDS round trip per step; a workgroup
+// reducing a few dozen values spent most of its time waiting on them.)  Every lane of the wave must
+// be active.
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1, 0, 3, 2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2, 3, 0, 1]
+constexpr int kDppHalfMirror = 0x141; // row_half_mirror
+constexpr int kDppMirror = 0x140;     // row_mirror
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_lane(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_lane(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over each row of 16 lanes; every lane of the row gets its row's total.
 template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+__device__ __forceinline__ T row16_sum(T v) {
+    v += dpp_lane<kDppXor1>(v);
+    v += dpp_lane<kDppXor2>(v);
+    v += dpp_lane<kDppHalfMirror>(v);
+    v += dpp_lane<kDppMirror>(v);
     return v;
 }
 
 template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+    v = row16_sum(v);
+    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
+
+// Sum over each half-wave of 32 lanes (lanes 0-31 and 32-63 reduced separately).
+template <typename T>
+__device__ __forceinline__ T half_wave_sum(T v) {
+    v = row16_sum(v);
+    const T a = read_lane(v, 0) + read_lane(v, 16), b = read_lane(v, 32) + read_lane(v, 48);
+    return (threadIdx.x & 32) ? b : a;
+}
+
+template <typename T>
 __device__ __forceinline__ T wave_max(T v) {
+    T o = dpp_lane<kDppXor1>(v);
+    v = o > v ? o : v;
+    o = dpp_lane<kDppXor2>(v);
+    v = o > v ? o : v;
+    o = dpp_lane<kDppHalfMirror>(v);
+    v = o > v ? o : v;
+    o = dpp_lane<kDppMirror>(v);
+    v = o > v ? o : v;
+    const T a = read_lane(v, 0), b = read_lane(v, 16), c = read_lane(v, 32), e = read_lane(v, 48);
+    const T ab = a > b ? a : b, ce = c > e ? c : e;
+    return ab > ce ? ab : ce;
+}
+
+// sum_{k = threadIdx.x, + kWG, + 2 kWG, ... < n} a[k], added in that order, four loads in flight
+// (clamped indices and a select instead of a branch around each load, so a load does not wait for
+// the one before it).
+__device__ __forceinline__ double strided_sum(const double* __restrict__ a, int n) {
+    double s = 0.0;
+    for (int k = threadIdx.x; k < n; k += 4 * kWG) {
+        double x[4];
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const T o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
+        for (int u = 0; u < 4; ++u) {
+            const int kk = k + u * kWG;
+            const double v = a[min(kk, n - 1)];
+            x[u] = kk < n ? v : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s += x[u];
     }
-    return v;
+    return s;
 }
 
 // Sums N per-thread values over the 256-thread workgroup. On return lds[i * kWaves] holds the
@@ -384,6 +449,8 @@ __global__ void __launch_bounds__(kWG) k_lin_cam(Dev<Real> d) {
     const int c = xcd_remap(blockIdx.x, d.nc);
     if (c >= d.nc) return;
     const CameraPrep<Real> P = d.prep[c];
+    double cc[3];  // camera centre
+    camera_centre<Real>(P, cc);
     double acc[18];
 #pragma unroll
     for (int i = 0; i < 18; ++i) acc[i] = 0.0;
@@ -391,7 +458,15 @@ __global__ void __launch_bounds__(kWG) k_lin_cam(Dev<Real> d) {
         int p;
         Real r[2], Jc[18], Jp[6], w;
         corrected_jacobian<Real, M>(d, P, j, p, r, Jc, Jp, &w);
-        if (M == kModelSnavely) d.co_w[j] = w;  // for the direction-form PCG products
+        if (M == kModelSnavely) {  // the direction-form PCG products' record
+            const double4 X = reinterpret_cast<const double4*>(d.pt_x)[p];
+            R4<Real> rec;
+            rec.x = (Real)(X.x - cc[0]);
+            rec.y = (Real)(X.y - cc[1]);
+            rec.z = (Real)(X.z - cc[2]);
+            rec.w = w;
+            d.co_rec[j] = rec;
+        }
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             acc[i] += (double)(Jc[i] * Jc[i] + Jc[9 + i] * Jc[9 + i]);
@@ -947,12 +1022,7 @@ __global__ void __launch_bounds__(kWG) k_pt_z(Dev<Real> d) {
             t[i] = s[i] * a[i];
         }
         sym3_mul<double>(d.Vinv64 + 6 * (size_t)p, t, z);  // the point block solve stays FP64
-        R4<Real> zo;
-        zo.x = (Real)(s[0] * z[0]);
-        zo.y = (Real)(s[1] * z[1]);
-        zo.z = (Real)(s[2] * z[2]);
-        zo.w = (Real)0;
-        d.zs[p] = zo;
+        d.zs[p] = make_double4(s[0] * z[0], s[1] * z[1], s[2] * z[2], 0.0);
         for (int i = 0; i < 3; ++i) zdz += z[i] * d.dp2[3 * (size_t)p + i] * z[i];
     }
     zdz = wave_sum(zdz);
@@ -986,9 +1056,10 @@ __global__ void __launch_bounds__(kWG) k_jz(Dev<Real> d) {
         if (j + kWG < j_end) load_obs<Real, M>(d, j + kWG, in);
         Real r[2], Jc[18], Jp[6];
         corrected_from<Real, M>(d, P, cur, r, Jc, Jp);
-        const R4<Real> z = d.zs[cur.p];
-        Real e0 = -(Jp[0] * z.x + Jp[1] * z.y + Jp[2] * z.z);
-        Real e1 = -(Jp[3] * z.x + Jp[4] * z.y + Jp[5] * z.z);
+        const double4 z4 = d.zs[cur.p];
+        const Real z[3] = {(Real)z4.x, (Real)z4.y, (Real)z4.z};
+        Real e0 = -(Jp[0] * z[0] + Jp[1] * z[1] + Jp[2] * z[2]);
+        Real e1 = -(Jp[3] * z[0] + Jp[4] * z[1] + Jp[5] * z[2]);
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             e0 += Jc[i] * ps[i];
@@ -1013,72 +1084,113 @@ __global__ void __launch_bounds__(kWG) k_jz(Dev<Real> d) {
 
 // The two camera-order PCG products of the Snavely model (Ceres additive update) in direction form
 // (snavely.h, rot_directional): the same u_j and q_c as k_jtx (mode 1) and k_jz, without forming
-// the 2x9 / 2x3 Jacobian per observation.  The Huber weight sqrt(rho') of each observation was
-// stored by k_lin_cam at the linearization point (co_w), so the measurement is not re-read.
-
-// Observation inputs of the direction-form kernels, loaded one round ahead of their use.
-template <typename Real>
-struct DirIn {
-    int p;
-    double4 X;
-    Real w;
-};
-
-template <typename Real>
-__device__ __forceinline__ void load_dir(const Dev<Real>& d, int j, DirIn<Real>& in) {
-    in.p = d.co_pt[j];
-    in.X = reinterpret_cast<const double4*>(d.pt_x)[in.p];
-    in.w = d.co_w[j];
-}
+// the 2x9 / 2x3 Jacobian per observation.  The Huber weight sqrt(rho') of each observation and its
+// point relative to the camera centre were stored by k_lin_cam at the linearization point
+// (co_rec), so neither the measurement nor the point is re-read.
 
 // Per-camera constants of a direction-form product: dR_v for the scaled direction v = S_c p_c,
-// the rotation matrix in working precision, the camera centre c = -R't (FP64) and v's translation /
-// intrinsic parts.  The rotation columns are taken about the camera centre,
+// the rotation matrix and v's translation / intrinsic parts in working precision.  The rotation
+// columns are taken about the camera centre c = -R't,
 //   dR_v X + v_t = dR_v (X - c) + (dR_v c + v_t),
-// with X - c and dR_v c + v_t formed in FP64: for a point near a camera far from the world origin,
-// dR_v X and v_t are large and nearly opposite, and in FP32 their sum would keep few digits.
+// with X - c (the observation record, formed in FP64 at the linearization) and dR_v c + v_t formed
+// in FP64: for a point near a camera far from the world origin, dR_v X and v_t are large and nearly
+// opposite, and in FP32 their sum would keep few digits.
 template <typename Real>
 struct DirCam {
     Real Rd[9], R[9], vt[3], vf, vk1, vk2;
-    double c[3];
 };
+
+// The jz product's difference vector dR_v (X - c) + (dR_v c + v_t) - R zs: the camera's motion and
+// the point's, seen from the camera, nearly cancel where the point follows the camera, so in mixed
+// precision it is formed in FP64 (from the record's X - c and the FP64 zs) and rounded once.
+template <typename Real>
+__device__ __forceinline__ void jz_difference(const CameraPrep<Real>& P, const double* rsh, const double vt64[3],
+                                              const R4<Real>& rec, const double4& z, Real dq[3]) {
+    const double X[3] = {(double)rec.x, (double)rec.y, (double)rec.z};
+    const double Z[3] = {z.x, z.y, z.z};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double q = rsh[3 * i] * X[0] + rsh[3 * i + 1] * X[1] + rsh[3 * i + 2] * X[2] + vt64[i];
+        const double zr = P.R64[3 * i] * Z[0] + P.R64[3 * i + 1] * Z[1] + P.R64[3 * i + 2] * Z[2];
+        dq[i] = (Real)(q - zr);
+    }
+}
+
+// A value that is the same in every lane, moved to a scalar register (the per-camera constants of
+// the direction-form kernels would otherwise occupy ~24 vector registers of every lane).
+__device__ __forceinline__ float wave_uniform(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double wave_uniform(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)b), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 template <typename Real>
 __device__ __forceinline__ void load_dircam(const CameraPrep<Real>& P, const double* vsh, const double* rsh,
                                             DirCam<Real>& C) {
+    double c[3];
+    camera_centre<Real>(P, c);
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        C.Rd[k] = (Real)rsh[k];
-        C.R[k] = (Real)P.R64[k];
+        C.Rd[k] = wave_uniform((Real)rsh[k]);
+        C.R[k] = wave_uniform((Real)P.R64[k]);
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-        C.c[i] = -(P.R64[i] * P.t64[0] + P.R64[3 + i] * P.t64[1] + P.R64[6 + i] * P.t64[2]);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-        C.vt[i] = (Real)(vsh[3 + i] + rsh[3 * i] * C.c[0] + rsh[3 * i + 1] * C.c[1] + rsh[3 * i + 2] * C.c[2]);
-    C.vf = (Real)vsh[6];
-    C.vk1 = (Real)vsh[7];
-    C.vk2 = (Real)vsh[8];
+        C.vt[i] = wave_uniform((Real)(vsh[3 + i] + rsh[3 * i] * c[0] + rsh[3 * i + 1] * c[1] + rsh[3 * i + 2] * c[2]));
+    C.vf = wave_uniform((Real)vsh[6]);
+    C.vk1 = wave_uniform((Real)vsh[7]);
+    C.vk2 = wave_uniform((Real)vsh[8]);
 }
 
-// a = Jc v / w for one observation (w: its Huber weight), and its linearization.
+// The projection's linearization from the record's X - c: p = R (X - c) in working precision.
 template <typename Real>
-__device__ __forceinline__ void dir_apply(const CameraPrep<Real>& P, const DirCam<Real>& C, const double4& X4,
+__device__ __forceinline__ void proj_lin_rec(const CameraPrep<Real>& P, const DirCam<Real>& C, const R4<Real>& rec,
+                                             ProjLin<Real>& L) {
+    Real p[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) p[i] = C.R[3 * i] * rec.x + C.R[3 * i + 1] * rec.y + C.R[3 * i + 2] * rec.z;
+    L.ginv = (Real)1 / p[2];
+    L.xp = -p[0] * L.ginv;
+    L.yp = -p[1] * L.ginv;
+    L.r2 = L.xp * L.xp + L.yp * L.yp;
+    const Real poly = P.k1 + P.k2 * L.r2;
+    L.dist = (Real)1 + L.r2 * poly;
+    L.fd = P.f * L.dist;
+    L.fe = (Real)2 * P.f * (poly + P.k2 * L.r2);
+}
+
+// a = Jc v / w - D R zR... for one observation record (w: its Huber weight): D (dR_v (X - c) +
+// dR_v c + v_t - zR) + K (v_f, v_k1, v_k2), and the projection's linearization.
+template <typename Real>
+__device__ __forceinline__ void dir_apply(const CameraPrep<Real>& P, const DirCam<Real>& C, const R4<Real>& rec,
                                           const Real zR[3], ProjLin<Real>& L, Real a[2]) {
-    const double X64[3] = {X4.x, X4.y, X4.z};
-    proj_lin<Real>(P, X64, L);
-    const Real X[3] = {(Real)(X4.x - C.c[0]), (Real)(X4.y - C.c[1]), (Real)(X4.z - C.c[2])};
+    proj_lin_rec<Real>(P, C, rec, L);
     Real q[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) q[i] = C.Rd[3 * i] * X[0] + C.Rd[3 * i + 1] * X[1] + C.Rd[3 * i + 2] * X[2] + C.vt[i] - zR[i];
+    for (int i = 0; i < 3; ++i)
+        q[i] = C.Rd[3 * i] * rec.x + C.Rd[3 * i + 1] * rec.y + C.Rd[3 * i + 2] * rec.z + C.vt[i] - zR[i];
     proj_apply<Real>(L, q, a);
     const Real kap = L.dist * C.vf + P.f * L.r2 * (C.vk1 + L.r2 * C.vk2);
     a[0] += L.xp * kap;
     a[1] += L.yp * kap;
 }
 
+// D dq + K (v_f, v_k1, v_k2) for a given camera-frame vector dq, and the projection's linearization.
+template <typename Real>
+__device__ __forceinline__ void dir_apply_dq(const CameraPrep<Real>& P, const DirCam<Real>& C, const R4<Real>& rec,
+                                             const Real dq[3], ProjLin<Real>& L, Real a[2]) {
+    proj_lin_rec<Real>(P, C, rec, L);
+    proj_apply<Real>(L, dq, a);
+    const Real kap = L.dist * C.vf + P.f * L.r2 * (C.vk1 + L.r2 * C.vk2);
+    a[0] += L.xp * kap;
+    a[1] += L.yp * kap;
+}
+
 // PCG iteration `it`: beta, p <- z + beta p, then u_j = Jp'(Jc S_c p_c) per observation (slot order).
+// An observation needs only its record and slot (no dependent loads).
 template <typename Real>
 __global__ void __launch_bounds__(kWG) k_jtx_dir(Dev<Real> d, int it) {
     LMState* st = d.st;
@@ -1089,15 +1201,25 @@ __global__ void __launch_bounds__(kWG) k_jtx_dir(Dev<Real> d, int it) {
     if (c >= d.nc) return;
     const int j_end = d.cam_off[c + 1];
     int j = d.cam_off[c] + threadIdx.x;
-    DirIn<Real> in{};
-    if (j < j_end) load_dir<Real>(d, j, in);
+    R4<Real> rec{};
+    int slot = 0;
+    if (j < j_end) {
+        rec = d.co_rec[j];
+        slot = d.slot_of[j];
+    }
+    double pz = 0.0, pp = 0.0, sc = 0.0;
+    if (threadIdx.x < 9) {
+        const size_t i = 9 * (size_t)c + threadIdx.x;
+        pz = d.pz[i];
+        pp = d.pp[i];
+        sc = d.sc[i];
+    }
     double beta;
     if (pcg_beta(d, it, lds, beta)) return;
     if (threadIdx.x < 9) {
-        const size_t i = 9 * (size_t)c + threadIdx.x;
-        const double v = d.pz[i] + beta * d.pp[i];
-        d.pp[i] = v;
-        vsh[threadIdx.x] = d.sc[i] * v;
+        const double v = pz + beta * pp;
+        d.pp[9 * (size_t)c + threadIdx.x] = v;
+        vsh[threadIdx.x] = sc * v;
     }
     __syncthreads();
     const CameraPrep<Real> P = d.prep[c];
@@ -1107,12 +1229,15 @@ __global__ void __launch_bounds__(kWG) k_jtx_dir(Dev<Real> d, int it) {
     load_dircam<Real>(P, vsh, rsh, C);
     const Real zero[3] = {(Real)0, (Real)0, (Real)0};
     for (; j < j_end; j += kWG) {
-        const DirIn<Real> cur = in;
-        if (j + kWG < j_end) load_dir<Real>(d, j + kWG, in);
-        const int slot = d.slot_of[j];
+        const R4<Real> cur = rec;
+        const int cur_slot = slot;
+        if (j + kWG < j_end) {
+            rec = d.co_rec[j + kWG];
+            slot = d.slot_of[j + kWG];
+        }
         ProjLin<Real> L;
         Real a[2];
-        dir_apply<Real>(P, C, cur.X, zero, L, a);
+        dir_apply<Real>(P, C, cur, zero, L, a);
         const Real ww = cur.w * cur.w;
         a[0] *= ww;
         a[1] *= ww;
@@ -1123,13 +1248,13 @@ __global__ void __launch_bounds__(kWG) k_jtx_dir(Dev<Real> d, int it) {
         u.y = C.R[1] * g[0] + C.R[4] * g[1] + C.R[7] * g[2];
         u.z = C.R[2] * g[0] + C.R[5] * g[1] + C.R[8] * g[2];
         u.w = (Real)0;
-        d.u[slot] = u;
+        d.u[cur_slot] = u;
     }
 }
 
 // q_c = D_c^2 p_c + S_c sum Jc' e_j with e_j = Jc S_c p_c - Jp zs_p; partial sum |e|^2 + p'D_c^2 p.
-// Per observation only g = Jc-frame D'(w e) is formed; the camera accumulates G = sum g X', sum g and
-// the intrinsic sums, and contracts G with dR/dw once at the end.
+// Per observation only g = D'(w e) is formed (one gather: the point's zs); the camera accumulates
+// G = sum g (X - c)', sum g and the intrinsic sums, and contracts G + (sum g) c' with dR/dw once.
 template <typename Real>
 __global__ void __launch_bounds__(kWG) k_jz_dir(Dev<Real> d) {
     LMState* st = d.st;
@@ -1140,11 +1265,11 @@ __global__ void __launch_bounds__(kWG) k_jz_dir(Dev<Real> d) {
     if (c >= d.nc) return;
     const int j_end = d.cam_off[c + 1];
     int j = d.cam_off[c] + threadIdx.x;
-    DirIn<Real> in{};
-    R4<Real> zin{};
+    R4<Real> rec{};
+    double4 zin{};
     if (j < j_end) {
-        load_dir<Real>(d, j, in);
-        zin = d.zs[in.p];
+        rec = d.co_rec[j];
+        zin = d.zs[d.co_pt[j]];
     }
     if (threadIdx.x < 9) {
         const size_t i = 9 * (size_t)c + threadIdx.x;
@@ -1157,32 +1282,39 @@ __global__ void __launch_bounds__(kWG) k_jz_dir(Dev<Real> d) {
     __syncthreads();
     DirCam<Real> C;
     load_dircam<Real>(P, vsh, rsh, C);
+    double vt64[3];  // dR_v c + v_t in FP64
+    {
+        double cc[3];
+        camera_centre<Real>(P, cc);
+        for (int i = 0; i < 3; ++i)
+            vt64[i] = wave_uniform(vsh[3 + i] + rsh[3 * i] * cc[0] + rsh[3 * i + 1] * cc[1] + rsh[3 * i + 2] * cc[2]);
+    }
     double acc[16];  // G (row-major 3x3), sum g, intrinsic sums (dist, r2, r2^2 times x'e0 + y'e1), |e|^2
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.0;
     for (; j < j_end; j += kWG) {
-        const DirIn<Real> cur = in;
-        const R4<Real> z = zin;
+        const R4<Real> cur = rec;
+        const double4 z = zin;
         if (j + kWG < j_end) {
-            load_dir<Real>(d, j + kWG, in);
-            zin = d.zs[in.p];
+            rec = d.co_rec[j + kWG];
+            zin = d.zs[d.co_pt[j + kWG]];
         }
-        const Real zR[3] = {C.R[0] * z.x + C.R[1] * z.y + C.R[2] * z.z, C.R[3] * z.x + C.R[4] * z.y + C.R[5] * z.z,
-                            C.R[6] * z.x + C.R[7] * z.y + C.R[8] * z.z};
+        Real dq[3];
+        jz_difference<Real>(P, rsh, vt64, cur, z, dq);
         ProjLin<Real> L;
         Real a[2];
-        dir_apply<Real>(P, C, cur.X, zR, L, a);
+        dir_apply_dq<Real>(P, C, cur, dq, L, a);
         const Real e0 = cur.w * a[0], e1 = cur.w * a[1];
         acc[15] += (double)e0 * (double)e0 + (double)e1 * (double)e1;
         const Real ew[2] = {cur.w * e0, cur.w * e1};
         Real g[3];
         proj_apply_t<Real>(L, ew, g);
-        const double X64[3] = {cur.X.x, cur.X.y, cur.X.z};
+        const double X[3] = {(double)cur.x, (double)cur.y, (double)cur.z};
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const double gi = (double)g[i];
 #pragma unroll
-            for (int m = 0; m < 3; ++m) acc[3 * i + m] += gi * X64[m];
+            for (int m = 0; m < 3; ++m) acc[3 * i + m] += gi * X[m];
             acc[9 + i] += gi;
         }
         const double se = (double)(L.xp * ew[0] + L.yp * ew[1]);
@@ -1194,11 +1326,14 @@ __global__ void __launch_bounds__(kWG) k_jz_dir(Dev<Real> d) {
     block_sum<16>(acc, lds);
     const int t = threadIdx.x;
     if (t == 0) {
-        double G[9], w3[3];
-        for (int k = 0; k < 9; ++k) G[k] = lds[k * kWaves];
+        double cc[3], G[9], w3[3];
+        camera_centre<Real>(P, cc);
+        const double gs[3] = {lds[9 * kWaves], lds[10 * kWaves], lds[11 * kWaves]};
+        for (int i = 0; i < 3; ++i)
+            for (int m = 0; m < 3; ++m) G[3 * i + m] = lds[(3 * i + m) * kWaves] + gs[i] * cc[m];
         rot_contract<Real>(P, G, w3);
         const double f = (double)P.f;
-        const double qv[9] = {w3[0], w3[1], w3[2], lds[9 * kWaves], lds[10 * kWaves], lds[11 * kWaves],
+        const double qv[9] = {w3[0], w3[1], w3[2], gs[0], gs[1], gs[2],
                               lds[12 * kWaves], f * lds[13 * kWaves], f * lds[14 * kWaves]};
         for (int k = 0; k < 9; ++k) qsh[k] = qv[k];
     }
@@ -1223,9 +1358,8 @@ __global__ void __launch_bounds__(kWG) k_cam_update(Dev<Real> d, int it) {
     if (st->done || st->step_invalid || st->pcg_done) return;
     __shared__ double lds[2 * kWaves];
     __shared__ double rsh[kCamsPerUpd][9];
-    double v = 0.0;
-    for (int k = threadIdx.x; k < d.nc; k += kWG) v += d.cam_part[PS_A * (size_t)d.nc + k];
-    for (int k = threadIdx.x; k < d.n_pt_blocks; k += kWG) v += d.pt_part[PS_B * (size_t)d.n_pt_blocks + k];
+    double v = strided_sum(d.cam_part + PS_A * (size_t)d.nc, d.nc);
+    v += strided_sum(d.pt_part + PS_B * (size_t)d.n_pt_blocks, d.n_pt_blocks);
     const double pSp = block_sum1(v, lds);
     if (!(pSp > 0.0) || !isfinite(pSp)) {
         // v'Sv is a sum of squares: zero means the direction vanished (rounding level)

@@ -101,6 +101,12 @@ BA_HD void camera_prep(const double cam[9], CameraPrep<Real>& P) {
     }
 }
 
+// Camera centre c = -R't of a prep (FP64).
+template <typename Real>
+BA_HD void camera_centre(const CameraPrep<Real>& P, double c[3]) {
+    for (int i = 0; i < 3; ++i) c[i] = -(P.R64[i] * P.t64[0] + P.R64[3 + i] * P.t64[1] + P.R64[6 + i] * P.t64[2]);
+}
+
 // Camera-frame point p = R(w) X + t.  FP64: the reference's Rodrigues expression order
 // (rotation.h:147-169).  FP32: formed in FP64 from the prep's rotation matrix and rounded once, so
 // that points close to the camera center (|p| << |X| + |t|) do not lose their digits to the

@@ -211,14 +211,15 @@ struct Solver final : SolverBase {
     DevBuf<int> cam_off, co_pt, chunk_off, chunk_len, slot_of, co_side;
     DevBuf<double> co_uv64;
     DevBuf<Real> co_uvr;
-    DevBuf<Real> co_w;  // Huber weight per observation at the linearization (direction-form PCG)
+    DevBuf<R4<Real>> co_rec;  // per observation: X - c and the Huber weight (direction-form PCG)
     DevBuf<double> cam_x, pt_x, cam_x0, pt_x0, cam_c, pt_c, dcam, dpt;
     DevBuf<CameraPrep<Real>> prep;
     DevBuf<CameraPrep<double>> prep_c;
     DevBuf<double> U, gc, sc, dc2, M, Minv, bc, px, pr, pz, pp, qc;
     DevBuf<double> V, gp, sp, dp2, Vinv64, upd_part, rz_hist;
     DevBuf<Real> VinvR;
-    DevBuf<R4<Real>> qp, zs, u;
+    DevBuf<R4<Real>> qp, u;
+    DevBuf<double4> zs;
     DevBuf<double4> pv;
     DevBuf<double> cam_part, pt_part;
     DevBuf<LMState> st;
@@ -253,7 +254,7 @@ struct Solver final : SolverBase {
     int64_t device_bytes() const override {
         int64_t b = 0;
         b += cam_off.bytes() + co_pt.bytes() + chunk_off.bytes() + chunk_len.bytes() + slot_of.bytes();
-        b += co_uv64.bytes() + co_uvr.bytes() + co_w.bytes();
+        b += co_uv64.bytes() + co_uvr.bytes() + co_rec.bytes();
         b += cam_x.bytes() * 4 + pt_x.bytes() * 4 + dcam.bytes() + dpt.bytes();
         b += prep.bytes() + prep_c.bytes();
         b += M.bytes() * 2 + gc.bytes() * 10 + upd_part.bytes() + rz_hist.bytes() + dp2.bytes();
@@ -321,7 +322,7 @@ struct Solver final : SolverBase {
         BA_TRY(VinvR.alloc(6 * (size_t)np));
         BA_TRY(qp.alloc(np));
         BA_TRY(zs.alloc(np));
-        BA_TRY(co_w.alloc((size_t)std::max(no, 1)));
+        BA_TRY(co_rec.alloc((size_t)std::max(no, 1)));
         // slot-order buffers: padding slots are never written and must read as zero
         const size_t nslots = (size_t)std::max(L.chunk_off[L.n_chunks], 1);
         BA_TRY(pv.alloc(3 * nslots));
@@ -358,7 +359,7 @@ struct Solver final : SolverBase {
             d.rig = *stereo->rig;
         }
         d.co_uv = sizeof(Real) == sizeof(double) ? (const Real*)co_uv64.p : co_uvr.p;
-        d.co_w = co_w.p;
+        d.co_rec = co_rec.p;
 
         d.chunk_off = chunk_off.p;
         d.chunk_len = chunk_len.p;
@@ -665,8 +666,8 @@ struct Solver final : SolverBase {
         // direction form: point index, Huber weight, slot, u out (jtx); point index, weight, point
         // and its zs (jz); per camera its prep and the 9-vectors read and written
         const double prep_b = sizeof(CameraPrep<Real>);
-        if (n == "jtx" && dir) return no * (4 + R + 4 + 4 * R) + np * 4 * 8 + nc * (prep_b + 9 * 8 * 4 + 2 * 8 * d.n_upd);
-        if (n == "jz" && dir) return no * (4 + R) + np * (4 * 8 + 4 * R) + nc * (prep_b + 9 * 8 * 5);
+        if (n == "jtx" && dir) return no * (4 * R + 4 + 4 * R) + nc * (prep_b + 9 * 8 * 4 + 2 * 8 * d.n_upd);
+        if (n == "jz" && dir) return no * (4 * R + 4) + np * 4 * R + nc * (prep_b + 9 * 8 * 5);
         if (n == "jtx" || n == "jtx_y") return no * (4 + 4 + 2 * R + 4 * R) + np * 4 * 8 + nc * 9 * 8 * 3;
         if (n == "jz") return no * (4 + 2 * R) + np * (4 * 8 + 4 * R) + nc * 9 * 8 * 8;
         if (n == "pt_z") return no * 4 * R + np * (3 * 8 + 3 * 8 + 6 * R + 4 * R);

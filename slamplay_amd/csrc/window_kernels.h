@@ -312,9 +312,7 @@ __device__ __forceinline__ void win_schur(const int4* __restrict__ blk, const in
         for (int r = 0; r < 5; ++r) {
 #pragma unroll
             for (int c = 0; c < 9; ++c) {
-                double v = (double)acc[r][c];
-#pragma unroll
-                for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+                const double v = half_wave_sum((double)acc[r][c]);
                 const int row = rb + r;
                 if (q == 0 && row < 9) {
                     const double s = (ca == cb && row == c ? dc2[9 * ca + row] : 0.0) - v;

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define BA_ABI_VERSION 1
+#define BA_ABI_VERSION 2
 
 typedef enum ba_status {
     BA_OK = 0,
@@ -128,7 +128,22 @@ typedef struct ba_iteration {
     double relative_decrease;
     double trust_region_radius;
     double model_cost_change;
+    double iteration_time_in_seconds;  /* device time since the previous row (row 0: 0) */
+    double cumulative_time_in_seconds; /* device time since the first evaluation */
 } ba_iteration;
+
+/* Which test ended the solve (Ceres's termination messages, Solver::Summary::message). */
+typedef enum ba_termination_reason {
+    BA_REASON_NONE = 0,
+    BA_REASON_MAX_ITERATIONS = 1,     /* NO_CONVERGENCE: max_num_iterations reached */
+    BA_REASON_FUNCTION_TOLERANCE = 2, /* CONVERGENCE: |cost_change| / cost <= function_tolerance */
+    BA_REASON_GRADIENT_TOLERANCE = 3, /* CONVERGENCE: max |gradient| <= gradient_tolerance */
+    BA_REASON_PARAMETER_TOLERANCE = 4,/* CONVERGENCE: |step| <= parameter_tolerance (|x| + parameter_tolerance) */
+    BA_REASON_MIN_RADIUS = 5,         /* CONVERGENCE: trust region radius below min_trust_region_radius */
+    BA_REASON_INVALID_STEPS = 6,      /* FAILURE: max_num_consecutive_invalid_steps invalid steps in a row */
+    BA_REASON_NON_FINITE_COST = 7,    /* FAILURE: the cost is not finite */
+    BA_REASON_G2O_STOP = 8            /* g2o mode: a trial budget exhausted or rho = 0 (g2o stops) */
+} ba_termination_reason;
 
 typedef struct ba_summary {
     double initial_cost;
@@ -142,7 +157,8 @@ typedef struct ba_summary {
     double total_time_ms;      /* wall time of the solve, device work included */
     ba_iteration* trace;       /* optional caller-owned array of trace_capacity rows */
     int32_t trace_capacity;
-    int32_t reserved0;
+    int32_t termination_reason; /* ba_termination_reason */
+    double termination_value;   /* the tested quantity (|cost_change| / cost, max |gradient|, ...) */
 } ba_summary;
 
 int32_t ba_abi_version(void);
@@ -159,6 +175,15 @@ int ba_eval_residual_jacobian(const ba_problem_view* problem, int32_t precision,
 
 /* 1/2 sum_i rho(|r_i|^2) on the GPU in FP64 (loss = ba_loss, scale = Huber a). */
 int ba_eval_cost(const ba_problem_view* problem, int32_t loss, double loss_scale, double* cost);
+
+/* The text of Ceres's Solver::Summary::FullReport() for a solve of `num_cameras` cameras,
+ * `num_points` points and `num_observations` observations with `options` (the reference driver
+ * prints it after ceres::Solve, test_bundle_adjustment_ceres.cpp:101): problem size, minimizer and
+ * linear solver, costs, step counts, times and the termination message.  Writes at most `capacity`
+ * bytes (NUL-terminated) into `out`; *length (optional) receives the full length. */
+int ba_summary_full_report(const ba_summary* summary, int32_t num_cameras, int32_t num_points,
+                           int32_t num_observations, const ba_options* options, char* out, int64_t capacity,
+                           int64_t* length);
 
 /* ba_eval_cost without any device allocation.  No counterpart in the reference: it is for callers
  * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses

@@ -64,12 +64,29 @@ struct SolveOptions : ba_options {
     SolveOptions() { ba_default_options(this); }
 };
 
-/// Solver summary with the per-iteration trace kept on the host.
+/// Solver summary with the per-iteration trace kept on the host; SolveBA records the problem
+/// size and options so that FullReport() prints what ceres::Solver::Summary::FullReport() does
+/// (test_bundle_adjustment_ceres.cpp:101).
 struct SolveSummary : ba_summary {
     ba_iteration rows[512];
+    int32_t num_cameras = 0, num_points = 0, num_observations = 0;
+    ba_options options;
     SolveSummary() : ba_summary{} {
         trace = rows;
         trace_capacity = 512;
+        ba_default_options(&options);
+    }
+    SolveSummary(const SolveSummary&) = delete;  // trace points into this object
+    SolveSummary& operator=(const SolveSummary&) = delete;
+
+    std::string FullReport() const {
+        int64_t n = 0;
+        if (ba_summary_full_report(this, num_cameras, num_points, num_observations, &options, nullptr, 0, &n) != BA_OK)
+            throw std::runtime_error(ba_last_error_string());
+        std::string r((size_t)n + 1, '\0');
+        ba_summary_full_report(this, num_cameras, num_points, num_observations, &options, &r[0], n + 1, nullptr);
+        r.resize((size_t)n);
+        return r;
     }
 };
 
@@ -89,6 +106,18 @@ int SolveBA(Problem& bal_problem, const ba_options& options = SolveOptions(), ba
     v.cameras = bal_problem.mutable_cameras();
     v.points = bal_problem.mutable_points();
     return ba_solve(&v, &options, summary);
+}
+
+/// SolveBA that also records what SolveSummary::FullReport() needs.
+template <typename Problem>
+int SolveBA(Problem& bal_problem, const ba_options& options, SolveSummary* summary) {
+    if (summary) {
+        summary->num_cameras = bal_problem.num_cameras();
+        summary->num_points = bal_problem.num_points();
+        summary->num_observations = bal_problem.num_observations();
+        summary->options = options;
+    }
+    return SolveBA(bal_problem, options, static_cast<ba_summary*>(summary));
 }
 
 }  // namespace slamplay_amd

@@ -26,7 +26,7 @@ from . import _capi as C
 from ._capi import BAError
 
 __all__ = ["BALProblem", "SolveBA", "Solver", "Batch", "Server", "simulate", "server_config", "local_ba_windows",
-           "default_options",
+           "default_options", "full_report",
            "eval_residual_jacobian", "eval_cost", "synthetic", "synth_params", "BAError", "device_count"]
 
 
@@ -150,14 +150,33 @@ def _summary(trace_cap):
     return s, rows
 
 
-def _summary_dict(s, rows):
+def full_report(summary, num_cameras, num_points, num_observations, options=None):
+    """Ceres's Solver::Summary::FullReport() text for a ctypes Summary (what the reference driver
+    prints after ceres::Solve, test_bundle_adjustment_ceres.cpp:101)."""
+    o = options if options is not None else default_options()
+    n = C.i64(0)
+    C.check(C.lib().ba_summary_full_report(ctypes.byref(summary), num_cameras, num_points, num_observations,
+                                           ctypes.byref(o), None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    C.check(C.lib().ba_summary_full_report(ctypes.byref(summary), num_cameras, num_points, num_observations,
+                                           ctypes.byref(o), buf, n.value + 1, None))
+    return buf.value.decode()
+
+
+def _summary_dict(s, rows, sizes=None, options=None):
+    """The summary as a dict; with sizes = (cameras, points, observations) it carries the
+    FullReport text as well."""
     n = min(s.num_iterations, s.trace_capacity)
     trace = [{f: getattr(rows[i], f) for f, _ in C.Iteration._fields_} for i in range(n)]
-    return dict(initial_cost=s.initial_cost, final_cost=s.final_cost, num_iterations=s.num_iterations,
-                num_successful_steps=s.num_successful_steps, num_unsuccessful_steps=s.num_unsuccessful_steps,
-                termination=s.termination_type, linear_solver=s.linear_solver_used,
-                linear_solver_iterations=s.total_linear_solver_iterations, total_time_ms=s.total_time_ms,
-                trace=trace)
+    out = dict(initial_cost=s.initial_cost, final_cost=s.final_cost, num_iterations=s.num_iterations,
+               num_successful_steps=s.num_successful_steps, num_unsuccessful_steps=s.num_unsuccessful_steps,
+               termination=s.termination_type, termination_reason=s.termination_reason,
+               termination_value=s.termination_value, linear_solver=s.linear_solver_used,
+               linear_solver_iterations=s.total_linear_solver_iterations, total_time_ms=s.total_time_ms,
+               trace=trace)
+    if sizes is not None:
+        out["full_report"] = full_report(s, *sizes, options=options)
+    return out
 
 
 def _trace_rows(o):
@@ -171,7 +190,7 @@ def SolveBA(problem, options=None, **kw):
     o = options if options is not None else default_options(**kw)
     s, rows = _summary(_trace_rows(o))
     C.check(C.lib().ba_solve(ctypes.byref(problem.view()), ctypes.byref(o), ctypes.byref(s)))
-    return _summary_dict(s, rows)
+    return _summary_dict(s, rows, (problem.num_cameras(), problem.num_points(), problem.num_observations()), o)
 
 
 class Solver:
@@ -181,6 +200,7 @@ class Solver:
     def __init__(self, problem, options=None, **kw):
         self.options = options if options is not None else default_options(**kw)
         self._nc, self._np = problem.num_cameras(), problem.num_points()
+        self._no = problem.num_observations()
         h = C.vp()
         C.check(C.lib().ba_solver_create(ctypes.byref(problem.view()), ctypes.byref(self.options), ctypes.byref(h)))
         self._h = h
@@ -188,7 +208,7 @@ class Solver:
     def run(self):
         s, rows = _summary(_trace_rows(self.options))
         C.check(C.lib().ba_solver_run(self._h, ctypes.byref(s)))
-        return _summary_dict(s, rows)
+        return _summary_dict(s, rows, (self._nc, self._np, self._no), self.options)
 
     def reset(self):
         C.check(C.lib().ba_solver_reset(self._h))
@@ -299,7 +319,9 @@ def _views(problems):
 def _summary_plain(s):
     return dict(initial_cost=s.initial_cost, final_cost=s.final_cost, num_iterations=s.num_iterations,
                 num_successful_steps=s.num_successful_steps, num_unsuccessful_steps=s.num_unsuccessful_steps,
-                termination=s.termination_type, linear_solver=s.linear_solver_used, finish_ms=s.total_time_ms)
+                termination=s.termination_type, termination_reason=s.termination_reason,
+                termination_value=s.termination_value, linear_solver=s.linear_solver_used,
+                finish_ms=s.total_time_ms)
 
 
 class Batch:

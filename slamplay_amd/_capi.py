@@ -46,14 +46,14 @@ class Iteration(ctypes.Structure):
     _fields_ = [("iteration", i32), ("step_is_successful", i32), ("step_is_valid", i32),
                 ("linear_solver_iterations", i32), ("cost", f64), ("cost_change", f64), ("gradient_max_norm", f64),
                 ("step_norm", f64), ("relative_decrease", f64), ("trust_region_radius", f64),
-                ("model_cost_change", f64)]
+                ("model_cost_change", f64), ("iteration_time_in_seconds", f64), ("cumulative_time_in_seconds", f64)]
 
 
 class Summary(ctypes.Structure):
     _fields_ = [("initial_cost", f64), ("final_cost", f64), ("num_iterations", i32), ("num_successful_steps", i32),
                 ("num_unsuccessful_steps", i32), ("termination_type", i32), ("linear_solver_used", i32),
                 ("total_linear_solver_iterations", i32), ("total_time_ms", f64), ("trace", ctypes.POINTER(Iteration)),
-                ("trace_capacity", i32), ("reserved0", i32)]
+                ("trace_capacity", i32), ("termination_reason", i32), ("termination_value", f64)]
 
 
 class KernelTime(ctypes.Structure):
@@ -95,6 +95,8 @@ SIGNATURES = {
     # device pointers throughout (the view's arrays, the one-double cost) and a hipStream_t
     "ba_eval_cost_noalloc": (ctypes.c_int, [ctypes.POINTER(ProblemView), i32, f64, vp, vp]),
     "ba_solve": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.POINTER(Options), ctypes.POINTER(Summary)]),
+    "ba_summary_full_report": (ctypes.c_int, [ctypes.POINTER(Summary), i32, i32, i32, ctypes.POINTER(Options),
+                                               ctypes.c_char_p, i64, ctypes.POINTER(i64)]),
     "ba_solver_create": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.POINTER(Options), ctypes.POINTER(vp)]),
     "ba_solver_run": (ctypes.c_int, [vp, ctypes.POINTER(Summary)]),
     "ba_solver_reset": (ctypes.c_int, [vp]),

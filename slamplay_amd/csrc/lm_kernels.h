@@ -74,7 +74,22 @@ struct LMState {
     int num_successful, num_unsuccessful;
     int pcg_iters, pcg_done, total_pcg_iters, trace_len;
     int g2o_iter, g2o_trials;  // g2o mode: outer iterations done, rejected trials of the current one
+    int reason;                // ba_termination_reason of the test that ended the solve
+    double reason_value;       // the quantity that test compared
+    unsigned long long t0;     // device wall clock at the first evaluation (trace row times)
 };
+
+// Ends the solve with `termination`, recording which test fired and the value it tested.
+template <class State>
+__device__ __forceinline__ void lm_finish(State* st, int termination, int reason, double value) {
+    st->done = 1;
+    st->termination = termination;
+    st->reason = reason;
+    st->reason_value = value;
+}
+
+// Device wall-clock ticks since the first evaluation (converted to seconds on the host).
+__device__ __forceinline__ double lm_ticks(const LMState* st) { return (double)(wall_clock64() - st->t0); }
 
 struct Opts {
     int loss;
@@ -665,6 +680,7 @@ __global__ void __launch_bounds__(kWG) k_post_lin(Dev<Real> d) {
     if (st->first_lin) {
         if (d.o.g2o) st->radius = 1.0 / (d.o.g2o_tau * hmax);  // radius = 1 / lambda
         st->x_cost = cost;
+        st->t0 = wall_clock64();
         if (st->trace_len < d.trace_cap) {
             ba_iteration row{};
             row.iteration = 0;
@@ -676,24 +692,16 @@ __global__ void __launch_bounds__(kWG) k_post_lin(Dev<Real> d) {
             d.trace[st->trace_len] = row;
         }
         st->trace_len++;
-        if (!isfinite(cost)) {
-            st->done = 1;
-            st->termination = BA_FAILURE;
-        } else if (st->iteration >= d.o.max_iterations) {
-            st->done = 1;
-            st->termination = BA_NO_CONVERGENCE;
-        }
+        if (!isfinite(cost))
+            lm_finish(st, BA_FAILURE, BA_REASON_NON_FINITE_COST, cost);
+        else if (st->iteration >= d.o.max_iterations)
+            lm_finish(st, BA_NO_CONVERGENCE, BA_REASON_MAX_ITERATIONS, st->iteration);
     } else if (st->trace_len > 0 && st->trace_len <= d.trace_cap) {
         d.trace[st->trace_len - 1].gradient_max_norm = gm;
     }
-    if (!d.o.g2o && !st->done && gm <= d.o.gtol) {
-        st->done = 1;
-        st->termination = BA_CONVERGENCE;
-    }
-    if (!d.o.g2o && !st->done && st->radius <= d.o.min_radius) {
-        st->done = 1;
-        st->termination = BA_CONVERGENCE;
-    }
+    if (!d.o.g2o && !st->done && gm <= d.o.gtol) lm_finish(st, BA_CONVERGENCE, BA_REASON_GRADIENT_TOLERANCE, gm);
+    if (!d.o.g2o && !st->done && st->radius <= d.o.min_radius)
+        lm_finish(st, BA_CONVERGENCE, BA_REASON_MIN_RADIUS, st->radius);
     st->need_lin = 0;
     st->first_lin = 0;
 }
@@ -1604,23 +1612,20 @@ __global__ void __launch_bounds__(kWG) k_decide(Dev<Real> d) {
         }
         row.gradient_max_norm = st->gmax;
         row.trust_region_radius = st->radius;
+        row.cumulative_time_in_seconds = lm_ticks(st);
         if (st->trace_len < d.trace_cap) d.trace[st->trace_len] = row;
         st->trace_len++;
-        if (st->g2o_iter >= o.max_iterations) {
-            st->done = 1;
-            st->termination = BA_NO_CONVERGENCE;
-        } else if (st->g2o_trials >= o.g2o_max_trials || rho == 0.0 || !isfinite(st->radius) || !(st->radius > 0.0)) {
-            st->done = 1;
-            st->termination = BA_CONVERGENCE;
-        }
+        if (st->g2o_iter >= o.max_iterations)
+            lm_finish(st, BA_NO_CONVERGENCE, BA_REASON_MAX_ITERATIONS, st->g2o_iter);
+        else if (st->g2o_trials >= o.g2o_max_trials || rho == 0.0 || !isfinite(st->radius) || !(st->radius > 0.0))
+            lm_finish(st, BA_CONVERGENCE, BA_REASON_G2O_STOP, rho);
         return;
     }
     const bool valid = have_step && model > 0.0 && isfinite(dx2);
     if (!valid) {
         st->step_invalid = 0;
         if (++st->consecutive_invalid >= o.max_invalid) {
-            st->done = 1;
-            st->termination = BA_FAILURE;
+            lm_finish(st, BA_FAILURE, BA_REASON_INVALID_STEPS, st->consecutive_invalid);
             return;
         }
         st->radius = st->radius / st->decrease_factor;
@@ -1638,15 +1643,13 @@ __global__ void __launch_bounds__(kWG) k_decide(Dev<Real> d) {
         const double step_norm = sqrt(dx2), x_norm = sqrt(x2);
         row.step_norm = step_norm;
         if (step_norm <= o.ptol * (x_norm + o.ptol)) {
-            st->done = 1;
-            st->termination = BA_CONVERGENCE;
+            lm_finish(st, BA_CONVERGENCE, BA_REASON_PARAMETER_TOLERANCE, step_norm / (x_norm + o.ptol));
             return;
         }
         const double cost_change = st->x_cost - cand;
         row.cost_change = cost_change;
         if (fabs(cost_change) <= o.ftol * st->x_cost) {
-            st->done = 1;
-            st->termination = BA_CONVERGENCE;
+            lm_finish(st, BA_CONVERGENCE, BA_REASON_FUNCTION_TOLERANCE, fabs(cost_change) / st->x_cost);
             return;
         }
         const double rho = cost_change / model;
@@ -1671,18 +1674,15 @@ __global__ void __launch_bounds__(kWG) k_decide(Dev<Real> d) {
         row.gradient_max_norm = st->gmax;
         row.trust_region_radius = st->radius;
     }
+    row.cumulative_time_in_seconds = lm_ticks(st);
     if (st->trace_len < d.trace_cap) d.trace[st->trace_len] = row;
     st->trace_len++;
-    if (st->iteration >= o.max_iterations) {
-        st->done = 1;
-        st->termination = BA_NO_CONVERGENCE;
-    } else if (!st->accepted && st->gmax <= o.gtol) {
-        st->done = 1;
-        st->termination = BA_CONVERGENCE;
-    } else if (!st->accepted && st->radius <= o.min_radius) {
-        st->done = 1;
-        st->termination = BA_CONVERGENCE;
-    }
+    if (st->iteration >= o.max_iterations)
+        lm_finish(st, BA_NO_CONVERGENCE, BA_REASON_MAX_ITERATIONS, st->iteration);
+    else if (!st->accepted && st->gmax <= o.gtol)
+        lm_finish(st, BA_CONVERGENCE, BA_REASON_GRADIENT_TOLERANCE, st->gmax);
+    else if (!st->accepted && st->radius <= o.min_radius)
+        lm_finish(st, BA_CONVERGENCE, BA_REASON_MIN_RADIUS, st->radius);
 }
 
 // Accepted step: x <- candidate (cameras then points, one thread per parameter).

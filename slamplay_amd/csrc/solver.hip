@@ -206,6 +206,7 @@ struct Solver final : SolverBase {
     hipStream_t s = nullptr;
     int device = 0;
     int linear_solver = BA_SOLVER_ITERATIVE_SCHUR;
+    double clock_khz = 100000.0;  // rate of the device wall clock (trace row times)
     Layout L;
     Dev<Real> d{};
     DevBuf<int> cam_off, co_pt, chunk_off, chunk_len, slot_of, co_side;
@@ -274,6 +275,9 @@ struct Solver final : SolverBase {
         opt = o;
         device = o.device;
         BA_TRY(use_device(device));
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+            clock_khz = khz;
         BA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         BA_HIP(hipHostMalloc((void**)&st_host, sizeof(LMState), hipHostMallocDefault));
         BA_TRY(build_layout(v, &L));
@@ -602,6 +606,8 @@ struct Solver final : SolverBase {
         sm->num_iterations = 0;
         sm->num_successful_steps = sm->num_unsuccessful_steps = 0;
         sm->termination_type = BA_CONVERGENCE;
+        sm->termination_reason = BA_REASON_NONE;
+        sm->termination_value = 0.0;
         sm->linear_solver_used = linear_solver;
         sm->total_linear_solver_iterations = 0;
         sm->total_time_ms = 0.0;
@@ -621,6 +627,13 @@ struct Solver final : SolverBase {
                 r.cost_change *= 2.0;
                 r.model_cost_change *= 2.0;
             }
+        // device clock ticks since the first evaluation -> seconds; per-row deltas
+        double prev = 0.0;
+        for (ba_iteration& r : rows) {
+            r.cumulative_time_in_seconds /= clock_khz * 1e3;
+            r.iteration_time_in_seconds = r.cumulative_time_in_seconds - prev;
+            prev = r.cumulative_time_in_seconds;
+        }
         const double cscale = opt.mode == BA_MODE_G2O ? 2.0 : 1.0;
         sm->initial_cost = n ? rows[0].cost : cscale * S.x_cost;
         sm->final_cost = cscale * S.x_cost;
@@ -628,6 +641,8 @@ struct Solver final : SolverBase {
         sm->num_successful_steps = S.num_successful;
         sm->num_unsuccessful_steps = S.num_unsuccessful;
         sm->termination_type = S.termination;
+        sm->termination_reason = S.reason;
+        sm->termination_value = S.reason_value;
         sm->linear_solver_used = linear_solver;
         sm->total_linear_solver_iterations = S.total_pcg_iters;
         sm->total_time_ms = ms;
@@ -642,11 +657,12 @@ struct Solver final : SolverBase {
     std::vector<ba_iteration> last_rows;
 
     void print_trace() {
-        printf("iter      cost      cost_change  |gradient|   |step|    tr_ratio  tr_radius  ls_iter\n");
+        printf("iter      cost      cost_change  |gradient|   |step|    tr_ratio  tr_radius  ls_iter  iter_time  "
+               "total_time\n");
         for (const ba_iteration& r : last_rows)
-            printf("%4d % 8.6e % 3.2e % 3.2e % 3.2e % 3.2e % 3.2e %4d\n", r.iteration, r.cost, r.cost_change,
-                   r.gradient_max_norm, r.step_norm, r.relative_decrease, r.trust_region_radius,
-                   r.linear_solver_iterations);
+            printf("%4d % 8.6e % 3.2e % 3.2e % 3.2e % 3.2e % 3.2e %4d % 3.2e % 3.2e\n", r.iteration, r.cost,
+                   r.cost_change, r.gradient_max_norm, r.step_norm, r.relative_decrease, r.trust_region_radius,
+                   r.linear_solver_iterations, r.iteration_time_in_seconds, r.cumulative_time_in_seconds);
         fflush(stdout);
     }
 
@@ -663,14 +679,16 @@ struct Solver final : SolverBase {
         // Algorithmic bytes of one launch: every stream counted once per pass, per-camera and per-point
         // parameters once per pass, FP64 parameters, int32 indices.
         const bool dir = d.model == kModelSnavely && linear_solver == BA_SOLVER_ITERATIVE_SCHUR;
-        // direction form: point index, Huber weight, slot, u out (jtx); point index, weight, point
-        // and its zs (jz); per camera its prep and the 9-vectors read and written
+        // direction form: the observation record (X - c, Huber weight), slot, u out (jtx); the record,
+        // point index and the point's FP64 zs (jz); per camera its prep and the 9-vectors read and
+        // written
         const double prep_b = sizeof(CameraPrep<Real>);
         if (n == "jtx" && dir) return no * (4 * R + 4 + 4 * R) + nc * (prep_b + 9 * 8 * 4 + 2 * 8 * d.n_upd);
-        if (n == "jz" && dir) return no * (4 * R + 4) + np * 4 * R + nc * (prep_b + 9 * 8 * 5);
+        if (n == "jz" && dir) return no * (4 * R + 4) + np * 32 + nc * (prep_b + 9 * 8 * 5);
         if (n == "jtx" || n == "jtx_y") return no * (4 + 4 + 2 * R + 4 * R) + np * 4 * 8 + nc * 9 * 8 * 3;
         if (n == "jz") return no * (4 + 2 * R) + np * (4 * 8 + 4 * R) + nc * 9 * 8 * 8;
-        if (n == "pt_z") return no * 4 * R + np * (3 * 8 + 3 * 8 + 6 * R + 4 * R);
+        // u by slot; per point its scale, FP64 V*^-1 (6), D_p^2, and the FP64 zs written
+        if (n == "pt_z") return no * 4 * R + np * (3 * 8 + 6 * 8 + 3 * 8 + 32);
         if (n == "lin_cam") return no * (4 + 4 + 2 * R + 12 * 8) + np * 4 * 8 + nc * 18 * 8;
         if (n == "lin_pt") return no * 12 * 8 + np * 9 * 8;
         if (n == "rhs_cam") return no * (4 + 2 * R) + np * (3 * 8 + 3 * 8 + 6 * R + 3 * R) + nc * (81 * 3 + 9 * 3) * 8;

@@ -25,8 +25,9 @@ struct WinState {
     int rows;       // trace rows of the equivalent single-problem solve (iteration 0 included)
     int invalid;    // this iteration's damped point blocks were not all positive definite
     int have_step;  // this iteration produced a step
-    int pad;
+    int reason;     // ba_termination_reason of the test that ended the solve
     unsigned long long t_done;  // device wall clock when the window finished
+    double reason_value;        // the quantity that test compared
 };
 
 struct WinDesc {

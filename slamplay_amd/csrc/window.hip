@@ -546,6 +546,8 @@ void fill_summary(const WinState& st, ba_summary* sm) {
     sm->num_successful_steps = st.num_successful;
     sm->num_unsuccessful_steps = st.num_unsuccessful;
     sm->termination_type = st.termination;
+    sm->termination_reason = st.reason;
+    sm->termination_value = st.reason_value;
     sm->linear_solver_used = BA_SOLVER_DENSE_SCHUR;
     sm->total_linear_solver_iterations = st.iteration;
 }

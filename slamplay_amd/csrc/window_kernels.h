@@ -673,26 +673,16 @@ __global__ void __launch_bounds__(NT) k_wdamp(const WinDesc* __restrict__ descs,
                 for (int c = 0; c < K; ++c) cost += w.cpart[WC_COUNT * (size_t)c + WC_COST];
                 st->x_cost = st->initial_cost = cost;
                 st->rows = 1;
-                if (!isfinite(cost)) {
-                    st->done = 1;
-                    st->termination = BA_FAILURE;
-                }
+                if (!isfinite(cost)) lm_finish(st, BA_FAILURE, BA_REASON_NON_FINITE_COST, cost);
             }
             st->gmax = gm;
             st->need_lin = 0;
             st->first_lin = 0;
-            if (!st->done && first && st->iteration >= o.max_iterations) {
-                st->done = 1;
-                st->termination = BA_NO_CONVERGENCE;
-            }
-            if (!st->done && gm <= o.gtol) {
-                st->done = 1;
-                st->termination = BA_CONVERGENCE;
-            }
-            if (!st->done && st->radius <= o.min_radius) {
-                st->done = 1;
-                st->termination = BA_CONVERGENCE;
-            }
+            if (!st->done && first && st->iteration >= o.max_iterations)
+                lm_finish(st, BA_NO_CONVERGENCE, BA_REASON_MAX_ITERATIONS, st->iteration);
+            if (!st->done && gm <= o.gtol) lm_finish(st, BA_CONVERGENCE, BA_REASON_GRADIENT_TOLERANCE, gm);
+            if (!st->done && st->radius <= o.min_radius)
+                lm_finish(st, BA_CONVERGENCE, BA_REASON_MIN_RADIUS, st->radius);
         }
         if (tid == 0 && st->done) st->t_done = wall_clock64();
         __syncthreads();
@@ -1143,8 +1133,7 @@ __global__ void __launch_bounds__(NT) k_wdecide(const WinDesc* __restrict__ desc
         bool record = true;
         if (!valid) {
             if (++st->consecutive_invalid >= o.max_invalid) {
-                st->done = 1;
-                st->termination = BA_FAILURE;
+                lm_finish(st, BA_FAILURE, BA_REASON_INVALID_STEPS, st->consecutive_invalid);
                 record = false;
             } else {
                 st->radius = st->radius / st->decrease_factor;
@@ -1156,14 +1145,12 @@ __global__ void __launch_bounds__(NT) k_wdecide(const WinDesc* __restrict__ desc
             if (!isfinite(cand)) cand = DBL_MAX;
             const double step_norm = sqrt(dx2), x_norm = sqrt(x2);
             if (step_norm <= o.ptol * (x_norm + o.ptol)) {
-                st->done = 1;
-                st->termination = BA_CONVERGENCE;
+                lm_finish(st, BA_CONVERGENCE, BA_REASON_PARAMETER_TOLERANCE, step_norm / (x_norm + o.ptol));
                 record = false;
             } else {
                 const double cost_change = st->x_cost - cand;
                 if (fabs(cost_change) <= o.ftol * st->x_cost) {
-                    st->done = 1;
-                    st->termination = BA_CONVERGENCE;
+                    lm_finish(st, BA_CONVERGENCE, BA_REASON_FUNCTION_TOLERANCE, fabs(cost_change) / st->x_cost);
                     record = false;
                 } else {
                     const double rho = cost_change / model;
@@ -1185,16 +1172,12 @@ __global__ void __launch_bounds__(NT) k_wdecide(const WinDesc* __restrict__ desc
         }
         if (record) st->rows++;
         if (record && !st->done) {
-            if (st->iteration >= o.max_iterations) {
-                st->done = 1;
-                st->termination = BA_NO_CONVERGENCE;
-            } else if (!accepted && st->gmax <= o.gtol) {
-                st->done = 1;
-                st->termination = BA_CONVERGENCE;
-            } else if (!accepted && st->radius <= o.min_radius) {
-                st->done = 1;
-                st->termination = BA_CONVERGENCE;
-            }
+            if (st->iteration >= o.max_iterations)
+                lm_finish(st, BA_NO_CONVERGENCE, BA_REASON_MAX_ITERATIONS, st->iteration);
+            else if (!accepted && st->gmax <= o.gtol)
+                lm_finish(st, BA_CONVERGENCE, BA_REASON_GRADIENT_TOLERANCE, st->gmax);
+            else if (!accepted && st->radius <= o.min_radius)
+                lm_finish(st, BA_CONVERGENCE, BA_REASON_MIN_RADIUS, st->radius);
         }
         if (st->done) st->t_done = wall_clock64();
     }

@@ -5,7 +5,8 @@
 //
 // Loads the BAL file, Normalize(), Perturb(0.1, 0.5, 0.5) on glibc rand() (this process never calls
 // srand, as the reference's driver), writes initial.ply, solves on the GPU in place, writes
-// final.ply, and prints one line: "initial_cost final_cost iterations termination".  With "g2o" the
+// final.ply, prints the summary's FullReport() as the reference does after ceres::Solve (:101), then
+// one line: "initial_cost final_cost iterations termination".  With "g2o" the
 // solve uses the g2o driver's rules and its optimize(40) (test_bundle_adjustment_g2o.cpp:166-218).
 #include <cstdio>
 #include <iostream>
@@ -46,6 +47,7 @@ int main(int argc, char** argv) {
             return 1;
         }
         bal_problem.WriteToPLYFile(out_dir + "final.ply");
+        std::cout << summary.FullReport() << "\n" << std::flush;
         std::printf("%.17g %.17g %d %d\n", summary.initial_cost, summary.final_cost, summary.num_iterations,
                     summary.termination_type);
     } catch (const std::exception& e) {

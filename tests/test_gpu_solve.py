@@ -47,6 +47,28 @@ def test_fp64_fixed_schedule_matches_oracle(oracle_mod, solver):
     np.testing.assert_allclose(q.mutable_points(), ref["pts"], rtol=0, atol=1e-6 * np.abs(ref["pts"]).max())
 
 
+def test_termination_reason_and_trace_times():
+    """termination_reason / termination_value name the test that ended the solve and the quantity it
+    compared (Ceres's termination message in FullReport); trace row times are device time since the
+    first evaluation: 0 on row 0, nondecreasing, within the solve's wall time."""
+    p = S.synthetic(1, seed=21)
+    out = S.SolveBA(_copy(p), max_num_iterations=3, **FIXED)
+    assert (out["termination"], out["termination_reason"], out["termination_value"]) == (1, 1, 3.0)
+    assert "Number of iterations: 3." in out["full_report"]
+    out = S.SolveBA(_copy(p), gradient_tolerance=1e30)
+    assert (out["termination"], out["termination_reason"], out["num_iterations"]) == (0, 3, 1)
+    assert out["termination_value"] == out["trace"][0]["gradient_max_norm"]
+    out = S.SolveBA(_copy(p), function_tolerance=1e-3, gradient_tolerance=0.0, parameter_tolerance=0.0)
+    assert out["termination"] == 0 and out["termination_reason"] == 2
+    assert 0.0 <= out["termination_value"] <= 1e-3
+    assert "CONVERGENCE (Function tolerance reached." in out["full_report"]
+    t = np.array([r["cumulative_time_in_seconds"] for r in out["trace"]])
+    dt = np.array([r["iteration_time_in_seconds"] for r in out["trace"]])
+    assert len(t) >= 3 and t[0] == 0.0 and dt[0] == 0.0 and np.all(dt[1:] > 0.0)
+    assert t[-1] <= out["total_time_ms"] * 1e-3
+    np.testing.assert_allclose(np.cumsum(dt), t, rtol=1e-12, atol=1e-15)
+
+
 def test_ladybug49_reference_driver_matches_oracle(oracle_mod, ladybug_path):
     """The reference driver's pipeline: load, Normalize, Perturb(0.1, 0.5, 0.5), SolveBA with default
     (Ceres) options: 50 iterations max, function/gradient/parameter tolerances, dense Schur.
@@ -354,7 +376,10 @@ def test_cpp_drop_in_driver(oracle_mod, ladybug_path, tmp_path, mode):
     args = [driver, ladybug_path, str(tmp_path)] + (["g2o"] if mode == "g2o" else [])
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    ic, fc, iters, term = r.stdout.split()
+    lines = r.stdout.strip().splitlines()
+    ic, fc, iters, term = lines[-1].split()
+    report = "\n".join(lines[:-1])
+    assert "Solver Summary" in report and "Termination:" in report
     p = S.BALProblem(ladybug_path)
     p.Normalize()
     oracle_mod.lib().oracle_srand(1)  # the ba_driver process never calls srand: glibc's seed 1

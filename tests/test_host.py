@@ -138,7 +138,44 @@ def test_library_exports_every_declared_symbol():
     L = C.lib()
     missing = [n for n in sorted(names) if not hasattr(L, n)]
     assert not missing, missing
-    assert L.ba_abi_version() == 1
+    assert L.ba_abi_version() == 2
+
+
+def test_full_report_text():
+    """ba_summary_full_report lays out a summary as Ceres's Summary::FullReport() (the reference
+    driver prints it after ceres::Solve): size table from the problem, costs, step counts, the
+    trace's device time and the termination test with its value and tolerance."""
+    s, rows = S._summary(4)
+    s.initial_cost, s.final_cost = 4.185660e+06, 1.803390e+04
+    s.num_iterations, s.num_successful_steps, s.num_unsuccessful_steps = 3, 2, 0
+    s.termination_type, s.linear_solver_used, s.total_time_ms = 0, 2, 250.0
+    s.termination_reason, s.termination_value = 2, 1.769717e-09
+    s.total_linear_solver_iterations = 37
+    rows[2].cumulative_time_in_seconds = 0.125
+    o = S.default_options()
+    text = S.full_report(s, 16, 22106, 83718, o)
+    table = {ln[:25].strip(): ln[25:].split() for ln in text.splitlines() if len(ln) > 25}
+    assert table["Parameter blocks"] == ["22122", "22122"]
+    assert table["Parameters"] == [str(9 * 16 + 3 * 22106)] * 2
+    assert table["Residual blocks"] == ["83718", "83718"] and table["Residuals"] == ["167436", "167436"]
+    assert table["Linear solver"] == ["SPARSE_SCHUR", "ITERATIVE_SCHUR"]
+    assert table["Linear solver ordering"] == ["AUTOMATIC", "22106,16"]
+    assert float(table["Initial"][0]) == 4.185660e+06 and float(table["Final"][0]) == 1.803390e+04
+    assert float(table["Change"][0]) == pytest.approx(4.185660e+06 - 1.803390e+04, rel=1e-6)
+    assert table["Minimizer iterations"] == ["2"] and table["Successful steps"] == ["2"]
+    assert table["Linear solver iterations"] == ["37"]
+    assert float(table["Minimizer (device)"][0]) == 0.125 and float(table["Total"][0]) == 0.25
+    assert text.rstrip().endswith("CONVERGENCE (Function tolerance reached. |cost_change|/cost: 1.769717e-09 "
+                                  "<= 1.000000e-06)")
+    s.termination_type, s.termination_reason, s.termination_value = 1, 1, 50
+    assert "NO_CONVERGENCE (Maximum number of iterations reached. Number of iterations: 50.)" in \
+        S.full_report(s, 16, 22106, 83718, o)
+    # truncation: NUL-terminated prefix, full length reported
+    n = C.i64(0)
+    buf = ctypes.create_string_buffer(11)
+    assert C.lib().ba_summary_full_report(ctypes.byref(s), 1, 1, 1, None, buf, 11, ctypes.byref(n)) == 0
+    assert len(buf.value) == 10 and n.value > 100
+    assert C.lib().ba_summary_full_report(None, 1, 1, 1, None, buf, 11, None) == 1
 
 
 def test_invalid_arguments_rejected_before_device_use(ladybug_path):

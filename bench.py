@@ -21,7 +21,9 @@ beside the FP64 single-problem solver run over the same mix one request at a tim
 The roofline entry is the dominant kernel's algorithmic bytes per launch over its average duration,
 timed with HIP events on the solver's stream, beside the spec HBM peak and a measured 1 GiB device
 copy; cpu_baseline is the FP64 CPU oracle (one thread) on a bounded sample of the same problem
-(4 LM iterations), cpu_baseline_all_cores the same on up to 16 host threads.  The device
+(median of 5 runs of 2 LM iterations after a warm-up, less the median set-up and initial
+linearization, which is reported beside it), cpu_baseline_all_cores the same on up to 16 host
+threads.  The device
 Normalize / Perturb of configuration 3 are timed against the host BALProblem methods.
 """
 import argparse
@@ -82,25 +84,40 @@ def profile_kernels(s):
     return kt
 
 
-CPU_ITERS = 4
+CPU_ITERS = 2  # LM iterations of a timed CPU run
+CPU_RUNS = 5
 
 
 def cpu_baseline(S, p, threads=1):
-    """The FP64 CPU oracle on the first CPU_ITERS LM iterations of the same problem and options;
-    LM iterations per second, the initial linearization included.  With threads > 1 the oracle splits
-    its loops by camera / point (same results, bitwise)."""
+    """The FP64 CPU oracle on the same problem and options: after one warm-up, CPU_RUNS runs that
+    stop after the initial linearization and CPU_RUNS runs of CPU_ITERS LM iterations; LM iterations
+    per second = CPU_ITERS / (median of the second - median of the first), the set-up and initial
+    linearization reported separately.  With threads > 1 the oracle splits its loops by camera /
+    point (same results, bitwise)."""
+    import numpy as np
     from oracle import oracle as O
     threads = O.set_threads(threads)
-    o = O.default_options(max_num_iterations=CPU_ITERS, linear_solver=2, pcg_max_iterations=50, pcg_tolerance=1e-6,
-                          function_tolerance=0.0, gradient_tolerance=0.0, parameter_tolerance=0.0)
-    t0 = time.perf_counter()
-    O.solve(p.num_cameras(), p.num_points(), p.camera_index(), p.point_index(), p.observations(),
-            p.mutable_cameras(), p.mutable_points(), o)
-    dt = time.perf_counter() - t0
+
+    def run(iters):
+        o = O.default_options(max_num_iterations=iters, linear_solver=2, pcg_max_iterations=50,
+                              pcg_tolerance=1e-6, function_tolerance=0.0, gradient_tolerance=0.0,
+                              parameter_tolerance=0.0)
+        t0 = time.perf_counter()
+        O.solve(p.num_cameras(), p.num_points(), p.camera_index(), p.point_index(), p.observations(),
+                p.mutable_cameras(), p.mutable_points(), o)
+        return time.perf_counter() - t0
+
+    run(0)  # warm-up
+    t_lin = float(np.median([run(0) for _ in range(CPU_RUNS)]))
+    t_run = [run(CPU_ITERS) for _ in range(CPU_RUNS)]
+    t_it = float(np.median(t_run))
     O.set_threads(1)
-    return {"value": CPU_ITERS / dt, "unit": "LM iters/s", "cores": threads, "kind": "port",
-            "sample": "FP64 CPU oracle, %d thread(s): initial linearization + %d LM iterations (PCG <= 50, tol 1e-6) "
-                      "of the full 1.2M-observation problem, %.1f s" % (threads, CPU_ITERS, dt)}
+    per_iter = max(t_it - t_lin, 1e-9) / CPU_ITERS
+    return {"value": 1.0 / per_iter, "unit": "LM iters/s", "cores": threads, "kind": "port",
+            "linearization_s": t_lin, "run_s": t_run,
+            "sample": "FP64 CPU oracle, %d thread(s), full 1.2M-observation problem (PCG <= 50, tol 1e-6): median "
+                      "of %d runs of %d LM iterations (%.2f s) less the median set-up + initial linearization "
+                      "(%.2f s), after one warm-up" % (threads, CPU_RUNS, CPU_ITERS, t_it, t_lin)}
 
 
 WINDOW_ITERS = 10

@@ -273,21 +273,21 @@ __device__ __forceinline__ double strided_sum(const double* __restrict__ a, int 
     return s;
 }
 
-// Sums N per-thread values over the 256-thread workgroup. On return lds[i * kWaves] holds the
-// total of value i for every thread (a barrier is included).
-template <int N, typename T>
+// Sums N per-thread values over a workgroup of NW waves (256 threads by default). On return
+// lds[i * NW] holds the total of value i for every thread (a barrier is included).
+template <int N, typename T, int NW = kWaves>
 __device__ __forceinline__ void block_sum(T (&v)[N], T* lds) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         const T s = wave_sum(v[i]);
-        if (lane == 0) lds[i * kWaves + wave] = s;
+        if (lane == 0) lds[i * NW + wave] = s;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < N; i += kWG) {
-        T s = lds[i * kWaves];
-        for (int w = 1; w < kWaves; ++w) s += lds[i * kWaves + w];
-        lds[i * kWaves] = s;
+    for (int i = threadIdx.x; i < N; i += 64 * NW) {
+        T s = lds[i * NW];
+        for (int w = 1; w < NW; ++w) s += lds[i * NW + w];
+        lds[i * NW] = s;
     }
     __syncthreads();
 }
@@ -1263,11 +1263,12 @@ __global__ void __launch_bounds__(kWG) k_jtx_dir(Dev<Real> d, int it) {
 // q_c = D_c^2 p_c + S_c sum Jc' e_j with e_j = Jc S_c p_c - Jp zs_p; partial sum |e|^2 + p'D_c^2 p.
 // Per observation only g = D'(w e) is formed (one gather: the point's zs); the camera accumulates
 // G = sum g (X - c)', sum g and the intrinsic sums, and contracts G + (sum g) c' with dR/dw once.
-template <typename Real>
-__global__ void __launch_bounds__(kWG) k_jz_dir(Dev<Real> d) {
+template <typename Real, int NT = kWG>
+__global__ void __launch_bounds__(NT) k_jz_dir(Dev<Real> d) {
+    constexpr int NW = NT / 64;
     LMState* st = d.st;
     if (st->done || st->step_invalid || st->pcg_done) return;
-    __shared__ double lds[16 * kWaves];
+    __shared__ double lds[16 * NW];
     __shared__ double psh[9], vsh[9], rsh[9], qsh[9];
     const int c = xcd_remap(blockIdx.x, d.nc);
     if (c >= d.nc) return;
@@ -1300,12 +1301,12 @@ __global__ void __launch_bounds__(kWG) k_jz_dir(Dev<Real> d) {
     double acc[16];  // G (row-major 3x3), sum g, intrinsic sums (dist, r2, r2^2 times x'e0 + y'e1), |e|^2
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.0;
-    for (; j < j_end; j += kWG) {
+    for (; j < j_end; j += NT) {
         const R4<Real> cur = rec;
         const double4 z = zin;
-        if (j + kWG < j_end) {
-            rec = d.co_rec[j + kWG];
-            zin = d.zs[d.co_pt[j + kWG]];
+        if (j + NT < j_end) {
+            rec = d.co_rec[j + NT];
+            zin = d.zs[d.co_pt[j + NT]];
         }
         Real dq[3];
         jz_difference<Real>(P, rsh, vt64, cur, z, dq);
@@ -1331,18 +1332,18 @@ __global__ void __launch_bounds__(kWG) k_jz_dir(Dev<Real> d) {
         acc[13] += r2 * se;
         acc[14] += r2 * r2 * se;
     }
-    block_sum<16>(acc, lds);
+    block_sum<16, double, NW>(acc, lds);
     const int t = threadIdx.x;
     if (t == 0) {
         double cc[3], G[9], w3[3];
         camera_centre<Real>(P, cc);
-        const double gs[3] = {lds[9 * kWaves], lds[10 * kWaves], lds[11 * kWaves]};
+        const double gs[3] = {lds[9 * NW], lds[10 * NW], lds[11 * NW]};
         for (int i = 0; i < 3; ++i)
-            for (int m = 0; m < 3; ++m) G[3 * i + m] = lds[(3 * i + m) * kWaves] + gs[i] * cc[m];
+            for (int m = 0; m < 3; ++m) G[3 * i + m] = lds[(3 * i + m) * NW] + gs[i] * cc[m];
         rot_contract<Real>(P, G, w3);
         const double f = (double)P.f;
         const double qv[9] = {w3[0], w3[1], w3[2], gs[0], gs[1], gs[2],
-                              lds[12 * kWaves], f * lds[13 * kWaves], f * lds[14 * kWaves]};
+                              lds[12 * NW], f * lds[13 * NW], f * lds[14 * NW]};
         for (int k = 0; k < 9; ++k) qsh[k] = qv[k];
     }
     __syncthreads();
@@ -1351,7 +1352,7 @@ __global__ void __launch_bounds__(kWG) k_jz_dir(Dev<Real> d) {
         d.qc[i] = d.dc2[i] * psh[t] + d.sc[i] * qsh[t];
     }
     if (t == 0) {
-        double s = lds[15 * kWaves];
+        double s = lds[15 * NW];
         for (int i = 0; i < 9; ++i) s += psh[i] * d.dc2[9 * (size_t)c + i] * psh[i];
         d.cam_part[PS_A * (size_t)d.nc + c] = s;
     }

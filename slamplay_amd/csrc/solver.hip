@@ -12,6 +12,8 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "dense.h"
 #include "lm_kernels.h"
 #include "preprocess.h"
@@ -200,6 +202,9 @@ void build_pair_blocks(const Layout& L, PairBlocks* out) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// Threads per workgroup of k_jz_dir (one workgroup per camera).
+constexpr int kJzWG = 256;
+
 template <typename Real>
 struct Solver final : SolverBase {
     ba_options opt{};
@@ -473,12 +478,13 @@ struct Solver final : SolverBase {
         if (profiling) {
             a = next_event();
             b = next_event();
-            if (a) (void)hipEventRecord(a, s);
         }
-        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shm, s, args...);
-        if (profiling && a && b) {
-            (void)hipEventRecord(b, s);
+        if (a && b) {
+            // the events are stamped by the dispatch itself: the kernel's own start and end
+            hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), (std::uint32_t)shm, s, a, b, 0u, args...);
             recs.push_back({name, a, b});
+        } else {
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shm, s, args...);
         }
     }
     template <typename K, typename... Args>
@@ -511,7 +517,7 @@ struct Solver final : SolverBase {
                     launch("jtx", BA_MODEL_KERNEL(k_jtx), gcam, kWG, d, 1, it, (const double*)d.pp);
                 launch("pt_z", k_pt_z<Real>, gpt, kWG, d);
                 if (dir)
-                    launch("jz", k_jz_dir<Real>, gcam, kWG, d);
+                    launch("jz", k_jz_dir<Real, kJzWG>, gcam, kJzWG, d);
                 else
                     launch("jz", BA_MODEL_KERNEL(k_jz), gcam, kWG, d);
                 launch("cam_update", k_cam_update<Real>, d.n_upd, kWG, d, it);

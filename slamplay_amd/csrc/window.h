@@ -35,18 +35,14 @@ struct WinDesc {
     int64_t o_cam_off, o_co_pt, o_co_uv64, o_slot_of, o_chunk_len, o_chunk_off, o_blk, o_pairs;
     int64_t o_cam_x, o_cam_x0, o_cam_c, o_pt_x, o_pt_x0, o_pt_c;
     int64_t o_prep, o_prep_c, o_udiag, o_gc, o_sc, o_dc2, o_bc, o_yc, o_cpart, o_ppart;
-    int64_t o_V, o_gp, o_sp, o_Vinv, o_qp, o_yp, o_pv, o_j64, o_js, o_S, o_L, o_state;  // o_L: 0 = factor in LDS
+    int64_t o_V, o_gp, o_sp, o_Vinv, o_qp, o_yp, o_pv, o_js, o_S, o_L, o_state;  // o_L: 0 = factor in LDS
 };
 
-// Per-observation records.
-//   j64 (FP64, 26 values, stored component-major j64[e][j]): corrected J_c (2x9, row-major),
-//       J_p (2x3), r (2) at the linearization point, unscaled -- written once per accepted step;
-//       the FP64 operator of the LM system;
+// Per-observation record (the FP64 residual and Jacobians are recomputed where needed):
 //   js (working precision, 36 values): scaled J~_c column by column (18: the two rows of column c
 //       at 2c, 2c + 1), J~_p (6, row-major), H = J~_p V*^-1 (6, row-major),
 //       I - G = I - J~_p V*^-1 J~_p' (3: g00, g01, g11), unused (3) -- the input of the reduced
 //       camera system, rewritten whenever the damping changes.
-constexpr int kWinJ64 = 26;
 constexpr int kWinJs = 36;
 // Partial sums per camera (cpart) and per point chunk (ppart).
 enum WinCamPart { WC_COST = 0, WC_GMAX, WC_MODEL, WC_CAND, WC_DX2, WC_X2, WC_COUNT = 8 };

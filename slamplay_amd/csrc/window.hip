@@ -169,7 +169,6 @@ int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowIm
     D.o_qp = take(32 * (int64_t)P);
     D.o_yp = take(24 * (int64_t)P);
     D.o_pv = take(96 * (int64_t)slots);
-    D.o_j64 = take(8 * kWinJ64 * (int64_t)No);
     D.o_js = take(kWinJs * sizeof(double) * (int64_t)No);
     D.o_S = take(sizeof(double) * (int64_t)n * n);  // zero: blocks without common points are never written
     D.o_L = n > max_window_n<Real>() ? take(sizeof(Real) * (int64_t)n * (n + 1) / 2) : 0;
@@ -428,14 +427,14 @@ struct WindowEngine {
                                         n_ptask, o);
         hipLaunchKernelGGL((k_wdamp<256>), dim3(n_wtask), dim3(256), 0, s, dd, arena.p, (const int*)wtask.p, n_wtask, o);
         if (n_ctask) hipLaunchKernelGGL((k_wrec_cam<double, kWinCamThreads>), dim3(n_ctask), dim3(kWinCamThreads), 0, s,
-                                        dd, arena.p, (const int2*)ctask.p, n_ctask);
+                                        dd, arena.p, (const int2*)ctask.p, n_ctask, o);
         if (n_stask)
             hipLaunchKernelGGL((k_wschur<double, NT>), dim3(n_stask), dim3(NT), (size_t)schur_lds<double>(NT), s, dd, arena.p,
                                (const int4*)stask.p, n_stask);
         hipLaunchKernelGGL((k_wsolve<Real, NT>), dim3(n_wtask), dim3(NT), solve_lds(), s,
                            dd, arena.p, (const int*)wtask.p, n_wtask, cg_iterations());
         if (n_ctask) hipLaunchKernelGGL((k_wback_cam<kWinCamThreads>), dim3(n_ctask), dim3(kWinCamThreads), 0, s, dd,
-                                        arena.p, (const int2*)ctask.p, n_ctask);
+                                        arena.p, (const int2*)ctask.p, n_ctask, o);
         if (n_ptask) hipLaunchKernelGGL(k_wback_pt, dim3(n_ptask), dim3(64), 0, s, dd, arena.p, (const int2*)ptask.p,
                                         n_ptask);
         if (n_ctask) hipLaunchKernelGGL((k_wcand_cam<kWinCamThreads>), dim3(n_ctask), dim3(kWinCamThreads), 0, s, dd,

@@ -1,9 +1,9 @@
 // window_kernels.h -- batched local bundle adjustment: the LM iteration of many windows as a
 // fixed sequence of phase kernels (the dense-Schur solve of ba_solve, restated for windows of a
 // few dozen cameras).  Each kernel covers every active window at once:
-//   k_wlin_cam    (window, camera) FP64 residuals and Jacobians of the camera's observations
-//                 into the j64 records, diag(J_c'J_c), J_c'r, J_p'J_p and J_p'r into slot order,
-//                 and at the start the FP64 cost (after an accepted step only);
+//   k_wlin_cam    (window, camera) FP64 residuals and Jacobians of the camera's observations:
+//                 diag(J_c'J_c), J_c'r, J_p'J_p and J_p'r into slot order, and at the start the
+//                 FP64 cost (after an accepted step only);
 //   k_wlin_pt     (window, point chunk) V and g_p from the slots;
 //   k_wdamp       (window) gradient test, Jacobi scales at the start, V*^-1 and q = V*^-1 g~_p
 //                 per point, D_c^2;
@@ -19,9 +19,12 @@
 //                 the FP64 step also where cond(S) is beyond what an FP32 factor resolves;
 //   k_wback_cam   (window, camera) u = J~_p'(r~ - J~_c y_c) into slot order (FP64);
 //   k_wback_pt    (window, point chunk) y_p = V*^-1 sum u, candidate points, |dx|^2, |x|^2;
-//   k_wcand_cam   (window, camera) candidate camera, model cost change from j64, FP64 candidate
-//                 cost;
+//   k_wcand_cam   (window, camera) candidate camera, model cost change, FP64 candidate cost;
 //   k_wdecide     (window) Ceres LM rule, accepted candidates copied into the parameters.
+// The per-observation FP64 residual and Jacobian are recomputed by each kernel that needs them
+// (k_wlin_cam, k_wrec_cam, k_wback_cam, k_wcand_cam) from the camera's prep, the point and the
+// measurement: about 300 FP64 operations against 26 stored doubles written once and read three
+// times per iteration.
 // Every kernel reads the window's LM state first and returns when it has nothing to do (or when its
 // task lies outside the window it names), so the sequence can be replayed as a graph.  Per-camera and per-chunk sums are written to their own
 // slots and combined by the per-window kernels in camera / chunk order, and every 9x9 block of S is
@@ -45,7 +48,7 @@ struct WinPtrs {
     const int2* pairs;
     const double* co_uv64;
     double *cam_x, *cam_c, *udiag, *gc, *sc, *dc2, *bc, *yc, *cpart, *ppart;
-    double *V, *gp, *sp, *Vinv, *yp, *j64;
+    double *V, *gp, *sp, *Vinv, *yp;
     double4 *pt_x, *pt_c, *pv, *qp;
     CameraPrep<double>*prep, *prep_c;
     WinState* st;
@@ -80,7 +83,6 @@ __device__ __forceinline__ WinPtrs win_ptrs(const WinDesc& D, char* arena) {
     w.sp = at<double>(arena, D.o_sp);
     w.Vinv = at<double>(arena, D.o_Vinv);
     w.yp = at<double>(arena, D.o_yp);
-    w.j64 = at<double>(arena, D.o_j64);
     w.pt_x = at<double4>(arena, D.o_pt_x);
     w.pt_c = at<double4>(arena, D.o_pt_c);
     w.pv = at<double4>(arena, D.o_pv);
@@ -111,22 +113,36 @@ __device__ __forceinline__ void wg_sum_fixed(double (&v)[N], double* lds) {
         }
 }
 
-// Component e of the FP64 record of observation j (stored component-major: j64[e][j], so the
-// lanes of a wave, which own consecutive observations, read consecutive addresses).
-__device__ __forceinline__ double j64_at(const WinPtrs& w, int e, int j) { return w.j64[(size_t)e * w.No + j]; }
+// Huber-corrected FP64 residual and Jacobians of observation j (point p) at the current parameters
+// (P: the camera's prep at its current parameters), as k_wlin_cam forms them.
+__device__ __forceinline__ void win_obs(const WinPtrs& w, const CameraPrep<double>& P, int j, int p, const Opts& o,
+                                        double r[2], double Jc[18], double Jp[6]) {
+    const double4 x4 = w.pt_x[p];
+    const double X[3] = {x4.x, x4.y, x4.z};
+    snavely_residual_jacobian<double>(P, X, w.co_uv64[2 * (size_t)j], w.co_uv64[2 * (size_t)j + 1], r, Jc, Jp);
+    const double s = huber_sqrt_rho1<double>(r[0] * r[0] + r[1] * r[1], o.loss, o.loss_a);
+#pragma unroll
+    for (int i = 0; i < 18; ++i) Jc[i] *= s;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) Jp[i] *= s;
+    r[0] *= s;
+    r[1] *= s;
+}
 
-// Scaled FP64 Jacobians of observation j: J~_c = J_c diag(s_c), J~_p = J_p diag(s_p).
-__device__ __forceinline__ void win_scaled(const WinPtrs& w, int j, const double sc[9], const double* sp, double Jc[18],
-                                           double Jp[6]) {
+// win_obs with the Jacobi scales applied: J~_c = J_c diag(s_c), J~_p = J_p diag(s_p).
+__device__ __forceinline__ void win_scaled(const WinPtrs& w, const CameraPrep<double>& P, int j, int p, const Opts& o,
+                                           const double sc[9], double r[2], double Jc[18], double Jp[6]) {
+    win_obs(w, P, j, p, o, r, Jc, Jp);
+    const double* sp = w.sp + 3 * (size_t)p;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-        Jc[i] = j64_at(w, i, j) * sc[i];
-        Jc[9 + i] = j64_at(w, 9 + i, j) * sc[i];
+        Jc[i] *= sc[i];
+        Jc[9 + i] *= sc[i];
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        Jp[i] = j64_at(w, 18 + i, j) * sp[i];
-        Jp[3 + i] = j64_at(w, 21 + i, j) * sp[i];
+        Jp[i] *= sp[i];
+        Jp[3 + i] *= sp[i];
     }
 }
 
@@ -559,7 +575,6 @@ __global__ void __launch_bounds__(NT) k_wlin_cam(const WinDesc* __restrict__ des
     double acc[19];
 #pragma unroll
     for (int i = 0; i < 19; ++i) acc[i] = 0.0;  // diag (9), J_c'r (9), cost
-    const size_t ld = (size_t)w.No;
     for (int j = w.cam_off[c] + threadIdx.x; j < w.cam_off[c + 1]; j += NT) {
         const int p = w.co_pt[j];
         const double4 x4 = w.pt_x[p];
@@ -569,15 +584,12 @@ __global__ void __launch_bounds__(NT) k_wlin_cam(const WinDesc* __restrict__ des
         const double s2 = r[0] * r[0] + r[1] * r[1];
         if (first) acc[18] += half_rho(s2, o.loss, o.loss_a);
         const double s = huber_sqrt_rho1<double>(s2, o.loss, o.loss_a);
-        double* rec = w.j64 + j;
 #pragma unroll
-        for (int i = 0; i < 18; ++i) rec[i * ld] = Jc[i] *= s;
+        for (int i = 0; i < 18; ++i) Jc[i] *= s;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) rec[(18 + i) * ld] = Jp[i] *= s;
+        for (int i = 0; i < 6; ++i) Jp[i] *= s;
         r[0] *= s;
         r[1] *= s;
-        rec[24 * ld] = r[0];
-        rec[25 * ld] = r[1];
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             acc[i] += Jc[i] * Jc[i] + Jc[9 + i] * Jc[9 + i];
@@ -726,10 +738,11 @@ __global__ void __launch_bounds__(NT) k_wdamp(const WinDesc* __restrict__ descs,
 // Per (window, camera): records of the reduced system and b_c.
 template <typename Real, int NT>
 __global__ void __launch_bounds__(NT) k_wrec_cam(const WinDesc* __restrict__ descs, char* __restrict__ arena,
-                                                 const int2* __restrict__ ctask, int ntask) {
+                                                 const int2* __restrict__ ctask, int ntask, Opts o) {
     constexpr int NW = NT / 64;
     __shared__ double lds[9 * NW];
     __shared__ double dacc[NW][45];  // per wave: upper triangle of sum J~_c'(I - G)J~_c
+    __shared__ CameraPrep<double> Psh;
     const int t = blockIdx.x;
     if (t >= ntask) return;
     const int2 tk = ctask[t];
@@ -739,6 +752,7 @@ __global__ void __launch_bounds__(NT) k_wrec_cam(const WinDesc* __restrict__ des
     if (st->done || st->invalid || tk.y >= w.K) return;
     Real* js = at<Real>(arena, D.o_js);
     const int c = tk.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) Psh = w.prep[c];
     double scc[9];
     for (int i = 0; i < 9; ++i) scc[i] = w.sc[9 * (size_t)c + i];
     double bacc[9];
@@ -755,8 +769,8 @@ __global__ void __launch_bounds__(NT) k_wrec_cam(const WinDesc* __restrict__ des
         for (int i = 0; i < 18; ++i) Jd[i] = 0.0;
         if (j < j_end) {
             const int p = w.co_pt[j];
-            double Jc[18], Jp[6];
-            win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
+            double r[2], Jc[18], Jp[6];
+            win_scaled(w, Psh, j, p, o, scc, r, Jc, Jp);
             const double* Vi = w.Vinv + 6 * (size_t)p;
             const double j0[3] = {Jp[0], Jp[1], Jp[2]}, j1[3] = {Jp[3], Jp[4], Jp[5]};
             double h0[3], h1[3];
@@ -781,8 +795,8 @@ __global__ void __launch_bounds__(NT) k_wrec_cam(const WinDesc* __restrict__ des
             out[31] = (Real)g01;
             out[32] = (Real)g11;
             const double4 q = w.qp[p];
-            const double e0 = j64_at(w, 24, j) - (Jp[0] * q.x + Jp[1] * q.y + Jp[2] * q.z);
-            const double e1 = j64_at(w, 25, j) - (Jp[3] * q.x + Jp[4] * q.y + Jp[5] * q.z);
+            const double e0 = r[0] - (Jp[0] * q.x + Jp[1] * q.y + Jp[2] * q.z);
+            const double e1 = r[1] - (Jp[3] * q.x + Jp[4] * q.y + Jp[5] * q.z);
 #pragma unroll
             for (int i = 0; i < 9; ++i) bacc[i] += Jc[i] * e0 + Jc[9 + i] * e1;
 #pragma unroll
@@ -960,7 +974,8 @@ __global__ void __launch_bounds__(NT) k_wsolve(const WinDesc* __restrict__ descs
 // Per (window, camera): u = J~_p'(r~ - J~_c y_c) into slot order.
 template <int NT>
 __global__ void __launch_bounds__(NT) k_wback_cam(const WinDesc* __restrict__ descs, char* __restrict__ arena,
-                                                  const int2* __restrict__ ctask, int ntask) {
+                                                  const int2* __restrict__ ctask, int ntask, Opts o) {
+    __shared__ CameraPrep<double> Psh;
     const int t = blockIdx.x;
     if (t >= ntask) return;
     const int2 tk = ctask[t];
@@ -969,16 +984,18 @@ __global__ void __launch_bounds__(NT) k_wback_cam(const WinDesc* __restrict__ de
     const WinState* st = w.st;
     if (st->done || !st->have_step || tk.y >= w.K) return;
     const int c = tk.y;
+    if (threadIdx.x == 0) Psh = w.prep[c];
     double scc[9], y[9];
     for (int i = 0; i < 9; ++i) {
         scc[i] = w.sc[9 * (size_t)c + i];
         y[i] = w.yc[9 * (size_t)c + i];
     }
+    __syncthreads();
     for (int j = w.cam_off[c] + threadIdx.x; j < w.cam_off[c + 1]; j += NT) {
         const int p = w.co_pt[j];
-        double Jc[18], Jp[6];
-        win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
-        double a0 = j64_at(w, 24, j), a1 = j64_at(w, 25, j);
+        double r[2], Jc[18], Jp[6];
+        win_scaled(w, Psh, j, p, o, scc, r, Jc, Jp);
+        double a0 = r[0], a1 = r[1];
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             a0 -= Jc[i] * y[i];
@@ -1031,7 +1048,7 @@ template <int NT>
 __global__ void __launch_bounds__(NT) k_wcand_cam(const WinDesc* __restrict__ descs, char* __restrict__ arena,
                                                   const int2* __restrict__ ctask, int ntask, Opts o) {
     __shared__ double lds[2 * (NT / 64)];
-    __shared__ CameraPrep<double> Psh;
+    __shared__ CameraPrep<double> Psh, Pxs;  // the candidate's prep, the current one's
     __shared__ double csh[9];
     const int t = blockIdx.x;
     if (t >= ntask) return;
@@ -1047,6 +1064,7 @@ __global__ void __launch_bounds__(NT) k_wcand_cam(const WinDesc* __restrict__ de
         w.cam_c[k] = v;
         csh[threadIdx.x] = v;
     }
+    if (threadIdx.x == NT - 1) Pxs = w.prep[c];
     __syncthreads();
     if (threadIdx.x == 0) camera_prep<double>(csh, Psh);
     __syncthreads();
@@ -1058,8 +1076,8 @@ __global__ void __launch_bounds__(NT) k_wcand_cam(const WinDesc* __restrict__ de
     double acc[2] = {0.0, 0.0};  // model, cost
     for (int j = w.cam_off[c] + threadIdx.x; j < w.cam_off[c + 1]; j += NT) {
         const int p = w.co_pt[j];
-        double Jc[18], Jp[6];
-        win_scaled(w, j, scc, w.sp + 3 * (size_t)p, Jc, Jp);
+        double rx[2], Jc[18], Jp[6];
+        win_scaled(w, Pxs, j, p, o, scc, rx, Jc, Jp);
         const double* yp = w.yp + 3 * (size_t)p;
         double m0 = 0.0, m1 = 0.0;
         for (int i = 0; i < 9; ++i) {
@@ -1070,7 +1088,7 @@ __global__ void __launch_bounds__(NT) k_wcand_cam(const WinDesc* __restrict__ de
             m0 -= Jp[i] * yp[i];
             m1 -= Jp[3 + i] * yp[i];
         }
-        acc[0] += m0 * (j64_at(w, 24, j) + 0.5 * m0) + m1 * (j64_at(w, 25, j) + 0.5 * m1);
+        acc[0] += m0 * (rx[0] + 0.5 * m0) + m1 * (rx[1] + 0.5 * m1);
         const double4 x4 = w.pt_c[p];
         const double X[3] = {x4.x, x4.y, x4.z};
         double r[2];

@@ -265,10 +265,12 @@ int ba_solver_info(ba_solver* solver, int64_t* device_bytes, int32_t* linear_sol
  * The caller of windowed BA is full_slam's local bundle adjustment (full_slam/src/backend.cpp:67-199,
  * 7-10 keyframes); here every window is a Snavely BA problem solved exactly like ba_solve with the
  * dense Schur solver (per window: explicit reduced camera system + Cholesky), each with its own LM
- * state.  One workgroup owns one window for a whole LM iteration, so a window's result does not
- * depend on which other windows share the launch.  A window's reduced camera system is factored in
- * the workgroup's LDS: up to 21 cameras in mixed precision, 14 in FP64; larger windows are solved by
- * the single-problem solver (ba_solver) and reported the same way. */
+ * state.  Every sum is formed in a fixed order, so a window's result does not depend on which other
+ * windows share the launch nor on the schedule.  Windows compute in FP64 whatever options.precision
+ * says (a reduced system of at most 288 unknowns is factored in FP64, as ba_solve does); it is
+ * factored in the workgroup's LDS up to 21 cameras and in the window's own device buffer up to 32;
+ * larger windows are solved by the single-problem solver (ba_solver, with the requested precision)
+ * and reported the same way. */
 typedef struct ba_batch ba_batch;
 int ba_batch_create(const ba_problem_view* windows, int32_t count, const ba_options* options, ba_batch** out);
 /* Solves every window from the parameters held on the device (options.max_num_iterations each).
@@ -280,7 +282,22 @@ int ba_batch_download(ba_batch* batch, int32_t window, double* cameras, double* 
 int ba_batch_destroy(ba_batch* batch);
 /* One-shot: create, run, write every window's parameters back in place, destroy. */
 int ba_batch_solve(const ba_problem_view* windows, int32_t count, const ba_options* options, ba_summary* summaries);
-/* Workgroup size used by the batched kernel (256 or 512; 0 = autotune, which create does). */
+/* The launch schedule of a batch's window kernels.  No choice in it changes a window's result
+ * (each 9x9 block of S is reduced by one wave in pair order, each factor entry by one thread), so
+ * ba_batch_create times every schedule on the batch itself and keeps the fastest. */
+typedef struct ba_batch_schedule {
+    int32_t schur_threads;       /* workgroup of the Schur-product kernel: 256 or 512 */
+    int32_t solve_threads;       /* workgroup of the per-window factor and solve: 256 or 512 */
+    int32_t blocks_per_wave;     /* 9x9 blocks of S per wave in one task of the Schur-product kernel: 1, 2, 4 or 8 */
+    int32_t candidates;          /* schedules the autotuner timed (0: set by the caller) */
+    double iteration_ms;         /* one LM iteration of the batch with this schedule (0: not timed) */
+    double default_iteration_ms; /* the same with the default schedule 256 / 256 / 2 (0: not timed) */
+} ba_batch_schedule;
+/* The schedule in use. */
+int ba_batch_get_schedule(ba_batch* batch, ba_batch_schedule* out);
+/* Sets the schedule's three choices (timing fields ignored); NULL runs the autotuner again. */
+int ba_batch_set_schedule(ba_batch* batch, const ba_batch_schedule* schedule);
+/* Workgroup size of both the Schur-product and the factor kernel (256 or 512; 0 = autotune). */
 int ba_batch_set_block(ba_batch* batch, int32_t threads);
 int ba_batch_info(ba_batch* batch, int32_t* threads, int64_t* device_bytes);
 

@@ -357,6 +357,24 @@ class Batch:
     def set_block(self, threads):
         C.check(C.lib().ba_batch_set_block(self._h, threads))
 
+    def schedule(self):
+        """The launch schedule in use (chosen by the autotuner at creation) with its timings."""
+        sc = C.BatchSchedule()
+        C.check(C.lib().ba_batch_get_schedule(self._h, ctypes.byref(sc)))
+        return {f: getattr(sc, f) for f, _ in C.BatchSchedule._fields_}
+
+    def set_schedule(self, schur_threads=None, solve_threads=None, blocks_per_wave=None):
+        """Sets the schedule (None everywhere: run the autotuner again)."""
+        if schur_threads is None and solve_threads is None and blocks_per_wave is None:
+            C.check(C.lib().ba_batch_set_schedule(self._h, None))
+            return
+        cur = self.schedule()
+        sc = C.BatchSchedule()
+        sc.schur_threads = schur_threads if schur_threads is not None else cur["schur_threads"]
+        sc.solve_threads = solve_threads if solve_threads is not None else cur["solve_threads"]
+        sc.blocks_per_wave = blocks_per_wave if blocks_per_wave is not None else cur["blocks_per_wave"]
+        C.check(C.lib().ba_batch_set_schedule(self._h, ctypes.byref(sc)))
+
     def info(self):
         t = C.i32(0)
         b = C.i64(0)

@@ -56,6 +56,11 @@ class Summary(ctypes.Structure):
                 ("trace_capacity", i32), ("termination_reason", i32), ("termination_value", f64)]
 
 
+class BatchSchedule(ctypes.Structure):
+    _fields_ = [("schur_threads", i32), ("solve_threads", i32), ("blocks_per_wave", i32), ("candidates", i32),
+                ("iteration_ms", f64), ("default_iteration_ms", f64)]
+
+
 class KernelTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 48), ("launches", i64), ("total_ms", f64), ("bytes_per_launch", f64),
                 ("flops_per_launch", f64)]
@@ -116,6 +121,8 @@ SIGNATURES = {
     "ba_batch_solve": (ctypes.c_int, [ctypes.POINTER(ProblemView), i32, ctypes.POINTER(Options),
                                        ctypes.POINTER(Summary)]),
     "ba_batch_set_block": (ctypes.c_int, [vp, i32]),
+    "ba_batch_get_schedule": (ctypes.c_int, [vp, ctypes.POINTER(BatchSchedule)]),
+    "ba_batch_set_schedule": (ctypes.c_int, [vp, ctypes.POINTER(BatchSchedule)]),
     "ba_batch_info": (ctypes.c_int, [vp, i32p, ctypes.POINTER(i64)]),
     "ba_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
     "ba_server_submit": (ctypes.c_int, [vp, ctypes.POINTER(ProblemView), i64]),

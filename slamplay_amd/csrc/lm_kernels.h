@@ -920,25 +920,26 @@ __global__ void __launch_bounds__(kWG) k_cam_factor(Dev<Real> d) {
 //   k_cam_update(it) alpha = r.z / p'Sp, x += alpha p, r -= alpha q, z = Minv r, partials.
 // Stop: ||r|| <= tol ||b|| or pcg_max_iterations (or a non-positive p'Sp, i.e. rounding level).
 
+template <int NW = kWaves>
 __device__ __forceinline__ void reduce_upd(const double* __restrict__ part, int n, double& a, double& b, double* lds) {
     double v[2] = {0.0, 0.0};
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int i = threadIdx.x; i < n; i += 64 * NW) {
         v[0] += part[i];
         v[1] += part[n + i];
     }
-    block_sum<2>(v, lds);
+    block_sum<2, double, NW>(v, lds);
     a = lds[0];
-    b = lds[kWaves];
+    b = lds[NW];
 }
 
 // Start of PCG iteration `it` (>= 1) in every workgroup: the convergence test and beta from the
 // previous update's partials (each workgroup reduces them in the same order, so all agree).
 // Returns true when the iteration does not run.
-template <typename Real>
+template <typename Real, int NW = kWaves>
 __device__ __forceinline__ bool pcg_beta(const Dev<Real>& d, int it, double* lds, double& beta) {
     LMState* st = d.st;
     double rz, rr;
-    reduce_upd(d.upd_part, d.n_upd, rz, rr, lds);
+    reduce_upd<NW>(d.upd_part, d.n_upd, rz, rr, lds);
     bool done;
     if (it == 1)
         done = !(rr > 0.0) || d.o.pcg_max_iterations <= 0;
@@ -1199,11 +1200,12 @@ __device__ __forceinline__ void dir_apply_dq(const CameraPrep<Real>& P, const Di
 
 // PCG iteration `it`: beta, p <- z + beta p, then u_j = Jp'(Jc S_c p_c) per observation (slot order).
 // An observation needs only its record and slot (no dependent loads).
-template <typename Real>
-__global__ void __launch_bounds__(kWG) k_jtx_dir(Dev<Real> d, int it) {
+template <typename Real, int NT = kWG>
+__global__ void __launch_bounds__(NT) k_jtx_dir(Dev<Real> d, int it) {
+    constexpr int NW = NT / 64;
     LMState* st = d.st;
     if (st->done || st->step_invalid || st->pcg_done) return;
-    __shared__ double lds[2 * kWaves];
+    __shared__ double lds[2 * NW];
     __shared__ double vsh[9], rsh[9];
     const int c = xcd_remap(blockIdx.x, d.nc);
     if (c >= d.nc) return;
@@ -1223,7 +1225,7 @@ __global__ void __launch_bounds__(kWG) k_jtx_dir(Dev<Real> d, int it) {
         sc = d.sc[i];
     }
     double beta;
-    if (pcg_beta(d, it, lds, beta)) return;
+    if (pcg_beta<Real, NW>(d, it, lds, beta)) return;
     if (threadIdx.x < 9) {
         const double v = pz + beta * pp;
         d.pp[9 * (size_t)c + threadIdx.x] = v;
@@ -1236,12 +1238,12 @@ __global__ void __launch_bounds__(kWG) k_jtx_dir(Dev<Real> d, int it) {
     DirCam<Real> C;
     load_dircam<Real>(P, vsh, rsh, C);
     const Real zero[3] = {(Real)0, (Real)0, (Real)0};
-    for (; j < j_end; j += kWG) {
+    for (; j < j_end; j += NT) {
         const R4<Real> cur = rec;
         const int cur_slot = slot;
-        if (j + kWG < j_end) {
-            rec = d.co_rec[j + kWG];
-            slot = d.slot_of[j + kWG];
+        if (j + NT < j_end) {
+            rec = d.co_rec[j + NT];
+            slot = d.slot_of[j + NT];
         }
         ProjLin<Real> L;
         Real a[2];

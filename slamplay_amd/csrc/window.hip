@@ -277,7 +277,10 @@ struct WindowEngine {
     DevBuf<int> dwork;
     DevBuf<WinState> dstates;
     std::vector<WinState> hstates;
-    int threads = 256;  // block size of k_wschur and k_wsolve
+    // schedule (see WinSchedule): none of these changes a window's result
+    int threads = 256;        // block size of k_wschur
+    int solve_threads = 256;  // block size of k_wsolve
+    int blocks_per_wave = 2;  // 9x9 blocks of S per wave of k_wschur (task granularity)
     int max_slots = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // the current work set: task lists of the phase kernels (windows in dispatch order)
@@ -285,7 +288,8 @@ struct WindowEngine {
     DevBuf<int2> ctask, ptask;
     DevBuf<int4> stask;
     DevBuf<int> wtask;
-    int n_ctask = 0, n_ptask = 0, n_stask = 0, n_wtask = 0, work_nmax = 9, work_threads = 0;
+    int n_ctask = 0, n_ptask = 0, n_stask = 0, n_wtask = 0, work_nmax = 9, work_threads = 0, work_solve = 0,
+        work_bpw = 0;
     hipGraphExec_t graph = nullptr;  // one LM iteration of the current work set
     double wait_timeout_s = 0.0;     // > 0: device waits give up with BA_ERR_TIMEOUT after this long
     hipEvent_t ev_wait = nullptr;
@@ -374,13 +378,15 @@ struct WindowEngine {
 
     // Task lists of the phase kernels for the windows in `slots` (in this dispatch order).
     int set_work(const std::vector<int>& slots) {
-        if (slots == work_slots && work_threads == threads) return BA_OK;
+        if (slots == work_slots && work_threads == threads && work_solve == solve_threads &&
+            work_bpw == blocks_per_wave)
+            return BA_OK;
         if (graph) {
             (void)hipGraphExecDestroy(graph);
             graph = nullptr;
         }
         std::vector<int2> ct, pt;
-        const int nw = threads / 64, group = 2 * nw;  // about two blocks per wave of k_wschur
+        const int nw = threads / 64, group = blocks_per_wave * nw;  // blocks per k_wschur task
         work_nmax = 9;
         // k_wschur: every block group of a window goes to the same XCD (workgroup k runs on XCD
         // k mod 8), so the window's records, read once per pair, are re-read from that XCD's L2;
@@ -415,10 +421,13 @@ struct WindowEngine {
         n_wtask = (int)slots.size();
         work_slots = slots;
         work_threads = threads;
+        work_solve = solve_threads;
+        work_bpw = blocks_per_wave;
         return BA_OK;
     }
 
-    template <int NT>
+    // NT: block size of k_wschur, NF: of k_wsolve
+    template <int NT, int NF>
     int enqueue_iteration_nt() {
         const WinDesc* dd = ddesc.p;
         if (n_ctask) hipLaunchKernelGGL((k_wlin_cam<kWinCamThreads>), dim3(n_ctask), dim3(kWinCamThreads), 0, s, dd,
@@ -431,7 +440,7 @@ struct WindowEngine {
         if (n_stask)
             hipLaunchKernelGGL((k_wschur<double, NT>), dim3(n_stask), dim3(NT), (size_t)schur_lds<double>(NT), s, dd, arena.p,
                                (const int4*)stask.p, n_stask);
-        hipLaunchKernelGGL((k_wsolve<Real, NT>), dim3(n_wtask), dim3(NT), solve_lds(), s,
+        hipLaunchKernelGGL((k_wsolve<Real, NF>), dim3(n_wtask), dim3(NF), solve_lds(), s,
                            dd, arena.p, (const int*)wtask.p, n_wtask, cg_iterations());
         if (n_ctask) hipLaunchKernelGGL((k_wback_cam<kWinCamThreads>), dim3(n_ctask), dim3(kWinCamThreads), 0, s, dd,
                                         arena.p, (const int2*)ctask.p, n_ctask, o);
@@ -444,10 +453,15 @@ struct WindowEngine {
         return BA_OK;
     }
 
-    template <int NT>
     int set_lds_limits() {
-        BA_TRY(allow_dynamic_lds((const void*)k_wschur<double, NT>, (int)schur_lds<double>(NT)));
-        BA_TRY(allow_dynamic_lds((const void*)k_wsolve<Real, NT>, (int)solve_lds()));
+        if (threads == 512)
+            BA_TRY(allow_dynamic_lds((const void*)k_wschur<double, 512>, (int)schur_lds<double>(512)));
+        else
+            BA_TRY(allow_dynamic_lds((const void*)k_wschur<double, 256>, (int)schur_lds<double>(256)));
+        if (solve_threads == 512)
+            BA_TRY(allow_dynamic_lds((const void*)k_wsolve<Real, 512>, (int)solve_lds()));
+        else
+            BA_TRY(allow_dynamic_lds((const void*)k_wsolve<Real, 256>, (int)solve_lds()));
         return BA_OK;
     }
 
@@ -455,8 +469,8 @@ struct WindowEngine {
     size_t solve_lds() const { return (size_t)work_nmax * (work_nmax + 1) / 2 * sizeof(Real); }
 
     int enqueue_iteration() {
-        if (threads == 512) return enqueue_iteration_nt<512>();
-        return enqueue_iteration_nt<256>();
+        if (threads == 512) return solve_threads == 512 ? enqueue_iteration_nt<512, 512>() : enqueue_iteration_nt<512, 256>();
+        return solve_threads == 512 ? enqueue_iteration_nt<256, 512>() : enqueue_iteration_nt<256, 256>();
     }
 
     // Runs `iters` LM iterations of the windows in `slots` (dispatched in the given order); with
@@ -464,7 +478,7 @@ struct WindowEngine {
     int run(const std::vector<int>& slots, int iters, bool use_graph = false) {
         if (slots.empty()) return BA_OK;
         BA_TRY(set_work(slots));
-        BA_TRY(threads == 512 ? set_lds_limits<512>() : set_lds_limits<256>());
+        BA_TRY(set_lds_limits());
         if (use_graph && !graph) {
             hipGraph_t g = nullptr;
             BA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -560,6 +574,8 @@ struct BatchBase {
     virtual int download(int w, double* cams, double* pts) = 0;
     virtual int set_threads(int t) = 0;
     virtual void info(int* threads, int64_t* bytes) = 0;
+    virtual int get_schedule(ba_batch_schedule* out) = 0;
+    virtual int set_schedule(const ba_batch_schedule* sch) = 0;
 };
 
 template <typename Real>
@@ -573,6 +589,7 @@ struct Batch final : BatchBase {
     std::vector<int> large_index;        // window -> index in large_solvers or -1
     std::vector<ba_summary> large_sum;
     ba_options opt{};
+    ba_batch_schedule sched{};  // the schedule in use and the autotuner's timings
 
     int create(const ba_problem_view* views, int count, const ba_options& o) {
         opt = o;
@@ -613,36 +630,74 @@ struct Batch final : BatchBase {
         return autotune();
     }
 
+    // Times every schedule (workgroup sizes of k_wschur and k_wsolve, blocks per wave of k_wschur)
+    // on the batch itself and keeps the fastest; the windows' results do not depend on the choice.
+    // Each candidate is timed twice over two LM iterations from the initial state (best of two).
     int autotune() {
+        sched = ba_batch_schedule{};
+        sched.schur_threads = eng.threads = 256;
+        sched.solve_threads = eng.solve_threads = 256;
+        sched.blocks_per_wave = eng.blocks_per_wave = 2;
         if (order.empty()) return BA_OK;
-        int nmax = 9;
-        for (int w : order) nmax = std::max(nmax, 9 * imgs[w].K);
-        float best = 1e30f;
-        int best_t = 256;
-        for (int t : {256, 512}) {
-            if (schur_lds<double>(t) + 64 > lds_per_block()) continue;
-            eng.threads = t;
-            float ms = 0.f;
-            BA_TRY(eng.time_iteration(order, &ms));
-            if (ms < best) {
-                best = ms;
-                best_t = t;
-            }
+        double best = 1e30;
+        ba_batch_schedule pick = sched;
+        for (int ts : {256, 512}) {
+            if (schur_lds<double>(ts) + 64 > lds_per_block()) continue;
+            for (int tf : {256, 512})
+                for (int bpw : {1, 2, 4}) {
+                    eng.threads = ts;
+                    eng.solve_threads = tf;
+                    eng.blocks_per_wave = bpw;
+                    double ms = 1e30;
+                    for (int rep = 0; rep < 2; ++rep) {
+                        float m = 0.f;
+                        BA_TRY(eng.time_iteration(order, &m));
+                        ms = std::min(ms, (double)m / 2.0);
+                    }
+                    sched.candidates++;
+                    if (ts == 256 && tf == 256 && bpw == 2) sched.default_iteration_ms = ms;
+                    if (ms < best) {
+                        best = ms;
+                        pick.schur_threads = ts;
+                        pick.solve_threads = tf;
+                        pick.blocks_per_wave = bpw;
+                    }
+                }
         }
-        eng.threads = best_t;
+        sched.schur_threads = eng.threads = pick.schur_threads;
+        sched.solve_threads = eng.solve_threads = pick.solve_threads;
+        sched.blocks_per_wave = eng.blocks_per_wave = pick.blocks_per_wave;
+        sched.iteration_ms = best;
+        return BA_OK;
+    }
+
+    int set_schedule(const ba_batch_schedule* sch) override {
+        if (!sch) return autotune();
+        const int ts = sch->schur_threads, tf = sch->solve_threads, bpw = sch->blocks_per_wave;
+        if ((ts != 256 && ts != 512) || (tf != 256 && tf != 512))
+            return fail(BA_ERR_INVALID_ARGUMENT, "workgroup sizes must be 256 or 512 (got %d, %d)", ts, tf);
+        if (bpw != 1 && bpw != 2 && bpw != 4 && bpw != 8)
+            return fail(BA_ERR_INVALID_ARGUMENT, "blocks_per_wave must be 1, 2, 4 or 8 (got %d)", bpw);
+        if (schur_lds<double>(ts) + 64 > lds_per_block())
+            return fail(BA_ERR_CAPACITY, "the Schur staging of %d threads does not fit the LDS", ts);
+        sched = ba_batch_schedule{};
+        sched.schur_threads = eng.threads = ts;
+        sched.solve_threads = eng.solve_threads = tf;
+        sched.blocks_per_wave = eng.blocks_per_wave = bpw;
+        return BA_OK;
+    }
+
+    int get_schedule(ba_batch_schedule* out) override {
+        *out = sched;
         return BA_OK;
     }
 
     int set_threads(int t) override {
         if (t == 0) return autotune();
-        if (t != 256 && t != 512) return fail(BA_ERR_INVALID_ARGUMENT, "threads must be 256 or 512");
-        int nmax = 9;
-        for (int w : order) nmax = std::max(nmax, 9 * imgs[w].K);
-        if (schur_lds<double>(t) + 64 > lds_per_block())
-            return fail(BA_ERR_CAPACITY, "the Schur staging of %d threads does not fit the LDS", t);
-        (void)nmax;
-        eng.threads = t;
-        return BA_OK;
+        ba_batch_schedule sc{};
+        sc.schur_threads = sc.solve_threads = t;
+        sc.blocks_per_wave = eng.blocks_per_wave;
+        return set_schedule(&sc);
     }
 
     void info(int* threads, int64_t* bytes) override {
@@ -694,16 +749,15 @@ struct Batch final : BatchBase {
     }
 };
 
+// Windows run in FP64 whatever options.precision asks: a window's reduced system has at most 288
+// unknowns, where ba_solve factors in FP64 as well (kDenseF64MaxN), and the FP32 factor would need
+// CG against the FP64 system to reach the FP64 step (measured on the 512-window batch: 0.73 ms per
+// LM iteration for the FP32 factor + CG against 0.32 ms for the FP64 factor).  Windows above 32
+// cameras go to the single-problem solver with the caller's precision.
 int make_batch(const ba_problem_view* views, int count, const ba_options& o, std::unique_ptr<BatchBase>* out) {
-    if (o.precision == BA_PRECISION_FP64) {
-        auto b = std::make_unique<Batch<double>>();
-        BA_TRY(b->create(views, count, o));
-        *out = std::move(b);
-    } else {
-        auto b = std::make_unique<Batch<float>>();
-        BA_TRY(b->create(views, count, o));
-        *out = std::move(b);
-    }
+    auto b = std::make_unique<Batch<double>>();
+    BA_TRY(b->create(views, count, o));
+    *out = std::move(b);
     return BA_OK;
 }
 
@@ -1144,16 +1198,11 @@ struct Server final : ServerBase {
     }
 };
 
+// Resident windows run in FP64 as in make_batch; the large problems keep the caller's precision.
 int make_server(const ba_server_config& c, std::unique_ptr<ServerBase>* out) {
-    if (c.options.precision == BA_PRECISION_FP64) {
-        auto s = std::make_unique<Server<double>>();
-        BA_TRY(s->create(c));
-        *out = std::move(s);
-    } else {
-        auto s = std::make_unique<Server<float>>();
-        BA_TRY(s->create(c));
-        *out = std::move(s);
-    }
+    auto s = std::make_unique<Server<double>>();
+    BA_TRY(s->create(c));
+    *out = std::move(s);
     return BA_OK;
 }
 
@@ -1214,6 +1263,18 @@ int ba_batch_set_block(ba_batch* b, int32_t threads) {
     clear_error();
     if (!b) return fail(BA_ERR_INVALID_ARGUMENT, "NULL batch");
     return b->impl->set_threads(threads);
+}
+
+int ba_batch_get_schedule(ba_batch* b, ba_batch_schedule* out) {
+    clear_error();
+    if (!b || !out) return fail(BA_ERR_INVALID_ARGUMENT, "NULL argument");
+    return b->impl->get_schedule(out);
+}
+
+int ba_batch_set_schedule(ba_batch* b, const ba_batch_schedule* schedule) {
+    clear_error();
+    if (!b) return fail(BA_ERR_INVALID_ARGUMENT, "batch is NULL");
+    return b->impl->set_schedule(schedule);
 }
 
 int ba_batch_info(ba_batch* b, int32_t* threads, int64_t* device_bytes) {

@@ -94,20 +94,30 @@ def test_alone_and_in_mix_bitwise_identical(precision):
     assert _same(s_alone, s_mix)
     np.testing.assert_array_equal(c_alone, c_mix)
     np.testing.assert_array_equal(p_alone, p_mix)
-    # every workgroup size that fits gives the same bits too (the reduction order is per wave and
-    # per block)
+    # every schedule the autotuner may pick gives the same bits too (each block of S is reduced by
+    # one wave in pair order, each factor entry by one thread)
+    tuned = mix.schedule()
+    assert tuned["candidates"] >= 6 and tuned["iteration_ms"] > 0 and tuned["default_iteration_ms"] > 0
+    assert tuned["iteration_ms"] <= tuned["default_iteration_ms"]
     sizes = 0
-    for t in (256, 512):
-        try:
-            mix.set_block(t)
-        except S.BAError as e:
-            assert e.status == 6  # BA_ERR_CAPACITY
-            continue
-        sizes += 1
-        mix.reset()
-        assert _same(mix.run()[3], s_alone)
-        np.testing.assert_array_equal(mix.download(3)[1], p_alone)
-    assert sizes >= 1
+    for ts in (256, 512):
+        for tf in (256, 512):
+            for bpw in (1, 2, 4, 8):
+                try:
+                    mix.set_schedule(ts, tf, bpw)
+                except S.BAError as e:
+                    assert e.status == 6  # BA_ERR_CAPACITY
+                    continue
+                assert mix.schedule()["blocks_per_wave"] == bpw
+                sizes += 1
+                mix.reset()
+                assert _same(mix.run()[3], s_alone)
+                np.testing.assert_array_equal(mix.download(3)[1], p_alone)
+    assert sizes >= 8
+    # windows compute in FP64 whatever the requested precision
+    other = S.Batch([target], **dict(kw, precision=1 - precision))
+    assert _same(other.run()[0], s_alone)
+    np.testing.assert_array_equal(other.download(0)[1], p_alone)
 
 
 def test_reset_rerun_and_empty_window():
@@ -202,6 +212,12 @@ def test_invalid_block_size():
     b = S.Batch([window(501, 6, 200)], precision=1)
     with pytest.raises(S.BAError):
         b.set_block(1024)
+    with pytest.raises(S.BAError, match="blocks_per_wave"):
+        b.set_schedule(256, 256, 3)
+    with pytest.raises(S.BAError, match="256 or 512"):
+        b.set_schedule(128, 256, 2)
+    b.set_schedule()  # autotune again
+    assert b.schedule()["candidates"] >= 6
 
 
 @pytest.mark.parametrize("arena", [1 << 30, 12 << 20])

@@ -38,9 +38,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 LM_ITERS = 30
 # HBM bytes per launch of the PCG kernels at this workload (mixed precision), measured with
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by a 1 GiB copy's counts
-# (profiles/c3_mixed_pcg_kernels.md); empty until measured for the current kernels
-MEASURED_TRAFFIC_BYTES = {}
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch) and scaled by what the same counters read
+# for a 1 GiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000): profiles/c3_mixed_pcg_kernels.md
+MEASURED_TRAFFIC_BYTES = {
+    "jz": (2.0 * 13949.215 + 117.188) * 1024,      # k_jz_dir, 128 threads
+    "jtx": (2.0 * 12071.969 + 18725.885) * 1024,   # k_jtx_dir
+    "pt_z": (2.0 * 17292.553 + 3161.850) * 1024,   # k_pt_z (SELL padding slots read)
+}
 
 
 _T0 = time.perf_counter()

@@ -289,13 +289,16 @@ typedef struct ba_batch_schedule {
     int32_t schur_threads;       /* workgroup of the Schur-product kernel: 256 or 512 */
     int32_t solve_threads;       /* workgroup of the per-window factor and solve: 256 or 512 */
     int32_t blocks_per_wave;     /* 9x9 blocks of S per wave in one task of the Schur-product kernel: 1, 2, 4 or 8 */
+    int32_t lds_cameras;         /* windows of at most this many cameras factor in LDS, larger ones in their device
+                                    buffer (capped at what the LDS holds, 21; the launch reserves the largest such
+                                    window's triangle for every workgroup) */
     int32_t candidates;          /* schedules the autotuner timed (0: set by the caller) */
     double iteration_ms;         /* one LM iteration of the batch with this schedule (0: not timed) */
-    double default_iteration_ms; /* the same with the default schedule 256 / 256 / 2 (0: not timed) */
+    double default_iteration_ms; /* the same with the default schedule 256 / 256 / 2 / 21 (0: not timed) */
 } ba_batch_schedule;
 /* The schedule in use. */
 int ba_batch_get_schedule(ba_batch* batch, ba_batch_schedule* out);
-/* Sets the schedule's three choices (timing fields ignored); NULL runs the autotuner again. */
+/* Sets the schedule's four choices (timing fields ignored); NULL runs the autotuner again. */
 int ba_batch_set_schedule(ba_batch* batch, const ba_batch_schedule* schedule);
 /* Workgroup size of both the Schur-product and the factor kernel (256 or 512; 0 = autotune). */
 int ba_batch_set_block(ba_batch* batch, int32_t threads);

@@ -363,16 +363,17 @@ class Batch:
         C.check(C.lib().ba_batch_get_schedule(self._h, ctypes.byref(sc)))
         return {f: getattr(sc, f) for f, _ in C.BatchSchedule._fields_}
 
-    def set_schedule(self, schur_threads=None, solve_threads=None, blocks_per_wave=None):
+    def set_schedule(self, schur_threads=None, solve_threads=None, blocks_per_wave=None, lds_cameras=None):
         """Sets the schedule (None everywhere: run the autotuner again)."""
-        if schur_threads is None and solve_threads is None and blocks_per_wave is None:
+        args = dict(schur_threads=schur_threads, solve_threads=solve_threads, blocks_per_wave=blocks_per_wave,
+                    lds_cameras=lds_cameras)
+        if all(v is None for v in args.values()):
             C.check(C.lib().ba_batch_set_schedule(self._h, None))
             return
         cur = self.schedule()
         sc = C.BatchSchedule()
-        sc.schur_threads = schur_threads if schur_threads is not None else cur["schur_threads"]
-        sc.solve_threads = solve_threads if solve_threads is not None else cur["solve_threads"]
-        sc.blocks_per_wave = blocks_per_wave if blocks_per_wave is not None else cur["blocks_per_wave"]
+        for k, v in args.items():
+            setattr(sc, k, v if v is not None else cur[k])
         C.check(C.lib().ba_batch_set_schedule(self._h, ctypes.byref(sc)))
 
     def info(self):

@@ -57,8 +57,8 @@ class Summary(ctypes.Structure):
 
 
 class BatchSchedule(ctypes.Structure):
-    _fields_ = [("schur_threads", i32), ("solve_threads", i32), ("blocks_per_wave", i32), ("candidates", i32),
-                ("iteration_ms", f64), ("default_iteration_ms", f64)]
+    _fields_ = [("schur_threads", i32), ("solve_threads", i32), ("blocks_per_wave", i32), ("lds_cameras", i32),
+                ("candidates", i32), ("iteration_ms", f64), ("default_iteration_ms", f64)]
 
 
 class KernelTime(ctypes.Structure):

@@ -222,36 +222,21 @@ __global__ void __launch_bounds__(64) k_potrf_tile(F* __restrict__ A, int ld, in
     if (!potrf_tile_wave<F>(T, dl, A, ld, k, Dinv) && l == 0) st->step_invalid = 1;
 }
 
-// Panel: L_ik = A_ik L_kk^-T = A_ik Dinv_k' for i > k (one workgroup of four waves per tile, one
-// 16 x 16 block per wave).  L_ik goes to tile (i, k) and its transpose to tile (k, i) of the upper
-// triangle, which the factorization does not use, so that both triangular sweeps of the solve read
-// rows.
+// Step k of the factorization in one launch, one workgroup per lower tile (i, j) of the trailing
+// matrix, k < j <= i, one 16 x 16 block per wave.  The panel tiles the update needs are formed in
+// the workgroup from the unscaled A_ik, A_jk and the kept inverse X_k = L_kk^-1 (L_ik = A_ik X_k',
+// the same matrix-core products in the same order as a separate panel launch would do), staged in
+// LDS, and A_ij -= L_ik L_jk' follows; so a step is one launch.  The workgroups of column k + 1
+// store L_ik' in the upper tile (k, i), which the factorization does not read; the lower tile
+// (i, k) still holds A_ik, which other workgroups of this launch read, and k_panel_down copies the
+// transposes into it once the factorization is done.  Block 0 (tile (k + 1, k + 1), dispatched
+// first) keeps its result in LDS and factors it, so the next step's diagonal factor costs no launch
+// of its own.
 template <typename F>
-__global__ void __launch_bounds__(256) k_trsm_tiles(F* __restrict__ A, int ld, int k, const F* __restrict__ Dinv,
-                                                    const LMState* st) {
+__global__ void __launch_bounds__(256) k_step_tiles(F* __restrict__ A, int ld, int k, F* __restrict__ Dinv, LMState* st) {
     if (st->done || st->step_invalid) return;
-    const int i = k + 1 + blockIdx.x, w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int si = w >> 1, sj = w & 1;
-    F* Aik = A + (size_t)(kTile * i) * ld + kTile * k;
-    F* Aki = A + (size_t)(kTile * k) * ld + kTile * i;
-    typename Mfma<F>::acc c = {0, 0, 0, 0};
-    c = tile_mma_nt<F>(Aik, ld, Dinv + (size_t)k * kTile * kTile, kTile, si, sj, c, (F)1);
-    __syncthreads();  // every wave has read A_ik before any block of it is overwritten
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int rr = 16 * si + Mfma<F>::row(l, e), cc = 16 * sj + (l & 15);
-        Aik[(size_t)rr * ld + cc] = c[e];
-        Aki[(size_t)cc * ld + rr] = c[e];
-    }
-}
-
-// Trailing update A_ij -= L_ik L_jk' for k < j <= i (one workgroup per lower tile, one 16 x 16
-// block per wave, the block above the diagonal of diagonal tiles skipped).  The workgroup of tile
-// (k + 1, k + 1) -- block 0, dispatched first -- keeps its result in LDS and factors it, so the
-// next step's diagonal factor costs no launch of its own.
-template <typename F>
-__global__ void __launch_bounds__(256) k_syrk_tiles(F* __restrict__ A, int ld, int k, F* __restrict__ Dinv, LMState* st) {
-    if (st->done || st->step_invalid) return;
+    constexpr int PP = kTile + 16 / (int)sizeof(F);  // LDS row pitch: rows start 16-byte aligned
+    __shared__ __align__(16) F P[2][kTile * PP];     // L_ik, L_jk
     __shared__ F T[kTile][kTile + 1];
     __shared__ F dl[kTile];
     const int b = blockIdx.x, w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -262,13 +247,31 @@ __global__ void __launch_bounds__(256) k_syrk_tiles(F* __restrict__ A, int ld, i
     const int i = k + 1 + ii, j = k + 1 + jj;
     const int si = w >> 1, sj = w & 1;
     const bool next_diag = b == 0;
+    const F* Xk = Dinv + (size_t)k * kTile * kTile;
+    // panel tiles of rows i and j (one of them when i == j)
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        if (t == 1 && i == j) break;
+        const F* Ark = A + (size_t)(kTile * (t == 0 ? i : j)) * ld + kTile * k;
+        typename Mfma<F>::acc c = {0, 0, 0, 0};
+        c = tile_mma_nt<F>(Ark, ld, Xk, kTile, si, sj, c, (F)1);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) P[t][(16 * si + Mfma<F>::row(l, e)) * PP + 16 * sj + (l & 15)] = c[e];
+    }
+    __syncthreads();
+    if (j == k + 1) {
+        F* Aki = A + (size_t)(kTile * k) * ld + kTile * i;
+        for (int idx = threadIdx.x; idx < kTile * kTile; idx += 256) {
+            const int cc = idx / kTile, rr = idx % kTile;
+            Aki[(size_t)cc * ld + rr] = P[0][rr * PP + cc];
+        }
+    }
     F* Aij = A + (size_t)(kTile * i) * ld + kTile * j;
     if (!(i == j && si < sj)) {
         typename Mfma<F>::acc c;
 #pragma unroll
         for (int e = 0; e < 4; ++e) c[e] = Aij[(size_t)(16 * si + Mfma<F>::row(l, e)) * ld + 16 * sj + (l & 15)];
-        c = tile_mma_nt<F>(A + (size_t)(kTile * i) * ld + kTile * k, ld, A + (size_t)(kTile * j) * ld + kTile * k, ld, si, sj,
-                           c, (F)-1);
+        c = tile_mma_nt<F>(P[0], PP, i == j ? P[0] : P[1], PP, si, sj, c, (F)-1);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int rr = 16 * si + Mfma<F>::row(l, e), cc = 16 * sj + (l & 15);
@@ -281,6 +284,24 @@ __global__ void __launch_bounds__(256) k_syrk_tiles(F* __restrict__ A, int ld, i
     if (!next_diag) return;
     __syncthreads();
     if (w == 0 && !potrf_tile_wave<F>(T, dl, A, ld, k + 1, Dinv) && l == 0) st->step_invalid = 1;
+}
+
+// After the factorization: L_ik = (tile (k, i))' into the lower tile (i, k), i > k (one workgroup per
+// tile; the triangular sweeps and the block inverses read the lower tiles).
+template <typename F>
+__global__ void __launch_bounds__(256) k_panel_down(F* __restrict__ A, int ld, const LMState* st) {
+    if (st->done || st->step_invalid) return;
+    __shared__ F T[kTile][kTile + 1];
+    const int b = blockIdx.x;
+    int ii = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+    while ((ii + 1) * (ii + 2) / 2 <= b) ++ii;
+    while (ii * (ii + 1) / 2 > b) --ii;
+    const int k = b - ii * (ii + 1) / 2, i = ii + 1;  // k < i
+    const F* Aki = A + (size_t)(kTile * k) * ld + kTile * i;
+    F* Aik = A + (size_t)(kTile * i) * ld + kTile * k;
+    for (int idx = threadIdx.x; idx < kTile * kTile; idx += 256) T[idx / kTile][idx % kTile] = Aki[(size_t)(idx / kTile) * ld + idx % kTile];
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < kTile * kTile; idx += 256) Aik[(size_t)(idx / kTile) * ld + idx % kTile] = T[idx % kTile][idx / kTile];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -590,9 +611,9 @@ struct DenseSchur {
         sv.launch("potrf", k_potrf_tile<F>, 1, 64, A, ld, 0, Dinv, d.st);
         for (int k = 0; k + 1 < nt; ++k) {
             const int m = nt - k - 1;
-            sv.launch("trsm", k_trsm_tiles<F>, m, 256, A, ld, k, (const F*)Dinv, (const LMState*)d.st);
-            sv.launch("syrk", k_syrk_tiles<F>, m * (m + 1) / 2, 256, A, ld, k, Dinv, d.st);
+            sv.launch("step", k_step_tiles<F>, m * (m + 1) / 2, 256, A, ld, k, Dinv, d.st);
         }
+        if (nt > 1) sv.launch("panel_down", k_panel_down<F>, nt * (nt - 1) / 2, 256, A, ld, (const LMState*)d.st);
         sv.launch("block_inverse", k_block_inverse<F>, nb * kSolveBlk, 256, (const F*)A, ld, (const F*)Dinv, Binv, BinvT,
                   nt, (const LMState*)d.st);
         solve<F>(sv, A, Binv, BinvT, (const double*)d.bc, d.px, 0);

@@ -39,11 +39,11 @@ struct WinDesc {
 };
 
 // Per-observation record (the FP64 residual and Jacobians are recomputed where needed):
-//   js (working precision, 36 values): scaled J~_c column by column (18: the two rows of column c
-//       at 2c, 2c + 1), J~_p (6, row-major), H = J~_p V*^-1 (6, row-major),
-//       I - G = I - J~_p V*^-1 J~_p' (3: g00, g01, g11), unused (3) -- the input of the reduced
-//       camera system, rewritten whenever the damping changes.
-constexpr int kWinJs = 36;
+//   js (FP64, 32 values): scaled J~_c column by column (18: the two rows of column c at 2c,
+//       2c + 1), H = J~_p V*^-1 (6, row-major), J~_p (6, row-major), unused (2) -- the input of
+//       the reduced camera system, rewritten whenever the damping changes; a pair product reads
+//       the first 24 values of one record and values 0-17, 24-29 of the other.
+constexpr int kWinJs = 32;
 // Partial sums per camera (cpart) and per point chunk (ppart).
 enum WinCamPart { WC_COST = 0, WC_GMAX, WC_MODEL, WC_CAND, WC_DX2, WC_X2, WC_COUNT = 8 };
 enum WinPtPart { WP_GMAX = 0, WP_DX2, WP_X2, WP_COUNT = 4 };

@@ -65,19 +65,15 @@ Opts opts_of(const ba_options& o) {
     return r;
 }
 
-// Static LDS of k_wsolve (xv, rv, zv, dv, qv, colv, dinv, sdot, bad) plus slack for alignment.
-template <typename Real>
-int64_t solve_static_lds() {
-    return 5 * kWinMaxN * 8 + 2 * kWinMaxN * sizeof(Real) + 16 + 256;
-}
+// Static LDS of k_wsolve (xv, colv, dinv, bad) plus slack for alignment.
+int64_t solve_static_lds() { return 3 * kWinMaxN * 8 + 16 + 256; }
 
 // Largest reduced system (9 K) k_wsolve can factor in LDS (0: none).
-template <typename Real>
 int max_window_n() {
-    const int64_t room = lds_per_block() - solve_static_lds<Real>();
+    const int64_t room = lds_per_block() - solve_static_lds();
     if (room <= 0) return 0;
-    int n = (int)std::floor(std::sqrt(2.0 * (double)room / sizeof(Real)));  // packed: n (n + 1) / 2
-    while (n > 0 && (int64_t)n * (n + 1) / 2 * (int64_t)sizeof(Real) > room) --n;
+    int n = (int)std::floor(std::sqrt(2.0 * (double)room / 8.0));  // packed: n (n + 1) / 2 doubles
+    while (n > 0 && (int64_t)n * (n + 1) / 2 * 8 > room) --n;
     n = std::min(n, kWinMaxN);
     return n - n % 9;
 }
@@ -171,7 +167,7 @@ int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowIm
     D.o_pv = take(96 * (int64_t)slots);
     D.o_js = take(kWinJs * sizeof(double) * (int64_t)No);
     D.o_S = take(sizeof(double) * (int64_t)n * n);  // zero: blocks without common points are never written
-    D.o_L = n > max_window_n<Real>() ? take(sizeof(Real) * (int64_t)n * (n + 1) / 2) : 0;
+    D.o_L = take(sizeof(double) * (int64_t)n * (n + 1) / 2);  // the factor when it is not kept in LDS
     img->total = off;
     img->bytes.assign(input_bytes, 0);
     char* b = img->bytes.data();
@@ -228,7 +224,6 @@ WinDesc rebase(const WinDesc& d, int64_t base) {
     int64_t* o = &r.o_cam_off;
     const int count = (int)((&r.o_state - &r.o_cam_off) + 1);
     for (int i = 0; i < count; ++i) o[i] += base;
-    if (d.o_L == 0) r.o_L = 0;  // factor in LDS
     return r;
 }
 
@@ -281,6 +276,7 @@ struct WindowEngine {
     int threads = 256;        // block size of k_wschur
     int solve_threads = 256;  // block size of k_wsolve
     int blocks_per_wave = 2;  // 9x9 blocks of S per wave of k_wschur (task granularity)
+    int lds_cameras = kWinMaxK;  // windows of at most this many cameras factor in LDS
     int max_slots = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // the current work set: task lists of the phase kernels (windows in dispatch order)
@@ -289,7 +285,7 @@ struct WindowEngine {
     DevBuf<int4> stask;
     DevBuf<int> wtask;
     int n_ctask = 0, n_ptask = 0, n_stask = 0, n_wtask = 0, work_nmax = 9, work_threads = 0, work_solve = 0,
-        work_bpw = 0;
+        work_bpw = 0, work_lds = 0;
     hipGraphExec_t graph = nullptr;  // one LM iteration of the current work set
     double wait_timeout_s = 0.0;     // > 0: device waits give up with BA_ERR_TIMEOUT after this long
     hipEvent_t ev_wait = nullptr;
@@ -364,9 +360,9 @@ struct WindowEngine {
         return BA_OK;
     }
 
-    // Mixed precision: the preconditioned CG of a window step runs up to 10 iterations per
-    // refinement step of the options (20 by default); FP64 windows solve directly.
-    int cg_iterations() const { return sizeof(Real) == sizeof(double) ? 0 : 10 * std::max(opt.refine_steps, 1); }
+    // Reduced systems of at most lds_n() unknowns are factored in LDS (the launch reserves the
+    // largest such window's triangle for every workgroup), larger ones in their global buffer.
+    int lds_n() const { return std::min(9 * lds_cameras, max_window_n()); }
 
     template <typename T>
     int upload(DevBuf<T>& buf, const std::vector<T>& v) {
@@ -379,7 +375,7 @@ struct WindowEngine {
     // Task lists of the phase kernels for the windows in `slots` (in this dispatch order).
     int set_work(const std::vector<int>& slots) {
         if (slots == work_slots && work_threads == threads && work_solve == solve_threads &&
-            work_bpw == blocks_per_wave)
+            work_bpw == blocks_per_wave && work_lds == lds_n())
             return BA_OK;
         if (graph) {
             (void)hipGraphExecDestroy(graph);
@@ -395,7 +391,7 @@ struct WindowEngine {
         std::vector<double> xload(kXcds, 0.0);
         for (int sl : slots) {
             const WinDesc& D = hdesc[sl];
-            if (D.o_L == 0) work_nmax = std::max(work_nmax, 9 * D.K);
+            if (9 * D.K <= lds_n()) work_nmax = std::max(work_nmax, 9 * D.K);
             for (int c = 0; c < D.K; ++c) ct.push_back(make_int2(sl, c));
             for (int ch = 0; ch < D.n_chunks; ++ch) pt.push_back(make_int2(sl, ch));
             const int x = (int)(std::min_element(xload.begin(), xload.end()) - xload.begin());
@@ -423,6 +419,7 @@ struct WindowEngine {
         work_threads = threads;
         work_solve = solve_threads;
         work_bpw = blocks_per_wave;
+        work_lds = lds_n();
         return BA_OK;
     }
 
@@ -440,8 +437,8 @@ struct WindowEngine {
         if (n_stask)
             hipLaunchKernelGGL((k_wschur<double, NT>), dim3(n_stask), dim3(NT), (size_t)schur_lds<double>(NT), s, dd, arena.p,
                                (const int4*)stask.p, n_stask);
-        hipLaunchKernelGGL((k_wsolve<Real, NF>), dim3(n_wtask), dim3(NF), solve_lds(), s,
-                           dd, arena.p, (const int*)wtask.p, n_wtask, cg_iterations());
+        hipLaunchKernelGGL((k_wsolve<NF>), dim3(n_wtask), dim3(NF), solve_lds(), s, dd, arena.p, (const int*)wtask.p,
+                           n_wtask, lds_n());
         if (n_ctask) hipLaunchKernelGGL((k_wback_cam<kWinCamThreads>), dim3(n_ctask), dim3(kWinCamThreads), 0, s, dd,
                                         arena.p, (const int2*)ctask.p, n_ctask, o);
         if (n_ptask) hipLaunchKernelGGL(k_wback_pt, dim3(n_ptask), dim3(64), 0, s, dd, arena.p, (const int2*)ptask.p,
@@ -459,14 +456,14 @@ struct WindowEngine {
         else
             BA_TRY(allow_dynamic_lds((const void*)k_wschur<double, 256>, (int)schur_lds<double>(256)));
         if (solve_threads == 512)
-            BA_TRY(allow_dynamic_lds((const void*)k_wsolve<Real, 512>, (int)solve_lds()));
+            BA_TRY(allow_dynamic_lds((const void*)k_wsolve<512>, (int)solve_lds()));
         else
-            BA_TRY(allow_dynamic_lds((const void*)k_wsolve<Real, 256>, (int)solve_lds()));
+            BA_TRY(allow_dynamic_lds((const void*)k_wsolve<256>, (int)solve_lds()));
         return BA_OK;
     }
 
     // dynamic LDS of k_wsolve: the packed factor of the largest window factored in LDS
-    size_t solve_lds() const { return (size_t)work_nmax * (work_nmax + 1) / 2 * sizeof(Real); }
+    size_t solve_lds() const { return (size_t)work_nmax * (work_nmax + 1) / 2 * sizeof(double); }
 
     int enqueue_iteration() {
         if (threads == 512) return solve_threads == 512 ? enqueue_iteration_nt<512, 512>() : enqueue_iteration_nt<512, 256>();
@@ -630,45 +627,67 @@ struct Batch final : BatchBase {
         return autotune();
     }
 
-    // Times every schedule (workgroup sizes of k_wschur and k_wsolve, blocks per wave of k_wschur)
-    // on the batch itself and keeps the fastest; the windows' results do not depend on the choice.
-    // Each candidate is timed twice over two LM iterations from the initial state (best of two).
+    // Times schedules on the batch itself and keeps the fastest; the windows' results do not depend on
+    // the choice.  Two stages (the two kernels they tune run one after the other): the factor's
+    // workgroup size and LDS limit with the default Schur-product launch, then the Schur-product
+    // workgroup size and blocks per wave with the chosen factor launch.  Each candidate is timed
+    // twice over two LM iterations from the initial state (best of two).
     int autotune() {
         sched = ba_batch_schedule{};
-        sched.schur_threads = eng.threads = 256;
-        sched.solve_threads = eng.solve_threads = 256;
-        sched.blocks_per_wave = eng.blocks_per_wave = 2;
+        apply(256, 256, 2, kWinMaxK);
         if (order.empty()) return BA_OK;
         double best = 1e30;
-        ba_batch_schedule pick = sched;
+        float warm = 0.f;
+        BA_TRY(eng.time_iteration(order, &warm));  // first launches of the kernels, not timed
+        auto time_it = [&](double* ms) -> int {
+            *ms = 1e30;
+            for (int rep = 0; rep < 2; ++rep) {
+                float m = 0.f;
+                BA_TRY(eng.time_iteration(order, &m));
+                *ms = std::min(*ms, (double)m / 2.0);
+            }
+            sched.candidates++;
+            return BA_OK;
+        };
+        int best_tf = 256, best_lds = kWinMaxK;
+        for (int tf : {256, 512})
+            for (int lc : {kWinMaxK, 14, 10}) {
+                apply(256, tf, 2, lc);
+                double ms;
+                BA_TRY(time_it(&ms));
+                if (tf == 256 && lc == kWinMaxK) sched.default_iteration_ms = ms;
+                if (ms < best) {
+                    best = ms;
+                    best_tf = tf;
+                    best_lds = lc;
+                }
+            }
+        int best_ts = 256, best_bpw = 2;
         for (int ts : {256, 512}) {
             if (schur_lds<double>(ts) + 64 > lds_per_block()) continue;
-            for (int tf : {256, 512})
-                for (int bpw : {1, 2, 4}) {
-                    eng.threads = ts;
-                    eng.solve_threads = tf;
-                    eng.blocks_per_wave = bpw;
-                    double ms = 1e30;
-                    for (int rep = 0; rep < 2; ++rep) {
-                        float m = 0.f;
-                        BA_TRY(eng.time_iteration(order, &m));
-                        ms = std::min(ms, (double)m / 2.0);
-                    }
-                    sched.candidates++;
-                    if (ts == 256 && tf == 256 && bpw == 2) sched.default_iteration_ms = ms;
-                    if (ms < best) {
-                        best = ms;
-                        pick.schur_threads = ts;
-                        pick.solve_threads = tf;
-                        pick.blocks_per_wave = bpw;
-                    }
+            for (int bpw : {1, 2, 4}) {
+                if (ts == 256 && bpw == 2) continue;  // timed in the first stage
+                apply(ts, best_tf, bpw, best_lds);
+                double ms;
+                BA_TRY(time_it(&ms));
+                if (ms < best) {
+                    best = ms;
+                    best_ts = ts;
+                    best_bpw = bpw;
                 }
+            }
         }
-        sched.schur_threads = eng.threads = pick.schur_threads;
-        sched.solve_threads = eng.solve_threads = pick.solve_threads;
-        sched.blocks_per_wave = eng.blocks_per_wave = pick.blocks_per_wave;
+        apply(best_ts, best_tf, best_bpw, best_lds);
         sched.iteration_ms = best;
         return BA_OK;
+    }
+
+    void apply(int ts, int tf, int bpw, int lds_cameras) {
+        sched.schur_threads = eng.threads = ts;
+        sched.solve_threads = eng.solve_threads = tf;
+        sched.blocks_per_wave = eng.blocks_per_wave = bpw;
+        eng.lds_cameras = lds_cameras;
+        sched.lds_cameras = eng.lds_n() / 9;
     }
 
     int set_schedule(const ba_batch_schedule* sch) override {
@@ -678,12 +697,11 @@ struct Batch final : BatchBase {
             return fail(BA_ERR_INVALID_ARGUMENT, "workgroup sizes must be 256 or 512 (got %d, %d)", ts, tf);
         if (bpw != 1 && bpw != 2 && bpw != 4 && bpw != 8)
             return fail(BA_ERR_INVALID_ARGUMENT, "blocks_per_wave must be 1, 2, 4 or 8 (got %d)", bpw);
+        if (sch->lds_cameras < 0) return fail(BA_ERR_INVALID_ARGUMENT, "lds_cameras < 0");
         if (schur_lds<double>(ts) + 64 > lds_per_block())
             return fail(BA_ERR_CAPACITY, "the Schur staging of %d threads does not fit the LDS", ts);
         sched = ba_batch_schedule{};
-        sched.schur_threads = eng.threads = ts;
-        sched.solve_threads = eng.solve_threads = tf;
-        sched.blocks_per_wave = eng.blocks_per_wave = bpw;
+        apply(ts, tf, bpw, sch->lds_cameras);
         return BA_OK;
     }
 
@@ -694,9 +712,8 @@ struct Batch final : BatchBase {
 
     int set_threads(int t) override {
         if (t == 0) return autotune();
-        ba_batch_schedule sc{};
+        ba_batch_schedule sc = sched;
         sc.schur_threads = sc.solve_threads = t;
-        sc.blocks_per_wave = eng.blocks_per_wave;
         return set_schedule(&sc);
     }
 

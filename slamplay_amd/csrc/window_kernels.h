@@ -11,12 +11,8 @@
 //                 H = J~_p V*^-1, I - G) and b_c = sum J~_c'(r~ - J~_p q) in FP64;
 //   k_wschur      (window, block group) 9x9 blocks of S, each by one wave from LDS-staged
 //                 observation pairs;
-//   k_wsolve      (window) S into LDS (packed lower triangle), right-looking Cholesky, blocked
-//                 triangular solves; mixed
-//                 precision: preconditioned CG in FP64 on the FP64 operator with the FP32 factor
-//                 as preconditioner (S y = D_c^2 y + sum J~_c'(J~_c y - J~_p z),
-//                 z = V*^-1 sum J~_p'J~_c y), until the relative residual is 1e-8, so the step is
-//                 the FP64 step also where cond(S) is beyond what an FP32 factor resolves;
+//   k_wsolve      (window) S copied as a packed lower triangle into LDS (or the window's global
+//                 buffer), right-looking FP64 Cholesky, blocked triangular solves;
 //   k_wback_cam   (window, camera) u = J~_p'(r~ - J~_c y_c) into slot order (FP64);
 //   k_wback_pt    (window, point chunk) y_p = V*^-1 sum u, candidate points, |dx|^2, |x|^2;
 //   k_wcand_cam   (window, camera) candidate camera, model cost change, FP64 candidate cost;
@@ -179,25 +175,31 @@ struct alignas(2 * sizeof(Real)) R2v {
     Real x, y;
 };
 
+// Row pitch of a staged record in LDS: 34 values, not the record's 32.  The 32 lanes of a half-wave
+// read the same 16-byte pair of 32 different records; at a 272-byte pitch those land on 16 distinct
+// 4-bank groups, the two passes 512 bytes need anyway (a 256-byte pitch puts all 32 on one group).
+constexpr int kWinStagePitch = 34;
+
 // LDS bytes of the per-wave staging area of win_schur: the records of 32 pairs (i and j).
 template <typename Real>
 constexpr int win_stage_bytes() {
-    return 2 * kWinPairsPerStage * kWinJs * (int)sizeof(Real);
+    return 2 * kWinPairsPerStage * kWinStagePitch * (int)sizeof(Real);
 }
 
 // One 64-lane round of the stage copy of win_schur: slot s = 64 r + lane of the stage's m pairs
-// (14 record chunks per pair: i chunks 0-4 and 6-8, j chunks 0-5).  load_round fetches the chunk
-// into registers, store_round puts it into the wave's staging area.
+// (13 record chunks of four values per pair: i chunks 0-5 (J~_c, H), j chunks 0-4 and 6-7 (J~_c,
+// J~_p)).  load_round fetches the chunk into registers, store_round puts it into the wave's
+// staging area.
 template <typename Real, int PS>
 __device__ __forceinline__ R4<Real> win_load_round(const Real* __restrict__ js, int2 pr, int m, int r) {
-    constexpr int CI = 8, CP = 14;
+    constexpr int CI = 6, CP = 13;
     const int s = 64 * r + (threadIdx.x & 63);
     const int k = min(s / CP, PS - 1), e = s - CP * (s / CP);
     const int oi = __shfl(pr.x, k, 64), oj = __shfl(pr.y, k, 64);
     R4<Real> v{};
     if (s < m * CP) {
         const bool is_j = e >= CI;
-        const int chunk = is_j ? e - CI : (e < 5 ? e : e + 1);
+        const int chunk = is_j ? (e - CI < 5 ? e - CI : e - CI + 1) : e;
         v = reinterpret_cast<const R4<Real>*>(js + kWinJs * (size_t)(is_j ? oj : oi))[chunk];
     }
     return v;
@@ -205,19 +207,19 @@ __device__ __forceinline__ R4<Real> win_load_round(const Real* __restrict__ js, 
 
 template <typename Real, int PS>
 __device__ __forceinline__ void win_store_round(Real* st_i, Real* st_j, int m, int r, const R4<Real>& v) {
-    constexpr int CI = 8, CP = 14;
+    constexpr int CI = 6, CP = 13;
     const int s = 64 * r + (threadIdx.x & 63);
     if (s < m * CP) {
         const int k = s / CP, e = s - CP * k;
         const bool is_j = e >= CI;
-        const int chunk = is_j ? e - CI : (e < 5 ? e : e + 1);
-        *reinterpret_cast<R4<Real>*>((is_j ? st_j : st_i) + kWinJs * k + 4 * chunk) = v;
+        const int chunk = is_j ? (e - CI < 5 ? e - CI : e - CI + 1) : e;
+        *reinterpret_cast<R4<Real>*>((is_j ? st_j : st_i) + kWinStagePitch * k + 4 * chunk) = v;
     }
 }
 
 // 9x9 blocks of the reduced camera system into S (n x n row-major, both triangles, global memory):
 // S_ab = [a == b] D_a^2 - sum_pairs J~_c,i' G_ij J~_c,j with G_ij = H_i J~_p,j' (i != j) and
-// G_ii = -(I - G)_i.  The blocks first + stride t, t < count, are taken through *counter.  Per stage of 32 pairs the
+// G_ii = H_i J~_p,i' - I = -(I - G)_i.  The blocks first + stride t, t < count, are taken through *counter.  Per stage of 32 pairs the
 // wave copies the record chunks the products read into its LDS staging area with 4-element vector
 // loads (lanes cover the chunks of consecutive pairs), then lanes 0-31 accumulate rows 0-4 and lanes
 // 32-63 rows 5-8 of pair q = lane % 32 in registers; the 32 partial sums of a block are combined by
@@ -230,7 +232,7 @@ __device__ __forceinline__ void win_schur(const int4* __restrict__ blk, const in
     const int lane = threadIdx.x & 63, q = lane & 31, h = lane >> 5;
     const int rb = 5 * h;
     Real* st_i = stage;
-    Real* st_j = stage + PS * kWinJs;
+    Real* st_j = stage + PS * kWinStagePitch;
     for (;;) {
         int t = 0;
         if (lane == 0) t = atomicAdd(counter, 1);
@@ -242,9 +244,9 @@ __device__ __forceinline__ void win_schur(const int4* __restrict__ blk, const in
         for (int r = 0; r < 5; ++r)
 #pragma unroll
             for (int c = 0; c < 9; ++c) acc[r][c] = (Real)0;
-        // record chunks the pair products read: i records chunks 0-4 (J~_c) and 6-8 (H, I - G), j
-        // records chunks 0-5 (J~_c, J~_p); 14 slots per pair, 7 rounds of 64 lanes per stage
-        static_assert(PS * 14 <= 7 * 64, "stage copy rounds");
+        // record chunks the pair products read: i records chunks 0-5 (J~_c, H), j records chunks
+        // 0-4 and 6-7 (J~_c, J~_p); 13 slots per pair, 7 rounds of 64 lanes per stage
+        static_assert(PS * 13 <= 7 * 64, "stage copy rounds");
         const int npairs = bk.w - bk.z;
         const int nst = (npairs + PS - 1) / PS;
         int2 pr = make_int2(0, 0), pr_next = make_int2(0, 0);
@@ -284,26 +286,22 @@ __device__ __forceinline__ void win_schur(const int4* __restrict__ blk, const in
             if (q < m) {
                 // pairs of consecutive elements from LDS (the two rows of a J~_c column are adjacent)
                 using R2 = R2v<Real>;
-                const Real* Ri = st_i + kWinJs * q;
-                const Real* Rj = st_j + kWinJs * q;
-                Real G00, G01, G10, G11;
+                const Real* Ri = st_i + kWinStagePitch * q;
+                const Real* Rj = st_j + kWinStagePitch * q;
+                const R2 h0 = *reinterpret_cast<const R2*>(Ri + 18), h1 = *reinterpret_cast<const R2*>(Ri + 20),
+                         h2 = *reinterpret_cast<const R2*>(Ri + 22);
+                const R2 p0 = *reinterpret_cast<const R2*>(Rj + 24), p1 = *reinterpret_cast<const R2*>(Rj + 26),
+                         p2 = *reinterpret_cast<const R2*>(Rj + 28);
+                // H_i = [h0.x h0.y h1.x; h1.y h2.x h2.y], J~_p,j = [p0.x p0.y p1.x; p1.y p2.x p2.y]
+                Real G00 = h0.x * p0.x + h0.y * p0.y + h1.x * p1.x;
+                Real G01 = h0.x * p1.y + h0.y * p2.x + h1.x * p2.y;
+                Real G10 = h1.y * p0.x + h2.x * p0.y + h2.y * p1.x;
+                Real G11 = h1.y * p1.y + h2.x * p2.x + h2.y * p2.y;
                 if (pi == pj) {
                     // an observation with itself carries its share of U*: J~c'(I - G)J~c, negated
                     // because the accumulated sum is subtracted
-                    const R2 g = *reinterpret_cast<const R2*>(Ri + 30);
-                    G00 = -g.x;
-                    G01 = G10 = -g.y;
-                    G11 = -Ri[32];
-                } else {
-                    const R2 h0 = *reinterpret_cast<const R2*>(Ri + 24), h1 = *reinterpret_cast<const R2*>(Ri + 26),
-                             h2 = *reinterpret_cast<const R2*>(Ri + 28);
-                    const R2 p0 = *reinterpret_cast<const R2*>(Rj + 18), p1 = *reinterpret_cast<const R2*>(Rj + 20),
-                             p2 = *reinterpret_cast<const R2*>(Rj + 22);
-                    // H_i = [h0.x h0.y h1.x; h1.y h2.x h2.y], J~_p,j = [p0.x p0.y p1.x; p1.y p2.x p2.y]
-                    G00 = h0.x * p0.x + h0.y * p0.y + h1.x * p1.x;
-                    G01 = h0.x * p1.y + h0.y * p2.x + h1.x * p2.y;
-                    G10 = h1.y * p0.x + h2.x * p0.y + h2.y * p1.x;
-                    G11 = h1.y * p1.y + h2.x * p2.x + h2.y * p2.y;
+                    G00 -= (Real)1;
+                    G11 -= (Real)1;
                 }
                 Real a0[5], a1[5];
 #pragma unroll
@@ -350,17 +348,12 @@ __device__ __forceinline__ int tri(int i) { return i * (i + 1) / 2; }
 // (double-buffered), so a column costs one barrier.  The trailing lower triangle is then updated
 // with the panel by 16 x 16 tiles on the matrix cores (one tile per wave at a time, K = 16 in four
 // steps; entries past n or above the diagonal are neither read nor written).  *bad is set when a
-// pivot is not positive (the factor is then meaningless and the caller rejects the step).  With
-// clamp (the FP32 factor of the mixed-precision solve, a CG preconditioner), a pivot below
-// kWinPivotFloor times the original diagonal entry -- FP32 cancellation in a direction the trust
-// region leaves nearly free -- is raised to that floor instead: the preconditioner then differs from
-// S in a few directions only, which CG resolves in a few steps; dinv holds the original diagonal on
-// entry.  Every entry's arithmetic is fixed by the algorithm, not by the workgroup size.
+// pivot is not positive (the factor is then meaningless and the caller rejects the step).  Every
+// entry's arithmetic is fixed by the algorithm, not by the workgroup size.
 constexpr int kWinNb = 16;
-constexpr double kWinPivotFloor = 1e-5;
 
 template <typename F, int NT>
-__device__ __forceinline__ void win_panel(F* L, int n, int jb, int je, F* colv, F* dinv, int* bad, bool clamp) {
+__device__ __forceinline__ void win_panel(F* L, int n, int jb, int je, F* colv, F* dinv, int* bad) {
     constexpr int NB = kWinNb, RS = (kWinMaxN + NT - 1) / NT;
     const int tid = threadIdx.x;
     F* cbuf = colv;           // 2 x NB published column entries
@@ -373,9 +366,6 @@ __device__ __forceinline__ void win_panel(F* L, int n, int jb, int je, F* colv, 
         for (int cc = 0; cc < NB; ++cc)
             a[s][cc] = (row < n && jb + cc < je && jb + cc <= row) ? L[tri(row) + jb + cc] : (F)0;
     }
-    F floor_[NB];  // read before the barrier: dinv[j] is overwritten during the panel
-#pragma unroll
-    for (int cc = 0; cc < NB; ++cc) floor_[cc] = clamp && jb + cc < je ? (F)kWinPivotFloor * dinv[jb + cc] : (F)0;
     if (tid == 0) pbuf[0] = a[0][0];
     __syncthreads();
 #pragma unroll
@@ -384,13 +374,7 @@ __device__ __forceinline__ void win_panel(F* L, int n, int jb, int je, F* colv, 
         if (j >= je) break;  // uniform; the columns past the last panel's end are not visited
         const int buf = jj & 1;
         F djj = pbuf[buf];
-        bool nonpos;
-        if (clamp) {
-            nonpos = !isfinite((double)djj) || !(floor_[jj] > (F)0);
-            if (!nonpos && !(djj > floor_[jj])) djj = floor_[jj];
-        } else {
-            nonpos = !(djj > (F)0);
-        }
+        const bool nonpos = !(djj > (F)0);
         if (nonpos) djj = (F)1;
         const F d = sqrt(djj), inv = (F)1 / d;
 #pragma unroll
@@ -424,12 +408,12 @@ __device__ __forceinline__ void win_panel(F* L, int n, int jb, int je, F* colv, 
 }
 
 template <typename F, int NT>
-__device__ __forceinline__ void win_cholesky(F* L, int n, F* colv, F* dinv, int* bad, bool clamp) {
+__device__ __forceinline__ void win_cholesky(F* L, int n, F* colv, F* dinv, int* bad) {
     constexpr int NW = NT / 64, NB = kWinNb;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int jb = 0; jb < n; jb += NB) {
         const int je = min(jb + NB, n);
-        win_panel<F, NT>(L, n, jb, je, colv, dinv, bad, clamp);
+        win_panel<F, NT>(L, n, jb, je, colv, dinv, bad);
         __syncthreads();
         if (je >= n) break;
         // trailing update: L[r][c] -= sum_{k in [jb, je)} L[r][k] L[c][k] for je <= c <= r < n
@@ -515,39 +499,6 @@ __device__ __forceinline__ void win_chol_solve(const F* L, const F* dinv, int n,
     }
 }
 
-// out = S y for the explicit FP64 reduced system of the window (row-major n x n in global memory,
-// L2-resident at these sizes); y and out are LDS vectors.  Waves take rows, lanes columns, rows
-// summed by a butterfly in a fixed order.
-template <int NT>
-__device__ __forceinline__ void win_apply_dense(const double* __restrict__ S, int n, const double* y, double* out) {
-    constexpr int NW = NT / 64;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int r = wave; r < n; r += NW) {
-        const double* row = S + (size_t)r * n;
-        double s = 0.0;
-        for (int c = lane; c < n; c += 64) s += row[c] * y[c];
-        s = wave_sum(s);
-        if (lane == 0) out[r] = s;
-    }
-    __syncthreads();
-}
-
-// a'b over n entries of two LDS vectors, by wave 0 in a fixed order; every thread gets the value.
-__device__ __forceinline__ double win_dot(const double* a, const double* b, int n, double* sh) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (wave == 0) {
-        double s = 0.0;
-        for (int k = lane; k < n; k += 64) s += a[k] * b[k];
-        s = wave_sum(s);
-        if (lane == 0) *sh = s;
-    }
-    __syncthreads();
-    const double v = *sh;
-    __syncthreads();
-    return v;
-}
-
-constexpr double kWinCgTol = 1e-8;  // relative residual at which the mixed-precision CG stops
 
 
 // ------------------------------------------------------------------------------------------------
@@ -782,18 +733,15 @@ __global__ void __launch_bounds__(NT) k_wrec_cam(const WinDesc* __restrict__ des
                 out[2 * i] = (Real)Jc[i];
                 out[2 * i + 1] = (Real)Jc[9 + i];
             }
-#pragma unroll
-            for (int i = 0; i < 6; ++i) out[18 + i] = (Real)Jp[i];
             for (int i = 0; i < 3; ++i) {
-                out[24 + i] = (Real)h0[i];
-                out[27 + i] = (Real)h1[i];
+                out[18 + i] = (Real)h0[i];
+                out[21 + i] = (Real)h1[i];
             }
+#pragma unroll
+            for (int i = 0; i < 6; ++i) out[24 + i] = (Real)Jp[i];
             g00 = 1.0 - (j0[0] * h0[0] + j0[1] * h0[1] + j0[2] * h0[2]);
             g01 = -(j0[0] * h1[0] + j0[1] * h1[1] + j0[2] * h1[2]);
             g11 = 1.0 - (j1[0] * h1[0] + j1[1] * h1[1] + j1[2] * h1[2]);
-            out[30] = (Real)g00;
-            out[31] = (Real)g01;
-            out[32] = (Real)g11;
             const double4 q = w.qp[p];
             const double e0 = r[0] - (Jp[0] * q.x + Jp[1] * q.y + Jp[2] * q.z);
             const double e1 = r[1] - (Jp[3] * q.x + Jp[4] * q.y + Jp[5] * q.z);
@@ -862,16 +810,14 @@ __global__ void __launch_bounds__(NT) k_wschur(const WinDesc* __restrict__ descs
                           tk.w, &counter, reinterpret_cast<Real*>(smem + (size_t)wave * win_stage_bytes<Real>()));
 }
 
-// Per window: factor S (in LDS) and solve for the camera step y_c (into yc); mixed precision
-// refines by preconditioned CG on the FP64 operator (at most cg_max iterations).
-template <typename Real, int NT>
+// Per window: factor S (FP64) and solve for the camera step y_c (into yc).  The packed factor lives
+// in LDS when the window's reduced system has at most lds_n unknowns and in the window's own global
+// buffer (L2-resident at these sizes) otherwise; the arithmetic is the same either way.
+template <int NT>
 __global__ void __launch_bounds__(NT) k_wsolve(const WinDesc* __restrict__ descs, char* __restrict__ arena,
-                                               const int* __restrict__ wtask, int ntask, int cg_max) {
-    using F = Real;
+                                               const int* __restrict__ wtask, int ntask, int lds_n) {
     extern __shared__ __align__(32) unsigned char smem[];
-    __shared__ double xv[kWinMaxN], rv[kWinMaxN], zv[kWinMaxN], dv[kWinMaxN], qv[kWinMaxN];
-    __shared__ F colv[kWinMaxN], dinv[kWinMaxN];
-    __shared__ double sdot;
+    __shared__ double xv[kWinMaxN], colv[kWinMaxN], dinv[kWinMaxN];
     __shared__ int bad;
     const int t = blockIdx.x;
     if (t >= ntask) return;
@@ -881,92 +827,33 @@ __global__ void __launch_bounds__(NT) k_wsolve(const WinDesc* __restrict__ descs
     if (st->done || st->invalid) return;
     const int K = w.K, n = 9 * K, tid = threadIdx.x;
     const double* Sg = at<double>(arena, D.o_S);
-    // the packed factor lives in LDS, or for reduced systems larger than the LDS holds in the
-    // window's own global buffer (L2-resident at these sizes)
-    F* S = D.o_L ? at<F>(arena, D.o_L) : reinterpret_cast<F*>(smem);
+    double* S = n > lds_n ? at<double>(arena, D.o_L) : reinterpret_cast<double*>(smem);
     const int ntri = n * (n + 1) / 2;
-    for (int attempt = 0;; ++attempt) {
-        F shift = (F)0;
-        if (attempt > 0) {
-            // largest diagonal entry, in a fixed order (wave 0)
-            if (tid < 64) {
-                double dmax = 0.0;
-                for (int k = tid; k < n; k += 64) dmax = fmax(dmax, (double)Sg[(size_t)k * n + k]);
-                dmax = wave_max(dmax);
-                if (tid == 0) sdot = dmax;
-            }
-            __syncthreads();
-            shift = (F)(sdot * 1e-6 * pow(100.0, attempt - 1));
-        }
-        // packed copy of the lower triangle, four loads in flight per thread
-        for (int b0 = tid; b0 < ntri; b0 += 4 * NT) {
-            F v[4];
-            int ix[4];
+    // packed copy of the lower triangle, eight loads in flight per thread
+    constexpr int U = 8;
+    for (int b0 = tid; b0 < ntri; b0 += U * NT) {
+        double v[U];
+        int ix[U];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int idx = min(b0 + u * NT, ntri - 1);
-                int i = (int)((sqrtf(8.0f * idx + 1.0f) - 1.0f) * 0.5f);
-                while (tri(i + 1) <= idx) ++i;
-                while (tri(i) > idx) --i;
-                ix[u] = idx;
-                v[u] = (F)Sg[(size_t)i * n + (idx - tri(i))];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (b0 + u * NT < ntri) S[ix[u]] = v[u];
+        for (int u = 0; u < U; ++u) {
+            const int idx = min(b0 + u * NT, ntri - 1);
+            int i = (int)((sqrtf(8.0f * idx + 1.0f) - 1.0f) * 0.5f);
+            while (tri(i + 1) <= idx) ++i;
+            while (tri(i) > idx) --i;
+            ix[u] = idx;
+            v[u] = Sg[(size_t)i * n + (idx - tri(i))];
         }
-        if (tid == 0) bad = 0;
-        __syncthreads();
-        if (attempt > 0)
-            for (int k = tid; k < n; k += NT) S[tri(k) + k] += shift;
-        __syncthreads();
-        for (int k = tid; k < n; k += NT) dinv[k] = S[tri(k) + k];  // the pivot floors' reference
-        __syncthreads();
-        win_cholesky<F, NT>(S, n, colv, dinv, &bad, cg_max > 0);
-        if (bad == 0 || cg_max == 0 || attempt == 3) break;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (b0 + u * NT < ntri) S[ix[u]] = v[u];
     }
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    win_cholesky<double, NT>(S, n, colv, dinv, &bad);
     if (bad) return;  // have_step stays 0: an invalid step
     for (int k = tid; k < n; k += NT) xv[k] = w.bc[k];
     __syncthreads();
-    win_chol_solve<F, NT>(S, dinv, n, xv);  // x0 = M^-1 b
-    if (cg_max > 0) {
-        for (int k = tid; k < n; k += NT) zv[k] = w.bc[k];
-        __syncthreads();
-        const double bb = win_dot(zv, zv, n, &sdot);
-        win_apply_dense<NT>(Sg, n, xv, qv);
-        for (int k = tid; k < n; k += NT) rv[k] = zv[k] - qv[k];
-        __syncthreads();
-        double rr = win_dot(rv, rv, n, &sdot);
-        if (rr > kWinCgTol * kWinCgTol * bb) {
-            for (int k = tid; k < n; k += NT) zv[k] = rv[k];
-            __syncthreads();
-            win_chol_solve<F, NT>(S, dinv, n, zv);
-            for (int k = tid; k < n; k += NT) dv[k] = zv[k];
-            __syncthreads();
-            double rz = win_dot(rv, zv, n, &sdot);
-            for (int cg = 0; cg < cg_max; ++cg) {
-                win_apply_dense<NT>(Sg, n, dv, qv);
-                const double dq = win_dot(dv, qv, n, &sdot);
-                if (!(dq > 0.0)) break;
-                const double alpha = rz / dq;
-                for (int k = tid; k < n; k += NT) {
-                    xv[k] += alpha * dv[k];
-                    rv[k] -= alpha * qv[k];
-                }
-                __syncthreads();
-                rr = win_dot(rv, rv, n, &sdot);
-                if (rr <= kWinCgTol * kWinCgTol * bb) break;
-                for (int k = tid; k < n; k += NT) zv[k] = rv[k];
-                __syncthreads();
-                win_chol_solve<F, NT>(S, dinv, n, zv);
-                const double rz_new = win_dot(rv, zv, n, &sdot);
-                const double beta = rz_new / rz;
-                rz = rz_new;
-                for (int k = tid; k < n; k += NT) dv[k] = zv[k] + beta * dv[k];
-                __syncthreads();
-            }
-        }
-    }
+    win_chol_solve<double, NT>(S, dinv, n, xv);
     for (int k = tid; k < n; k += NT) w.yc[k] = xv[k];
     if (tid == 0) st->have_step = 1;
 }

@@ -102,13 +102,13 @@ def test_alone_and_in_mix_bitwise_identical(precision):
     sizes = 0
     for ts in (256, 512):
         for tf in (256, 512):
-            for bpw in (1, 2, 4, 8):
+            for bpw, lds in ((1, 21), (2, 0), (4, 10), (8, 21)):
                 try:
-                    mix.set_schedule(ts, tf, bpw)
+                    mix.set_schedule(ts, tf, bpw, lds)
                 except S.BAError as e:
                     assert e.status == 6  # BA_ERR_CAPACITY
                     continue
-                assert mix.schedule()["blocks_per_wave"] == bpw
+                assert mix.schedule()["blocks_per_wave"] == bpw and mix.schedule()["lds_cameras"] == lds
                 sizes += 1
                 mix.reset()
                 assert _same(mix.run()[3], s_alone)

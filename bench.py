@@ -373,6 +373,25 @@ def bench_stereo_and_g2o(S, reps=10):
                             "observations": int(len(w["side"])), "gpu_ms_per_solve": best * 1e3,
                             "cpu_oracle_ms_per_solve": cpu * 1e3, "iterations": r["num_iterations"],
                             "outliers_flagged": int(r["outlier"].sum())}
+    # 64 such windows (3-11 keyframes, 60-360 points) at once on the batched engine, against the
+    # same windows one ba_stereo_solve call after the other
+    wins = [S.stereo_window(seed=500 + i, num_keyframes=3 + i % 9, num_points=60 + 25 * (i % 13)) for i in range(64)]
+    S.stereo_batch_solve(wins)
+    best_b = None
+    for _ in range(max(2, reps // 3)):
+        t0 = time.perf_counter()
+        S.stereo_batch_solve(wins)
+        dt = time.perf_counter() - t0
+        best_b = dt if best_b is None else min(best_b, dt)
+    t0 = time.perf_counter()
+    for x in wins:
+        S.stereo_solve(**{k: x[k] for k in args})
+    serial = time.perf_counter() - t0
+    out["stereo_windows_batched"] = {"windows": len(wins), "observations": int(sum(len(x["side"]) for x in wins)),
+                                     "gpu_ms_per_batch": best_b * 1e3, "gpu_ms_one_call_per_window": serial * 1e3,
+                                     "note": "end to end per call (upload, layout, solve, download, "
+                                             "outlier rule); batch = one ba_stereo_batch_solve call"}
+    log("bench: 64 stereo windows: %.1f ms batched, %.1f ms one call per window" % (best_b * 1e3, serial * 1e3))
     gz = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "ladybug49.txt.gz")
     with tempfile.TemporaryDirectory() as td:
         path = os.path.join(td, "ladybug49.txt")

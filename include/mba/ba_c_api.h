@@ -227,6 +227,14 @@ void ba_stereo_default_options(ba_options* options);
  * (optional) the final classification threshold. */
 int ba_stereo_solve(const ba_stereo_view* problem, const ba_options* options, ba_summary* summary, uint8_t* outlier,
                     double* chi2_threshold);
+/* Many full_slam windows at once (the local BA of backend.cpp:67-199 for several maps, robots or
+ * sliding windows): every window is solved as ba_stereo_solve would (same model, options and outlier
+ * rule) by the batched window engine (all windows advanced together, one launch sequence per LM
+ * trial; see ba_batch_create), in place.  summaries (optional) has `count` entries; outliers
+ * (optional) `count` pointers, each NULL or to num_observations bytes of that window;
+ * chi2_thresholds (optional) `count` doubles.  options NULL = ba_stereo_default_options. */
+int ba_stereo_batch_solve(const ba_stereo_view* windows, int32_t count, const ba_options* options,
+                          ba_summary* summaries, uint8_t* const* outliers, double* chi2_thresholds);
 
 /* Device-resident solver: create uploads the problem and builds the device layout once; run solves
  * from the parameters currently held on the device; reset restores the parameters given at create;

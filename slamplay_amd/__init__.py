@@ -26,7 +26,7 @@ from . import _capi as C
 from ._capi import BAError
 
 __all__ = ["BALProblem", "SolveBA", "Solver", "Batch", "Server", "simulate", "server_config", "local_ba_windows",
-           "default_options", "full_report",
+           "default_options", "full_report", "stereo_batch_solve",
            "eval_residual_jacobian", "eval_cost", "synthetic", "synth_params", "BAError", "device_count"]
 
 
@@ -485,6 +485,61 @@ def stereo_default_options(**kw):
             raise AttributeError(f"unknown option {k}")
         setattr(o, k, v)
     return o
+
+
+class _StereoArrays:
+    """A ba_stereo_view over private copies of one window's arrays (kept alive with the view)."""
+
+    def __init__(self, pose_index, point_index, side, observations, poses, points, K, extrinsics):
+        self.pi = np.ascontiguousarray(pose_index, dtype=np.int32)
+        self.qi = np.ascontiguousarray(point_index, dtype=np.int32)
+        self.sd = np.ascontiguousarray(side, dtype=np.int32)
+        self.obs = np.ascontiguousarray(observations, dtype=np.float64).reshape(-1)
+        self.P = np.array(poses, dtype=np.float64, copy=True).reshape(-1, 6)
+        self.X = np.array(points, dtype=np.float64, copy=True).reshape(-1, 3)
+        self.outlier = np.zeros(max(self.pi.size, 1), dtype=np.uint8)
+        v = C.StereoView()
+        v.num_poses, v.num_points, v.num_observations = self.P.shape[0], self.X.shape[0], self.pi.size
+        v.pose_index = self.pi.ctypes.data_as(C.i32p)
+        v.point_index = self.qi.ctypes.data_as(C.i32p)
+        v.side = self.sd.ctypes.data_as(C.i32p)
+        v.observations = self.obs.ctypes.data_as(C.f64p)
+        v.poses = self.P.ctypes.data_as(C.f64p)
+        v.points = self.X.ctypes.data_as(C.f64p)
+        for i in range(4):
+            v.K[i] = float(K[i])
+        ext = np.asarray(extrinsics, dtype=np.float64).reshape(2, 6)
+        for s in range(2):
+            for i in range(6):
+                v.extrinsics[s][i] = ext[s, i]
+        self.view = v
+
+
+def stereo_batch_solve(windows, options=None, **kw):
+    """Solves many full_slam windows together on the batched engine, each as stereo_solve would
+    (same model, rules and outlier rule).  windows: dicts with the arguments of stereo_solve
+    (pose_index, point_index, side, observations, poses, points, K, extrinsics).  Returns one dict per
+    window (poses, points, outlier, chi2_threshold and the summary fields); the inputs are not
+    modified."""
+    o = options if options is not None else stereo_default_options(**kw)
+    keys = ("pose_index", "point_index", "side", "observations", "poses", "points", "K", "extrinsics")
+    arrs = [_StereoArrays(*[w[k] for k in keys]) for w in windows]
+    n = len(arrs)
+    views = (C.StereoView * max(n, 1))()
+    for i, a in enumerate(arrs):
+        views[i] = a.view
+    sums = (C.Summary * max(n, 1))()
+    outp = (ctypes.POINTER(ctypes.c_uint8) * max(n, 1))()
+    for i, a in enumerate(arrs):
+        outp[i] = a.outlier.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    th = np.zeros(max(n, 1))
+    C.check(C.lib().ba_stereo_batch_solve(views, n, ctypes.byref(o), sums, outp, th.ctypes.data_as(C.f64p)))
+    res = []
+    for i, a in enumerate(arrs):
+        r = _summary_plain(sums[i])
+        r.update(poses=a.P, points=a.X, outlier=a.outlier[: a.pi.size].astype(bool), chi2_threshold=float(th[i]))
+        res.append(r)
+    return res
 
 
 def stereo_solve(pose_index, point_index, side, observations, poses, points, K, extrinsics, options=None, **kw):

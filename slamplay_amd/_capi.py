@@ -133,6 +133,9 @@ SIGNATURES = {
     "ba_server_simulate": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(ProblemView), i32, f64, i32,
                                            ctypes.c_uint64, ctypes.POINTER(ServerResult)]),
     "ba_stereo_default_options": (None, [ctypes.POINTER(Options)]),
+    "ba_stereo_batch_solve": (ctypes.c_int, [ctypes.POINTER(StereoView), i32, ctypes.POINTER(Options),
+                                              ctypes.POINTER(Summary), ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8)),
+                                              f64p]),
     "ba_stereo_solve": (ctypes.c_int, [ctypes.POINTER(StereoView), ctypes.POINTER(Options), ctypes.POINTER(Summary),
                                         ctypes.POINTER(ctypes.c_uint8), f64p]),
     "ba_bal_load": (ctypes.c_int, [ctypes.c_char_p, i32, ctypes.POINTER(vp)]),

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "common.h"
+#include "stereo.h"
 
 namespace mba {
 
@@ -49,7 +50,6 @@ struct SolverBase {
 };
 
 // The full_slam stereo model (stereo.h): rig and per-observation camera side (caller order).
-struct StereoRig;
 struct StereoInput {
     const StereoRig* rig;
     const int32_t* side;
@@ -58,7 +58,25 @@ struct StereoInput {
 int make_solver(const ba_problem_view* v, const ba_options& o, std::unique_ptr<SolverBase>* out,
                 const StereoInput* stereo = nullptr);
 
+// A full_slam stereo window in the solver's terms: poses in 9-parameter camera slots, the rig, and
+// the problem view over them (its points are the caller's).
+struct StereoProblem {
+    std::vector<double> cams;
+    StereoRig rig;
+    ba_problem_view view;
+    StereoInput input;
+};
+// Validates a stereo view and fills sp (the view's pose / rig conventions of ba_stereo_solve).
+int stereo_prepare(const ba_stereo_view* v, StereoProblem* sp);
+// Writes the solved poses (sp.cams) back into v and applies the reference's outlier rule
+// (full_slam/src/backend.cpp:153-186) from chi2_start to the residuals of v's points.
+void stereo_finish(const ba_stereo_view* v, const StereoProblem& sp, double chi2_start, uint8_t* outlier,
+                   double* chi2_threshold);
+
 // BA_OK, or BA_ERR_INVALID_ARGUMENT with the reason for an option out of range.
 int validate_options(const ba_options* o);
+// The device option block of ba_options (g2o mode: lambda-LM, no column scaling).
+struct Opts;
+Opts solver_opts(const ba_options& o);
 
 }  // namespace mba

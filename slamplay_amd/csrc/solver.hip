@@ -80,6 +80,8 @@ int check_options(const ba_options* o) {
 
 int validate_options(const ba_options* o) { return check_options(o); }
 
+Opts solver_opts(const ba_options& o) { return make_opts(o); }
+
 // ---------------------------------------------------------------------------------------------
 // Host-built device layout shared by the FP64 and mixed solvers.
 int build_layout(const ba_problem_view* v, Layout* L) {
@@ -783,9 +785,11 @@ void ba_stereo_default_options(ba_options* o) {
     o->linear_solver = BA_SOLVER_DENSE_SCHUR;    // LinearSolverCSparse on the 6-dof pose blocks
 }
 
-int ba_stereo_solve(const ba_stereo_view* v, const ba_options* options, ba_summary* summary, uint8_t* outlier,
-                    double* chi2_threshold) {
-    clear_error();
+}  // extern "C"
+
+namespace mba {
+
+int stereo_prepare(const ba_stereo_view* v, StereoProblem* out) {
     if (!v) return fail(BA_ERR_INVALID_ARGUMENT, "problem view is NULL");
     if (v->num_observations > 0 && !v->side) return fail(BA_ERR_INVALID_ARGUMENT, "side array is NULL");
     for (int k = 0; k < v->num_observations; ++k)
@@ -794,49 +798,44 @@ int ba_stereo_solve(const ba_stereo_view* v, const ba_options* options, ba_summa
     for (int i = 0; i < 4; ++i)
         if (!std::isfinite(v->K[i])) return fail(BA_ERR_NON_FINITE, "K is not finite");
     if (v->K[0] == 0.0 || v->K[1] == 0.0) return fail(BA_ERR_INVALID_ARGUMENT, "focal lengths must be nonzero");
-    ba_options o;
-    if (options)
-        o = *options;
-    else
-        ba_stereo_default_options(&o);
-    BA_TRY(check_options(&o));
     if (v->num_poses > 0 && !v->poses) return fail(BA_ERR_INVALID_ARGUMENT, "poses is NULL");
     // poses in the solver's 9-parameter slots (angle-axis, t, three unused entries)
-    std::vector<double> cams(9 * (size_t)std::max(v->num_poses, 0), 0.0);
+    out->cams.assign(9 * (size_t)std::max(v->num_poses, 0), 0.0);
     for (int i = 0; i < v->num_poses; ++i)
-        for (int k = 0; k < 6; ++k) cams[9 * (size_t)i + k] = v->poses[6 * (size_t)i + k];
-    ba_problem_view pv{v->num_poses, v->num_points, v->num_observations, 9, v->pose_index, v->point_index,
-                       v->observations, cams.data(), v->points};
-    BA_TRY(validate_view(&pv, true));
-    StereoRig rig{};
-    for (int i = 0; i < 4; ++i) rig.K[i] = v->K[i];
+        for (int k = 0; k < 6; ++k) out->cams[9 * (size_t)i + k] = v->poses[6 * (size_t)i + k];
+    out->view = ba_problem_view{v->num_poses, v->num_points, v->num_observations, 9, v->pose_index, v->point_index,
+                                v->observations, out->cams.data(), v->points};
+    BA_TRY(validate_view(&out->view, true));
+    out->rig = StereoRig{};
+    for (int i = 0; i < 4; ++i) out->rig.K[i] = v->K[i];
     for (int sd = 0; sd < 2; ++sd) {
         double e9[9] = {0};
         for (int k = 0; k < 6; ++k) e9[k] = v->extrinsics[sd][k];
         CameraPrep<double> P;
         camera_prep<double>(e9, P);
-        for (int k = 0; k < 9; ++k) rig.R[sd][k] = P.R64[k];
-        for (int k = 0; k < 3; ++k) rig.t[sd][k] = P.t64[k];
+        for (int k = 0; k < 9; ++k) out->rig.R[sd][k] = P.R64[k];
+        for (int k = 0; k < 3; ++k) out->rig.t[sd][k] = P.t64[k];
     }
-    const StereoInput in{&rig, v->side};
-    std::unique_ptr<SolverBase> sv;
-    BA_TRY(make_solver(&pv, o, &sv, &in));
-    BA_TRY(sv->run(summary));
-    BA_TRY(sv->download(cams.data(), v->points));
+    out->input = StereoInput{&out->rig, v->side};
+    return BA_OK;
+}
+
+void stereo_finish(const ba_stereo_view* v, const StereoProblem& sp, double chi2_start, uint8_t* outlier,
+                   double* chi2_threshold) {
     for (int i = 0; i < v->num_poses; ++i)
-        for (int k = 0; k < 6; ++k) v->poses[6 * (size_t)i + k] = cams[9 * (size_t)i + k];
+        for (int k = 0; k < 6; ++k) v->poses[6 * (size_t)i + k] = sp.cams[9 * (size_t)i + k];
     // the outlier rule of backend.cpp:153-186 on the optimised parameters
     const int no = v->num_observations;
     std::vector<double> chi2(no);
     for (int k = 0; k < no; ++k) {
         CameraPrep<double> P;
-        camera_prep<double>(&cams[9 * (size_t)v->pose_index[k]], P);
+        camera_prep<double>(&sp.cams[9 * (size_t)v->pose_index[k]], P);
         double r[2];
-        stereo_residual<double>(P, &v->points[3 * (size_t)v->point_index[k]], rig, v->side[k], v->observations[2 * (size_t)k],
-                                v->observations[2 * (size_t)k + 1], r);
+        stereo_residual<double>(P, &v->points[3 * (size_t)v->point_index[k]], sp.rig, v->side[k],
+                                v->observations[2 * (size_t)k], v->observations[2 * (size_t)k + 1], r);
         chi2[k] = r[0] * r[0] + r[1] * r[1];
     }
-    double th = o.loss_scale;
+    double th = chi2_start;
     for (int it = 0; it < 5; ++it) {
         int in_n = 0;
         for (int k = 0; k < no; ++k) in_n += chi2[k] > th ? 0 : 1;
@@ -846,6 +845,28 @@ int ba_stereo_solve(const ba_stereo_view* v, const ba_options* options, ba_summa
     if (outlier)
         for (int k = 0; k < no; ++k) outlier[k] = chi2[k] > th ? 1 : 0;
     if (chi2_threshold) *chi2_threshold = th;
+}
+
+}  // namespace mba
+
+extern "C" {
+
+int ba_stereo_solve(const ba_stereo_view* v, const ba_options* options, ba_summary* summary, uint8_t* outlier,
+                    double* chi2_threshold) {
+    clear_error();
+    ba_options o;
+    if (options)
+        o = *options;
+    else
+        ba_stereo_default_options(&o);
+    BA_TRY(check_options(&o));
+    StereoProblem sp;
+    BA_TRY(stereo_prepare(v, &sp));
+    std::unique_ptr<SolverBase> sv;
+    BA_TRY(make_solver(&sp.view, o, &sv, &sp.input));
+    BA_TRY(sv->run(summary));
+    BA_TRY(sv->download(sp.cams.data(), v->points));
+    stereo_finish(v, sp, o.loss_scale, outlier, chi2_threshold);
     return BA_OK;
 }
 

@@ -25,6 +25,7 @@ struct WinState {
     int rows;       // trace rows of the equivalent single-problem solve (iteration 0 included)
     int invalid;    // this iteration's damped point blocks were not all positive definite
     int have_step;  // this iteration produced a step
+    int g2o_iter, g2o_trials;  // g2o mode: iterations done, rejected trials of the current one
     int reason;     // ba_termination_reason of the test that ended the solve
     unsigned long long t_done;  // device wall clock when the window finished
     double reason_value;        // the quantity that test compared
@@ -35,7 +36,9 @@ struct WinDesc {
     int64_t o_cam_off, o_co_pt, o_co_uv64, o_slot_of, o_chunk_len, o_chunk_off, o_blk, o_pairs;
     int64_t o_cam_x, o_cam_x0, o_cam_c, o_pt_x, o_pt_x0, o_pt_c;
     int64_t o_prep, o_prep_c, o_udiag, o_gc, o_sc, o_dc2, o_bc, o_yc, o_cpart, o_ppart;
-    int64_t o_V, o_gp, o_sp, o_Vinv, o_qp, o_yp, o_pv, o_js, o_S, o_L, o_state;  // o_L: 0 = factor in LDS
+    int64_t o_V, o_gp, o_sp, o_Vinv, o_qp, o_yp, o_pv, o_js, o_S, o_L;
+    int64_t o_side, o_rig;  // stereo model: camera side per camera-order observation, the rig
+    int64_t o_state;
 };
 
 // Per-observation record (the FP64 residual and Jacobians are recomputed where needed):

@@ -45,25 +45,6 @@ int64_t lds_per_block() {
 
 int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
-Opts opts_of(const ba_options& o) {
-    Opts r{};
-    r.loss = o.loss;
-    r.max_iterations = o.max_num_iterations;
-    r.max_invalid = o.max_num_consecutive_invalid_steps;
-    r.jacobi_scaling = o.jacobi_scaling;
-    r.pcg_max_iterations = o.pcg_max_iterations;
-    r.loss_a = o.loss_scale;
-    r.ftol = o.function_tolerance;
-    r.gtol = o.gradient_tolerance;
-    r.ptol = o.parameter_tolerance;
-    r.max_radius = o.max_trust_region_radius;
-    r.min_radius = o.min_trust_region_radius;
-    r.min_rel_decrease = o.min_relative_decrease;
-    r.min_diag = o.min_lm_diagonal;
-    r.max_diag = o.max_lm_diagonal;
-    r.pcg_tol = o.pcg_tolerance;
-    return r;
-}
 
 // Static LDS of k_wsolve (xv, colv, dinv, bad) plus slack for alignment.
 int64_t solve_static_lds() { return 3 * kWinMaxN * 8 + 16 + 256; }
@@ -96,8 +77,11 @@ struct WindowImage {
     int64_t cost = 0;  // work estimate for dispatch order
 };
 
+// stereo: the full_slam model (rig and per-observation camera side, caller order); NULL for the
+// Snavely model.
 template <typename Real>
-int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowImage* img) {
+int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowImage* img,
+                       const StereoInput* stereo = nullptr) {
     Layout L;
     BA_TRY(build_layout(v, &L));
     PairBlocks pb;
@@ -144,6 +128,8 @@ int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowIm
     D.o_cam_x0 = take(72 * (int64_t)K);
     D.o_pt_x = take(32 * (int64_t)P);
     D.o_pt_x0 = take(32 * (int64_t)P);
+    D.o_side = take(stereo ? 4 * (int64_t)No : 0);
+    D.o_rig = take(stereo ? (int64_t)sizeof(StereoRig) : 0);
     D.o_state = take(sizeof(WinState));
     const int64_t input_bytes = off;  // everything below is work space, zeroed on the device
     D.o_cam_c = take(72 * (int64_t)K);
@@ -196,6 +182,11 @@ int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowIm
     }
     std::memcpy(b + D.o_cam_x, v->cameras, 72 * (size_t)K);
     std::memcpy(b + D.o_cam_x0, v->cameras, 72 * (size_t)K);
+    if (stereo) {
+        int* sd = reinterpret_cast<int*>(b + D.o_side);
+        for (int j = 0; j < No; ++j) sd[j] = stereo->side[L.co_orig[j]] ? 1 : 0;
+        std::memcpy(b + D.o_rig, stereo->rig, sizeof(StereoRig));
+    }
     double* px = reinterpret_cast<double*>(b + D.o_pt_x);
     double* px0 = reinterpret_cast<double*>(b + D.o_pt_x0);
     for (int ip = 0; ip < P; ++ip)
@@ -277,6 +268,7 @@ struct WindowEngine {
     int solve_threads = 256;  // block size of k_wsolve
     int blocks_per_wave = 2;  // 9x9 blocks of S per wave of k_wschur (task granularity)
     int lds_cameras = kWinMaxK;  // windows of at most this many cameras factor in LDS
+    int model = kModelSnavely;   // residual model of every window (kModelSnavely or kModelStereo)
     int max_slots = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // the current work set: task lists of the phase kernels (windows in dispatch order)
@@ -305,7 +297,7 @@ struct WindowEngine {
 
     int init(const ba_options& options, int64_t arena_bytes, int slots) {
         opt = options;
-        o = opts_of(options);
+        o = solver_opts(options);
         BA_TRY(use_device(options.device));
         BA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         BA_HIP(hipEventCreate(&ev0));
@@ -423,27 +415,27 @@ struct WindowEngine {
         return BA_OK;
     }
 
-    // NT: block size of k_wschur, NF: of k_wsolve
-    template <int NT, int NF>
+    // M: residual model, NT: block size of k_wschur, NF: of k_wsolve
+    template <int M, int NT, int NF>
     int enqueue_iteration_nt() {
         const WinDesc* dd = ddesc.p;
-        if (n_ctask) hipLaunchKernelGGL((k_wlin_cam<kWinCamThreads>), dim3(n_ctask), dim3(kWinCamThreads), 0, s, dd,
+        if (n_ctask) hipLaunchKernelGGL((k_wlin_cam<kWinCamThreads, M>), dim3(n_ctask), dim3(kWinCamThreads), 0, s, dd,
                                         arena.p, (const int2*)ctask.p, n_ctask, o);
         if (n_ptask) hipLaunchKernelGGL(k_wlin_pt, dim3(n_ptask), dim3(64), 0, s, dd, arena.p, (const int2*)ptask.p,
                                         n_ptask, o);
         hipLaunchKernelGGL((k_wdamp<256>), dim3(n_wtask), dim3(256), 0, s, dd, arena.p, (const int*)wtask.p, n_wtask, o);
-        if (n_ctask) hipLaunchKernelGGL((k_wrec_cam<double, kWinCamThreads>), dim3(n_ctask), dim3(kWinCamThreads), 0, s,
+        if (n_ctask) hipLaunchKernelGGL((k_wrec_cam<double, kWinCamThreads, M>), dim3(n_ctask), dim3(kWinCamThreads), 0, s,
                                         dd, arena.p, (const int2*)ctask.p, n_ctask, o);
         if (n_stask)
             hipLaunchKernelGGL((k_wschur<double, NT>), dim3(n_stask), dim3(NT), (size_t)schur_lds<double>(NT), s, dd, arena.p,
                                (const int4*)stask.p, n_stask);
         hipLaunchKernelGGL((k_wsolve<NF>), dim3(n_wtask), dim3(NF), solve_lds(), s, dd, arena.p, (const int*)wtask.p,
                            n_wtask, lds_n());
-        if (n_ctask) hipLaunchKernelGGL((k_wback_cam<kWinCamThreads>), dim3(n_ctask), dim3(kWinCamThreads), 0, s, dd,
+        if (n_ctask) hipLaunchKernelGGL((k_wback_cam<kWinCamThreads, M>), dim3(n_ctask), dim3(kWinCamThreads), 0, s, dd,
                                         arena.p, (const int2*)ctask.p, n_ctask, o);
         if (n_ptask) hipLaunchKernelGGL(k_wback_pt, dim3(n_ptask), dim3(64), 0, s, dd, arena.p, (const int2*)ptask.p,
                                         n_ptask);
-        if (n_ctask) hipLaunchKernelGGL((k_wcand_cam<kWinCamThreads>), dim3(n_ctask), dim3(kWinCamThreads), 0, s, dd,
+        if (n_ctask) hipLaunchKernelGGL((k_wcand_cam<kWinCamThreads, M>), dim3(n_ctask), dim3(kWinCamThreads), 0, s, dd,
                                         arena.p, (const int2*)ctask.p, n_ctask, o);
         hipLaunchKernelGGL((k_wdecide<256>), dim3(n_wtask), dim3(256), 0, s, dd, arena.p, (const int*)wtask.p, n_wtask,
                            o);
@@ -465,14 +457,20 @@ struct WindowEngine {
     // dynamic LDS of k_wsolve: the packed factor of the largest window factored in LDS
     size_t solve_lds() const { return (size_t)work_nmax * (work_nmax + 1) / 2 * sizeof(double); }
 
+    template <int M>
+    int enqueue_iteration_m() {
+        if (threads == 512)
+            return solve_threads == 512 ? enqueue_iteration_nt<M, 512, 512>() : enqueue_iteration_nt<M, 512, 256>();
+        return solve_threads == 512 ? enqueue_iteration_nt<M, 256, 512>() : enqueue_iteration_nt<M, 256, 256>();
+    }
+
     int enqueue_iteration() {
-        if (threads == 512) return solve_threads == 512 ? enqueue_iteration_nt<512, 512>() : enqueue_iteration_nt<512, 256>();
-        return solve_threads == 512 ? enqueue_iteration_nt<256, 512>() : enqueue_iteration_nt<256, 256>();
+        return model == kModelStereo ? enqueue_iteration_m<kModelStereo>() : enqueue_iteration_m<kModelSnavely>();
     }
 
     // Runs `iters` LM iterations of the windows in `slots` (dispatched in the given order); with
     // use_graph the iteration is captured once per work set and replayed.
-    int run(const std::vector<int>& slots, int iters, bool use_graph = false) {
+    int run(const std::vector<int>& slots, int iters, bool use_graph = false, bool restart_clock = true) {
         if (slots.empty()) return BA_OK;
         BA_TRY(set_work(slots));
         BA_TRY(set_lds_limits());
@@ -489,7 +487,7 @@ struct WindowEngine {
                 return fail(BA_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e2));
             }
         }
-        hipLaunchKernelGGL(k_wclock, dim3(1), dim3(64), 0, s, dclock.p);
+        if (restart_clock) hipLaunchKernelGGL(k_wclock, dim3(1), dim3(64), 0, s, dclock.p);
         for (int it = 0; it < iters; ++it) {
             if (use_graph)
                 BA_HIP(hipGraphLaunch(graph, s));
@@ -549,9 +547,10 @@ struct WindowEngine {
 };
 
 template <typename Real>
-void fill_summary(const WinState& st, ba_summary* sm) {
-    sm->initial_cost = st.initial_cost;
-    sm->final_cost = st.x_cost;
+void fill_summary(const WinState& st, ba_summary* sm, bool g2o = false) {
+    const double cscale = g2o ? 2.0 : 1.0;  // g2o reports chi2 = sum rho(s), without the 1/2
+    sm->initial_cost = cscale * st.initial_cost;
+    sm->final_cost = cscale * st.x_cost;
     sm->num_iterations = st.rows;
     sm->num_successful_steps = st.num_successful;
     sm->num_unsuccessful_steps = st.num_unsuccessful;
@@ -588,31 +587,38 @@ struct Batch final : BatchBase {
     ba_options opt{};
     ba_batch_schedule sched{};  // the schedule in use and the autotuner's timings
 
-    int create(const ba_problem_view* views, int count, const ba_options& o) {
+    // stereo: per window the full_slam stereo model (rig, sides), or NULL for the Snavely model.
+    // The g2o rules are available with the stereo model (full_slam's solver); Snavely windows run the
+    // Ceres rules (the g2o driver's rotation update is the single-problem solver's).
+    int create(const ba_problem_view* views, int count, const ba_options& o, const StereoInput* stereo = nullptr,
+               bool tune = true) {
         opt = o;
         BA_TRY(validate_options(&o));
-        if (o.mode != BA_MODE_CERES) return fail(BA_ERR_INVALID_ARGUMENT, "batched windows run the Ceres mode only");
+        if (o.mode != BA_MODE_CERES && !stereo)
+            return fail(BA_ERR_INVALID_ARGUMENT, "batched Snavely windows run the Ceres mode only");
         BA_TRY(use_device(o.device));
         imgs.resize(count);
         large_index.assign(count, -1);
         int64_t total = 0;
         for (int w = 0; w < count; ++w) {
             BA_TRY(validate_view(&views[w], true));
+            const StereoInput* sw = stereo ? &stereo[w] : nullptr;
             if (9 * views[w].num_cameras > kWinMaxN) {
                 large_index[w] = (int)large_solvers.size();
                 ba_options lo = o;
                 lo.linear_solver = BA_SOLVER_DENSE_SCHUR;
                 std::unique_ptr<SolverBase> sv;
-                BA_TRY(make_solver(&views[w], lo, &sv));
+                BA_TRY(make_solver(&views[w], lo, &sv, sw));
                 large_solvers.push_back(std::move(sv));
                 large.push_back(w);
                 continue;
             }
-            BA_TRY(build_window_image<Real>(&views[w], o, &imgs[w]));
+            BA_TRY(build_window_image<Real>(&views[w], o, &imgs[w], sw));
             total += imgs[w].total;
             small.push_back(w);
         }
         large_sum.assign(large_solvers.size(), ba_summary{});
+        eng.model = stereo ? kModelStereo : kModelSnavely;
         BA_TRY(eng.init(o, total, count));
         int64_t base = 0;
         for (int w : small) {
@@ -624,7 +630,10 @@ struct Batch final : BatchBase {
         BA_HIP(hipStreamSynchronize(eng.s));
         // the host images are only needed for their permutations from here on
         for (int w : small) std::vector<char>().swap(imgs[w].bytes);
-        return autotune();
+        if (tune) return autotune();
+        sched = ba_batch_schedule{};
+        apply(256, 256, 2, kWinMaxK);
+        return BA_OK;
     }
 
     // Times schedules on the batch itself and keeps the fastest; the windows' results do not depend on
@@ -724,7 +733,22 @@ struct Batch final : BatchBase {
 
     int run(ba_summary* summaries) override {
         BA_TRY(use_device(opt.device));
-        BA_TRY(eng.run(order, opt.max_num_iterations + 1, opt.launch_mode == 0));
+        const bool graphs = opt.launch_mode == 0;
+        int iters = opt.max_num_iterations + 1;  // the linearization, then one trial per launch sequence
+        BA_TRY(eng.run(order, iters, graphs));
+        if (opt.mode == BA_MODE_G2O) {
+            // a g2o iteration takes up to g2o_max_trials trials: continue four at a time while a
+            // window is still running
+            const int cap = opt.max_num_iterations * std::max(opt.g2o_max_trials, 1) + 1;
+            while (iters < cap) {
+                std::vector<WinState> st;
+                BA_TRY(eng.states(order, &st));
+                if (std::all_of(st.begin(), st.end(), [](const WinState& x) { return x.done != 0; })) break;
+                const int more = std::min(4, cap - iters);
+                BA_TRY(eng.run(order, more, graphs, false));
+                iters += more;
+            }
+        }
         for (size_t i = 0; i < large.size(); ++i) BA_TRY(large_solvers[i]->run(&large_sum[i]));
         std::vector<WinState> st;
         BA_TRY(eng.states(order, &st));
@@ -735,7 +759,7 @@ struct Batch final : BatchBase {
                 ba_summary& sm = summaries[order[i]];
                 ba_iteration* tr = sm.trace;
                 const int cap = sm.trace_capacity;
-                fill_summary<Real>(st[i], &sm);
+                fill_summary<Real>(st[i], &sm, opt.mode == BA_MODE_G2O);
                 sm.total_time_ms = fin[i];
                 sm.trace = tr;
                 sm.trace_capacity = cap;
@@ -771,9 +795,10 @@ struct Batch final : BatchBase {
 // CG against the FP64 system to reach the FP64 step (measured on the 512-window batch: 0.73 ms per
 // LM iteration for the FP32 factor + CG against 0.32 ms for the FP64 factor).  Windows above 32
 // cameras go to the single-problem solver with the caller's precision.
-int make_batch(const ba_problem_view* views, int count, const ba_options& o, std::unique_ptr<BatchBase>* out) {
+int make_batch(const ba_problem_view* views, int count, const ba_options& o, std::unique_ptr<BatchBase>* out,
+               const StereoInput* stereo = nullptr, bool tune = true) {
     auto b = std::make_unique<Batch<double>>();
-    BA_TRY(b->create(views, count, o));
+    BA_TRY(b->create(views, count, o, stereo, tune));
     *out = std::move(b);
     return BA_OK;
 }
@@ -1311,6 +1336,38 @@ int ba_batch_solve(const ba_problem_view* windows, int32_t count, const ba_optio
     std::unique_ptr<ba_batch> guard(b);
     BA_TRY(b->impl->run(summaries));
     for (int w = 0; w < count; ++w) BA_TRY(b->impl->download(w, windows[w].cameras, windows[w].points));
+    return BA_OK;
+}
+
+int ba_stereo_batch_solve(const ba_stereo_view* windows, int32_t count, const ba_options* options,
+                          ba_summary* summaries, uint8_t* const* outliers, double* chi2_thresholds) {
+    clear_error();
+    if (count < 0 || (count > 0 && !windows)) return fail(BA_ERR_INVALID_ARGUMENT, "invalid window array");
+    if (count == 0) return BA_OK;
+    ba_options o;
+    if (options)
+        o = *options;
+    else
+        ba_stereo_default_options(&o);
+    BA_TRY(validate_options(&o));
+    std::vector<StereoProblem> sp(count);
+    std::vector<ba_problem_view> views(count);
+    std::vector<StereoInput> in(count);
+    for (int w = 0; w < count; ++w) {
+        const int st = stereo_prepare(&windows[w], &sp[w]);
+        if (st != BA_OK) return fail(st, "window %d: %s", w, ba_last_error_string());
+        views[w] = sp[w].view;
+        in[w] = StereoInput{&sp[w].rig, windows[w].side};
+    }
+    // one solve per layout: the default schedule (autotuning would time more iterations than the solve)
+    std::unique_ptr<BatchBase> b;
+    BA_TRY(make_batch(views.data(), count, o, &b, in.data(), false));
+    BA_TRY(b->run(summaries));
+    for (int w = 0; w < count; ++w) {
+        BA_TRY(b->download(w, sp[w].cams.data(), windows[w].points));
+        stereo_finish(&windows[w], sp[w], o.loss_scale, outliers ? outliers[w] : nullptr,
+                      chi2_thresholds ? &chi2_thresholds[w] : nullptr);
+    }
     return BA_OK;
 }
 

@@ -16,7 +16,10 @@
 //   k_wback_cam   (window, camera) u = J~_p'(r~ - J~_c y_c) into slot order (FP64);
 //   k_wback_pt    (window, point chunk) y_p = V*^-1 sum u, candidate points, |dx|^2, |x|^2;
 //   k_wcand_cam   (window, camera) candidate camera, model cost change, FP64 candidate cost;
-//   k_wdecide     (window) Ceres LM rule, accepted candidates copied into the parameters.
+//   k_wdecide     (window) Ceres LM rule (or g2o's Levenberg rule), accepted candidates copied into
+//                 the parameters.
+// Residual model M (a template parameter of the kernels that evaluate residuals): the Snavely camera
+// of the BAL path, or full_slam's stereo rig with the SE3 pose update (stereo.h).
 // The per-observation FP64 residual and Jacobian are recomputed by each kernel that needs them
 // (k_wlin_cam, k_wrec_cam, k_wback_cam, k_wcand_cam) from the camera's prep, the point and the
 // measurement: about 300 FP64 operations against 26 stored doubles written once and read three
@@ -47,6 +50,8 @@ struct WinPtrs {
     double *V, *gp, *sp, *Vinv, *yp;
     double4 *pt_x, *pt_c, *pv, *qp;
     CameraPrep<double>*prep, *prep_c;
+    const int* side;       // stereo model: camera side per camera-order observation
+    const StereoRig* rig;  // stereo model
     WinState* st;
 };
 
@@ -85,8 +90,39 @@ __device__ __forceinline__ WinPtrs win_ptrs(const WinDesc& D, char* arena) {
     w.qp = at<double4>(arena, D.o_qp);
     w.prep = at<CameraPrep<double>>(arena, D.o_prep);
     w.prep_c = at<CameraPrep<double>>(arena, D.o_prep_c);
+    w.side = at<int>(arena, D.o_side);
+    w.rig = at<StereoRig>(arena, D.o_rig);
     w.st = at<WinState>(arena, D.o_state);
     return w;
+}
+
+// The window's rig into the workgroup's copy (stereo model; a no-op for the others).  Thread 0
+// copies; the caller's next barrier publishes it.
+template <int M>
+__device__ __forceinline__ void win_load_rig(const WinPtrs& w, StereoRig& g) {
+    if constexpr (M == kModelStereo)
+        if (threadIdx.x == 0) g = *w.rig;
+}
+
+// Residual and Jacobians of observation j (point p, measurement (u, v)) under model M.
+template <int M>
+__device__ __forceinline__ void win_model_jacobian(const WinPtrs& w, const CameraPrep<double>& P, const StereoRig& g,
+                                                   int j, const double X[3], double r[2], double Jc[18], double Jp[6]) {
+    const double u = w.co_uv64[2 * (size_t)j], v = w.co_uv64[2 * (size_t)j + 1];
+    if constexpr (M == kModelStereo)
+        stereo_residual_jacobian<double>(P, X, g, w.side[j], u, v, r, Jc, Jp);
+    else
+        snavely_residual_jacobian<double>(P, X, u, v, r, Jc, Jp);
+}
+
+template <int M>
+__device__ __forceinline__ void win_model_residual(const WinPtrs& w, const CameraPrep<double>& P, const StereoRig& g,
+                                                   int j, const double X[3], double r[2]) {
+    const double u = w.co_uv64[2 * (size_t)j], v = w.co_uv64[2 * (size_t)j + 1];
+    if constexpr (M == kModelStereo)
+        stereo_residual<double>(P, X, g, w.side[j], u, v, r);
+    else
+        snavely_residual<double>(P, X, u, v, r);
 }
 
 // Sum of N per-thread values over a workgroup of NT threads in a fixed order (wave butterflies,
@@ -111,11 +147,12 @@ __device__ __forceinline__ void wg_sum_fixed(double (&v)[N], double* lds) {
 
 // Huber-corrected FP64 residual and Jacobians of observation j (point p) at the current parameters
 // (P: the camera's prep at its current parameters), as k_wlin_cam forms them.
-__device__ __forceinline__ void win_obs(const WinPtrs& w, const CameraPrep<double>& P, int j, int p, const Opts& o,
-                                        double r[2], double Jc[18], double Jp[6]) {
+template <int M>
+__device__ __forceinline__ void win_obs(const WinPtrs& w, const CameraPrep<double>& P, const StereoRig& g, int j, int p,
+                                        const Opts& o, double r[2], double Jc[18], double Jp[6]) {
     const double4 x4 = w.pt_x[p];
     const double X[3] = {x4.x, x4.y, x4.z};
-    snavely_residual_jacobian<double>(P, X, w.co_uv64[2 * (size_t)j], w.co_uv64[2 * (size_t)j + 1], r, Jc, Jp);
+    win_model_jacobian<M>(w, P, g, j, X, r, Jc, Jp);
     const double s = huber_sqrt_rho1<double>(r[0] * r[0] + r[1] * r[1], o.loss, o.loss_a);
 #pragma unroll
     for (int i = 0; i < 18; ++i) Jc[i] *= s;
@@ -126,9 +163,11 @@ __device__ __forceinline__ void win_obs(const WinPtrs& w, const CameraPrep<doubl
 }
 
 // win_obs with the Jacobi scales applied: J~_c = J_c diag(s_c), J~_p = J_p diag(s_p).
-__device__ __forceinline__ void win_scaled(const WinPtrs& w, const CameraPrep<double>& P, int j, int p, const Opts& o,
-                                           const double sc[9], double r[2], double Jc[18], double Jp[6]) {
-    win_obs(w, P, j, p, o, r, Jc, Jp);
+template <int M>
+__device__ __forceinline__ void win_scaled(const WinPtrs& w, const CameraPrep<double>& P, const StereoRig& g, int j,
+                                           int p, const Opts& o, const double sc[9], double r[2], double Jc[18],
+                                           double Jp[6]) {
+    win_obs<M>(w, P, g, j, p, o, r, Jc, Jp);
     const double* sp = w.sp + 3 * (size_t)p;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
@@ -505,11 +544,12 @@ __device__ __forceinline__ void win_chol_solve(const F* L, const F* dinv, int n,
 // Phase kernels.  ctask: (slot, camera) per workgroup; ptask: (slot, chunk) per 64-thread
 // workgroup; wtask: slot per workgroup; stask: (slot, first block, block stride, block count).
 
-template <int NT>
+template <int NT, int M>
 __global__ void __launch_bounds__(NT) k_wlin_cam(const WinDesc* __restrict__ descs, char* __restrict__ arena,
                                                  const int2* __restrict__ ctask, int ntask, Opts o) {
     __shared__ double lds[19 * (NT / 64)];
     __shared__ CameraPrep<double> Psh;
+    __shared__ StereoRig Gsh;
     const int t = blockIdx.x;
     if (t >= ntask) return;
     const int2 tk = ctask[t];
@@ -522,6 +562,7 @@ __global__ void __launch_bounds__(NT) k_wlin_cam(const WinDesc* __restrict__ des
         camera_prep<double>(w.cam_x + 9 * (size_t)c, Psh);
         w.prep[c] = Psh;
     }
+    win_load_rig<M>(w, Gsh);
     __syncthreads();
     double acc[19];
 #pragma unroll
@@ -531,7 +572,7 @@ __global__ void __launch_bounds__(NT) k_wlin_cam(const WinDesc* __restrict__ des
         const double4 x4 = w.pt_x[p];
         const double X[3] = {x4.x, x4.y, x4.z};
         double r[2], Jc[18], Jp[6];
-        snavely_residual_jacobian<double>(Psh, X, w.co_uv64[2 * (size_t)j], w.co_uv64[2 * (size_t)j + 1], r, Jc, Jp);
+        win_model_jacobian<M>(w, Psh, Gsh, j, X, r, Jc, Jp);
         const double s2 = r[0] * r[0] + r[1] * r[1];
         if (first) acc[18] += half_rho(s2, o.loss, o.loss_a);
         const double s = huber_sqrt_rho1<double>(s2, o.loss, o.loss_a);
@@ -637,14 +678,24 @@ __global__ void __launch_bounds__(NT) k_wdamp(const WinDesc* __restrict__ descs,
                 st->x_cost = st->initial_cost = cost;
                 st->rows = 1;
                 if (!isfinite(cost)) lm_finish(st, BA_FAILURE, BA_REASON_NON_FINITE_COST, cost);
+                if (o.g2o) {
+                    // g2o's computeLambdaInit: tau times the largest diagonal entry of J'J
+                    double hmax = 0.0;
+                    for (int k = 0; k < n; ++k) hmax = fmax(hmax, w.udiag[k]);
+                    for (int p = 0; p < P; ++p) {
+                        const double* V = w.V + 6 * (size_t)p;
+                        hmax = fmax(hmax, fmax(V[0], fmax(V[3], V[5])));
+                    }
+                    st->radius = 1.0 / (o.g2o_tau * hmax);  // radius = 1 / lambda
+                }
             }
             st->gmax = gm;
             st->need_lin = 0;
             st->first_lin = 0;
             if (!st->done && first && st->iteration >= o.max_iterations)
                 lm_finish(st, BA_NO_CONVERGENCE, BA_REASON_MAX_ITERATIONS, st->iteration);
-            if (!st->done && gm <= o.gtol) lm_finish(st, BA_CONVERGENCE, BA_REASON_GRADIENT_TOLERANCE, gm);
-            if (!st->done && st->radius <= o.min_radius)
+            if (!o.g2o && !st->done && gm <= o.gtol) lm_finish(st, BA_CONVERGENCE, BA_REASON_GRADIENT_TOLERANCE, gm);
+            if (!o.g2o && !st->done && st->radius <= o.min_radius)
                 lm_finish(st, BA_CONVERGENCE, BA_REASON_MIN_RADIUS, st->radius);
         }
         if (tid == 0 && st->done) st->t_done = wall_clock64();
@@ -687,13 +738,14 @@ __global__ void __launch_bounds__(NT) k_wdamp(const WinDesc* __restrict__ descs,
 }
 
 // Per (window, camera): records of the reduced system and b_c.
-template <typename Real, int NT>
+template <typename Real, int NT, int M>
 __global__ void __launch_bounds__(NT) k_wrec_cam(const WinDesc* __restrict__ descs, char* __restrict__ arena,
                                                  const int2* __restrict__ ctask, int ntask, Opts o) {
     constexpr int NW = NT / 64;
     __shared__ double lds[9 * NW];
     __shared__ double dacc[NW][45];  // per wave: upper triangle of sum J~_c'(I - G)J~_c
     __shared__ CameraPrep<double> Psh;
+    __shared__ StereoRig Gsh;
     const int t = blockIdx.x;
     if (t >= ntask) return;
     const int2 tk = ctask[t];
@@ -704,6 +756,7 @@ __global__ void __launch_bounds__(NT) k_wrec_cam(const WinDesc* __restrict__ des
     Real* js = at<Real>(arena, D.o_js);
     const int c = tk.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (threadIdx.x == 0) Psh = w.prep[c];
+    win_load_rig<M>(w, Gsh);
     double scc[9];
     for (int i = 0; i < 9; ++i) scc[i] = w.sc[9 * (size_t)c + i];
     double bacc[9];
@@ -721,7 +774,7 @@ __global__ void __launch_bounds__(NT) k_wrec_cam(const WinDesc* __restrict__ des
         if (j < j_end) {
             const int p = w.co_pt[j];
             double r[2], Jc[18], Jp[6];
-            win_scaled(w, Psh, j, p, o, scc, r, Jc, Jp);
+            win_scaled<M>(w, Psh, Gsh, j, p, o, scc, r, Jc, Jp);
             const double* Vi = w.Vinv + 6 * (size_t)p;
             const double j0[3] = {Jp[0], Jp[1], Jp[2]}, j1[3] = {Jp[3], Jp[4], Jp[5]};
             double h0[3], h1[3];
@@ -859,10 +912,11 @@ __global__ void __launch_bounds__(NT) k_wsolve(const WinDesc* __restrict__ descs
 }
 
 // Per (window, camera): u = J~_p'(r~ - J~_c y_c) into slot order.
-template <int NT>
+template <int NT, int M>
 __global__ void __launch_bounds__(NT) k_wback_cam(const WinDesc* __restrict__ descs, char* __restrict__ arena,
                                                   const int2* __restrict__ ctask, int ntask, Opts o) {
     __shared__ CameraPrep<double> Psh;
+    __shared__ StereoRig Gsh;
     const int t = blockIdx.x;
     if (t >= ntask) return;
     const int2 tk = ctask[t];
@@ -872,6 +926,7 @@ __global__ void __launch_bounds__(NT) k_wback_cam(const WinDesc* __restrict__ de
     if (st->done || !st->have_step || tk.y >= w.K) return;
     const int c = tk.y;
     if (threadIdx.x == 0) Psh = w.prep[c];
+    win_load_rig<M>(w, Gsh);
     double scc[9], y[9];
     for (int i = 0; i < 9; ++i) {
         scc[i] = w.sc[9 * (size_t)c + i];
@@ -881,7 +936,7 @@ __global__ void __launch_bounds__(NT) k_wback_cam(const WinDesc* __restrict__ de
     for (int j = w.cam_off[c] + threadIdx.x; j < w.cam_off[c + 1]; j += NT) {
         const int p = w.co_pt[j];
         double r[2], Jc[18], Jp[6];
-        win_scaled(w, Psh, j, p, o, scc, r, Jc, Jp);
+        win_scaled<M>(w, Psh, Gsh, j, p, o, scc, r, Jc, Jp);
         double a0 = r[0], a1 = r[1];
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
@@ -931,11 +986,12 @@ __global__ void __launch_bounds__(64) k_wback_pt(const WinDesc* __restrict__ des
 
 // Per (window, camera): candidate camera, model cost change -(J~s)'(r~ + J~s/2) (step s = -y) and
 // FP64 candidate cost of the camera's observations, |dx_c|^2, |x_c|^2.
-template <int NT>
+template <int NT, int M>
 __global__ void __launch_bounds__(NT) k_wcand_cam(const WinDesc* __restrict__ descs, char* __restrict__ arena,
                                                   const int2* __restrict__ ctask, int ntask, Opts o) {
     __shared__ double lds[2 * (NT / 64)];
     __shared__ CameraPrep<double> Psh, Pxs;  // the candidate's prep, the current one's
+    __shared__ StereoRig Gsh;
     __shared__ double csh[9];
     const int t = blockIdx.x;
     if (t >= ntask) return;
@@ -947,13 +1003,22 @@ __global__ void __launch_bounds__(NT) k_wcand_cam(const WinDesc* __restrict__ de
     const int c = tk.y;
     if (threadIdx.x < 9) {
         const size_t k = 9 * (size_t)c + threadIdx.x;
-        const double v = w.cam_x[k] - w.sc[k] * w.yc[k];
-        w.cam_c[k] = v;
-        csh[threadIdx.x] = v;
+        csh[threadIdx.x] = w.cam_x[k] - w.sc[k] * w.yc[k];
     }
     if (threadIdx.x == NT - 1) Pxs = w.prep[c];
+    win_load_rig<M>(w, Gsh);
     __syncthreads();
-    if (threadIdx.x == 0) camera_prep<double>(csh, Psh);
+    if (threadIdx.x == 0) {
+        if constexpr (M == kModelStereo) {
+            // pose <- exp(xi) pose, xi = (rho, phi) = the step's first six entries
+            const double* x = w.cam_x + 9 * (size_t)c;
+            double xi[6];
+            for (int i = 0; i < 6; ++i) xi[i] = -w.sc[9 * (size_t)c + i] * w.yc[9 * (size_t)c + i];
+            se3_left_update(xi, x, x + 3, csh, csh + 3);
+        }
+        for (int i = 0; i < 9; ++i) w.cam_c[9 * (size_t)c + i] = csh[i];
+        camera_prep<double>(csh, Psh);
+    }
     __syncthreads();
     double scc[9], y[9];
     for (int i = 0; i < 9; ++i) {
@@ -964,7 +1029,7 @@ __global__ void __launch_bounds__(NT) k_wcand_cam(const WinDesc* __restrict__ de
     for (int j = w.cam_off[c] + threadIdx.x; j < w.cam_off[c + 1]; j += NT) {
         const int p = w.co_pt[j];
         double rx[2], Jc[18], Jp[6];
-        win_scaled(w, Pxs, j, p, o, scc, rx, Jc, Jp);
+        win_scaled<M>(w, Pxs, Gsh, j, p, o, scc, rx, Jc, Jp);
         const double* yp = w.yp + 3 * (size_t)p;
         double m0 = 0.0, m1 = 0.0;
         for (int i = 0; i < 9; ++i) {
@@ -979,7 +1044,7 @@ __global__ void __launch_bounds__(NT) k_wcand_cam(const WinDesc* __restrict__ de
         const double4 x4 = w.pt_c[p];
         const double X[3] = {x4.x, x4.y, x4.z};
         double r[2];
-        snavely_residual<double>(Psh, X, w.co_uv64[2 * (size_t)j], w.co_uv64[2 * (size_t)j + 1], r);
+        win_model_residual<M>(w, Psh, Gsh, j, X, r);
         acc[1] += half_rho(r[0] * r[0] + r[1] * r[1], o.loss, o.loss_a);
     }
     wg_sum_fixed<2, NT>(acc, lds);
@@ -1004,7 +1069,98 @@ __global__ void k_wclock(unsigned long long* t) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *t = wall_clock64();
 }
 
-// Per window: the LM decision (Ceres rules) and, for an accepted step, the new parameters.
+// The Ceres LM rule on a window's trial (k_decide's, without the trace): returns whether the
+// candidate is accepted.
+__device__ __forceinline__ int ceres_decide(WinState* st, const Opts& o, bool have_step, double model, double cand,
+                                            double dx2, double x2) {
+    int accepted = 0;
+    const bool valid = have_step && model > 0.0 && isfinite(dx2);
+    bool record = true;
+    if (!valid) {
+        if (++st->consecutive_invalid >= o.max_invalid) {
+            lm_finish(st, BA_FAILURE, BA_REASON_INVALID_STEPS, st->consecutive_invalid);
+            record = false;
+        } else {
+            st->radius = st->radius / st->decrease_factor;
+            st->decrease_factor *= 2.0;
+            st->num_unsuccessful++;
+        }
+    } else {
+        st->consecutive_invalid = 0;
+        if (!isfinite(cand)) cand = DBL_MAX;
+        const double step_norm = sqrt(dx2), x_norm = sqrt(x2);
+        if (step_norm <= o.ptol * (x_norm + o.ptol)) {
+            lm_finish(st, BA_CONVERGENCE, BA_REASON_PARAMETER_TOLERANCE, step_norm / (x_norm + o.ptol));
+            record = false;
+        } else {
+            const double cost_change = st->x_cost - cand;
+            if (fabs(cost_change) <= o.ftol * st->x_cost) {
+                lm_finish(st, BA_CONVERGENCE, BA_REASON_FUNCTION_TOLERANCE, fabs(cost_change) / st->x_cost);
+                record = false;
+            } else {
+                const double rho = cost_change / model;
+                if (rho > o.min_rel_decrease) {
+                    st->x_cost = cand;
+                    const double tt = 2.0 * rho - 1.0;
+                    st->radius = fmin(o.max_radius, st->radius / fmax(1.0 / 3.0, 1.0 - tt * tt * tt));
+                    st->decrease_factor = 2.0;
+                    st->need_lin = 1;
+                    st->num_successful++;
+                    accepted = 1;
+                } else {
+                    st->radius = st->radius / st->decrease_factor;
+                    st->decrease_factor *= 2.0;
+                    st->num_unsuccessful++;
+                }
+            }
+        }
+    }
+    if (record) st->rows++;
+    if (record && !st->done) {
+        if (st->iteration >= o.max_iterations)
+            lm_finish(st, BA_NO_CONVERGENCE, BA_REASON_MAX_ITERATIONS, st->iteration);
+        else if (!accepted && st->gmax <= o.gtol)
+            lm_finish(st, BA_CONVERGENCE, BA_REASON_GRADIENT_TOLERANCE, st->gmax);
+        else if (!accepted && st->radius <= o.min_radius)
+            lm_finish(st, BA_CONVERGENCE, BA_REASON_MIN_RADIUS, st->radius);
+    }
+    return accepted;
+}
+
+// g2o's OptimizationAlgorithmLevenberg on a window's trial (k_decide's g2o branch): rho = (chi2 -
+// chi2_new) / (x'(lambda x + b) + 1e-3) with chi2 = 2 cost and x'(lambda x + b) = 2 model; accept
+// when rho > 0, lambda *= max(1/3, min(2/3, 1 - (2 rho - 1)^3)), else lambda *= nu, nu *= 2; an
+// iteration ends with an accepted trial, the solve after g2o_max_trials rejected ones (radius =
+// 1 / lambda, decrease_factor = nu); every trial is a row.
+__device__ __forceinline__ int g2o_decide(WinState* st, const Opts& o, bool have_step, double model, double cand) {
+    int accepted = 0;
+    if (!isfinite(cand)) cand = DBL_MAX;
+    const double rho = have_step ? 2.0 * (st->x_cost - cand) / (2.0 * model + 1e-3) : -1.0;
+    if (have_step && rho > 0.0 && cand < DBL_MAX) {
+        const double tt = 2.0 * rho - 1.0;
+        st->radius = st->radius / fmax(1.0 / 3.0, fmin(2.0 / 3.0, 1.0 - tt * tt * tt));
+        st->decrease_factor = 2.0;
+        st->x_cost = cand;
+        st->need_lin = 1;
+        st->g2o_iter++;
+        st->g2o_trials = 0;
+        st->num_successful++;
+        accepted = 1;
+    } else {
+        st->radius = st->radius / st->decrease_factor;
+        st->decrease_factor *= 2.0;
+        st->g2o_trials++;
+        st->num_unsuccessful++;
+    }
+    st->rows++;
+    if (st->g2o_iter >= o.max_iterations)
+        lm_finish(st, BA_NO_CONVERGENCE, BA_REASON_MAX_ITERATIONS, st->g2o_iter);
+    else if (st->g2o_trials >= o.g2o_max_trials || rho == 0.0 || !isfinite(st->radius) || !(st->radius > 0.0))
+        lm_finish(st, BA_CONVERGENCE, BA_REASON_G2O_STOP, rho);
+    return accepted;
+}
+
+// Per window: the LM decision (Ceres or g2o rules) and, for an accepted step, the new parameters.
 template <int NT>
 __global__ void __launch_bounds__(NT) k_wdecide(const WinDesc* __restrict__ descs, char* __restrict__ arena,
                                                 const int* __restrict__ wtask, int ntask, Opts o) {
@@ -1034,56 +1190,10 @@ __global__ void __launch_bounds__(NT) k_wdecide(const WinDesc* __restrict__ desc
         }
         model = -model;
         st->iteration++;
-        const bool valid = have_step && model > 0.0 && isfinite(dx2);
-        bool record = true;
-        if (!valid) {
-            if (++st->consecutive_invalid >= o.max_invalid) {
-                lm_finish(st, BA_FAILURE, BA_REASON_INVALID_STEPS, st->consecutive_invalid);
-                record = false;
-            } else {
-                st->radius = st->radius / st->decrease_factor;
-                st->decrease_factor *= 2.0;
-                st->num_unsuccessful++;
-            }
-        } else {
-            st->consecutive_invalid = 0;
-            if (!isfinite(cand)) cand = DBL_MAX;
-            const double step_norm = sqrt(dx2), x_norm = sqrt(x2);
-            if (step_norm <= o.ptol * (x_norm + o.ptol)) {
-                lm_finish(st, BA_CONVERGENCE, BA_REASON_PARAMETER_TOLERANCE, step_norm / (x_norm + o.ptol));
-                record = false;
-            } else {
-                const double cost_change = st->x_cost - cand;
-                if (fabs(cost_change) <= o.ftol * st->x_cost) {
-                    lm_finish(st, BA_CONVERGENCE, BA_REASON_FUNCTION_TOLERANCE, fabs(cost_change) / st->x_cost);
-                    record = false;
-                } else {
-                    const double rho = cost_change / model;
-                    if (rho > o.min_rel_decrease) {
-                        st->x_cost = cand;
-                        const double tt = 2.0 * rho - 1.0;
-                        st->radius = fmin(o.max_radius, st->radius / fmax(1.0 / 3.0, 1.0 - tt * tt * tt));
-                        st->decrease_factor = 2.0;
-                        st->need_lin = 1;
-                        st->num_successful++;
-                        accepted = 1;
-                    } else {
-                        st->radius = st->radius / st->decrease_factor;
-                        st->decrease_factor *= 2.0;
-                        st->num_unsuccessful++;
-                    }
-                }
-            }
-        }
-        if (record) st->rows++;
-        if (record && !st->done) {
-            if (st->iteration >= o.max_iterations)
-                lm_finish(st, BA_NO_CONVERGENCE, BA_REASON_MAX_ITERATIONS, st->iteration);
-            else if (!accepted && st->gmax <= o.gtol)
-                lm_finish(st, BA_CONVERGENCE, BA_REASON_GRADIENT_TOLERANCE, st->gmax);
-            else if (!accepted && st->radius <= o.min_radius)
-                lm_finish(st, BA_CONVERGENCE, BA_REASON_MIN_RADIUS, st->radius);
-        }
+        if (o.g2o)
+            accepted = g2o_decide(st, o, have_step, model, cand);
+        else
+            accepted = ceres_decide(st, o, have_step, model, cand, dx2, x2);
         if (st->done) st->t_done = wall_clock64();
     }
     __syncthreads();

@@ -332,6 +332,51 @@ def test_stereo_local_ba_matches_oracle(oracle_mod, seed):
     assert out["outlier"][~planted].mean() < 0.08  # 1 px noise: 5% of inliers exceed chi2 5.991
 
 
+def _stereo_batch_windows(n, seed0):
+    return [S.stereo_window(seed=seed0 + i, num_keyframes=3 + i % 9, num_points=60 + 25 * (i % 13)) for i in range(n)]
+
+
+def test_stereo_windows_batched_match_oracle(oracle_mod):
+    """full_slam windows on the batched engine (ba_stereo_batch_solve): 48 windows of 3 to 11
+    keyframes solved together with the reference's rules (g2o Levenberg, 10 iterations, Huber 5.991),
+    each against the CPU oracle's solve_stereo: final chi2 within 1e-9, poses and points within 1e-7
+    of their scale, outlier flags and threshold equal to the reference's rule on the oracle's final
+    residuals; and each window's result the one ba_stereo_solve gives it alone, to the same bars."""
+    wins = _stereo_batch_windows(48, 700)
+    res = S.stereo_batch_solve(wins)
+    assert len(res) == len(wins)
+    for k, (w, r) in enumerate(zip(wins, res)):
+        ref = _stereo_oracle(oracle_mod, w)
+        final = ref["final_cost"]
+        assert abs(r["final_cost"] - final) <= 1e-9 * final, (k, r["final_cost"], final)
+        assert abs(r["initial_cost"] - ref["initial_cost"]) <= 1e-12 * ref["initial_cost"], k
+        np.testing.assert_allclose(r["poses"], ref["poses"], rtol=0, atol=1e-7 * np.abs(ref["poses"]).max())
+        np.testing.assert_allclose(r["points"], ref["pts"], rtol=0, atol=1e-7 * np.abs(ref["pts"]).max())
+        chi2 = np.array([(oracle_mod.stereo_residual(ref["poses"][w["pose_index"][j]], ref["pts"][w["point_index"][j]],
+                                                     w["K"], w["extrinsics"], w["side"][j], *w["observations"][j]) ** 2).sum()
+                         for j in range(len(w["side"]))])
+        flags, th = _outlier_rule(chi2)
+        assert r["chi2_threshold"] == th, k
+        if np.abs(chi2 - th).min() > 1e-6 * th:  # a chi2 within rounding of the threshold could flip
+            np.testing.assert_array_equal(r["outlier"], flags)
+        one = S.stereo_solve(**_stereo_args(w), precision=0)
+        assert abs(one["final_cost"] - r["final_cost"]) <= 1e-9 * final, k
+        assert r["termination"] == one["termination"], k
+
+
+def test_stereo_window_alone_and_in_batch_bitwise_identical():
+    """A stereo window's batched result does not depend on the other windows of the batch."""
+    wins = _stereo_batch_windows(12, 900)
+    alone = S.stereo_batch_solve([wins[5]])[0]
+    mixed = S.stereo_batch_solve(wins)[5]
+    for key in ("final_cost", "initial_cost", "num_iterations", "num_successful_steps", "termination",
+                "chi2_threshold"):
+        assert alone[key] == mixed[key], key
+    np.testing.assert_array_equal(alone["poses"], mixed["poses"])
+    np.testing.assert_array_equal(alone["points"], mixed["points"])
+    np.testing.assert_array_equal(alone["outlier"], mixed["outlier"])
+
+
 def test_stereo_local_ba_mixed_final_cost(oracle_mod):
     w = S.stereo_window(seed=3)
     ref = _stereo_oracle(oracle_mod, w)

@@ -223,6 +223,16 @@ def test_g2o_mode_rejected_by_batch_and_options():
             S.Batch(wins, S.default_options(**bad))
 
 
+def test_stereo_batch_inputs_rejected():
+    """ba_stereo_batch_solve checks every window before any device work and names the bad one."""
+    wins = [S.stereo_window(seed=40 + i, num_keyframes=3, num_points=20) for i in range(3)]
+    wins[1] = dict(wins[1], side=wins[1]["side"].copy())
+    wins[1]["side"][3] = 5
+    with pytest.raises(C.BAError, match="window 1.*side"):
+        S.stereo_batch_solve(wins)
+    assert S.stereo_batch_solve([]) == []
+
+
 def test_stereo_inputs_rejected():
     """ba_stereo_solve checks the side flags and the focal lengths before any device work."""
     w = S.stereo_window(seed=4, num_keyframes=3, num_points=20)

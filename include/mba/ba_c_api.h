@@ -61,7 +61,14 @@ typedef enum ba_mode {
     BA_MODE_G2O = 1    /* test_bundle_adjustment_g2o.cpp's SolveBA (g2o Levenberg, SO(3) rotation update) */
 } ba_mode;
 
-typedef enum ba_termination { BA_CONVERGENCE = 0, BA_NO_CONVERGENCE = 1, BA_FAILURE = 2 } ba_termination;
+/* ceres::TerminationType's values; BA_USER_FAILURE (Ceres: a callback aborted the solve) marks a
+ * serving request ended by ba_server_cancel. */
+typedef enum ba_termination {
+    BA_CONVERGENCE = 0,
+    BA_NO_CONVERGENCE = 1,
+    BA_FAILURE = 2,
+    BA_USER_FAILURE = 4
+} ba_termination;
 
 /* Zero-copy view of a BAL problem.  cameras: 9 doubles per camera (angle-axis[3], t[3], f, k1, k2),
  * points: 3 doubles per point; both updated in place by ba_solve.  observations: x,y interleaved. */
@@ -336,6 +343,7 @@ typedef struct ba_server_result {
     int32_t num_points;
     const double* cameras;  /* solved parameters, owned by the server, valid until the next poll */
     const double* points;
+    int32_t evictions;      /* times the window was evicted (ba_server_evict) and admitted again */
 } ba_server_result;
 typedef struct ba_server ba_server;
 int ba_server_create(const ba_server_config* config, ba_server** out);
@@ -344,6 +352,18 @@ int ba_server_submit(ba_server* server, const ba_problem_view* window, int64_t t
 /* Admits queued windows that fit, runs one step, retires finished windows.  *resident receives the
  * number of windows on the device after the step, *queued the number still waiting. */
 int ba_server_step(ba_server* server, int32_t* resident, int32_t* queued);
+/* Cancels request `tag` at the next ba_server_step: a queued or resident window leaves the server
+ * with a result of termination_type BA_USER_FAILURE carrying the parameters it has then (those of
+ * its last accepted step, the submitted ones if it never ran; costs NaN if it never ran), a large
+ * problem still waiting for the single-problem solver likewise.  A tag that names nothing pending
+ * then (already finished, unknown, a large problem being solved) is ignored.  Thread-safe. */
+int ba_server_cancel(ba_server* server, int64_t tag);
+/* Evicts resident window `tag` at the next ba_server_step: its whole device region (parameters,
+ * LM state, trust region, working arrays) is copied to host memory, its slot and arena range are
+ * freed, and it rejoins the back of the queue.  Admitted again (into any free slot and range), it
+ * continues from that state, and its result is bit for bit that of a solve never evicted.  A tag
+ * that is not resident then is ignored.  Thread-safe. */
+int ba_server_evict(ba_server* server, int64_t tag);
 /* Pops up to `capacity` finished results (oldest first). */
 int ba_server_poll(ba_server* server, ba_server_result* out, int32_t capacity, int32_t* count);
 double ba_server_clock(ba_server* server);

@@ -404,7 +404,7 @@ def server_config(arena_bytes=1 << 30, max_windows=512, iterations_per_step=1, o
 def _result_dict(r, with_params):
     d = dict(tag=r.tag, initial_cost=r.initial_cost, final_cost=r.final_cost, iterations=r.iterations,
              termination=r.termination_type, submit_time_s=r.submit_time_s, admit_time_s=r.admit_time_s,
-             finish_time_s=r.finish_time_s, latency_s=r.finish_time_s - r.submit_time_s)
+             finish_time_s=r.finish_time_s, latency_s=r.finish_time_s - r.submit_time_s, evictions=r.evictions)
     if with_params:
         d["cameras"] = np.ctypeslib.as_array(r.cameras, (9 * r.num_cameras,)).copy() if r.num_cameras else np.zeros(0)
         d["points"] = np.ctypeslib.as_array(r.points, (3 * r.num_points,)).copy() if r.num_points else np.zeros(0)
@@ -428,6 +428,14 @@ class Server:
         q = C.i32(0)
         C.check(C.lib().ba_server_step(self._h, ctypes.byref(r), ctypes.byref(q)))
         return r.value, q.value
+
+    def cancel(self, tag):
+        """Request `tag` leaves at the next step with termination BA_USER_FAILURE (4)."""
+        C.check(C.lib().ba_server_cancel(self._h, tag))
+
+    def evict(self, tag):
+        """Resident window `tag` is moved to host memory at the next step and queued again."""
+        C.check(C.lib().ba_server_evict(self._h, tag))
 
     def poll(self, capacity=256):
         buf = (C.ServerResult * capacity)()

@@ -86,7 +86,8 @@ class ServerConfig(ctypes.Structure):
 class ServerResult(ctypes.Structure):
     _fields_ = [("tag", i64), ("initial_cost", f64), ("final_cost", f64), ("iterations", i32),
                 ("termination_type", i32), ("submit_time_s", f64), ("admit_time_s", f64), ("finish_time_s", f64),
-                ("num_cameras", i32), ("num_points", i32), ("cameras", f64p), ("points", f64p)]
+                ("num_cameras", i32), ("num_points", i32), ("cameras", f64p), ("points", f64p),
+                ("evictions", i32)]
 
 
 # name -> (restype, argtypes)
@@ -128,6 +129,8 @@ SIGNATURES = {
     "ba_server_submit": (ctypes.c_int, [vp, ctypes.POINTER(ProblemView), i64]),
     "ba_server_step": (ctypes.c_int, [vp, i32p, i32p]),
     "ba_server_poll": (ctypes.c_int, [vp, ctypes.POINTER(ServerResult), i32, i32p]),
+    "ba_server_cancel": (ctypes.c_int, [vp, i64]),
+    "ba_server_evict": (ctypes.c_int, [vp, i64]),
     "ba_server_clock": (f64, [vp]),
     "ba_server_destroy": (ctypes.c_int, [vp]),
     "ba_server_simulate": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(ProblemView), i32, f64, i32,

@@ -39,6 +39,7 @@ const char* termination_name(int t) {
     switch (t) {
         case BA_CONVERGENCE: return "CONVERGENCE";
         case BA_NO_CONVERGENCE: return "NO_CONVERGENCE";
+        case BA_USER_FAILURE: return "USER_FAILURE";
         default: return "FAILURE";
     }
 }

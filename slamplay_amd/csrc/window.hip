@@ -13,6 +13,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -810,6 +811,9 @@ struct ServerBase {
     virtual int submit(const ba_problem_view* v, int64_t tag) = 0;
     virtual int step(int* resident, int* queued) = 0;
     virtual int poll(ba_server_result* out, int cap, int* n) = 0;
+    // Recorded from any thread, applied by the next step.
+    virtual int cancel(int64_t tag) = 0;
+    virtual int evict(int64_t tag) = 0;
     virtual double clock() const = 0;
     // One line of state for error messages: queue, ready images, resident windows, large problems.
     virtual std::string describe() = 0;
@@ -831,6 +835,9 @@ struct Request {
     std::unique_ptr<WindowImage> img;
     int build_status = BA_OK;
     std::atomic<int> ready{0};
+    // times the window left the device with its state (its image then holds the whole region:
+    // parameters, LM state and working arrays, relocatable because every offset is window-local)
+    int evictions = 0;
     ba_problem_view view() {
         ba_problem_view v{};
         v.num_cameras = nc;
@@ -870,6 +877,9 @@ struct Server final : ServerBase {
     // thread than ba_server_step)
     std::mutex qmu;
     std::deque<std::unique_ptr<Request>> queue;
+    // cancellations and evictions recorded by cancel() / evict() (guarded by qmu), applied by step()
+    enum { kOpCancel = 0, kOpEvict = 1 };
+    std::vector<std::pair<int64_t, int>> ops;
     std::deque<Done> done;
     std::vector<Done> polled;   // keeps the last poll's parameter arrays alive
     // requests too large for a window slot: solved by the single-problem solver on worker threads
@@ -1035,6 +1045,7 @@ struct Server final : ServerBase {
             ba_summary sm{};
             Done d;
             d.r.tag = r->tag;
+            d.r.evictions = r->evictions;
             d.r.submit_time_s = r->submit;
             d.r.admit_time_s = r->admit;
             int st = make_solver(&v, o, &sv);
@@ -1060,6 +1071,7 @@ struct Server final : ServerBase {
     void reject(std::unique_ptr<Request> r) {
         Done d;
         d.r.tag = r->tag;
+        d.r.evictions = r->evictions;
         d.r.submit_time_s = r->submit;
         d.r.admit_time_s = d.r.finish_time_s = clock();
         d.r.termination_type = BA_FAILURE;
@@ -1067,6 +1079,155 @@ struct Server final : ServerBase {
         d.pts = std::move(r->pts);
         std::lock_guard<std::mutex> lk(mu);
         done.push_back(std::move(d));
+    }
+
+    int cancel(int64_t tag) override {
+        std::lock_guard<std::mutex> lk(qmu);
+        ops.emplace_back(tag, (int)kOpCancel);
+        return BA_OK;
+    }
+
+    int evict(int64_t tag) override {
+        std::lock_guard<std::mutex> lk(qmu);
+        ops.emplace_back(tag, (int)kOpEvict);
+        return BA_OK;
+    }
+
+    // A cancelled request's result: BA_USER_FAILURE with the parameters it had (those of its last
+    // accepted step when it had run; `st` is its state then, NULL if it never ran).
+    void post_cancelled(std::unique_ptr<Request> r, const WinState* st) {
+        Done d;
+        d.r.tag = r->tag;
+        d.r.evictions = r->evictions;
+        d.r.submit_time_s = r->submit;
+        d.r.admit_time_s = r->admit > 0.0 ? r->admit : clock();
+        d.r.finish_time_s = clock();
+        d.r.termination_type = BA_USER_FAILURE;
+        d.r.initial_cost = d.r.final_cost = std::numeric_limits<double>::quiet_NaN();
+        if (st) {
+            d.r.initial_cost = st->initial_cost;
+            d.r.final_cost = st->x_cost;
+            d.r.iterations = st->rows;
+        }
+        d.cams = std::move(r->cams);
+        d.pts = std::move(r->pts);
+        std::lock_guard<std::mutex> lk(mu);
+        done.push_back(std::move(d));
+    }
+
+    // Frees slot `sl` and its arena range; the resident's request is returned.
+    std::unique_ptr<Request> vacate(int sl) {
+        Resident& r = res[sl];
+        std::unique_ptr<Request> q = std::move(r.req);
+        release(r.off, r.size);
+        r = Resident{};
+        free_slots.push_back(sl);
+        active.erase(std::find(active.begin(), active.end(), sl));
+        return q;
+    }
+
+    // Cancels the resident window in slot `sl`: its current parameters and state are read back.
+    int cancel_resident(int sl) {
+        std::vector<WinState> st;
+        BA_TRY(eng.states({sl}, &st));
+        Request& q = *res[sl].req;
+        BA_TRY(eng.download(sl, res[sl].img.pt_perm, q.cams.data(), q.pts.data()));
+        post_cancelled(vacate(sl), &st[0]);
+        return BA_OK;
+    }
+
+    // Evicts the resident window in slot `sl`: its whole region is copied to host memory and the
+    // request rejoins the back of the queue, ready to be placed anywhere in the arena again.
+    int evict_resident(int sl) {
+        Resident& r = res[sl];
+        auto img = std::make_unique<WindowImage>(std::move(r.img));
+        img->bytes.resize((size_t)r.size);
+        BA_HIP(hipMemcpyAsync(img->bytes.data(), eng.arena.p + r.off, (size_t)r.size, hipMemcpyDeviceToHost, eng.s));
+        BA_TRY(eng.sync("evicted window"));
+        std::unique_ptr<Request> q = vacate(sl);
+        q->img = std::move(img);
+        q->evictions++;
+        std::lock_guard<std::mutex> lk(qmu);
+        queue.push_back(std::move(q));
+        return BA_OK;
+    }
+
+    // A queued request that had been evicted carries its parameters and state in its image.
+    void cancel_queued(std::unique_ptr<Request> r) {
+        if (r->evictions == 0 || !r->img) {
+            post_cancelled(std::move(r), nullptr);
+            return;
+        }
+        const WindowImage& img = *r->img;
+        const WinDesc& D = img.desc;
+        WinState st;
+        std::memcpy(&st, img.bytes.data() + D.o_state, sizeof(WinState));
+        std::memcpy(r->cams.data(), img.bytes.data() + D.o_cam_x, 72 * (size_t)D.K);
+        const double* px = reinterpret_cast<const double*>(img.bytes.data() + D.o_pt_x);
+        for (int ip = 0; ip < D.P; ++ip)
+            for (int k = 0; k < 3; ++k) r->pts[3 * (size_t)img.pt_perm[ip] + k] = px[4 * (size_t)ip + k];
+        post_cancelled(std::move(r), &st);
+    }
+
+    // Applies the recorded cancellations and evictions.  A cancellation of a request whose image
+    // is still being built waits for a later step; tags that name nothing pending (finished,
+    // unknown, a large problem already on the single-problem solver, an eviction of a window that
+    // is not resident) are dropped.
+    int apply_ops() {
+        std::vector<std::pair<int64_t, int>> todo, later;
+        {
+            std::lock_guard<std::mutex> lk(qmu);
+            todo.swap(ops);
+        }
+        for (const auto& op : todo) {
+            const int64_t tag = op.first;
+            const bool is_cancel = op.second == kOpCancel;
+            const auto it = std::find_if(active.begin(), active.end(), [&](int sl) { return res[sl].req->tag == tag; });
+            if (it != active.end()) {
+                BA_TRY(is_cancel ? cancel_resident(*it) : evict_resident(*it));
+                continue;
+            }
+            if (!is_cancel) continue;
+            std::unique_ptr<Request> r;
+            bool building = false;
+            {
+                std::lock_guard<std::mutex> lk(qmu);
+                for (auto q = queue.begin(); q != queue.end(); ++q)
+                    if ((*q)->tag == tag) {
+                        if ((*q)->ready.load(std::memory_order_acquire)) {
+                            r = std::move(*q);
+                            queue.erase(q);
+                        } else {
+                            building = true;
+                        }
+                        break;
+                    }
+            }
+            if (building) {
+                later.push_back(op);
+                continue;
+            }
+            if (r) {
+                cancel_queued(std::move(r));
+                continue;
+            }
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                for (auto q = big_queue.begin(); q != big_queue.end(); ++q)
+                    if ((*q)->tag == tag) {
+                        r = std::move(*q);
+                        big_queue.erase(q);
+                        big_inflight--;
+                        break;
+                    }
+            }
+            if (r) post_cancelled(std::move(r), nullptr);
+        }
+        if (!later.empty()) {
+            std::lock_guard<std::mutex> lk(qmu);
+            ops.insert(ops.begin(), later.begin(), later.end());
+        }
+        return BA_OK;
     }
 
     int admit() {
@@ -1103,7 +1264,7 @@ struct Server final : ServerBase {
                 r.req = std::move(queue.front());
                 queue.pop_front();
             }
-            r.req->admit = clock();
+            if (r.req->evictions == 0) r.req->admit = clock();  // the first admission
             r.img = std::move(*r.req->img);
             r.req->img.reset();
             BA_TRY(eng.place(r.img, slot, off));
@@ -1140,6 +1301,7 @@ struct Server final : ServerBase {
 
     int step(int* resident, int* queued) override {
         BA_TRY(use_device(cfg.options.device));
+        BA_TRY(apply_ops());
         BA_TRY(admit());
         if (!active.empty()) {
             std::vector<int> order = active;
@@ -1181,6 +1343,7 @@ struct Server final : ServerBase {
                 const WinDesc& D = eng.hdesc[sl];
                 Done d;
                 d.r.tag = r.req->tag;
+                d.r.evictions = r.req->evictions;
                 d.r.submit_time_s = r.req->submit;
                 d.r.admit_time_s = r.req->admit;
                 d.r.finish_time_s = now;
@@ -1394,6 +1557,18 @@ int ba_server_step(ba_server* s, int32_t* resident, int32_t* queued) {
     if (resident) *resident = r;
     if (queued) *queued = q;
     return BA_OK;
+}
+
+int ba_server_cancel(ba_server* s, int64_t tag) {
+    clear_error();
+    if (!s) return fail(BA_ERR_INVALID_ARGUMENT, "NULL server");
+    return s->impl->cancel(tag);
+}
+
+int ba_server_evict(ba_server* s, int64_t tag) {
+    clear_error();
+    if (!s) return fail(BA_ERR_INVALID_ARGUMENT, "NULL server");
+    return s->impl->evict(tag);
 }
 
 int ba_server_poll(ba_server* s, ba_server_result* out, int32_t capacity, int32_t* count) {

@@ -4,6 +4,8 @@ Bars: FP64 windows follow the oracle's dense-Schur LM (final cost within 1e-9 re
 termination and step counts on a fixed schedule); mixed-precision windows end within 1e-5 of the
 FP64 oracle; a window's result is bitwise the same alone, in any mix and through the server.
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -245,6 +247,91 @@ def test_server_results_equal_alone(arena):
         np.testing.assert_array_equal(r["cameras"], cams)
         np.testing.assert_array_equal(r["points"], pts)
         assert r["finish_time_s"] >= r["admit_time_s"] >= r["submit_time_s"]
+
+
+def _alone(p, opts):
+    b = S.Batch([p], options=opts)
+    s = b.run()[0]
+    cams, pts = b.download(0)
+    return s, cams, pts
+
+
+def _step_until_resident(srv, n):
+    """Steps until n windows are resident (images are built by the server's threads after submit;
+    admission is FIFO and waits for the head's image)."""
+    time.sleep(0.5)
+    for _ in range(2000):
+        resident, _ = srv.step()
+        if resident == n:
+            return
+        time.sleep(0.005)
+    raise AssertionError("windows were not admitted")
+
+
+def test_server_evict_and_readmit_bitwise():
+    """A window evicted to host memory mid-solve (twice for one of them) and admitted again into
+    another slot and arena range finishes with bitwise the result of a solve never evicted."""
+    probs = [window(620 + i, 5 + (5 * i) % 14, 300 + 131 * i) for i in range(6)]
+    opts = S.default_options(precision=1, max_num_iterations=10, **FIXED)
+    srv = S.Server(S.server_config(arena_bytes=1 << 30, max_windows=3, iterations_per_step=1, options=opts))
+    for i, p in enumerate(probs):
+        srv.submit(p, i)
+    _step_until_resident(srv, 3)  # 0, 1, 2, after at least one iteration each
+    srv.step()
+    results = srv.poll()
+    assert not results
+    srv.evict(0)
+    srv.evict(2)
+    srv.evict(99)  # not resident: ignored
+    for _ in range(4):
+        srv.step()
+        results += srv.poll()
+    srv.evict(0)  # a second time if it is resident again by now
+    results += srv.drain()
+    assert sorted(r["tag"] for r in results) == list(range(6))
+    ev = {r["tag"]: r["evictions"] for r in results}
+    assert ev[0] >= 1 and ev[2] == 1 and ev[1] == ev[3] == ev[4] == ev[5] == 0
+    for r in results:
+        s, cams, pts = _alone(probs[r["tag"]], opts)
+        assert r["termination"] == s["termination"] and r["termination"] in (0, 1)
+        assert r["final_cost"] == s["final_cost"] and r["iterations"] == s["num_iterations"]
+        np.testing.assert_array_equal(r["cameras"], cams)
+        np.testing.assert_array_equal(r["points"], pts)
+
+
+def test_server_cancel():
+    """Cancelled requests leave with termination BA_USER_FAILURE (4): a resident one with the
+    parameters of its last accepted step (their cost is the reported final cost), a queued one with
+    the submitted parameters; the other requests are unaffected."""
+    probs = [window(640 + i, 6 + 2 * i, 300 + 100 * i) for i in range(5)]
+    opts = S.default_options(precision=1, max_num_iterations=10, **FIXED)
+    srv = S.Server(S.server_config(arena_bytes=1 << 30, max_windows=2, iterations_per_step=1, options=opts))
+    for i, p in enumerate(probs):
+        srv.submit(p, i)
+    _step_until_resident(srv, 2)  # 0 and 1
+    srv.step()
+    results = srv.poll()
+    assert not results
+    srv.cancel(1)    # resident
+    srv.cancel(3)    # queued
+    srv.cancel(777)  # unknown: ignored
+    results += srv.drain()
+    res = {r["tag"]: r for r in results}
+    assert sorted(res) == [0, 1, 2, 3, 4]
+    r1, r3 = res[1], res[3]
+    assert r1["termination"] == 4 and r3["termination"] == 4
+    assert r1["iterations"] >= 2 and r1["final_cost"] <= r1["initial_cost"]
+    q = S.BALProblem.from_arrays(probs[1].camera_index(), probs[1].point_index(), probs[1].observations(),
+                                 r1["cameras"].reshape(-1, 9), r1["points"].reshape(-1, 3))
+    assert abs(S.eval_cost(q) - r1["final_cost"]) <= 1e-11 * r1["final_cost"]
+    assert r3["iterations"] == 0 and np.isnan(r3["final_cost"])
+    np.testing.assert_array_equal(r3["cameras"], probs[3].mutable_cameras().reshape(-1))
+    np.testing.assert_array_equal(r3["points"], probs[3].mutable_points().reshape(-1))
+    for t in (0, 2, 4):
+        s, cams, pts = _alone(probs[t], opts)
+        assert res[t]["termination"] in (0, 1) and res[t]["final_cost"] == s["final_cost"]
+        np.testing.assert_array_equal(res[t]["cameras"], cams)
+        np.testing.assert_array_equal(res[t]["points"], pts)
 
 
 def test_simulate_poisson():

@@ -4,6 +4,9 @@ Bars: FP64 per-iteration cost within 1e-10 relative of the oracle on a fixed sch
 termination tolerances 0) and final cost within 1e-9; mixed precision final cost within 1e-5 of the
 FP64 oracle; bitwise-identical results run to run.
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -69,13 +72,24 @@ def test_termination_reason_and_trace_times():
     np.testing.assert_allclose(np.cumsum(dt), t, rtol=1e-12, atol=1e-15)
 
 
+def _ladybug_spread(case):
+    """The final-cost bar of a ladybug49 pipeline: how far the oracle's own final cost moves when
+    one input parameter is moved by one ulp (the largest of eight trials, committed in
+    tests/golden/ladybug_sensitivity.json by make_ladybug_sensitivity.py; in those trials the
+    oracle's trajectory leaves its unperturbed one at rows 10-15, as this engine's does)."""
+    with open(os.path.join(os.path.dirname(__file__), "golden", "ladybug_sensitivity.json")) as fh:
+        return json.load(fh)[case]["max_final_rel_diff"]
+
+
 def test_ladybug49_reference_driver_matches_oracle(oracle_mod, ladybug_path):
     """The reference driver's pipeline: load, Normalize, Perturb(0.1, 0.5, 0.5), SolveBA with default
     (Ceres) options: 50 iterations max, function/gradient/parameter tolerances, dense Schur.
 
     This problem is ill-conditioned at the radii LM reaches (~1e10, i.e. Gauss-Newton on weakly
     constrained points), so last-digit differences in the dense factorization order grow from
-    ~1e-12 (first iterations) to ~1e-5 after ~35 iterations; both runs end on the same plateau."""
+    ~1e-12 (first iterations) to ~1e-5 after ~35 iterations; both runs end on the same plateau.
+    The oracle moves as far from itself under a one-ulp change of one input (up to 6.8e-5), and that
+    spread is the final-cost bar."""
     p = S.BALProblem(ladybug_path)
     p.Normalize()
     oracle_mod.lib().oracle_srand(1)
@@ -89,7 +103,7 @@ def test_ladybug49_reference_driver_matches_oracle(oracle_mod, ladybug_path):
     # which stopping rule fires first (function tolerance or the 50-iteration cap) depends on where
     # the diverged trajectories are when the cost plateaus, so only its kind is checked
     assert out["termination"] in (0, 1) and ref["termination"] in (0, 1)
-    assert abs(out["final_cost"] - ref["final_cost"]) / ref["final_cost"] < 1e-4
+    assert abs(out["final_cost"] - ref["final_cost"]) / ref["final_cost"] <= _ladybug_spread("driver")
     assert out["final_cost"] < 0.01 * out["initial_cost"]
 
 
@@ -108,7 +122,7 @@ def test_ladybug49_simple_adjuster_matches_oracle(oracle_mod, ladybug_path):
     np.testing.assert_array_equal(s_gpu[:n], s_ref[:n])
     assert (np.abs(c_gpu[:n] - c_ref[:n]) / c_ref[:n]).max() < 1e-9
     assert out["termination"] in (0, 1) and ref["termination"] in (0, 1)
-    assert abs(out["final_cost"] - ref["final_cost"]) / ref["final_cost"] < 1e-4
+    assert abs(out["final_cost"] - ref["final_cost"]) / ref["final_cost"] <= _ladybug_spread("simple")
     assert out["final_cost"] < out["initial_cost"]
 
 

@@ -202,6 +202,7 @@ def test_invalid_arguments_rejected_before_device_use(ladybug_path):
     with pytest.raises(S.BAError) as e:
         S.BALProblem("/nonexistent/problem.txt")
     assert e.value.status == 7
+    assert C.lib().ba_server_cancel(None, 1) == 1 and C.lib().ba_server_evict(None, 1) == 1
 
 
 def test_no_device_is_an_error_not_a_fallback():

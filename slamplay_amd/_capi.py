@@ -58,7 +58,7 @@ class Summary(ctypes.Structure):
 
 class BatchSchedule(ctypes.Structure):
     _fields_ = [("schur_threads", i32), ("solve_threads", i32), ("blocks_per_wave", i32), ("lds_cameras", i32),
-                ("candidates", i32), ("iteration_ms", f64), ("default_iteration_ms", f64)]
+                ("candidates", i32), ("iteration_ms", f64), ("default_iteration_ms", f64), ("lanes", i32)]
 
 
 class KernelTime(ctypes.Structure):

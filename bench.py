@@ -165,7 +165,7 @@ def bench_batch(S, seed, steps, warmup):
                     "windows_per_s": len(wins) / (float(np.median(makespan)) / 1e3),
                     "threads": info["threads"], "device_bytes": info["device_bytes"], "schedule": sched}
         log("bench: batch %s: makespan %.2f ms, window p50 %.2f ms p99 %.2f ms; %d lanes, schedule %d/%d/%d/%d, "
-            "one iteration %.3f ms (default schedule %.3f ms, %d timed; lane sums)" %
+            "one iteration %.3f ms (default schedule %.3f ms, %d timed)" %
             (tag, out[tag]["batch_ms_median"], out[tag]["window_latency_p50_ms"], out[tag]["window_latency_p99_ms"],
              sched["lanes"], sched["schur_threads"], sched["solve_threads"], sched["blocks_per_wave"],
              sched["lds_cameras"], sched["iteration_ms"], sched["default_iteration_ms"], sched["candidates"]))

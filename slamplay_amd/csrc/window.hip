@@ -152,7 +152,7 @@ int build_window_image(const ba_problem_view* v, const ba_options& opt, WindowIm
     D.o_qp = take(32 * (int64_t)P);
     D.o_yp = take(24 * (int64_t)P);
     D.o_pv = take(96 * (int64_t)slots);
-    D.o_js = take(kWinJs * sizeof(double) * (int64_t)No);
+    D.o_js = take(kWinJs * sizeof(Real) * (int64_t)No);
     D.o_S = take(sizeof(double) * (int64_t)n * n);  // zero: blocks without common points are never written
     D.o_L = take(sizeof(double) * (int64_t)n * (n + 1) / 2);  // the factor when it is not kept in LDS
     img->total = off;
@@ -431,10 +431,10 @@ struct WindowEngine {
         if (n_ptask) hipLaunchKernelGGL(k_wlin_pt, dim3(n_ptask), dim3(64), 0, s, dd, arena.p, (const int2*)ptask.p,
                                         n_ptask, o);
         hipLaunchKernelGGL((k_wdamp<256>), dim3(n_wtask), dim3(256), 0, s, dd, arena.p, (const int*)wtask.p, n_wtask, o);
-        if (n_ctask) hipLaunchKernelGGL((k_wrec_cam<double, kWinCamThreads, M>), dim3(n_ctask), dim3(kWinCamThreads), 0, s,
+        if (n_ctask) hipLaunchKernelGGL((k_wrec_cam<Real, kWinCamThreads, M>), dim3(n_ctask), dim3(kWinCamThreads), 0, s,
                                         dd, arena.p, (const int2*)ctask.p, n_ctask, o);
         if (n_stask)
-            hipLaunchKernelGGL((k_wschur<double, NT>), dim3(n_stask), dim3(NT), (size_t)schur_lds<double>(NT), s, dd, arena.p,
+            hipLaunchKernelGGL((k_wschur<Real, NT>), dim3(n_stask), dim3(NT), (size_t)schur_lds<Real>(NT), s, dd, arena.p,
                                (const int4*)stask.p, n_stask);
         hipLaunchKernelGGL((k_wsolve<NF>), dim3(n_wtask), dim3(NF), solve_lds(), s, dd, arena.p, (const int*)wtask.p,
                            n_wtask, lds_n());
@@ -451,9 +451,9 @@ struct WindowEngine {
 
     int set_lds_limits() {
         if (threads == 512)
-            BA_TRY(allow_dynamic_lds((const void*)k_wschur<double, 512>, (int)schur_lds<double>(512)));
+            BA_TRY(allow_dynamic_lds((const void*)k_wschur<Real, 512>, (int)schur_lds<Real>(512)));
         else
-            BA_TRY(allow_dynamic_lds((const void*)k_wschur<double, 256>, (int)schur_lds<double>(256)));
+            BA_TRY(allow_dynamic_lds((const void*)k_wschur<Real, 256>, (int)schur_lds<Real>(256)));
         if (solve_threads == 512)
             BA_TRY(allow_dynamic_lds((const void*)k_wsolve<512>, (int)solve_lds()));
         else
@@ -664,44 +664,36 @@ struct Batch final : BatchBase {
         return BA_OK;
     }
 
-    // Times schedules on the batch itself and keeps the fastest for each lane; the windows' results
-    // do not depend on the choice.  Two stages (the two kernels they tune run one after the other):
-    // the factor's workgroup size and LDS limit with the default Schur-product launch, then the
-    // Schur-product workgroup size and blocks per wave with the chosen factor launch.  Each candidate
-    // is timed twice over two LM iterations from the initial state (best of two).  The reported
-    // iteration times are the lanes' sums.
+    // Times schedules on the batch itself (all lanes running together, as run() runs them) and keeps
+    // the fastest; the windows' results do not depend on the choice.  Two stages (the two kernels
+    // they tune run one after the other): the factor's workgroup size and LDS limit with the default
+    // Schur-product launch, then the Schur-product workgroup size and blocks per wave with the
+    // chosen factor launch.  Each candidate is timed twice over two LM iterations from the initial
+    // state (best of two).
     int autotune() {
         sched = ba_batch_schedule{};
         apply(256, 256, 2, kWinMaxK);
-        for (size_t l = 0; l < lanes.size(); ++l) BA_TRY(autotune_lane(l));
-        sched.lanes = (int)lanes.size();
-        return BA_OK;
-    }
-
-    int autotune_lane(size_t l) {
-        WindowEngine<Real>& eng = *lanes[l];
-        const std::vector<int>& order = lane_order[l];
-        if (order.empty()) return BA_OK;
+        if (small.empty()) return BA_OK;
         double best = 1e30;
-        float warm = 0.f;
-        BA_TRY(eng.time_iteration(order, &warm));  // first launches of the kernels, not timed
+        double warm = 0.0;
+        BA_TRY(time_iteration(&warm));  // first launches of the kernels, not timed
         auto time_it = [&](double* ms) -> int {
             *ms = 1e30;
             for (int rep = 0; rep < 2; ++rep) {
-                float m = 0.f;
-                BA_TRY(eng.time_iteration(order, &m));
-                *ms = std::min(*ms, (double)m / 2.0);
+                double m = 0.0;
+                BA_TRY(time_iteration(&m));
+                *ms = std::min(*ms, m / 2.0);
             }
-            if (l == 0) sched.candidates++;
+            sched.candidates++;
             return BA_OK;
         };
         int best_tf = 256, best_lds = kWinMaxK;
         for (int tf : {256, 512})
             for (int lc : {kWinMaxK, 14, 10}) {
-                apply_lane(eng, 256, tf, 2, lc);
+                apply(256, tf, 2, lc);
                 double ms;
                 BA_TRY(time_it(&ms));
-                if (tf == 256 && lc == kWinMaxK) sched.default_iteration_ms += ms;
+                if (tf == 256 && lc == kWinMaxK) sched.default_iteration_ms = ms;
                 if (ms < best) {
                     best = ms;
                     best_tf = tf;
@@ -710,10 +702,10 @@ struct Batch final : BatchBase {
             }
         int best_ts = 256, best_bpw = 2;
         for (int ts : {256, 512}) {
-            if (schur_lds<double>(ts) + 64 > lds_per_block()) continue;
+            if (schur_lds<Real>(ts) + 64 > lds_per_block()) continue;
             for (int bpw : {1, 2, 4}) {
                 if (ts == 256 && bpw == 2) continue;  // timed in the first stage
-                apply_lane(eng, ts, best_tf, bpw, best_lds);
+                apply(ts, best_tf, bpw, best_lds);
                 double ms;
                 BA_TRY(time_it(&ms));
                 if (ms < best) {
@@ -723,14 +715,31 @@ struct Batch final : BatchBase {
                 }
             }
         }
-        apply_lane(eng, best_ts, best_tf, best_bpw, best_lds);
-        sched.iteration_ms += best;
-        if (l == 0) {
-            sched.schur_threads = best_ts;
-            sched.solve_threads = best_tf;
-            sched.blocks_per_wave = best_bpw;
-            sched.lds_cameras = eng.lds_n() / 9;
+        const int cand = sched.candidates;
+        const double dflt = sched.default_iteration_ms;
+        apply(best_ts, best_tf, best_bpw, best_lds);
+        sched.candidates = cand;
+        sched.default_iteration_ms = dflt;
+        sched.iteration_ms = best;
+        return BA_OK;
+    }
+
+    // Milliseconds of two LM iterations of every lane, launched together (host clock around the
+    // launches and the waits); the windows are reset before and after.
+    int time_iteration(double* ms) {
+        const int nl = (int)lanes.size();
+        for (int l = 0; l < nl; ++l) {
+            BA_TRY(lanes[l]->reset(lane_order[l]));
+            BA_TRY(lanes[l]->run(lane_order[l], 1));  // iteration 0 + first step (warm)
+            BA_TRY(lanes[l]->reset(lane_order[l]));
         }
+        for (int l = 0; l < nl; ++l) BA_TRY(lanes[l]->sync("autotune"));
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int l = 0; l < nl; ++l) BA_TRY(lanes[l]->run(lane_order[l], 2));
+        for (int l = 0; l < nl; ++l) BA_TRY(lanes[l]->sync("autotune"));
+        *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        for (int l = 0; l < nl; ++l) BA_TRY(lanes[l]->reset(lane_order[l]));
+        for (int l = 0; l < nl; ++l) BA_TRY(lanes[l]->sync("autotune"));
         return BA_OK;
     }
 
@@ -758,7 +767,7 @@ struct Batch final : BatchBase {
         if (bpw != 1 && bpw != 2 && bpw != 4 && bpw != 8)
             return fail(BA_ERR_INVALID_ARGUMENT, "blocks_per_wave must be 1, 2, 4 or 8 (got %d)", bpw);
         if (sch->lds_cameras < 0) return fail(BA_ERR_INVALID_ARGUMENT, "lds_cameras < 0");
-        if (schur_lds<double>(ts) + 64 > lds_per_block())
+        if (schur_lds<Real>(ts) + 64 > lds_per_block())
             return fail(BA_ERR_CAPACITY, "the Schur staging of %d threads does not fit the LDS", ts);
         sched = ba_batch_schedule{};
         apply(ts, tf, bpw, sch->lds_cameras);
@@ -859,11 +868,14 @@ struct Batch final : BatchBase {
     }
 };
 
-// Windows run in FP64 whatever options.precision asks: a window's reduced system has at most 288
-// unknowns, where ba_solve factors in FP64 as well (kDenseF64MaxN), and the FP32 factor would need
-// CG against the FP64 system to reach the FP64 step (measured on the 512-window batch: 0.73 ms per
-// LM iteration for the FP32 factor + CG against 0.32 ms for the FP64 factor).  Windows above 32
-// cameras go to the single-problem solver with the caller's precision.
+// Windows run in FP64 whatever options.precision asks (Real, the type of the per-observation records
+// and of the Schur pair products, is double for both).  A window's reduced system has at most 288
+// unknowns, where ba_solve factors in FP64 as well (kDenseF64MaxN): an FP32 factor would need CG
+// against the FP64 system to reach the FP64 step (measured on the 512-window batch: 0.73 ms per LM
+// iteration for the FP32 factor + CG against 0.32 ms for the FP64 factor), and FP32 records with
+// the FP64 factor (Batch<float>) gave 22.0 ms against 23.5 ms for the 512-window batch but ended
+// up to 4.7e-5 from the FP64 final cost on its worst windows.  Windows above 32 cameras go to the
+// single-problem solver with the caller's precision.
 int make_batch(const ba_problem_view* views, int count, const ba_options& o, std::unique_ptr<BatchBase>* out,
                const StereoInput* stereo = nullptr, bool tune = true) {
     auto b = std::make_unique<Batch<double>>();

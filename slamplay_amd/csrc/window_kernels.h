@@ -264,20 +264,15 @@ __device__ __forceinline__ void win_store_round(Real* st_i, Real* st_j, int m, i
 // 32-63 rows 5-8 of pair q = lane % 32 in registers; the 32 partial sums of a block are combined by
 // a butterfly in FP64.
 template <typename Real, typename F>
-__device__ __forceinline__ void win_schur(const int4* __restrict__ blk, const int2* __restrict__ pairs,
-                                          const Real* __restrict__ js, const double* __restrict__ dc2, F* S, int n,
-                                          int first, int stride, int count, int* counter, Real* stage) {
+__device__ __forceinline__ void win_schur_block(const int4 bk, const int2* __restrict__ pairs,
+                                                const Real* __restrict__ js, const double* __restrict__ dc2, F* S,
+                                                int n, Real* stage) {
     constexpr int PS = kWinPairsPerStage;
     const int lane = threadIdx.x & 63, q = lane & 31, h = lane >> 5;
     const int rb = 5 * h;
     Real* st_i = stage;
     Real* st_j = stage + PS * kWinStagePitch;
-    for (;;) {
-        int t = 0;
-        if (lane == 0) t = atomicAdd(counter, 1);
-        t = __shfl(t, 0, 64);
-        if (t >= count) break;
-        const int4 bk = blk[first + stride * t];
+    {
         Real acc[5][9];
 #pragma unroll
         for (int r = 0; r < 5; ++r)
@@ -374,6 +369,21 @@ __device__ __forceinline__ void win_schur(const int4* __restrict__ blk, const in
                 }
             }
         }
+    }
+}
+
+// The blocks first + stride t, t < count, of one window, taken one at a time through *counter.
+template <typename Real, typename F>
+__device__ __forceinline__ void win_schur(const int4* __restrict__ blk, const int2* __restrict__ pairs,
+                                          const Real* __restrict__ js, const double* __restrict__ dc2, F* S, int n,
+                                          int first, int stride, int count, int* counter, Real* stage) {
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(counter, 1);
+        t = __shfl(t, 0, 64);
+        if (t >= count) break;
+        win_schur_block<Real, F>(blk[first + stride * t], pairs, js, dc2, S, n, stage);
     }
 }
 
@@ -744,6 +754,7 @@ __global__ void __launch_bounds__(NT) k_wrec_cam(const WinDesc* __restrict__ des
     constexpr int NW = NT / 64;
     __shared__ double lds[9 * NW];
     __shared__ double dacc[NW][45];  // per wave: upper triangle of sum J~_c'(I - G)J~_c
+    __shared__ double red[NW][16][45];  // per wave: one round's terms summed over lane quads
     __shared__ CameraPrep<double> Psh;
     __shared__ StereoRig Gsh;
     const int t = blockIdx.x;
@@ -810,14 +821,26 @@ __global__ void __launch_bounds__(NT) k_wrec_cam(const WinDesc* __restrict__ des
             t0[i] = g00 * Jd[i] + g01 * Jd[9 + i];
             t1[i] = g01 * Jd[i] + g11 * Jd[9 + i];
         }
+        // the 45 terms summed over lane quads by two DPP steps, the 16 quad sums of each term then
+        // added in order by lane e (an LDS transpose instead of 45 full-wave butterflies)
         int e = 0;
 #pragma unroll
         for (int r = 0; r < 9; ++r)
 #pragma unroll
             for (int cc = r; cc < 9; ++cc, ++e) {
-                const double v = wave_sum(Jd[r] * t0[cc] + Jd[9 + r] * t1[cc]);
-                if (lane == 0) dacc[wave][e] += v;
+                double v = Jd[r] * t0[cc] + Jd[9 + r] * t1[cc];
+                v += dpp_lane<kDppXor1>(v);
+                v += dpp_lane<kDppXor2>(v);
+                if ((lane & 3) == 0) red[wave][lane >> 2][e] = v;
             }
+        wave_sync_lds();
+        if (lane < 45) {
+            double v = red[wave][0][lane];
+#pragma unroll
+            for (int m = 1; m < 16; ++m) v += red[wave][m][lane];
+            dacc[wave][lane] += v;
+        }
+        wave_sync_lds();
     }
     wg_sum_fixed<9, NT>(bacc, lds);
     if (threadIdx.x == 0)
@@ -859,7 +882,7 @@ __global__ void __launch_bounds__(NT) k_wschur(const WinDesc* __restrict__ descs
     if (threadIdx.x == 0) counter = 0;
     __syncthreads();
     const int wave = threadIdx.x >> 6;
-    win_schur<Real, Real>(w.blk, w.pairs, at<Real>(arena, D.o_js), w.dc2, at<Real>(arena, D.o_S), 9 * w.K, tk.y, tk.z,
+    win_schur<Real, double>(w.blk, w.pairs, at<Real>(arena, D.o_js), w.dc2, at<double>(arena, D.o_S), 9 * w.K, tk.y, tk.z,
                           tk.w, &counter, reinterpret_cast<Real*>(smem + (size_t)wave * win_stage_bytes<Real>()));
 }
 

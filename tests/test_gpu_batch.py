@@ -110,7 +110,8 @@ def test_alone_and_in_mix_bitwise_identical(precision):
                 except S.BAError as e:
                     assert e.status == 6  # BA_ERR_CAPACITY
                     continue
-                assert mix.schedule()["blocks_per_wave"] == bpw and mix.schedule()["lds_cameras"] == lds
+                sc = mix.schedule()
+                assert sc["blocks_per_wave"] == bpw and sc["lds_cameras"] == lds
                 sizes += 1
                 mix.reset()
                 assert _same(mix.run()[3], s_alone)

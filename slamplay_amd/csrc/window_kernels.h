@@ -301,15 +301,17 @@ __device__ __forceinline__ void win_schur_block(const int4 bk, const int2* __res
         win_store_round<Real, PS>(st_i, st_j, 2, b2);
         win_store_round<Real, PS>(st_i, st_j, 3, b3);
         const int pi = __shfl(pr.x, q, 64), pj = __shfl(pr.y, q, 64);
+        // the pair indices two stages ahead are requested before the next stage's records and
+        // taken over after this stage's products, so that neither waits for the records (a wave's
+        // loads complete in order)
+        int2 pr_far = make_int2(0, 0);
         if (stg + 1 < nst) {
+            if (stg + 2 < nst && lane < min(PS, bk.w - base - 2 * PS)) pr_far = pairs[base + 2 * PS + lane];
             m_ld = min(PS, bk.w - base - PS);
             b0 = win_load_round<Real, PS>(js, pr_next, m_ld, 0);
             b1 = win_load_round<Real, PS>(js, pr_next, m_ld, 1);
             b2 = win_load_round<Real, PS>(js, pr_next, m_ld, 2);
             b3 = win_load_round<Real, PS>(js, pr_next, m_ld, 3);
-            pr = pr_next;
-            pr_next = make_int2(0, 0);
-            if (stg + 2 < nst && lane < min(PS, bk.w - base - 2 * PS)) pr_next = pairs[base + 2 * PS + lane];
         }
         wave_sync_lds();
         if (q < m) {
@@ -347,6 +349,8 @@ __device__ __forceinline__ void win_schur_block(const int4 bk, const int2* __res
             }
         }
         wave_sync_lds();
+        pr = pr_next;
+        pr_next = pr_far;
     }
     const int ca = bk.x, cb = bk.y;
 #pragma unroll
@@ -383,78 +387,91 @@ __device__ __forceinline__ void win_schur(const int4* __restrict__ blk, const in
 __device__ __forceinline__ int tri(int i) { return i * (i + 1) / 2; }
 
 // In-place blocked right-looking Cholesky of the packed n x n matrix L (LDS, or the window's global
-// buffer); dinv receives 1 / L_jj.  Panels of kWinNb columns: thread t keeps the panel part of
-// rows jb + t + NT s in registers; per column every thread takes the pivot from LDS, scales its
-// rows, and the owners of the panel's top rows publish the scaled entries l_cj and the next pivot
-// (double-buffered), so a column costs one barrier.  The trailing lower triangle is then updated
-// with the panel by 16 x 16 tiles on the matrix cores (one tile per wave at a time, K = 16 in four
-// steps; entries past n or above the diagonal are neither read nor written).  *bad is set when a
-// pivot is not positive (the factor is then meaningless and the caller rejects the step).  Every
-// entry's arithmetic is fixed by the algorithm, not by the workgroup size.
+// buffer); dinv receives 1 / L_jj.  Panels of kWinNb columns: wave 0 factors the panel's diagonal
+// block in registers (lane i holds row i; each column's entries move between lanes through LDS, no
+// barrier), then every thread solves its rows of the panel below the block against it
+// (x L11' = a, the entries of L11 read from LDS by all threads at once), so a panel costs two
+// barriers.  The trailing lower triangle is then updated with the panel by 16 x 16 tiles on the
+// matrix cores (one tile per wave at a time, K = 16 in four steps; entries past n or above the
+// diagonal are neither read nor written).  *bad is set when a pivot is not positive (the factor is
+// then meaningless and the caller rejects the step).  Every entry receives the same sequence of
+// operations as in a column-by-column right-looking factorization (its updates by the earlier
+// columns in order, then the scaling by 1 / sqrt(pivot)), fixed by the algorithm, not by the
+// workgroup size.
 constexpr int kWinNb = 16;
 
 template <typename F, int NT>
-__device__ __forceinline__ void win_panel(F* L, int n, int jb, int je, F* colv, F* dinv, int* bad) {
-    constexpr int NB = kWinNb, RS = (kWinMaxN + NT - 1) / NT;
-    const int tid = threadIdx.x;
-    F* cbuf = colv;           // 2 x NB published column entries
-    F* pbuf = colv + 2 * NB;  // 2 pivots
-    F a[RS][NB];
+__device__ __forceinline__ void win_panel(F* L, int n, int jb, int je, F* dinv, F* xch, int* bad) {
+    constexpr int NB = kWinNb, RS = (kWinMaxN - NB + NT - 1) / NT;
+    const int tid = threadIdx.x, nb = je - jb;
+    if (tid < 64) {
+        // the diagonal block: lane i < nb holds row jb + i; the pivot's reciprocal square root by
+        // the hardware estimate and Newton steps, each column's entries to the other lanes through
+        // xch (a wave's LDS operations complete in order)
+        const int i = tid;
+        F a[NB];
 #pragma unroll
-    for (int s = 0; s < RS; ++s) {
-        const int row = jb + tid + NT * s;
+        for (int c = 0; c < NB; ++c) a[c] = (i < nb && c <= i) ? L[tri(jb + i) + jb + c] : (F)0;
 #pragma unroll
-        for (int cc = 0; cc < NB; ++cc)
-            a[s][cc] = (row < n && jb + cc < je && jb + cc <= row) ? L[tri(row) + jb + cc] : (F)0;
+        for (int j = 0; j < NB; ++j) {
+            if (j < nb) {  // uniform
+                F djj = read_lane(a[j], j);
+                const bool nonpos = !(djj > (F)0);
+                if (nonpos) djj = (F)1;
+                const F inv = inv_sqrt(djj), d = djj * inv;
+                a[j] = i > j ? a[j] * inv : (i == j ? d : a[j]);
+                if (i == 0) {
+                    dinv[jb + j] = inv;
+                    if (nonpos) *bad = 1;
+                }
+                if (i < NB) xch[i] = a[j];
+                wave_sync_lds();
+#pragma unroll
+                for (int c = j + 1; c < NB; ++c) {
+                    if (c < nb) {  // uniform
+                        const F lcj = xch[c];
+                        if (i >= c && i < nb) a[c] = fma(-a[j], lcj, a[c]);
+                    }
+                }
+                wave_sync_lds();
+            }
+        }
+        if (i < nb)
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                if (c <= i) L[tri(jb + i) + jb + c] = a[c];
     }
-    if (tid == 0) pbuf[0] = a[0][0];
     __syncthreads();
-#pragma unroll
-    for (int jj = 0; jj < NB; ++jj) {
-        const int j = jb + jj;
-        if (j >= je) break;  // uniform; the columns past the last panel's end are not visited
-        const int buf = jj & 1;
-        F djj = pbuf[buf];
-        const bool nonpos = !(djj > (F)0);
-        if (nonpos) djj = (F)1;
-        const F d = sqrt(djj), inv = (F)1 / d;
-#pragma unroll
-        for (int s = 0; s < RS; ++s) {
-            const int row = jb + tid + NT * s;
-            a[s][jj] = row > j ? a[s][jj] * inv : (row == j ? d : a[s][jj]);
-        }
-        if (tid > jj && tid < NB) cbuf[buf * NB + tid] = a[0][jj];  // l_(jb + tid) j
-        if (jj + 1 < NB && tid == jj + 1) pbuf[buf ^ 1] = fma(-a[0][jj], a[0][jj], a[0][jj + 1]);
-        if (tid == 0) {
-            dinv[j] = inv;
-            if (nonpos) *bad = 1;
-        }
-        __syncthreads();
-        // (a predicate, not a break: the loop must unroll so that a[][cc] stays in registers)
-#pragma unroll
-        for (int cc = jj + 1; cc < NB; ++cc) {
-            const F lcj = jb + cc < je ? cbuf[buf * NB + cc] : (F)0;
-#pragma unroll
-            for (int s = 0; s < RS; ++s)
-                if (jb + tid + NT * s >= jb + cc) a[s][cc] = fma(-a[s][jj], lcj, a[s][cc]);
-        }
-    }
+    // the panel's rows below the block: x_j = (a_j - sum_k<j x_k l_jk) / l_jj, k in order
 #pragma unroll
     for (int s = 0; s < RS; ++s) {
-        const int row = jb + tid + NT * s;
+        const int row = je + tid + NT * s;
+        if (row < n) {
+            F a[NB];
 #pragma unroll
-        for (int cc = 0; cc < NB; ++cc)
-            if (row < n && jb + cc < je && jb + cc <= row) L[tri(row) + jb + cc] = a[s][cc];
+            for (int c = 0; c < NB; ++c) a[c] = c < nb ? L[tri(row) + jb + c] : (F)0;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                if (j < nb) {
+#pragma unroll
+                    for (int k = 0; k < j; ++k) a[j] = fma(-a[k], L[tri(jb + j) + jb + k], a[j]);
+                    a[j] = a[j] * dinv[jb + j];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                if (c < nb) L[tri(row) + jb + c] = a[c];
+        }
     }
 }
 
 template <typename F, int NT>
-__device__ __forceinline__ void win_cholesky(F* L, int n, F* colv, F* dinv, int* bad) {
+__device__ __forceinline__ void win_cholesky(F* L, int n, F* dinv, F* xch, int* bad) {
     constexpr int NW = NT / 64, NB = kWinNb;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int jb = 0; jb < n; jb += NB) {
         const int je = min(jb + NB, n);
-        win_panel<F, NT>(L, n, jb, je, colv, dinv, bad);
+        win_panel<F, NT>(L, n, jb, je, dinv, xch, bad);
         __syncthreads();
         if (je >= n) break;
         // trailing update: L[r][c] -= sum_{k in [jb, je)} L[r][k] L[c][k] for je <= c <= r < n
@@ -502,7 +519,7 @@ __device__ __forceinline__ void win_chol_solve(const F* L, const F* dinv, int n,
         if (wave == 0) {
             double v = lane < m ? x[kb + lane] : 0.0;
             for (int c = 0; c < m; ++c) {
-                const double zc = __shfl(v, c, 64) * (double)dinv[kb + c];
+                const double zc = read_lane(v, c) * (double)dinv[kb + c];
                 if (lane == c) v = zc;
                 if (lane > c && lane < m) v -= (double)L[tri(kb + lane) + kb + c] * zc;
             }
@@ -524,7 +541,7 @@ __device__ __forceinline__ void win_chol_solve(const F* L, const F* dinv, int n,
         if (wave == 0) {
             double v = lane < m ? x[kb + lane] : 0.0;
             for (int c = m - 1; c >= 0; --c) {
-                const double yc = __shfl(v, c, 64) * (double)dinv[kb + c];
+                const double yc = read_lane(v, c) * (double)dinv[kb + c];
                 if (lane == c) v = yc;
                 if (lane < c) v -= (double)L[tri(kb + c) + kb + lane] * yc;
             }
@@ -885,7 +902,7 @@ template <int NT>
 __global__ void __launch_bounds__(NT) k_wsolve(const WinDesc* __restrict__ descs, char* __restrict__ arena,
                                                const int* __restrict__ wtask, int ntask, int lds_n) {
     extern __shared__ __align__(32) unsigned char smem[];
-    __shared__ double xv[kWinMaxN], colv[kWinMaxN], dinv[kWinMaxN];
+    __shared__ double xv[kWinMaxN], dinv[kWinMaxN], xch[kWinNb];
     __shared__ int bad;
     const int t = blockIdx.x;
     if (t >= ntask) return;
@@ -917,7 +934,7 @@ __global__ void __launch_bounds__(NT) k_wsolve(const WinDesc* __restrict__ descs
     }
     if (tid == 0) bad = 0;
     __syncthreads();
-    win_cholesky<double, NT>(S, n, colv, dinv, &bad);
+    win_cholesky<double, NT>(S, n, dinv, xch, &bad);
     if (bad) return;  // have_step stays 0: an invalid step
     for (int k = tid; k < n; k += NT) xv[k] = w.bc[k];
     __syncthreads();

@@ -512,16 +512,28 @@ __device__ __forceinline__ void win_cholesky(F* L, int n, F* dinv, F* xch, int* 
 // Expects x complete on entry (after a barrier) and leaves it complete.
 template <typename F, int NT>
 __device__ __forceinline__ void win_chol_solve(const F* L, const F* dinv, int n, double* x) {
+    constexpr int TB = kWinTb;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // forward: L z = x
-    for (int kb = 0; kb < n; kb += kWinTb) {
-        const int m = min(kWinTb, n - kb);
+    for (int kb = 0; kb < n; kb += TB) {
+        const int m = min(TB, n - kb);
         if (wave == 0) {
+            // the lane's row of the diagonal block and the block's 1 / l_cc first, so that the
+            // serial chain below waits on no memory
+            double lr[TB], dv[TB];
+#pragma unroll
+            for (int c = 0; c < TB; ++c) {
+                lr[c] = (lane < m && c < lane) ? (double)L[tri(kb + lane) + kb + c] : 0.0;
+                dv[c] = c < m ? (double)dinv[kb + c] : 0.0;
+            }
             double v = lane < m ? x[kb + lane] : 0.0;
-            for (int c = 0; c < m; ++c) {
-                const double zc = read_lane(v, c) * (double)dinv[kb + c];
-                if (lane == c) v = zc;
-                if (lane > c && lane < m) v -= (double)L[tri(kb + lane) + kb + c] * zc;
+#pragma unroll
+            for (int c = 0; c < TB; ++c) {
+                if (c < m) {
+                    const double zc = read_lane(v, c) * dv[c];
+                    if (lane == c) v = zc;
+                    if (lane > c) v -= lr[c] * zc;
+                }
             }
             if (lane < m) x[kb + lane] = v;
         }
@@ -529,28 +541,41 @@ __device__ __forceinline__ void win_chol_solve(const F* L, const F* dinv, int n,
         for (int i = kb + m + tid; i < n; i += NT) {
             const F* row = L + tri(i) + kb;
             double s = 0.0;
-            for (int c = 0; c < m; ++c) s += (double)row[c] * x[kb + c];
+#pragma unroll
+            for (int c = 0; c < TB; ++c)
+                if (c < m) s += (double)row[c] * x[kb + c];
             x[i] -= s;
         }
         __syncthreads();
     }
     // backward: L' y = z
-    const int last = ((n - 1) / kWinTb) * kWinTb;
-    for (int kb = last; kb >= 0; kb -= kWinTb) {
-        const int m = min(kWinTb, n - kb);
+    const int last = ((n - 1) / TB) * TB;
+    for (int kb = last; kb >= 0; kb -= TB) {
+        const int m = min(TB, n - kb);
         if (wave == 0) {
+            double lc[TB], dv[TB];
+#pragma unroll
+            for (int c = 0; c < TB; ++c) {
+                lc[c] = (c < m && lane < c) ? (double)L[tri(kb + c) + kb + lane] : 0.0;
+                dv[c] = c < m ? (double)dinv[kb + c] : 0.0;
+            }
             double v = lane < m ? x[kb + lane] : 0.0;
-            for (int c = m - 1; c >= 0; --c) {
-                const double yc = read_lane(v, c) * (double)dinv[kb + c];
-                if (lane == c) v = yc;
-                if (lane < c) v -= (double)L[tri(kb + c) + kb + lane] * yc;
+#pragma unroll
+            for (int c = TB - 1; c >= 0; --c) {
+                if (c < m) {
+                    const double yc = read_lane(v, c) * dv[c];
+                    if (lane == c) v = yc;
+                    if (lane < c) v -= lc[c] * yc;
+                }
             }
             if (lane < m) x[kb + lane] = v;
         }
         __syncthreads();
         for (int i = tid; i < kb; i += NT) {
             double s = 0.0;
-            for (int c = 0; c < m; ++c) s += (double)L[tri(kb + c) + i] * x[kb + c];
+#pragma unroll
+            for (int c = 0; c < TB; ++c)
+                if (c < m) s += (double)L[tri(kb + c) + i] * x[kb + c];
             x[i] -= s;
         }
         __syncthreads();

@@ -1043,8 +1043,9 @@ __global__ void __launch_bounds__(64) k_wback_pt(const WinDesc* __restrict__ des
 
 // Per (window, camera): candidate camera, model cost change -(J~s)'(r~ + J~s/2) (step s = -y) and
 // FP64 candidate cost of the camera's observations, |dx_c|^2, |x_c|^2.
+// (At least two workgroups per CU: unbounded, the compiler's register allocation leaves one.)
 template <int NT, int M>
-__global__ void __launch_bounds__(NT) k_wcand_cam(const WinDesc* __restrict__ descs, char* __restrict__ arena,
+__global__ void __launch_bounds__(NT, 2) k_wcand_cam(const WinDesc* __restrict__ descs, char* __restrict__ arena,
                                                   const int2* __restrict__ ctask, int ntask, Opts o) {
     __shared__ double lds[2 * (NT / 64)];
     __shared__ CameraPrep<double> Psh, Pxs;  // the candidate's prep, the current one's

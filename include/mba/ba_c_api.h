@@ -311,8 +311,8 @@ typedef struct ba_batch_schedule {
     double iteration_ms;         /* one LM iteration of the batch with this schedule (0: not timed) */
     double default_iteration_ms; /* the same with the default schedule 256 / 256 / 2 / 21 (0: not timed) */
     int32_t lanes;               /* output only: launch streams the windows are dealt to (largest work first, in
-                                    turn) and run concurrently -- 2 from 64 windows, else 1; the schedule applies to
-                                    every lane and the timings are of all lanes launched together */
+                                    turn) and run concurrently -- one per 32 windows, at most 4; the schedule
+                                    applies to every lane and the timings are of all lanes launched together */
 } ba_batch_schedule;
 /* The schedule in use. */
 int ba_batch_get_schedule(ba_batch* batch, ba_batch_schedule* out);

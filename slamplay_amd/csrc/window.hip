@@ -581,14 +581,16 @@ struct BatchBase {
     virtual int set_schedule(const ba_batch_schedule* sch) = 0;
 };
 
-// The windows of a batch are dealt to kBatchLanes lanes (largest work first, in turn), each an engine
-// with its own arena, stream, task lists and graph: the lanes' launch sequences run concurrently,
-// so one lane's low-occupancy phases (the per-window factorization, the small per-window kernels)
-// overlap the other's wide ones.  Measured on 512 windows: 26.8 ms with one lane, 23.9 ms with two
-// (three and four lanes: 27.5 ms; their concurrent Schur products contend for the L2).  A window's
-// result does not depend on its lane.
-constexpr int kBatchLanes = 2;
-constexpr int kBatchLaneMinWindows = 64;  // fewer windows: one lane
+// The windows of a batch are dealt to up to kBatchLanes lanes (largest work first, in turn), each an
+// engine with its own arena, stream, task lists and graph: the lanes' launch sequences run
+// concurrently, so one lane's low-occupancy phases (the per-window factorization, the small
+// per-window kernels) overlap the others' wide ones.  Measured on 512 windows (the 512-window
+// makespan, best of ten runs on one device): one lane 26.8 ms before the Schur-product and
+// factorization changes; with the present kernels 2 lanes 20.0-20.3 ms, 3 lanes 19.9, 4 lanes
+// 19.8, 6 lanes 22.1 and 8 lanes 21.6 (more lanes' Schur products contend for the L2).  A lane
+// gets at least kBatchLaneMinWindows windows.  A window's result does not depend on its lane.
+constexpr int kBatchLanes = 4;
+constexpr int kBatchLaneMinWindows = 32;
 
 template <typename Real>
 struct Batch final : BatchBase {
@@ -636,7 +638,7 @@ struct Batch final : BatchBase {
         large_sum.assign(large_solvers.size(), ba_summary{});
         std::vector<int> order = small;
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return imgs[a].cost > imgs[b].cost; });
-        const int nl = (int)order.size() >= kBatchLaneMinWindows ? kBatchLanes : 1;
+        const int nl = std::max(1, std::min(kBatchLanes, (int)order.size() / kBatchLaneMinWindows));
         lane_order.assign(nl, {});
         for (size_t i = 0; i < order.size(); ++i) {
             lane_order[i % nl].push_back(order[i]);
@@ -798,26 +800,27 @@ struct Batch final : BatchBase {
         BA_TRY(use_device(opt.device));
         const bool graphs = opt.launch_mode == 0;
         const int nl = (int)lanes.size();
-        int iters = opt.max_num_iterations + 1;  // the linearization, then one trial per launch sequence
+        // one LM trial per launch sequence, the first after the initial linearization: under the
+        // Ceres rules every window has decided its last trial after max_num_iterations sequences
+        const int iters = std::max(1, opt.max_num_iterations);
         for (int l = 0; l < nl; ++l) BA_TRY(lanes[l]->run(lane_order[l], iters, graphs));  // lanes run concurrently
         std::vector<std::vector<WinState>> st(nl);
-        if (opt.mode == BA_MODE_G2O) {
-            // a g2o iteration takes up to g2o_max_trials trials: continue four at a time while a
-            // window of the lane is still running
-            const int cap = opt.max_num_iterations * std::max(opt.g2o_max_trials, 1) + 1;
-            std::vector<int> lane_iters(nl, iters);
-            for (bool more_work = true; more_work;) {
-                more_work = false;
-                for (int l = 0; l < nl; ++l) {
-                    if (lane_iters[l] >= cap) continue;
-                    BA_TRY(lanes[l]->states(lane_order[l], &st[l]));
-                    if (std::all_of(st[l].begin(), st[l].end(), [](const WinState& x) { return x.done != 0; }))
-                        continue;
-                    const int more = std::min(4, cap - lane_iters[l]);
-                    BA_TRY(lanes[l]->run(lane_order[l], more, graphs, false));
-                    lane_iters[l] += more;
-                    more_work = true;
-                }
+        // a g2o iteration takes up to g2o_max_trials trials: continue four at a time while a window
+        // of the lane is still running (under the Ceres rules this finds every window done)
+        const int cap = (opt.mode == BA_MODE_G2O ? opt.max_num_iterations * std::max(opt.g2o_max_trials, 1)
+                                                 : opt.max_num_iterations) + 1;
+        std::vector<int> lane_iters(nl, iters);
+        for (bool more_work = true; more_work;) {
+            more_work = false;
+            for (int l = 0; l < nl; ++l) {
+                if (lane_iters[l] >= cap) continue;
+                BA_TRY(lanes[l]->states(lane_order[l], &st[l]));
+                if (std::all_of(st[l].begin(), st[l].end(), [](const WinState& x) { return x.done != 0; }))
+                    continue;
+                const int more = std::min(4, cap - lane_iters[l]);
+                BA_TRY(lanes[l]->run(lane_order[l], more, graphs, false));
+                lane_iters[l] += more;
+                more_work = true;
             }
         }
         for (size_t i = 0; i < large.size(); ++i) BA_TRY(large_solvers[i]->run(&large_sum[i]));

@@ -123,23 +123,23 @@ def test_alone_and_in_mix_bitwise_identical(precision):
     np.testing.assert_array_equal(other.download(0)[1], p_alone)
 
 
-def test_two_lanes_bitwise_identical():
-    """From 64 windows a batch deals its windows to two concurrently running lanes; every window's
-    result equals bit for bit that of a one-lane batch holding it."""
-    probs = [window(900 + i, 5 + i % 9, 200 + 13 * i) for i in range(64)]
+def test_lanes_bitwise_identical():
+    """A batch deals its windows to concurrently running lanes (one per 32 windows, at most four);
+    every window's result equals bit for bit that of a one-lane batch holding it."""
+    probs = [window(900 + i, 5 + i % 9, 200 + 13 * i) for i in range(128)]
     kw = dict(precision=0, max_num_iterations=6)
-    two = S.Batch(probs, **kw)
-    assert two.schedule()["lanes"] == 2
-    s2 = two.run()
-    for off in (0, 32):
-        one = S.Batch(probs[off:off + 32], **kw)
+    many = S.Batch(probs, **kw)
+    assert many.schedule()["lanes"] == 4
+    s2 = many.run()
+    for off in range(0, 128, 31):
+        one = S.Batch(probs[off:off + 31], **kw)
         assert one.schedule()["lanes"] == 1
         s1 = one.run()
-        for j in range(32):
+        for j in range(len(probs[off:off + 31])):
             w = off + j
             assert _same(s1[j], s2[w])
             c1, p1 = one.download(j)
-            c2, p2 = two.download(w)
+            c2, p2 = many.download(w)
             np.testing.assert_array_equal(c1, c2)
             np.testing.assert_array_equal(p1, p2)
     assert all(s["finish_ms"] > 0 for s in s2)

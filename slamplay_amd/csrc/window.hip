@@ -803,7 +803,10 @@ struct Batch final : BatchBase {
         // one LM trial per launch sequence, the first after the initial linearization: under the
         // Ceres rules every window has decided its last trial after max_num_iterations sequences
         const int iters = std::max(1, opt.max_num_iterations);
-        for (int l = 0; l < nl; ++l) BA_TRY(lanes[l]->run(lane_order[l], iters, graphs));  // lanes run concurrently
+        // the lanes run concurrently, each lane's launch sequences issued before the next lane's: the
+        // lanes start a few graph launches apart, so their phases stay offset (issuing the lanes'
+        // sequences in turn, which starts them together, measured 0.3 ms slower at 512 windows)
+        for (int l = 0; l < nl; ++l) BA_TRY(lanes[l]->run(lane_order[l], iters, graphs));
         std::vector<std::vector<WinState>> st(nl);
         // a g2o iteration takes up to g2o_max_trials trials: continue four at a time while a window
         // of the lane is still running (under the Ceres rules this finds every window done)

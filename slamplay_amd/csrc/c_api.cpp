@@ -9,6 +9,7 @@
 #include <string>
 #include <thread>
 #include <utility>
+#include <vector>
 
 #include "common.h"
 
@@ -30,13 +31,61 @@ int fail(int code, const char* fmt, ...) {
 
 void clear_error() { g_last_error.clear(); }
 
+namespace {
+std::mutex g_lane_mu;
+std::map<int, std::vector<hipStream_t>> g_lane_free;  // per device: pooled lane streams not in use
+std::map<int, bool> g_lane_made;
+}  // namespace
+
 int use_device(int device) {
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
     if (e != hipSuccess || n == 0) return fail(BA_ERR_NO_DEVICE, "no HIP device available");
     if (device < 0 || device >= n) return fail(BA_ERR_NO_DEVICE, "device %d out of range (%d devices)", device, n);
     BA_HIP(hipSetDevice(device));
+    std::lock_guard<std::mutex> lk(g_lane_mu);
+    if (!g_lane_made[device]) {
+        std::vector<hipStream_t>& pool = g_lane_free[device];
+        for (int i = 0; i < kLaneStreams; ++i) {
+            hipStream_t st = nullptr;
+            BA_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            pool.push_back(st);
+        }
+        g_lane_made[device] = true;
+    }
     return BA_OK;
+}
+
+int acquire_lane_streams(int device, int count, std::vector<hipStream_t>* out, std::vector<bool>* pooled) {
+    out->clear();
+    pooled->clear();
+    {
+        std::lock_guard<std::mutex> lk(g_lane_mu);
+        std::vector<hipStream_t>& pool = g_lane_free[device];
+        while ((int)out->size() < count && !pool.empty()) {
+            out->push_back(pool.front());
+            pool.erase(pool.begin());
+            pooled->push_back(true);
+        }
+    }
+    while ((int)out->size() < count) {
+        hipStream_t st = nullptr;
+        BA_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        out->push_back(st);
+        pooled->push_back(false);
+    }
+    return BA_OK;
+}
+
+void release_lane_streams(int device, const std::vector<hipStream_t>& streams, const std::vector<bool>& pooled) {
+    std::lock_guard<std::mutex> lk(g_lane_mu);
+    std::vector<hipStream_t>& pool = g_lane_free[device];
+    for (size_t i = 0; i < streams.size(); ++i) {
+        if (pooled[i])
+            pool.push_back(streams[i]);
+        else
+            (void)hipStreamDestroy(streams[i]);
+    }
 }
 
 int wait_stream(hipStream_t s, hipEvent_t ev, double timeout_s, const char* what) {

@@ -70,8 +70,21 @@ int validate_view(const ba_problem_view* v, bool need_params);
 
 inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
-// Sets the calling thread's device; returns BA_ERR_NO_DEVICE if it does not exist.
+// Sets the calling thread's device; returns BA_ERR_NO_DEVICE if it does not exist.  The library's
+// first use of a device also creates that device's lane streams (below).
 int use_device(int device);
+
+// Streams for the concurrently running lanes of a window batch.  The runtime maps streams onto a
+// few hardware queues (GPU_MAX_HW_QUEUES, 4 by default) in the order they are created, and two
+// lanes on one queue run one after the other: with one single-problem solver created and destroyed
+// before a 512-window batch, its four lanes' own new streams ran it in 24 ms against 20 ms.  So
+// kLaneStreams streams are created on each device's first use, before any other stream of the
+// library, and kept until the process ends; a batch takes them from this pool and gives them back
+// when it is destroyed (a batch finding the pool empty, because another holds its streams, creates
+// its own and destroys them).
+constexpr int kLaneStreams = 4;
+int acquire_lane_streams(int device, int count, std::vector<hipStream_t>* out, std::vector<bool>* pooled);
+void release_lane_streams(int device, const std::vector<hipStream_t>& streams, const std::vector<bool>& pooled);
 
 // Allows `kernel` at least `bytes` of dynamic LDS on the current device.  The attribute is process
 // wide: it is set under a lock and only when a larger value is asked for, so solvers and servers

@@ -285,22 +285,30 @@ struct WindowEngine {
     DevBuf<unsigned long long> dclock;  // device wall clock at the start of the last run
     double clock_khz = 100000.0;        // rate of the device wall clock
 
+    bool own_stream = true;  // false: the stream is a lane stream of the pool (not destroyed here)
+
     ~WindowEngine() {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (ev_wait) (void)hipEventDestroy(ev_wait);
-        if (s) (void)hipStreamDestroy(s);
+        if (s && own_stream) (void)hipStreamDestroy(s);
     }
 
     // Waits for the engine stream (bounded when wait_timeout_s > 0).
     int sync(const char* what) { return wait_stream(s, ev_wait, wait_timeout_s, what); }
 
-    int init(const ba_options& options, int64_t arena_bytes, int slots) {
+    // stream: a lane stream to launch on (kept by its owner), or NULL for a stream of the engine's own
+    int init(const ba_options& options, int64_t arena_bytes, int slots, hipStream_t stream = nullptr) {
         opt = options;
         o = solver_opts(options);
         BA_TRY(use_device(options.device));
-        BA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        if (stream) {
+            s = stream;
+            own_stream = false;
+        } else {
+            BA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        }
         BA_HIP(hipEventCreate(&ev0));
         BA_HIP(hipEventCreate(&ev1));
         BA_HIP(hipEventCreateWithFlags(&ev_wait, hipEventDisableTiming));
@@ -605,6 +613,14 @@ struct Batch final : BatchBase {
     std::vector<ba_summary> large_sum;
     ba_options opt{};
     ba_batch_schedule sched{};  // the schedule in use and the autotuner's timings
+    std::vector<hipStream_t> lane_streams;  // from the device's pool (acquire_lane_streams)
+    std::vector<bool> lane_pooled;
+
+    ~Batch() override {
+        for (auto& e : lanes) (void)hipStreamSynchronize(e->s);
+        lanes.clear();
+        if (!lane_streams.empty()) release_lane_streams(opt.device, lane_streams, lane_pooled);
+    }
 
     // stereo: per window the full_slam stereo model (rig, sides), or NULL for the Snavely model.
     // The g2o rules are available with the stereo model (full_slam's solver); Snavely windows run the
@@ -644,12 +660,13 @@ struct Batch final : BatchBase {
             lane_order[i % nl].push_back(order[i]);
             lane_of[order[i]] = (int)(i % nl);
         }
+        BA_TRY(acquire_lane_streams(o.device, nl, &lane_streams, &lane_pooled));
         for (int l = 0; l < nl; ++l) {
             auto eng = std::make_unique<WindowEngine<Real>>();
             eng->model = stereo ? kModelStereo : kModelSnavely;
             int64_t total = 0;
             for (int w : lane_order[l]) total += imgs[w].total;
-            BA_TRY(eng->init(o, total, count));
+            BA_TRY(eng->init(o, total, count, lane_streams[l]));
             int64_t base = 0;
             for (int w : lane_order[l]) {
                 BA_TRY(eng->place(imgs[w], w, base));

@@ -1276,12 +1276,16 @@ __global__ void __launch_bounds__(NT) k_jz_dir(Dev<Real> d) {
     if (c >= d.nc) return;
     const int j_end = d.cam_off[c + 1];
     int j = d.cam_off[c] + threadIdx.x;
+    // loads in flight ahead of the products: the point index two rounds ahead, the record and the
+    // point's z (a gather through that index) one round ahead
     R4<Real> rec{};
     double4 zin{};
+    int pt_next = 0;
     if (j < j_end) {
         rec = d.co_rec[j];
         zin = d.zs[d.co_pt[j]];
     }
+    if (j + NT < j_end) pt_next = d.co_pt[j + NT];
     if (threadIdx.x < 9) {
         const size_t i = 9 * (size_t)c + threadIdx.x;
         psh[threadIdx.x] = d.pp[i];
@@ -1308,7 +1312,8 @@ __global__ void __launch_bounds__(NT) k_jz_dir(Dev<Real> d) {
         const double4 z = zin;
         if (j + NT < j_end) {
             rec = d.co_rec[j + NT];
-            zin = d.zs[d.co_pt[j + NT]];
+            zin = d.zs[pt_next];
+            if (j + 2 * NT < j_end) pt_next = d.co_pt[j + 2 * NT];
         }
         Real dq[3];
         jz_difference<Real>(P, rsh, vt64, cur, z, dq);

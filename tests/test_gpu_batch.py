@@ -357,6 +357,35 @@ def test_server_cancel():
         np.testing.assert_array_equal(res[t]["points"], pts)
 
 
+def test_server_large_problems_on_workers():
+    """Requests above 32 cameras go to the server's two worker threads, each creating, running and
+    destroying a single-problem solver while the resident windows are stepped; their results equal
+    bit for bit those of a single-problem solve with the options the workers use (solver choice
+    automatic, stream launches), and the windows beside them are unaffected."""
+    small = [window(680 + i, 6 + 3 * i, 300 + 150 * i) for i in range(4)]
+    large = [window(690, 38, 800), window(691, 40, 900)]
+    opts = S.default_options(precision=0, max_num_iterations=6)
+    srv = S.Server(S.server_config(arena_bytes=1 << 30, max_windows=4, iterations_per_step=1, options=opts))
+    for i, p in enumerate(small + large):
+        srv.submit(p, i)
+    res = {r["tag"]: r for r in srv.drain()}
+    assert sorted(res) == list(range(6))
+    for i, p in enumerate(large):
+        r = res[4 + i]
+        q = S.BALProblem.from_arrays(p.camera_index(), p.point_index(), p.observations(), p.mutable_cameras(),
+                                     p.mutable_points())
+        out = S.SolveBA(q, precision=0, max_num_iterations=6, linear_solver=0, launch_mode=1)
+        assert r["termination"] == out["termination"] and r["iterations"] == out["num_iterations"]
+        assert r["final_cost"] == out["final_cost"]
+        np.testing.assert_array_equal(r["cameras"], q.mutable_cameras().reshape(-1))
+        np.testing.assert_array_equal(r["points"], q.mutable_points().reshape(-1))
+    for i, p in enumerate(small):
+        s, cams, pts = _alone(p, opts)
+        assert res[i]["final_cost"] == s["final_cost"]
+        np.testing.assert_array_equal(res[i]["cameras"], cams)
+        np.testing.assert_array_equal(res[i]["points"], pts)
+
+
 def test_simulate_poisson():
     pool = [window(700 + i, 5 + i % 12, 200 + 50 * i) for i in range(16)]
     res = S.simulate(pool, rate_hz=2000.0, count=64, seed=3, max_windows=64, precision=1, max_num_iterations=10)

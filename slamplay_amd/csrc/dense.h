@@ -5,8 +5,9 @@
 // with U*'s per-observation part folded into the (i, i) pairs: S_aa = D_a^2 + sum J~'(I - G)J~ - ...
 // is assembled without atomics: the host enumerates every ordered observation pair (i, j) of a
 // point with camera(i) <= camera(j) once per problem and groups the pairs by camera block (a, b);
-// one workgroup owns one 9x9 block and reduces its pair list in a fixed order.  A pair contributes
-// Jc~_i' (Hp_i Jp~_j') Jc~_j with Hp_i = Jp~_i Vinv_p, i.e. a 2x2 middle factor.
+// one wave owns one 9x9 block and reduces its pair list in a fixed order (pair_block.h, the kernel
+// the batched windows use too).  A pair contributes Jc~_i' (Hp_i Jp~_j') Jc~_j with
+// Hp_i = Jp~_i Vinv_p, i.e. a 2x2 middle factor.
 //
 // The factorization is a tiled right-looking Cholesky (32 x 32 tiles): per step, the diagonal
 // tile is factored by one wave and its triangular inverse kept, the panel below is multiplied by
@@ -23,15 +24,18 @@
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "lm_kernels.h"
 #include "mfma.h"
+#include "pair_block.h"
 #include "solver.h"
 
 namespace mba {
 
 constexpr int kTile = 32;
-constexpr int kJs = 36;  // per-observation stride of Jc~ (18), Jp~ (6), Hp (6), I - G (3)
+constexpr int kJs = 36;  // per-observation stride of Jc~ (18, by columns), Jp~ (6), Hp (6), I - G (3)
+static_assert(kJs == RecLayout<kRecDense>::stride, "the dense record is pair_block's kRecDense");
 
 template <typename Real, int M>
 __global__ void __launch_bounds__(kWG) k_dense_store(Dev<Real> d, Real* __restrict__ js) {
@@ -47,9 +51,9 @@ __global__ void __launch_bounds__(kWG) k_dense_store(Dev<Real> d, Real* __restri
         Real r[2], Jc[18], Jp[6];
         corrected_jacobian<Real, M>(d, P, j, p, r, Jc, Jp);
         Real* o = js + kJs * (size_t)j;
-        for (int i = 0; i < 9; ++i) {
-            o[i] = Jc[i] * sc[i];
-            o[9 + i] = Jc[9 + i] * sc[i];
+        for (int i = 0; i < 9; ++i) {  // J~_c by columns: the two residual rows of column i adjacent
+            o[2 * i] = Jc[i] * sc[i];
+            o[2 * i + 1] = Jc[9 + i] * sc[i];
         }
         for (int i = 0; i < 3; ++i) {
             const Real s = (Real)d.sp[3 * (size_t)p + i];
@@ -74,53 +78,20 @@ __global__ void k_fill_S(double* __restrict__ S, int n, int ld, const LMState* s
     S[i] = (r == c && r >= n) ? 1.0 : 0.0;
 }
 
+// One 9x9 block of the reduced camera system per workgroup of one wave (the blocks' pair lists, the
+// 36-value records of k_dense_store): pair_block in pair_block.h.
 template <typename Real>
-__global__ void __launch_bounds__(kWG) k_schur_pairs(Dev<Real> d, const Real* __restrict__ js,
+__global__ void __launch_bounds__(64) k_schur_blocks(Dev<Real> d, const Real* __restrict__ js,
                                                      const int* __restrict__ blk_a, const int* __restrict__ blk_b,
-                                                     const int* __restrict__ blk_off, const int* __restrict__ pair_i,
-                                                     const int* __restrict__ pair_j, double* __restrict__ S, int ld) {
+                                                     const int* __restrict__ blk_off, const int2* __restrict__ pairs,
+                                                     double* __restrict__ S, int ld, int nblocks) {
     LMState* st = d.st;
     if (st->done || st->step_invalid) return;
-    __shared__ Real A[32][18];
-    __shared__ Real T[32][18];
-    const int b = blockIdx.x, t = threadIdx.x;
-    const int ca = blk_a[b], cb = blk_b[b];
-    const int r = t / 9, cc = t % 9;
-    double acc = 0.0;
-    for (int base = blk_off[b]; base < blk_off[b + 1]; base += 32) {
-        const int n = min(32, blk_off[b + 1] - base);
-        if (t < n) {
-            const Real* Ji = js + kJs * (size_t)pair_i[base + t];
-            const Real* Jj = js + kJs * (size_t)pair_j[base + t];
-            const Real* Hp = Ji + 24;
-            const Real* Jp = Jj + 18;
-            Real G00 = Hp[0] * Jp[0] + Hp[1] * Jp[1] + Hp[2] * Jp[2];
-            Real G01 = Hp[0] * Jp[3] + Hp[1] * Jp[4] + Hp[2] * Jp[5];
-            Real G10 = Hp[3] * Jp[0] + Hp[4] * Jp[1] + Hp[5] * Jp[2];
-            Real G11 = Hp[3] * Jp[3] + Hp[4] * Jp[4] + Hp[5] * Jp[5];
-            if (pair_i[base + t] == pair_j[base + t]) {
-                // an observation with itself: its J~c'J~c share of U* folded in, J~c'(I - G)J~c
-                // with I - G formed in FP64 (see point_complement); negated as acc is subtracted
-                G00 = -Ji[30];
-                G01 = G10 = -Ji[31];
-                G11 = -Ji[32];
-            }
-            for (int k = 0; k < 9; ++k) {
-                T[t][k] = G00 * Jj[k] + G01 * Jj[9 + k];
-                T[t][9 + k] = G10 * Jj[k] + G11 * Jj[9 + k];
-            }
-            for (int k = 0; k < 18; ++k) A[t][k] = Ji[k];
-        }
-        __syncthreads();
-        if (t < 81)
-            for (int k = 0; k < n; ++k) acc += (double)(A[k][r] * T[k][cc] + A[k][9 + r] * T[k][9 + cc]);
-        __syncthreads();
-    }
-    if (t < 81) {
-        const double v = (ca == cb && r == cc ? d.dc2[9 * (size_t)ca + r] : 0.0) - acc;
-        S[(size_t)(9 * ca + r) * ld + 9 * cb + cc] = v;
-        if (ca != cb) S[(size_t)(9 * cb + cc) * ld + 9 * ca + r] = v;
-    }
+    __shared__ __align__(16) unsigned char stage[pair_stage_bytes<Real, kRecDense>()];
+    const int b = blockIdx.x;
+    if (b >= nblocks) return;
+    const int4 bk = make_int4(blk_a[b], blk_b[b], blk_off[b], blk_off[b + 1]);
+    pair_block<Real, kRecDense>(bk, pairs, js, d.dc2, S, ld, reinterpret_cast<Real*>(stage));
 }
 
 template <typename F>
@@ -545,12 +516,13 @@ struct DenseSchur {
     int n = 0, ld = 0, nt = 0, nb = 0, nblocks = 0, refine = 0;
     bool f64 = true;  // factor in FP64 (FP64 solver, or a small system)
     DevBuf<Real> js;
-    DevBuf<int> blk_a, blk_b, blk_off, pair_i, pair_j;
+    DevBuf<int> blk_a, blk_b, blk_off;
+    DevBuf<int2> pairs;
     DevBuf<double> S, r, z;
     DevBuf<float> Af, Dinvf, Binvf, BinvTf;
     DevBuf<double> Dinvd, Binvd, BinvTd;
     int64_t device_bytes() const {
-        return js.bytes() + blk_a.bytes() * 3 + pair_i.bytes() * 2 + S.bytes() + r.bytes() + z.bytes() + Af.bytes() +
+        return js.bytes() + blk_a.bytes() * 3 + pairs.bytes() + S.bytes() + r.bytes() + z.bytes() + Af.bytes() +
                Dinvf.bytes() + Binvf.bytes() + BinvTf.bytes() + Dinvd.bytes() + Binvd.bytes() + BinvTd.bytes();
     }
 
@@ -570,8 +542,9 @@ struct DenseSchur {
         BA_TRY(blk_a.upload(ba.data(), ba.size(), s));
         BA_TRY(blk_b.upload(bb.data(), bb.size(), s));
         BA_TRY(blk_off.upload(boff.data(), boff.size(), s));
-        BA_TRY(pair_i.upload(pi.data(), pi.size(), s));
-        BA_TRY(pair_j.upload(pj.data(), pj.size(), s));
+        std::vector<int2> pr(pi.size());
+        for (size_t q = 0; q < pi.size(); ++q) pr[q] = make_int2(pi[q], pj[q]);
+        BA_TRY(pairs.upload(pr.data(), pr.size(), s));
         BA_TRY(js.alloc(kJs * (size_t)std::max(L.no, 1)));
         BA_TRY(S.alloc((size_t)ld * ld));
         BA_TRY(r.alloc(ld));
@@ -634,8 +607,9 @@ struct DenseSchur {
                   xcd_grid(nc), kWG, d, js.p);
         const size_t nn = (size_t)ld * ld;
         sv.launch("fill_S", k_fill_S, (int)((nn + 255) / 256), 256, S.p, n, ld, (const LMState*)d.st);
-        sv.launch("schur_pairs", k_schur_pairs<Real>, nblocks, kWG, d, (const Real*)js.p, (const int*)blk_a.p,
-                  (const int*)blk_b.p, (const int*)blk_off.p, (const int*)pair_i.p, (const int*)pair_j.p, S.p, ld);
+        if (nblocks)
+            sv.launch("schur_pairs", k_schur_blocks<Real>, nblocks, 64, d, (const Real*)js.p, (const int*)blk_a.p,
+                      (const int*)blk_b.p, (const int*)blk_off.p, (const int2*)pairs.p, S.p, ld, nblocks);
         if (f64) {
             factor_and_solve<double>(sv, S.p, Dinvd.p, Binvd.p, BinvTd.p);
         } else {

@@ -32,6 +32,7 @@
 #pragma once
 
 #include "mfma.h"
+#include "pair_block.h"
 #include "window.h"
 
 namespace mba {
@@ -206,172 +207,10 @@ __device__ __forceinline__ void win_slot_sum3(const WinPtrs& w, int p, double t[
     }
 }
 
-constexpr int kWinPairsPerStage = 16;
-
-// Two consecutive record elements, read as one LDS access.
-template <typename Real>
-struct alignas(2 * sizeof(Real)) R2v {
-    Real x, y;
-};
-
-// Row pitch of a staged record in LDS: 34 values, not the record's 32.  The 16 lanes of a row
-// read the same 16-byte pair of 16 different records; at a 272-byte pitch those land on 16 distinct
-// 4-bank groups (a 256-byte pitch puts all of them on one group).
-constexpr int kWinStagePitch = 34;
-
-// Rows of each half (i records, j records) of a wave's staging area: the stage's pairs plus the rows
-// the copy's spare slots (past 13 chunks x 16 pairs) write to, so that every lane stores.
-constexpr int kWinStageRows = kWinPairsPerStage + 4;
-
-// LDS bytes of the per-wave staging area of win_schur.
-template <typename Real>
-constexpr int win_stage_bytes() {
-    return 2 * kWinStageRows * kWinStagePitch * (int)sizeof(Real);
-}
-
-// One 64-lane round of the stage copy of win_schur: slot s = 64 r + lane of the stage's m pairs
-// (13 record chunks of four values per pair: i chunks 0-5 (J~_c, H), j chunks 0-4 and 6-7 (J~_c,
-// J~_p)).  load_round fetches the chunk into registers, store_round puts it into the wave's
-// staging area.
-template <typename Real, int PS>
-__device__ __forceinline__ R4<Real> win_load_round(const Real* __restrict__ js, int2 pr, int m, int r) {
-    constexpr int CI = 6, CP = 13;
-    const int s = 64 * r + (threadIdx.x & 63);
-    // slots past the stage's m pairs load a chunk of its last pair (always a valid address, so the
-    // load needs no branch) and are not stored
-    const int k = min(s / CP, m - 1), e = s - CP * (s / CP);
-    const int oi = __shfl(pr.x, k, 64), oj = __shfl(pr.y, k, 64);
-    const bool is_j = e >= CI;
-    const int chunk = is_j ? (e - CI < 5 ? e - CI : e - CI + 1) : e;
-    return reinterpret_cast<const R4<Real>*>(js + kWinJs * (size_t)(is_j ? oj : oi))[chunk];
-}
-
-template <typename Real, int PS>
-__device__ __forceinline__ void win_store_round(Real* st_i, Real* st_j, int r, const R4<Real>& v) {
-    constexpr int CI = 6, CP = 13;
-    static_assert(4 * 64 <= kWinStageRows * CP, "spare slots stay inside the staging area");
-    // rows past the stage's pairs receive copies of its last pair and are not read
-    const int s = 64 * r + (threadIdx.x & 63);
-    const int k = s / CP, e = s - CP * k;
-    const bool is_j = e >= CI;
-    const int chunk = is_j ? (e - CI < 5 ? e - CI : e - CI + 1) : e;
-    *reinterpret_cast<R4<Real>*>((is_j ? st_j : st_i) + kWinStagePitch * k + 4 * chunk) = v;
-}
-
-// One 9x9 block of the reduced camera system into S (n x n row-major, both triangles, global
-// memory): S_ab = [a == b] D_a^2 - sum_pairs J~_c,i' G_ij J~_c,j with G_ij = H_i J~_p,j' (i != j)
-// and G_ii = H_i J~_p,i' - I = -(I - G)_i, reduced by one wave.  Per stage of 16 pairs the wave
-// copies the record chunks the products read into its LDS staging area with 4-element vector loads
-// (lanes cover the chunks of consecutive pairs; the next stage's loads are in flight while this one
-// is computed); then each row of 16 lanes takes one column group (columns 0-2, 3-4, 5-6, 7-8) of
-// all nine rows, lane q of the row the stage's pair q, accumulating in registers; the 16 partial
-// sums of an entry are combined by a DPP butterfly inside the row in FP64.  Sixteen pairs and a
-// 27-value accumulator per lane (against 32 pairs and 45 values) keep a wave's staging area at
-// 8.7 KB and its registers near 128, so four waves share each SIMD.
-template <typename Real, typename F>
-__device__ __forceinline__ void win_schur_block(const int4 bk, const int2* __restrict__ pairs,
-                                                const Real* __restrict__ js, const double* __restrict__ dc2, F* S,
-                                                int n, Real* stage) {
-    constexpr int PS = kWinPairsPerStage;
-    static_assert(PS == 16, "one pair per lane of a 16-lane row");
-    static_assert(PS * 13 <= 4 * 64, "stage copy rounds");
-    const int lane = threadIdx.x & 63, q = lane & 15, h = lane >> 4;
-    const int c0 = h == 0 ? 0 : 2 * h + 1;  // first column of the lane's group
-    const int nc = h == 0 ? 3 : 2;          // columns in the group (a third slot repeats column 8 or c0 + 1)
-    Real* st_i = stage;
-    Real* st_j = stage + kWinStageRows * kWinStagePitch;
-    Real acc[9][3];
-#pragma unroll
-    for (int r = 0; r < 9; ++r)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) acc[r][k] = (Real)0;
-    const int npairs = bk.w - bk.z;
-    const int nst = (npairs + PS - 1) / PS;
-    int2 pr = make_int2(0, 0), pr_next = make_int2(0, 0);
-    if (lane < min(PS, npairs)) pr = pairs[bk.z + lane];
-    if (nst > 1 && lane < min(PS, npairs - PS)) pr_next = pairs[bk.z + PS + lane];
-    int m_ld = min(PS, npairs);
-    R4<Real> b0 = win_load_round<Real, PS>(js, pr, m_ld, 0), b1 = win_load_round<Real, PS>(js, pr, m_ld, 1),
-             b2 = win_load_round<Real, PS>(js, pr, m_ld, 2), b3 = win_load_round<Real, PS>(js, pr, m_ld, 3);
-    for (int stg = 0; stg < nst; ++stg) {
-        const int base = bk.z + stg * PS;
-        const int m = min(PS, bk.w - base);
-        win_store_round<Real, PS>(st_i, st_j, 0, b0);
-        win_store_round<Real, PS>(st_i, st_j, 1, b1);
-        win_store_round<Real, PS>(st_i, st_j, 2, b2);
-        win_store_round<Real, PS>(st_i, st_j, 3, b3);
-        const int pi = __shfl(pr.x, q, 64), pj = __shfl(pr.y, q, 64);
-        // the pair indices two stages ahead are requested before the next stage's records and
-        // taken over after this stage's products, so that neither waits for the records (a wave's
-        // loads complete in order)
-        int2 pr_far = make_int2(0, 0);
-        if (stg + 1 < nst) {
-            if (stg + 2 < nst && lane < min(PS, bk.w - base - 2 * PS)) pr_far = pairs[base + 2 * PS + lane];
-            m_ld = min(PS, bk.w - base - PS);
-            b0 = win_load_round<Real, PS>(js, pr_next, m_ld, 0);
-            b1 = win_load_round<Real, PS>(js, pr_next, m_ld, 1);
-            b2 = win_load_round<Real, PS>(js, pr_next, m_ld, 2);
-            b3 = win_load_round<Real, PS>(js, pr_next, m_ld, 3);
-        }
-        wave_sync_lds();
-        if (q < m) {
-            using R2 = R2v<Real>;
-            const Real* Ri = st_i + kWinStagePitch * q;
-            const Real* Rj = st_j + kWinStagePitch * q;
-            const R2 h0 = *reinterpret_cast<const R2*>(Ri + 18), h1 = *reinterpret_cast<const R2*>(Ri + 20),
-                     h2 = *reinterpret_cast<const R2*>(Ri + 22);
-            const R2 p0 = *reinterpret_cast<const R2*>(Rj + 24), p1 = *reinterpret_cast<const R2*>(Rj + 26),
-                     p2 = *reinterpret_cast<const R2*>(Rj + 28);
-            // H_i = [h0.x h0.y h1.x; h1.y h2.x h2.y], J~_p,j = [p0.x p0.y p1.x; p1.y p2.x p2.y]
-            Real G00 = h0.x * p0.x + h0.y * p0.y + h1.x * p1.x;
-            Real G01 = h0.x * p1.y + h0.y * p2.x + h1.x * p2.y;
-            Real G10 = h1.y * p0.x + h2.x * p0.y + h2.y * p1.x;
-            Real G11 = h1.y * p1.y + h2.x * p2.x + h2.y * p2.y;
-            // an observation with itself carries its share of U*: J~c'(I - G)J~c, negated because
-            // the accumulated sum is subtracted
-            const Real self = pi == pj ? (Real)1 : (Real)0;
-            G00 -= self;
-            G11 -= self;
-            Real t0[3], t1[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const int c = min(c0 + k, 8);
-                const R2 jc = *reinterpret_cast<const R2*>(Rj + 2 * c);
-                t0[k] = G00 * jc.x + G01 * jc.y;
-                t1[k] = G10 * jc.x + G11 * jc.y;
-            }
-#pragma unroll
-            for (int r = 0; r < 9; ++r) {
-                // the two rows of a J~_c column are adjacent
-                const R2 a = *reinterpret_cast<const R2*>(Ri + 2 * r);
-#pragma unroll
-                for (int k = 0; k < 3; ++k) acc[r][k] = fma(a.y, t1[k], fma(a.x, t0[k], acc[r][k]));
-            }
-        }
-        wave_sync_lds();
-        pr = pr_next;
-        pr_next = pr_far;
-    }
-    const int ca = bk.x, cb = bk.y;
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const double v = row16_sum((double)acc[r][k]);
-            const int c = c0 + k;
-            if (q == 0 && k < nc) {
-                const double s = (ca == cb && r == c ? dc2[9 * ca + r] : 0.0) - v;
-                S[(size_t)(9 * ca + r) * n + 9 * cb + c] = (F)s;
-                if (ca != cb) S[(size_t)(9 * cb + c) * n + 9 * ca + r] = (F)s;
-            }
-        }
-    }
-}
-
 // The blocks first + stride t, t < count, of one window, taken one at a time through *counter.
-template <typename Real, typename F>
+template <typename Real>
 __device__ __forceinline__ void win_schur(const int4* __restrict__ blk, const int2* __restrict__ pairs,
-                                          const Real* __restrict__ js, const double* __restrict__ dc2, F* S, int n,
+                                          const Real* __restrict__ js, const double* __restrict__ dc2, double* S, int n,
                                           int first, int stride, int count, int* counter, Real* stage) {
     const int lane = threadIdx.x & 63;
     for (;;) {
@@ -379,8 +218,14 @@ __device__ __forceinline__ void win_schur(const int4* __restrict__ blk, const in
         if (lane == 0) t = atomicAdd(counter, 1);
         t = __shfl(t, 0, 64);
         if (t >= count) break;
-        win_schur_block<Real, F>(blk[first + stride * t], pairs, js, dc2, S, n, stage);
+        pair_block<Real, kRecWindow>(blk[first + stride * t], pairs, js, dc2, S, n, stage);
     }
+}
+
+// LDS bytes of the per-wave staging area of win_schur.
+template <typename Real>
+constexpr int win_stage_bytes() {
+    return pair_stage_bytes<Real, kRecWindow>();
 }
 
 // Packed lower-triangular storage: element (i, j), j <= i, at i (i + 1) / 2 + j (row i contiguous).
@@ -916,7 +761,7 @@ __global__ void __launch_bounds__(NT) k_wschur(const WinDesc* __restrict__ descs
     if (threadIdx.x == 0) counter = 0;
     __syncthreads();
     const int wave = threadIdx.x >> 6;
-    win_schur<Real, double>(w.blk, w.pairs, at<Real>(arena, D.o_js), w.dc2, at<double>(arena, D.o_S), 9 * w.K, tk.y, tk.z,
+    win_schur<Real>(w.blk, w.pairs, at<Real>(arena, D.o_js), w.dc2, at<double>(arena, D.o_S), 9 * w.K, tk.y, tk.z,
                           tk.w, &counter, reinterpret_cast<Real*>(smem + (size_t)wave * win_stage_bytes<Real>()));
 }
 

@@ -2,6 +2,9 @@
 oracle's traces on the same seeded problems (tests/golden/fullsize_traces.json, written by
 tests/golden/make_fullsize_traces.py).
 
+Configuration 4 (the 512-window batch) is checked on a sample of its windows against the oracle
+run in the test.
+
 Bars: FP64 per-iteration cost within 1e-10 relative of the oracle while the two runs take the same
 accept/reject decisions, final cost within 1e-6 relative (FP64) and 1e-5 (mixed precision)."""
 import json
@@ -119,3 +122,24 @@ def test_config3_near_exact_steps(traces, precision):
             tol = 1e-7 if b["step_is_successful"] else 1e-4
             assert abs(a["cost"] - b["cost"]) <= tol * b["cost"], (a, b)
     assert rel < (1e-6 if precision == 0 else 1e-5)
+
+
+@pytest.mark.gpu
+def test_config4_batch_sample_matches_oracle(oracle_mod):
+    """Configuration 4 as benchmarked: the 512 windows solved in one batch (four lanes), 10 LM
+    iterations, FP64, all termination tolerances 0; every 16th window against the oracle's dense
+    Schur LM (final cost within 1e-9 relative, the same iteration and step counts)."""
+    wins = S.local_ba_windows(512, seed=1)
+    b = S.Batch(wins, precision=0, linear_solver=1, max_num_iterations=10, **FIXED)
+    assert b.schedule()["lanes"] == 4
+    sums = b.run()
+    for w in range(0, 512, 16):
+        p = wins[w]
+        o = oracle_mod.default_options(linear_solver=1, max_num_iterations=10, **FIXED)
+        ref = oracle_mod.solve(p.num_cameras(), p.num_points(), p.camera_index(), p.point_index(), p.observations(),
+                               p.mutable_cameras(), p.mutable_points(), o)
+        s = sums[w]
+        assert s["num_iterations"] == len(ref["trace"]), w
+        assert s["num_successful_steps"] == sum(r["step_is_successful"] for r in ref["trace"][1:]), w
+        assert abs(s["final_cost"] - ref["final_cost"]) <= 1e-9 * ref["final_cost"], (w, s["final_cost"],
+                                                                                      ref["final_cost"])

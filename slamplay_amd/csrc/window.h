@@ -49,7 +49,7 @@ struct WinDesc {
 constexpr int kWinJs = 32;
 // Partial sums per camera (cpart) and per point chunk (ppart).
 enum WinCamPart { WC_COST = 0, WC_GMAX, WC_MODEL, WC_CAND, WC_DX2, WC_X2, WC_COUNT = 8 };
-enum WinPtPart { WP_GMAX = 0, WP_DX2, WP_X2, WP_COUNT = 4 };
+enum WinPtPart { WP_GMAX = 0, WP_DX2, WP_X2, WP_MODEL, WP_COUNT = 4 };
 
 template <typename T>
 __host__ __device__ __forceinline__ T* at(char* base, int64_t off) {

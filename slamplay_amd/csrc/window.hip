@@ -449,7 +449,7 @@ struct WindowEngine {
         if (n_ctask) hipLaunchKernelGGL((k_wback_cam<kWinCamThreads, M>), dim3(n_ctask), dim3(kWinCamThreads), 0, s, dd,
                                         arena.p, (const int2*)ctask.p, n_ctask, o);
         if (n_ptask) hipLaunchKernelGGL(k_wback_pt, dim3(n_ptask), dim3(64), 0, s, dd, arena.p, (const int2*)ptask.p,
-                                        n_ptask);
+                                        n_ptask, o);
         if (n_ctask) hipLaunchKernelGGL((k_wcand_cam<kWinCamThreads, M>), dim3(n_ctask), dim3(kWinCamThreads), 0, s, dd,
                                         arena.p, (const int2*)ctask.p, n_ctask, o);
         hipLaunchKernelGGL((k_wdecide<256>), dim3(n_wtask), dim3(256), 0, s, dd, arena.p, (const int*)wtask.p, n_wtask,

@@ -852,7 +852,7 @@ __global__ void __launch_bounds__(NT) k_wback_cam(const WinDesc* __restrict__ de
 
 // One wave per (window, chunk): y_p, candidate points, |dx_p|^2 and |x_p|^2 of the chunk.
 __global__ void __launch_bounds__(64) k_wback_pt(const WinDesc* __restrict__ descs, char* __restrict__ arena,
-                                                 const int2* __restrict__ ptask, int ntask) {
+                                                 const int2* __restrict__ ptask, int ntask, Opts o) {
     const int t = blockIdx.x;
     if (t >= ntask) return;
     const int2 tk = ptask[t];
@@ -861,39 +861,52 @@ __global__ void __launch_bounds__(64) k_wback_pt(const WinDesc* __restrict__ des
     const WinState* st = w.st;
     if (st->done || !st->have_step || tk.y >= D.n_chunks) return;
     const int ch = tk.y, p = 64 * ch + threadIdx.x;
-    double dx2 = 0.0, x2 = 0.0;
+    double dx2 = 0.0, x2 = 0.0, model = 0.0;
     if (p < w.P) {
         double tt[3], y[3];
         win_slot_sum3(w, p, tt);
         sym3_mul<double>(w.Vinv + 6 * (size_t)p, tt, y);
         const double4 x4 = w.pt_x[p];
         const double xs[3] = {x4.x, x4.y, x4.z};
+        const double* V = w.V + 6 * (size_t)p;
+        const double vd[3] = {V[0], V[3], V[5]};
+        const double radius = st->radius;
         double cand[3];
         for (int i = 0; i < 3; ++i) {
             w.yp[3 * (size_t)p + i] = y[i];
-            const double dl = -w.sp[3 * (size_t)p + i] * y[i];
+            const double s = w.sp[3 * (size_t)p + i];
+            const double dl = -s * y[i];
             cand[i] = xs[i] + dl;
             dx2 += dl * dl;
             x2 += xs[i] * xs[i];
+            // the point's share of the model cost change y'(g~ + D^2 y) / 2 (D^2 as k_wdamp forms it)
+            const double d2 = fmin(fmax(s * vd[i] * s, o.min_diag), o.max_diag) / radius;
+            model += y[i] * (s * w.gp[3 * (size_t)p + i] + d2 * y[i]);
         }
         w.pt_c[p] = make_double4(cand[0], cand[1], cand[2], 0.0);
     }
     dx2 = wave_sum(dx2);
     x2 = wave_sum(x2);
+    model = wave_sum(model);
     if (threadIdx.x == 0) {
         w.ppart[WP_COUNT * (size_t)ch + WP_DX2] = dx2;
         w.ppart[WP_COUNT * (size_t)ch + WP_X2] = x2;
+        w.ppart[WP_COUNT * (size_t)ch + WP_MODEL] = 0.5 * model;
     }
 }
 
-// Per (window, camera): candidate camera, model cost change -(J~s)'(r~ + J~s/2) (step s = -y) and
-// FP64 candidate cost of the camera's observations, |dx_c|^2, |x_c|^2.
+// Per (window, camera): candidate camera, FP64 candidate cost of the camera's observations, the
+// camera's share of the model cost change and |dx_c|^2, |x_c|^2.  The model cost change of the
+// step, -(J~s)'(r~ + J~s/2) summed over the residuals (step s = -y in Jacobi-scaled coordinates), is
+// formed without the Jacobian: the damped system (J~'J~ + D^2) y = g~ = J~'r~ is solved exactly
+// (Cholesky of the reduced system, exact back-substitution), so it equals y'(g~ + D^2 y) / 2, a
+// sum over parameters (cameras here, points in k_wback_pt).
 // (At least two workgroups per CU: unbounded, the compiler's register allocation leaves one.)
 template <int NT, int M>
 __global__ void __launch_bounds__(NT, 2) k_wcand_cam(const WinDesc* __restrict__ descs, char* __restrict__ arena,
                                                   const int2* __restrict__ ctask, int ntask, Opts o) {
-    __shared__ double lds[2 * (NT / 64)];
-    __shared__ CameraPrep<double> Psh, Pxs;  // the candidate's prep, the current one's
+    __shared__ double lds[NT / 64];
+    __shared__ CameraPrep<double> Psh;  // the candidate's prep
     __shared__ StereoRig Gsh;
     __shared__ double csh[9];
     const int t = blockIdx.x;
@@ -908,7 +921,6 @@ __global__ void __launch_bounds__(NT, 2) k_wcand_cam(const WinDesc* __restrict__
         const size_t k = 9 * (size_t)c + threadIdx.x;
         csh[threadIdx.x] = w.cam_x[k] - w.sc[k] * w.yc[k];
     }
-    if (threadIdx.x == NT - 1) Pxs = w.prep[c];
     win_load_rig<M>(w, Gsh);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -923,45 +935,28 @@ __global__ void __launch_bounds__(NT, 2) k_wcand_cam(const WinDesc* __restrict__
         camera_prep<double>(csh, Psh);
     }
     __syncthreads();
-    double scc[9], y[9];
-    for (int i = 0; i < 9; ++i) {
-        scc[i] = w.sc[9 * (size_t)c + i];
-        y[i] = w.yc[9 * (size_t)c + i];
-    }
-    double acc[2] = {0.0, 0.0};  // model, cost
+    double acc[1] = {0.0};  // candidate cost
     for (int j = w.cam_off[c] + threadIdx.x; j < w.cam_off[c + 1]; j += NT) {
         const int p = w.co_pt[j];
-        double rx[2], Jc[18], Jp[6];
-        win_scaled<M>(w, Pxs, Gsh, j, p, o, scc, rx, Jc, Jp);
-        const double* yp = w.yp + 3 * (size_t)p;
-        double m0 = 0.0, m1 = 0.0;
-        for (int i = 0; i < 9; ++i) {
-            m0 -= Jc[i] * y[i];
-            m1 -= Jc[9 + i] * y[i];
-        }
-        for (int i = 0; i < 3; ++i) {
-            m0 -= Jp[i] * yp[i];
-            m1 -= Jp[3 + i] * yp[i];
-        }
-        acc[0] += m0 * (rx[0] + 0.5 * m0) + m1 * (rx[1] + 0.5 * m1);
         const double4 x4 = w.pt_c[p];
         const double X[3] = {x4.x, x4.y, x4.z};
         double r[2];
         win_model_residual<M>(w, Psh, Gsh, j, X, r);
-        acc[1] += half_rho(r[0] * r[0] + r[1] * r[1], o.loss, o.loss_a);
+        acc[0] += half_rho(r[0] * r[0] + r[1] * r[1], o.loss, o.loss_a);
     }
-    wg_sum_fixed<2, NT>(acc, lds);
+    wg_sum_fixed<1, NT>(acc, lds);
     if (threadIdx.x == 0) {
-        double dx2 = 0.0, x2 = 0.0;
+        double dx2 = 0.0, x2 = 0.0, model = 0.0;
         for (int i = 0; i < 9; ++i) {
             const size_t k = 9 * (size_t)c + i;
-            const double dl = -w.sc[k] * w.yc[k], x = w.cam_x[k];
+            const double y = w.yc[k], dl = -w.sc[k] * y, x = w.cam_x[k];
             dx2 += dl * dl;
             x2 += x * x;
+            model += y * (w.sc[k] * w.gc[k] + w.dc2[k] * y);
         }
         double* cp = w.cpart + WC_COUNT * (size_t)c;
-        cp[WC_MODEL] = acc[0];
-        cp[WC_CAND] = acc[1];
+        cp[WC_MODEL] = 0.5 * model;
+        cp[WC_CAND] = acc[0];
         cp[WC_DX2] = dx2;
         cp[WC_X2] = x2;
     }
@@ -1089,9 +1084,9 @@ __global__ void __launch_bounds__(NT) k_wdecide(const WinDesc* __restrict__ desc
             for (int ch = 0; ch < D.n_chunks; ++ch) {
                 dx2 += w.ppart[WP_COUNT * (size_t)ch + WP_DX2];
                 x2 += w.ppart[WP_COUNT * (size_t)ch + WP_X2];
+                model += w.ppart[WP_COUNT * (size_t)ch + WP_MODEL];
             }
         }
-        model = -model;
         st->iteration++;
         if (o.g2o)
             accepted = g2o_decide(st, o, have_step, model, cand);

@@ -21,9 +21,9 @@
 // Residual model M (a template parameter of the kernels that evaluate residuals): the Snavely camera
 // of the BAL path, or full_slam's stereo rig with the SE3 pose update (stereo.h).
 // The per-observation FP64 residual and Jacobian are recomputed by each kernel that needs them
-// (k_wlin_cam, k_wrec_cam, k_wback_cam, k_wcand_cam) from the camera's prep, the point and the
-// measurement: about 300 FP64 operations against 26 stored doubles written once and read three
-// times per iteration.
+// (k_wlin_cam, k_wrec_cam, k_wback_cam) from the camera's prep, the point and the measurement:
+// about 300 FP64 operations against 26 stored doubles written once and read three times per
+// iteration; k_wcand_cam evaluates residuals at the candidate only.
 // Every kernel reads the window's LM state first and returns when it has nothing to do (or when its
 // task lies outside the window it names), so the sequence can be replayed as a graph.  Per-camera and per-chunk sums are written to their own
 // slots and combined by the per-window kernels in camera / chunk order, and every 9x9 block of S is
@@ -901,9 +901,8 @@ __global__ void __launch_bounds__(64) k_wback_pt(const WinDesc* __restrict__ des
 // formed without the Jacobian: the damped system (J~'J~ + D^2) y = g~ = J~'r~ is solved exactly
 // (Cholesky of the reduced system, exact back-substitution), so it equals y'(g~ + D^2 y) / 2, a
 // sum over parameters (cameras here, points in k_wback_pt).
-// (At least two workgroups per CU: unbounded, the compiler's register allocation leaves one.)
 template <int NT, int M>
-__global__ void __launch_bounds__(NT, 2) k_wcand_cam(const WinDesc* __restrict__ descs, char* __restrict__ arena,
+__global__ void __launch_bounds__(NT) k_wcand_cam(const WinDesc* __restrict__ descs, char* __restrict__ arena,
                                                   const int2* __restrict__ ctask, int ntask, Opts o) {
     __shared__ double lds[NT / 64];
     __shared__ CameraPrep<double> Psh;  // the candidate's prep

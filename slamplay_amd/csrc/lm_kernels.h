@@ -1367,13 +1367,29 @@ __global__ void __launch_bounds__(NT) k_jz_dir(Dev<Real> d) {
 
 // 16 threads per camera: alpha = rz / p'Sp (each block reduces the same partials in the same
 // order), x += alpha p, r -= alpha q, z = Minv r (coalesced Minv rows); per-block partials r.z
-// and r.r for k_jtx.
+// and r.r for k_jtx.  The camera's vectors and Minv row do not depend on alpha: their loads are
+// issued before the reduction, so the kernel waits on memory once.
 template <typename Real>
 __global__ void __launch_bounds__(kWG) k_cam_update(Dev<Real> d, int it) {
     LMState* st = d.st;
     if (st->done || st->step_invalid || st->pcg_done) return;
     __shared__ double lds[2 * kWaves];
     __shared__ double rsh[kCamsPerUpd][9];
+    const int lc = threadIdx.x >> 4, i = threadIdx.x & 15;
+    const int c = blockIdx.x * kCamsPerUpd + lc;
+    const bool act = c < d.nc && i < 9;
+    const size_t k = act ? 9 * (size_t)c + i : 0;
+    double x = 0.0, p = 0.0, r = 0.0, q = 0.0, Mrow[9];
+    if (act) {
+        x = d.px[k];
+        p = d.pp[k];
+        r = d.pr[k];
+        q = d.qc[k];
+    }
+    const double* Mi = d.Minv + (act ? 81 * (size_t)c + 9 * i : 0);
+#pragma unroll
+    for (int m = 0; m < 9; ++m) Mrow[m] = act ? Mi[m] : 0.0;
+    const double rz_it = d.rz_hist[it];
     double v = strided_sum(d.cam_part + PS_A * (size_t)d.nc, d.nc);
     v += strided_sum(d.pt_part + PS_B * (size_t)d.n_pt_blocks, d.n_pt_blocks);
     const double pSp = block_sum1(v, lds);
@@ -1385,24 +1401,19 @@ __global__ void __launch_bounds__(kWG) k_cam_update(Dev<Real> d, int it) {
         }
         return;
     }
-    const double alpha = d.rz_hist[it] / pSp;
-    const int lc = threadIdx.x >> 4, i = threadIdx.x & 15;
-    const int c = blockIdx.x * kCamsPerUpd + lc;
-    double r = 0.0;
-    if (c < d.nc && i < 9) {
-        const size_t k = 9 * (size_t)c + i;
-        d.px[k] += alpha * d.pp[k];
-        r = d.pr[k] - alpha * d.qc[k];
+    const double alpha = rz_it / pSp;
+    if (act) {
+        d.px[k] = x + alpha * p;
+        r = r - alpha * q;
         d.pr[k] = r;
         rsh[lc][i] = r;
     }
     __syncthreads();
     double rz = 0.0, rr = 0.0;
-    if (c < d.nc && i < 9) {
-        const double* Mrow = d.Minv + 81 * (size_t)c + 9 * i;
+    if (act) {
         double z = 0.0;
-        for (int k = 0; k < 9; ++k) z += Mrow[k] * rsh[lc][k];
-        d.pz[9 * (size_t)c + i] = z;
+        for (int m = 0; m < 9; ++m) z += Mrow[m] * rsh[lc][m];
+        d.pz[k] = z;
         rz = r * z;
         rr = r * r;
     }

@@ -1019,20 +1019,25 @@ __global__ void __launch_bounds__(kWG) k_pt_z(Dev<Real> d) {
     __shared__ double lds[(kWaves - 1) * 3 * 64];
     const int ch = xcd_remap(blockIdx.x, d.n_chunks);
     if (ch >= d.n_chunks) return;
+    // the point's scaling, V*^-1 and D_p^2 are requested before the sum (wave 0 uses them)
+    const int p = 64 * ch + threadIdx.x;
+    const bool own = threadIdx.x < 64 && p < d.np;
+    double s[3], Vi[6], dp[3];
+    for (int i = 0; i < 3; ++i) {
+        s[i] = own ? d.sp[3 * (size_t)p + i] : 0.0;
+        dp[i] = own ? d.dp2[3 * (size_t)p + i] : 0.0;
+    }
+    for (int i = 0; i < 6; ++i) Vi[i] = own ? d.Vinv64[6 * (size_t)p + i] : 0.0;
     double a[3];
     chunk_sum_r4<Real>(d.chunk_len[ch], (size_t)d.chunk_off[ch], d.u, a, lds);
     if (threadIdx.x >= 64) return;
-    const int p = 64 * ch + threadIdx.x;
     double zdz = 0.0;
-    if (p < d.np) {
-        double s[3], t[3], z[3];
-        for (int i = 0; i < 3; ++i) {
-            s[i] = d.sp[3 * (size_t)p + i];
-            t[i] = s[i] * a[i];
-        }
-        sym3_mul<double>(d.Vinv64 + 6 * (size_t)p, t, z);  // the point block solve stays FP64
+    if (own) {
+        double t[3], z[3];
+        for (int i = 0; i < 3; ++i) t[i] = s[i] * a[i];
+        sym3_mul<double>(Vi, t, z);  // the point block solve stays FP64
         d.zs[p] = make_double4(s[0] * z[0], s[1] * z[1], s[2] * z[2], 0.0);
-        for (int i = 0; i < 3; ++i) zdz += z[i] * d.dp2[3 * (size_t)p + i] * z[i];
+        for (int i = 0; i < 3; ++i) zdz += z[i] * dp[i] * z[i];
     }
     zdz = wave_sum(zdz);
     if (threadIdx.x == 0) d.pt_part[PS_B * (size_t)d.n_pt_blocks + ch] = zdz;
@@ -1224,6 +1229,8 @@ __global__ void __launch_bounds__(NT) k_jtx_dir(Dev<Real> d, int it) {
         pp = d.pp[i];
         sc = d.sc[i];
     }
+    // the camera's prep is requested before the reductions (a load after a barrier waits for it)
+    const CameraPrep<Real> P = d.prep[c];
     double beta;
     if (pcg_beta<Real, NW>(d, it, lds, beta)) return;
     if (threadIdx.x < 9) {
@@ -1232,7 +1239,6 @@ __global__ void __launch_bounds__(NT) k_jtx_dir(Dev<Real> d, int it) {
         vsh[threadIdx.x] = sc * v;
     }
     __syncthreads();
-    const CameraPrep<Real> P = d.prep[c];
     if (threadIdx.x == 0) rot_directional<Real>(P, vsh, rsh);
     __syncthreads();
     DirCam<Real> C;
@@ -1286,13 +1292,13 @@ __global__ void __launch_bounds__(NT) k_jz_dir(Dev<Real> d) {
         zin = d.zs[d.co_pt[j]];
     }
     if (j + NT < j_end) pt_next = d.co_pt[j + NT];
+    const CameraPrep<Real> P = d.prep[c];  // requested before the barrier (a load after it waits for it)
     if (threadIdx.x < 9) {
         const size_t i = 9 * (size_t)c + threadIdx.x;
         psh[threadIdx.x] = d.pp[i];
         vsh[threadIdx.x] = d.sc[i] * d.pp[i];
     }
     __syncthreads();
-    const CameraPrep<Real> P = d.prep[c];
     if (threadIdx.x == 0) rot_directional<Real>(P, vsh, rsh);
     __syncthreads();
     DirCam<Real> C;

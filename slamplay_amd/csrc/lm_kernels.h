@@ -1277,7 +1277,7 @@ __global__ void __launch_bounds__(NT) k_jz_dir(Dev<Real> d) {
     LMState* st = d.st;
     if (st->done || st->step_invalid || st->pcg_done) return;
     __shared__ double lds[16 * NW];
-    __shared__ double psh[9], vsh[9], rsh[9], qsh[9];
+    __shared__ double psh[9], vsh[9], rsh[9], qsh[9], dsh[9];
     const int c = xcd_remap(blockIdx.x, d.nc);
     if (c >= d.nc) return;
     const int j_end = d.cam_off[c + 1];
@@ -1293,10 +1293,14 @@ __global__ void __launch_bounds__(NT) k_jz_dir(Dev<Real> d) {
     }
     if (j + NT < j_end) pt_next = d.co_pt[j + NT];
     const CameraPrep<Real> P = d.prep[c];  // requested before the barrier (a load after it waits for it)
+    double sc_t = 0.0, dc2_t = 0.0;          // thread t < 9: S_c and D_c^2 entry t, kept for the end
     if (threadIdx.x < 9) {
         const size_t i = 9 * (size_t)c + threadIdx.x;
+        sc_t = d.sc[i];
+        dc2_t = d.dc2[i];
         psh[threadIdx.x] = d.pp[i];
-        vsh[threadIdx.x] = d.sc[i] * d.pp[i];
+        vsh[threadIdx.x] = sc_t * d.pp[i];
+        dsh[threadIdx.x] = dc2_t;
     }
     __syncthreads();
     if (threadIdx.x == 0) rot_directional<Real>(P, vsh, rsh);
@@ -1360,13 +1364,10 @@ __global__ void __launch_bounds__(NT) k_jz_dir(Dev<Real> d) {
         for (int k = 0; k < 9; ++k) qsh[k] = qv[k];
     }
     __syncthreads();
-    if (t < 9) {
-        const size_t i = 9 * (size_t)c + t;
-        d.qc[i] = d.dc2[i] * psh[t] + d.sc[i] * qsh[t];
-    }
+    if (t < 9) d.qc[9 * (size_t)c + t] = dc2_t * psh[t] + sc_t * qsh[t];
     if (t == 0) {
         double s = lds[15 * NW];
-        for (int i = 0; i < 9; ++i) s += psh[i] * d.dc2[9 * (size_t)c + i] * psh[i];
+        for (int i = 0; i < 9; ++i) s += psh[i] * dsh[i] * psh[i];
         d.cam_part[PS_A * (size_t)d.nc + c] = s;
     }
 }

@@ -548,23 +548,19 @@ __device__ __forceinline__ void chunk_sum_r4(int len, size_t base, const R4<Real
     constexpr int U = 4;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     acc[0] = acc[1] = acc[2] = 0.0;
-    int r = wave;
-    for (; r + (U - 1) * kWaves < len; r += U * kWaves) {
+    // every trip requests U rows at once (rows past the chunk re-read its last row and add zero), so
+    // a wave with fewer than U rows left still waits on memory once
+    for (int r = wave; r < len; r += U * kWaves) {
         R4<Real> v[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = src[base + 64 * (size_t)(r + u * kWaves) + lane];
+        for (int u = 0; u < U; ++u) v[u] = src[base + 64 * (size_t)min(r + u * kWaves, len - 1) + lane];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            acc[0] += (double)v[u].x;
-            acc[1] += (double)v[u].y;
-            acc[2] += (double)v[u].z;
+            const bool in = r + u * kWaves < len;
+            acc[0] += in ? (double)v[u].x : 0.0;
+            acc[1] += in ? (double)v[u].y : 0.0;
+            acc[2] += in ? (double)v[u].z : 0.0;
         }
-    }
-    for (; r < len; r += kWaves) {
-        const R4<Real> v = src[base + 64 * (size_t)r + lane];
-        acc[0] += (double)v.x;
-        acc[1] += (double)v.y;
-        acc[2] += (double)v.z;
     }
     if (wave > 0)
 #pragma unroll

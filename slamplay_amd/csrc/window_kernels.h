@@ -447,6 +447,11 @@ __global__ void __launch_bounds__(NT) k_wlin_cam(const WinDesc* __restrict__ des
     const WinState* st = w.st;
     if (st->done || !st->need_lin || tk.y >= w.K) return;
     const int c = tk.y, first = st->first_lin;
+    // the first round's point index and point are requested before the prep and the barrier
+    const int j_end = w.cam_off[c + 1];
+    int j = w.cam_off[c] + threadIdx.x;
+    double4 x4n = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (j < j_end) x4n = w.pt_x[w.co_pt[j]];
     if (threadIdx.x == 0) {
         camera_prep<double>(w.cam_x + 9 * (size_t)c, Psh);
         w.prep[c] = Psh;
@@ -456,9 +461,9 @@ __global__ void __launch_bounds__(NT) k_wlin_cam(const WinDesc* __restrict__ des
     double acc[19];
 #pragma unroll
     for (int i = 0; i < 19; ++i) acc[i] = 0.0;  // diag (9), J_c'r (9), cost
-    for (int j = w.cam_off[c] + threadIdx.x; j < w.cam_off[c + 1]; j += NT) {
-        const int p = w.co_pt[j];
-        const double4 x4 = w.pt_x[p];
+    for (; j < j_end; j += NT) {
+        const double4 x4 = x4n;
+        if (j + NT < j_end) x4n = w.pt_x[w.co_pt[j + NT]];
         const double X[3] = {x4.x, x4.y, x4.z};
         double r[2], Jc[18], Jp[6];
         win_model_jacobian<M>(w, Psh, Gsh, j, X, r, Jc, Jp);
@@ -645,6 +650,9 @@ __global__ void __launch_bounds__(NT) k_wrec_cam(const WinDesc* __restrict__ des
     if (st->done || st->invalid || tk.y >= w.K) return;
     Real* js = at<Real>(arena, D.o_js);
     const int c = tk.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the camera's range and the first round's point index are requested before the barrier
+    const int j_beg = w.cam_off[c], j_end = w.cam_off[c + 1];
+    int pn = j_beg + (int)threadIdx.x < j_end ? w.co_pt[j_beg + threadIdx.x] : 0;
     if (threadIdx.x == 0) Psh = w.prep[c];
     win_load_rig<M>(w, Gsh);
     double scc[9];
@@ -655,14 +663,14 @@ __global__ void __launch_bounds__(NT) k_wrec_cam(const WinDesc* __restrict__ des
     for (int k = threadIdx.x; k < NW * 45; k += NT) (&dacc[0][0])[k] = 0.0;
     __syncthreads();
     // rounds of NT observations, every lane taking part in the per-round butterflies
-    const int j_end = w.cam_off[c + 1];
-    for (int j0r = w.cam_off[c]; j0r < j_end; j0r += NT) {
+    for (int j0r = j_beg; j0r < j_end; j0r += NT) {
         const int j = j0r + threadIdx.x;
+        const int p = pn;
+        if (j + NT < j_end) pn = w.co_pt[j + NT];
         double Jd[18], g00 = 0.0, g01 = 0.0, g11 = 0.0;
 #pragma unroll
         for (int i = 0; i < 18; ++i) Jd[i] = 0.0;
         if (j < j_end) {
-            const int p = w.co_pt[j];
             double r[2], Jc[18], Jp[6];
             win_scaled<M>(w, Psh, Gsh, j, p, o, scc, r, Jc, Jp);
             const double* Vi = w.Vinv + 6 * (size_t)p;
@@ -827,6 +835,9 @@ __global__ void __launch_bounds__(NT) k_wback_cam(const WinDesc* __restrict__ de
     const WinState* st = w.st;
     if (st->done || !st->have_step || tk.y >= w.K) return;
     const int c = tk.y;
+    const int j_end = w.cam_off[c + 1];
+    int j = w.cam_off[c] + threadIdx.x;
+    int pn = j < j_end ? w.co_pt[j] : 0;  // the next round's point index, one round ahead
     if (threadIdx.x == 0) Psh = w.prep[c];
     win_load_rig<M>(w, Gsh);
     double scc[9], y[9];
@@ -835,8 +846,9 @@ __global__ void __launch_bounds__(NT) k_wback_cam(const WinDesc* __restrict__ de
         y[i] = w.yc[9 * (size_t)c + i];
     }
     __syncthreads();
-    for (int j = w.cam_off[c] + threadIdx.x; j < w.cam_off[c + 1]; j += NT) {
-        const int p = w.co_pt[j];
+    for (; j < j_end; j += NT) {
+        const int p = pn;
+        if (j + NT < j_end) pn = w.co_pt[j + NT];
         double r[2], Jc[18], Jp[6];
         win_scaled<M>(w, Psh, Gsh, j, p, o, scc, r, Jc, Jp);
         double a0 = r[0], a1 = r[1];
@@ -916,6 +928,11 @@ __global__ void __launch_bounds__(NT) k_wcand_cam(const WinDesc* __restrict__ de
     const WinState* st = w.st;
     if (st->done || !st->have_step || tk.y >= w.K) return;
     const int c = tk.y;
+    // the first round's point index and candidate point are requested before the barriers
+    const int j_end = w.cam_off[c + 1];
+    int j = w.cam_off[c] + threadIdx.x;
+    double4 x4n = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (j < j_end) x4n = w.pt_c[w.co_pt[j]];
     if (threadIdx.x < 9) {
         const size_t k = 9 * (size_t)c + threadIdx.x;
         csh[threadIdx.x] = w.cam_x[k] - w.sc[k] * w.yc[k];
@@ -935,9 +952,9 @@ __global__ void __launch_bounds__(NT) k_wcand_cam(const WinDesc* __restrict__ de
     }
     __syncthreads();
     double acc[1] = {0.0};  // candidate cost
-    for (int j = w.cam_off[c] + threadIdx.x; j < w.cam_off[c + 1]; j += NT) {
-        const int p = w.co_pt[j];
-        const double4 x4 = w.pt_c[p];
+    for (; j < j_end; j += NT) {
+        const double4 x4 = x4n;
+        if (j + NT < j_end) x4n = w.pt_c[w.co_pt[j + NT]];
         const double X[3] = {x4.x, x4.y, x4.z};
         double r[2];
         win_model_residual<M>(w, Psh, Gsh, j, X, r);

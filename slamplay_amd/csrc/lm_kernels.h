@@ -968,6 +968,10 @@ __global__ void __launch_bounds__(kWG) k_jtx(Dev<Real> d, int mode, int it, cons
     int j = d.cam_off[c] + threadIdx.x;
     ObsIn<Real> in{};
     if (j < j_end) load_obs<Real, M>(d, j, in);
+    // the camera's prep and scaling are requested before the reduction and the barrier
+    const CameraPrep<Real> P = d.prep[c];
+    double scl[9];
+    for (int i = 0; i < 9; ++i) scl[i] = d.sc[9 * (size_t)c + i];
     double beta = 0.0;
     if (mode == 1 && pcg_beta(d, it, lds, beta)) return;
     if (threadIdx.x < 9) {
@@ -981,8 +985,7 @@ __global__ void __launch_bounds__(kWG) k_jtx(Dev<Real> d, int mode, int it, cons
     }
     __syncthreads();
     Real vs[9];
-    for (int i = 0; i < 9; ++i) vs[i] = (Real)(d.sc[9 * (size_t)c + i] * vsh[i]);
-    const CameraPrep<Real> P = d.prep[c];
+    for (int i = 0; i < 9; ++i) vs[i] = (Real)(scl[i] * vsh[i]);
     for (; j < j_end; j += kWG) {
         const ObsIn<Real> cur = in;
         if (j + kWG < j_end) load_obs<Real, M>(d, j + kWG, in);
@@ -1053,11 +1056,14 @@ __global__ void __launch_bounds__(kWG) k_jz(Dev<Real> d) {
     int j = d.cam_off[c] + threadIdx.x;
     ObsIn<Real> in{};
     if (j < j_end) load_obs<Real, M>(d, j, in);
+    // the camera's prep and scaling are requested before the barrier
+    const CameraPrep<Real> P = d.prep[c];
+    double scl[9];
+    for (int i = 0; i < 9; ++i) scl[i] = d.sc[9 * (size_t)c + i];
     if (threadIdx.x < 9) psh[threadIdx.x] = d.pp[9 * (size_t)c + threadIdx.x];
     __syncthreads();
     Real ps[9];
-    for (int i = 0; i < 9; ++i) ps[i] = (Real)(d.sc[9 * (size_t)c + i] * psh[i]);
-    const CameraPrep<Real> P = d.prep[c];
+    for (int i = 0; i < 9; ++i) ps[i] = (Real)(scl[i] * psh[i]);
     double acc[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) acc[i] = 0.0;

@@ -27,6 +27,7 @@ threads.  The device
 Normalize / Perturb of configuration 3 are timed against the host BALProblem methods.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -399,6 +400,9 @@ def bench_stereo_and_g2o(S, reps=10):
             shutil.copyfileobj(src, dst)
         base = S.BALProblem(path)
         base.Normalize()
+        # the reference's g2o driver perturbs in a process that never calls srand (glibc's seed 1);
+        # earlier legs of this run may have drawn from the same generator
+        ctypes.CDLL("libc.so.6").srand(1)
         base.Perturb(0.1, 0.5, 0.5)
     cams, pts = base.mutable_cameras().copy(), base.mutable_points().copy()
 
